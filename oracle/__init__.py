@@ -1,0 +1,329 @@
+"""CPU oracle for the 2D WENO5/HLLC hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
+leg may import this package, and only as the checker (never as the thing that
+is measured as the GPU product or shipped).
+
+Two layers:
+
+* ``libweno_oracle.so`` (``weno_oracle.c``): a plain-C, FP64, single-threaded
+  restatement of the reference Rust crate ``weno_ext`` (``weno_ext/src/*.rs``),
+  reached here through ctypes with the same call surface as the reference
+  Python module (``weno_ext/src/lib.rs:18-153``).
+* numpy restatements of the reference's Python drivers around it: ghost fill,
+  primitive sync, SSP-RK3 and the CFL time step
+  (``Rie_prob_2d_rs.py:42-207,311-319``, ``double_mach_reflection.py:97-134,
+  229-260``, ``unitest.py:169-227``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libweno_oracle.so")
+
+
+class _View3(ctypes.Structure):
+    _fields_ = [("p", ctypes.c_void_p),
+                ("d0", ctypes.c_int64), ("d1", ctypes.c_int64), ("d2", ctypes.c_int64),
+                ("s0", ctypes.c_int64), ("s1", ctypes.c_int64), ("s2", ctypes.c_int64)]
+
+
+_lib = None
+
+
+def build() -> str:
+    """Compile the C oracle in place (make -C oracle)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.POINTER(_View3)
+        d = ctypes.c_double
+        dp = ctypes.c_void_p
+        i = ctypes.c_int
+        L.orc_flux.argtypes = [P, d, i, dp]
+        L.orc_con2primi.argtypes = [P, d, dp, dp, dp, dp, dp, dp]
+        L.orc_weno.argtypes = [P, d, i, i, dp, dp, dp, dp]
+        L.orc_riemann.argtypes = [P, P, P, P, d, i, i, dp]
+        L.orc_sensor.argtypes = [dp, ctypes.c_int64, ctypes.c_int64, d, d, dp]
+        L.orc_blend.argtypes = [P, P, P, P, d, i, d, d, dp]
+        L.orc_l_local.argtypes = [P, d, d, i, d, d, i, dp]
+        for f in (L.orc_flux, L.orc_con2primi, L.orc_weno, L.orc_riemann,
+                  L.orc_sensor, L.orc_blend, L.orc_l_local):
+            f.restype = None
+        _lib = L
+    return _lib
+
+
+def _view(a: np.ndarray) -> _View3:
+    assert a.dtype == np.float64 and a.ndim == 3
+    s = [st // 8 for st in a.strides]
+    return _View3(a.ctypes.data, a.shape[0], a.shape[1], a.shape[2], s[0], s[1], s[2])
+
+
+def _out(*shape):
+    return np.zeros(shape, dtype=np.float64)
+
+
+# --------------------------------------------------------------------------
+# weno_ext surface (lib.rs:18-153), oracle semantics
+# --------------------------------------------------------------------------
+def flux_x_py(q, gamma):
+    o = _out(q.shape[0], q.shape[1], 4)
+    lib().orc_flux(ctypes.byref(_view(q)), gamma, 0, o.ctypes.data)
+    return o
+
+
+def flux_y_py(q, gamma):
+    o = _out(q.shape[0], q.shape[1], 4)
+    lib().orc_flux(ctypes.byref(_view(q)), gamma, 1, o.ctypes.data)
+    return o
+
+
+def con2primi_py(q, gamma):
+    outs = [_out(q.shape[0], q.shape[1]) for _ in range(6)]
+    lib().orc_con2primi(ctypes.byref(_view(q)), gamma, *[o.ctypes.data for o in outs])
+    return tuple(outs)
+
+
+def _weno(u, gamma, direction, cons):
+    m, n = u.shape[0], u.shape[1]
+    shp = (m - 5, n - 6, 4) if direction == 0 else (m - 6, n - 5, 4)
+    outs = [_out(*shp) for _ in range(4)]
+    lib().orc_weno(ctypes.byref(_view(u)), gamma, direction, int(cons),
+                   *[o.ctypes.data for o in outs])
+    return tuple(outs)
+
+
+def weno_x_py(u, gamma, cons=False):
+    return _weno(u, gamma, 0, cons)
+
+
+def weno_y_py(u, gamma, cons=False):
+    return _weno(u, gamma, 1, cons)
+
+
+def _riemann(q_l, f_l, q_r, f_r, gamma, direction, kind):
+    o = _out(q_l.shape[0], q_l.shape[1], 4)
+    lib().orc_riemann(*[ctypes.byref(_view(a)) for a in (q_l, f_l, q_r, f_r)],
+                      gamma, direction, kind, o.ctypes.data)
+    return o
+
+
+def hllc_x_rs_py(q_l, f_l, q_r, f_r, gamma):
+    return _riemann(q_l, f_l, q_r, f_r, gamma, 0, 0)
+
+
+def hllc_y_rs_py(q_l, f_l, q_r, f_r, gamma):
+    return _riemann(q_l, f_l, q_r, f_r, gamma, 1, 0)
+
+
+def hlle_x_rs_py(q_l, f_l, q_r, f_r, gamma):
+    return _riemann(q_l, f_l, q_r, f_r, gamma, 0, 1)
+
+
+def hlle_y_rs_py(q_l, f_l, q_r, f_r, gamma):
+    return _riemann(q_l, f_l, q_r, f_r, gamma, 1, 1)
+
+
+def _blend(q_l, f_l, q_r, f_r, gamma, jp_lo, jp_hi, force_hlle, direction):
+    if force_hlle:
+        return _riemann(q_l, f_l, q_r, f_r, gamma, direction, 1)
+    o = _out(q_l.shape[0], q_l.shape[1], 4)
+    lib().orc_blend(*[ctypes.byref(_view(a)) for a in (q_l, f_l, q_r, f_r)],
+                    gamma, direction, jp_lo, jp_hi, o.ctypes.data)
+    return o
+
+
+def hllc_hlle_x_rs_py(q_l, f_l, q_r, f_r, gamma, jp_lo, jp_hi, force_hlle):
+    return _blend(q_l, f_l, q_r, f_r, gamma, jp_lo, jp_hi, force_hlle, 0)
+
+
+def hllc_hlle_y_rs_py(q_l, f_l, q_r, f_r, gamma, jp_lo, jp_hi, force_hlle):
+    return _blend(q_l, f_l, q_r, f_r, gamma, jp_lo, jp_hi, force_hlle, 1)
+
+
+def shock_sensor_grad_p(p, jp_lo, jp_hi):
+    """riemann.rs:352-410 on a C-contiguous 2D face-grid pressure."""
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    o = np.zeros_like(p)
+    lib().orc_sensor(p.ctypes.data, p.shape[0], p.shape[1], jp_lo, jp_hi, o.ctypes.data)
+    return o
+
+
+def l_local_py(u, dx, gamma, force_hlle, jp_cri, cons_switch=False):
+    o = _out(u.shape[0] - 6, u.shape[1] - 6, 4)
+    lib().orc_l_local(ctypes.byref(_view(u)), dx, gamma, int(force_hlle),
+                      float(jp_cri[0]), float(jp_cri[1]), int(cons_switch), o.ctypes.data)
+    return o
+
+
+# --------------------------------------------------------------------------
+# Drivers (numpy restatements of the reference's Python)
+# --------------------------------------------------------------------------
+NG = 3
+
+
+def apply_bc_zero_gradient(u):
+    """Even mirror, y first then x, all channels (Rie_prob_2d_rs.py:123-152)."""
+    nx, ny = u.shape[0], u.shape[1]
+    for g in range(NG):
+        u[:, g, :] = u[:, 2 * NG - 1 - g, :]
+    for g in range(NG):
+        u[:, ny - 1 - g, :] = u[:, ny - 2 * NG + g, :]
+    for g in range(NG):
+        u[g, :, :] = u[2 * NG - 1 - g, :, :]
+    for g in range(NG):
+        u[nx - 1 - g, :, :] = u[nx - 2 * NG + g, :, :]
+    return u
+
+
+_S_Y = np.array([1, 1, -1, 1, 1, 1, -1, 1], dtype=np.float64)
+_S_X = np.array([1, -1, 1, 1, 1, -1, 1, 1], dtype=np.float64)
+_S_C = np.array([1, -1, -1, 1, 1, -1, -1, 1], dtype=np.float64)
+
+
+def apply_bc_reflecting(u):
+    """Rie_prob_2d_rs.py:42-120, including its (m, n) = (shape[1], shape[0])."""
+    m, n = u.shape[1], u.shape[0]
+    S_Y, S_X, S_C = (_S_Y[:u.shape[2]], _S_X[:u.shape[2]], _S_C[:u.shape[2]])
+    for g in range(NG):
+        u[g, NG:n - NG, :] = u[2 * NG - 1 - g, NG:n - NG, :] * S_Y
+    for g in range(NG):
+        u[m - 1 - g, NG:n - NG, :] = u[m - 2 * NG + g, NG:n - NG, :] * S_Y
+    for g in range(NG):
+        u[NG:m - NG, g, :] = u[NG:m - NG, 2 * NG - 1 - g, :] * S_X
+    for g in range(NG):
+        u[NG:m - NG, n - 1 - g, :] = u[NG:m - NG, n - 2 * NG + g, :] * S_X
+    for gi in range(NG):
+        for gj in range(NG):
+            mi, mj = 2 * NG - 1 - gi, 2 * NG - 1 - gj
+            int_j, gh_j = n - 2 * NG + gj, n - NG + gj
+            int_i, gh_i = m - 2 * NG + gi, m - NG + gi
+            u[gi, gj, :] = u[mi, mj, :] * S_C
+            u[gi, gh_j, :] = u[mi, int_j, :] * S_C
+            u[gh_i, gj, :] = u[int_i, mj, :] * S_C
+            u[gh_i, gh_j, :] = u[int_i, int_j, :] * S_C
+    return u
+
+
+DMR_POST_ROW = [8, 33, -57.1576, 563.499, 8, 4.125, -7.1447, 116.5]
+
+
+def apply_bc_refac(q, bound, nx, ny):
+    """Double-Mach ghost fill of the reference (unitest.py:207-227)."""
+    for i in range(3):
+        q[i, :, :] = DMR_POST_ROW
+        q[:, ny + 3 + i, :] = q[:, ny + 2, :]
+        q[nx + 3 + i, :, :] = q[nx + 2, :, :]
+        q[:bound + 1, i, :] = DMR_POST_ROW
+        q[bound + 1:bound + 10, i, :] = q[bound + 1:bound + 10, 3, :]
+        q[bound + 10:, i, :] = q[bound + 10:, 5 - i, :]
+        q[bound + 10:, i, 2] *= -1
+        q[bound + 10:, i, 6] *= -1
+    return q
+
+
+def sync_primitives(q, gamma):
+    """Rie_prob_2d_rs.py:157-163."""
+    p, a, rho, u, v, h = con2primi_py(q, gamma)
+    q[:, :, 4] = rho
+    q[:, :, 5] = u
+    q[:, :, 6] = v
+    q[:, :, 7] = p
+    return q
+
+
+def rk3_cc(q, nx, ny, dx, dt, gamma, force_hlle, weno_switch, jp_cri=(0.2, 1.0)):
+    """Shu-Osher SSP-RK3 with mirror ghost fill (Rie_prob_2d_rs.py:169-207)."""
+    sl = np.s_[3:nx + 3, 3:ny + 3, :4]
+    apply_bc_zero_gradient(q)
+    l0 = l_local_py(q, dx, gamma, force_hlle, jp_cri, weno_switch)
+    q1 = q.copy()
+    q1[sl] = q[sl] + dt * l0
+    apply_bc_zero_gradient(q1)
+    sync_primitives(q1, gamma)
+    apply_bc_zero_gradient(q1)
+    l1 = l_local_py(q1, dx, gamma, force_hlle, jp_cri, weno_switch)
+    q2 = q.copy()
+    q2[sl] = 0.75 * q[sl] + 0.25 * q1[sl] + 0.25 * dt * l1
+    apply_bc_zero_gradient(q2)
+    sync_primitives(q2, gamma)
+    apply_bc_zero_gradient(q2)
+    l2 = l_local_py(q2, dx, gamma, force_hlle, jp_cri, weno_switch)
+    qn = q.copy()
+    qn[sl] = (1 / 3) * q[sl] + (2 / 3) * q2[sl] + (2 / 3) * dt * l2
+    apply_bc_zero_gradient(qn)
+    sync_primitives(qn, gamma)
+    apply_bc_zero_gradient(qn)
+    return qn
+
+
+def rk3_refac(q, nx, ny, dx, dt, gamma, force_hlle, bound, jp_cri=(0.2, 1.0),
+              cons_switch=False):
+    """DMR SSP-RK3 (double_mach_reflection.py:97-134); 5-arg l_local calls
+    select the characteristic path."""
+    sl = np.s_[3:nx + 3, 3:ny + 3, :4]
+    l0 = l_local_py(q, dx, gamma, force_hlle, jp_cri, cons_switch)
+    q1 = q.copy()
+    q1[sl] = q[sl] + dt * l0
+    q1 = apply_bc_refac(q1, bound, nx, ny)
+    l1 = l_local_py(q1, dx, gamma, force_hlle, jp_cri, cons_switch)
+    q2 = q.copy()
+    q2[sl] = 0.75 * q[sl] + 0.25 * q1[sl] + 0.25 * dt * l1
+    q2 = apply_bc_refac(q2, bound, nx, ny)
+    l2 = l_local_py(q2, dx, gamma, force_hlle, jp_cri, cons_switch)
+    qn = q.copy()
+    qn[sl] = 1 / 3 * q[sl] + 2 / 3 * q2[sl] + 2 / 3 * dt * l2
+    qn = apply_bc_refac(qn, bound, nx, ny)
+    p, a, rho, u, v, h = con2primi_py(qn, gamma)
+    qn[:, :, 4] = rho
+    qn[:, :, 5] = u
+    qn[:, :, 6] = v
+    qn[:, :, 7] = p
+    return apply_bc_refac(qn, bound, nx, ny)
+
+
+def cfl_dt(q, nx, ny, gamma, cfl, dx, remaining=None):
+    """Rie_prob_2d_rs.py:311-319 (and double_mach_reflection.py:233-240)."""
+    speed = np.sqrt(q[3:3 + nx, 3:3 + ny, 5] ** 2 + q[3:3 + nx, 3:3 + ny, 6] ** 2)
+    c = np.sqrt(gamma * q[3:3 + nx, 3:3 + ny, 7]
+                / np.maximum(q[3:3 + nx, 3:3 + ny, 0], 1e-14))
+    c_max = float(np.max(c + speed)) + 1e-16
+    dt = float(cfl) * dx / c_max
+    if remaining is not None:
+        dt = min(dt, remaining)
+    return dt
+
+
+def run_riemann(q, nx, ny, dx, gamma, cfl, force_hlle, weno_switch, jp_cri,
+                nstep=None, t_final=None):
+    """Time loop of Rie_prob_2d_rs.main_rs (:328-365) without I/O."""
+    T, step, ts = 0.0, 0, [0.0]
+    max_steps = int(nstep) if nstep is not None else int(1e8)
+    if t_final is not None:
+        while T < t_final - 1e-12 and step < max_steps:
+            dt = cfl_dt(q, nx, ny, gamma, cfl, dx, remaining=t_final - T)
+            q = rk3_cc(q, nx, ny, dx, dt, gamma, force_hlle, weno_switch, jp_cri)
+            T += dt
+            step += 1
+            ts.append(T)
+    else:
+        for _ in range(max_steps):
+            dt = cfl_dt(q, nx, ny, gamma, cfl, dx)
+            q = rk3_cc(q, nx, ny, dx, dt, gamma, force_hlle, weno_switch, jp_cri)
+            T += dt
+            step += 1
+            ts.append(T)
+    return q, ts
