@@ -1,0 +1,157 @@
+/*
+ * wenox.h -- C ABI of the MI355X (gfx950) WENO5 / HLLC hot path.
+ *
+ * Two groups of entry points:
+ *
+ *  1. Drop-in functions.  One per function of the reference Python extension
+ *     module `weno_ext` (PyO3, weno_ext/src/lib.rs:156-170), with the same
+ *     argument meaning, shapes and results.  Inputs are host arrays described
+ *     by wenox_view3 (any element strides, like the reference's
+ *     PyReadonlyArray3 views); outputs are caller-allocated C-contiguous host
+ *     arrays of the shapes given below.  The computation runs on the current
+ *     HIP device; these calls copy in, compute and copy out, so they are
+ *     PCIe-bound at large sizes.
+ *
+ *  2. Device-resident solver.  Holds the padded conserved state in HBM and
+ *     advances it with the reference's SSP-RK3 + ghost fill + CFL loop
+ *     (Rie_prob_2d_rs.py:169-207,311-345; double_mach_reflection.py:97-134,
+ *     229-260), one hipGraph replay per time step, dt computed on the device.
+ *
+ * Every function returns WENOX_OK (0) on success, a negative value for an
+ * argument or shape error (the cases where the reference raises), or a
+ * positive value for a HIP runtime failure; wenox_last_error() then returns a
+ * message (thread-local, valid until the next call on the same thread).
+ * There is no CPU fallback: without a usable gfx950 device every compute call
+ * fails with WENOX_E_DEVICE.
+ */
+#ifndef WENOX_H
+#define WENOX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WENOX_OK 0
+#define WENOX_E_ARG (-1)    /* invalid argument (null pointer, bad enum, c < 4) */
+#define WENOX_E_SHAPE (-2)  /* a shape the reference rejects with a Rust panic */
+#define WENOX_E_DEVICE 1    /* HIP runtime error or no usable device */
+
+/* Arithmetic variant of the device kernels. */
+#define WENOX_EXACT 0  /* the reference's operation order, no FMA contraction */
+#define WENOX_FAST 1   /* FMA contraction and shared reciprocals (rounding-level differences) */
+
+/* A read-only host array view: extents and element (not byte) strides. */
+typedef struct wenox_view3 {
+    const double *data;
+    int64_t dim[3];
+    int64_t stride[3];
+} wenox_view3;
+
+const char *wenox_last_error(void);
+/* Library version string. */
+const char *wenox_version(void);
+/* Selects the arithmetic variant used by the drop-in functions (default WENOX_EXACT). */
+int wenox_set_variant(int variant);
+/* Number of visible HIP devices (0 when none); does not create a context. */
+int wenox_device_count(void);
+
+/* ---- 1. drop-in functions ------------------------------------------------ */
+
+/* lib.rs:18-24 flux_x_py(q, gamma) -> (M, N, 4); q is (M, N, c>=4). */
+int wenox_flux_x(const wenox_view3 *q, double gamma, double *out);
+/* lib.rs:25-31 flux_y_py(q, gamma) -> (M, N, 4). */
+int wenox_flux_y(const wenox_view3 *q, double gamma, double *out);
+/* lib.rs:32-39 con2primi_py(q, gamma) -> (p, a, rho_safe, u, v, h), each (M, N). */
+int wenox_con2primi(const wenox_view3 *q, double gamma, double *p, double *a, double *rho,
+                    double *u, double *v, double *h);
+/* lib.rs:40-46 weno_x_py(u, gamma) -> (q_l, f_l, q_r, f_r), each (m-5, n-6, 4).
+ * cons_switch = 0 is the reference binding (characteristic path,
+ * weno.rs:367-502); 1 selects the component-wise path (weno.rs:635-852). */
+int wenox_weno_x(const wenox_view3 *u, double gamma, int cons_switch, double *q_l, double *f_l,
+                 double *q_r, double *f_r);
+/* lib.rs:47-53 weno_y_py(u, gamma) -> (q_l, f_l, q_r, f_r), each (m-6, n-5, 4). */
+int wenox_weno_y(const wenox_view3 *u, double gamma, int cons_switch, double *q_l, double *f_l,
+                 double *q_r, double *f_r);
+/* lib.rs:54-64 hllc_x_rs_py(q_l, f_l, q_r, f_r, gamma) -> (F1, F2, 4), F1 x F2 from q_l. */
+int wenox_hllc_x(const wenox_view3 *q_l, const wenox_view3 *f_l, const wenox_view3 *q_r,
+                 const wenox_view3 *f_r, double gamma, double *out);
+/* lib.rs:65-75 hllc_y_rs_py */
+int wenox_hllc_y(const wenox_view3 *q_l, const wenox_view3 *f_l, const wenox_view3 *q_r,
+                 const wenox_view3 *f_r, double gamma, double *out);
+/* lib.rs:76-86 hlle_x_rs_py */
+int wenox_hlle_x(const wenox_view3 *q_l, const wenox_view3 *f_l, const wenox_view3 *q_r,
+                 const wenox_view3 *f_r, double gamma, double *out);
+/* lib.rs:87-97 hlle_y_rs_py */
+int wenox_hlle_y(const wenox_view3 *q_l, const wenox_view3 *f_l, const wenox_view3 *q_r,
+                 const wenox_view3 *f_r, double gamma, double *out);
+/* lib.rs:98-115 hllc_hlle_x_rs_py(q_l, f_l, q_r, f_r, gamma, jp_lo, jp_hi, force_hlle) */
+int wenox_hllc_hlle_x(const wenox_view3 *q_l, const wenox_view3 *f_l, const wenox_view3 *q_r,
+                      const wenox_view3 *f_r, double gamma, double jp_lo, double jp_hi,
+                      int force_hlle, double *out);
+/* lib.rs:116-133 hllc_hlle_y_rs_py */
+int wenox_hllc_hlle_y(const wenox_view3 *q_l, const wenox_view3 *f_l, const wenox_view3 *q_r,
+                      const wenox_view3 *f_r, double gamma, double jp_lo, double jp_hi,
+                      int force_hlle, double *out);
+/* lib.rs:134-153 l_local_py(u, dx, gamma, force_hlle, jp_cri, cons_switch)
+ * -> (m-6, n-6, 4) = -(dF/dx + dG/dy) (riemann.rs:489-541). */
+int wenox_l_local(const wenox_view3 *u, double dx, double gamma, int force_hlle, double jp_lo,
+                  double jp_hi, int cons_switch, double *out);
+
+/* ---- 2. device-resident solver --------------------------------------------- */
+
+#define WENOX_BC_MIRROR 0    /* Rie_prob_2d_rs.py:123-152 apply_bc_zero_gradient (even mirror) */
+#define WENOX_BC_PERIODIC 1
+#define WENOX_BC_DMR_REF 2   /* unitest.py:207-227 apply_bc_refac */
+#define WENOX_BC_DMR_STD 3   /* standard double Mach reflection, exact moving shock on top */
+
+#define WENOX_ENGINE_UNFUSED 0  /* the reference's pass structure, face arrays in HBM */
+#define WENOX_ENGINE_FUSED 1    /* one fused kernel per direction sweep per RK stage */
+
+#define WENOX_PREC_FP64 0
+#define WENOX_PREC_FP32SPLIT 1  /* FP32 bulk arithmetic, state as FP32 hi+lo pairs */
+
+typedef struct wenox_solver_desc {
+    int64_t nx, ny;      /* interior cells; the state is (nx+6, ny+6) with 3 ghosts */
+    double dx;           /* cell size (the reference uses dx for both directions) */
+    double gamma;
+    double cfl;
+    int cons_switch;     /* 0 characteristic, 1 component-wise reconstruction */
+    int force_hlle;      /* 1: HLLE everywhere; 0: HLLC/HLLE blend (sensor jp_lo, jp_hi) */
+    double jp_lo, jp_hi;
+    int bc;              /* WENOX_BC_* */
+    int dmr_bound;       /* WENOX_BC_DMR_REF: `bound` from unitest.init_refac */
+    double dmr_x0;       /* WENOX_BC_DMR_STD: wall start (1/6) */
+    int engine;          /* WENOX_ENGINE_* */
+    int variant;         /* WENOX_EXACT / WENOX_FAST */
+    int precision;       /* WENOX_PREC_* */
+    int use_graph;       /* capture one time step in a hipGraph and replay it */
+} wenox_solver_desc;
+
+typedef struct wenox_solver wenox_solver;
+
+int wenox_solver_create(const wenox_solver_desc *desc, wenox_solver **out);
+void wenox_solver_destroy(wenox_solver *s);
+/* Upload q (nx+6, ny+6, c>=4) host array (channels 0-3 conserved; when c>=8
+ * channels 4-7 are the caller's primitives (rho, u, v, p), used for the first
+ * CFL step as the reference does).  Applies the solver's ghost fill and sets
+ * the simulated time to t0. */
+int wenox_solver_upload(wenox_solver *s, const wenox_view3 *q, double t0);
+/* Download the state as (nx+6, ny+6, 8): conserved channels and the
+ * primitives (rho_safe, u, v, p) of con2primi. */
+int wenox_solver_download(wenox_solver *s, double *q8);
+/* Advance: at most max_steps steps, and while T < t_final - 1e-12 when
+ * t_final >= 0 (Rie_prob_2d_rs.py:335).  Reports the time and steps taken. */
+int wenox_solver_run(wenox_solver *s, int64_t max_steps, double t_final, double *t_out,
+                     int64_t *steps_out);
+/* Evaluate L(U) of the current state into out (nx, ny, 4) host array. */
+int wenox_solver_rhs(wenox_solver *s, double *out);
+/* Average device time per step of the last run (ms), from HIP events. */
+double wenox_solver_last_ms_per_step(wenox_solver *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WENOX_H */

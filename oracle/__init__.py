@@ -59,8 +59,14 @@ def lib():
         L.orc_sensor.argtypes = [dp, ctypes.c_int64, ctypes.c_int64, d, d, dp]
         L.orc_blend.argtypes = [P, P, P, P, d, i, d, d, dp]
         L.orc_l_local.argtypes = [P, d, d, i, d, d, i, dp]
+        L.orc_test_roe_eig.argtypes = [i, dp, dp, d, dp, dp, dp]
+        L.orc_test_pp_limit.argtypes = [dp, dp, d]
+        L.orc_test_weno5.argtypes = [dp, dp, dp]
+        L.orc_test_gq.argtypes = [dp, dp, dp]
+        L.orc_test_riemann_pt.argtypes = [i, i, dp, dp, dp, dp, d, dp]
         for f in (L.orc_flux, L.orc_con2primi, L.orc_weno, L.orc_riemann,
-                  L.orc_sensor, L.orc_blend, L.orc_l_local):
+                  L.orc_sensor, L.orc_blend, L.orc_l_local, L.orc_test_roe_eig,
+                  L.orc_test_pp_limit, L.orc_test_weno5, L.orc_test_gq, L.orc_test_riemann_pt):
             f.restype = None
         _lib = L
     return _lib
@@ -167,6 +173,55 @@ def l_local_py(u, dx, gamma, force_hlle, jp_cri, cons_switch=False):
     lib().orc_l_local(ctypes.byref(_view(u)), dx, gamma, int(force_hlle),
                       float(jp_cri[0]), float(jp_cri[1]), int(cons_switch), o.ctypes.data)
     return o
+
+
+# --------------------------------------------------------------------------
+# point-function hooks (for the oracle's own invariant tests)
+# --------------------------------------------------------------------------
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def roe_eig(direction, ul, ur, gamma):
+    """Roe state (rho, u, v, H, c) and the left/right eigenvector matrices
+    (weno.rs:13-189) of the characteristic basis between states ul | ur."""
+    ul, ur = _c(ul), _c(ur)
+    roe, Lm, Rm = np.zeros(5), np.zeros((4, 4)), np.zeros((4, 4))
+    lib().orc_test_roe_eig(direction, ul.ctypes.data, ur.ctypes.data, gamma,
+                           roe.ctypes.data, Lm.ctypes.data, Rm.ctypes.data)
+    return roe, Lm, Rm
+
+
+def pp_limit(q, ub, gamma):
+    """weno.rs:327-352 applied to a copy of q."""
+    q = _c(q).copy()
+    ub = _c(ub)
+    lib().orc_test_pp_limit(q.ctypes.data, ub.ctypes.data, gamma)
+    return q
+
+
+def weno5_pair(u6):
+    """(weno5_left(u0..u4), weno5_right(u1..u5)) (weno.rs:214-257)."""
+    u6 = _c(u6)
+    a, b = ctypes.c_double(), ctypes.c_double()
+    lib().orc_test_weno5(u6.ctypes.data, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def weno5_gq(q5):
+    """(ga, gb) Gauss-point values (weno.rs:262-303)."""
+    q5 = _c(q5)
+    a, b = ctypes.c_double(), ctypes.c_double()
+    lib().orc_test_gq(q5.ctypes.data, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def riemann_pt(direction, kind, ql, fl, qr, fr, gamma):
+    """One face of HLLC (kind 0, riemann.rs:6-189) or HLLE (kind 1, :191-349)."""
+    arrs = [_c(x) for x in (ql, fl, qr, fr)]
+    out = np.zeros(4)
+    lib().orc_test_riemann_pt(direction, kind, *[a.ctypes.data for a in arrs], gamma, out.ctypes.data)
+    return out
 
 
 # --------------------------------------------------------------------------

@@ -588,3 +588,40 @@ void orc_l_local(const orc_view3 *u, double dx, double gamma, int force_hlle,
     for (int s = 0; s < 4; ++s) { free(xq[s]); free(yq[s]); }
     free(fx); free(fy); free(dfdx); free(dgdy);
 }
+
+/* ------------------------------------------------------------------ */
+/* Test hooks: the point functions above, exposed for the oracle's own */
+/* invariant tests (eigenvectors, Roe property, limiter, WENO5/GQ).    */
+/* ------------------------------------------------------------------ */
+void orc_test_roe_eig(int dir, const double ul[4], const double ur[4], double gamma,
+                      double roe5[5], double L[16], double R[16])
+{
+    double rho, u, v, h, c;
+    roe_avg(ul, ur, gamma, &rho, &u, &v, &h, &c);
+    double LL[4][4], RR[4][4];
+    eig_left(dir, rho, u, v, c, gamma, LL);
+    eig_right(dir, rho, u, v, h, c, RR);
+    roe5[0] = rho; roe5[1] = u; roe5[2] = v; roe5[3] = h; roe5[4] = c;
+    for (int r = 0; r < 4; ++r)
+        for (int k = 0; k < 4; ++k) { L[r * 4 + k] = LL[r][k]; R[r * 4 + k] = RR[r][k]; }
+}
+
+void orc_test_pp_limit(double q[4], const double ub[4], double gamma) { pp_limit(q, ub, gamma); }
+
+void orc_test_weno5(const double u[6], double *left, double *right)
+{
+    *left = weno5_left(u[0], u[1], u[2], u[3], u[4]);
+    *right = weno5_right(u[1], u[2], u[3], u[4], u[5]);
+}
+
+void orc_test_gq(const double q[5], double *ga, double *gb)
+{
+    weno5_gq(q[0], q[1], q[2], q[3], q[4], ga, gb);
+}
+
+void orc_test_riemann_pt(int dir, int kind, const double ql[4], const double fl[4],
+                         const double qr[4], const double fr[4], double gamma, double out[4])
+{
+    if (kind == 0) hllc_pt(dir, ql, fl, qr, fr, gamma, out);
+    else hlle_pt(dir, ql, fl, qr, fr, gamma, out);
+}

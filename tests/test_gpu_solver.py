@@ -1,0 +1,116 @@
+"""GPU parity of the device-resident solver against the oracle's time loop.
+
+Criterion: relative L1 = sum|U_gpu - U_ref| / sum|U_ref| per conserved
+variable over the interior, <= 1e-11 after the stated number of SSP-RK3
+steps (the "exact" variant is expected to be bitwise equal; the bound is the
+contract).  The oracle runs the reference drivers' numpy time loop around the
+C restatement of the Rust RHS.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-11
+
+
+def rel_l1(a, b, nx, ny):
+    out = []
+    for k in range(4):
+        d = np.abs(a[3:nx + 3, 3:ny + 3, k] - b[3:nx + 3, 3:ny + 3, k]).sum()
+        s = np.abs(b[3:nx + 3, 3:ny + 3, k]).sum()
+        out.append(d / s)
+    return np.array(out)
+
+
+def run_pair(oracle_mod, q0, nx, ny, dx, steps, cons, force_hlle, jp, engine, variant, cfl=0.4,
+             t_final=None):
+    from wenox import Solver
+    ref, ts = oracle_mod.run_riemann(q0.copy(), nx, ny, dx, 1.4, cfl, force_hlle, cons, jp,
+                                     nstep=steps, t_final=t_final)
+    with Solver(nx, ny, dx, gamma=1.4, cfl=cfl, cons_switch=cons, force_hlle=force_hlle,
+                jp_cri=jp, bc="mirror", engine=engine, variant=variant) as s:
+        s.upload(q0)
+        T, k = s.run(nstep=steps, t_final=t_final)
+        got = s.download()
+    return ref, ts, got, T, k
+
+
+@pytest.mark.parametrize("engine", ["unfused", "fused"])
+@pytest.mark.parametrize("variant", ["exact", "fast"])
+def test_four_quadrant_50_steps(oracle_mod, engine, variant):
+    from wenox import problems
+    n = 128
+    q0, dx = problems.four_quadrant(n)
+    print(f"[solver] four-quadrant {n}^2 char+HLLC 50 steps engine={engine} variant={variant}", flush=True)
+    ref, ts, got, T, k = run_pair(oracle_mod, q0, n, n, dx, 50, False, False, (1e4, 1.1e4), engine, variant)
+    assert k == 50
+    assert abs(T - ts[-1]) <= 1e-12 * ts[-1]
+    err = rel_l1(got, ref, n, n)
+    print("rel L1", err)
+    assert np.all(err <= TOL), err
+    if variant == "exact":
+        np.testing.assert_array_equal(got[..., :4], ref[..., :4])
+    # primitives of the download match the reference's synced channels
+    np.testing.assert_allclose(got[3:-3, 3:-3, 4:], ref[3:-3, 3:-3, 4:], rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("engine", ["unfused", "fused"])
+def test_componentwise_blend(oracle_mod, engine):
+    from wenox import problems
+    q0, _ = problems.riemann_reference(64, 64)
+    print(f"[solver] reference IC 64^2 cons+blend 20 steps engine={engine}", flush=True)
+    ref, ts, got, T, k = run_pair(oracle_mod, q0, 64, 64, 1 / 64, 20, True, False, (0.5, 2.0),
+                                  engine, "exact", cfl=0.3)
+    err = rel_l1(got, ref, 64, 64)
+    assert np.all(err <= TOL), err
+
+
+@pytest.mark.parametrize("engine", ["unfused", "fused"])
+def test_t_final_loop(oracle_mod, engine):
+    """The time-controlled loop: same number of steps and final time as
+    Rie_prob_2d_rs.main_rs (:335-344), the last dt clipped to t_final - T."""
+    from wenox import problems
+    q0, dx = problems.four_quadrant(48)
+    print(f"[solver] t_final loop 48^2 engine={engine}", flush=True)
+    ref, ts, got, T, k = run_pair(oracle_mod, q0, 48, 48, dx, None, False, True, (0.2, 1.0),
+                                  engine, "exact", t_final=0.05)
+    assert k == len(ts) - 1
+    assert T == ts[-1]
+    err = rel_l1(got, ref, 48, 48)
+    assert np.all(err <= TOL), err
+
+
+def test_dmr_reference_bc(oracle_mod):
+    from wenox import Solver, problems
+    nx, ny = 80, 20
+    q0, bound = problems.dmr_reference(nx, ny)
+    dx = 1.0 / ny
+    print(f"[solver] DMR reference BC {nx}x{ny} bound={bound}", flush=True)
+    q = q0.copy()
+    T = 0.0
+    for _ in range(8):
+        dt = oracle_mod.cfl_dt(q, nx, ny, 1.4, 0.1, dx, remaining=0.2 - T)
+        q = oracle_mod.rk3_refac(q, nx, ny, dx, dt, 1.4, False, bound, (10000, 11000))
+        T += dt
+    with Solver(nx, ny, dx, gamma=1.4, cfl=0.1, cons_switch=False, force_hlle=False,
+                jp_cri=(10000, 11000), bc="dmr_ref", dmr_bound=bound) as s:
+        s.upload(q0)
+        T2, k = s.run(nstep=8, t_final=0.2)
+        got = s.download()
+    assert k == 8 and T2 == T
+    err = rel_l1(got, q, nx, ny)
+    assert np.all(err <= TOL), err
+    np.testing.assert_array_equal(got, q)
+
+
+def test_solver_rhs_matches_l_local(oracle_mod):
+    from wenox import Solver, problems
+    q0, dx = problems.four_quadrant(40)
+    ref = oracle_mod.l_local_py(q0, dx, 1.4, False, (0.5, 2.0), False)
+    for engine in ("unfused", "fused"):
+        with Solver(40, 40, dx, cons_switch=False, force_hlle=False, jp_cri=(0.5, 2.0),
+                    engine=engine) as s:
+            s.upload(q0)
+            got = s.rhs()
+        np.testing.assert_array_equal(got, ref, err_msg=engine)

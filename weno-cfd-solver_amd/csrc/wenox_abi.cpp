@@ -1,0 +1,758 @@
+// wenox_abi.cpp -- C ABI (include/wenox.h): the drop-in functions of the
+// reference module weno_ext and the device-resident solver.
+//
+// Host code only; every computation is a gfx950 kernel launched through the
+// Ops tables of wenox_kernels.hip / wenox_sweep.hip.  There is no CPU compute
+// path: without a device every compute entry point returns WENOX_E_DEVICE.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/wenox.h"
+#include "wenox_ops.h"
+#include "wenox_sweep.h"
+
+using namespace wenox;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(WENOX_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+int g_variant = WENOX_EXACT;
+
+const Ops* ops_for(int variant) { return variant == WENOX_FAST ? &ops_fast : &ops_exact; }
+
+// Checks that a gfx950 device is current; the code objects are built for it only.
+int require_device()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return fail(WENOX_E_DEVICE, "no HIP device visible: the wenox kernels need an MI355X (gfx950)");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(WENOX_E_DEVICE, std::string("device is ") + prop.gcnArchName +
+                                        "; the wenox kernels are built for gfx950 only");
+    return WENOX_OK;
+}
+
+// ------------------------------------------------------------------------
+// Drop-in workspace: one stream, a growing device arena and pinned staging.
+// Calls are serialised (the reference holds the GIL for the whole call).
+// ------------------------------------------------------------------------
+struct Dropin {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    double* dev = nullptr;
+    size_t dev_cap = 0;
+    double* pin = nullptr;
+    size_t pin_cap = 0;
+};
+Dropin g_drop;
+
+int dropin_reserve(size_t dev_doubles, size_t pin_doubles)
+{
+    int rc = require_device();
+    if (rc) return rc;
+    if (!g_drop.stream) HIPCHK(hipStreamCreateWithFlags(&g_drop.stream, hipStreamNonBlocking));
+    if (dev_doubles > g_drop.dev_cap) {
+        if (g_drop.dev) HIPCHK(hipFree(g_drop.dev));
+        g_drop.dev = nullptr;
+        size_t cap = std::max(dev_doubles, g_drop.dev_cap * 3 / 2);
+        HIPCHK(hipMalloc(&g_drop.dev, std::max<size_t>(cap, 1) * sizeof(double)));
+        g_drop.dev_cap = cap;
+    }
+    if (pin_doubles > g_drop.pin_cap) {
+        if (g_drop.pin) HIPCHK(hipHostFree(g_drop.pin));
+        g_drop.pin = nullptr;
+        size_t cap = std::max(pin_doubles, g_drop.pin_cap * 3 / 2);
+        HIPCHK(hipHostMalloc(&g_drop.pin, std::max<size_t>(cap, 1) * sizeof(double)));
+        g_drop.pin_cap = cap;
+    }
+    return WENOX_OK;
+}
+
+// Packs channels 0..3 of rows [0, M) x cols [0, N) of a strided view into an
+// AoS4 (M, N, 4) contiguous buffer.
+void pack4(const wenox_view3* v, int64_t M, int64_t N, double* dst)
+{
+    const int64_t s0 = v->stride[0], s1 = v->stride[1], s2 = v->stride[2];
+    const double* p = v->data;
+    if (s2 == 1 && s1 == 4 && s0 == 4 * N) {
+        std::memcpy(dst, p, sizeof(double) * (size_t)(M * N * 4));
+        return;
+    }
+    for (int64_t i = 0; i < M; ++i)
+        for (int64_t j = 0; j < N; ++j) {
+            const double* s = p + i * s0 + j * s1;
+            double* d = dst + (i * N + j) * 4;
+            d[0] = s[0]; d[1] = s[s2]; d[2] = s[2 * s2]; d[3] = s[3 * s2];
+        }
+}
+
+bool bad_view(const wenox_view3* v)
+{
+    return !v || (!v->data && v->dim[0] * v->dim[1] * v->dim[2] != 0) || v->dim[0] < 0 ||
+           v->dim[1] < 0 || v->dim[2] < 0;
+}
+
+int shape_err(const char* fn, const std::string& why)
+{
+    return fail(WENOX_E_SHAPE, std::string(fn) + ": " + why);
+}
+
+std::string dims(const wenox_view3* v)
+{
+    char b[96];
+    std::snprintf(b, sizeof b, "(%lld, %lld, %lld)", (long long)v->dim[0], (long long)v->dim[1],
+                  (long long)v->dim[2]);
+    return b;
+}
+
+int sync_copy_out(double* host, const double* dev, size_t n)
+{
+    if (n == 0) return WENOX_OK;
+    HIPCHK(hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, g_drop.stream));
+    return WENOX_OK;
+}
+
+int upload4(const wenox_view3* v, int64_t M, int64_t N, double* dev, double* pin)
+{
+    size_t n = (size_t)(M * N * 4);
+    if (n == 0) return WENOX_OK;
+    pack4(v, M, N, pin);
+    HIPCHK(hipMemcpyAsync(dev, pin, n * sizeof(double), hipMemcpyHostToDevice, g_drop.stream));
+    return WENOX_OK;
+}
+
+int finish()
+{
+    HIPCHK(hipStreamSynchronize(g_drop.stream));
+    HIPCHK(hipGetLastError());
+    return WENOX_OK;
+}
+
+int flux_common(const char* fn, const wenox_view3* q, double gamma, int dir, double* out)
+{
+    if (bad_view(q) || (!out && q->dim[0] * q->dim[1] > 0)) return fail(WENOX_E_ARG, std::string(fn) + ": null array");
+    const int64_t M = q->dim[0], N = q->dim[1];
+    if (M * N > 0 && q->dim[2] < 4) return shape_err(fn, "q needs at least 4 channels, got " + dims(q));
+    std::lock_guard<std::mutex> lk(g_drop.mu);
+    const size_t n = (size_t)(M * N * 4);
+    int rc = dropin_reserve(2 * n, n);
+    if (rc) return rc;
+    double* dq = g_drop.dev;
+    double* dout = dq + n;
+    if ((rc = upload4(q, M, N, dq, g_drop.pin))) return rc;
+    if (M * N > 0)
+        ops_for(g_variant)->flux(dq, lay_aos4(M, N), M, N, dir, gamma, dout, lay_aos4(M, N), g_drop.stream);
+    if ((rc = sync_copy_out(out, dout, n))) return rc;
+    return finish();
+}
+
+int weno_common(const char* fn, const wenox_view3* u, double gamma, int cons, int dir, double* ql,
+                double* fl, double* qr, double* fr)
+{
+    if (bad_view(u)) return fail(WENOX_E_ARG, std::string(fn) + ": null array");
+    const int64_t m = u->dim[0], n = u->dim[1];
+    // weno.rs:371 assert!(c >= 4); :385 m-5 / :461 n-6 (x) and :525 n-5 / :590 m-6 (y) in usize
+    if (u->dim[2] < 4) return shape_err(fn, "u must have at least 4 channels, got " + dims(u));
+    if (dir == 0 && (m < 5 || n < 6)) return shape_err(fn, "x reconstruction needs m >= 5 and n >= 6, got " + dims(u));
+    if (dir == 1 && (n < 5 || m < 6)) return shape_err(fn, "y reconstruction needs n >= 5 and m >= 6, got " + dims(u));
+    const int64_t F1 = dir == 0 ? m - 5 : m - 6, F2 = dir == 0 ? n - 6 : n - 5;
+    const int64_t p1 = dir == 0 ? (m - 5) * n * 4 : m * (n - 5) * 4;
+    const size_t nu = (size_t)(m * n * 4), nf = (size_t)(F1 * F2 * 4);
+    if (F1 * F2 > 0 && (!ql || !fl || !qr || !fr)) return fail(WENOX_E_ARG, std::string(fn) + ": null output");
+    std::lock_guard<std::mutex> lk(g_drop.mu);
+    int rc = dropin_reserve(nu + 2 * (size_t)p1 + 4 * nf, nu);
+    if (rc) return rc;
+    double* du = g_drop.dev;
+    double* pl = du + nu;
+    double* pr = pl + p1;
+    double* o[4];
+    o[0] = pr + p1;
+    for (int k = 1; k < 4; ++k) o[k] = o[k - 1] + nf;
+    if ((rc = upload4(u, m, n, du, g_drop.pin))) return rc;
+    ops_for(g_variant)->weno(du, lay_aos4(m, n), m, n, dir, cons, gamma, pl, pr, o[0], o[1], o[2],
+                             o[3], lay_aos4(F1, F2), g_drop.stream);
+    double* outs[4] = {ql, fl, qr, fr};
+    for (int k = 0; k < 4; ++k)
+        if ((rc = sync_copy_out(outs[k], o[k], nf))) return rc;
+    return finish();
+}
+
+// riemann.rs:6-486 entry: kind 0 HLLC, 1 HLLE, 2 blend
+int riemann_common(const char* fn, const wenox_view3* ql, const wenox_view3* fl,
+                   const wenox_view3* qr, const wenox_view3* fr, double gamma, int dir, int kind,
+                   double lo, double hi, double* out)
+{
+    if (bad_view(ql) || bad_view(fl) || bad_view(qr) || bad_view(fr))
+        return fail(WENOX_E_ARG, std::string(fn) + ": null array");
+    const int64_t F1 = ql->dim[0], F2 = ql->dim[1];
+    const wenox_view3* in[4] = {ql, fl, qr, fr};
+    if (F1 * F2 > 0) {
+        if (!out) return fail(WENOX_E_ARG, std::string(fn) + ": null output");
+        for (int k = 0; k < 4; ++k)
+            if (in[k]->dim[0] < F1 || in[k]->dim[1] < F2 || in[k]->dim[2] < 4)
+                return shape_err(fn, "face arrays must cover q_l's (F1, F2, 4) " + dims(ql) + ", got " + dims(in[k]));
+    }
+    if (kind == 2) {
+        // shock_sensor_grad_p (riemann.rs:360-387) indexes p[1], p[m-2], p[:, 1], p[:, n-2]
+        if (!((F1 >= 2 && F2 >= 2) || (F1 == 0 && F2 == 0)))
+            return shape_err(fn, "the pressure-gradient sensor needs a face grid of at least 2 x 2, got " + dims(ql));
+        // con2primi(q_r) is formed on q_r's own extent and read on q_l's
+        if (qr->dim[0] < F1 || qr->dim[1] < F2) return shape_err(fn, "q_r smaller than q_l");
+    }
+    std::lock_guard<std::mutex> lk(g_drop.mu);
+    const size_t nf = (size_t)(F1 * F2 * 4);
+    int rc = dropin_reserve(5 * nf + (size_t)(F1 * F2), 4 * nf);
+    if (rc) return rc;
+    double* d[4];
+    for (int k = 0; k < 4; ++k) d[k] = g_drop.dev + k * nf;
+    double* dout = g_drop.dev + 4 * nf;
+    double* pbar = dout + nf;
+    for (int k = 0; k < 4; ++k)
+        if ((rc = upload4(in[k], F1, F2, d[k], g_drop.pin + k * nf))) return rc;
+    if (F1 * F2 > 0)
+        ops_for(g_variant)->riemann(d[0], d[1], d[2], d[3], lay_aos4(F1, F2), F1, F2, dir, kind,
+                                    gamma, lo, hi, pbar, dout, lay_aos4(F1, F2), g_drop.stream);
+    if ((rc = sync_copy_out(out, dout, nf))) return rc;
+    return finish();
+}
+
+}  // namespace
+
+// ==========================================================================
+// extern "C": misc
+// ==========================================================================
+extern "C" const char* wenox_last_error(void) { return g_err.c_str(); }
+
+extern "C" const char* wenox_version(void) { return "wenox 0.1.0 (gfx950)"; }
+
+extern "C" int wenox_set_variant(int variant)
+{
+    if (variant != WENOX_EXACT && variant != WENOX_FAST) return fail(WENOX_E_ARG, "variant must be 0 or 1");
+    g_variant = variant;
+    return WENOX_OK;
+}
+
+extern "C" int wenox_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+// ==========================================================================
+// extern "C": drop-in functions (weno_ext/src/lib.rs:18-153)
+// ==========================================================================
+extern "C" int wenox_flux_x(const wenox_view3* q, double gamma, double* out)
+{
+    return flux_common("flux_x_py", q, gamma, 0, out);
+}
+
+extern "C" int wenox_flux_y(const wenox_view3* q, double gamma, double* out)
+{
+    return flux_common("flux_y_py", q, gamma, 1, out);
+}
+
+extern "C" int wenox_con2primi(const wenox_view3* q, double gamma, double* p, double* a,
+                               double* rho, double* u, double* v, double* h)
+{
+    const char* fn = "con2primi_py";
+    if (bad_view(q)) return fail(WENOX_E_ARG, "con2primi_py: null array");
+    const int64_t M = q->dim[0], N = q->dim[1];
+    if (M * N > 0 && q->dim[2] < 4) return shape_err(fn, "q needs at least 4 channels, got " + dims(q));
+    double* outs[6] = {p, a, rho, u, v, h};
+    for (int k = 0; k < 6; ++k)
+        if (!outs[k] && M * N > 0) return fail(WENOX_E_ARG, "con2primi_py: null output");
+    std::lock_guard<std::mutex> lk(g_drop.mu);
+    const size_t n4 = (size_t)(M * N * 4), n1 = (size_t)(M * N);
+    int rc = dropin_reserve(n4 + 6 * n1, n4);
+    if (rc) return rc;
+    double* dq = g_drop.dev;
+    double* o = dq + n4;
+    if ((rc = upload4(q, M, N, dq, g_drop.pin))) return rc;
+    if (M * N > 0)
+        ops_for(g_variant)->con2primi(dq, lay_aos4(M, N), M, N, gamma, o, o + n1, o + 2 * n1,
+                                      o + 3 * n1, o + 4 * n1, o + 5 * n1, g_drop.stream);
+    for (int k = 0; k < 6; ++k)
+        if ((rc = sync_copy_out(outs[k], o + k * n1, n1))) return rc;
+    return finish();
+}
+
+extern "C" int wenox_weno_x(const wenox_view3* u, double gamma, int cons, double* ql, double* fl,
+                            double* qr, double* fr)
+{
+    return weno_common("weno_x_py", u, gamma, cons, 0, ql, fl, qr, fr);
+}
+
+extern "C" int wenox_weno_y(const wenox_view3* u, double gamma, int cons, double* ql, double* fl,
+                            double* qr, double* fr)
+{
+    return weno_common("weno_y_py", u, gamma, cons, 1, ql, fl, qr, fr);
+}
+
+extern "C" int wenox_hllc_x(const wenox_view3* ql, const wenox_view3* fl, const wenox_view3* qr,
+                            const wenox_view3* fr, double gamma, double* out)
+{
+    return riemann_common("hllc_x_rs_py", ql, fl, qr, fr, gamma, 0, 0, 0, 0, out);
+}
+
+extern "C" int wenox_hllc_y(const wenox_view3* ql, const wenox_view3* fl, const wenox_view3* qr,
+                            const wenox_view3* fr, double gamma, double* out)
+{
+    return riemann_common("hllc_y_rs_py", ql, fl, qr, fr, gamma, 1, 0, 0, 0, out);
+}
+
+extern "C" int wenox_hlle_x(const wenox_view3* ql, const wenox_view3* fl, const wenox_view3* qr,
+                            const wenox_view3* fr, double gamma, double* out)
+{
+    return riemann_common("hlle_x_rs_py", ql, fl, qr, fr, gamma, 0, 1, 0, 0, out);
+}
+
+extern "C" int wenox_hlle_y(const wenox_view3* ql, const wenox_view3* fl, const wenox_view3* qr,
+                            const wenox_view3* fr, double gamma, double* out)
+{
+    return riemann_common("hlle_y_rs_py", ql, fl, qr, fr, gamma, 1, 1, 0, 0, out);
+}
+
+extern "C" int wenox_hllc_hlle_x(const wenox_view3* ql, const wenox_view3* fl,
+                                 const wenox_view3* qr, const wenox_view3* fr, double gamma,
+                                 double jp_lo, double jp_hi, int force_hlle, double* out)
+{
+    return riemann_common("hllc_hlle_x_rs_py", ql, fl, qr, fr, gamma, 0, force_hlle ? 1 : 2,
+                          jp_lo, jp_hi, out);
+}
+
+extern "C" int wenox_hllc_hlle_y(const wenox_view3* ql, const wenox_view3* fl,
+                                 const wenox_view3* qr, const wenox_view3* fr, double gamma,
+                                 double jp_lo, double jp_hi, int force_hlle, double* out)
+{
+    return riemann_common("hllc_hlle_y_rs_py", ql, fl, qr, fr, gamma, 1, force_hlle ? 1 : 2,
+                          jp_lo, jp_hi, out);
+}
+
+extern "C" int wenox_l_local(const wenox_view3* u, double dx, double gamma, int force_hlle,
+                             double jp_lo, double jp_hi, int cons, double* out)
+{
+    const char* fn = "l_local_py";
+    if (bad_view(u)) return fail(WENOX_E_ARG, "l_local_py: null array");
+    const int64_t m = u->dim[0], n = u->dim[1];
+    if (u->dim[2] < 4) return shape_err(fn, "u must have at least 4 channels, got " + dims(u));
+    if (m < 6 || n < 6) return shape_err(fn, "u needs m >= 6 and n >= 6 (3 ghost cells per side), got " + dims(u));
+    if (!force_hlle && (m < 8 || n < 8))
+        return shape_err(fn, "the HLLC/HLLE blend's sensor needs at least 2 x 2 interior cells, got " + dims(u));
+    const int64_t nx = m - 6, ny = n - 6;
+    const size_t nu = (size_t)(m * n * 4), no = (size_t)(nx * ny * 4);
+    if (no > 0 && !out) return fail(WENOX_E_ARG, "l_local_py: null output");
+    const size_t scratch = rhs_unfused_scratch(nx, ny);
+    std::lock_guard<std::mutex> lk(g_drop.mu);
+    int rc = dropin_reserve(nu + scratch + no, nu);
+    if (rc) return rc;
+    double* du = g_drop.dev;
+    double* work = du + nu;
+    double* dout = work + scratch;
+    if ((rc = upload4(u, m, n, du, g_drop.pin))) return rc;
+    // reference pass structure on AoS4 face arrays, output in AoS4
+    const Ops* ops = ops_for(g_variant);
+    const int64_t p1 = std::max((m - 5) * n, m * (n - 5)) * 4;
+    const int64_t fxs = 4 * (nx + 1) * ny, fys = 4 * nx * (ny + 1), fm = std::max(fxs, fys);
+    double* pl = work;
+    double* pr = pl + p1;
+    double* f0 = pr + p1;
+    double* fx = f0 + 4 * fm;
+    double* fy = fx + fxs;
+    double* pbar = fy + fys;
+    const int kind = force_hlle ? 1 : 2;
+    hipStream_t s = g_drop.stream;
+    Lay lx = lay_aos4(nx + 1, ny), ly = lay_aos4(nx, ny + 1);
+    ops->weno(du, lay_aos4(m, n), m, n, 0, cons, gamma, pl, pr, f0, f0 + fm, f0 + 2 * fm,
+              f0 + 3 * fm, lx, s);
+    ops->riemann(f0, f0 + fm, f0 + 2 * fm, f0 + 3 * fm, lx, nx + 1, ny, 0, kind, gamma, jp_lo,
+                 jp_hi, pbar, fx, lx, s);
+    ops->weno(du, lay_aos4(m, n), m, n, 1, cons, gamma, pl, pr, f0, f0 + fm, f0 + 2 * fm,
+              f0 + 3 * fm, ly, s);
+    ops->riemann(f0, f0 + fm, f0 + 2 * fm, f0 + 3 * fm, ly, nx, ny + 1, 1, kind, gamma, jp_lo,
+                 jp_hi, pbar, fy, ly, s);
+    ops->divergence(fx, lx, fy, ly, nx, ny, dx, dout, lay_aos4(nx, ny), s);
+    if ((rc = sync_copy_out(out, dout, no))) return rc;
+    return finish();
+}
+
+// ==========================================================================
+// Device-resident solver
+// ==========================================================================
+struct wenox_solver {
+    wenox_solver_desc d{};
+    const Ops* ops = nullptr;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double* U = nullptr;     // U^n, padded SoA4
+    double* A = nullptr;     // stage 1 output
+    double* B = nullptr;     // stage 2 output
+    double* UT = nullptr;    // transposed copies (fused engine only)
+    double* AT = nullptr;
+    double* BT = nullptr;
+    double* L = nullptr;     // RHS (unfused) / Dx (fused), interior SoA4
+    double* DyT = nullptr;   // fused: y flux differences, transposed interior
+    double* work = nullptr;  // unfused face arrays, download scratch
+    size_t work_n = 0;
+    StepCtl* ctl = nullptr;  // device
+    BcParams bp{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double last_ms = 0.0;
+    SweepPlan plan{};
+    const SweepOps* sops = nullptr;
+};
+
+namespace {
+
+int64_t padded(const wenox_solver* s) { return (s->d.nx + 6) * (s->d.ny + 6); }
+
+// One time step on the solver's stream: CFL -> 3 x (RHS, RK combine, ghost fill) -> advance.
+void enqueue_step(wenox_solver* s)
+{
+    const auto& d = s->d;
+    const Ops* o = s->ops;
+    hipStream_t st = s->stream;
+    o->cfl(s->U, d.nx, d.ny, d.gamma, s->ctl, st);
+    double* src[3] = {s->U, s->A, s->B};
+    double* dst[3] = {s->A, s->B, s->U};
+    double* srcT[3] = {s->UT, s->AT, s->BT};
+    double* dstT[3] = {s->AT, s->BT, s->UT};
+    for (int stage = 1; stage <= 3; ++stage) {
+        const int i = stage - 1;
+        if (d.engine == WENOX_ENGINE_FUSED) {
+            s->sops->stage(s->plan, src[i], srcT[i], s->U, s->L, s->DyT, dst[i], dstT[i], stage,
+                           s->ctl, st);
+        } else {
+            o->rhs_unfused(src[i], d.nx, d.ny, d.dx, d.gamma, d.cons_switch, d.force_hlle,
+                           d.jp_lo, d.jp_hi, s->work, s->L, s->ctl, st);
+            o->rk(s->U, src[i], s->L, dst[i], d.nx, d.ny, stage, s->ctl, st);
+        }
+        o->bc(dst[i], dstT[i], d.nx, d.ny, s->bp, s->ctl, stage, st);
+    }
+    o->advance(s->ctl, st);
+}
+
+int read_ctl(wenox_solver* s, StepCtl* h)
+{
+    HIPCHK(hipMemcpyAsync(h, s->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return WENOX_OK;
+}
+
+int write_ctl(wenox_solver* s, const StepCtl* h)
+{
+    HIPCHK(hipMemcpyAsync(s->ctl, h, sizeof(StepCtl), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return WENOX_OK;
+}
+
+// Host copy of the DMR ghost fill (unitest.py:207-227) on an 8-channel array,
+// applied after the primitive sync exactly as rk3_refac does (:127-134).
+void dmr_ref_fill8(double* q, int64_t nx, int64_t ny, int bound)
+{
+    static const double post[8] = {8, 33, -57.1576, 563.499, 8, 4.125, -7.1447, 116.5};
+    const int64_t n = ny + 6;
+    auto at = [&](int64_t i, int64_t j) { return q + (i * n + j) * 8; };
+    for (int64_t i = 0; i < nx + 6; ++i)
+        for (int64_t j = 0; j < n; ++j) {
+            bool ghost = i < 3 || i >= nx + 3 || j < 3 || j >= ny + 3;
+            if (!ghost) continue;
+            const double* src = nullptr;
+            bool flip = false;
+            int64_t si = i, sj = j;
+            if (i < 3) {
+                src = post;
+            } else {
+                if (i >= nx + 3) si = nx + 2;
+                if (j >= ny + 3) sj = ny + 2;
+                else if (j < 3) {
+                    if (i >= nx + 3) { sj = 5 - j; flip = true; }
+                    else if (i <= bound) src = post;
+                    else if (i <= bound + 9) sj = 3;
+                    else { sj = 5 - j; flip = true; }
+                }
+            }
+            double v[8];
+            std::memcpy(v, src ? src : at(si, sj), sizeof v);
+            if (flip) { v[2] = -v[2]; v[6] = -v[6]; }
+            std::memcpy(at(i, j), v, sizeof v);
+        }
+}
+
+}  // namespace
+
+extern "C" int wenox_solver_create(const wenox_solver_desc* desc, wenox_solver** out)
+{
+    if (!desc || !out) return fail(WENOX_E_ARG, "wenox_solver_create: null argument");
+    *out = nullptr;
+    const auto& d = *desc;
+    if (d.nx < 3 || d.ny < 3) return fail(WENOX_E_SHAPE, "solver needs nx >= 3 and ny >= 3");
+    if (!d.force_hlle && (d.nx < 2 || d.ny < 2)) return fail(WENOX_E_SHAPE, "blend needs nx, ny >= 2");
+    if (d.bc < 0 || d.bc > 3) return fail(WENOX_E_ARG, "unknown boundary kind");
+    if (d.bc == WENOX_BC_DMR_REF && d.nx + 2 < d.dmr_bound + 10)
+        return fail(WENOX_E_SHAPE, "DMR reference ghost fill needs nx + 2 >= bound + 10");
+    if (d.variant != WENOX_EXACT && d.variant != WENOX_FAST) return fail(WENOX_E_ARG, "unknown variant");
+    if (d.engine != WENOX_ENGINE_UNFUSED && d.engine != WENOX_ENGINE_FUSED)
+        return fail(WENOX_E_ARG, "unknown engine");
+    if (d.precision != WENOX_PREC_FP64) return fail(WENOX_E_ARG, "precision not available in this build");
+    if (!(d.dx > 0.0) || !(d.cfl > 0.0)) return fail(WENOX_E_ARG, "dx and cfl must be positive");
+    int rc = require_device();
+    if (rc) return rc;
+    auto* s = new wenox_solver();
+    s->d = d;
+    s->ops = ops_for(d.variant);
+    (void)hipGetDevice(&s->device);
+    auto cleanup = [&](int code) {
+        wenox_solver_destroy(s);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
+        return cleanup(fail(WENOX_E_DEVICE, "stream/event creation failed"));
+    const size_t P = (size_t)padded(s) * 4, I = (size_t)(d.nx * d.ny) * 4;
+    const bool fused = d.engine == WENOX_ENGINE_FUSED;
+    s->work_n = std::max(rhs_unfused_scratch(d.nx, d.ny), (size_t)padded(s) * 6);
+    if (fused) {
+        s->plan = make_sweep_plan(d.nx, d.ny, d.dx, d.gamma, d.cons_switch, d.force_hlle, d.jp_lo, d.jp_hi);
+        s->sops = d.variant == WENOX_FAST ? &sweep_ops_fast : &sweep_ops_exact;
+        s->work_n = (size_t)padded(s) * 6;
+    }
+    bool ok = hipMalloc(&s->U, P * 8) == hipSuccess && hipMalloc(&s->A, P * 8) == hipSuccess &&
+              hipMalloc(&s->B, P * 8) == hipSuccess &&
+              hipMalloc(&s->L, std::max<size_t>(I, 4) * 8) == hipSuccess &&
+              hipMalloc(&s->work, s->work_n * 8) == hipSuccess &&
+              hipMalloc(&s->ctl, sizeof(StepCtl)) == hipSuccess;
+    if (ok && fused)
+        ok = hipMalloc(&s->UT, P * 8) == hipSuccess && hipMalloc(&s->AT, P * 8) == hipSuccess &&
+             hipMalloc(&s->BT, P * 8) == hipSuccess && hipMalloc(&s->DyT, std::max<size_t>(I, 4) * 8) == hipSuccess;
+    if (!ok)
+        return cleanup(fail(WENOX_E_DEVICE, "device allocation failed (state of " +
+                                                std::to_string(((fused ? 6 : 3) * P + 2 * I + s->work_n) * 8 >> 20) +
+                                                " MiB)"));
+    s->bp.kind = d.bc;
+    s->bp.dmr_bound = d.dmr_bound;
+    s->bp.dx = d.dx;
+    s->bp.dy = d.dx;
+    s->bp.x0 = d.dmr_x0;
+    s->bp.gamma = d.gamma;
+    StepCtl h{};
+    h.cfl_dx = d.cfl * d.dx;
+    h.t_final = -1.0;
+    h.active = 1;
+    if ((rc = write_ctl(s, &h))) return cleanup(rc);
+    *out = s;
+    return WENOX_OK;
+}
+
+extern "C" void wenox_solver_destroy(wenox_solver* s)
+{
+    if (!s) return;
+    if (s->exec) (void)hipGraphExecDestroy(s->exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    for (double* p : {s->U, s->A, s->B, s->UT, s->AT, s->BT, s->L, s->DyT, s->work})
+        if (p) (void)hipFree(p);
+    if (s->ctl) (void)hipFree(s->ctl);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+extern "C" int wenox_solver_upload(wenox_solver* s, const wenox_view3* q, double t0)
+{
+    if (!s || bad_view(q)) return fail(WENOX_E_ARG, "wenox_solver_upload: null argument");
+    const auto& d = s->d;
+    const int64_t m = d.nx + 6, n = d.ny + 6;
+    if (q->dim[0] != m || q->dim[1] != n || q->dim[2] < 4)
+        return fail(WENOX_E_SHAPE, "state must be (nx+6, ny+6, c>=4), got " + dims(q));
+    HIPCHK(hipSetDevice(s->device));
+    // pack to SoA4 in pinned memory
+    const size_t P = (size_t)(m * n);
+    double* pin = nullptr;
+    HIPCHK(hipHostMalloc(&pin, P * 4 * sizeof(double)));
+    const int64_t s0 = q->stride[0], s1 = q->stride[1], s2 = q->stride[2];
+    for (int64_t i = 0; i < m; ++i)
+        for (int64_t j = 0; j < n; ++j) {
+            const double* src = q->data + i * s0 + j * s1;
+            for (int k = 0; k < 4; ++k) pin[k * P + i * n + j] = src[k * s2];
+        }
+    StepCtl h{};
+    h.cfl_dx = d.cfl * d.dx;
+    h.t_final = -1.0;
+    h.T = t0;
+    h.active = 1;
+    if (q->dim[2] >= 8) {
+        // first-step wave speed from the caller's primitives (Rie_prob_2d_rs.py:312-315)
+        double cm = 0.0;
+        bool nan = false;
+        for (int64_t i = 3; i < 3 + d.nx && !nan; ++i)
+            for (int64_t j = 3; j < 3 + d.ny; ++j) {
+                const double* c = q->data + i * s0 + j * s1;
+                double u = c[5 * s2], v = c[6 * s2], p = c[7 * s2], r = c[0];
+                double w = std::sqrt(d.gamma * p / std::max(r, 1e-14)) + std::sqrt(u * u + v * v);
+                if (w != w) { cm = w; nan = true; break; }
+                if (w > cm) cm = w;
+            }
+        h.cmax0 = cm;
+        h.use_cmax0 = 1;
+    }
+    hipError_t e = hipMemcpy(s->U, pin, P * 4 * sizeof(double), hipMemcpyHostToDevice);
+    (void)hipHostFree(pin);
+    if (e != hipSuccess) return fail(WENOX_E_DEVICE, std::string("upload: ") + hipGetErrorString(e));
+    int rc = write_ctl(s, &h);
+    if (rc) return rc;
+    if (s->UT) s->sops->transpose(s->U, s->UT, m, n, s->stream);
+    s->ops->bc(s->U, s->UT, d.nx, d.ny, s->bp, s->ctl, 0, s->stream);
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipGetLastError());
+    return WENOX_OK;
+}
+
+extern "C" int wenox_solver_download(wenox_solver* s, double* q8)
+{
+    if (!s || !q8) return fail(WENOX_E_ARG, "wenox_solver_download: null argument");
+    const auto& d = s->d;
+    const int64_t m = d.nx + 6, n = d.ny + 6;
+    const size_t P = (size_t)(m * n);
+    HIPCHK(hipSetDevice(s->device));
+    double* w = s->work;
+    s->ops->con2primi(s->U, lay_soa4(m, n), m, n, d.gamma, w, w + P, w + 2 * P, w + 3 * P,
+                      w + 4 * P, w + 5 * P, s->stream);
+    std::vector<double> cons(P * 4), prim(P * 6);
+    HIPCHK(hipMemcpyAsync(cons.data(), s->U, P * 4 * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(prim.data(), w, P * 6 * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (size_t c = 0; c < P; ++c) {
+        double* o = q8 + c * 8;
+        for (int k = 0; k < 4; ++k) o[k] = cons[k * P + c];
+        o[4] = prim[2 * P + c];   // rho_safe
+        o[5] = prim[3 * P + c];   // u
+        o[6] = prim[4 * P + c];   // v
+        o[7] = prim[0 * P + c];   // p
+    }
+    if (d.bc == WENOX_BC_DMR_REF) dmr_ref_fill8(q8, d.nx, d.ny, d.dmr_bound);
+    return WENOX_OK;
+}
+
+extern "C" int wenox_solver_rhs(wenox_solver* s, double* out)
+{
+    if (!s || !out) return fail(WENOX_E_ARG, "wenox_solver_rhs: null argument");
+    const auto& d = s->d;
+    HIPCHK(hipSetDevice(s->device));
+    StepCtl h;
+    int rc = read_ctl(s, &h);
+    if (rc) return rc;
+    StepCtl act = h;
+    act.active = 1;
+    act.dt = 0.0;
+    if ((rc = write_ctl(s, &act))) return rc;
+    if (d.engine == WENOX_ENGINE_FUSED) {
+        s->sops->rhs(s->plan, s->U, s->UT, s->L, s->DyT, s->work, s->ctl, s->stream);
+        HIPCHK(hipMemcpyAsync(s->L, s->work, (size_t)(d.nx * d.ny) * 4 * 8, hipMemcpyDeviceToDevice, s->stream));
+    } else {
+        s->ops->rhs_unfused(s->U, d.nx, d.ny, d.dx, d.gamma, d.cons_switch, d.force_hlle, d.jp_lo,
+                            d.jp_hi, s->work, s->L, s->ctl, s->stream);
+    }
+    const size_t I = (size_t)(d.nx * d.ny);
+    std::vector<double> soa(I * 4);
+    HIPCHK(hipMemcpyAsync(soa.data(), s->L, I * 4 * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipGetLastError());
+    for (size_t c = 0; c < I; ++c)
+        for (int k = 0; k < 4; ++k) out[c * 4 + k] = soa[k * I + c];
+    return write_ctl(s, &h);
+}
+
+extern "C" int wenox_solver_run(wenox_solver* s, int64_t max_steps, double t_final, double* t_out,
+                                int64_t* steps_out)
+{
+    if (!s) return fail(WENOX_E_ARG, "wenox_solver_run: null solver");
+    if (max_steps < 0) return fail(WENOX_E_ARG, "max_steps must be >= 0");
+    HIPCHK(hipSetDevice(s->device));
+    StepCtl h;
+    int rc = read_ctl(s, &h);
+    if (rc) return rc;
+    const long long step0 = h.step;
+    h.max_steps = step0 + max_steps;
+    h.t_final = t_final;
+    h.active = 1;
+    if ((rc = write_ctl(s, &h))) return rc;
+    if (s->d.use_graph && !s->exec) {
+        HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_step(s);
+        HIPCHK(hipStreamEndCapture(s->stream, &s->graph));
+        HIPCHK(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
+    }
+    auto launch = [&](int64_t k) -> int {
+        for (int64_t i = 0; i < k; ++i) {
+            if (s->exec) {
+                HIPCHK(hipGraphLaunch(s->exec, s->stream));
+            } else {
+                enqueue_step(s);
+            }
+        }
+        return WENOX_OK;
+    };
+    HIPCHK(hipEventRecord(s->ev0, s->stream));
+    int64_t launched = 0;
+    if (t_final < 0.0) {
+        if ((rc = launch(max_steps))) return rc;
+        launched = max_steps;
+    } else {
+        // Time-controlled: launch chunks sized by the remaining time / dt.
+        while (launched < max_steps) {
+            if ((rc = read_ctl(s, &h))) return rc;
+            if (h.step >= h.max_steps) break;
+            if (launched > 0 && !h.active) break;
+            if (launched > 0 && !(h.T < t_final - 1e-12)) break;
+            int64_t chunk = 1;
+            if (h.dt > 0.0 && launched > 0) {
+                double est = (t_final - h.T) / h.dt;
+                chunk = (int64_t)std::min(64.0, std::max(1.0, std::floor(est)));
+            }
+            chunk = std::min(chunk, max_steps - launched);
+            if ((rc = launch(chunk))) return rc;
+            launched += chunk;
+        }
+    }
+    HIPCHK(hipEventRecord(s->ev1, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipGetLastError());
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    if ((rc = read_ctl(s, &h))) return rc;
+    const int64_t taken = h.step - step0;
+    s->last_ms = taken > 0 ? ms / (double)taken : 0.0;
+    if (t_out) *t_out = h.T;
+    if (steps_out) *steps_out = taken;
+    return WENOX_OK;
+}
+
+extern "C" double wenox_solver_last_ms_per_step(wenox_solver* s) { return s ? s->last_ms : 0.0; }

@@ -1,0 +1,551 @@
+// wenox_numerics.h -- point-wise numerics of the 2D Euler WENO5 / HLLC
+// right-hand side, shared by every kernel (device) and usable on the host.
+//
+// Each function restates one piece of the reference Rust crate weno_ext; the
+// file:line it follows is given above it.  The translation unit that includes
+// this header picks one of two arithmetic variants:
+//
+//   WX_FAST == 0  ("exact"): the reference's operation order, true IEEE
+//                 divisions, built with -ffp-contract=off.  Bitwise equal to
+//                 the reference semantics wherever division and sqrt are
+//                 correctly rounded (they are on gfx950).
+//   WX_FAST == 1  ("fast"):  algebraically equal rewrites that remove most
+//                 divisions (one division per WENO5 evaluation instead of
+//                 seven, reciprocals shared across a face) and FMA
+//                 contraction.  Results differ from "exact" by rounding only.
+//
+// T is double (FP64 path) or float (the bulk arithmetic of the FP32 path).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#ifndef WX_FAST
+#error "WX_FAST must be defined to 0 or 1"
+#endif
+
+#define WX_HD __host__ __device__ __forceinline__
+
+namespace wx {
+
+enum Dir : int { DIR_X = 0, DIR_Y = 1 };
+
+// utils.rs:3-6
+template <typename T> struct K {
+    static constexpr T eps = T(1e-12);        // DEFAULT_EPS = RHO_MIN = P_MIN
+    static constexpr T sqrt3_12 = T(0.1443375673);
+};
+
+template <typename T> WX_HD T sq(T x) { return x * x; }
+template <typename T> WX_HD T vmax(T a, T b) { return fmax(a, b); }   // Rust f64::max
+template <typename T> WX_HD T vmin(T a, T b) { return fmin(a, b); }   // Rust f64::min
+
+// Reciprocal used only by the fast variant.  Divisors passed here are
+// strictly positive and finite by construction (squares offset by 1e-12,
+// floored densities, sums of positive terms).
+WX_HD double rcp(double x)
+{
+#if WX_FAST && defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+#else
+    return 1.0 / x;
+#endif
+}
+WX_HD float rcp(float x)
+{
+#if WX_FAST && defined(__HIP_DEVICE_COMPILE__)
+    float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(r, fmaf(-x, r, 1.0f), r);
+#else
+    return 1.0f / x;
+#endif
+}
+
+template <typename T> struct V4 { T v[4]; };
+
+// ---------------------------------------------------------------------
+// utils.rs:74-108  Euler flux normal to x (dir 0) or y (dir 1).
+// Density is floored, pressure is not.
+// ---------------------------------------------------------------------
+template <typename T> WX_HD void flux_pt(int dir, const T q[4], T gam, T f[4])
+{
+    T rho = vmax(q[0], K<T>::eps);
+#if WX_FAST
+    T ir = rcp(rho);
+    T u = q[1] * ir, v = q[2] * ir;
+#else
+    T u = q[1] / rho, v = q[2] / rho;
+#endif
+    T p = (gam - T(1)) * (q[3] - T(0.5) * rho * (u * u + v * v));
+    if (dir == DIR_X) {
+        f[0] = q[1];
+        f[1] = q[1] * u + p;
+        f[2] = q[2] * u;
+        f[3] = u * (q[3] + p);
+    } else {
+        f[0] = q[2];
+        f[1] = q[1] * v;
+        f[2] = q[2] * v + p;
+        f[3] = v * (q[3] + p);
+    }
+}
+
+// utils.rs:9-35  conservative -> (p, a, rho_safe, u, v, h); floors on rho and p.
+template <typename T> struct Prim { T p, a, rho, u, v, h; };
+template <typename T> WX_HD Prim<T> con2primi_pt(const T q[4], T gam)
+{
+    Prim<T> r;
+    r.rho = vmax(q[0], K<T>::eps);
+    r.u = q[1] / r.rho;
+    r.v = q[2] / r.rho;
+    r.p = vmax((gam - T(1)) * (q[3] - T(0.5) * r.rho * (r.u * r.u + r.v * r.v)), K<T>::eps);
+    r.a = sqrt(gam * r.p / r.rho);
+    r.h = (q[3] + r.p) / r.rho;
+    return r;
+}
+
+// Floored pressure only (the sensor's input), same arithmetic as con2primi_pt.
+template <typename T> WX_HD T pressure_floored(const T q[4], T gam)
+{
+    T rho = vmax(q[0], K<T>::eps);
+#if WX_FAST
+    T ir = rcp(rho);
+    T u = q[1] * ir, v = q[2] * ir;
+#else
+    T u = q[1] / rho, v = q[2] / rho;
+#endif
+    return vmax((gam - T(1)) * (q[3] - T(0.5) * rho * (u * u + v * v)), K<T>::eps);
+}
+
+// ---------------------------------------------------------------------
+// weno.rs:13-63  Roe average for the characteristic basis (same for x, y).
+// ---------------------------------------------------------------------
+template <typename T> struct Roe { T rho, u, v, h, c; };
+template <typename T> WX_HD Roe<T> roe_avg(const T ul[4], const T ur[4], T gam)
+{
+    const T eps = K<T>::eps;
+    T rho_l = vmax(ul[0], eps);
+    T rho_r = vmax(ur[0], eps);
+    T sl = sqrt(rho_l), sr = sqrt(rho_r);
+    T den = sl + sr;
+#if WX_FAST
+    T il = rcp(rho_l), ir = rcp(rho_r), id = rcp(den);
+    T u_l = ul[1] * il, v_l = ul[2] * il;
+    T u_r = ur[1] * ir, v_r = ur[2] * ir;
+    T p_l = vmax((gam - T(1)) * (ul[3] - T(0.5) * rho_l * (u_l * u_l + v_l * v_l)), eps);
+    T p_r = vmax((gam - T(1)) * (ur[3] - T(0.5) * rho_r * (u_r * u_r + v_r * v_r)), eps);
+    T h_l = (ul[3] + p_l) * il;
+    T h_r = (ur[3] + p_r) * ir;
+    Roe<T> r;
+    r.u = (sl * u_l + sr * u_r) * id;
+    r.v = (sl * v_l + sr * v_r) * id;
+    r.h = (sl * h_l + sr * h_r) * id;
+    T kin = T(0.5) * (r.u * r.u + r.v * r.v);
+    r.c = sqrt(vmax((gam - T(1)) * (r.h - kin), eps * vmax(il, ir)));
+#else
+    T u_l = ul[1] / rho_l, v_l = ul[2] / rho_l;
+    T u_r = ur[1] / rho_r, v_r = ur[2] / rho_r;
+    T p_l = vmax((gam - T(1)) * (ul[3] - T(0.5) * rho_l * (u_l * u_l + v_l * v_l)), eps);
+    T p_r = vmax((gam - T(1)) * (ur[3] - T(0.5) * rho_r * (u_r * u_r + v_r * v_r)), eps);
+    T h_l = (ul[3] + p_l) / rho_l;
+    T h_r = (ur[3] + p_r) / rho_r;
+    Roe<T> r;
+    r.u = (sl * u_l + sr * u_r) / den;
+    r.v = (sl * v_l + sr * v_r) / den;
+    r.h = (sl * h_l + sr * h_r) / den;
+    T kin = T(0.5) * (r.u * r.u + r.v * r.v);
+    r.c = sqrt(vmax((gam - T(1)) * (r.h - kin), eps / vmin(rho_l, rho_r)));
+#endif
+    r.rho = sl * sr;
+    return r;
+}
+
+// weno.rs:76-189  left (rows: u-c, entropy, shear, u+c) and right
+// eigenvector matrices of the x- or y-flux Jacobian at the Roe state.
+template <typename T> struct Mat4 { T m[4][4]; };
+
+template <typename T> WX_HD void eig_left(int dir, const Roe<T>& s, T gam, T L[4][4])
+{
+    const T rho = s.rho, u = s.u, v = s.v, c = s.c;
+    T g1 = gam - T(1);
+    T kin = T(0.5) * (u * u + v * v);
+    T un = dir == DIR_X ? u : v;
+#if WX_FAST
+    T ic = rcp(c);
+    T inv_c2 = ic * ic;
+    T irho = rcp(rho);
+    T inv_rc = irho * ic;
+    T a = g1 * kin * (T(0.5) * inv_c2);
+    T b = un * (T(0.5) * ic);
+    T hu = T(0.5) * g1 * u * inv_c2, hv = T(0.5) * g1 * v * inv_c2;
+    T h3 = T(0.5) * g1 * inv_c2;
+    T hr = T(0.5) * inv_rc;
+    L[0][0] = a + b;
+    L[3][0] = a - b;
+    L[1][0] = T(1) - g1 * kin * inv_c2;
+    L[1][1] = g1 * u * inv_c2;
+    L[1][2] = g1 * v * inv_c2;
+    L[1][3] = -g1 * inv_c2;
+    L[0][3] = h3;
+    L[3][3] = h3;
+    if (dir == DIR_X) {
+        L[0][1] = -hu - hr; L[0][2] = -hv;
+        L[3][1] = -hu + hr; L[3][2] = -hv;
+        L[2][0] = -v * irho; L[2][1] = T(0); L[2][2] = irho; L[2][3] = T(0);
+    } else {
+        L[0][1] = -hu; L[0][2] = -hv - hr;
+        L[3][1] = -hu; L[3][2] = -hv + hr;
+        L[2][0] = -u * irho; L[2][1] = irho; L[2][2] = T(0); L[2][3] = T(0);
+    }
+#else
+    T rc = rho * c;
+    T inv_rc = T(1) / rc;
+    T inv_c2 = T(1) / (c * c);
+    L[0][0] = g1 * kin / (T(2) * c * c) + un / (T(2) * c);
+    L[3][0] = g1 * kin / (T(2) * c * c) - un / (T(2) * c);
+    L[1][0] = T(1) - g1 * kin * inv_c2;
+    L[1][1] = g1 * u * inv_c2;
+    L[1][2] = g1 * v * inv_c2;
+    L[1][3] = -g1 * inv_c2;
+    L[0][3] = g1 * inv_c2 / T(2);
+    L[3][3] = g1 * inv_c2 / T(2);
+    if (dir == DIR_X) {
+        L[0][1] = -g1 * u * inv_c2 / T(2) - T(0.5) * inv_rc;
+        L[0][2] = -g1 * v * inv_c2 / T(2);
+        L[3][1] = -g1 * u * inv_c2 / T(2) + T(0.5) * inv_rc;
+        L[3][2] = -g1 * v * inv_c2 / T(2);
+        L[2][0] = -v / rho; L[2][1] = T(0); L[2][2] = T(1) / rho; L[2][3] = T(0);
+    } else {
+        L[0][1] = -g1 * u * inv_c2 / T(2);
+        L[0][2] = -g1 * v * inv_c2 / T(2) - T(0.5) * inv_rc;
+        L[3][1] = -g1 * u * inv_c2 / T(2);
+        L[3][2] = -g1 * v * inv_c2 / T(2) + T(0.5) * inv_rc;
+        L[2][0] = -u / rho; L[2][1] = T(1) / rho; L[2][2] = T(0); L[2][3] = T(0);
+    }
+#endif
+}
+
+template <typename T> WX_HD void eig_right(int dir, const Roe<T>& s, T R[4][4])
+{
+    const T rho = s.rho, u = s.u, v = s.v, h = s.h, c = s.c;
+    T kin = T(0.5) * (u * u + v * v);
+    R[0][0] = T(1); R[0][1] = T(1); R[0][2] = T(0); R[0][3] = T(1);
+    if (dir == DIR_X) {
+        R[1][0] = u - c; R[1][1] = u; R[1][2] = T(0); R[1][3] = u + c;
+        R[2][0] = v; R[2][1] = v; R[2][2] = rho; R[2][3] = v;
+        R[3][0] = h - u * c; R[3][1] = kin; R[3][2] = rho * v; R[3][3] = h + u * c;
+    } else {
+        R[1][0] = u; R[1][1] = u; R[1][2] = rho; R[1][3] = u;
+        R[2][0] = v - c; R[2][1] = v; R[2][2] = T(0); R[2][3] = v + c;
+        R[3][0] = h - v * c; R[3][1] = kin; R[3][2] = rho * u; R[3][3] = h + v * c;
+    }
+}
+
+// weno.rs:197-205  accumulation from 0.0 in column order
+template <typename T> WX_HD void mat4_vec4(const T M[4][4], const T x[4], T y[4])
+{
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#if WX_FAST
+        y[r] = M[r][0] * x[0] + M[r][1] * x[1] + M[r][2] * x[2] + M[r][3] * x[3];
+#else
+        T acc = T(0);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc += M[r][c] * x[c];
+        y[r] = acc;
+#endif
+    }
+}
+
+// ---------------------------------------------------------------------
+// WENO5-JS (weno.rs:214-257).  weno5_left: value at the right face of the
+// centre cell u2 from u0..u4; weno5_right: value at the left face of the
+// centre cell u3 from u1..u5.  eps = 1e-12, alpha_k = d_k / (eps+beta_k)^2.
+// The fast variant evaluates sum(alpha_k p_k) / (6 sum(alpha_k)) with all
+// three alphas scaled by prod_k (eps+beta_k)^2, i.e. one division.
+// ---------------------------------------------------------------------
+template <typename T> WX_HD T weno5_core(T b0, T b1, T b2, T d0, T d1, T d2, T p0, T p1, T p2)
+{
+    const T eps = K<T>::eps;
+#if WX_FAST
+    T t0 = sq(eps + b0), t1 = sq(eps + b1), t2 = sq(eps + b2);
+    T a0 = d0 * (t1 * t2), a1 = d1 * (t0 * t2), a2 = d2 * (t0 * t1);
+    return (a0 * p0 + a1 * p1 + a2 * p2) * rcp(T(6) * (a0 + a1 + a2));
+#else
+    T a0 = d0 / sq(eps + b0);
+    T a1 = d1 / sq(eps + b1);
+    T a2 = d2 / sq(eps + b2);
+    T s = a0 + a1 + a2;
+    T w0 = a0 / s, w1 = a1 / s, w2 = a2 / s;
+    return (w0 * p0 + w1 * p1 + w2 * p2) / T(6);
+#endif
+}
+
+template <typename T> WX_HD T weno5_left(T u0, T u1, T u2, T u3, T u4)
+{
+    const T c13 = T(13.0 / 12.0);
+    T b0 = c13 * sq(u2 - T(2) * u3 + u4) + T(0.25) * sq(T(3) * u2 - T(4) * u3 + u4);
+    T b1 = c13 * sq(u1 - T(2) * u2 + u3) + T(0.25) * sq(u1 - u3);
+    T b2 = c13 * sq(u0 - T(2) * u1 + u2) + T(0.25) * sq(u0 - T(4) * u1 + T(3) * u2);
+    T p0 = -u4 + T(5) * u3 + T(2) * u2;
+    T p1 = -u1 + T(5) * u2 + T(2) * u3;
+    T p2 = T(2) * u0 - T(7) * u1 + T(11) * u2;
+    return weno5_core(b0, b1, b2, T(0.3), T(0.6), T(0.1), p0, p1, p2);
+}
+
+template <typename T> WX_HD T weno5_right(T u1, T u2, T u3, T u4, T u5)
+{
+    const T c13 = T(13.0 / 12.0);
+    T b0 = c13 * sq(u3 - T(2) * u4 + u5) + T(0.25) * sq(T(3) * u3 - T(4) * u4 + u5);
+    T b1 = c13 * sq(u2 - T(2) * u3 + u4) + T(0.25) * sq(u2 - u4);
+    T b2 = c13 * sq(u1 - T(2) * u2 + u3) + T(0.25) * sq(u1 - T(4) * u2 + T(3) * u3);
+    T p0 = T(2) * u5 - T(7) * u4 + T(11) * u3;
+    T p1 = -u4 + T(5) * u3 + T(2) * u2;
+    T p2 = -u1 + T(5) * u2 + T(2) * u3;
+    return weno5_core(b0, b1, b2, T(0.1), T(0.6), T(0.3), p0, p1, p2);
+}
+
+// weno.rs:262-303  Gauss-point values (ga at -h/(2 sqrt 3), gb at +h/(2 sqrt 3))
+// from five face values, sharing the smoothness indicators.
+template <typename T> WX_HD void weno5_gq(T q0, T q1, T q2, T q3, T q4, T& ga, T& gb)
+{
+    const T c13 = T(13.0 / 12.0);
+    const T S = K<T>::sqrt3_12;
+    const T eps = K<T>::eps;
+    T b0 = c13 * sq(q2 - T(2) * q3 + q4) + T(0.25) * sq(T(3) * q2 - T(4) * q3 + q4);
+    T b1 = c13 * sq(q1 - T(2) * q2 + q3) + T(0.25) * sq(q1 - q3);
+    T b2 = c13 * sq(q0 - T(2) * q1 + q2) + T(0.25) * sq(q0 - T(4) * q1 + T(3) * q2);
+    const T dA = T(0.1928406937), dB = T(0.6111111111), dC = T(0.1960481952);
+#if WX_FAST
+    T s0 = T(3) * q2 - T(4) * q3 + q4;
+    T s1 = q1 - q3;
+    T s2 = T(3) * q2 - T(4) * q1 + q0;
+    T t0 = sq(eps + b0), t1 = sq(eps + b1), t2 = sq(eps + b2);
+    T e0 = t1 * t2, e1 = t0 * t2, e2 = t0 * t1;
+    {
+        T a0 = dA * e0, a1 = dB * e1, a2 = dC * e2;
+        T p0 = q2 + s0 * S, p1 = q2 + s1 * S, p2 = q2 - s2 * S;
+        ga = (a0 * p0 + a1 * p1 + a2 * p2) * rcp(a0 + a1 + a2);
+    }
+    {
+        T a0 = dC * e0, a1 = dB * e1, a2 = dA * e2;
+        T p0 = q2 - s0 * S, p1 = q2 - s1 * S, p2 = q2 + s2 * S;
+        gb = (a0 * p0 + a1 * p1 + a2 * p2) * rcp(a0 + a1 + a2);
+    }
+#else
+    {
+        T a0 = dA / sq(eps + b0), a1 = dB / sq(eps + b1), a2 = dC / sq(eps + b2);
+        T s = a0 + a1 + a2;
+        T w0 = a0 / s, w1 = a1 / s, w2 = a2 / s;
+        T p0 = q2 + (T(3) * q2 - T(4) * q3 + q4) * S;
+        T p1 = q2 - (-q1 + q3) * S;
+        T p2 = q2 - (T(3) * q2 - T(4) * q1 + q0) * S;
+        ga = w0 * p0 + w1 * p1 + w2 * p2;
+    }
+    {
+        T a0 = dC / sq(eps + b0), a1 = dB / sq(eps + b1), a2 = dA / sq(eps + b2);
+        T s = a0 + a1 + a2;
+        T w0 = a0 / s, w1 = a1 / s, w2 = a2 / s;
+        T p0 = q2 - (T(3) * q2 - T(4) * q3 + q4) * S;
+        T p1 = q2 - (q1 - q3) * S;
+        T p2 = q2 + (T(3) * q2 - T(4) * q1 + q0) * S;
+        gb = w0 * p0 + w1 * p1 + w2 * p2;
+    }
+#endif
+}
+
+// weno.rs:312-352  positivity limiter: density pass then pressure pass,
+// scaling the face state toward the cell average ub.
+template <typename T> WX_HD T pressure_pp(const T q[4], T gam)
+{
+    if (q[0] <= T(0)) return T(0);
+    return (gam - T(1)) * (q[3] - T(0.5) * (q[1] * q[1] + q[2] * q[2]) / q[0]);
+}
+
+template <typename T> WX_HD void pp_limit(T q[4], const T ub[4], T gam)
+{
+    const T eps = K<T>::eps;
+    if (q[0] < eps) {
+        T rb = vmax(ub[0], eps);
+        T th = vmax(vmin((rb - eps) / (rb - q[0]), T(1)), T(0));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) q[k] = th * (q[k] - ub[k]) + ub[k];
+    }
+    T pq = pressure_pp(q, gam);
+    if (pq < eps) {
+        T pb = vmax(pressure_pp(ub, gam), eps);
+        T th = vmax(vmin((pb - eps) / (pb - pq), T(1)), T(0));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) q[k] = th * (q[k] - ub[k]) + ub[k];
+    }
+}
+
+// ---------------------------------------------------------------------
+// Normal-direction reconstruction at one face from its six-cell stencil
+// c[0..5] (the face lies between c[2] and c[3]).
+//   characteristic: weno.rs:385-454 (basis from the Roe state of c2|c3,
+//                   projected WENO5, back-projection, positivity limiter);
+//   component-wise: weno.rs:654-717 (no projection, no limiter).
+// ---------------------------------------------------------------------
+template <typename T> WX_HD void recon_normal(int dir, bool cons, const T c[6][4], T gam,
+                                              T ql[4], T qr[4])
+{
+    if (cons) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            ql[k] = weno5_left(c[0][k], c[1][k], c[2][k], c[3][k], c[4][k]);
+            qr[k] = weno5_right(c[1][k], c[2][k], c[3][k], c[4][k], c[5][k]);
+        }
+        return;
+    }
+    Roe<T> s = roe_avg(c[2], c[3], gam);
+    T L[4][4], R[4][4];
+    eig_left(dir, s, gam, L);
+    eig_right(dir, s, R);
+    T w[6][4];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) mat4_vec4(L, c[t], w[t]);
+    T chl[4], chr[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        chl[k] = weno5_left(w[0][k], w[1][k], w[2][k], w[3][k], w[4][k]);
+        chr[k] = weno5_right(w[1][k], w[2][k], w[3][k], w[4][k], w[5][k]);
+    }
+    mat4_vec4(R, chl, ql);
+    mat4_vec4(R, chr, qr);
+    pp_limit(ql, c[2], gam);
+    pp_limit(qr, c[3], gam);
+}
+
+// ---------------------------------------------------------------------
+// Riemann solvers (riemann.rs:6-349).  Both use Roe-average wave speeds
+// s_l = ũ_n - ã, s_r = ũ_n + ã with ã^2 floored at 0, and the guard
+// |d| < 1e-12 -> copysign(1e-12, d) + 1e-12.
+// ---------------------------------------------------------------------
+template <typename T> struct FaceState {
+    T ql0, ql1, ql2, ql3, qr0, qr1, qr2, qr3;   // floored densities
+    T u_l, v_l, u_r, v_r, p_l, p_r;
+    T s_l, s_r;
+};
+
+template <typename T> WX_HD FaceState<T> face_speeds(int dir, const T ql[4], const T qr[4], T gam)
+{
+    const T eps = K<T>::eps;
+    FaceState<T> s;
+    s.ql0 = vmax(ql[0], eps); s.ql1 = ql[1]; s.ql2 = ql[2]; s.ql3 = ql[3];
+    s.qr0 = vmax(qr[0], eps); s.qr1 = qr[1]; s.qr2 = qr[2]; s.qr3 = qr[3];
+#if WX_FAST
+    T il = rcp(s.ql0), ir = rcp(s.qr0);
+    s.u_l = s.ql1 * il; s.v_l = s.ql2 * il;
+    s.u_r = s.qr1 * ir; s.v_r = s.qr2 * ir;
+#else
+    s.u_l = s.ql1 / s.ql0; s.v_l = s.ql2 / s.ql0;
+    s.u_r = s.qr1 / s.qr0; s.v_r = s.qr2 / s.qr0;
+#endif
+    T p_l = (gam - T(1)) * (s.ql3 - T(0.5) * s.ql0 * (s.u_l * s.u_l + s.v_l * s.v_l));
+    T p_r = (gam - T(1)) * (s.qr3 - T(0.5) * s.qr0 * (s.u_r * s.u_r + s.v_r * s.v_r));
+    s.p_l = vmax(p_l, eps);
+    s.p_r = vmax(p_r, eps);
+    T sl = sqrt(s.ql0), sr = sqrt(s.qr0);
+    T dr = sl + sr;
+#if WX_FAST
+    T h_l = (s.ql3 + s.p_l) * il;
+    T h_r = (s.qr3 + s.p_r) * ir;
+    T idr = rcp(dr);
+    T ut = (sl * s.u_l + sr * s.u_r) * idr;
+    T vt = (sl * s.v_l + sr * s.v_r) * idr;
+    T ht = (sl * h_l + sr * h_r) * idr;
+#else
+    T h_l = (s.ql3 + s.p_l) / s.ql0;
+    T h_r = (s.qr3 + s.p_r) / s.qr0;
+    T ut = (sl * s.u_l + sr * s.u_r) / dr;
+    T vt = (sl * s.v_l + sr * s.v_r) / dr;
+    T ht = (sl * h_l + sr * h_r) / dr;
+#endif
+    T tmp = ht - T(0.5) * (ut * ut + vt * vt);
+    T at = sqrt(vmax((gam - T(1)) * tmp, T(0)));
+    T wt = dir == DIR_X ? ut : vt;
+    s.s_l = wt - at;
+    s.s_r = wt + at;
+    return s;
+}
+
+template <typename T> WX_HD T guard_denom(T d)
+{
+    const T eps = K<T>::eps;
+    if (fabs(d) < eps) d = copysign(eps, d) + eps;
+    return d;
+}
+
+// riemann.rs:6-96 (x) / :99-189 (y): HLLC.  ql/qr are the raw (unfloored)
+// face states; the star-state difference uses them as given.
+template <typename T> WX_HD void hllc_from(int dir, const FaceState<T>& s, const T ql[4], const T fl[4],
+                                           const T qr[4], const T fr[4], T out[4])
+{
+    T wl = dir == DIR_X ? s.u_l : s.v_l;
+    T wr = dir == DIR_X ? s.u_r : s.v_r;
+    T den = guard_denom(s.ql0 * (s.s_l - wl) - s.qr0 * (s.s_r - wr));
+    T s_st = (s.p_r - s.p_l + s.ql0 * wl * (s.s_l - wl) - s.qr0 * wr * (s.s_r - wr)) / den;
+    if (s.s_l >= T(0)) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = fl[k];
+    } else if (s.s_l < T(0) && s_st >= T(0)) {
+        T cf = s.ql0 * (s.s_l - wl) / (s.s_l - s_st);
+        T e = s.ql3 / s.ql0 + (s_st - wl) * (s_st + s.p_l / (s.ql0 * (s.s_l - wl)));
+        T st[4];
+        st[0] = cf;
+        st[1] = dir == DIR_X ? cf * s_st : cf * s.u_l;
+        st[2] = dir == DIR_X ? cf * s.v_l : cf * s_st;
+        st[3] = cf * e;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = fl[k] + s.s_l * (st[k] - ql[k]);
+    } else if (s_st < T(0) && s.s_r > T(0)) {
+        T cf = s.qr0 * (s.s_r - wr) / (s.s_r - s_st);
+        T e = s.qr3 / s.qr0 + (s_st - wr) * (s_st + s.p_r / (s.qr0 * (s.s_r - wr)));
+        T st[4];
+        st[0] = cf;
+        st[1] = dir == DIR_X ? cf * s_st : cf * s.u_r;
+        st[2] = dir == DIR_X ? cf * s.v_r : cf * s_st;
+        st[3] = cf * e;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = fr[k] + s.s_r * (st[k] - qr[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = fr[k];
+    }
+}
+
+// riemann.rs:191-269 (x) / :271-349 (y): HLLE with floored densities.
+template <typename T> WX_HD void hlle_from(const FaceState<T>& s, const T fl[4], const T fr[4], T out[4])
+{
+    T den = guard_denom(s.s_r - s.s_l);
+    if (s.s_l >= T(0)) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = fl[k];
+    } else if (s.s_r < T(0)) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = fr[k];
+    } else {
+        const T a[4] = {s.ql0, s.ql1, s.ql2, s.ql3};
+        const T b[4] = {s.qr0, s.qr1, s.qr2, s.qr3};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            out[k] = (s.s_r * fl[k] - s.s_l * fr[k] + s.s_r * s.s_l * (b[k] - a[k])) / den;
+    }
+}
+
+// riemann.rs:389-407  blend weight from the sensor value
+template <typename T> WX_HD T sensor_phi(T dpdx, T dpdy, T pbar, T lo, T hi)
+{
+    T grad = sqrt(dpdx * dpdx + dpdy * dpdy);
+    T jp = grad / vmax(pbar, K<T>::eps);
+    if (jp <= lo) return T(1);
+    if (jp >= hi) return T(0);
+    return T(1) - (jp - lo) / (hi - lo);
+}
+
+}  // namespace wx

@@ -1,0 +1,96 @@
+// wenox_ops.h -- host-side launch table of the device kernels.
+//
+// wenox_kernels.hip is compiled twice (arithmetic variants "exact" and
+// "fast", see wenox_numerics.h); each build exports one Ops table.  All
+// pointers are device pointers; every launch is asynchronous on `s`.
+//
+// Array conventions:
+//   AoS4  (M, N, 4) C-contiguous, the numpy layout of the reference's arrays;
+//   SoA4  four planes [4][M][N], j (y) fastest -- the stepper's HBM layout.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wenox {
+
+// Device-resident time control of one solve (read by every step kernel).
+struct StepCtl {
+    double T;              // simulated time
+    double dt;             // dt of the current step
+    double t_final;        // < 0: step-count controlled
+    double cfl_dx;         // CFL * dx (host product, as the reference forms it)
+    double cmax0;          // first-step wave speed from the caller's primitives
+    long long step;        // steps taken
+    long long max_steps;   // safety cap / step count
+    int active;            // current step runs (loop condition held)
+    int use_cmax0;         // next CFL uses cmax0 instead of the state
+    unsigned long long cmax_bits;   // running max of c+|V| (bit pattern, >= 0)
+    unsigned int blocks_done;       // last-block detection of the reduction
+    int pad;
+};
+
+enum BcKind : int {
+    BC_MIRROR = 0,      // even mirror, y then x (Rie_prob_2d_rs.py:123-152)
+    BC_PERIODIC = 1,
+    BC_DMR_REF = 2,     // unitest.py:207-227 (final state of its write sequence)
+    BC_DMR_STD = 3,     // standard double Mach reflection with exact moving shock
+};
+
+struct BcParams {
+    int kind;
+    int dmr_bound;      // BC_DMR_REF: `bound` of unitest.init_refac
+    double dx, dy;      // BC_DMR_STD geometry
+    double x0;          // BC_DMR_STD wall start (1/6)
+    double gamma;
+};
+
+// Strides (in elements) of a 3-index array (i, j, k): element at
+// i*is + j*js + k*ks.  AoS4 (M,N,4): {4N, 4, 1}.  SoA4 [4][M][N]: {N, 1, M*N}.
+struct Lay {
+    int64_t is, js, ks;
+};
+inline Lay lay_aos4(int64_t M, int64_t N) { return Lay{4 * N, 4, 1}; }
+inline Lay lay_soa4(int64_t M, int64_t N) { return Lay{N, 1, M * N}; }
+
+struct Ops {
+    // ---- drop-in building blocks (reference shapes, any layout) ---------
+    void (*flux)(const double* q, Lay lq, int64_t M, int64_t N, int dir, double gam, double* out,
+                 Lay lo, hipStream_t s);
+    void (*con2primi)(const double* q, Lay lq, int64_t M, int64_t N, double gam, double* p,
+                      double* a, double* rho, double* u, double* v, double* h, hipStream_t s);
+    // pass 1 (normal) into pl/pr, then pass 2 (transverse Gauss points) and the
+    // averaged fluxes into ql/fl/qr/fr.  u is (m, n); pl/pr are SoA4 of
+    // (m-5, n) for x or (m, n-5) for y; outputs are (F1, F2) in layout lo.
+    void (*weno)(const double* u, Lay lu, int64_t m, int64_t n, int dir, int cons, double gam,
+                 double* pl, double* pr, double* ql, double* fl, double* qr, double* fr, Lay lo,
+                 hipStream_t s);
+    // kind 0 HLLC, 1 HLLE, 2 blend (pbar: scratch of F1*F2 doubles)
+    void (*riemann)(const double* ql, const double* fl, const double* qr, const double* fr,
+                    Lay li, int64_t F1, int64_t F2, int dir, int kind, double gam, double lo,
+                    double hi, double* pbar, double* out, Lay lout, hipStream_t s);
+    // out (Nx, Ny) = -((fx[i+1]-fx[i])/dx + (fy[j+1]-fy[j])/dx)
+    void (*divergence)(const double* fx, Lay lfx, const double* fy, Lay lfy, int64_t Nx,
+                       int64_t Ny, double dx, double* out, Lay lout, hipStream_t s);
+
+    // ---- stepper (SoA4 padded state, (nx+6) x (ny+6) per plane) ---------
+    // ghost fill of U, and of its transposed copy UT when UT != nullptr
+    void (*bc)(double* U, double* UT, int64_t nx, int64_t ny, BcParams bp, const StepCtl* ctl,
+               int stage, hipStream_t s);
+    void (*cfl)(const double* U, int64_t nx, int64_t ny, double gam, StepCtl* ctl, hipStream_t s);
+    void (*advance)(StepCtl* ctl, hipStream_t s);
+    // Unfused RHS: L (interior Nx x Ny, SoA4) from U (padded SoA4), through
+    // face arrays in `work` (rhs_unfused_scratch(nx, ny) doubles).
+    void (*rhs_unfused)(const double* U, int64_t nx, int64_t ny, double dx, double gam, int cons,
+                        int force_hlle, double lo, double hi, double* work, double* L,
+                        const StepCtl* ctl, hipStream_t s);
+    // RK stage combine on the SoA4 interior (stage 1..3), skipped when !ctl->active.
+    void (*rk)(const double* Un, const double* Us, const double* L, double* out, int64_t nx,
+               int64_t ny, int stage, const StepCtl* ctl, hipStream_t s);
+};
+
+size_t rhs_unfused_scratch(int64_t nx, int64_t ny);
+
+extern const Ops ops_exact;
+extern const Ops ops_fast;
+
+}  // namespace wenox

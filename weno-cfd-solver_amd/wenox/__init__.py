@@ -1,0 +1,114 @@
+"""wenox: MI355X-native (gfx950) 2D WENO5 / HLLC Euler hot path.
+
+``Solver`` keeps the padded conserved state in HBM and advances it with the
+reference's SSP-RK3 + ghost fill + CFL time loop, one hipGraph replay per
+step (``Rie_prob_2d_rs.py:169-207,311-345``; ``double_mach_reflection.py:
+97-134,229-260``).  The drop-in module ``weno_ext`` (sibling package) keeps
+the reference's per-call surface.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import EXACT, FAST, PanicException, WenoxError  # noqa: F401
+
+BC_MIRROR, BC_PERIODIC, BC_DMR_REF, BC_DMR_STD = 0, 1, 2, 3
+ENGINE_UNFUSED, ENGINE_FUSED = 0, 1
+PREC_FP64, PREC_FP32SPLIT = 0, 1
+
+_BC_NAMES = {"mirror": BC_MIRROR, "transmissive": BC_MIRROR, "zero_gradient": BC_MIRROR,
+             "periodic": BC_PERIODIC, "dmr_ref": BC_DMR_REF, "dmr": BC_DMR_STD}
+
+
+def device_count() -> int:
+    return _lib.load().wenox_device_count()
+
+
+class Solver:
+    """Device-resident SSP-RK3 solver of one 2D problem.
+
+    Parameters follow the reference drivers' dictionaries: ``nx``, ``ny``,
+    ``dx`` (grid_dict), ``cfl``, ``force_hlle``, ``jp_cri``, ``cons_switch``
+    (control_dict "weno type"), ``gamma`` (phys_dict).
+    """
+
+    def __init__(self, nx, ny, dx, gamma=1.4, cfl=0.4, cons_switch=False, force_hlle=False,
+                 jp_cri=(0.2, 1.0), bc="mirror", dmr_bound=0, dmr_x0=1.0 / 6.0,
+                 engine="unfused", variant="exact", precision="fp64", use_graph=True):
+        d = _lib.SolverDesc()
+        d.nx, d.ny, d.dx, d.gamma, d.cfl = int(nx), int(ny), float(dx), float(gamma), float(cfl)
+        d.cons_switch = int(bool(cons_switch))
+        d.force_hlle = int(bool(force_hlle))
+        d.jp_lo, d.jp_hi = float(jp_cri[0]), float(jp_cri[1])
+        d.bc = _BC_NAMES[bc] if isinstance(bc, str) else int(bc)
+        d.dmr_bound = int(dmr_bound)
+        d.dmr_x0 = float(dmr_x0)
+        d.engine = {"unfused": ENGINE_UNFUSED, "fused": ENGINE_FUSED}[engine] if isinstance(engine, str) else int(engine)
+        d.variant = {"exact": EXACT, "fast": FAST}[variant] if isinstance(variant, str) else int(variant)
+        d.precision = {"fp64": PREC_FP64, "fp32split": PREC_FP32SPLIT}[precision] if isinstance(precision, str) else int(precision)
+        d.use_graph = int(bool(use_graph))
+        self.desc = d
+        self.nx, self.ny = d.nx, d.ny
+        self._h = ctypes.c_void_p()
+        L = _lib.load()
+        _lib.check(L.wenox_solver_create(ctypes.byref(d), ctypes.byref(self._h)))
+        self.t = 0.0
+        self.steps = 0
+
+    def close(self):
+        if self._h:
+            _lib.load().wenox_solver_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, q: np.ndarray, t0: float = 0.0):
+        """q: (nx+6, ny+6, c>=4) float64; with c >= 8 the first CFL step uses
+        channels 5-7 (u, v, p) as the reference does after init."""
+        q = np.asarray(q, dtype=np.float64)
+        v = _lib.view(q)
+        _lib.check(_lib.load().wenox_solver_upload(self._h, ctypes.byref(v), float(t0)))
+        self.t = float(t0)
+
+    def download(self) -> np.ndarray:
+        """(nx+6, ny+6, 8): conserved channels and the primitives (rho_safe, u, v, p)."""
+        out = np.empty((self.nx + 6, self.ny + 6, 8))
+        _lib.check(_lib.load().wenox_solver_download(self._h, out.ctypes.data))
+        return out
+
+    def run(self, nstep=None, t_final=None):
+        """Advance by at most ``nstep`` steps and, if ``t_final`` is given, while
+        T < t_final - 1e-12 (Rie_prob_2d_rs.py:335).  Returns (T, steps taken)."""
+        if nstep is None and t_final is None:
+            raise ValueError("Either 't_final' or 'nstep' must be provided.")
+        ms = int(nstep) if nstep is not None else int(1e8)
+        tf = float(t_final) if t_final is not None else -1.0
+        t = ctypes.c_double()
+        k = ctypes.c_int64()
+        _lib.check(_lib.load().wenox_solver_run(self._h, ms, tf, ctypes.byref(t), ctypes.byref(k)))
+        self.t = t.value
+        self.steps += k.value
+        return t.value, k.value
+
+    def rhs(self) -> np.ndarray:
+        """L(U) of the current (ghost-filled) state, (nx, ny, 4)."""
+        out = np.empty((self.nx, self.ny, 4))
+        _lib.check(_lib.load().wenox_solver_rhs(self._h, out.ctypes.data))
+        return out
+
+    @property
+    def ms_per_step(self) -> float:
+        return _lib.load().wenox_solver_last_ms_per_step(self._h)

@@ -1,0 +1,118 @@
+"""ctypes binding of libwenox.so (C ABI declared in include/wenox.h)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwenox.so")
+
+OK, E_ARG, E_SHAPE, E_DEVICE = 0, -1, -2, 1
+EXACT, FAST = 0, 1
+
+
+class WenoxError(RuntimeError):
+    """A HIP runtime failure or a missing/unsupported device."""
+
+
+class PanicException(BaseException):
+    """Raised where the reference Rust extension panics (too-small arrays,
+    fewer than 4 channels); like pyo3_runtime.PanicException it derives from
+    BaseException."""
+
+
+class View3(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p),
+                ("dim", ctypes.c_int64 * 3),
+                ("stride", ctypes.c_int64 * 3)]
+
+
+class SolverDesc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int64), ("ny", ctypes.c_int64),
+                ("dx", ctypes.c_double), ("gamma", ctypes.c_double), ("cfl", ctypes.c_double),
+                ("cons_switch", ctypes.c_int), ("force_hlle", ctypes.c_int),
+                ("jp_lo", ctypes.c_double), ("jp_hi", ctypes.c_double),
+                ("bc", ctypes.c_int), ("dmr_bound", ctypes.c_int), ("dmr_x0", ctypes.c_double),
+                ("engine", ctypes.c_int), ("variant", ctypes.c_int), ("precision", ctypes.c_int),
+                ("use_graph", ctypes.c_int)]
+
+
+_lib = None
+
+DROPIN_SYMBOLS = [
+    "wenox_flux_x", "wenox_flux_y", "wenox_con2primi", "wenox_weno_x", "wenox_weno_y",
+    "wenox_hllc_x", "wenox_hllc_y", "wenox_hlle_x", "wenox_hlle_y", "wenox_hllc_hlle_x",
+    "wenox_hllc_hlle_y", "wenox_l_local",
+]
+
+
+def load():
+    """Load libwenox.so (raises WenoxError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise WenoxError(f"{LIB_PATH} is missing: build it with "
+                         "`python weno-cfd-solver_amd/build.py` (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    V = ctypes.POINTER(View3)
+    d, i, dp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+    sig = {
+        "wenox_last_error": ([], ctypes.c_char_p),
+        "wenox_version": ([], ctypes.c_char_p),
+        "wenox_set_variant": ([i], i),
+        "wenox_device_count": ([], i),
+        "wenox_flux_x": ([V, d, dp], i),
+        "wenox_flux_y": ([V, d, dp], i),
+        "wenox_con2primi": ([V, d, dp, dp, dp, dp, dp, dp], i),
+        "wenox_weno_x": ([V, d, i, dp, dp, dp, dp], i),
+        "wenox_weno_y": ([V, d, i, dp, dp, dp, dp], i),
+        "wenox_hllc_x": ([V, V, V, V, d, dp], i),
+        "wenox_hllc_y": ([V, V, V, V, d, dp], i),
+        "wenox_hlle_x": ([V, V, V, V, d, dp], i),
+        "wenox_hlle_y": ([V, V, V, V, d, dp], i),
+        "wenox_hllc_hlle_x": ([V, V, V, V, d, d, d, i, dp], i),
+        "wenox_hllc_hlle_y": ([V, V, V, V, d, d, d, i, dp], i),
+        "wenox_l_local": ([V, d, d, i, d, d, i, dp], i),
+        "wenox_solver_create": ([ctypes.POINTER(SolverDesc), ctypes.POINTER(ctypes.c_void_p)], i),
+        "wenox_solver_destroy": ([dp], None),
+        "wenox_solver_upload": ([dp, V, d], i),
+        "wenox_solver_download": ([dp, dp], i),
+        "wenox_solver_run": ([dp, ctypes.c_int64, d, ctypes.POINTER(d),
+                              ctypes.POINTER(ctypes.c_int64)], i),
+        "wenox_solver_rhs": ([dp, dp], i),
+        "wenox_solver_last_ms_per_step": ([dp], d),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return load().wenox_last_error().decode(errors="replace")
+
+
+def check(rc: int) -> None:
+    if rc == OK:
+        return
+    msg = last_error()
+    if rc == E_SHAPE:
+        raise PanicException(msg)
+    if rc == E_ARG:
+        raise ValueError(msg)
+    raise WenoxError(msg)
+
+
+def view(a: np.ndarray) -> View3:
+    """Describe a float64 array; non-multiple-of-8 strides get a contiguous copy
+    (the caller keeps the returned array alive through the View3's _keep)."""
+    if any(s % 8 for s in a.strides):
+        a = np.ascontiguousarray(a)
+    v = View3(a.ctypes.data, (ctypes.c_int64 * 3)(*a.shape), (ctypes.c_int64 * 3)(*[s // 8 for s in a.strides]))
+    v._keep = a
+    return v

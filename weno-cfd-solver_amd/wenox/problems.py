@@ -1,0 +1,255 @@
+"""Initial conditions (host numpy) for the solver's workloads.
+
+Arrays are (nx+6, ny+6, 8): channels 0-3 conserved (rho, rho u, rho v, E),
+4-7 primitives (rho, u, v, p), three ghost cells per side
+(``Rie_prob_2d_rs.py:28-32``).  Ghost cells are filled by the solver's
+boundary kernel on upload.
+
+``riemann_reference`` and ``dmr_reference`` restate the reference drivers'
+own initial data (``Rie_prob_2d_rs.py:213-263``, ``unitest.py:169-205``); the
+others build the workloads named by the benchmark configurations.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+SEED = 20261203
+
+
+def _from_prims(q, gamma):
+    rho, u, v, p = q[:, :, 4], q[:, :, 5], q[:, :, 6], q[:, :, 7]
+    q[:, :, 0] = rho
+    q[:, :, 1] = rho * u
+    q[:, :, 2] = rho * v
+    q[:, :, 3] = p / (gamma - 1.0) + 0.5 * rho * (u ** 2 + v ** 2)
+    return q
+
+
+def _mirror_fill(q):
+    """Even mirror ghost fill, y then x (Rie_prob_2d_rs.py:123-152)."""
+    nx, ny = q.shape[0], q.shape[1]
+    for g in range(3):
+        q[:, g, :] = q[:, 5 - g, :]
+    for g in range(3):
+        q[:, ny - 1 - g, :] = q[:, ny - 6 + g, :]
+    for g in range(3):
+        q[g, :, :] = q[5 - g, :, :]
+    for g in range(3):
+        q[nx - 1 - g, :, :] = q[nx - 6 + g, :, :]
+    return q
+
+
+def four_quadrant(n: int, gamma: float = 1.4, ny: int | None = None):
+    """Four-quadrant Riemann problem on [0,1]^2 (benchmark configuration 1,
+    Lax-Liu configuration 3): (rho,u,v,p) = (1.5,0,0,1.5) for x>0.5,y>0.5;
+    (0.5323,1.206,0,0.3) for x<0.5,y>0.5; (0.138,1.206,1.206,0.029) for
+    x<0.5,y<0.5; (0.5323,0,1.206,0.3) for x>0.5,y<0.5.  Returns (q, dx)."""
+    nx = n
+    ny = n if ny is None else ny
+    q = np.zeros((nx + 6, ny + 6, 8))
+    x = (np.arange(nx + 6) - 3 + 0.5) / nx
+    y = (np.arange(ny + 6) - 3 + 0.5) / ny
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    states = [((X > 0.5) & (Y > 0.5), (1.5, 0.0, 0.0, 1.5)),
+              ((X < 0.5) & (Y > 0.5), (0.5323, 1.206, 0.0, 0.3)),
+              ((X < 0.5) & (Y < 0.5), (0.138, 1.206, 1.206, 0.029)),
+              ((X > 0.5) & (Y < 0.5), (0.5323, 0.0, 1.206, 0.3))]
+    for mask, st in states:
+        for k in range(4):
+            q[:, :, 4 + k][mask] = st[k]
+    _from_prims(q, gamma)
+    return _mirror_fill(q), 1.0 / nx
+
+
+def riemann_reference(nx: int, ny: int, gamma: float = 1.4):
+    """The reference driver's own initial data (Rie_prob_2d_rs.py:213-263):
+    quadrant split at padded index 3 + int(0.75 nx), followed by its
+    reflecting ghost fill (:42-120)."""
+    q = np.zeros((nx + 6, ny + 6, 8))
+    k = 3 + int(nx * 0.75)
+    q[:k, :k, 4:8] = (0.125, 1.0, 1.0, 0.025)
+    q[k:, :k, 4:8] = (0.5, 0.0, 1.0, 0.25)
+    q[:k, k:, 4:8] = (0.5, 1.0, 0.0, 0.25)
+    q[k:, k:, 4:8] = (1.0, 0.0, 0.0, 1.0)
+    _from_prims(q, gamma)
+    return _reflecting_fill(q)
+
+
+_S_Y = np.array([1, 1, -1, 1, 1, 1, -1, 1], dtype=np.float64)
+_S_X = np.array([1, -1, 1, 1, 1, -1, 1, 1], dtype=np.float64)
+_S_C = np.array([1, -1, -1, 1, 1, -1, -1, 1], dtype=np.float64)
+
+
+def _reflecting_fill(u):
+    """Rie_prob_2d_rs.py:42-120 (uses (m, n) = (shape[1], shape[0]); square grids)."""
+    m, n = u.shape[1], u.shape[0]
+    for g in range(3):
+        u[g, 3:n - 3, :] = u[5 - g, 3:n - 3, :] * _S_Y
+        u[m - 1 - g, 3:n - 3, :] = u[m - 6 + g, 3:n - 3, :] * _S_Y
+    for g in range(3):
+        u[3:m - 3, g, :] = u[3:m - 3, 5 - g, :] * _S_X
+        u[3:m - 3, n - 1 - g, :] = u[3:m - 3, n - 6 + g, :] * _S_X
+    for gi in range(3):
+        for gj in range(3):
+            mi, mj = 5 - gi, 5 - gj
+            ij, gj_ = n - 6 + gj, n - 3 + gj
+            ii, gi_ = m - 6 + gi, m - 3 + gi
+            u[gi, gj, :] = u[mi, mj, :] * _S_C
+            u[gi, gj_, :] = u[mi, ij, :] * _S_C
+            u[gi_, gj, :] = u[ii, mj, :] * _S_C
+            u[gi_, gj_, :] = u[ii, ij, :] * _S_C
+    return u
+
+
+DMR_POST_ROW = [8, 33, -57.1576, 563.499, 8, 4.125, -7.1447, 116.5]
+
+
+def dmr_reference(nx: int, ny: int, lx: float = 4.0, ly: float = 1.0, gamma: float = 1.4):
+    """unitest.init_refac (unitest.py:169-205) and its ghost fill; returns (q, bound)."""
+    q = np.zeros((nx + 6, ny + 6, 8))
+    q[:, :, 4], q[:, :, 5], q[:, :, 6], q[:, :, 7] = 1.4, 0.0, 0.0, 1.0
+    bound = 0
+    for i in range(3, nx + 3):
+        x = (i - 3 + 0.5) / nx * lx
+        y_pos = np.sqrt(3) * x - np.sqrt(3) / 6
+        j = int(np.round(y_pos / ly * ny))
+        if j == 0:
+            bound = i
+        j = max(j, 0)
+        q[i, j + 3:, 4:8] = (8, 4.125, -7.1447, 116.5)
+    sl = np.s_[3:nx + 3, 3:ny + 3]
+    q[sl + (0,)] = q[sl + (4,)]
+    q[sl + (1,)] = q[sl + (4,)] * q[sl + (5,)]
+    q[sl + (2,)] = q[sl + (4,)] * q[sl + (6,)]
+    q[sl + (3,)] = q[sl + (7,)] / (gamma - 1) + 0.5 * q[sl + (4,)] * (q[sl + (5,)] ** 2 + q[sl + (6,)] ** 2)
+    return dmr_reference_fill(q, bound, nx, ny), bound
+
+
+def dmr_reference_fill(q, bound, nx, ny):
+    """unitest.apply_bc_refac (unitest.py:207-227)."""
+    for i in range(3):
+        q[i, :, :] = DMR_POST_ROW
+        q[:, ny + 3 + i, :] = q[:, ny + 2, :]
+        q[nx + 3 + i, :, :] = q[nx + 2, :, :]
+        q[:bound + 1, i, :] = DMR_POST_ROW
+        q[bound + 1:bound + 10, i, :] = q[bound + 1:bound + 10, 3, :]
+        q[bound + 10:, i, :] = q[bound + 10:, 5 - i, :]
+        q[bound + 10:, i, 2] *= -1
+        q[bound + 10:, i, 6] *= -1
+    return q
+
+
+def double_mach(nx: int = 1920, ny: int = 480, gamma: float = 1.4):
+    """Standard double Mach reflection on [0,4]x[0,1] (benchmark configuration
+    2): Mach-10 shock at 60 degrees to the wall through (1/6, 0); pre-shock
+    (1.4, 0, 0, 1), post-shock (8, 8.25 cos30, -8.25 sin30, 116.5).
+    Returns (q, dx)."""
+    dx = 4.0 / nx
+    q = np.zeros((nx + 6, ny + 6, 8))
+    x = (np.arange(nx + 6) - 3 + 0.5) * dx
+    y = (np.arange(ny + 6) - 3 + 0.5) * (1.0 / ny)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    post = X < 1.0 / 6.0 + Y / math.sqrt(3.0)
+    q[:, :, 4:8] = (1.4, 0.0, 0.0, 1.0)
+    q[post, 4:8] = (8.0, 8.25 * math.cos(math.pi / 6), -8.25 * math.sin(math.pi / 6), 116.5)
+    return _from_prims(q, gamma), dx
+
+
+def _gauss4():
+    a = math.sqrt(3.0 / 7.0 - 2.0 / 7.0 * math.sqrt(6.0 / 5.0))
+    b = math.sqrt(3.0 / 7.0 + 2.0 / 7.0 * math.sqrt(6.0 / 5.0))
+    wa = (18.0 + math.sqrt(30.0)) / 36.0
+    wb = (18.0 - math.sqrt(30.0)) / 36.0
+    return np.array([-b, -a, a, b]) * 0.5, np.array([wb, wa, wa, wb]) * 0.5
+
+
+def vortex_exact(x, y, t, gamma=1.4, eps=5.0, L=10.0):
+    """Shu's isentropic vortex advected with (1, 1) on the periodic [0, L]^2
+    (benchmark configuration 3): point values (rho, u, v, p)."""
+    xc = (5.0 + t) % L
+    yc = (5.0 + t) % L
+    dxv = (x - xc + 0.5 * L) % L - 0.5 * L
+    dyv = (y - yc + 0.5 * L) % L - 0.5 * L
+    r2 = dxv ** 2 + dyv ** 2
+    f = np.exp(0.5 * (1.0 - r2))
+    u = 1.0 - eps / (2 * math.pi) * f * dyv
+    v = 1.0 + eps / (2 * math.pi) * f * dxv
+    T = 1.0 - (gamma - 1.0) * eps ** 2 / (8.0 * gamma * math.pi ** 2) * f ** 2
+    rho = T ** (1.0 / (gamma - 1.0))
+    p = rho ** gamma
+    return rho, u, v, p
+
+
+def vortex_cell_average(n: int, t: float = 0.0, gamma: float = 1.4, L: float = 10.0):
+    """Cell averages of the conserved variables (4x4 Gauss-Legendre per cell),
+    (n, n, 4) interior only."""
+    h = L / n
+    gp, gw = _gauss4()
+    xc = (np.arange(n) + 0.5) * h
+    acc = np.zeros((n, n, 4))
+    for a, wa in zip(gp, gw):
+        for b, wb in zip(gp, gw):
+            X, Y = np.meshgrid(xc + a * h, xc + b * h, indexing="ij")
+            rho, u, v, p = vortex_exact(X, Y, t, gamma, L=L)
+            w = wa * wb
+            acc[..., 0] += w * rho
+            acc[..., 1] += w * rho * u
+            acc[..., 2] += w * rho * v
+            acc[..., 3] += w * (p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v))
+    return acc
+
+
+def isentropic_vortex(n: int, gamma: float = 1.4, L: float = 10.0):
+    """Returns (q, dx) with cell-averaged conserved variables and primitives
+    formed from them."""
+    q = np.zeros((n + 6, n + 6, 8))
+    q[3:-3, 3:-3, :4] = vortex_cell_average(n, 0.0, gamma, L)
+    _prims_from_cons(q, gamma)
+    return q, L / n
+
+
+def kelvin_helmholtz(n: int, gamma: float = 1.4, seed: int = SEED, amp: float = 0.01):
+    """Shear layer on periodic [0,1]^2 (benchmark configuration 5): rho = 2,
+    u = -0.5 for |y - 0.5| < 0.25, else rho = 1, u = +0.5; p = 2.5; v is
+    seeded white noise of amplitude `amp`.  Returns (q, dx)."""
+    q = np.zeros((n + 6, n + 6, 8))
+    y = (np.arange(n) + 0.5) / n
+    band = np.abs(y - 0.5) < 0.25
+    rng = np.random.default_rng(seed)
+    rho = np.where(band, 2.0, 1.0)[None, :] * np.ones((n, 1))
+    u = np.where(band, -0.5, 0.5)[None, :] * np.ones((n, 1))
+    v = amp * rng.uniform(-1.0, 1.0, size=(n, n))
+    q[3:-3, 3:-3, 4] = rho
+    q[3:-3, 3:-3, 5] = u
+    q[3:-3, 3:-3, 6] = v
+    q[3:-3, 3:-3, 7] = 2.5
+    _from_prims(q, gamma)
+    return q, 1.0 / n
+
+
+def _prims_from_cons(q, gamma):
+    rho = np.maximum(q[:, :, 0], 1e-12)
+    u = q[:, :, 1] / rho
+    v = q[:, :, 2] / rho
+    p = np.maximum((gamma - 1.0) * (q[:, :, 3] - 0.5 * rho * (u * u + v * v)), 1e-12)
+    q[:, :, 4], q[:, :, 5], q[:, :, 6], q[:, :, 7] = rho, u, v, p
+    return q
+
+
+def random_riemann_instance(rng, n_x: int, n_y: int, gamma: float = 1.4):
+    """One instance of the batched workload (benchmark configuration 4):
+    four-quadrant data with per-quadrant rho~U[0.1,2], p~U[0.02,2],
+    u,v~U[-1.5,1.5] on [0,1]^2."""
+    q = np.zeros((n_x + 6, n_y + 6, 8))
+    x = (np.arange(n_x + 6) - 3 + 0.5) / n_x
+    y = (np.arange(n_y + 6) - 3 + 0.5) / n_y
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    for mask in ((X > 0.5) & (Y > 0.5), (X < 0.5) & (Y > 0.5), (X < 0.5) & (Y < 0.5), (X > 0.5) & (Y < 0.5)):
+        q[mask, 4] = rng.uniform(0.1, 2.0)
+        q[mask, 5] = rng.uniform(-1.5, 1.5)
+        q[mask, 6] = rng.uniform(-1.5, 1.5)
+        q[mask, 7] = rng.uniform(0.02, 2.0)
+    _from_prims(q, gamma)
+    return _mirror_fill(q), 1.0 / n_x
