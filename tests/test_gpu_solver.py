@@ -114,3 +114,34 @@ def test_solver_rhs_matches_l_local(oracle_mod):
             s.upload(q0)
             got = s.rhs()
         np.testing.assert_array_equal(got, ref, err_msg=engine)
+
+
+def test_fp32_split_accuracy(oracle_mod):
+    """FP32-split (FP32 bulk arithmetic, hi+lo state, FP64 combine/CFL) after
+    50 steps: relative L1 density error against the FP64 oracle <= 1e-6
+    (the accuracy bound of the FP32 path); the other variables are reported."""
+    from wenox import Solver, problems
+    n = 128
+    q0, dx = problems.four_quadrant(n)
+    print(f"[solver] fp32-split four-quadrant {n}^2 50 steps", flush=True)
+    ref, ts = oracle_mod.run_riemann(q0.copy(), n, n, dx, 1.4, 0.4, False, False, (1e4, 1.1e4), nstep=50)
+    with Solver(n, n, dx, cfl=0.4, jp_cri=(1e4, 1.1e4), engine="fused", variant="fast",
+                precision="fp32split") as s:
+        s.upload(q0)
+        T, k = s.run(nstep=50)
+        got = s.download()
+    err = rel_l1(got, ref, n, n)
+    print("fp32-split rel L1 per variable", err)
+    assert k == 50
+    assert err[0] <= 1e-6, err
+
+
+def test_fp32_split_rhs(oracle_mod):
+    from wenox import Solver, problems
+    q0, dx = problems.four_quadrant(64)
+    ref = oracle_mod.l_local_py(q0, dx, 1.4, False, (0.5, 2.0), False)
+    with Solver(64, 64, dx, jp_cri=(0.5, 2.0), engine="fused", variant="fast", precision="fp32split") as s:
+        s.upload(q0)
+        got = s.rhs()
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() <= 1e-4 * scale

@@ -427,6 +427,8 @@ struct wenox_solver {
     double last_ms = 0.0;
     SweepPlan plan{};
     const SweepOps* sops = nullptr;
+    bool split = false;      // FP32-split state: buffers hold float planes [8][M][N]
+    int* flags = nullptr;    // FP32-split: per-workgroup FP64-recompute flags
 };
 
 namespace {
@@ -439,6 +441,21 @@ void enqueue_step(wenox_solver* s)
     const auto& d = s->d;
     const Ops* o = s->ops;
     hipStream_t st = s->stream;
+    if (s->split) {
+        float* src[3] = {(float*)s->U, (float*)s->A, (float*)s->B};
+        float* dst[3] = {(float*)s->A, (float*)s->B, (float*)s->U};
+        float* srcT[3] = {(float*)s->UT, (float*)s->AT, (float*)s->BT};
+        float* dstT[3] = {(float*)s->AT, (float*)s->BT, (float*)s->UT};
+        s->sops->cfl_split((const float*)s->U, d.nx, d.ny, d.gamma, s->ctl, st);
+        for (int stage = 1; stage <= 3; ++stage) {
+            const int i = stage - 1;
+            s->sops->stage_split(s->plan, src[i], srcT[i], (const float*)s->U, (float*)s->L,
+                                 (float*)s->DyT, dst[i], dstT[i], s->flags, stage, s->ctl, st);
+            s->sops->bc_split(dst[i], dstT[i], d.nx, d.ny, s->bp, s->ctl, stage, st);
+        }
+        o->advance(s->ctl, st);
+        return;
+    }
     o->cfl(s->U, d.nx, d.ny, d.gamma, s->ctl, st);
     double* src[3] = {s->U, s->A, s->B};
     double* dst[3] = {s->A, s->B, s->U};
@@ -521,7 +538,10 @@ extern "C" int wenox_solver_create(const wenox_solver_desc* desc, wenox_solver**
     if (d.variant != WENOX_EXACT && d.variant != WENOX_FAST) return fail(WENOX_E_ARG, "unknown variant");
     if (d.engine != WENOX_ENGINE_UNFUSED && d.engine != WENOX_ENGINE_FUSED)
         return fail(WENOX_E_ARG, "unknown engine");
-    if (d.precision != WENOX_PREC_FP64) return fail(WENOX_E_ARG, "precision not available in this build");
+    if (d.precision != WENOX_PREC_FP64 && d.precision != WENOX_PREC_FP32SPLIT)
+        return fail(WENOX_E_ARG, "unknown precision");
+    if (d.precision == WENOX_PREC_FP32SPLIT && d.engine != WENOX_ENGINE_FUSED)
+        return fail(WENOX_E_ARG, "the FP32-split precision runs on the fused engine only");
     if (!(d.dx > 0.0) || !(d.cfl > 0.0)) return fail(WENOX_E_ARG, "dx and cfl must be positive");
     int rc = require_device();
     if (rc) return rc;
@@ -543,6 +563,12 @@ extern "C" int wenox_solver_create(const wenox_solver_desc* desc, wenox_solver**
         s->plan = make_sweep_plan(d.nx, d.ny, d.dx, d.gamma, d.cons_switch, d.force_hlle, d.jp_lo, d.jp_hi);
         s->sops = d.variant == WENOX_FAST ? &sweep_ops_fast : &sweep_ops_exact;
         s->work_n = (size_t)padded(s) * 6;
+        if (d.precision == WENOX_PREC_FP32SPLIT) {
+            // the FP32 kernels are built in the fast variant only
+            s->split = true;
+            s->sops = &sweep_ops_fast;
+            s->work_n = (size_t)padded(s) * 10;
+        }
     }
     bool ok = hipMalloc(&s->U, P * 8) == hipSuccess && hipMalloc(&s->A, P * 8) == hipSuccess &&
               hipMalloc(&s->B, P * 8) == hipSuccess &&
@@ -552,6 +578,8 @@ extern "C" int wenox_solver_create(const wenox_solver_desc* desc, wenox_solver**
     if (ok && fused)
         ok = hipMalloc(&s->UT, P * 8) == hipSuccess && hipMalloc(&s->AT, P * 8) == hipSuccess &&
              hipMalloc(&s->BT, P * 8) == hipSuccess && hipMalloc(&s->DyT, std::max<size_t>(I, 4) * 8) == hipSuccess;
+    if (ok && s->split)
+        ok = hipMalloc(&s->flags, (size_t)std::max<int64_t>(sweep_flag_count(s->plan), 1) * sizeof(int)) == hipSuccess;
     if (!ok)
         return cleanup(fail(WENOX_E_DEVICE, "device allocation failed (state of " +
                                                 std::to_string(((fused ? 6 : 3) * P + 2 * I + s->work_n) * 8 >> 20) +
@@ -579,6 +607,7 @@ extern "C" void wenox_solver_destroy(wenox_solver* s)
     for (double* p : {s->U, s->A, s->B, s->UT, s->AT, s->BT, s->L, s->DyT, s->work})
         if (p) (void)hipFree(p);
     if (s->ctl) (void)hipFree(s->ctl);
+    if (s->flags) (void)hipFree(s->flags);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -623,13 +652,19 @@ extern "C" int wenox_solver_upload(wenox_solver* s, const wenox_view3* q, double
         h.cmax0 = cm;
         h.use_cmax0 = 1;
     }
-    hipError_t e = hipMemcpy(s->U, pin, P * 4 * sizeof(double), hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpy(s->split ? s->work : s->U, pin, P * 4 * sizeof(double), hipMemcpyHostToDevice);
     (void)hipHostFree(pin);
     if (e != hipSuccess) return fail(WENOX_E_DEVICE, std::string("upload: ") + hipGetErrorString(e));
     int rc = write_ctl(s, &h);
     if (rc) return rc;
-    if (s->UT) s->sops->transpose(s->U, s->UT, m, n, s->stream);
-    s->ops->bc(s->U, s->UT, d.nx, d.ny, s->bp, s->ctl, 0, s->stream);
+    if (s->split) {
+        s->sops->to_split(s->work, (float*)s->U, (int64_t)P, s->stream);
+        s->sops->transpose_split((const float*)s->U, (float*)s->UT, m, n, s->stream);
+        s->sops->bc_split((float*)s->U, (float*)s->UT, d.nx, d.ny, s->bp, s->ctl, 0, s->stream);
+    } else {
+        if (s->UT) s->sops->transpose(s->U, s->UT, m, n, s->stream);
+        s->ops->bc(s->U, s->UT, d.nx, d.ny, s->bp, s->ctl, 0, s->stream);
+    }
     HIPCHK(hipStreamSynchronize(s->stream));
     HIPCHK(hipGetLastError());
     return WENOX_OK;
@@ -643,10 +678,16 @@ extern "C" int wenox_solver_download(wenox_solver* s, double* q8)
     const size_t P = (size_t)(m * n);
     HIPCHK(hipSetDevice(s->device));
     double* w = s->work;
-    s->ops->con2primi(s->U, lay_soa4(m, n), m, n, d.gamma, w, w + P, w + 2 * P, w + 3 * P,
+    const double* state = s->U;
+    if (s->split) {
+        // FP64 value hi + lo of every cell, after the six primitive planes
+        s->sops->from_split((const float*)s->U, w + 6 * P, (int64_t)P, s->stream);
+        state = w + 6 * P;
+    }
+    s->ops->con2primi(state, lay_soa4(m, n), m, n, d.gamma, w, w + P, w + 2 * P, w + 3 * P,
                       w + 4 * P, w + 5 * P, s->stream);
     std::vector<double> cons(P * 4), prim(P * 6);
-    HIPCHK(hipMemcpyAsync(cons.data(), s->U, P * 4 * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(cons.data(), state, P * 4 * 8, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipMemcpyAsync(prim.data(), w, P * 6 * 8, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     for (size_t c = 0; c < P; ++c) {
@@ -673,7 +714,11 @@ extern "C" int wenox_solver_rhs(wenox_solver* s, double* out)
     act.active = 1;
     act.dt = 0.0;
     if ((rc = write_ctl(s, &act))) return rc;
-    if (d.engine == WENOX_ENGINE_FUSED) {
+    if (s->split) {
+        s->sops->rhs_split(s->plan, (const float*)s->U, (const float*)s->UT, (float*)s->L,
+                           (float*)s->DyT, s->flags, s->work, s->ctl, s->stream);
+        HIPCHK(hipMemcpyAsync(s->L, s->work, (size_t)(d.nx * d.ny) * 4 * 8, hipMemcpyDeviceToDevice, s->stream));
+    } else if (d.engine == WENOX_ENGINE_FUSED) {
         s->sops->rhs(s->plan, s->U, s->UT, s->L, s->DyT, s->work, s->ctl, s->stream);
         HIPCHK(hipMemcpyAsync(s->L, s->work, (size_t)(d.nx * d.ny) * 4 * 8, hipMemcpyDeviceToDevice, s->stream));
     } else {

@@ -273,8 +273,15 @@ template <typename T> WX_HD T weno5_core(T b0, T b1, T b2, T d0, T d1, T d2, T p
     const T eps = K<T>::eps;
 #if WX_FAST
     T t0 = sq(eps + b0), t1 = sq(eps + b1), t2 = sq(eps + b2);
-    T a0 = d0 * (t1 * t2), a1 = d1 * (t0 * t2), a2 = d2 * (t0 * t1);
-    return (a0 * p0 + a1 * p1 + a2 * p2) * rcp(T(6) * (a0 + a1 + a2));
+    if constexpr (sizeof(T) == 4) {
+        // FP32: products of two (eps+beta)^2 underflow below 1e-38; use one
+        // reciprocal per weight instead
+        T a0 = d0 * rcp(t0), a1 = d1 * rcp(t1), a2 = d2 * rcp(t2);
+        return (a0 * p0 + a1 * p1 + a2 * p2) * rcp(T(6) * (a0 + a1 + a2));
+    } else {
+        T a0 = d0 * (t1 * t2), a1 = d1 * (t0 * t2), a2 = d2 * (t0 * t1);
+        return (a0 * p0 + a1 * p1 + a2 * p2) * rcp(T(6) * (a0 + a1 + a2));
+    }
 #else
     T a0 = d0 / sq(eps + b0);
     T a1 = d1 / sq(eps + b1);
@@ -325,7 +332,12 @@ template <typename T> WX_HD void weno5_gq(T q0, T q1, T q2, T q3, T q4, T& ga, T
     T s1 = q1 - q3;
     T s2 = T(3) * q2 - T(4) * q1 + q0;
     T t0 = sq(eps + b0), t1 = sq(eps + b1), t2 = sq(eps + b2);
-    T e0 = t1 * t2, e1 = t0 * t2, e2 = t0 * t1;
+    T e0, e1, e2;   // weights up to a common factor (see weno5_core)
+    if constexpr (sizeof(T) == 4) {
+        e0 = rcp(t0); e1 = rcp(t1); e2 = rcp(t2);
+    } else {
+        e0 = t1 * t2; e1 = t0 * t2; e2 = t0 * t1;
+    }
     {
         T a0 = dA * e0, a1 = dB * e1, a2 = dC * e2;
         T p0 = q2 + s0 * S, p1 = q2 + s1 * S, p2 = q2 - s2 * S;

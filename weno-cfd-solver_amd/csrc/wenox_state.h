@@ -1,0 +1,256 @@
+// wenox_state.h -- storage policies of the padded state and the kernels that
+// touch every cell generically: the ghost fill and the single-pass CFL
+// reduction.  Included inside namespace wenox::WX_NS by each kernel
+// translation unit (names stay distinct per arithmetic variant).
+//
+// Storage policies (element (k, idx) of plane k at flat index idx):
+//   StF64   double planes [4][M][N]
+//   StSplit float planes [8][M][N]: value = (double)hi[k] + (double)lo[k+4],
+//           written back as hi = (float)v, lo = (float)(v - hi)   (FP32-split)
+#pragma once
+
+struct StF64 {
+    double* p;
+    int64_t P;
+    __device__ __forceinline__ double ld(int k, int64_t idx) const { return p[k * P + idx]; }
+    __device__ __forceinline__ void st(int k, int64_t idx, double v) const { p[k * P + idx] = v; }
+};
+
+struct StSplit {
+    float* p;
+    int64_t P;
+    __device__ __forceinline__ double ld(int k, int64_t idx) const
+    {
+        return (double)p[k * P + idx] + (double)p[(k + 4) * P + idx];
+    }
+    __device__ __forceinline__ void st(int k, int64_t idx, double v) const
+    {
+        float h = (float)v;
+        p[k * P + idx] = h;
+        p[(k + 4) * P + idx] = (float)(v - (double)h);
+    }
+};
+
+constexpr int ST_TB = 256;
+
+// ------------------------------------------------------------------------
+// Stepper: ghost fill on the padded SoA4 state
+// ------------------------------------------------------------------------
+// Ghost cells of an (nx+6) x (ny+6) array: 3 full rows at each x end plus 3
+// columns at each y end of the interior rows.
+__device__ __forceinline__ void ghost_index(int64_t t, int64_t nx, int64_t ny, int64_t& i, int64_t& j)
+{
+    const int64_t n = ny + 6;
+    const int64_t rows = 6 * n;                 // x-ghost rows, all columns
+    if (t < rows) {
+        int64_t r = t / n;
+        j = t - r * n;
+        i = r < 3 ? r : nx + r;                 // 0,1,2, nx+3, nx+4, nx+5
+    } else {
+        t -= rows;
+        int64_t r = t / 6;                      // interior row
+        int64_t c = t - r * 6;
+        i = 3 + r;
+        j = c < 3 ? c : ny + c;                 // 0,1,2, ny+3, ny+4, ny+5
+    }
+}
+
+// Even mirror (Rie_prob_2d_rs.py:123-152): ghost g <- 5-g at the low end,
+// ghost N+5-g <- N+g at the high end; the y pass runs first over all rows and
+// the x pass over all columns, so a corner takes the doubly mirrored cell.
+__device__ __forceinline__ int64_t mirror_src(int64_t g, int64_t N)
+{
+    if (g < 3) return 5 - g;
+    if (g >= N + 3) return 2 * N + 5 - g;
+    return g;
+}
+
+__device__ __forceinline__ int64_t periodic_src(int64_t g, int64_t N)
+{
+    if (g < 3) return g + N;
+    if (g >= N + 3) return g - N;
+    return g;
+}
+
+// unitest.py:207-227, written as the final value of each ghost cell after the
+// reference's sequence of row/column assignments (valid for nx+2 >= bound+10,
+// ny >= 3): left ghost rows = post-shock row; top ghosts copy column ny+2;
+// right ghost rows copy row nx+2; bottom ghosts of rows <= bound are post-shock,
+// rows bound+1..bound+9 copy column 3, the rest mirror with rho*v negated.
+__constant__ double c_dmr_post[4] = {8.0, 33.0, -57.1576, 563.499};
+
+template <class St>
+__global__ void k_bc(St U, St UT, bool has_t, int64_t nx, int64_t ny, BcParams bp, const StepCtl* ctl,
+                     int stage)
+{
+    if (ctl && !ctl->active) return;
+    const int64_t m = nx + 6, n = ny + 6;
+    const int64_t count = 6 * n + 6 * nx;
+    int64_t t = (int64_t)blockIdx.x * ST_TB + threadIdx.x;
+    if (t >= count) return;
+    int64_t i, j;
+    ghost_index(t, nx, ny, i, j);
+    double v[4];
+    if (bp.kind == BC_MIRROR || bp.kind == BC_PERIODIC) {
+        int64_t si = bp.kind == BC_MIRROR ? mirror_src(i, nx) : periodic_src(i, nx);
+        int64_t sj = bp.kind == BC_MIRROR ? mirror_src(j, ny) : periodic_src(j, ny);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = U.ld(k, si * n + sj);
+    } else if (bp.kind == BC_DMR_REF) {
+        const int64_t B = bp.dmr_bound;
+        int64_t si = i, sj = j;
+        bool post = false, flip = false;
+        if (i < 3) {
+            post = true;
+        } else {
+            if (i >= nx + 3) si = nx + 2;
+            if (j >= ny + 3) {
+                sj = ny + 2;
+            } else if (j < 3) {
+                if (i >= nx + 3) {
+                    sj = 5 - j; flip = true;
+                } else if (i <= B) {
+                    post = true;
+                } else if (i <= B + 9) {
+                    sj = 3;
+                } else {
+                    sj = 5 - j; flip = true;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = post ? c_dmr_post[k] : U.ld(k, si * n + sj);
+        if (flip) v[2] = -v[2];
+    } else {
+        // Standard double Mach reflection (Woodward & Colella 1984): left
+        // inflow at the post-shock state; right outflow copies the last
+        // interior row; bottom: post-shock for x < x0, reflecting wall
+        // (rho*v negated) beyond; top: post-shock where x lies behind the
+        // exact shock trace x0 + (1 + 20 t) / sqrt(3) at the stage time, else
+        // the resting pre-shock gas.  Ghost columns of the right ghost rows
+        // follow the same rules at the last interior row.
+        const double g = bp.gamma;
+        const double rho1 = 8.0, u1 = 8.25 * 0.86602540378443864676, v1 = -8.25 * 0.5, p1 = 116.5;
+        const double post_q[4] = {rho1, rho1 * u1, rho1 * v1,
+                                  p1 / (g - 1.0) + 0.5 * rho1 * (u1 * u1 + v1 * v1)};
+        const double pre_q[4] = {1.4, 0.0, 0.0, 1.0 / (g - 1.0)};
+        double ts = 0.0;
+        if (ctl) ts = ctl->T + (stage == 2 ? 0.5 * ctl->dt : (stage == 0 ? 0.0 : ctl->dt));
+        const int64_t ie = i > nx + 2 ? nx + 2 : i;
+        const double xc = ((double)(ie - 3) + 0.5) * bp.dx;
+        int src = 0;            // 0 copy (ie, sj), 1 post, 2 pre
+        int64_t sj = j;
+        bool flip = false;
+        if (i < 3) {
+            src = 1;
+        } else if (j >= ny + 3) {
+            src = xc < bp.x0 + (1.0 + 20.0 * ts) / 1.7320508075688772935 ? 1 : 2;
+        } else if (j < 3) {
+            if (xc < bp.x0) src = 1;
+            else { sj = 5 - j; flip = true; }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            v[k] = src == 1 ? post_q[k] : (src == 2 ? pre_q[k] : U.ld(k, ie * n + sj));
+        if (flip) v[2] = -v[2];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) U.st(k, i * n + j, v[k]);
+    if (has_t) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) UT.st(k, j * m + i, v[k]);
+    }
+}
+
+// ------------------------------------------------------------------------
+// Single-pass CFL reduction (Rie_prob_2d_rs.py:311-319):
+//   c_max = max over the interior of sqrt(g p / max(rho, 1e-14)) + sqrt(u^2+v^2)
+// with (rho, u, v, p) the synced primitives of con2primi (utils.rs:9-35).
+// Blocks reduce with wave shuffles, publish with one atomicMax on the bit
+// pattern (a non-negative double orders like its bits), and the last block
+// to finish turns the max into dt and re-arms the accumulator.
+// ------------------------------------------------------------------------
+constexpr int CFL_BLOCKS = 1024;
+
+// max that propagates NaN like numpy.max
+__device__ __forceinline__ double nan_max(double a, double b)
+{
+    return (a != a) ? a : ((b > a || b != b) ? b : a);
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = nan_max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <class St>
+__global__ void __launch_bounds__(ST_TB) k_cfl(St U, int64_t nx, int64_t ny, double gam, StepCtl* ctl)
+{
+    __shared__ double part[ST_TB / 64];
+    __shared__ bool last;
+    const int64_t n = ny + 6;
+    double vmaxl = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * ST_TB + threadIdx.x; t < nx * ny;
+         t += (int64_t)gridDim.x * ST_TB) {
+        int64_t i = t / ny + 3, j = t - (t / ny) * ny + 3;
+        int64_t o = i * n + j;
+        double q[4] = {U.ld(0, o), U.ld(1, o), U.ld(2, o), U.ld(3, o)};
+        Prim<double> r = con2primi_pt(q, gam);
+        double spd = sqrt(r.u * r.u + r.v * r.v);
+        double c = sqrt(gam * r.p / fmax(q[0], 1e-14));
+        double w = c + spd;
+        vmaxl = nan_max(vmaxl, w);
+    }
+    vmaxl = wave_max(vmaxl);
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) part[wv] = vmaxl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = part[0];
+        for (int w = 1; w < ST_TB / 64; ++w) b = nan_max(b, part[w]);
+        atomicMax(&ctl->cmax_bits, (unsigned long long)__double_as_longlong(b));
+        __threadfence();
+        unsigned prev = atomicAdd(&ctl->blocks_done, 1u);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last || threadIdx.x != 0) return;
+    __threadfence();
+    unsigned long long bits = atomicAdd(&ctl->cmax_bits, 0ull);
+    double cmax = ctl->use_cmax0 ? ctl->cmax0 : __longlong_as_double((long long)bits);
+    ctl->use_cmax0 = 0;
+    ctl->cmax_bits = 0ull;
+    ctl->blocks_done = 0u;
+    // Rie_prob_2d_rs.py:335-336 loop condition, then :315-318
+    double T = ctl->T;
+    bool run = ctl->step < ctl->max_steps;
+    if (ctl->t_final >= 0.0) run = run && (T < ctl->t_final - 1e-12);
+    ctl->active = run ? 1 : 0;
+    double c_max = cmax + 1e-16;
+    double dt = ctl->cfl_dx / c_max;
+    if (ctl->t_final >= 0.0) {
+        double rem = ctl->t_final - T;
+        dt = rem < dt ? rem : dt;
+    }
+    ctl->dt = dt;
+}
+
+
+template <class St>
+static void launch_bc(St U, St UT, bool has_t, int64_t nx, int64_t ny, BcParams bp, const StepCtl* ctl,
+                      int stage, hipStream_t s)
+{
+    int64_t count = 6 * (ny + 6) + 6 * nx;
+    k_bc<St><<<(unsigned)((count + ST_TB - 1) / ST_TB), ST_TB, 0, s>>>(U, UT, has_t, nx, ny, bp, ctl, stage);
+}
+
+template <class St>
+static void launch_cfl(St U, int64_t nx, int64_t ny, double gam, StepCtl* ctl, hipStream_t s)
+{
+    int64_t nb = (nx * ny + ST_TB - 1) / ST_TB;
+    if (nb > CFL_BLOCKS) nb = CFL_BLOCKS;
+    if (nb < 1) nb = 1;
+    k_cfl<St><<<(unsigned)nb, ST_TB, 0, s>>>(U, nx, ny, gam, ctl);
+}

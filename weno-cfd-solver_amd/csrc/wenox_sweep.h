@@ -33,6 +33,20 @@ struct SweepOps {
                 double* L, const StepCtl* ctl, hipStream_t s);
     // out[k][j][i] = in[k][i][j] for an M x N padded plane set
     void (*transpose)(const double* in, double* out, int64_t M, int64_t N, hipStream_t s);
+
+    // ---- FP32-split state: float planes [8][M][N], value = hi[k] + lo[k+4] ----
+    // (fast build only; nullptr in the exact build)
+    void (*stage_split)(const SweepPlan& p, const float* Us, const float* UsT, const float* Un,
+                        float* Dx, float* DyT, float* out, float* outT, int* flags, int stage,
+                        const StepCtl* ctl, hipStream_t s);
+    void (*rhs_split)(const SweepPlan& p, const float* U, const float* UT, float* Dx, float* DyT,
+                      int* flags, double* L, const StepCtl* ctl, hipStream_t s);
+    void (*transpose_split)(const float* in, float* out, int64_t M, int64_t N, hipStream_t s);
+    void (*to_split)(const double* in, float* out, int64_t P, hipStream_t s);
+    void (*from_split)(const float* in, double* out, int64_t P, hipStream_t s);
+    void (*bc_split)(float* U, float* UT, int64_t nx, int64_t ny, BcParams bp, const StepCtl* ctl,
+                     int stage, hipStream_t s);
+    void (*cfl_split)(const float* U, int64_t nx, int64_t ny, double gam, StepCtl* ctl, hipStream_t s);
 };
 
 extern const SweepOps sweep_ops_exact;
@@ -40,5 +54,8 @@ extern const SweepOps sweep_ops_fast;
 
 SweepPlan make_sweep_plan(int64_t nx, int64_t ny, double dx, double gamma, int cons,
                           int force_hlle, double jp_lo, double jp_hi);
+
+// Workgroups of the larger sweep grid (size of the FP32-split chunk flags).
+int64_t sweep_flag_count(const SweepPlan& p);
 
 }  // namespace wenox

@@ -1,11 +1,11 @@
 // wenox_sweep.hip -- fused direction sweeps of the device-resident solver.
 //
 // Compiled twice like wenox_kernels.hip (-DWX_FAST=0 / 1); each build exports
-// one SweepOps table.
+// one SweepOps table.  The FP32-split kernels exist in the fast build only.
 //
-// k_sweep<DIR> computes, for every interior cell r of a chunk, the normal
-// flux difference D = (F[r+1] - F[r]) / dx of one direction, where the
-// normal direction is the SLOW axis of the padded SoA4 array it reads:
+// k_sweep computes, for every interior cell r of a chunk, the normal flux
+// difference D = (F[r+1] - F[r]) / dx of one direction, where the normal
+// direction is the SLOW axis of the padded plane set it reads:
 //   DIR = x: the state U itself ((nx+6) x (ny+6) planes);
 //   DIR = y: the transposed state U^T ((ny+6) x (nx+6) planes), evaluated with
 //            the reference's y-direction formulas (no channel swap), so both
@@ -24,6 +24,12 @@
 //   blend   (riemann.rs:433-447) and the flux difference of the cell between
 //           the last two finished faces.
 // Face arrays never reach HBM.  58 output columns per wave (lanes 3..60).
+//
+// Template parameters: T compute type; IN the state encoding read (FP64
+// planes, FP32 hi planes, FP32 hi+lo planes read as FP64); DT the type of D;
+// MODE plain, DETECT (FP32: flags the chunk when a Gauss-point state has
+// rho <= 0 or p <= 0 or is not finite) or FIXUP (FP64 recompute of flagged
+// chunks only, overwriting their D).
 //
 // k_combine applies the SSP-RK3 stage combination with L = -(Dx + Dy^T)
 // (riemann.rs:536; Rie_prob_2d_rs.py:183-202) through an LDS tile transpose,
@@ -45,32 +51,60 @@ namespace WX_NS {
 
 using namespace wx;
 
+#include "wenox_state.h"
+
 constexpr int WAVE = 64;
 constexpr int OUT_COLS = WAVE - 6;   // lanes 3..60
 
+enum { IN_F64 = 0, IN_F32 = 1, IN_SPLIT = 2 };
+enum { MODE_PLAIN = 0, MODE_DETECT = 1, MODE_FIXUP = 2 };
+
 struct SweepArgs {
-    const double* U;      // padded SoA4, M x N per plane, normal = rows
+    const void* U;        // padded planes, M x N each, normal = rows
     int64_t M, N;         // padded extents (nn + 6, nt + 6)
     int64_t nn, nt;       // interior cells along normal / transverse
     int chunk;            // interior rows per workgroup
     double gam, dx, two_h, lo, hi;
-    double* D;            // interior SoA4 nn x nt
+    void* D;              // interior planes nn x nt (DT)
+    int* flags;           // one per workgroup (DETECT writes, FIXUP reads)
     const StepCtl* ctl;
 };
 
-template <int DIR, bool CONS, bool BLEND>
+template <typename T, int IN>
+__device__ __forceinline__ T ld_state(const void* U, int64_t P, int k, int64_t idx)
+{
+    if constexpr (IN == IN_F64) {
+        return (T)((const double*)U)[k * P + idx];
+    } else if constexpr (IN == IN_F32) {
+        return (T)((const float*)U)[k * P + idx];
+    } else {
+        const float* f = (const float*)U;
+        return (T)((double)f[k * P + idx] + (double)f[(k + 4) * P + idx]);
+    }
+}
+
+template <typename T> __device__ __forceinline__ bool bad_state(const T q[4], T gam)
+{
+    if (!(q[0] > T(0))) return true;
+    T p = (gam - T(1)) * (q[3] - T(0.5) * (q[1] * q[1] + q[2] * q[2]) / q[0]);
+    return !(p > T(0)) || !(p < T(3.0e38));
+}
+
+template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND>
 __global__ void __launch_bounds__(WAVE) k_sweep(SweepArgs A)
 {
     if (A.ctl && !A.ctl->active) return;
-    __shared__ double s_p1[8][WAVE];     // pass-1 q_l (0..3), q_r (4..7) of the current face
-    __shared__ double s_pb[2][WAVE];     // p̄ ring (face parity)
+    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+    if (MODE == MODE_FIXUP && A.flags[bid] == 0) return;
+    __shared__ T s_p1[8][WAVE];     // pass-1 q_l (0..3), q_r (4..7) of the current face
+    __shared__ T s_pb[2][WAVE];     // p̄ ring (face parity)
 
+    const T gam = (T)A.gam, two_h = (T)A.two_h, jlo = (T)A.lo, jhi = (T)A.hi, dx = (T)A.dx;
     const int l = threadIdx.x;
     const int64_t c0 = (int64_t)blockIdx.x * OUT_COLS;      // padded column of lane 0
     const int64_t pc = c0 + l;                              // this lane's padded column
     const int64_t t = pc - 3;                               // interior transverse index
-    const bool col_ok = pc < A.N;
-    const int64_t pcl = col_ok ? pc : A.N - 1;              // clamped for loads
+    const int64_t pcl = pc < A.N ? pc : A.N - 1;            // clamped for loads
     const bool gauss_lane = l >= 2 && l < WAVE - 2;
     const bool out_lane = l >= 3 && l < WAVE - 3 && t >= 0 && t < A.nt;
     const int lm = l > 0 ? l - 1 : 0, lp = l < WAVE - 1 ? l + 1 : WAVE - 1;   // sensor neighbours
@@ -80,30 +114,41 @@ __global__ void __launch_bounds__(WAVE) k_sweep(SweepArgs A)
     const int64_t a0 = BLEND ? (r0 > 0 ? r0 - 1 : 0) : r0;
     const int64_t a1 = BLEND ? (r1 < A.nn ? r1 + 1 : A.nn) : r1;     // inclusive
     const int64_t P = A.M * A.N;
-    const double* U0 = A.U + pcl;
+    bool bad = false;
 
     // stencil window rows a..a+5 (padded), 4 conserved components
-    double w[6][4];
+    T w[6][4];
 #pragma unroll
     for (int s = 0; s < 5; ++s)
 #pragma unroll
-        for (int k = 0; k < 4; ++k) w[s + 1][k] = U0[k * P + (a0 + s) * A.N];
+        for (int k = 0; k < 4; ++k) w[s + 1][k] = ld_state<T, IN>(A.U, P, k, (a0 + s) * A.N + pcl);
 
-    double fc_prev[4], fe_prev[4], F_prev[4];
-    double pb_m2 = 0.0, pb_m1 = 0.0;    // p̄ of faces a-2, a-1 (this column)
+    T fc_prev[4], fe_prev[4], F_prev[4];
+    T pb_m2 = T(0), pb_m1 = T(0);    // p̄ of faces a-2, a-1 (this column)
 #pragma unroll
-    for (int k = 0; k < 4; ++k) { fc_prev[k] = 0.0; fe_prev[k] = 0.0; F_prev[k] = 0.0; }
+    for (int k = 0; k < 4; ++k) { fc_prev[k] = T(0); fe_prev[k] = T(0); F_prev[k] = T(0); }
     const int64_t Pd = A.nn * A.nt;
+    DT* Dout = (DT*)A.D;
 
-    auto finish_face = [&](int64_t f, const double F[4]) {
+    auto finish_face = [&](int64_t f, const T F[4]) {
         // cell f-1 lies between faces f-1 and f
         if (out_lane && f - 1 >= r0 && f - 1 < r1) {
-            double* d = A.D + (f - 1) * A.nt + t;
+            DT* d = Dout + (f - 1) * A.nt + t;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) d[k * Pd] = (F[k] - F_prev[k]) / A.dx;
+            for (int k = 0; k < 4; ++k) d[k * Pd] = (DT)((F[k] - F_prev[k]) / dx);
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) F_prev[k] = F[k];
+    };
+
+    auto sensor = [&](int64_t f, T dn, T pbf) -> T {
+        const T* ring = s_pb[f & 1];
+        T dt_;
+        if (t <= 0) dt_ = (ring[lp] - ring[l]) / two_h;
+        else if (t >= A.nt - 1) dt_ = (ring[l] - ring[lm]) / two_h;
+        else dt_ = (ring[lp] - ring[lm]) / two_h;
+        // the reference's dpdx is along x: normal for x faces, transverse for y faces
+        return DIR == DIR_X ? sensor_phi(dn, dt_, pbf, jlo, jhi) : sensor_phi(dt_, dn, pbf, jlo, jhi);
     };
 
     for (int64_t a = a0; a <= a1; ++a) {
@@ -113,12 +158,12 @@ __global__ void __launch_bounds__(WAVE) k_sweep(SweepArgs A)
 #pragma unroll
             for (int k = 0; k < 4; ++k) w[s][k] = w[s + 1][k];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) w[5][k] = U0[k * P + (a + 5) * A.N];
+        for (int k = 0; k < 4; ++k) w[5][k] = ld_state<T, IN>(A.U, P, k, (a + 5) * A.N + pcl);
 
         // pass 1 (normal reconstruction) on every lane
         {
-            double ql[4], qr[4];
-            recon_normal(DIR, CONS, w, A.gam, ql, qr);
+            T ql[4], qr[4];
+            recon_normal(DIR, CONS, w, gam, ql, qr);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 s_p1[k][l] = ql[k];
@@ -127,10 +172,10 @@ __global__ void __launch_bounds__(WAVE) k_sweep(SweepArgs A)
         }
         __syncthreads();
 
-        double fc[4], fe[4], pb = 1.0;
+        T fc[4], fe[4], pb = T(1);
         if (gauss_lane) {
             // pass 2: Gauss points along the face from lanes l-2..l+2
-            double l1[4], l2[4], q1[4], q2[4];
+            T l1[4], l2[4], q1[4], q2[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 weno5_gq(s_p1[k][l - 2], s_p1[k][l - 1], s_p1[k][l], s_p1[k][l + 1], s_p1[k][l + 2],
@@ -138,28 +183,31 @@ __global__ void __launch_bounds__(WAVE) k_sweep(SweepArgs A)
                 weno5_gq(s_p1[4 + k][l - 2], s_p1[4 + k][l - 1], s_p1[4 + k][l], s_p1[4 + k][l + 1],
                          s_p1[4 + k][l + 2], q1[k], q2[k]);
             }
-            double fa[4], fb[4], fcc[4], fd[4];
-            flux_pt(DIR, l1, A.gam, fa);
-            flux_pt(DIR, l2, A.gam, fb);
-            flux_pt(DIR, q1, A.gam, fcc);
-            flux_pt(DIR, q2, A.gam, fd);
-            double QL[4], FL[4], QR[4], FR[4];
+            if (MODE == MODE_DETECT)
+                bad = bad || bad_state(l1, gam) || bad_state(l2, gam) || bad_state(q1, gam) ||
+                      bad_state(q2, gam);
+            T fa[4], fb[4], fcc[4], fd[4];
+            flux_pt(DIR, l1, gam, fa);
+            flux_pt(DIR, l2, gam, fb);
+            flux_pt(DIR, q1, gam, fcc);
+            flux_pt(DIR, q2, gam, fd);
+            T QL[4], FL[4], QR[4], FR[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                FL[k] = 0.5 * (fa[k] + fb[k]);
-                FR[k] = 0.5 * (fd[k] + fcc[k]);
-                QL[k] = 0.5 * (l1[k] + l2[k]);
-                QR[k] = 0.5 * (q1[k] + q2[k]);
+                FL[k] = T(0.5) * (fa[k] + fb[k]);
+                FR[k] = T(0.5) * (fd[k] + fcc[k]);
+                QL[k] = T(0.5) * (l1[k] + l2[k]);
+                QR[k] = T(0.5) * (q1[k] + q2[k]);
             }
-            FaceState<double> st = face_speeds(DIR, QL, QR, A.gam);
+            FaceState<T> st = face_speeds(DIR, QL, QR, gam);
             hlle_from(st, FL, FR, fe);
             if (BLEND) {
                 hllc_from(DIR, st, QL, FL, QR, FR, fc);
-                pb = 0.5 * (pressure_floored(QL, A.gam) + pressure_floored(QR, A.gam));
+                pb = T(0.5) * (pressure_floored(QL, gam) + pressure_floored(QR, gam));
             }
         } else {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) { fc[k] = 0.0; fe[k] = 0.0; }
+            for (int k = 0; k < 4; ++k) { fc[k] = T(0); fe[k] = T(0); }
         }
 
         if (!BLEND) {
@@ -170,17 +218,11 @@ __global__ void __launch_bounds__(WAVE) k_sweep(SweepArgs A)
             if (a > a0) {
                 // sensor and blend of face f = a-1 (riemann.rs:360-407)
                 const int64_t f = a - 1;
-                double dn = f == 0 ? (pb - pb_m1) / A.two_h : (pb - pb_m2) / A.two_h;
-                const double* ring = s_pb[f & 1];
-                double dt_;
-                if (t <= 0) dt_ = (ring[lp] - ring[l]) / A.two_h;
-                else if (t >= A.nt - 1) dt_ = (ring[l] - ring[lm]) / A.two_h;
-                else dt_ = (ring[lp] - ring[lm]) / A.two_h;
-                double phi = DIR == DIR_X ? sensor_phi(dn, dt_, pb_m1, A.lo, A.hi)
-                                          : sensor_phi(dt_, dn, pb_m1, A.lo, A.hi);
-                double F[4];
+                T dn = f == 0 ? (pb - pb_m1) / two_h : (pb - pb_m2) / two_h;
+                T phi = sensor(f, dn, pb_m1);
+                T F[4];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) F[k] = phi * fc_prev[k] + (1.0 - phi) * fe_prev[k];
+                for (int k = 0; k < 4; ++k) F[k] = phi * fc_prev[k] + (T(1) - phi) * fe_prev[k];
                 if (f >= r0) finish_face(f, F);
             }
             pb_m2 = pb_m1;
@@ -194,18 +236,16 @@ __global__ void __launch_bounds__(WAVE) k_sweep(SweepArgs A)
     if (BLEND && a1 == A.nn) {
         // last face of the grid: one-sided normal difference, no face a1+1
         const int64_t f = a1;
-        double dn = (pb_m1 - pb_m2) / A.two_h;
-        const double* ring = s_pb[f & 1];
-        double dt_;
-        if (t <= 0) dt_ = (ring[lp] - ring[l]) / A.two_h;
-        else if (t >= A.nt - 1) dt_ = (ring[l] - ring[lm]) / A.two_h;
-        else dt_ = (ring[lp] - ring[lm]) / A.two_h;
-        double phi = DIR == DIR_X ? sensor_phi(dn, dt_, pb_m1, A.lo, A.hi)
-                                  : sensor_phi(dt_, dn, pb_m1, A.lo, A.hi);
-        double F[4];
+        T dn = (pb_m1 - pb_m2) / two_h;
+        T phi = sensor(f, dn, pb_m1);
+        T F[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) F[k] = phi * fc_prev[k] + (1.0 - phi) * fe_prev[k];
+        for (int k = 0; k < 4; ++k) F[k] = phi * fc_prev[k] + (T(1) - phi) * fe_prev[k];
         finish_face(f, F);
+    }
+    if (MODE == MODE_DETECT) {
+        unsigned long long m = __ballot(bad ? 1 : 0);
+        if (l == 0) A.flags[bid] = m != 0ull ? 1 : 0;
     }
 }
 
@@ -216,23 +256,23 @@ __global__ void __launch_bounds__(WAVE) k_sweep(SweepArgs A)
 // ------------------------------------------------------------------------
 constexpr int TT = 32;
 
-__global__ void __launch_bounds__(256) k_combine(const double* __restrict__ Dx, const double* __restrict__ DyT,
-                                                 const double* Un, const double* Us, double* out,
-                                                 double* outT, int64_t nx, int64_t ny, int stage,
-                                                 const StepCtl* ctl)
+template <class St, typename DT>
+__global__ void __launch_bounds__(256) k_combine(const DT* __restrict__ Dx, const DT* __restrict__ DyT,
+                                                 St Un, St Us, St out, St outT, int64_t nx, int64_t ny,
+                                                 int stage, const StepCtl* ctl)
 {
     if (ctl && !ctl->active) return;
     __shared__ double tile[TT][TT + 1];
     const int tx = threadIdx.x & (TT - 1);   // fast index
     const int ty = threadIdx.x / TT;         // 0..7
     const int64_t i0 = (int64_t)blockIdx.y * TT, j0 = (int64_t)blockIdx.x * TT;
-    const int64_t m = nx + 6, n = ny + 6, P = m * n, PI = nx * ny;
+    const int64_t m = nx + 6, n = ny + 6, PI = nx * ny;
     const double dt = ctl ? ctl->dt : 0.0;
     for (int k = 0; k < 4; ++k) {
         // Dy^T[k][j][i]: rows j, contiguous i
         for (int r = ty; r < TT; r += 8) {
             int64_t j = j0 + r, i = i0 + tx;
-            if (j < ny && i < nx) tile[r][tx] = DyT[k * PI + j * nx + i];
+            if (j < ny && i < nx) tile[r][tx] = (double)DyT[k * PI + j * nx + i];
         }
         __syncthreads();
         double res[TT / 8];
@@ -241,14 +281,14 @@ __global__ void __launch_bounds__(256) k_combine(const double* __restrict__ Dx, 
             res[q] = 0.0;
             if (i < nx && j < ny) {
                 double dgdy = tile[tx][r];
-                double l = -(Dx[k * PI + i * ny + j] + dgdy);
+                double l = -((double)Dx[k * PI + i * ny + j] + dgdy);
                 int64_t o = (i + 3) * n + (j + 3);
-                double qn = Un[k * P + o];
+                double qn = Un.ld(k, o);
                 double v;
                 if (stage == 1) v = qn + dt * l;
-                else if (stage == 2) v = 0.75 * qn + 0.25 * Us[k * P + o] + (0.25 * dt) * l;
-                else v = (1.0 / 3.0) * qn + (2.0 / 3.0) * Us[k * P + o] + ((2.0 / 3.0) * dt) * l;
-                out[k * P + o] = v;
+                else if (stage == 2) v = 0.75 * qn + 0.25 * Us.ld(k, o) + (0.25 * dt) * l;
+                else v = (1.0 / 3.0) * qn + (2.0 / 3.0) * Us.ld(k, o) + ((2.0 / 3.0) * dt) * l;
+                out.st(k, o, v);
                 res[q] = v;
             }
         }
@@ -258,29 +298,32 @@ __global__ void __launch_bounds__(256) k_combine(const double* __restrict__ Dx, 
         // outT[k][j+3][i+3], contiguous i
         for (int r = ty; r < TT; r += 8) {
             int64_t j = j0 + r, i = i0 + tx;
-            if (j < ny && i < nx) outT[k * P + (j + 3) * m + (i + 3)] = tile[tx][r];
+            if (j < ny && i < nx) outT.st(k, (j + 3) * m + (i + 3), tile[tx][r]);
         }
         __syncthreads();
     }
 }
 
 // Plain RHS from the two sweeps (for tests): L = -(Dx + Dy^T), SoA4 interior.
-__global__ void k_rhs_out(const double* Dx, const double* DyT, double* L, int64_t nx, int64_t ny)
+template <typename DT>
+__global__ void k_rhs_out(const DT* Dx, const DT* DyT, double* L, int64_t nx, int64_t ny)
 {
     int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= nx * ny) return;
     int64_t i = t / ny, j = t - i * ny;
-    for (int k = 0; k < 4; ++k) L[k * nx * ny + t] = -(Dx[k * nx * ny + t] + DyT[k * nx * ny + j * nx + i]);
+    for (int k = 0; k < 4; ++k)
+        L[k * nx * ny + t] = -((double)Dx[k * nx * ny + t] + (double)DyT[k * nx * ny + j * nx + i]);
 }
 
-// Padded SoA4 transpose: out[k][j][i] = in[k][i][j] for an M x N plane set.
-__global__ void __launch_bounds__(256) k_transpose(const double* in, double* out, int64_t M, int64_t N)
+// Padded plane-set transpose: out[k][j][i] = in[k][i][j] for NP planes of M x N.
+template <typename E, int NP>
+__global__ void __launch_bounds__(256) k_transpose(const E* in, E* out, int64_t M, int64_t N)
 {
-    __shared__ double tile[TT][TT + 1];
+    __shared__ E tile[TT][TT + 1];
     const int tx = threadIdx.x & (TT - 1), ty = threadIdx.x / TT;
     const int64_t i0 = (int64_t)blockIdx.y * TT, j0 = (int64_t)blockIdx.x * TT;
     const int64_t P = M * N;
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < NP; ++k) {
         for (int r = ty; r < TT; r += 8) {
             int64_t i = i0 + r, j = j0 + tx;
             if (i < M && j < N) tile[r][tx] = in[k * P + i * N + j];
@@ -294,29 +337,43 @@ __global__ void __launch_bounds__(256) k_transpose(const double* in, double* out
     }
 }
 
-template <int DIR>
+template <typename T, int IN, typename DT, int MODE, int DIR>
 static void launch_sweep(const SweepArgs& a, int cons, int blend, hipStream_t s)
 {
     dim3 grid((unsigned)((a.nt + OUT_COLS - 1) / OUT_COLS), (unsigned)((a.nn + a.chunk - 1) / a.chunk));
     if (cons) {
-        if (blend) k_sweep<DIR, true, true><<<grid, WAVE, 0, s>>>(a);
-        else k_sweep<DIR, true, false><<<grid, WAVE, 0, s>>>(a);
+        if (blend) k_sweep<T, IN, DT, MODE, DIR, true, true><<<grid, WAVE, 0, s>>>(a);
+        else k_sweep<T, IN, DT, MODE, DIR, true, false><<<grid, WAVE, 0, s>>>(a);
     } else {
-        if (blend) k_sweep<DIR, false, true><<<grid, WAVE, 0, s>>>(a);
-        else k_sweep<DIR, false, false><<<grid, WAVE, 0, s>>>(a);
+        if (blend) k_sweep<T, IN, DT, MODE, DIR, false, true><<<grid, WAVE, 0, s>>>(a);
+        else k_sweep<T, IN, DT, MODE, DIR, false, false><<<grid, WAVE, 0, s>>>(a);
     }
 }
 
+static SweepArgs args_x(const SweepPlan& p, const void* U, void* D, int* flags, const StepCtl* ctl)
+{
+    return SweepArgs{U, p.nx + 6, p.ny + 6, p.nx, p.ny, p.chunk_x, p.gamma, p.dx, p.two_h_x,
+                     p.jp_lo, p.jp_hi, D, flags, ctl};
+}
+
+static SweepArgs args_y(const SweepPlan& p, const void* UT, void* D, int* flags, const StepCtl* ctl)
+{
+    return SweepArgs{UT, p.ny + 6, p.nx + 6, p.ny, p.nx, p.chunk_y, p.gamma, p.dx, p.two_h_y,
+                     p.jp_lo, p.jp_hi, D, flags, ctl};
+}
+
+// ---- FP64 ---------------------------------------------------------------
 static void h_sweeps(const SweepPlan& p, const double* Us, const double* UsT, double* Dx, double* DyT,
                      const StepCtl* ctl, hipStream_t s)
 {
     const int blend = p.force_hlle ? 0 : 1;
-    SweepArgs ax{Us, p.nx + 6, p.ny + 6, p.nx, p.ny, p.chunk_x, p.gamma, p.dx, p.two_h_x,
-                 p.jp_lo, p.jp_hi, Dx, ctl};
-    launch_sweep<DIR_X>(ax, p.cons, blend, s);
-    SweepArgs ay{UsT, p.ny + 6, p.nx + 6, p.ny, p.nx, p.chunk_y, p.gamma, p.dx, p.two_h_y,
-                 p.jp_lo, p.jp_hi, DyT, ctl};
-    launch_sweep<DIR_Y>(ay, p.cons, blend, s);
+    launch_sweep<double, IN_F64, double, MODE_PLAIN, DIR_X>(args_x(p, Us, Dx, nullptr, ctl), p.cons, blend, s);
+    launch_sweep<double, IN_F64, double, MODE_PLAIN, DIR_Y>(args_y(p, UsT, DyT, nullptr, ctl), p.cons, blend, s);
+}
+
+static dim3 combine_grid(const SweepPlan& p)
+{
+    return dim3((unsigned)((p.ny + TT - 1) / TT), (unsigned)((p.nx + TT - 1) / TT));
 }
 
 static void h_stage(const SweepPlan& p, const double* Us, const double* UsT, const double* Un,
@@ -324,8 +381,10 @@ static void h_stage(const SweepPlan& p, const double* Us, const double* UsT, con
                     hipStream_t s)
 {
     h_sweeps(p, Us, UsT, Dx, DyT, ctl, s);
-    dim3 grid((unsigned)((p.ny + TT - 1) / TT), (unsigned)((p.nx + TT - 1) / TT));
-    k_combine<<<grid, 256, 0, s>>>(Dx, DyT, Un, Us, out, outT, p.nx, p.ny, stage, ctl);
+    const int64_t P = (p.nx + 6) * (p.ny + 6);
+    k_combine<StF64, double><<<combine_grid(p), 256, 0, s>>>(
+        Dx, DyT, StF64{const_cast<double*>(Un), P}, StF64{const_cast<double*>(Us), P}, StF64{out, P},
+        StF64{outT, P}, p.nx, p.ny, stage, ctl);
 }
 
 static void h_rhs(const SweepPlan& p, const double* U, const double* UT, double* Dx, double* DyT,
@@ -333,21 +392,105 @@ static void h_rhs(const SweepPlan& p, const double* U, const double* UT, double*
 {
     h_sweeps(p, U, UT, Dx, DyT, ctl, s);
     int64_t nc = p.nx * p.ny;
-    k_rhs_out<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(Dx, DyT, L, p.nx, p.ny);
+    k_rhs_out<double><<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(Dx, DyT, L, p.nx, p.ny);
 }
 
 static void h_transpose(const double* in, double* out, int64_t M, int64_t N, hipStream_t s)
 {
     dim3 grid((unsigned)((N + TT - 1) / TT), (unsigned)((M + TT - 1) / TT));
-    k_transpose<<<grid, 256, 0, s>>>(in, out, M, N);
+    k_transpose<double, 4><<<grid, 256, 0, s>>>(in, out, M, N);
 }
+
+#if WX_FAST
+// ---- FP32-split -----------------------------------------------------------
+// Bulk sweep in FP32 from the hi planes; chunks with a non-positive or
+// non-finite Gauss-point state are recomputed in FP64 from hi+lo.
+static void h_sweeps_split(const SweepPlan& p, const float* Us, const float* UsT, float* Dx,
+                           float* DyT, int* flags, const StepCtl* ctl, hipStream_t s)
+{
+    const int blend = p.force_hlle ? 0 : 1;
+    launch_sweep<float, IN_F32, float, MODE_DETECT, DIR_X>(args_x(p, Us, Dx, flags, ctl), p.cons, blend, s);
+    launch_sweep<double, IN_SPLIT, float, MODE_FIXUP, DIR_X>(args_x(p, Us, Dx, flags, ctl), p.cons, blend, s);
+    launch_sweep<float, IN_F32, float, MODE_DETECT, DIR_Y>(args_y(p, UsT, DyT, flags, ctl), p.cons, blend, s);
+    launch_sweep<double, IN_SPLIT, float, MODE_FIXUP, DIR_Y>(args_y(p, UsT, DyT, flags, ctl), p.cons, blend, s);
+}
+
+static void h_stage_split(const SweepPlan& p, const float* Us, const float* UsT, const float* Un,
+                          float* Dx, float* DyT, float* out, float* outT, int* flags, int stage,
+                          const StepCtl* ctl, hipStream_t s)
+{
+    h_sweeps_split(p, Us, UsT, Dx, DyT, flags, ctl, s);
+    const int64_t P = (p.nx + 6) * (p.ny + 6);
+    k_combine<StSplit, float><<<combine_grid(p), 256, 0, s>>>(
+        Dx, DyT, StSplit{const_cast<float*>(Un), P}, StSplit{const_cast<float*>(Us), P},
+        StSplit{out, P}, StSplit{outT, P}, p.nx, p.ny, stage, ctl);
+}
+
+static void h_rhs_split(const SweepPlan& p, const float* U, const float* UT, float* Dx, float* DyT,
+                        int* flags, double* L, const StepCtl* ctl, hipStream_t s)
+{
+    h_sweeps_split(p, U, UT, Dx, DyT, flags, ctl, s);
+    int64_t nc = p.nx * p.ny;
+    k_rhs_out<float><<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(Dx, DyT, L, p.nx, p.ny);
+}
+
+static void h_transpose_split(const float* in, float* out, int64_t M, int64_t N, hipStream_t s)
+{
+    dim3 grid((unsigned)((N + TT - 1) / TT), (unsigned)((M + TT - 1) / TT));
+    k_transpose<float, 8><<<grid, 256, 0, s>>>(in, out, M, N);
+}
+
+__global__ void k_to_split(const double* in, float* out, int64_t P)
+{
+    int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= P) return;
+    StSplit o{out, P};
+    for (int k = 0; k < 4; ++k) o.st(k, t, in[k * P + t]);
+}
+
+__global__ void k_from_split(const float* in, double* out, int64_t P)
+{
+    int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= P) return;
+    StSplit q{const_cast<float*>(in), P};
+    for (int k = 0; k < 4; ++k) out[k * P + t] = q.ld(k, t);
+}
+
+static void h_to_split(const double* in, float* out, int64_t P, hipStream_t s)
+{
+    k_to_split<<<(unsigned)((P + 255) / 256), 256, 0, s>>>(in, out, P);
+}
+
+static void h_from_split(const float* in, double* out, int64_t P, hipStream_t s)
+{
+    k_from_split<<<(unsigned)((P + 255) / 256), 256, 0, s>>>(in, out, P);
+}
+
+static void h_bc_split(float* U, float* UT, int64_t nx, int64_t ny, BcParams bp, const StepCtl* ctl,
+                       int stage, hipStream_t s)
+{
+    const int64_t P = (nx + 6) * (ny + 6);
+    launch_bc(StSplit{U, P}, StSplit{UT, P}, UT != nullptr, nx, ny, bp, ctl, stage, s);
+}
+
+static void h_cfl_split(const float* U, int64_t nx, int64_t ny, double gam, StepCtl* ctl, hipStream_t s)
+{
+    launch_cfl(StSplit{const_cast<float*>(U), (nx + 6) * (ny + 6)}, nx, ny, gam, ctl, s);
+}
+#endif
 
 }  // namespace WX_NS
 
 #ifndef __HIP_DEVICE_COMPILE__
-const SweepOps WX_SWEEP_OPS = {WX_NS::h_stage, WX_NS::h_rhs, WX_NS::h_transpose};
+#if WX_FAST
+const SweepOps WX_SWEEP_OPS = {WX_NS::h_stage,       WX_NS::h_rhs,        WX_NS::h_transpose,
+                               WX_NS::h_stage_split, WX_NS::h_rhs_split,  WX_NS::h_transpose_split,
+                               WX_NS::h_to_split,    WX_NS::h_from_split, WX_NS::h_bc_split,
+                               WX_NS::h_cfl_split};
+#else
+const SweepOps WX_SWEEP_OPS = {WX_NS::h_stage, WX_NS::h_rhs, WX_NS::h_transpose, nullptr, nullptr,
+                               nullptr,        nullptr,      nullptr,            nullptr, nullptr};
 
-#if !WX_FAST
 // Chunk length: enough waves to cover the 1024 SIMDs twice where the grid
 // allows it, within [8, 64] rows (a chunk recomputes 2 halo faces for the
 // sensor, so longer is cheaper per cell).
@@ -373,6 +516,14 @@ SweepPlan make_sweep_plan(int64_t nx, int64_t ny, double dx, double gamma, int c
     p.chunk_x = pick_chunk(nx, ny);
     p.chunk_y = pick_chunk(ny, nx);
     return p;
+}
+
+int64_t sweep_flag_count(const SweepPlan& p)
+{
+    const int64_t c = WX_NS::OUT_COLS;
+    int64_t fx = ((p.ny + c - 1) / c) * ((p.nx + p.chunk_x - 1) / p.chunk_x);
+    int64_t fy = ((p.nx + c - 1) / c) * ((p.ny + p.chunk_y - 1) / p.chunk_y);
+    return fx > fy ? fx : fy;
 }
 #endif
 #endif

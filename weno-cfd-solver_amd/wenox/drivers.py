@@ -1,0 +1,94 @@
+"""Device-resident counterparts of the reference drivers.
+
+``main_rs`` follows ``Rie_prob_2d_rs.main_rs`` (:269-376) and
+``main_rs_dmr`` follows ``double_mach_reflection.main_rs`` (:177-303): the
+same dictionaries, loop control (t_final primary, nstep as a cap; CFL dt
+clipped to the remaining time) and snapshot format
+(``data/<YYYY-mm-dd_HH-MM-SS>/{step}.npy`` of the (nx+6, ny+6, 8) array and
+``time.npy``), so ``plot_utils.interactive_plot_keyboard`` reads the output.
+Every step runs on the GPU; snapshots are the only host copies.
+"""
+from __future__ import annotations
+
+import datetime
+import os
+
+import numpy as np
+
+from . import Solver, problems
+
+
+def _run(solver, q0, control_dict, nx, ny, snapshot_dir):
+    t_final = control_dict.get("t_final", None)
+    nstep_user = control_dict.get("nstep", None)
+    if t_final is None and nstep_user is None:
+        raise ValueError("Either 't_final' or 'nstep' must be provided.")
+    max_steps = int(nstep_user) if nstep_user is not None else int(1e8)
+    solver.upload(q0)
+    t_list = [0.0]
+    if snapshot_dir:
+        np.save(os.path.join(snapshot_dir, "0.npy"), q0)
+    if snapshot_dir:
+        # one step per call so every state can be written (Rie_prob_2d_rs.py:342-343)
+        for step in range(1, max_steps + 1):
+            T, k = solver.run(nstep=1, t_final=t_final)
+            if k == 0:
+                break
+            np.save(os.path.join(snapshot_dir, f"{step}.npy"), solver.download())
+            t_list.append(T)
+        np.save(os.path.join(snapshot_dir, "time.npy"), np.array(t_list))
+    else:
+        solver.run(nstep=max_steps, t_final=t_final)
+    return solver.download()
+
+
+def _snapshot_dir(control_dict):
+    if not bool(control_dict.get("file_storage", False)):
+        return ""
+    folder = datetime.datetime.now().strftime("%Y-%m-%d_%H-%M-%S")
+    path = os.path.join("data", folder)
+    os.makedirs(path, exist_ok=True)
+    return path
+
+
+def main_rs(grid_dict: dict, control_dict: dict, phys_dict: dict, q0=None, engine="fused",
+            variant="exact"):
+    """Rie_prob_2d_rs.main_rs on the device.  q0 defaults to the reference's
+    own initial data (Rie_prob_2d_rs.init, :213-263); the ghost fill is the
+    even mirror of apply_bc_zero_gradient (:123-152).  Returns the final
+    (nx+6, ny+6, 8) state when control_dict["mode"] == "opt" (as the
+    reference) and always when q0 is given."""
+    gamma = phys_dict.get("gamma", 1.4)
+    nx, ny = int(grid_dict["nx"]), int(grid_dict["ny"])
+    dx = float(grid_dict.get("dx", 1.0))
+    given = q0 is not None
+    if q0 is None:
+        q0 = problems.riemann_reference(nx, ny, gamma)
+    with Solver(nx, ny, dx, gamma=gamma, cfl=float(control_dict.get("CFL", 0.5)),
+                cons_switch=bool(control_dict.get("weno type")),
+                force_hlle=bool(control_dict.get("force_hlle", False)),
+                jp_cri=control_dict.get("jp_cri", (0.2, 1.0)), bc="mirror",
+                engine=engine, variant=variant) as s:
+        q = _run(s, q0, control_dict, nx, ny, _snapshot_dir(control_dict))
+    if given or control_dict.get("mode", "") == "opt":
+        return q
+    return None
+
+
+def main_rs_dmr(grid_dict: dict, control_dict: dict, phys_dict: dict, engine="fused",
+                variant="exact"):
+    """double_mach_reflection.main_rs on the device: unitest.init_refac data
+    and apply_bc_refac ghost fill, characteristic reconstruction (its
+    5-argument l_local calls), CFL clipped to t_final."""
+    gamma = phys_dict.get("gamma", 1.4)
+    nx, ny = int(grid_dict["nx"]), int(grid_dict["ny"])
+    dx = float(grid_dict.get("dx", 1.0))
+    q0, bound = problems.dmr_reference(nx, ny, grid_dict.get("x", 4), grid_dict.get("y", 1), gamma)
+    jp = control_dict.get("jp_cri", None) or (0.2, 1.0)
+    with Solver(nx, ny, dx, gamma=gamma, cfl=float(control_dict.get("CFL", 0.5)),
+                cons_switch=False, force_hlle=bool(control_dict.get("force_hlle", False)),
+                jp_cri=jp, bc="dmr_ref", dmr_bound=bound, engine=engine, variant=variant) as s:
+        q = _run(s, q0, control_dict, nx, ny, _snapshot_dir(control_dict))
+    if control_dict.get("mode", "") == "opt":
+        return q
+    return None
