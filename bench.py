@@ -1,13 +1,22 @@
 """Throughput of the device-resident WENO5/HLLC SSP-RK3 step on one MI355X.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dmr|quad4096|kh8192]
-                [--engine fused|unfused] [--variant fast|exact] [--no-cpu]
+                [--engine fused|unfused] [--variant fast|exact] [--no-cpu] [--no-extra]
 
-A step is one full SSP-RK3 time step (CFL reduction, 3 x (x sweep, y sweep +
-RK combine, ghost fill)) over the whole grid with the state resident in HBM.
-Default workload: double Mach reflection, 1920 x 480 cells, characteristic
-WENO5 + HLLC (sensor window (1e4, 1.1e4), i.e. phi = 1), CFL 0.4, FP64.
-Prints one JSON line.
+A step is one full SSP-RK3 time step over the whole grid -- CFL reduction,
+3 x (x sweep, y sweep, RK combine, ghost fill), time advance -- replayed as one
+hipGraph with the state resident in HBM.
+
+Default workload (the configuration the metric is quoted on): double Mach
+reflection, 1920 x 480 cells on [0,4]x[0,1], characteristic WENO5 + HLLC
+(sensor window (1e4, 1.1e4): phi = 1), SSP-RK3, CFL 0.4, FP64; synthetic
+initial data built on the host.  Prints one JSON line.
+
+`value` is the FP64 throughput.  The same line carries the FP32-split
+throughput on the same workload with its relative L1 density error against
+the FP64 run after the same steps, the 4096^2 four-quadrant grid, per-kernel
+device times from HIP events (the roofline block), and the CPU oracle timed
+on the host (cpu_baseline).
 """
 import argparse
 import json
@@ -21,7 +30,7 @@ for p in (ROOT, os.path.join(ROOT, "weno-cfd-solver_amd")):
         sys.path.insert(0, p)
 
 HBM_PEAK = 8.0e12
-BYTES_PER_CELL_UPDATE = 256.0   # 4 conserved FP64 values: 64 B stage 1 + 2 x 96 B stages 2, 3
+BYTES_PER_CELL_UPDATE = 256.0   # 4 conserved values x 8 B: 64 B stage 1 + 2 x 96 B stages 2, 3
 
 WORKLOADS = {
     "dmr": dict(name="double_mach_reflection", nx=1920, ny=480, cons=False, force_hlle=False,
@@ -33,15 +42,40 @@ WORKLOADS = {
 }
 
 
+def describe(w):
+    return (f"{w['name']} {w['nx']}x{w['ny']}, characteristic WENO5 + HLLC (jp {w['jp'][0]:g},{w['jp'][1]:g}), "
+            f"SSP-RK3, CFL {w['cfl']}, bc {w['bc']}")
+
+
 def make_state(w):
     from wenox import problems
     if w["name"] == "double_mach_reflection":
-        q, dx = problems.double_mach(w["nx"], w["ny"])
-    elif w["name"] == "four_quadrant_riemann":
-        q, dx = problems.four_quadrant(w["nx"])
-    else:
-        q, dx = problems.kelvin_helmholtz(w["nx"])
-    return q, dx
+        return problems.double_mach(w["nx"], w["ny"])
+    if w["name"] == "four_quadrant_riemann":
+        return problems.four_quadrant(w["nx"])
+    return problems.kelvin_helmholtz(w["nx"])
+
+
+def run_gpu(w, q, dx, steps, warmup, engine, variant, precision, profile=True):
+    from wenox import Solver
+    s = Solver(w["nx"], w["ny"], dx, gamma=1.4, cfl=w["cfl"], cons_switch=w["cons"],
+               force_hlle=w["force_hlle"], jp_cri=w["jp"], bc=w["bc"], engine=engine,
+               variant=variant, precision=precision)
+    try:
+        s.upload(q)
+        s.run(nstep=warmup)
+        t0 = time.perf_counter()
+        T, k = s.run(nstep=steps)
+        el = time.perf_counter() - t0
+        assert k == steps, (k, steps)
+        out = dict(seconds=el, steps=k, value=w["nx"] * w["ny"] * k / el, dev_ms_per_step=s.ms_per_step,
+                   state=s.download()[3:-3, 3:-3, :4])
+        if profile and engine == "fused":
+            out["kernels_ms"] = s.profile(reps=10)
+            out["chunks"] = s.chunks
+        return out
+    finally:
+        s.close()
 
 
 def cpu_baseline(q, dx, w, budget_s=12.0):
@@ -74,7 +108,30 @@ def cpu_baseline(q, dx, w, budget_s=12.0):
                        f"(3 oracle l_local calls + stage combines each), {el:.1f} s")
 
 
+def roofline(res, w):
+    """Whole-step roofline at 256 algorithmic B per cell-update, from the
+    device time of one step (sum of the HIP-event kernel times), plus the
+    dominant kernel."""
+    km = res.get("kernels_ms")
+    cells = w["nx"] * w["ny"]
+    step_ms = km["step_sum"] if km else res["dev_ms_per_step"]
+    achieved = cells * BYTES_PER_CELL_UPDATE / (step_ms * 1e-3)
+    out = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK, "traffic": None,
+           "kernel": "one SSP-RK3 step (CFL + 3 x (x sweep, y sweep, RK combine, ghost fill))",
+           "algorithmic_bytes_per_launch": cells * BYTES_PER_CELL_UPDATE,
+           "avg_launch_ms": step_ms}
+    if km:
+        dom = max(("sweep_x", "sweep_y", "combine"), key=lambda k: km[k])
+        out["dominant_kernel"] = {"name": f"k_sweep ({dom})" if dom.startswith("sweep") else "k_combine",
+                                  "avg_ms": km[dom], "launches_per_step": 3,
+                                  "share_of_step": 3 * km[dom] / step_ms}
+        out["kernels_ms"] = {k: round(v, 5) for k, v in km.items()}
+    return out
+
+
 def main():
+    import numpy as np
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
@@ -83,51 +140,54 @@ def main():
     ap.add_argument("--engine", default="fused", choices=["fused", "unfused"])
     ap.add_argument("--variant", default="fast", choices=["fast", "exact"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
               "the multi-GPU decomposition is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
 
-    from wenox import Solver
     w = WORKLOADS[args.workload]
     q, dx = make_state(w)
-    nx, ny = w["nx"], w["ny"]
-    s = Solver(nx, ny, dx, gamma=1.4, cfl=w["cfl"], cons_switch=w["cons"], force_hlle=w["force_hlle"],
-               jp_cri=w["jp"], bc=w["bc"], engine=args.engine, variant=args.variant)
-    s.upload(q)
-    s.run(nstep=args.warmup)
-    t0 = time.perf_counter()
-    T, k = s.run(nstep=args.steps)
-    el = time.perf_counter() - t0
-    assert k == args.steps
-    ms_step = el / k * 1e3
-    value = nx * ny * k / el
-    dev_ms = s.ms_per_step
-    achieved = value * BYTES_PER_CELL_UPDATE
+    r64 = run_gpu(w, q, dx, args.steps, args.warmup, args.engine, args.variant, "fp64")
     res = {
         "metric": "cell-updates/s FP64 vs FP32-split @<=1e-6 L1 err; % of HBM roofline",
-        "value": value,
+        "value": r64["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
-        "steps": k,
+        "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": r64["seconds"] / r64["steps"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{w['name']} {nx}x{ny}, characteristic WENO5 + HLLC (jp 1e4,1.1e4), "
-                               f"SSP-RK3, CFL {w['cfl']}, bc {w['bc']}",
-                   "engine": args.engine, "variant": args.variant},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
-                     "note": "step-level: 256 B/cell-update x cell-updates/s; device ms/step %.4f" % dev_ms},
+        "config": {"workload": describe(w), "engine": args.engine, "variant": args.variant,
+                   "chunks": r64.get("chunks")},
+        "roofline": roofline(r64, w),
     }
+    if not args.no_extra and args.engine == "fused":
+        r32 = run_gpu(w, q, dx, args.steps, args.warmup, "fused", "fast", "fp32split")
+        rho64, rho32 = r64["state"][..., 0], r32["state"][..., 0]
+        res["fp32_split"] = {
+            "value": r32["value"], "unit": "cell-updates/s", "dtype": "f32 bulk, f32 hi+lo state",
+            "speedup_vs_fp64": r32["value"] / r64["value"],
+            "rel_l1_density_vs_fp64": float(np.abs(rho32 - rho64).sum() / np.abs(rho64).sum()),
+            "after_steps": args.warmup + args.steps,
+            "roofline": roofline(r32, w),
+        }
+        if args.workload != "quad4096":
+            wq = WORKLOADS["quad4096"]
+            qq, dq = make_state(wq)
+            rq = run_gpu(wq, qq, dq, 20, 3, "fused", args.variant, "fp64")
+            rq32 = run_gpu(wq, qq, dq, 20, 3, "fused", "fast", "fp32split")
+            res["quad4096"] = {"workload": describe(wq), "value_fp64": rq["value"],
+                               "value_fp32split": rq32["value"],
+                               "roofline_frac_fp64": roofline(rq, wq)["frac"],
+                               "kernels_ms_fp64": rq.get("kernels_ms")}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(q, dx, w)
-    s.close()
     print(json.dumps(res), flush=True)
 
 

@@ -149,6 +149,33 @@ int wenox_solver_run(wenox_solver *s, int64_t max_steps, double t_final, double 
 int wenox_solver_rhs(wenox_solver *s, double *out);
 /* Average device time per step of the last run (ms), from HIP events. */
 double wenox_solver_last_ms_per_step(wenox_solver *s);
+/* Per-kernel device times (ms, HIP events on the solver's stream, averaged
+ * over reps launches of each kernel alone), fused engine only:
+ * out_ms[0] CFL reduction, [1] x sweep, [2] y sweep, [3] RK combine,
+ * [4] ghost fill, [5] their sum over one step (3 stages).  n_out >= 6. */
+int wenox_solver_profile(wenox_solver *s, int reps, double *out_ms, int n_out);
+/* Rows marched per workgroup by the x and y sweeps (the fused engine's tile length). */
+int wenox_solver_plan(wenox_solver *s, int *chunk_x, int *chunk_y);
+int wenox_solver_set_chunks(wenox_solver *s, int chunk_x, int chunk_y);
+
+/* ---- 3. batched solver: many independent problems of uneven sizes ---------- */
+/* The n instances share the numerics of desc (nx, ny, dx of desc are ignored;
+ * bc must be MIRROR or PERIODIC) and each has its own grid, dx, time and CFL
+ * dt.  Every phase of a step (CFL, x and y sweeps, RK combine, ghost fill) is
+ * one launch over all instances; a step is one hipGraph replay.  chunk: rows
+ * marched per sweep workgroup (<= 0: 16).  No counterpart in the reference,
+ * which runs one problem per process; this serves its batched workload. */
+typedef struct wenox_batch wenox_batch;
+int wenox_batch_create(const wenox_solver_desc *desc, int n, const int64_t *nx, const int64_t *ny,
+                       const double *dx, int chunk, wenox_batch **out);
+void wenox_batch_destroy(wenox_batch *b);
+int wenox_batch_upload(wenox_batch *b, int i, const wenox_view3 *q, double t0);
+int wenox_batch_download(wenox_batch *b, int i, double *q8);
+/* Every instance advances at most max_steps steps (and while T < t_final - 1e-12
+ * when t_final >= 0); *steps_out = steps summed over instances. */
+int wenox_batch_run(wenox_batch *b, int64_t max_steps, double t_final, int64_t *steps_out);
+int wenox_batch_status(wenox_batch *b, double *T, int64_t *steps);
+double wenox_batch_last_ms_per_step(wenox_batch *b);
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,7 @@ UNITS = [
     ("wenox_sweep.hip", "sweep_exact.o", ["-DWX_FAST=0", "-ffp-contract=off"]),
     ("wenox_sweep.hip", "sweep_fast.o", ["-DWX_FAST=1", "-ffp-contract=fast"]),
     ("wenox_abi.cpp", "abi.o", ["-x", "hip", "-ffp-contract=off"]),
+    ("wenox_batch.cpp", "batch.o", ["-x", "hip", "-ffp-contract=off"]),
 ]
 
 

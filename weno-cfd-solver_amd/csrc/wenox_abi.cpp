@@ -243,6 +243,11 @@ int riemann_common(const char* fn, const wenox_view3* ql, const wenox_view3* fl,
 
 }  // namespace
 
+namespace wenox {
+// error reporting shared with the batched engine's ABI (wenox_batch.cpp)
+int abi_fail(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace wenox
+
 // ==========================================================================
 // extern "C": misc
 // ==========================================================================
@@ -801,3 +806,82 @@ extern "C" int wenox_solver_run(wenox_solver* s, int64_t max_steps, double t_fin
 }
 
 extern "C" double wenox_solver_last_ms_per_step(wenox_solver* s) { return s ? s->last_ms : 0.0; }
+
+extern "C" int wenox_solver_profile(wenox_solver* s, int reps, double* out_ms, int n_out)
+{
+    if (!s || !out_ms || n_out < 6 || reps < 1) return fail(WENOX_E_ARG, "wenox_solver_profile: bad argument");
+    const auto& d = s->d;
+    if (d.engine != WENOX_ENGINE_FUSED) return fail(WENOX_E_ARG, "per-kernel profile needs the fused engine");
+    HIPCHK(hipSetDevice(s->device));
+    StepCtl h;
+    int rc = read_ctl(s, &h);
+    if (rc) return rc;
+    StepCtl act = h;
+    act.active = 1;
+    act.dt = 0.0;
+    if ((rc = write_ctl(s, &act))) return rc;
+    const int split = s->split ? 1 : 0;
+    hipStream_t st = s->stream;
+    auto timed = [&](int slot, auto&& launch) -> int {
+        launch();   // warm
+        HIPCHK(hipEventRecord(s->ev0, st));
+        for (int r = 0; r < reps; ++r) launch();
+        HIPCHK(hipEventRecord(s->ev1, st));
+        HIPCHK(hipEventSynchronize(s->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        out_ms[slot] = ms / reps;
+        return WENOX_OK;
+    };
+    // scratch targets: the sweeps read the stage-1 input (U, U^T) and write Dx / Dy^T;
+    // the combine writes into B / B^T, which the next step overwrites anyway.
+    if ((rc = timed(0, [&] {
+             if (split) s->sops->cfl_split((const float*)s->U, d.nx, d.ny, d.gamma, s->ctl, st);
+             else s->ops->cfl(s->U, d.nx, d.ny, d.gamma, s->ctl, st);
+         })))
+        return rc;
+    if ((rc = write_ctl(s, &act))) return rc;
+    if ((rc = timed(1, [&] { s->sops->sweep1(s->plan, 0, split, s->U, s->L, s->flags, s->ctl, st); })))
+        return rc;
+    if ((rc = timed(2, [&] { s->sops->sweep1(s->plan, 1, split, s->UT, s->DyT, s->flags, s->ctl, st); })))
+        return rc;
+    if ((rc = timed(3, [&] {
+             s->sops->combine1(s->plan, split, s->L, s->DyT, s->U, s->U, s->B, s->BT, s->ctl, st);
+         })))
+        return rc;
+    if ((rc = timed(4, [&] {
+             if (split) s->sops->bc_split((float*)s->B, (float*)s->BT, d.nx, d.ny, s->bp, s->ctl, 2, st);
+             else s->ops->bc(s->B, s->BT, d.nx, d.ny, s->bp, s->ctl, 2, st);
+         })))
+        return rc;
+    out_ms[5] = 3.0 * (out_ms[1] + out_ms[2] + out_ms[3] + out_ms[4]) + out_ms[0];
+    return write_ctl(s, &h);
+}
+
+extern "C" int wenox_solver_plan(wenox_solver* s, int* chunk_x, int* chunk_y)
+{
+    if (!s) return fail(WENOX_E_ARG, "wenox_solver_plan: null solver");
+    if (chunk_x) *chunk_x = s->plan.chunk_x;
+    if (chunk_y) *chunk_y = s->plan.chunk_y;
+    return WENOX_OK;
+}
+
+extern "C" int wenox_solver_set_chunks(wenox_solver* s, int chunk_x, int chunk_y)
+{
+    if (!s || chunk_x < 1 || chunk_y < 1) return fail(WENOX_E_ARG, "wenox_solver_set_chunks: bad argument");
+    if (s->d.engine != WENOX_ENGINE_FUSED) return fail(WENOX_E_ARG, "chunks apply to the fused engine");
+    SweepPlan p = s->plan;
+    p.chunk_x = chunk_x;
+    p.chunk_y = chunk_y;
+    if (s->split && sweep_flag_count(p) > sweep_flag_count(s->plan)) {
+        int* f = nullptr;
+        HIPCHK(hipMalloc(&f, (size_t)sweep_flag_count(p) * sizeof(int)));
+        if (s->flags) (void)hipFree(s->flags);
+        s->flags = f;
+    }
+    s->plan = p;
+    // the captured step graph holds the old launch shapes
+    if (s->exec) { (void)hipGraphExecDestroy(s->exec); s->exec = nullptr; }
+    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    return WENOX_OK;
+}

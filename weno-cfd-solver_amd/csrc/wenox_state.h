@@ -79,14 +79,14 @@ __device__ __forceinline__ int64_t periodic_src(int64_t g, int64_t N)
 // rows bound+1..bound+9 copy column 3, the rest mirror with rho*v negated.
 __constant__ double c_dmr_post[4] = {8.0, 33.0, -57.1576, 563.499};
 
+// Ghost cell number t (0 <= t < 6 (ny+6) + 6 nx) of one state.
 template <class St>
-__global__ void k_bc(St U, St UT, bool has_t, int64_t nx, int64_t ny, BcParams bp, const StepCtl* ctl,
-                     int stage)
+__device__ __forceinline__ void bc_cell(St U, St UT, bool has_t, int64_t nx, int64_t ny, BcParams bp,
+                                        const StepCtl* ctl, int stage, int64_t t)
 {
     if (ctl && !ctl->active) return;
     const int64_t m = nx + 6, n = ny + 6;
     const int64_t count = 6 * n + 6 * nx;
-    int64_t t = (int64_t)blockIdx.x * ST_TB + threadIdx.x;
     if (t >= count) return;
     int64_t i, j;
     ghost_index(t, nx, ny, i, j);
@@ -186,14 +186,23 @@ __device__ __forceinline__ double wave_max(double v)
 }
 
 template <class St>
-__global__ void __launch_bounds__(ST_TB) k_cfl(St U, int64_t nx, int64_t ny, double gam, StepCtl* ctl)
+__global__ void k_bc(St U, St UT, bool has_t, int64_t nx, int64_t ny, BcParams bp, const StepCtl* ctl,
+                     int stage)
+{
+    bc_cell<St>(U, UT, has_t, nx, ny, bp, ctl, stage, (int64_t)blockIdx.x * ST_TB + threadIdx.x);
+}
+
+// Block `blk` of `nblk` blocks reducing one state's interior; the last block
+// to finish writes dt into ctl.
+template <class St>
+__device__ __forceinline__ void cfl_block(St U, int64_t nx, int64_t ny, double gam, StepCtl* ctl,
+                                          int64_t blk, int64_t nblk)
 {
     __shared__ double part[ST_TB / 64];
     __shared__ bool last;
     const int64_t n = ny + 6;
     double vmaxl = 0.0;
-    for (int64_t t = (int64_t)blockIdx.x * ST_TB + threadIdx.x; t < nx * ny;
-         t += (int64_t)gridDim.x * ST_TB) {
+    for (int64_t t = blk * ST_TB + threadIdx.x; t < nx * ny; t += nblk * ST_TB) {
         int64_t i = t / ny + 3, j = t - (t / ny) * ny + 3;
         int64_t o = i * n + j;
         double q[4] = {U.ld(0, o), U.ld(1, o), U.ld(2, o), U.ld(3, o)};
@@ -213,7 +222,7 @@ __global__ void __launch_bounds__(ST_TB) k_cfl(St U, int64_t nx, int64_t ny, dou
         atomicMax(&ctl->cmax_bits, (unsigned long long)__double_as_longlong(b));
         __threadfence();
         unsigned prev = atomicAdd(&ctl->blocks_done, 1u);
-        last = (prev == gridDim.x - 1);
+        last = (prev == (unsigned)(nblk - 1));
     }
     __syncthreads();
     if (!last || threadIdx.x != 0) return;
@@ -235,6 +244,12 @@ __global__ void __launch_bounds__(ST_TB) k_cfl(St U, int64_t nx, int64_t ny, dou
         dt = rem < dt ? rem : dt;
     }
     ctl->dt = dt;
+}
+
+template <class St>
+__global__ void __launch_bounds__(ST_TB) k_cfl(St U, int64_t nx, int64_t ny, double gam, StepCtl* ctl)
+{
+    cfl_block<St>(U, nx, ny, gam, ctl, blockIdx.x, gridDim.x);
 }
 
 

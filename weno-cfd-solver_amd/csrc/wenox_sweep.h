@@ -47,7 +47,66 @@ struct SweepOps {
     void (*bc_split)(float* U, float* UT, int64_t nx, int64_t ny, BcParams bp, const StepCtl* ctl,
                      int stage, hipStream_t s);
     void (*cfl_split)(const float* U, int64_t nx, int64_t ny, double gam, StepCtl* ctl, hipStream_t s);
+
+    // Single kernels, for per-kernel timing: one direction's sweep of the
+    // stage input U (U^T for dir 1) into D, and the stage-2 combine.
+    void (*sweep1)(const SweepPlan& p, int dir, int split, const void* U, void* D, int* flags,
+                   const StepCtl* ctl, hipStream_t s);
+    void (*combine1)(const SweepPlan& p, int split, const void* Dx, const void* DyT, const void* Un,
+                     const void* Us, void* out, void* outT, const StepCtl* ctl, hipStream_t s);
 };
+
+// ---- batched engine: many independent problems in each launch ----------------
+// Every state buffer holds the instances back to back: instance i's padded
+// planes start `soff` 8-byte words in (FP64 planes [4][M][N] or FP32-split
+// planes [8][M][N], the same bytes); Dx / Dy^T hold its interior planes at
+// element offset `ioff`.  Workgroups of each launch map to instances through
+// the blk_* tables.
+struct BatchInst {
+    int64_t nx, ny;
+    int64_t soff, ioff;
+    double dx, two_h_x, two_h_y;
+    int chunk_x, chunk_y;
+    int gx_x, gx_y;        // column blocks of the x / y sweeps
+    int blk0_x, blk0_y;    // first workgroup of this instance in the x / y sweep grids
+    int tiles_j, tile0;    // combine: tile columns, first tile
+    int ghost_blk0;        // first ghost-fill block
+    int cfl_blk0, cfl_nblk;
+};
+
+struct BatchTables {       // device pointers and grid sizes
+    const BatchInst* inst;
+    const int* blk_x;
+    const int* blk_y;
+    const int* blk_c;
+    const int* blk_g;
+    const int* blk_f;
+    int n_x, n_y, n_c, n_g, n_f, n_inst;
+};
+
+struct BatchCommon {
+    double gam, jp_lo, jp_hi;
+    int cons, force_hlle;
+};
+
+struct BatchOps {
+    // one RK stage of every instance (sweeps, FP64 fix-up when split, combine)
+    void (*stage)(const BatchTables& t, const BatchCommon& c, int split, const void* Us,
+                  const void* UsT, const void* Un, void* Dx, void* DyT, void* out, void* outT,
+                  int* flags, int stage, StepCtl* ctls, hipStream_t s);
+    void (*bc)(const BatchTables& t, BcParams bp, int split, void* U, void* UT, StepCtl* ctls,
+               int stage, hipStream_t s);
+    void (*cfl)(const BatchTables& t, int split, const void* U, double gam, StepCtl* ctls,
+                hipStream_t s);
+    void (*advance)(StepCtl* ctls, int n, hipStream_t s);
+};
+
+extern const BatchOps batch_ops_exact;
+extern const BatchOps batch_ops_fast;
+// workgroups per sweep column block (the wave's output columns)
+constexpr int SWEEP_OUT_COLS = 58;
+constexpr int COMBINE_TILE = 32;
+constexpr int GHOST_TB = 256;
 
 extern const SweepOps sweep_ops_exact;
 extern const SweepOps sweep_ops_fast;

@@ -90,18 +90,18 @@ template <typename T> __device__ __forceinline__ bool bad_state(const T q[4], T 
     return !(p > T(0)) || !(p < T(3.0e38));
 }
 
+// One workgroup's chunk: column block bx, row chunk by; `bid` indexes the flags.
 template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND>
-__global__ void __launch_bounds__(WAVE) k_sweep(SweepArgs A)
+__device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64_t by, int64_t bid)
 {
     if (A.ctl && !A.ctl->active) return;
-    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
     if (MODE == MODE_FIXUP && A.flags[bid] == 0) return;
     __shared__ T s_p1[8][WAVE];     // pass-1 q_l (0..3), q_r (4..7) of the current face
     __shared__ T s_pb[2][WAVE];     // p̄ ring (face parity)
 
     const T gam = (T)A.gam, two_h = (T)A.two_h, jlo = (T)A.lo, jhi = (T)A.hi, dx = (T)A.dx;
     const int l = threadIdx.x;
-    const int64_t c0 = (int64_t)blockIdx.x * OUT_COLS;      // padded column of lane 0
+    const int64_t c0 = bx * OUT_COLS;                       // padded column of lane 0
     const int64_t pc = c0 + l;                              // this lane's padded column
     const int64_t t = pc - 3;                               // interior transverse index
     const int64_t pcl = pc < A.N ? pc : A.N - 1;            // clamped for loads
@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep(SweepArgs A)
     const bool out_lane = l >= 3 && l < WAVE - 3 && t >= 0 && t < A.nt;
     const int lm = l > 0 ? l - 1 : 0, lp = l < WAVE - 1 ? l + 1 : WAVE - 1;   // sensor neighbours
 
-    const int64_t r0 = (int64_t)blockIdx.y * A.chunk;
+    const int64_t r0 = by * A.chunk;
     const int64_t r1 = r0 + A.chunk < A.nn ? r0 + A.chunk : A.nn;   // cells [r0, r1), faces [r0, r1]
     const int64_t a0 = BLEND ? (r0 > 0 ? r0 - 1 : 0) : r0;
     const int64_t a1 = BLEND ? (r1 < A.nn ? r1 + 1 : A.nn) : r1;     // inclusive
@@ -249,6 +249,13 @@ __global__ void __launch_bounds__(WAVE) k_sweep(SweepArgs A)
     }
 }
 
+template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND>
+__global__ void __launch_bounds__(WAVE) k_sweep(SweepArgs A)
+{
+    sweep_body<T, IN, DT, MODE, DIR, CONS, BLEND>(A, blockIdx.x, blockIdx.y,
+                                                  (int64_t)blockIdx.y * gridDim.x + blockIdx.x);
+}
+
 // ------------------------------------------------------------------------
 // RK stage combine with L = -(Dx + Dy^T), both orientations written.
 // Tile of 32 x 32 interior cells, 256 threads; Dy^T and the transposed
@@ -256,16 +263,17 @@ __global__ void __launch_bounds__(WAVE) k_sweep(SweepArgs A)
 // ------------------------------------------------------------------------
 constexpr int TT = 32;
 
+// One 32 x 32 tile: tile row ti (x), tile column tj (y).
 template <class St, typename DT>
-__global__ void __launch_bounds__(256) k_combine(const DT* __restrict__ Dx, const DT* __restrict__ DyT,
-                                                 St Un, St Us, St out, St outT, int64_t nx, int64_t ny,
-                                                 int stage, const StepCtl* ctl)
+__device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT* __restrict__ DyT, St Un,
+                                             St Us, St out, St outT, int64_t nx, int64_t ny, int stage,
+                                             const StepCtl* ctl, int64_t ti, int64_t tj)
 {
     if (ctl && !ctl->active) return;
     __shared__ double tile[TT][TT + 1];
     const int tx = threadIdx.x & (TT - 1);   // fast index
     const int ty = threadIdx.x / TT;         // 0..7
-    const int64_t i0 = (int64_t)blockIdx.y * TT, j0 = (int64_t)blockIdx.x * TT;
+    const int64_t i0 = ti * TT, j0 = tj * TT;
     const int64_t m = nx + 6, n = ny + 6, PI = nx * ny;
     const double dt = ctl ? ctl->dt : 0.0;
     for (int k = 0; k < 4; ++k) {
@@ -302,6 +310,14 @@ __global__ void __launch_bounds__(256) k_combine(const DT* __restrict__ Dx, cons
         }
         __syncthreads();
     }
+}
+
+template <class St, typename DT>
+__global__ void __launch_bounds__(256) k_combine(const DT* __restrict__ Dx, const DT* __restrict__ DyT,
+                                                 St Un, St Us, St out, St outT, int64_t nx, int64_t ny,
+                                                 int stage, const StepCtl* ctl)
+{
+    combine_tile<St, DT>(Dx, DyT, Un, Us, out, outT, nx, ny, stage, ctl, blockIdx.y, blockIdx.x);
 }
 
 // Plain RHS from the two sweeps (for tests): L = -(Dx + Dy^T), SoA4 interior.
@@ -401,6 +417,45 @@ static void h_transpose(const double* in, double* out, int64_t M, int64_t N, hip
     k_transpose<double, 4><<<grid, 256, 0, s>>>(in, out, M, N);
 }
 
+static void h_sweep1(const SweepPlan& p, int dir, int split, const void* U, void* D, int* flags,
+                     const StepCtl* ctl, hipStream_t s)
+{
+    const int blend = p.force_hlle ? 0 : 1;
+#if WX_FAST
+    if (split) {
+        if (dir == 0) {
+            launch_sweep<float, IN_F32, float, MODE_DETECT, DIR_X>(args_x(p, U, D, flags, ctl), p.cons, blend, s);
+            launch_sweep<double, IN_SPLIT, float, MODE_FIXUP, DIR_X>(args_x(p, U, D, flags, ctl), p.cons, blend, s);
+        } else {
+            launch_sweep<float, IN_F32, float, MODE_DETECT, DIR_Y>(args_y(p, U, D, flags, ctl), p.cons, blend, s);
+            launch_sweep<double, IN_SPLIT, float, MODE_FIXUP, DIR_Y>(args_y(p, U, D, flags, ctl), p.cons, blend, s);
+        }
+        return;
+    }
+#endif
+    if (dir == 0)
+        launch_sweep<double, IN_F64, double, MODE_PLAIN, DIR_X>(args_x(p, U, D, nullptr, ctl), p.cons, blend, s);
+    else
+        launch_sweep<double, IN_F64, double, MODE_PLAIN, DIR_Y>(args_y(p, U, D, nullptr, ctl), p.cons, blend, s);
+}
+
+static void h_combine1(const SweepPlan& p, int split, const void* Dx, const void* DyT, const void* Un,
+                       const void* Us, void* out, void* outT, const StepCtl* ctl, hipStream_t s)
+{
+    const int64_t P = (p.nx + 6) * (p.ny + 6);
+#if WX_FAST
+    if (split) {
+        k_combine<StSplit, float><<<combine_grid(p), 256, 0, s>>>(
+            (const float*)Dx, (const float*)DyT, StSplit{(float*)Un, P}, StSplit{(float*)Us, P},
+            StSplit{(float*)out, P}, StSplit{(float*)outT, P}, p.nx, p.ny, 2, ctl);
+        return;
+    }
+#endif
+    k_combine<StF64, double><<<combine_grid(p), 256, 0, s>>>(
+        (const double*)Dx, (const double*)DyT, StF64{(double*)Un, P}, StF64{(double*)Us, P},
+        StF64{(double*)out, P}, StF64{(double*)outT, P}, p.nx, p.ny, 2, ctl);
+}
+
 #if WX_FAST
 // ---- FP32-split -----------------------------------------------------------
 // Bulk sweep in FP32 from the hi planes; chunks with a non-positive or
@@ -479,17 +534,172 @@ static void h_cfl_split(const float* U, int64_t nx, int64_t ny, double gam, Step
 }
 #endif
 
+// ------------------------------------------------------------------------
+// Batched engine: the same bodies, workgroups mapped to instances by table.
+// ------------------------------------------------------------------------
+template <class St> __device__ __forceinline__ St make_st(void* base, int64_t soff, int64_t P);
+template <> __device__ __forceinline__ StF64 make_st<StF64>(void* base, int64_t soff, int64_t P)
+{
+    return StF64{(double*)base + soff, P};
+}
+template <> __device__ __forceinline__ StSplit make_st<StSplit>(void* base, int64_t soff, int64_t P)
+{
+    return StSplit{(float*)base + 2 * soff, P};
+}
+
+template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND>
+__global__ void __launch_bounds__(WAVE) k_sweep_batch(BatchTables B, BatchCommon C, const void* Ubase,
+                                                      void* Dbase, int* flags, StepCtl* ctls)
+{
+    const int b = blockIdx.x;
+    const int i = (DIR == DIR_X ? B.blk_x : B.blk_y)[b];
+    const BatchInst I = B.inst[i];
+    SweepArgs A;
+    A.U = (const char*)Ubase + I.soff * 8;
+    A.M = DIR == DIR_X ? I.nx + 6 : I.ny + 6;
+    A.N = DIR == DIR_X ? I.ny + 6 : I.nx + 6;
+    A.nn = DIR == DIR_X ? I.nx : I.ny;
+    A.nt = DIR == DIR_X ? I.ny : I.nx;
+    A.chunk = DIR == DIR_X ? I.chunk_x : I.chunk_y;
+    A.gam = C.gam;
+    A.dx = I.dx;
+    A.two_h = DIR == DIR_X ? I.two_h_x : I.two_h_y;
+    A.lo = C.jp_lo;
+    A.hi = C.jp_hi;
+    A.D = (DT*)Dbase + I.ioff;
+    A.flags = flags;
+    A.ctl = &ctls[i];
+    const int local = b - (DIR == DIR_X ? I.blk0_x : I.blk0_y);
+    const int gx = DIR == DIR_X ? I.gx_x : I.gx_y;
+    sweep_body<T, IN, DT, MODE, DIR, CONS, BLEND>(A, local % gx, local / gx, b);
+}
+
+template <class St, typename DT>
+__global__ void __launch_bounds__(256) k_combine_batch(BatchTables B, const DT* Dx, const DT* DyT, void* Un,
+                                                       void* Us, void* out, void* outT, int stage, StepCtl* ctls)
+{
+    const int b = blockIdx.x;
+    const int i = B.blk_c[b];
+    const BatchInst I = B.inst[i];
+    const int64_t P = (I.nx + 6) * (I.ny + 6);
+    const int local = b - I.tile0;
+    combine_tile<St, DT>(Dx + I.ioff, DyT + I.ioff, make_st<St>(Un, I.soff, P), make_st<St>(Us, I.soff, P),
+                         make_st<St>(out, I.soff, P), make_st<St>(outT, I.soff, P), I.nx, I.ny, stage,
+                         &ctls[i], local / I.tiles_j, local % I.tiles_j);
+}
+
+template <class St>
+__global__ void k_bc_batch(BatchTables B, BcParams bp, void* U, void* UT, StepCtl* ctls, int stage)
+{
+    const int b = blockIdx.x;
+    const int i = B.blk_g[b];
+    const BatchInst I = B.inst[i];
+    const int64_t P = (I.nx + 6) * (I.ny + 6);
+    bc_cell<St>(make_st<St>(U, I.soff, P), make_st<St>(UT, I.soff, P), true, I.nx, I.ny, bp, &ctls[i],
+                stage, (int64_t)(b - I.ghost_blk0) * ST_TB + threadIdx.x);
+}
+
+template <class St>
+__global__ void __launch_bounds__(ST_TB) k_cfl_batch(BatchTables B, void* U, double gam, StepCtl* ctls)
+{
+    const int b = blockIdx.x;
+    const int i = B.blk_f[b];
+    const BatchInst I = B.inst[i];
+    const int64_t P = (I.nx + 6) * (I.ny + 6);
+    cfl_block<St>(make_st<St>(U, I.soff, P), I.nx, I.ny, gam, &ctls[i], b - I.cfl_blk0, I.cfl_nblk);
+}
+
+__global__ void k_advance_batch(StepCtl* ctls, int n)
+{
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n && ctls[i].active) {
+        ctls[i].T += ctls[i].dt;
+        ctls[i].step += 1;
+    }
+}
+
+template <typename T, int IN, typename DT, int MODE, int DIR>
+static void launch_sweep_batch(const BatchTables& t, const BatchCommon& c, const void* U, void* D,
+                               int* flags, StepCtl* ctls, hipStream_t s)
+{
+    const unsigned n = (unsigned)(DIR == DIR_X ? t.n_x : t.n_y);
+    if (!n) return;
+    const bool blend = !c.force_hlle;
+    if (c.cons) {
+        if (blend) k_sweep_batch<T, IN, DT, MODE, DIR, true, true><<<n, WAVE, 0, s>>>(t, c, U, D, flags, ctls);
+        else k_sweep_batch<T, IN, DT, MODE, DIR, true, false><<<n, WAVE, 0, s>>>(t, c, U, D, flags, ctls);
+    } else {
+        if (blend) k_sweep_batch<T, IN, DT, MODE, DIR, false, true><<<n, WAVE, 0, s>>>(t, c, U, D, flags, ctls);
+        else k_sweep_batch<T, IN, DT, MODE, DIR, false, false><<<n, WAVE, 0, s>>>(t, c, U, D, flags, ctls);
+    }
+}
+
+static void hb_stage(const BatchTables& t, const BatchCommon& c, int split, const void* Us, const void* UsT,
+                     const void* Un, void* Dx, void* DyT, void* out, void* outT, int* flags, int stage,
+                     StepCtl* ctls, hipStream_t s)
+{
+#if WX_FAST
+    if (split) {
+        launch_sweep_batch<float, IN_F32, float, MODE_DETECT, DIR_X>(t, c, Us, Dx, flags, ctls, s);
+        launch_sweep_batch<double, IN_SPLIT, float, MODE_FIXUP, DIR_X>(t, c, Us, Dx, flags, ctls, s);
+        launch_sweep_batch<float, IN_F32, float, MODE_DETECT, DIR_Y>(t, c, UsT, DyT, flags, ctls, s);
+        launch_sweep_batch<double, IN_SPLIT, float, MODE_FIXUP, DIR_Y>(t, c, UsT, DyT, flags, ctls, s);
+        k_combine_batch<StSplit, float><<<(unsigned)t.n_c, 256, 0, s>>>(
+            t, (const float*)Dx, (const float*)DyT, (void*)Un, (void*)Us, out, outT, stage, ctls);
+        return;
+    }
+#endif
+    launch_sweep_batch<double, IN_F64, double, MODE_PLAIN, DIR_X>(t, c, Us, Dx, nullptr, ctls, s);
+    launch_sweep_batch<double, IN_F64, double, MODE_PLAIN, DIR_Y>(t, c, UsT, DyT, nullptr, ctls, s);
+    k_combine_batch<StF64, double><<<(unsigned)t.n_c, 256, 0, s>>>(
+        t, (const double*)Dx, (const double*)DyT, (void*)Un, (void*)Us, out, outT, stage, ctls);
+}
+
+static void hb_bc(const BatchTables& t, BcParams bp, int split, void* U, void* UT, StepCtl* ctls, int stage,
+                  hipStream_t s)
+{
+#if WX_FAST
+    if (split) {
+        k_bc_batch<StSplit><<<(unsigned)t.n_g, ST_TB, 0, s>>>(t, bp, U, UT, ctls, stage);
+        return;
+    }
+#endif
+    k_bc_batch<StF64><<<(unsigned)t.n_g, ST_TB, 0, s>>>(t, bp, U, UT, ctls, stage);
+}
+
+static void hb_cfl(const BatchTables& t, int split, const void* U, double gam, StepCtl* ctls, hipStream_t s)
+{
+#if WX_FAST
+    if (split) {
+        k_cfl_batch<StSplit><<<(unsigned)t.n_f, ST_TB, 0, s>>>(t, (void*)U, gam, ctls);
+        return;
+    }
+#endif
+    k_cfl_batch<StF64><<<(unsigned)t.n_f, ST_TB, 0, s>>>(t, (void*)U, gam, ctls);
+}
+
+static void hb_advance(StepCtl* ctls, int n, hipStream_t s)
+{
+    k_advance_batch<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(ctls, n);
+}
+
 }  // namespace WX_NS
 
 #ifndef __HIP_DEVICE_COMPILE__
 #if WX_FAST
+const BatchOps batch_ops_fast = {WX_NS::hb_stage, WX_NS::hb_bc, WX_NS::hb_cfl, WX_NS::hb_advance};
+#else
+const BatchOps batch_ops_exact = {WX_NS::hb_stage, WX_NS::hb_bc, WX_NS::hb_cfl, WX_NS::hb_advance};
+#endif
+#if WX_FAST
 const SweepOps WX_SWEEP_OPS = {WX_NS::h_stage,       WX_NS::h_rhs,        WX_NS::h_transpose,
                                WX_NS::h_stage_split, WX_NS::h_rhs_split,  WX_NS::h_transpose_split,
                                WX_NS::h_to_split,    WX_NS::h_from_split, WX_NS::h_bc_split,
-                               WX_NS::h_cfl_split};
+                               WX_NS::h_cfl_split,   WX_NS::h_sweep1,     WX_NS::h_combine1};
 #else
 const SweepOps WX_SWEEP_OPS = {WX_NS::h_stage, WX_NS::h_rhs, WX_NS::h_transpose, nullptr, nullptr,
-                               nullptr,        nullptr,      nullptr,            nullptr, nullptr};
+                               nullptr,        nullptr,      nullptr,            nullptr, nullptr,
+                               WX_NS::h_sweep1, WX_NS::h_combine1};
 
 // Chunk length: enough waves to cover the 1024 SIMDs twice where the grid
 // allows it, within [8, 64] rows (a chunk recomputes 2 halo faces for the

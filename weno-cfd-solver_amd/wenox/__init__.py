@@ -112,3 +112,110 @@ class Solver:
     @property
     def ms_per_step(self) -> float:
         return _lib.load().wenox_solver_last_ms_per_step(self._h)
+
+    KERNELS = ("cfl", "sweep_x", "sweep_y", "combine", "ghost_fill", "step_sum")
+
+    def profile(self, reps: int = 10) -> dict:
+        """Per-kernel device time (ms) of the fused engine, HIP events on the
+        solver's stream, each kernel launched `reps` times alone."""
+        out = np.zeros(6)
+        _lib.check(_lib.load().wenox_solver_profile(self._h, int(reps), out.ctypes.data, 6))
+        return dict(zip(self.KERNELS, out.tolist()))
+
+    @property
+    def chunks(self):
+        cx, cy = ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.load().wenox_solver_plan(self._h, ctypes.byref(cx), ctypes.byref(cy)))
+        return cx.value, cy.value
+
+    def set_chunks(self, chunk_x: int, chunk_y: int):
+        _lib.check(_lib.load().wenox_solver_set_chunks(self._h, int(chunk_x), int(chunk_y)))
+
+    def autotune(self, candidates=(8, 12, 16, 24, 32, 48, 64), reps: int = 5) -> dict:  # noqa: D401
+        """Pick the x- and y-sweep tile lengths (rows marched per workgroup)
+        with the lowest measured kernel time on this device; keeps the best."""
+        best = {}
+        for d, key in ((0, "sweep_x"), (1, "sweep_y")):
+            times = {}
+            cx0, cy0 = self.chunks
+            for c in candidates:
+                self.set_chunks(c if d == 0 else cx0, c if d == 1 else cy0)
+                times[c] = self.profile(reps)[key]
+            cbest = min(times, key=times.get)
+            cx0, cy0 = self.chunks
+            self.set_chunks(cbest if d == 0 else cx0, cbest if d == 1 else cy0)
+            best[key] = (cbest, times)
+        return best
+
+
+class Batch:
+    """Many independent problems of uneven sizes advanced together on the
+    device, each with its own dx, time and CFL dt; every phase of a step is
+    one launch over all instances and a step is one hipGraph replay."""
+
+    def __init__(self, sizes, dx, gamma=1.4, cfl=0.4, cons_switch=False, force_hlle=False,
+                 jp_cri=(0.2, 1.0), bc="mirror", variant="fast", precision="fp64", chunk=16,
+                 use_graph=True):
+        d = _lib.SolverDesc()
+        d.gamma, d.cfl = float(gamma), float(cfl)
+        d.cons_switch, d.force_hlle = int(bool(cons_switch)), int(bool(force_hlle))
+        d.jp_lo, d.jp_hi = float(jp_cri[0]), float(jp_cri[1])
+        d.bc = _BC_NAMES[bc] if isinstance(bc, str) else int(bc)
+        d.engine = ENGINE_FUSED
+        d.variant = {"exact": EXACT, "fast": FAST}[variant] if isinstance(variant, str) else int(variant)
+        d.precision = {"fp64": PREC_FP64, "fp32split": PREC_FP32SPLIT}[precision] if isinstance(precision, str) else int(precision)
+        d.use_graph = int(bool(use_graph))
+        self.sizes = [(int(a), int(b)) for a, b in sizes]
+        self.n = len(self.sizes)
+        nx = np.array([s[0] for s in self.sizes], dtype=np.int64)
+        ny = np.array([s[1] for s in self.sizes], dtype=np.int64)
+        dxa = np.broadcast_to(np.asarray(dx, dtype=np.float64), (self.n,)).copy()
+        self._h = ctypes.c_void_p()
+        _lib.check(_lib.load().wenox_batch_create(ctypes.byref(d), self.n, nx.ctypes.data, ny.ctypes.data,
+                                                  dxa.ctypes.data, int(chunk), ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            _lib.load().wenox_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, i, q, t0=0.0):
+        q = np.asarray(q, dtype=np.float64)
+        v = _lib.view(q)
+        _lib.check(_lib.load().wenox_batch_upload(self._h, int(i), ctypes.byref(v), float(t0)))
+
+    def download(self, i):
+        nx, ny = self.sizes[i]
+        out = np.empty((nx + 6, ny + 6, 8))
+        _lib.check(_lib.load().wenox_batch_download(self._h, int(i), out.ctypes.data))
+        return out
+
+    def run(self, nstep=None, t_final=None):
+        """Returns the number of instance-steps taken (summed over instances)."""
+        ms = int(nstep) if nstep is not None else int(1e8)
+        tf = float(t_final) if t_final is not None else -1.0
+        k = ctypes.c_int64()
+        _lib.check(_lib.load().wenox_batch_run(self._h, ms, tf, ctypes.byref(k)))
+        return k.value
+
+    def status(self):
+        T = np.zeros(self.n)
+        k = np.zeros(self.n, dtype=np.int64)
+        _lib.check(_lib.load().wenox_batch_status(self._h, T.ctypes.data, k.ctypes.data))
+        return T, k
+
+    @property
+    def ms_per_step(self) -> float:
+        return _lib.load().wenox_batch_last_ms_per_step(self._h)

@@ -84,6 +84,17 @@ def load():
                               ctypes.POINTER(ctypes.c_int64)], i),
         "wenox_solver_rhs": ([dp, dp], i),
         "wenox_solver_last_ms_per_step": ([dp], d),
+        "wenox_solver_profile": ([dp, i, dp, i], i),
+        "wenox_solver_plan": ([dp, ctypes.POINTER(i), ctypes.POINTER(i)], i),
+        "wenox_solver_set_chunks": ([dp, i, i], i),
+        "wenox_batch_create": ([ctypes.POINTER(SolverDesc), i, dp, dp, dp, i,
+                                ctypes.POINTER(ctypes.c_void_p)], i),
+        "wenox_batch_destroy": ([dp], None),
+        "wenox_batch_upload": ([dp, i, V, d], i),
+        "wenox_batch_download": ([dp, i, dp], i),
+        "wenox_batch_run": ([dp, ctypes.c_int64, d, ctypes.POINTER(ctypes.c_int64)], i),
+        "wenox_batch_status": ([dp, dp, dp], i),
+        "wenox_batch_last_ms_per_step": ([dp], d),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
