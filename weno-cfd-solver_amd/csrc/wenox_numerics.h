@@ -375,7 +375,11 @@ template <typename T> WX_HD void weno5_gq(T q0, T q1, T q2, T q3, T q4, T& ga, T
 template <typename T> WX_HD T pressure_pp(const T q[4], T gam)
 {
     if (q[0] <= T(0)) return T(0);
+#if WX_FAST
+    return (gam - T(1)) * (q[3] - T(0.5) * (q[1] * q[1] + q[2] * q[2]) * rcp(q[0]));
+#else
     return (gam - T(1)) * (q[3] - T(0.5) * (q[1] * q[1] + q[2] * q[2]) / q[0]);
+#endif
 }
 
 template <typename T> WX_HD void pp_limit(T q[4], const T ub[4], T gam)
@@ -546,7 +550,14 @@ template <typename T> WX_HD void hlle_from(const FaceState<T>& s, const T fl[4],
         const T b[4] = {s.qr0, s.qr1, s.qr2, s.qr3};
 #pragma unroll
         for (int k = 0; k < 4; ++k)
+#if WX_FAST
+        {
+            const T rden = T(1) / den;   // one division for the four components
+            out[k] = (s.s_r * fl[k] - s.s_l * fr[k] + s.s_r * s.s_l * (b[k] - a[k])) * rden;
+        }
+#else
             out[k] = (s.s_r * fl[k] - s.s_l * fr[k] + s.s_r * s.s_l * (b[k] - a[k])) / den;
+#endif
     }
 }
 

@@ -100,6 +100,15 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
     __shared__ T s_pb[2][WAVE];     // p̄ ring (face parity)
 
     const T gam = (T)A.gam, two_h = (T)A.two_h, jlo = (T)A.lo, jhi = (T)A.hi, dx = (T)A.dx;
+#if WX_FAST
+    // the fast variant multiplies by host-formed reciprocals of the constant divisors
+    const T inv_two_h = (T)(1.0 / A.two_h), inv_dx = (T)(1.0 / A.dx);
+#define WX_DIV_2H(x) ((x) * inv_two_h)
+#define WX_DIV_DX(x) ((x) * inv_dx)
+#else
+#define WX_DIV_2H(x) ((x) / two_h)
+#define WX_DIV_DX(x) ((x) / dx)
+#endif
     const int l = threadIdx.x;
     const int64_t c0 = bx * OUT_COLS;                       // padded column of lane 0
     const int64_t pc = c0 + l;                              // this lane's padded column
@@ -135,7 +144,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
         if (out_lane && f - 1 >= r0 && f - 1 < r1) {
             DT* d = Dout + (f - 1) * A.nt + t;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) d[k * Pd] = (DT)((F[k] - F_prev[k]) / dx);
+            for (int k = 0; k < 4; ++k) d[k * Pd] = (DT)WX_DIV_DX(F[k] - F_prev[k]);
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) F_prev[k] = F[k];
@@ -144,9 +153,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
     auto sensor = [&](int64_t f, T dn, T pbf) -> T {
         const T* ring = s_pb[f & 1];
         T dt_;
-        if (t <= 0) dt_ = (ring[lp] - ring[l]) / two_h;
-        else if (t >= A.nt - 1) dt_ = (ring[l] - ring[lm]) / two_h;
-        else dt_ = (ring[lp] - ring[lm]) / two_h;
+        if (t <= 0) dt_ = WX_DIV_2H(ring[lp] - ring[l]);
+        else if (t >= A.nt - 1) dt_ = WX_DIV_2H(ring[l] - ring[lm]);
+        else dt_ = WX_DIV_2H(ring[lp] - ring[lm]);
         // the reference's dpdx is along x: normal for x faces, transverse for y faces
         return DIR == DIR_X ? sensor_phi(dn, dt_, pbf, jlo, jhi) : sensor_phi(dt_, dn, pbf, jlo, jhi);
     };
@@ -218,7 +227,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
             if (a > a0) {
                 // sensor and blend of face f = a-1 (riemann.rs:360-407)
                 const int64_t f = a - 1;
-                T dn = f == 0 ? (pb - pb_m1) / two_h : (pb - pb_m2) / two_h;
+                T dn = f == 0 ? WX_DIV_2H(pb - pb_m1) : WX_DIV_2H(pb - pb_m2);
                 T phi = sensor(f, dn, pb_m1);
                 T F[4];
 #pragma unroll
@@ -236,7 +245,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
     if (BLEND && a1 == A.nn) {
         // last face of the grid: one-sided normal difference, no face a1+1
         const int64_t f = a1;
-        T dn = (pb_m1 - pb_m2) / two_h;
+        T dn = WX_DIV_2H(pb_m1 - pb_m2);
         T phi = sensor(f, dn, pb_m1);
         T F[4];
 #pragma unroll
@@ -248,6 +257,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
         if (l == 0) A.flags[bid] = m != 0ull ? 1 : 0;
     }
 }
+
+#undef WX_DIV_2H
+#undef WX_DIV_DX
 
 template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND>
 __global__ void __launch_bounds__(WAVE) k_sweep(SweepArgs A)
