@@ -105,6 +105,7 @@ int wenox_l_local(const wenox_view3 *u, double dx, double gamma, int force_hlle,
 #define WENOX_BC_PERIODIC 1
 #define WENOX_BC_DMR_REF 2   /* unitest.py:207-227 apply_bc_refac */
 #define WENOX_BC_DMR_STD 3   /* standard double Mach reflection, exact moving shock on top */
+#define WENOX_BC_COPY 4      /* Rie_prob_1D_rs.py:59-76 apply_bc_corrected (constant extrapolation) */
 
 #define WENOX_ENGINE_UNFUSED 0  /* the reference's pass structure, face arrays in HBM */
 #define WENOX_ENGINE_FUSED 1    /* one fused kernel per direction sweep per RK stage */
@@ -154,6 +155,10 @@ double wenox_solver_last_ms_per_step(wenox_solver *s);
  * out_ms[0] CFL reduction, [1] x sweep, [2] y sweep, [3] RK combine,
  * [4] ghost fill, [5] their sum over one step (3 stages).  n_out >= 6. */
 int wenox_solver_profile(wenox_solver *s, int reps, double *out_ms, int n_out);
+/* CFL rule dt = CFL dx / (max(sqrt(g p / max(rho, rho_floor)) + |V|) + cmax_eps):
+ * (1e-14, 1e-16) by default as Rie_prob_2d_rs.py:311-319; the 1-D driver
+ * Rie_prob_1D_rs.py:165-168 uses (1e-10, 0). */
+int wenox_solver_set_cfl_rule(wenox_solver *s, double rho_floor, double cmax_eps);
 /* Rows marched per workgroup by the x and y sweeps (the fused engine's tile length). */
 int wenox_solver_plan(wenox_solver *s, int *chunk_x, int *chunk_y);
 int wenox_solver_set_chunks(wenox_solver *s, int chunk_x, int chunk_y);

@@ -350,6 +350,56 @@ def rk3_refac(q, nx, ny, dx, dt, gamma, force_hlle, bound, jp_cri=(0.2, 1.0),
     return apply_bc_refac(qn, bound, nx, ny)
 
 
+def apply_bc_copy(u, nx, ny):
+    """Constant extrapolation (Rie_prob_1D_rs.py:59-76)."""
+    for i in (0, 1, 2):
+        u[i, :, :] = u[3, :, :]
+    for i in (nx + 3, nx + 4, nx + 5):
+        u[i, :, :] = u[nx + 2, :, :]
+    for j in (0, 1, 2):
+        u[:, j, :] = u[:, 3, :]
+    for j in (ny + 3, ny + 4, ny + 5):
+        u[:, j, :] = u[:, ny + 2, :]
+    return u
+
+
+def rk3_1d(q, nx, ny, dx, dt, gamma, force_hlle=True, jp_cri=(1, 5)):
+    """Rie_prob_1D_rs.rk3 (:20-57): component-wise l_local, ghost fill after
+    each stage, primitive sync at the end."""
+    sl = np.s_[3:nx + 3, 3:ny + 3, :4]
+    l0 = l_local_py(q, dx, gamma, force_hlle, jp_cri, True)
+    q1 = q.copy()
+    q1[sl] = q[sl] + dt * l0
+    q1 = apply_bc_copy(q1, nx, ny)
+    l1 = l_local_py(q1, dx, gamma, force_hlle, jp_cri, True)
+    q2 = q.copy()
+    q2[sl] = 0.75 * q[sl] + 0.25 * q1[sl] + 0.25 * dt * l1
+    q2 = apply_bc_copy(q2, nx, ny)
+    l2 = l_local_py(q2, dx, gamma, force_hlle, jp_cri, True)
+    qn = q.copy()
+    qn[sl] = 1 / 3 * q[sl] + 2 / 3 * q2[sl] + 2 / 3 * dt * l2
+    qn = apply_bc_copy(qn, nx, ny)
+    p, a, rho, u, v, h = con2primi_py(qn, gamma)
+    qn[:, :, 4] = rho
+    qn[:, :, 5] = u
+    qn[:, :, 6] = v
+    qn[:, :, 7] = p
+    return apply_bc_copy(qn, nx, ny)
+
+
+def run_1d(q, nx, ny, dx, gamma, cfl, nstep):
+    """The time loop of Rie_prob_1D_rs.main (:159-175) without I/O."""
+    T = 0.0
+    for _ in range(1, nstep):
+        speed = np.sqrt(q[3:nx + 3, 3:ny + 3, 5] ** 2 + q[3:nx + 3, 3:ny + 3, 6] ** 2)
+        c_max = np.max(np.sqrt(gamma * q[3:nx + 3, 3:ny + 3, 7]
+                               / np.maximum(q[3:nx + 3, 3:ny + 3, 0], 1e-10)) + speed)
+        dt = cfl * dx / c_max
+        q = rk3_1d(q, nx, ny, dx, dt, gamma)
+        T += dt
+    return q, T
+
+
 def cfl_dt(q, nx, ny, gamma, cfl, dx, remaining=None):
     """Rie_prob_2d_rs.py:311-319 (and double_mach_reflection.py:233-240)."""
     speed = np.sqrt(q[3:3 + nx, 3:3 + ny, 5] ** 2 + q[3:3 + nx, 3:3 + ny, 6] ** 2)

@@ -434,6 +434,8 @@ struct wenox_solver {
     const SweepOps* sops = nullptr;
     bool split = false;      // FP32-split state: buffers hold float planes [8][M][N]
     int* flags = nullptr;    // FP32-split: per-workgroup FP64-recompute flags
+    double cfl_rho_floor = 1e-14;   // Rie_prob_2d_rs.py:314
+    double cfl_cmax_eps = 1e-16;    // Rie_prob_2d_rs.py:315
 };
 
 namespace {
@@ -537,7 +539,7 @@ extern "C" int wenox_solver_create(const wenox_solver_desc* desc, wenox_solver**
     const auto& d = *desc;
     if (d.nx < 3 || d.ny < 3) return fail(WENOX_E_SHAPE, "solver needs nx >= 3 and ny >= 3");
     if (!d.force_hlle && (d.nx < 2 || d.ny < 2)) return fail(WENOX_E_SHAPE, "blend needs nx, ny >= 2");
-    if (d.bc < 0 || d.bc > 3) return fail(WENOX_E_ARG, "unknown boundary kind");
+    if (d.bc < 0 || d.bc > 4) return fail(WENOX_E_ARG, "unknown boundary kind");
     if (d.bc == WENOX_BC_DMR_REF && d.nx + 2 < d.dmr_bound + 10)
         return fail(WENOX_E_SHAPE, "DMR reference ghost fill needs nx + 2 >= bound + 10");
     if (d.variant != WENOX_EXACT && d.variant != WENOX_FAST) return fail(WENOX_E_ARG, "unknown variant");
@@ -597,6 +599,8 @@ extern "C" int wenox_solver_create(const wenox_solver_desc* desc, wenox_solver**
     s->bp.gamma = d.gamma;
     StepCtl h{};
     h.cfl_dx = d.cfl * d.dx;
+    h.rho_floor = s->cfl_rho_floor;
+    h.cmax_eps = s->cfl_cmax_eps;
     h.t_final = -1.0;
     h.active = 1;
     if ((rc = write_ctl(s, &h))) return cleanup(rc);
@@ -639,6 +643,8 @@ extern "C" int wenox_solver_upload(wenox_solver* s, const wenox_view3* q, double
         }
     StepCtl h{};
     h.cfl_dx = d.cfl * d.dx;
+    h.rho_floor = s->cfl_rho_floor;
+    h.cmax_eps = s->cfl_cmax_eps;
     h.t_final = -1.0;
     h.T = t0;
     h.active = 1;
@@ -650,7 +656,7 @@ extern "C" int wenox_solver_upload(wenox_solver* s, const wenox_view3* q, double
             for (int64_t j = 3; j < 3 + d.ny; ++j) {
                 const double* c = q->data + i * s0 + j * s1;
                 double u = c[5 * s2], v = c[6 * s2], p = c[7 * s2], r = c[0];
-                double w = std::sqrt(d.gamma * p / std::max(r, 1e-14)) + std::sqrt(u * u + v * v);
+                double w = std::sqrt(d.gamma * p / std::max(r, s->cfl_rho_floor)) + std::sqrt(u * u + v * v);
                 if (w != w) { cm = w; nan = true; break; }
                 if (w > cm) cm = w;
             }
@@ -855,6 +861,19 @@ extern "C" int wenox_solver_profile(wenox_solver* s, int reps, double* out_ms, i
          })))
         return rc;
     out_ms[5] = 3.0 * (out_ms[1] + out_ms[2] + out_ms[3] + out_ms[4]) + out_ms[0];
+    return write_ctl(s, &h);
+}
+
+extern "C" int wenox_solver_set_cfl_rule(wenox_solver* s, double rho_floor, double cmax_eps)
+{
+    if (!s || !(rho_floor >= 0.0) || !(cmax_eps >= 0.0)) return fail(WENOX_E_ARG, "wenox_solver_set_cfl_rule: bad argument");
+    s->cfl_rho_floor = rho_floor;
+    s->cfl_cmax_eps = cmax_eps;
+    StepCtl h;
+    int rc = read_ctl(s, &h);
+    if (rc) return rc;
+    h.rho_floor = rho_floor;
+    h.cmax_eps = cmax_eps;
     return write_ctl(s, &h);
 }
 

@@ -113,8 +113,8 @@ extern "C" int wenox_batch_create(const wenox_solver_desc* desc, int n, const in
     if (!desc || !out || n < 1 || !nx || !ny || !dx) return abi_fail(WENOX_E_ARG, "wenox_batch_create: bad argument");
     *out = nullptr;
     const auto& d = *desc;
-    if (d.bc != WENOX_BC_MIRROR && d.bc != WENOX_BC_PERIODIC)
-        return abi_fail(WENOX_E_ARG, "the batched engine supports mirror and periodic ghost fills");
+    if (d.bc != WENOX_BC_MIRROR && d.bc != WENOX_BC_PERIODIC && d.bc != WENOX_BC_COPY)
+        return abi_fail(WENOX_E_ARG, "the batched engine supports the mirror, periodic and copy ghost fills");
     if (d.precision != WENOX_PREC_FP64 && d.precision != WENOX_PREC_FP32SPLIT)
         return abi_fail(WENOX_E_ARG, "unknown precision");
     for (int i = 0; i < n; ++i)
@@ -204,6 +204,8 @@ extern "C" int wenox_batch_create(const wenox_solver_desc* desc, int n, const in
     for (int i = 0; i < n; ++i) {
         c[i] = StepCtl{};
         c[i].cfl_dx = d.cfl * dx[i];
+        c[i].rho_floor = 1e-14;   // Rie_prob_2d_rs.py:314-315
+        c[i].cmax_eps = 1e-16;
         c[i].t_final = -1.0;
         c[i].active = 1;
     }
@@ -235,7 +237,7 @@ extern "C" int wenox_batch_upload(wenox_batch* b, int i, const wenox_view3* q, d
         for (int64_t a = 3; a < 3 + I.nx; ++a)
             for (int64_t c = 3; c < 3 + I.ny; ++c) {
                 const double* v = q->data + a * s0 + c * s1;
-                double w = std::sqrt(b->d.gamma * v[7 * s2] / std::max(v[0], 1e-14)) +
+                double w = std::sqrt(b->d.gamma * v[7 * s2] / std::max(v[0], h.rho_floor)) +
                            std::sqrt(v[5 * s2] * v[5 * s2] + v[6 * s2] * v[6 * s2]);
                 if (w != w || w > cm) cm = w;
             }

@@ -20,6 +20,8 @@ struct StepCtl {
     double t_final;        // < 0: step-count controlled
     double cfl_dx;         // CFL * dx (host product, as the reference forms it)
     double cmax0;          // first-step wave speed from the caller's primitives
+    double rho_floor;      // CFL: c = sqrt(g p / max(rho, rho_floor))   (1e-14; the 1-D driver 1e-10)
+    double cmax_eps;       // CFL: dt = cfl_dx / (c_max + cmax_eps)      (1e-16; the 1-D driver 0)
     long long step;        // steps taken
     long long max_steps;   // safety cap / step count
     int active;            // current step runs (loop condition held)
@@ -34,6 +36,7 @@ enum BcKind : int {
     BC_PERIODIC = 1,
     BC_DMR_REF = 2,     // unitest.py:207-227 (final state of its write sequence)
     BC_DMR_STD = 3,     // standard double Mach reflection with exact moving shock
+    BC_COPY = 4,        // constant extrapolation: ghost <- nearest interior (Rie_prob_1D_rs.py:59-76)
 };
 
 struct BcParams {

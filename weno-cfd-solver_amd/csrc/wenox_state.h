@@ -91,7 +91,13 @@ __device__ __forceinline__ void bc_cell(St U, St UT, bool has_t, int64_t nx, int
     int64_t i, j;
     ghost_index(t, nx, ny, i, j);
     double v[4];
-    if (bp.kind == BC_MIRROR || bp.kind == BC_PERIODIC) {
+    if (bp.kind == BC_COPY) {
+        // x rows first, then y columns over all rows: ghost (i, j) <- (clamp i, clamp j)
+        int64_t si = i < 3 ? 3 : (i > nx + 2 ? nx + 2 : i);
+        int64_t sj = j < 3 ? 3 : (j > ny + 2 ? ny + 2 : j);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = U.ld(k, si * n + sj);
+    } else if (bp.kind == BC_MIRROR || bp.kind == BC_PERIODIC) {
         int64_t si = bp.kind == BC_MIRROR ? mirror_src(i, nx) : periodic_src(i, nx);
         int64_t sj = bp.kind == BC_MIRROR ? mirror_src(j, ny) : periodic_src(j, ny);
 #pragma unroll
@@ -208,7 +214,7 @@ __device__ __forceinline__ void cfl_block(St U, int64_t nx, int64_t ny, double g
         double q[4] = {U.ld(0, o), U.ld(1, o), U.ld(2, o), U.ld(3, o)};
         Prim<double> r = con2primi_pt(q, gam);
         double spd = sqrt(r.u * r.u + r.v * r.v);
-        double c = sqrt(gam * r.p / fmax(q[0], 1e-14));
+        double c = sqrt(gam * r.p / fmax(q[0], ctl->rho_floor));
         double w = c + spd;
         vmaxl = nan_max(vmaxl, w);
     }
@@ -237,7 +243,7 @@ __device__ __forceinline__ void cfl_block(St U, int64_t nx, int64_t ny, double g
     bool run = ctl->step < ctl->max_steps;
     if (ctl->t_final >= 0.0) run = run && (T < ctl->t_final - 1e-12);
     ctl->active = run ? 1 : 0;
-    double c_max = cmax + 1e-16;
+    double c_max = cmax + ctl->cmax_eps;
     double dt = ctl->cfl_dx / c_max;
     if (ctl->t_final >= 0.0) {
         double rem = ctl->t_final - T;

@@ -15,12 +15,12 @@ import numpy as np
 from . import _lib
 from ._lib import EXACT, FAST, PanicException, WenoxError  # noqa: F401
 
-BC_MIRROR, BC_PERIODIC, BC_DMR_REF, BC_DMR_STD = 0, 1, 2, 3
+BC_MIRROR, BC_PERIODIC, BC_DMR_REF, BC_DMR_STD, BC_COPY = 0, 1, 2, 3, 4
 ENGINE_UNFUSED, ENGINE_FUSED = 0, 1
 PREC_FP64, PREC_FP32SPLIT = 0, 1
 
 _BC_NAMES = {"mirror": BC_MIRROR, "transmissive": BC_MIRROR, "zero_gradient": BC_MIRROR,
-             "periodic": BC_PERIODIC, "dmr_ref": BC_DMR_REF, "dmr": BC_DMR_STD}
+             "periodic": BC_PERIODIC, "dmr_ref": BC_DMR_REF, "dmr": BC_DMR_STD, "copy": BC_COPY}
 
 
 def device_count() -> int:
@@ -130,6 +130,11 @@ class Solver:
 
     def set_chunks(self, chunk_x: int, chunk_y: int):
         _lib.check(_lib.load().wenox_solver_set_chunks(self._h, int(chunk_x), int(chunk_y)))
+
+    def set_cfl_rule(self, rho_floor: float = 1e-14, cmax_eps: float = 1e-16):
+        """dt = CFL dx / (max(sqrt(g p / max(rho, rho_floor)) + |V|) + cmax_eps);
+        set before upload() for the first step to follow it too."""
+        _lib.check(_lib.load().wenox_solver_set_cfl_rule(self._h, float(rho_floor), float(cmax_eps)))
 
     def autotune(self, candidates=(8, 12, 16, 24, 32, 48, 64), reps: int = 5) -> dict:  # noqa: D401
         """Pick the x- and y-sweep tile lengths (rows marched per workgroup)

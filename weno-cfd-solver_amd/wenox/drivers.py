@@ -75,6 +75,27 @@ def main_rs(grid_dict: dict, control_dict: dict, phys_dict: dict, q0=None, engin
     return None
 
 
+def main_1d(grid_info: dict, control_dict: dict, gamma: float = 1.4, file_storage: bool = False,
+            engine="fused", variant="exact"):
+    """Rie_prob_1D_rs.main (:110-181) on the device: the Sod tube through the
+    2D kernels with ny = 3, constant-extrapolation ghost fill
+    (apply_bc_corrected, :59-76), component-wise reconstruction and HLLE
+    (its rk3 calls l_local_py(..., force_hlle=True, jp_cri=(1, 5), True)),
+    CFL dt = CFL dx / max(sqrt(g p / max(rho, 1e-10)) + |V|) and
+    control_dict["nstep"] - 1 steps.  Returns (q, time list)."""
+    nx, ny = int(grid_info["nx"]), int(grid_info["ny"])
+    dx = float(grid_info["dx"])
+    q0 = problems.sod_reference(nx, ny, gamma)
+    ctrl = {"nstep": int(control_dict["nstep"]) - 1, "file_storage": file_storage}
+    with Solver(nx, ny, dx, gamma=gamma, cfl=float(control_dict["CFL"]), cons_switch=True,
+                force_hlle=True, jp_cri=(1, 5), bc="copy", engine=engine, variant=variant) as s:
+        s.set_cfl_rule(1e-10, 0.0)
+        snapshot = _snapshot_dir(ctrl)
+        q = _run(s, q0, ctrl, nx, ny, snapshot)
+        t = s.t
+    return q, t
+
+
 def main_rs_dmr(grid_dict: dict, control_dict: dict, phys_dict: dict, engine="fused",
                 variant="exact"):
     """double_mach_reflection.main_rs on the device: unitest.init_refac data

@@ -103,6 +103,89 @@ def _reflecting_fill(u):
     return u
 
 
+def sod_reference(nx: int, ny: int, gamma: float = 1.4):
+    """Rie_prob_1D_rs.main's initial data (:121-153): Sod tube split at padded
+    row int((max(nx, ny) + 6) / 2), (1, 0, 0, 1) | (0.125, 0, 0, 0.1), then its
+    constant-extrapolation ghost fill."""
+    q = np.zeros((nx + 6, ny + 6, 8))
+    bound = int((max(nx, ny) + 6) / 2)
+    q[:bound, :, 4:8] = (1.0, 0.0, 0.0, 1.0)
+    q[bound:, :, 4:8] = (0.125, 0.0, 0.0, 0.1)
+    _from_prims(q, gamma)
+    for i in (0, 1, 2):
+        q[i] = q[3]
+    for i in (nx + 3, nx + 4, nx + 5):
+        q[i] = q[nx + 2]
+    for j in (0, 1, 2):
+        q[:, j] = q[:, 3]
+    for j in (ny + 3, ny + 4, ny + 5):
+        q[:, j] = q[:, ny + 2]
+    return q
+
+
+def sod_exact(x, t, x0=0.5, gamma=1.4, left=(1.0, 0.0, 1.0), right=(0.125, 0.0, 0.1)):
+    """Exact solution (rho, u, p) of the Riemann problem at points x, time t
+    (Toro, Riemann Solvers and Numerical Methods, ch. 4: Newton iteration for
+    p*, then sampling of the wave pattern)."""
+    rl, ul, pl = left
+    rr, ur, pr = right
+    g = gamma
+    cl, cr = math.sqrt(g * pl / rl), math.sqrt(g * pr / rr)
+
+    def fk(p, rk, pk, ck):
+        if p > pk:
+            A, B = 2 / ((g + 1) * rk), (g - 1) / (g + 1) * pk
+            f = (p - pk) * math.sqrt(A / (p + B))
+            df = math.sqrt(A / (B + p)) * (1 - (p - pk) / (2 * (B + p)))
+        else:
+            f = 2 * ck / (g - 1) * ((p / pk) ** ((g - 1) / (2 * g)) - 1)
+            df = 1 / (rk * ck) * (p / pk) ** (-(g + 1) / (2 * g))
+        return f, df
+
+    p = max(1e-8, 0.5 * (pl + pr))
+    for _ in range(100):
+        f1, d1 = fk(p, rl, pl, cl)
+        f2, d2 = fk(p, rr, pr, cr)
+        dp = (f1 + f2 + ur - ul) / (d1 + d2)
+        p = max(1e-12, p - dp)
+        if abs(dp) < 1e-14 * p:
+            break
+    us = 0.5 * (ul + ur) + 0.5 * (fk(p, rr, pr, cr)[0] - fk(p, rl, pl, cl)[0])
+    out = np.zeros((3,) + np.shape(x))
+    s = (np.asarray(x) - x0) / max(t, 1e-300)
+    for idx, si in np.ndenumerate(s):
+        if si <= us:
+            if p > pl:   # left shock
+                sl = ul - cl * math.sqrt((g + 1) / (2 * g) * p / pl + (g - 1) / (2 * g))
+                val = (rl, ul, pl) if si <= sl else (rl * ((p / pl + (g - 1) / (g + 1)) / ((g - 1) / (g + 1) * p / pl + 1)), us, p)
+            else:        # left rarefaction
+                shl, stl = ul - cl, us - cl * (p / pl) ** ((g - 1) / (2 * g))
+                if si <= shl:
+                    val = (rl, ul, pl)
+                elif si >= stl:
+                    val = (rl * (p / pl) ** (1 / g), us, p)
+                else:
+                    c = 2 / (g + 1) * (cl + (g - 1) / 2 * (ul - si))
+                    u = 2 / (g + 1) * (cl + (g - 1) / 2 * ul + si)
+                    val = (rl * (c / cl) ** (2 / (g - 1)), u, pl * (c / cl) ** (2 * g / (g - 1)))
+        else:
+            if p > pr:   # right shock
+                sr = ur + cr * math.sqrt((g + 1) / (2 * g) * p / pr + (g - 1) / (2 * g))
+                val = (rr, ur, pr) if si >= sr else (rr * ((p / pr + (g - 1) / (g + 1)) / ((g - 1) / (g + 1) * p / pr + 1)), us, p)
+            else:        # right rarefaction
+                shr, str_ = ur + cr, us + cr * (p / pr) ** ((g - 1) / (2 * g))
+                if si >= shr:
+                    val = (rr, ur, pr)
+                elif si <= str_:
+                    val = (rr * (p / pr) ** (1 / g), us, p)
+                else:
+                    c = 2 / (g + 1) * (cr - (g - 1) / 2 * (ur - si))
+                    u = 2 / (g + 1) * (-cr + (g - 1) / 2 * ur + si)
+                    val = (rr * (c / cr) ** (2 / (g - 1)), u, pr * (c / cr) ** (2 * g / (g - 1)))
+        out[(slice(None),) + idx] = val
+    return out
+
+
 DMR_POST_ROW = [8, 33, -57.1576, 563.499, 8, 4.125, -7.1447, 116.5]
 
 
