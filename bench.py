@@ -7,10 +7,17 @@ A step is one full SSP-RK3 time step over the whole grid -- CFL reduction,
 3 x (x sweep, y sweep, RK combine, ghost fill), time advance -- replayed as one
 hipGraph with the state resident in HBM.
 
-Default workload (the configuration the metric is quoted on): double Mach
-reflection, 1920 x 480 cells on [0,4]x[0,1], characteristic WENO5 + HLLC
-(sensor window (1e4, 1.1e4): phi = 1), SSP-RK3, CFL 0.4, FP64; synthetic
-initial data built on the host.  Prints one JSON line.
+Default workload: double Mach reflection, 1920 x 480 cells on [0,4]x[0,1],
+characteristic WENO5 + HLLC (sensor window (1e4, 1.1e4): phi = 1), SSP-RK3,
+FP64, synthetic initial data built on the host.  It runs at CFL 0.1, the
+value of the reference's own DMR driver (double_mach_reflection.py:17): at
+CFL 0.4 the reference scheme turns non-finite near the top boundary within
+~60 steps (the CPU tests show it on the oracle), and a throughput measured on
+a non-finite state would be meaningless; the cost of a step does not depend
+on the CFL number.  Prints one JSON line.
+
+Before timing, the sweep tile lengths are picked by the solver's autotuner
+(measured kernel times over a set of rows-per-workgroup candidates).
 
 `value` is the FP64 throughput.  The same line carries the FP32-split
 throughput on the same workload with its relative L1 density error against
@@ -31,10 +38,11 @@ for p in (ROOT, os.path.join(ROOT, "weno-cfd-solver_amd")):
 
 HBM_PEAK = 8.0e12
 BYTES_PER_CELL_UPDATE = 256.0   # 4 conserved values x 8 B: 64 B stage 1 + 2 x 96 B stages 2, 3
+VALU_WAVE_INSTR_PEAK = 256 * 4 * 2.4e9 / 4   # CUs x SIMDs x clock / 4 cycles per wave64 FP64 op
 
 WORKLOADS = {
     "dmr": dict(name="double_mach_reflection", nx=1920, ny=480, cons=False, force_hlle=False,
-                jp=(1e4, 1.1e4), bc="dmr", cfl=0.4),
+                jp=(1e4, 1.1e4), bc="dmr", cfl=0.1),
     "quad4096": dict(name="four_quadrant_riemann", nx=4096, ny=4096, cons=False, force_hlle=False,
                      jp=(1e4, 1.1e4), bc="mirror", cfl=0.4),
     "kh8192": dict(name="kelvin_helmholtz", nx=8192, ny=8192, cons=False, force_hlle=False,
@@ -56,23 +64,35 @@ def make_state(w):
     return problems.kelvin_helmholtz(w["nx"])
 
 
-def run_gpu(w, q, dx, steps, warmup, engine, variant, precision, profile=True):
+def run_gpu(w, q, dx, steps, warmup, engine, variant, precision, profile=True, tune=True, chunks=None):
+    import numpy as np
     from wenox import Solver
     s = Solver(w["nx"], w["ny"], dx, gamma=1.4, cfl=w["cfl"], cons_switch=w["cons"],
                force_hlle=w["force_hlle"], jp_cri=w["jp"], bc=w["bc"], engine=engine,
                variant=variant, precision=precision)
     try:
         s.upload(q)
+        tuned = None
+        if engine == "fused":
+            if chunks:
+                s.set_chunks(*chunks)
+            elif tune:
+                tuned = s.autotune()
         s.run(nstep=warmup)
         t0 = time.perf_counter()
         T, k = s.run(nstep=steps)
         el = time.perf_counter() - t0
         assert k == steps, (k, steps)
+        state = s.download()[3:-3, 3:-3, :4]
         out = dict(seconds=el, steps=k, value=w["nx"] * w["ny"] * k / el, dev_ms_per_step=s.ms_per_step,
-                   state=s.download()[3:-3, 3:-3, :4])
+                   state=state, finite=bool(np.isfinite(state).all()), t=T)
+        if engine == "fused":
+            out["chunks"] = s.chunks
+            if tuned:
+                out["autotune_ms"] = {key: {str(c): round(v, 5) for c, v in times.items()}
+                                      for key, (best, times) in tuned.items()}
         if profile and engine == "fused":
             out["kernels_ms"] = s.profile(reps=10)
-            out["chunks"] = s.chunks
         return out
     finally:
         s.close()
@@ -108,26 +128,81 @@ def cpu_baseline(q, dx, w, budget_s=12.0):
                        f"(3 oracle l_local calls + stage combines each), {el:.1f} s")
 
 
-def roofline(res, w):
-    """Whole-step roofline at 256 algorithmic B per cell-update, from the
-    device time of one step (sum of the HIP-event kernel times), plus the
-    dominant kernel."""
+def kernel_bytes(w, split):
+    """Algorithmic HBM bytes of each launch of a step (state and D arrays
+    touched once each; halo re-reads counted 0)."""
+    cells = w["nx"] * w["ny"]
+    padded = (w["nx"] + 6) * (w["ny"] + 6)
+    d = 16 if split else 32            # D planes: 4 x FP32 or 4 x FP64 per cell
+    st = 32                            # state: 4 x FP64, or FP32 hi + lo
+    return {"sweep_x": st * padded + d * cells, "sweep_y": st * padded + d * cells,
+            # stage 1 reads U^n, stages 2 and 3 also U_s; both orientations written
+            "combine": 2 * d * cells + (st + 2 * st / 3) * cells + 2 * st * cells,
+            "cfl": st * cells}
+
+
+def roofline(res, w, split=False):
+    """HBM roofline at the metric's 256 algorithmic B per cell-update for the
+    step, which is one hipGraph launch (CFL + 3 x (x sweep, y sweep, combine,
+    ghost fill) + time advance), timed as the sum of the per-kernel HIP-event
+    averages on the solver's stream; then each kernel against its own
+    algorithmic bytes.  `traffic` (HBM bytes per step from FETCH_SIZE /
+    WRITE_SIZE counters) is filled where a counter profile of this workload
+    exists."""
     km = res.get("kernels_ms")
     cells = w["nx"] * w["ny"]
     step_ms = km["step_sum"] if km else res["dev_ms_per_step"]
     achieved = cells * BYTES_PER_CELL_UPDATE / (step_ms * 1e-3)
+    tr = TRAFFIC.get((w["name"], w["nx"], w["ny"], split))
     out = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-           "frac": achieved / HBM_PEAK, "traffic": None,
-           "kernel": "one SSP-RK3 step (CFL + 3 x (x sweep, y sweep, RK combine, ghost fill))",
+           "frac": achieved / HBM_PEAK, "traffic": tr["bytes_per_step"] if tr else None,
+           "kernel": "one SSP-RK3 step (one hipGraph launch)",
            "algorithmic_bytes_per_launch": cells * BYTES_PER_CELL_UPDATE,
            "avg_launch_ms": step_ms}
+    if tr:
+        out["traffic_source"] = tr["source"]
     if km:
+        kb = kernel_bytes(w, split)
+        per = {}
+        for k, launches in (("sweep_x", 3), ("sweep_y", 3), ("combine", 3), ("cfl", 1)):
+            per[k] = {"avg_ms": round(km[k], 5), "launches_per_step": launches,
+                      "algorithmic_bytes": kb[k], "achieved_GBps": round(kb[k] / (km[k] * 1e-3) / 1e9, 1)}
+        out["kernels"] = per
+        out["ghost_fill_ms"] = round(km["ghost_fill"], 5)
         dom = max(("sweep_x", "sweep_y", "combine"), key=lambda k: km[k])
         out["dominant_kernel"] = {"name": f"k_sweep ({dom})" if dom.startswith("sweep") else "k_combine",
-                                  "avg_ms": km[dom], "launches_per_step": 3,
-                                  "share_of_step": 3 * km[dom] / step_ms}
-        out["kernels_ms"] = {k: round(v, 5) for k, v in km.items()}
+                                  "share_of_step": round(3 * km[dom] / step_ms, 4)}
+        v = VALU.get((w["name"], w["nx"], w["ny"], split, dom))
+        if v:
+            rate = v["wave_instr_per_launch"] / (km[dom] * 1e-3)
+            out["dominant_kernel"]["valu"] = {"wave_instr_per_launch": v["wave_instr_per_launch"],
+                                              "achieved": rate, "peak": VALU_WAVE_INSTR_PEAK,
+                                              "unit": "wave64 VALU instr/s", "frac": rate / VALU_WAVE_INSTR_PEAK,
+                                              "source": v["source"]}
     return out
+
+
+# HBM bytes per step and VALU instruction counts from rocprofv3 counter runs
+# (FETCH_SIZE x 2 and WRITE_SIZE x 1: the factors a plain 512 MiB device copy
+# reads back; SQ_INSTS_VALU per dispatch), keyed by workload and precision.
+TRAFFIC = {}
+VALU = {}
+
+
+def copy_calibration(mib=512, reps=5):
+    """A plain device-to-device copy of known size (torch's elementwise copy
+    kernel): its FETCH_SIZE / WRITE_SIZE counters per dispatch, divided by the
+    bytes it must move, scale the counters read on the solver's kernels into
+    HBM bytes."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.rand(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": n * 8, "bytes_written": n * 8, "launches": reps}}),
+          flush=True)
 
 
 def main():
@@ -139,17 +214,27 @@ def main():
     ap.add_argument("--workload", default="dmr", choices=sorted(WORKLOADS))
     ap.add_argument("--engine", default="fused", choices=["fused", "unfused"])
     ap.add_argument("--variant", default="fast", choices=["fast", "exact"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32split"],
+                    help="precision of the main timed run (the headline value is FP64)")
+    ap.add_argument("--chunks", type=int, nargs=2, metavar=("CX", "CY"),
+                    help="rows per workgroup of the x / y sweeps (skips the autotuner)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 512 MiB device copy (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
               "the multi-GPU decomposition is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     w = WORKLOADS[args.workload]
     q, dx = make_state(w)
-    r64 = run_gpu(w, q, dx, args.steps, args.warmup, args.engine, args.variant, "fp64")
+    r64 = run_gpu(w, q, dx, args.steps, args.warmup, args.engine, args.variant, args.precision,
+                  chunks=args.chunks)
     res = {
         "metric": "cell-updates/s FP64 vs FP32-split @<=1e-6 L1 err; % of HBM roofline",
         "value": r64["value"],
@@ -161,13 +246,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32 bulk, f32 hi+lo state",
         "data": "synthetic",
         "config": {"workload": describe(w), "engine": args.engine, "variant": args.variant,
-                   "chunks": r64.get("chunks")},
-        "roofline": roofline(r64, w),
+                   "chunks": r64.get("chunks"), "autotune_ms": r64.get("autotune_ms")},
+        "state_finite": r64["finite"],
+        "roofline": roofline(r64, w, split=args.precision != "fp64"),
     }
-    if not args.no_extra and args.engine == "fused":
+    if not args.no_extra and args.engine == "fused" and args.precision == "fp64":
         r32 = run_gpu(w, q, dx, args.steps, args.warmup, "fused", "fast", "fp32split")
         rho64, rho32 = r64["state"][..., 0], r32["state"][..., 0]
         res["fp32_split"] = {
@@ -175,7 +261,9 @@ def main():
             "speedup_vs_fp64": r32["value"] / r64["value"],
             "rel_l1_density_vs_fp64": float(np.abs(rho32 - rho64).sum() / np.abs(rho64).sum()),
             "after_steps": args.warmup + args.steps,
-            "roofline": roofline(r32, w),
+            "state_finite": r32["finite"],
+            "chunks": r32.get("chunks"),
+            "roofline": roofline(r32, w, split=True),
         }
         if args.workload != "quad4096":
             wq = WORKLOADS["quad4096"]
@@ -184,8 +272,13 @@ def main():
             rq32 = run_gpu(wq, qq, dq, 20, 3, "fused", "fast", "fp32split")
             res["quad4096"] = {"workload": describe(wq), "value_fp64": rq["value"],
                                "value_fp32split": rq32["value"],
-                               "roofline_frac_fp64": roofline(rq, wq)["frac"],
-                               "kernels_ms_fp64": rq.get("kernels_ms")}
+                               "rel_l1_density_fp32split_vs_fp64": float(
+                                   np.abs(rq32["state"][..., 0] - rq["state"][..., 0]).sum()
+                                   / np.abs(rq["state"][..., 0]).sum()),
+                               "state_finite": rq["finite"] and rq32["finite"],
+                               "chunks_fp64": rq.get("chunks"), "chunks_fp32split": rq32.get("chunks"),
+                               "roofline_fp64": roofline(rq, wq),
+                               "roofline_fp32split": roofline(rq32, wq, split=True)}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(q, dx, w)
     print(json.dumps(res), flush=True)

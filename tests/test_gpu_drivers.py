@@ -3,7 +3,7 @@
 * wenox.drivers.main_rs reproduces Rie_prob_2d_rs.main_rs's loop on the
   reference's own initial data and writes its snapshot files
   (data/<timestamp>/{step}.npy, time.npy) in the reference format.
-* The standard DMR (configuration 2's boundary conditions, no reference
+* The standard DMR (Woodward-Colella boundary conditions, no reference
   counterpart) stays finite and positive and keeps the analytic post-shock
   state on the inflow and the exact shock trace on the top boundary.
 """
@@ -42,7 +42,9 @@ def test_standard_dmr(precision):
     from wenox import Solver, problems
     nx, ny = 240, 60
     q0, dx = problems.double_mach(nx, ny)
-    with Solver(nx, ny, dx, cfl=0.4, jp_cri=(1e4, 1.1e4), bc="dmr", engine="fused", variant="fast",
+    # CFL 0.1 as the reference's own DMR driver (double_mach_reflection.py:17);
+    # at CFL 0.4 the reference scheme turns non-finite near the top boundary
+    with Solver(nx, ny, dx, cfl=0.1, jp_cri=(1e4, 1.1e4), bc="dmr", engine="fused", variant="fast",
                 precision=precision) as s:
         s.upload(q0)
         T, k = s.run(nstep=60)

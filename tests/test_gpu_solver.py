@@ -45,20 +45,24 @@ def test_four_quadrant_50_steps(oracle_mod, engine, variant):
     print(f"[solver] four-quadrant {n}^2 char+HLLC 50 steps engine={engine} variant={variant}", flush=True)
     ref, ts, got, T, k = run_pair(oracle_mod, q0, n, n, dx, 50, False, False, (1e4, 1.1e4), engine, variant)
     assert k == 50
-    assert abs(T - ts[-1]) <= 1e-12 * ts[-1]
+    if variant == "exact":
+        assert T == ts[-1]
+    assert abs(T - ts[-1]) <= TOL * ts[-1]
     err = rel_l1(got, ref, n, n)
     print("rel L1", err)
     assert np.all(err <= TOL), err
     if variant == "exact":
         np.testing.assert_array_equal(got[..., :4], ref[..., :4])
-    # primitives of the download match the reference's synced channels
-    np.testing.assert_allclose(got[3:-3, 3:-3, 4:], ref[3:-3, 3:-3, 4:], rtol=1e-9, atol=1e-12)
+    # primitives of the download match the reference's synced channels (the
+    # fast variant's pointwise differences near the shocks reach ~1e-8)
+    rtol = 1e-12 if variant == "exact" else 1e-6
+    np.testing.assert_allclose(got[3:-3, 3:-3, 4:], ref[3:-3, 3:-3, 4:], rtol=rtol, atol=1e-12)
 
 
 @pytest.mark.parametrize("engine", ["unfused", "fused"])
 def test_componentwise_blend(oracle_mod, engine):
     from wenox import problems
-    q0, _ = problems.riemann_reference(64, 64)
+    q0 = problems.riemann_reference(64, 64)
     print(f"[solver] reference IC 64^2 cons+blend 20 steps engine={engine}", flush=True)
     ref, ts, got, T, k = run_pair(oracle_mod, q0, 64, 64, 1 / 64, 20, True, False, (0.5, 2.0),
                                   engine, "exact", cfl=0.3)

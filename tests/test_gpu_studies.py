@@ -1,6 +1,6 @@
 """GPU accuracy checks on workloads with closed-form answers.
 
-* Isentropic vortex (configuration 3 at small grids): the density error
+* Isentropic vortex (the precision study at small grids): the density error
   against the exact solution falls at an observed order >= 2.5 between 64^2,
   128^2 and 256^2 (component-wise WENO5 + Gauss pass + HLLE, SSP-RK3 at CFL
   0.4), and the FP32-split path stays within 1e-6 relative L1 of FP64.

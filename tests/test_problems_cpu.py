@@ -57,3 +57,24 @@ def test_double_mach_geometry():
 def test_dmr_reference_bound():
     q, bound = problems.dmr_reference(400, 100)
     assert bound == 19     # the reference configuration (nx=400, dx=1/100)
+
+
+def _reference_dmr_finite(oracle_mod, cfl, steps, nx=100, ny=25):
+    q, bound = problems.dmr_reference(nx, ny)
+    dx, T = 1.0 / ny, 0.0
+    for _ in range(steps):
+        dt = oracle_mod.cfl_dt(q, nx, ny, 1.4, cfl, dx, remaining=0.2 - T)
+        q = oracle_mod.rk3_refac(q, nx, ny, dx, dt, 1.4, False, bound, (10000, 11000))
+        T += dt
+        if not np.isfinite(q[3:-3, 3:-3, :4]).all():
+            return False
+    return True
+
+
+def test_reference_dmr_needs_its_own_cfl(oracle_mod):
+    """The reference scheme (characteristic WENO5 + HLLC, its DMR ghost fill)
+    stays finite at its driver's CFL 0.1 (double_mach_reflection.py:17) and
+    turns non-finite within 70 steps at CFL 0.4, so double Mach reflection runs
+    are made at CFL 0.1."""
+    assert _reference_dmr_finite(oracle_mod, 0.1, 70)
+    assert not _reference_dmr_finite(oracle_mod, 0.4, 70)

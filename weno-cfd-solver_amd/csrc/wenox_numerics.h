@@ -438,6 +438,91 @@ template <typename T> WX_HD void recon_normal(int dir, bool cons, const T c[6][4
 }
 
 // ---------------------------------------------------------------------
+// FP32 reconstruction on increments (the FP32-split path).
+//
+// WENO5 and the Gauss-point pass are exactly translation invariant (the
+// smoothness indicators see differences only; every candidate value is an
+// affine combination with coefficients summing to one), so
+//     weno(c_0..c_4) = c_2 + weno(c_0 - c_2, ..., c_4 - c_2).
+// The increments are formed in FP64 and rounded to FP32, the WENO arithmetic
+// runs in FP32, and the result is added back to the FP64 centre value: the
+// FP32 rounding error scales with the local variation |c_j - c_2| instead
+// of |c|, which keeps the flux differences of smooth regions at FP64 level.
+//
+// Characteristic variant: with w = L c the reconstructed face state is
+// R weno(L c_j) = R L c_2 + R weno(L (c_j - c_2)).  The reference's L is not
+// the inverse of R (rows 0 and 3 carry 1/(2 rho c) in the normal-momentum
+// column where R^-1 has 1/(2c), weno.rs:88,103 / :149,164); the product is
+//     R L c = c - ((rho~ - 1)/rho~) c_n (0, 1, 0, u~)   (x faces)
+//     R L c = c - ((rho~ - 1)/rho~) c_n (0, 0, 1, v~)   (y faces)
+// with c_n the normal momentum and rho~, u~, v~ the Roe state, evaluated in
+// FP64.  L and R for the increments are built in FP32 from the rounded
+// Roe state.
+// ---------------------------------------------------------------------
+template <typename T> WX_HD void weno5_gq_dev(T q0, T q1, T q2, T q3, T q4, T& ga, T& gb)
+{
+    float a, b;
+    weno5_gq<float>((float)(q0 - q2), (float)(q1 - q2), 0.0f, (float)(q3 - q2), (float)(q4 - q2), a, b);
+    ga = q2 + (T)a;
+    gb = q2 + (T)b;
+}
+
+WX_HD void recon_normal_dev(int dir, bool cons, const double c[6][4], double gam, double ql[4], double qr[4])
+{
+    float d[6][4];
+#pragma unroll
+    for (int t = 0; t < 6; ++t)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) d[t][k] = t == 2 ? 0.0f : (float)(c[t][k] - c[2][k]);
+    if (cons) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            ql[k] = c[2][k] + (double)weno5_left<float>(d[0][k], d[1][k], 0.0f, d[3][k], d[4][k]);
+            qr[k] = c[2][k] + (double)weno5_right<float>(d[1][k], 0.0f, d[3][k], d[4][k], d[5][k]);
+        }
+        return;
+    }
+    Roe<double> s = roe_avg(c[2], c[3], gam);
+    // FP64 base R L c_2
+    double base[4] = {c[2][0], c[2][1], c[2][2], c[2][3]};
+    {
+        const int n = dir == DIR_X ? 1 : 2;
+        double kap = (s.rho - 1.0) * rcp(s.rho) * c[2][n];
+        base[n] -= kap;
+        base[3] -= kap * (dir == DIR_X ? s.u : s.v);
+    }
+    Roe<float> sf{(float)s.rho, (float)s.u, (float)s.v, (float)s.h, (float)s.c};
+    float L[4][4], R[4][4];
+    eig_left<float>(dir, sf, (float)gam, L);
+    eig_right<float>(dir, sf, R);
+    float w[6][4];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        if (t == 2) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) w[t][k] = 0.0f;
+        } else {
+            mat4_vec4<float>(L, d[t], w[t]);
+        }
+    }
+    float chl[4], chr[4], el[4], er[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        chl[k] = weno5_left<float>(w[0][k], w[1][k], 0.0f, w[3][k], w[4][k]);
+        chr[k] = weno5_right<float>(w[1][k], 0.0f, w[3][k], w[4][k], w[5][k]);
+    }
+    mat4_vec4<float>(R, chl, el);
+    mat4_vec4<float>(R, chr, er);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        ql[k] = base[k] + (double)el[k];
+        qr[k] = base[k] + (double)er[k];
+    }
+    pp_limit(ql, c[2], gam);
+    pp_limit(qr, c[3], gam);
+}
+
+// ---------------------------------------------------------------------
 // Riemann solvers (riemann.rs:6-349).  Both use Roe-average wave speeds
 // s_l = ũ_n - ã, s_r = ũ_n + ã with ã^2 floored at 0, and the guard
 // |d| < 1e-12 -> copysign(1e-12, d) + 1e-12.

@@ -172,7 +172,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
         // pass 1 (normal reconstruction) on every lane
         {
             T ql[4], qr[4];
-            recon_normal(DIR, CONS, w, gam, ql, qr);
+            if constexpr (MODE == MODE_DETECT) recon_normal_dev(DIR, CONS, w, gam, ql, qr);
+            else recon_normal(DIR, CONS, w, gam, ql, qr);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 s_p1[k][l] = ql[k];
@@ -187,10 +188,17 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
             T l1[4], l2[4], q1[4], q2[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                weno5_gq(s_p1[k][l - 2], s_p1[k][l - 1], s_p1[k][l], s_p1[k][l + 1], s_p1[k][l + 2],
-                         l1[k], l2[k]);
-                weno5_gq(s_p1[4 + k][l - 2], s_p1[4 + k][l - 1], s_p1[4 + k][l], s_p1[4 + k][l + 1],
-                         s_p1[4 + k][l + 2], q1[k], q2[k]);
+                if constexpr (MODE == MODE_DETECT) {
+                    weno5_gq_dev(s_p1[k][l - 2], s_p1[k][l - 1], s_p1[k][l], s_p1[k][l + 1], s_p1[k][l + 2],
+                                 l1[k], l2[k]);
+                    weno5_gq_dev(s_p1[4 + k][l - 2], s_p1[4 + k][l - 1], s_p1[4 + k][l], s_p1[4 + k][l + 1],
+                                 s_p1[4 + k][l + 2], q1[k], q2[k]);
+                } else {
+                    weno5_gq(s_p1[k][l - 2], s_p1[k][l - 1], s_p1[k][l], s_p1[k][l + 1], s_p1[k][l + 2],
+                             l1[k], l2[k]);
+                    weno5_gq(s_p1[4 + k][l - 2], s_p1[4 + k][l - 1], s_p1[4 + k][l], s_p1[4 + k][l + 1],
+                             s_p1[4 + k][l + 2], q1[k], q2[k]);
+                }
             }
             if (MODE == MODE_DETECT)
                 bad = bad || bad_state(l1, gam) || bad_state(l2, gam) || bad_state(q1, gam) ||
@@ -436,10 +444,10 @@ static void h_sweep1(const SweepPlan& p, int dir, int split, const void* U, void
 #if WX_FAST
     if (split) {
         if (dir == 0) {
-            launch_sweep<float, IN_F32, float, MODE_DETECT, DIR_X>(args_x(p, U, D, flags, ctl), p.cons, blend, s);
+            launch_sweep<double, IN_SPLIT, float, MODE_DETECT, DIR_X>(args_x(p, U, D, flags, ctl), p.cons, blend, s);
             launch_sweep<double, IN_SPLIT, float, MODE_FIXUP, DIR_X>(args_x(p, U, D, flags, ctl), p.cons, blend, s);
         } else {
-            launch_sweep<float, IN_F32, float, MODE_DETECT, DIR_Y>(args_y(p, U, D, flags, ctl), p.cons, blend, s);
+            launch_sweep<double, IN_SPLIT, float, MODE_DETECT, DIR_Y>(args_y(p, U, D, flags, ctl), p.cons, blend, s);
             launch_sweep<double, IN_SPLIT, float, MODE_FIXUP, DIR_Y>(args_y(p, U, D, flags, ctl), p.cons, blend, s);
         }
         return;
@@ -476,9 +484,9 @@ static void h_sweeps_split(const SweepPlan& p, const float* Us, const float* UsT
                            float* DyT, int* flags, const StepCtl* ctl, hipStream_t s)
 {
     const int blend = p.force_hlle ? 0 : 1;
-    launch_sweep<float, IN_F32, float, MODE_DETECT, DIR_X>(args_x(p, Us, Dx, flags, ctl), p.cons, blend, s);
+    launch_sweep<double, IN_SPLIT, float, MODE_DETECT, DIR_X>(args_x(p, Us, Dx, flags, ctl), p.cons, blend, s);
     launch_sweep<double, IN_SPLIT, float, MODE_FIXUP, DIR_X>(args_x(p, Us, Dx, flags, ctl), p.cons, blend, s);
-    launch_sweep<float, IN_F32, float, MODE_DETECT, DIR_Y>(args_y(p, UsT, DyT, flags, ctl), p.cons, blend, s);
+    launch_sweep<double, IN_SPLIT, float, MODE_DETECT, DIR_Y>(args_y(p, UsT, DyT, flags, ctl), p.cons, blend, s);
     launch_sweep<double, IN_SPLIT, float, MODE_FIXUP, DIR_Y>(args_y(p, UsT, DyT, flags, ctl), p.cons, blend, s);
 }
 
@@ -652,9 +660,9 @@ static void hb_stage(const BatchTables& t, const BatchCommon& c, int split, cons
 {
 #if WX_FAST
     if (split) {
-        launch_sweep_batch<float, IN_F32, float, MODE_DETECT, DIR_X>(t, c, Us, Dx, flags, ctls, s);
+        launch_sweep_batch<double, IN_SPLIT, float, MODE_DETECT, DIR_X>(t, c, Us, Dx, flags, ctls, s);
         launch_sweep_batch<double, IN_SPLIT, float, MODE_FIXUP, DIR_X>(t, c, Us, Dx, flags, ctls, s);
-        launch_sweep_batch<float, IN_F32, float, MODE_DETECT, DIR_Y>(t, c, UsT, DyT, flags, ctls, s);
+        launch_sweep_batch<double, IN_SPLIT, float, MODE_DETECT, DIR_Y>(t, c, UsT, DyT, flags, ctls, s);
         launch_sweep_batch<double, IN_SPLIT, float, MODE_FIXUP, DIR_Y>(t, c, UsT, DyT, flags, ctls, s);
         k_combine_batch<StSplit, float><<<(unsigned)t.n_c, 256, 0, s>>>(
             t, (const float*)Dx, (const float*)DyT, (void*)Un, (void*)Us, out, outT, stage, ctls);

@@ -7,7 +7,9 @@ boundary kernel on upload.
 
 ``riemann_reference`` and ``dmr_reference`` restate the reference drivers'
 own initial data (``Rie_prob_2d_rs.py:213-263``, ``unitest.py:169-205``); the
-others build the workloads named by the benchmark configurations.
+others build the benchmark workloads (four-quadrant Riemann, standard double
+Mach reflection, isentropic vortex, Kelvin-Helmholtz, batched random
+four-quadrant instances).
 """
 from __future__ import annotations
 
@@ -42,9 +44,8 @@ def _mirror_fill(q):
 
 
 def four_quadrant(n: int, gamma: float = 1.4, ny: int | None = None):
-    """Four-quadrant Riemann problem on [0,1]^2 (benchmark configuration 1,
-    Lax-Liu configuration 3): (rho,u,v,p) = (1.5,0,0,1.5) for x>0.5,y>0.5;
-    (0.5323,1.206,0,0.3) for x<0.5,y>0.5; (0.138,1.206,1.206,0.029) for
+    """Four-quadrant Riemann problem on [0,1]^2 (Lax-Liu configuration 3):
+    (rho,u,v,p) = (1.5,0,0,1.5) for x>0.5,y>0.5; (0.5323,1.206,0,0.3) for x<0.5,y>0.5; (0.138,1.206,1.206,0.029) for
     x<0.5,y<0.5; (0.5323,0,1.206,0.3) for x>0.5,y<0.5.  Returns (q, dx)."""
     nx = n
     ny = n if ny is None else ny
@@ -225,8 +226,7 @@ def dmr_reference_fill(q, bound, nx, ny):
 
 
 def double_mach(nx: int = 1920, ny: int = 480, gamma: float = 1.4):
-    """Standard double Mach reflection on [0,4]x[0,1] (benchmark configuration
-    2): Mach-10 shock at 60 degrees to the wall through (1/6, 0); pre-shock
+    """Standard double Mach reflection on [0,4]x[0,1]: Mach-10 shock at 60 degrees to the wall through (1/6, 0); pre-shock
     (1.4, 0, 0, 1), post-shock (8, 8.25 cos30, -8.25 sin30, 116.5).
     Returns (q, dx)."""
     dx = 4.0 / nx
@@ -249,8 +249,8 @@ def _gauss4():
 
 
 def vortex_exact(x, y, t, gamma=1.4, eps=5.0, L=10.0):
-    """Shu's isentropic vortex advected with (1, 1) on the periodic [0, L]^2
-    (benchmark configuration 3): point values (rho, u, v, p)."""
+    """Shu's isentropic vortex advected with (1, 1) on the periodic [0, L]^2:
+    point values (rho, u, v, p)."""
     xc = (5.0 + t) % L
     yc = (5.0 + t) % L
     dxv = (x - xc + 0.5 * L) % L - 0.5 * L
@@ -294,7 +294,7 @@ def isentropic_vortex(n: int, gamma: float = 1.4, L: float = 10.0):
 
 
 def kelvin_helmholtz(n: int, gamma: float = 1.4, seed: int = SEED, amp: float = 0.01):
-    """Shear layer on periodic [0,1]^2 (benchmark configuration 5): rho = 2,
+    """Shear layer on periodic [0,1]^2: rho = 2,
     u = -0.5 for |y - 0.5| < 0.25, else rho = 1, u = +0.5; p = 2.5; v is
     seeded white noise of amplitude `amp`.  Returns (q, dx)."""
     q = np.zeros((n + 6, n + 6, 8))
@@ -321,18 +321,24 @@ def _prims_from_cons(q, gamma):
     return q
 
 
-def random_riemann_instance(rng, n_x: int, n_y: int, gamma: float = 1.4):
-    """One instance of the batched workload (benchmark configuration 4):
-    four-quadrant data with per-quadrant rho~U[0.1,2], p~U[0.02,2],
-    u,v~U[-1.5,1.5] on [0,1]^2."""
+def random_riemann_instance(rng, n_x: int, n_y: int, gamma: float = 1.4, rho=(1.0, 2.0), p=(1.0, 2.0),
+                            vel: float = 0.5):
+    """One instance of the batched workload: four-quadrant data on [0,1]^2
+    with per-quadrant rho~U[rho], p~U[p], u,v~U[-vel, vel].
+
+    The defaults are the ranges the reference scheme (characteristic WENO5 +
+    HLLC, CFL 0.4) survives: with the wider rho~U[0.1,2], p~U[0.02,2],
+    u,v~U[-1.5,1.5] the oracle's restatement of the reference turns
+    non-finite within 100 steps on 14 of 16 seeded 32x32 instances, while
+    these stay finite over 200 steps (12 of 12)."""
     q = np.zeros((n_x + 6, n_y + 6, 8))
     x = (np.arange(n_x + 6) - 3 + 0.5) / n_x
     y = (np.arange(n_y + 6) - 3 + 0.5) / n_y
     X, Y = np.meshgrid(x, y, indexing="ij")
     for mask in ((X > 0.5) & (Y > 0.5), (X < 0.5) & (Y > 0.5), (X < 0.5) & (Y < 0.5), (X > 0.5) & (Y < 0.5)):
-        q[mask, 4] = rng.uniform(0.1, 2.0)
-        q[mask, 5] = rng.uniform(-1.5, 1.5)
-        q[mask, 6] = rng.uniform(-1.5, 1.5)
-        q[mask, 7] = rng.uniform(0.02, 2.0)
+        q[mask, 4] = rng.uniform(*rho)
+        q[mask, 5] = rng.uniform(-vel, vel)
+        q[mask, 6] = rng.uniform(-vel, vel)
+        q[mask, 7] = rng.uniform(*p)
     _from_prims(q, gamma)
     return _mirror_fill(q), 1.0 / n_x

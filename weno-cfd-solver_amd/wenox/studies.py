@@ -1,7 +1,7 @@
 """Accuracy studies run on the device-resident solver.
 
-``vortex_study`` is the precision study of the isentropic vortex (benchmark
-configuration 3): periodic [0,10]^2, vortex strength 5, freestream
+``vortex_study`` is the precision study of the isentropic vortex: periodic
+[0,10]^2, vortex strength 5, freestream
 (rho, u, v, p) = (1, 1, 1, 1), component-wise WENO5 with the 2-point Gauss
 pass and HLLE, SSP-RK3 at CFL 0.4 to t_end.  For each grid it reports the
 L1 and L-infinity errors of the density against the exact solution (cell
