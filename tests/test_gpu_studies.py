@@ -29,7 +29,11 @@ def test_vortex_convergence_order():
 def test_periodic_conservation(precision):
     from wenox import Solver, problems
     q0, dx = problems.kelvin_helmholtz(96)
-    with Solver(96, 96, dx, cfl=0.4, jp_cri=(0.5, 2.0), bc="periodic", engine="fused",
+    # HLLC everywhere (phi = 1): with an active blend window the reference's
+    # one-sided sensor differences at the face-grid edges give the two copies
+    # of the periodic seam face different weights, and the seam stops
+    # telescoping
+    with Solver(96, 96, dx, cfl=0.4, jp_cri=(1e4, 1.1e4), bc="periodic", engine="fused",
                 variant="fast", precision=precision) as s:
         s.upload(q0)
         s.run(nstep=30)

@@ -177,6 +177,41 @@ def test_eigenvectors(oracle_mod, d):
 
 
 @pytest.mark.parametrize("d", [0, 1])
+def test_product_of_bases_closed_form(oracle_mod, d):
+    """R L q = q - ((rho~ - 1)/rho~) q_n (0, 1, 0, u~) for x faces and
+    (0, 0, 1, v~) for y faces (q_n the normal momentum): the deviation above
+    times R collapses to one vector because columns 0 and 3 of R differ only
+    in their normal-velocity and energy entries.  The FP32-split kernels use
+    this form for the FP64 part of the characteristic reconstruction."""
+    rng = np.random.default_rng(28 + d)
+    for ul, ur in _states(rng, 60):
+        roe, Lm, Rm = oracle_mod.roe_eig(d, ul, ur, G)
+        rho, u, v = roe[0], roe[1], roe[2]
+        q = rng.uniform(-3, 3, 4)
+        e = np.array([0.0, 1.0, 0.0, u]) if d == 0 else np.array([0.0, 0.0, 1.0, v])
+        np.testing.assert_allclose(Rm @ (Lm @ q), q - (rho - 1.0) / rho * q[1 + d] * e,
+                                   rtol=1e-12, atol=1e-13 * np.abs(q).max())
+
+
+@pytest.mark.parametrize("d", ["x", "y"])
+def test_uniform_flow_faces_are_product_of_bases(oracle_mod, d):
+    """On a uniform field the characteristic reconstruction returns R L q, not
+    q (the WENO weights sum to one in every characteristic component), so the
+    face states of a moving flow with rho != 1 are shifted by the closed form
+    while the flux differences still vanish."""
+    q = cons(1.7, 0.6, -0.4, 2.1)
+    u = np.broadcast_to(q, (14, 13, 4)).copy()
+    ql = getattr(oracle_mod, f"weno_{d}_py")(u, G, False)[0]
+    n = 1 if d == "x" else 2
+    un = q[n] / q[0]
+    e = np.array([0.0, 1.0, 0.0, un]) if d == "x" else np.array([0.0, 0.0, 1.0, un])
+    expect = q - (q[0] - 1.0) / q[0] * q[n] * e
+    np.testing.assert_allclose(ql, np.broadcast_to(expect, ql.shape), rtol=1e-13)
+    rhs = oracle_mod.l_local_py(u, 0.1, G, False, (0.2, 1.0), False)
+    assert np.abs(rhs).max() <= 1e-12 * np.abs(q).max() / 0.1
+
+
+@pytest.mark.parametrize("d", [0, 1])
 def test_roe_property(oracle_mod, d):
     """A(Roe)(U_R - U_L) = F(U_R) - F(U_L) with A = R diag(lambda) R^-1."""
     rng = np.random.default_rng(18 + d)
