@@ -75,7 +75,7 @@ def run_gpu(w, q, dx, steps, warmup, engine, variant, precision, profile=True, t
         tuned = None
         if engine == "fused":
             if chunks:
-                s.set_chunks(*chunks)
+                s.set_tiling(*chunks)
             elif tune:
                 tuned = s.autotune()
         s.run(nstep=warmup)
@@ -87,9 +87,9 @@ def run_gpu(w, q, dx, steps, warmup, engine, variant, precision, profile=True, t
         out = dict(seconds=el, steps=k, value=w["nx"] * w["ny"] * k / el, dev_ms_per_step=s.ms_per_step,
                    state=state, finite=bool(np.isfinite(state).all()), t=T)
         if engine == "fused":
-            out["chunks"] = s.chunks
+            out["chunks"] = list(s.tiling)
             if tuned:
-                out["autotune_ms"] = {key: {str(c): round(v, 5) for c, v in times.items()}
+                out["autotune_ms"] = {key: {f"w{w}c{c}": round(v, 5) for (w, c), v in times.items()}
                                       for key, (best, times) in tuned.items()}
         if profile and engine == "fused":
             out["kernels_ms"] = s.profile(reps=10)
@@ -216,8 +216,9 @@ def main():
     ap.add_argument("--variant", default="fast", choices=["fast", "exact"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32split"],
                     help="precision of the main timed run (the headline value is FP64)")
-    ap.add_argument("--chunks", type=int, nargs=2, metavar=("CX", "CY"),
-                    help="rows per workgroup of the x / y sweeps (skips the autotuner)")
+    ap.add_argument("--chunks", type=int, nargs=4, metavar=("CX", "CY", "WX", "WY"),
+                    help="sweep tiling: rows per wavefront and wavefronts per workgroup of the x / y "
+                         "sweeps (skips the autotuner)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -159,9 +159,14 @@ int wenox_solver_profile(wenox_solver *s, int reps, double *out_ms, int n_out);
  * (1e-14, 1e-16) by default as Rie_prob_2d_rs.py:311-319; the 1-D driver
  * Rie_prob_1D_rs.py:165-168 uses (1e-10, 0). */
 int wenox_solver_set_cfl_rule(wenox_solver *s, double rho_floor, double cmax_eps);
-/* Rows marched per workgroup by the x and y sweeps (the fused engine's tile length). */
+/* Tiling of the fused engine's sweeps: rows marched per wavefront (chunk) and
+ * wavefronts per workgroup along the normal (waves: 1, 2 or 4; consecutive
+ * chunks of one workgroup share their sensor halo through LDS, chunk >= 3
+ * then).  No reference counterpart (the reference has no device tiling). */
 int wenox_solver_plan(wenox_solver *s, int *chunk_x, int *chunk_y);
 int wenox_solver_set_chunks(wenox_solver *s, int chunk_x, int chunk_y);
+int wenox_solver_tiling(wenox_solver *s, int *chunk_x, int *chunk_y, int *waves_x, int *waves_y);
+int wenox_solver_set_tiling(wenox_solver *s, int chunk_x, int chunk_y, int waves_x, int waves_y);
 
 /* ---- 3. batched solver: many independent problems of uneven sizes ---------- */
 /* The n instances share the numerics of desc (nx, ny, dx of desc are ignored;

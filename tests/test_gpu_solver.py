@@ -154,3 +154,32 @@ def test_fp32_split_rhs(oracle_mod):
         got = s.rhs()
     scale = np.abs(ref).max()
     assert np.abs(got - ref).max() <= 1e-4 * scale
+
+
+@pytest.mark.parametrize("cons,force_hlle", [(False, False), (True, False), (False, True)])
+def test_sweep_tilings_bitwise(oracle_mod, cons, force_hlle):
+    """Every sweep tiling -- 1, 2 or 4 wavefronts per workgroup sharing the
+    sensor halo, any rows per wavefront, grids that are not a multiple of the
+    group -- gives the oracle's RHS bit for bit (exact variant)."""
+    from wenox import Solver, problems
+    nx, ny = 45, 37
+    q0, dx = problems.four_quadrant(nx, ny=ny)
+    ref = oracle_mod.l_local_py(q0, dx, 1.4, force_hlle, (0.5, 2.0), cons)
+    print(f"[solver] tilings 45x37 cons={cons} hlle={force_hlle}", flush=True)
+    with Solver(nx, ny, dx, cons_switch=cons, force_hlle=force_hlle, jp_cri=(0.5, 2.0), engine="fused") as s:
+        s.upload(q0)
+        for wx, cx in ((1, 8), (1, 45), (2, 3), (2, 7), (4, 3), (4, 4), (4, 11)):
+            for wy, cy in ((1, 5), (4, 3), (2, 10)):
+                s.set_tiling(cx, cy, wx, wy)
+                assert s.tiling == (cx, cy, wx, wy)
+                np.testing.assert_array_equal(s.rhs(), ref, err_msg=f"tiling {(cx, cy, wx, wy)}")
+
+
+def test_tiling_arguments_checked():
+    from wenox import Solver, problems
+    from wenox._lib import WenoxError
+    q0, dx = problems.four_quadrant(16)
+    with Solver(16, 16, dx, engine="fused") as s:
+        for bad in ((2, 8, 4, 1), (8, 8, 3, 1), (0, 8, 1, 1)):
+            with pytest.raises((ValueError, WenoxError)):
+                s.set_tiling(*bad)

@@ -885,13 +885,29 @@ extern "C" int wenox_solver_plan(wenox_solver* s, int* chunk_x, int* chunk_y)
     return WENOX_OK;
 }
 
-extern "C" int wenox_solver_set_chunks(wenox_solver* s, int chunk_x, int chunk_y)
+extern "C" int wenox_solver_tiling(wenox_solver* s, int* chunk_x, int* chunk_y, int* waves_x, int* waves_y)
 {
-    if (!s || chunk_x < 1 || chunk_y < 1) return fail(WENOX_E_ARG, "wenox_solver_set_chunks: bad argument");
-    if (s->d.engine != WENOX_ENGINE_FUSED) return fail(WENOX_E_ARG, "chunks apply to the fused engine");
+    if (!s) return fail(WENOX_E_ARG, "wenox_solver_tiling: null solver");
+    if (chunk_x) *chunk_x = s->plan.chunk_x;
+    if (chunk_y) *chunk_y = s->plan.chunk_y;
+    if (waves_x) *waves_x = s->plan.waves_x;
+    if (waves_y) *waves_y = s->plan.waves_y;
+    return WENOX_OK;
+}
+
+extern "C" int wenox_solver_set_tiling(wenox_solver* s, int chunk_x, int chunk_y, int waves_x, int waves_y)
+{
+    if (!s) return fail(WENOX_E_ARG, "wenox_solver_set_tiling: null solver");
+    auto ok_w = [](int w) { return w == 1 || w == 2 || w == 4; };
+    if (chunk_x < 1 || chunk_y < 1 || !ok_w(waves_x) || !ok_w(waves_y) || (waves_x > 1 && chunk_x < 3) ||
+        (waves_y > 1 && chunk_y < 3))
+        return fail(WENOX_E_ARG, "wenox_solver_set_tiling: chunk >= 1 (>= 3 with several waves), waves 1, 2 or 4");
+    if (s->d.engine != WENOX_ENGINE_FUSED) return fail(WENOX_E_ARG, "tiling applies to the fused engine");
     SweepPlan p = s->plan;
     p.chunk_x = chunk_x;
     p.chunk_y = chunk_y;
+    p.waves_x = waves_x;
+    p.waves_y = waves_y;
     if (s->split && sweep_flag_count(p) > sweep_flag_count(s->plan)) {
         int* f = nullptr;
         HIPCHK(hipMalloc(&f, (size_t)sweep_flag_count(p) * sizeof(int)));
@@ -903,4 +919,10 @@ extern "C" int wenox_solver_set_chunks(wenox_solver* s, int chunk_x, int chunk_y
     if (s->exec) { (void)hipGraphExecDestroy(s->exec); s->exec = nullptr; }
     if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
     return WENOX_OK;
+}
+
+extern "C" int wenox_solver_set_chunks(wenox_solver* s, int chunk_x, int chunk_y)
+{
+    if (!s) return fail(WENOX_E_ARG, "wenox_solver_set_chunks: null solver");
+    return wenox_solver_set_tiling(s, chunk_x, chunk_y, s->plan.waves_x, s->plan.waves_y);
 }

@@ -19,7 +19,8 @@ struct SweepPlan {
     int cons, force_hlle;
     double jp_lo, jp_hi;
     double two_h_x, two_h_y;   // 2 / max(min(F1, F2), 1) of the x and y face grids (riemann.rs:360-361)
-    int chunk_x, chunk_y;      // interior rows marched per workgroup
+    int chunk_x, chunk_y;      // interior rows marched per wavefront
+    int waves_x, waves_y;      // wavefronts per workgroup along the normal (1, 2 or 4)
 };
 
 struct SweepOps {

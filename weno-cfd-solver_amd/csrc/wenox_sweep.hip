@@ -63,7 +63,8 @@ struct SweepArgs {
     const void* U;        // padded planes, M x N each, normal = rows
     int64_t M, N;         // padded extents (nn + 6, nt + 6)
     int64_t nn, nt;       // interior cells along normal / transverse
-    int chunk;            // interior rows per workgroup
+    int chunk;            // interior rows per wavefront
+    int waves;            // wavefronts per workgroup (consecutive chunks)
     double gam, dx, two_h, lo, hi;
     void* D;              // interior planes nn x nt (DT)
     int* flags;           // one per workgroup (DETECT writes, FIXUP reads)
@@ -90,14 +91,47 @@ template <typename T> __device__ __forceinline__ bool bad_state(const T q[4], T 
     return !(p > T(0)) || !(p < T(3.0e38));
 }
 
-// One workgroup's chunk: column block bx, row chunk by; `bid` indexes the flags.
-template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND>
+// Lanes of one wavefront exchange values through LDS inside the march.  The
+// DS instructions of a wavefront execute in program order, so the only
+// requirement is that the compiler keeps the accesses on their side of this
+// point (the same three builtins as HIP's __syncwarp).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One workgroup: column block bx, row group by of W consecutive chunks, one
+// wavefront per chunk; `bid` indexes the flags (W per workgroup).
+//
+// Wave w owns the cells [r0, r1) of its chunk and computes the faces it owns,
+// [r0, r1) plus, for the group's last wave, face r1.  The sensor of a face
+// needs the p̄ of both normal neighbours, so the group's first wave also
+// computes the halo face r0 - 1 and its last wave the halo face r1 + 1 (the
+// reference's one-sided differences apply at the grid's first and last face
+// instead).  Inside the group the neighbours' p̄ come through LDS: a wave
+// defers the blend of its first face (it needs the previous wave's last p̄)
+// and of its last face (it needs the next wave's first p̄) to the end of the
+// march, and the flux differences of its first and last cell follow them.
+// Halo cost per group: 2 faces for W * chunk cells.  Needs chunk >= 3 when
+// W > 1 (the host enforces it).
+template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND, int W>
 __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64_t by, int64_t bid)
 {
     if (A.ctl && !A.ctl->active) return;
-    if (MODE == MODE_FIXUP && A.flags[bid] == 0) return;
-    __shared__ T s_p1[8][WAVE];     // pass-1 q_l (0..3), q_r (4..7) of the current face
-    __shared__ T s_pb[2][WAVE];     // p̄ ring (face parity)
+    if constexpr (MODE == MODE_FIXUP) {
+        bool any = false;
+#pragma unroll
+        for (int i = 0; i < W; ++i) any = any || A.flags[bid * W + i] != 0;
+        if (!any) return;
+    }
+    constexpr int XW = W > 1 ? W : 1;
+    __shared__ T s_p1[W][8][WAVE];     // pass-1 q_l (0..3), q_r (4..7) of the current face
+    __shared__ T s_pb[W][2][WAVE];     // p̄ ring (face parity)
+    __shared__ T s_xpb[XW][2][WAVE];   // p̄ of the first [0] and last [1] owned face
+    __shared__ T s_xf[XW][12][WAVE];   // deferred first face: HLLC (0..3), HLLE (4..7), F of the next face (8..11)
+    __shared__ T s_xF[XW][4][WAVE];    // blended flux of the first owned face
 
     const T gam = (T)A.gam, two_h = (T)A.two_h, jlo = (T)A.lo, jhi = (T)A.hi, dx = (T)A.dx;
 #if WX_FAST
@@ -109,7 +143,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
 #define WX_DIV_2H(x) ((x) / two_h)
 #define WX_DIV_DX(x) ((x) / dx)
 #endif
-    const int l = threadIdx.x;
+    // the wave index is uniform across the wave: keep it (and every row bound
+    // derived from it) in scalar registers
+    const int wv = W > 1 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE)) : 0;
+    const int l = (int)(threadIdx.x % WAVE);
     const int64_t c0 = bx * OUT_COLS;                       // padded column of lane 0
     const int64_t pc = c0 + l;                              // this lane's padded column
     const int64_t t = pc - 3;                               // interior transverse index
@@ -118,40 +155,44 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
     const bool out_lane = l >= 3 && l < WAVE - 3 && t >= 0 && t < A.nt;
     const int lm = l > 0 ? l - 1 : 0, lp = l < WAVE - 1 ? l + 1 : WAVE - 1;   // sensor neighbours
 
-    const int64_t r0 = by * A.chunk;
-    const int64_t r1 = r0 + A.chunk < A.nn ? r0 + A.chunk : A.nn;   // cells [r0, r1), faces [r0, r1]
-    const int64_t a0 = BLEND ? (r0 > 0 ? r0 - 1 : 0) : r0;
-    const int64_t a1 = BLEND ? (r1 < A.nn ? r1 + 1 : A.nn) : r1;     // inclusive
+    const int64_t r0 = (by * W + wv) * (int64_t)A.chunk;
+    const int64_t r1 = r0 + A.chunk < A.nn ? r0 + A.chunk : A.nn;   // cells [r0, r1)
+    const bool empty = r0 >= A.nn;
+    const bool last = wv == W - 1 || r1 == A.nn;                    // owns face r1
+    const bool dfirst = BLEND && wv > 0;                             // first face blended at the end
+    const int64_t fa = (BLEND && wv == 0 && r0 > 0) ? r0 - 1 : r0;
+    const int64_t fb = last ? ((BLEND && r1 < A.nn) ? r1 + 1 : r1) : r1 - 1;   // inclusive
     const int64_t P = A.M * A.N;
-    bool bad = false;
-
-    // stencil window rows a..a+5 (padded), 4 conserved components
-    T w[6][4];
-#pragma unroll
-    for (int s = 0; s < 5; ++s)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) w[s + 1][k] = ld_state<T, IN>(A.U, P, k, (a0 + s) * A.N + pcl);
-
-    T fc_prev[4], fe_prev[4], F_prev[4];
-    T pb_m2 = T(0), pb_m1 = T(0);    // p̄ of faces a-2, a-1 (this column)
-#pragma unroll
-    for (int k = 0; k < 4; ++k) { fc_prev[k] = T(0); fe_prev[k] = T(0); F_prev[k] = T(0); }
     const int64_t Pd = A.nn * A.nt;
     DT* Dout = (DT*)A.D;
+    bool bad = false;
 
-    auto finish_face = [&](int64_t f, const T F[4]) {
-        // cell f-1 lies between faces f-1 and f
-        if (out_lane && f - 1 >= r0 && f - 1 < r1) {
-            DT* d = Dout + (f - 1) * A.nt + t;
+    T F_prev[4], fc_prev[4], fe_prev[4];
+    T pb_m2 = T(0), pb_m1 = T(0), pb_r0n = T(0);   // p̄ of faces a-2, a-1; p̄(r0 + 1)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) d[k * Pd] = (DT)WX_DIV_DX(F[k] - F_prev[k]);
+    for (int k = 0; k < 4; ++k) { fc_prev[k] = T(0); fe_prev[k] = T(0); F_prev[k] = T(0); }
+
+    auto put_d = [&](int64_t cell, const T Fr[4], const T Fl[4]) {
+        if (out_lane) {
+            DT* d = Dout + cell * A.nt + t;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) d[k * Pd] = (DT)WX_DIV_DX(Fr[k] - Fl[k]);
+        }
+    };
+    // face f finished with flux F: cell f-1 lies between faces f-1 and f
+    auto finish_face = [&](int64_t f, const T F[4]) {
+        if (f - 1 >= r0 && f - 1 < r1) {
+            if (dfirst && f - 1 == r0) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) s_xf[wv][8 + k][l] = F[k];
+            } else {
+                put_d(f - 1, F, F_prev);
+            }
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) F_prev[k] = F[k];
     };
-
-    auto sensor = [&](int64_t f, T dn, T pbf) -> T {
-        const T* ring = s_pb[f & 1];
+    auto sensor = [&](const T* ring, T dn, T pbf) -> T {
         T dt_;
         if (t <= 0) dt_ = WX_DIV_2H(ring[lp] - ring[l]);
         else if (t >= A.nt - 1) dt_ = WX_DIV_2H(ring[l] - ring[lm]);
@@ -160,120 +201,183 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
         return DIR == DIR_X ? sensor_phi(dn, dt_, pbf, jlo, jhi) : sensor_phi(dt_, dn, pbf, jlo, jhi);
     };
 
-    for (int64_t a = a0; a <= a1; ++a) {
-        // slide the window and load row a+5
+    if (!empty) {
+        // stencil window rows a..a+5 (padded), 4 conserved components
+        T w[6][4];
 #pragma unroll
         for (int s = 0; s < 5; ++s)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) w[s][k] = w[s + 1][k];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) w[5][k] = ld_state<T, IN>(A.U, P, k, (a + 5) * A.N + pcl);
+            for (int k = 0; k < 4; ++k) w[s + 1][k] = ld_state<T, IN>(A.U, P, k, (fa + s) * A.N + pcl);
 
-        // pass 1 (normal reconstruction) on every lane
-        {
-            T ql[4], qr[4];
-            if constexpr (MODE == MODE_DETECT) recon_normal_dev(DIR, CONS, w, gam, ql, qr);
-            else recon_normal(DIR, CONS, w, gam, ql, qr);
+        for (int64_t a = fa; a <= fb; ++a) {
+            // slide the window and load row a+5
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                s_p1[k][l] = ql[k];
-                s_p1[4 + k][l] = qr[k];
-            }
-        }
-        __syncthreads();
+            for (int s = 0; s < 5; ++s)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) w[s][k] = w[s + 1][k];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) w[5][k] = ld_state<T, IN>(A.U, P, k, (a + 5) * A.N + pcl);
 
-        T fc[4], fe[4], pb = T(1);
-        if (gauss_lane) {
-            // pass 2: Gauss points along the face from lanes l-2..l+2
-            T l1[4], l2[4], q1[4], q2[4];
+            // pass 1 (normal reconstruction) on every lane
+            {
+                T ql[4], qr[4];
+                if constexpr (MODE == MODE_DETECT) recon_normal_dev(DIR, CONS, w, gam, ql, qr);
+                else recon_normal(DIR, CONS, w, gam, ql, qr);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if constexpr (MODE == MODE_DETECT) {
-                    weno5_gq_dev(s_p1[k][l - 2], s_p1[k][l - 1], s_p1[k][l], s_p1[k][l + 1], s_p1[k][l + 2],
-                                 l1[k], l2[k]);
-                    weno5_gq_dev(s_p1[4 + k][l - 2], s_p1[4 + k][l - 1], s_p1[4 + k][l], s_p1[4 + k][l + 1],
-                                 s_p1[4 + k][l + 2], q1[k], q2[k]);
-                } else {
-                    weno5_gq(s_p1[k][l - 2], s_p1[k][l - 1], s_p1[k][l], s_p1[k][l + 1], s_p1[k][l + 2],
-                             l1[k], l2[k]);
-                    weno5_gq(s_p1[4 + k][l - 2], s_p1[4 + k][l - 1], s_p1[4 + k][l], s_p1[4 + k][l + 1],
-                             s_p1[4 + k][l + 2], q1[k], q2[k]);
+                for (int k = 0; k < 4; ++k) {
+                    s_p1[wv][k][l] = ql[k];
+                    s_p1[wv][4 + k][l] = qr[k];
                 }
             }
-            if (MODE == MODE_DETECT)
-                bad = bad || bad_state(l1, gam) || bad_state(l2, gam) || bad_state(q1, gam) ||
-                      bad_state(q2, gam);
-            T fa[4], fb[4], fcc[4], fd[4];
-            flux_pt(DIR, l1, gam, fa);
-            flux_pt(DIR, l2, gam, fb);
-            flux_pt(DIR, q1, gam, fcc);
-            flux_pt(DIR, q2, gam, fd);
-            T QL[4], FL[4], QR[4], FR[4];
+            wave_sync();
+
+            T fc[4], fe[4], pb = T(1);
+            if (gauss_lane) {
+                // pass 2: Gauss points along the face from lanes l-2..l+2
+                const T(*sp)[WAVE] = s_p1[wv];
+                T l1[4], l2[4], q1[4], q2[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                FL[k] = T(0.5) * (fa[k] + fb[k]);
-                FR[k] = T(0.5) * (fd[k] + fcc[k]);
-                QL[k] = T(0.5) * (l1[k] + l2[k]);
-                QR[k] = T(0.5) * (q1[k] + q2[k]);
-            }
-            FaceState<T> st = face_speeds(DIR, QL, QR, gam);
-            hlle_from(st, FL, FR, fe);
-            if (BLEND) {
-                hllc_from(DIR, st, QL, FL, QR, FR, fc);
-                pb = T(0.5) * (pressure_floored(QL, gam) + pressure_floored(QR, gam));
-            }
-        } else {
+                for (int k = 0; k < 4; ++k) {
+                    if constexpr (MODE == MODE_DETECT) {
+                        weno5_gq_dev(sp[k][l - 2], sp[k][l - 1], sp[k][l], sp[k][l + 1], sp[k][l + 2], l1[k], l2[k]);
+                        weno5_gq_dev(sp[4 + k][l - 2], sp[4 + k][l - 1], sp[4 + k][l], sp[4 + k][l + 1],
+                                     sp[4 + k][l + 2], q1[k], q2[k]);
+                    } else {
+                        weno5_gq(sp[k][l - 2], sp[k][l - 1], sp[k][l], sp[k][l + 1], sp[k][l + 2], l1[k], l2[k]);
+                        weno5_gq(sp[4 + k][l - 2], sp[4 + k][l - 1], sp[4 + k][l], sp[4 + k][l + 1],
+                                 sp[4 + k][l + 2], q1[k], q2[k]);
+                    }
+                }
+                if (MODE == MODE_DETECT)
+                    bad = bad || bad_state(l1, gam) || bad_state(l2, gam) || bad_state(q1, gam) ||
+                          bad_state(q2, gam);
+                T fa_[4], fb_[4], fcc[4], fd[4];
+                flux_pt(DIR, l1, gam, fa_);
+                flux_pt(DIR, l2, gam, fb_);
+                flux_pt(DIR, q1, gam, fcc);
+                flux_pt(DIR, q2, gam, fd);
+                T QL[4], FL[4], QR[4], FR[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) { fc[k] = T(0); fe[k] = T(0); }
+                for (int k = 0; k < 4; ++k) {
+                    FL[k] = T(0.5) * (fa_[k] + fb_[k]);
+                    FR[k] = T(0.5) * (fd[k] + fcc[k]);
+                    QL[k] = T(0.5) * (l1[k] + l2[k]);
+                    QR[k] = T(0.5) * (q1[k] + q2[k]);
+                }
+                FaceState<T> st = face_speeds(DIR, QL, QR, gam);
+                hlle_from(st, FL, FR, fe);
+                if (BLEND) {
+                    hllc_from(DIR, st, QL, FL, QR, FR, fc);
+                    pb = T(0.5) * (pressure_floored(QL, gam) + pressure_floored(QR, gam));
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) { fc[k] = T(0); fe[k] = T(0); }
+            }
+
+            if (!BLEND) {
+                if (W > 1 && a == r0 && wv > 0) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) s_xF[wv][k][l] = fe[k];
+                }
+                finish_face(a, fe);
+            } else {
+                s_pb[wv][a & 1][l] = pb;
+                if (dfirst && a == r0) {
+                    s_xpb[wv][0][l] = pb;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) { s_xf[wv][k][l] = fc[k]; s_xf[wv][4 + k][l] = fe[k]; }
+                }
+                if (dfirst && a == r0 + 1) pb_r0n = pb;
+                wave_sync();
+                if (a > fa) {
+                    // sensor and blend of face f = a-1 (riemann.rs:360-407)
+                    const int64_t f = a - 1;
+                    if (!(dfirst && f == r0)) {
+                        T dn = f == 0 ? WX_DIV_2H(pb - pb_m1) : WX_DIV_2H(pb - pb_m2);
+                        T phi = sensor(s_pb[wv][f & 1], dn, pb_m1);
+                        T F[4];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) F[k] = phi * fc_prev[k] + (T(1) - phi) * fe_prev[k];
+                        if (f >= r0) finish_face(f, F);
+                    }
+                }
+                pb_m2 = pb_m1;
+                pb_m1 = pb;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) { fc_prev[k] = fc[k]; fe_prev[k] = fe[k]; }
+            }
+            wave_sync();
         }
 
-        if (!BLEND) {
-            finish_face(a, fe);
-        } else {
-            s_pb[a & 1][l] = pb;
-            __syncthreads();
-            if (a > a0) {
-                // sensor and blend of face f = a-1 (riemann.rs:360-407)
-                const int64_t f = a - 1;
-                T dn = f == 0 ? WX_DIV_2H(pb - pb_m1) : WX_DIV_2H(pb - pb_m2);
-                T phi = sensor(f, dn, pb_m1);
-                T F[4];
+        if (BLEND && fb == A.nn) {
+            // last face of the grid: one-sided normal difference, no face nn+1
+            const int64_t f = fb;
+            T dn = WX_DIV_2H(pb_m1 - pb_m2);
+            T phi = sensor(s_pb[wv][f & 1], dn, pb_m1);
+            T F[4];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) F[k] = phi * fc_prev[k] + (T(1) - phi) * fe_prev[k];
-                if (f >= r0) finish_face(f, F);
-            }
-            pb_m2 = pb_m1;
-            pb_m1 = pb;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) { fc_prev[k] = fc[k]; fe_prev[k] = fe[k]; }
+            for (int k = 0; k < 4; ++k) F[k] = phi * fc_prev[k] + (T(1) - phi) * fe_prev[k];
+            finish_face(f, F);
         }
-        __syncthreads();
+        if (BLEND && !last) s_xpb[wv][1][l] = pb_m1;     // p̄ of face r1 - 1
     }
 
-    if (BLEND && a1 == A.nn) {
-        // last face of the grid: one-sided normal difference, no face a1+1
-        const int64_t f = a1;
-        T dn = WX_DIV_2H(pb_m1 - pb_m2);
-        T phi = sensor(f, dn, pb_m1);
-        T F[4];
+    if constexpr (W > 1) {
+        __syncthreads();
+        T FLast[4];
+        if (!empty && BLEND) {
+            if (dfirst) {
+                // face r0: previous wave's last p̄ below, p̄(r0 + 1) above
+                T dn = WX_DIV_2H(pb_r0n - s_xpb[wv - 1][1][l]);
+                T pb0 = s_xpb[wv][0][l];
+                T phi = sensor(s_xpb[wv][0], dn, pb0);
+                T F0[4], F1[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) F[k] = phi * fc_prev[k] + (T(1) - phi) * fe_prev[k];
-        finish_face(f, F);
+                for (int k = 0; k < 4; ++k) {
+                    F0[k] = phi * s_xf[wv][k][l] + (T(1) - phi) * s_xf[wv][4 + k][l];
+                    F1[k] = s_xf[wv][8 + k][l];
+                    s_xF[wv][k][l] = F0[k];
+                }
+                put_d(r0, F1, F0);
+            }
+            if (!last) {
+                // face r1 - 1: p̄(r1 - 2) below, next wave's first p̄ above
+                T dn = WX_DIV_2H(s_xpb[wv + 1][0][l] - pb_m2);
+                T phi = sensor(s_pb[wv][(r1 - 1) & 1], dn, pb_m1);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) FLast[k] = phi * fc_prev[k] + (T(1) - phi) * fe_prev[k];
+                put_d(r1 - 2, FLast, F_prev);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) FLast[k] = F_prev[k];
+        }
+        __syncthreads();
+        if (!empty && !last) {
+            // cell r1 - 1 closes with the next wave's first face
+            T Fn[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Fn[k] = s_xF[wv + 1][k][l];
+            put_d(r1 - 1, Fn, FLast);
+        }
     }
     if (MODE == MODE_DETECT) {
         unsigned long long m = __ballot(bad ? 1 : 0);
-        if (l == 0) A.flags[bid] = m != 0ull ? 1 : 0;
+        if (l == 0) A.flags[bid * W + wv] = m != 0ull ? 1 : 0;
     }
 }
 
 #undef WX_DIV_2H
 #undef WX_DIV_DX
 
-template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND>
-__global__ void __launch_bounds__(WAVE) k_sweep(SweepArgs A)
+// Two wavefronts per SIMD (at most 256 VGPRs): the sweeps are VALU-bound and
+// need the second wave to cover LDS and memory latency.
+template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND, int W>
+__global__ void __launch_bounds__(WAVE * W) __attribute__((amdgpu_waves_per_eu(2))) k_sweep(SweepArgs A)
 {
-    sweep_body<T, IN, DT, MODE, DIR, CONS, BLEND>(A, blockIdx.x, blockIdx.y,
-                                                  (int64_t)blockIdx.y * gridDim.x + blockIdx.x);
+    sweep_body<T, IN, DT, MODE, DIR, CONS, BLEND, W>(A, blockIdx.x, blockIdx.y,
+                                                     (int64_t)blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // ------------------------------------------------------------------------
@@ -373,28 +477,37 @@ __global__ void __launch_bounds__(256) k_transpose(const E* in, E* out, int64_t 
     }
 }
 
+template <typename T, int IN, typename DT, int MODE, int DIR, int W>
+static void launch_sweep_w(const SweepArgs& a, int cons, int blend, hipStream_t s)
+{
+    const int64_t rows = (int64_t)W * a.chunk;
+    dim3 grid((unsigned)((a.nt + OUT_COLS - 1) / OUT_COLS), (unsigned)((a.nn + rows - 1) / rows));
+    if (cons) {
+        if (blend) k_sweep<T, IN, DT, MODE, DIR, true, true, W><<<grid, WAVE * W, 0, s>>>(a);
+        else k_sweep<T, IN, DT, MODE, DIR, true, false, W><<<grid, WAVE * W, 0, s>>>(a);
+    } else {
+        if (blend) k_sweep<T, IN, DT, MODE, DIR, false, true, W><<<grid, WAVE * W, 0, s>>>(a);
+        else k_sweep<T, IN, DT, MODE, DIR, false, false, W><<<grid, WAVE * W, 0, s>>>(a);
+    }
+}
+
 template <typename T, int IN, typename DT, int MODE, int DIR>
 static void launch_sweep(const SweepArgs& a, int cons, int blend, hipStream_t s)
 {
-    dim3 grid((unsigned)((a.nt + OUT_COLS - 1) / OUT_COLS), (unsigned)((a.nn + a.chunk - 1) / a.chunk));
-    if (cons) {
-        if (blend) k_sweep<T, IN, DT, MODE, DIR, true, true><<<grid, WAVE, 0, s>>>(a);
-        else k_sweep<T, IN, DT, MODE, DIR, true, false><<<grid, WAVE, 0, s>>>(a);
-    } else {
-        if (blend) k_sweep<T, IN, DT, MODE, DIR, false, true><<<grid, WAVE, 0, s>>>(a);
-        else k_sweep<T, IN, DT, MODE, DIR, false, false><<<grid, WAVE, 0, s>>>(a);
-    }
+    if (a.waves == 4) launch_sweep_w<T, IN, DT, MODE, DIR, 4>(a, cons, blend, s);
+    else if (a.waves == 2) launch_sweep_w<T, IN, DT, MODE, DIR, 2>(a, cons, blend, s);
+    else launch_sweep_w<T, IN, DT, MODE, DIR, 1>(a, cons, blend, s);
 }
 
 static SweepArgs args_x(const SweepPlan& p, const void* U, void* D, int* flags, const StepCtl* ctl)
 {
-    return SweepArgs{U, p.nx + 6, p.ny + 6, p.nx, p.ny, p.chunk_x, p.gamma, p.dx, p.two_h_x,
+    return SweepArgs{U, p.nx + 6, p.ny + 6, p.nx, p.ny, p.chunk_x, p.waves_x, p.gamma, p.dx, p.two_h_x,
                      p.jp_lo, p.jp_hi, D, flags, ctl};
 }
 
 static SweepArgs args_y(const SweepPlan& p, const void* UT, void* D, int* flags, const StepCtl* ctl)
 {
-    return SweepArgs{UT, p.ny + 6, p.nx + 6, p.ny, p.nx, p.chunk_y, p.gamma, p.dx, p.two_h_y,
+    return SweepArgs{UT, p.ny + 6, p.nx + 6, p.ny, p.nx, p.chunk_y, p.waves_y, p.gamma, p.dx, p.two_h_y,
                      p.jp_lo, p.jp_hi, D, flags, ctl};
 }
 
@@ -581,6 +694,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep_batch(BatchTables B, BatchCommon
     A.nn = DIR == DIR_X ? I.nx : I.ny;
     A.nt = DIR == DIR_X ? I.ny : I.nx;
     A.chunk = DIR == DIR_X ? I.chunk_x : I.chunk_y;
+    A.waves = 1;
     A.gam = C.gam;
     A.dx = I.dx;
     A.two_h = DIR == DIR_X ? I.two_h_x : I.two_h_y;
@@ -591,7 +705,7 @@ __global__ void __launch_bounds__(WAVE) k_sweep_batch(BatchTables B, BatchCommon
     A.ctl = &ctls[i];
     const int local = b - (DIR == DIR_X ? I.blk0_x : I.blk0_y);
     const int gx = DIR == DIR_X ? I.gx_x : I.gx_y;
-    sweep_body<T, IN, DT, MODE, DIR, CONS, BLEND>(A, local % gx, local / gx, b);
+    sweep_body<T, IN, DT, MODE, DIR, CONS, BLEND, 1>(A, local % gx, local / gx, b);
 }
 
 template <class St, typename DT>
@@ -721,16 +835,22 @@ const SweepOps WX_SWEEP_OPS = {WX_NS::h_stage, WX_NS::h_rhs, WX_NS::h_transpose,
                                nullptr,        nullptr,      nullptr,            nullptr, nullptr,
                                WX_NS::h_sweep1, WX_NS::h_combine1};
 
-// Chunk length: enough waves to cover the 1024 SIMDs twice where the grid
-// allows it, within [8, 64] rows (a chunk recomputes 2 halo faces for the
-// sensor, so longer is cheaper per cell).
-static int pick_chunk(int64_t nn, int64_t nt)
+// Default tiling: about two wavefronts per SIMD (2048 on the 1024 SIMDs).
+// Long grids march 8..64 rows per single-wave workgroup; where that would
+// leave fewer rows per wave than 24, four waves share one group of chunks
+// (two halo faces per group instead of per chunk).
+static void pick_tiling(int64_t nn, int64_t nt, int& chunk, int& waves)
 {
     int64_t cols = (nt + WX_NS::OUT_COLS - 1) / WX_NS::OUT_COLS;
-    int64_t c = nn * cols / 2048;
-    if (c < 8) c = 8;
-    if (c > 64) c = 64;
-    return (int)c;
+    int64_t r = nn * cols / 2048;
+    if (r >= 24) {
+        waves = 1;
+        chunk = (int)(r > 64 ? 64 : r);
+    } else {
+        waves = 4;
+        chunk = (int)(r < 4 ? 4 : r);
+        if (4 * chunk > nn) { waves = 1; chunk = nn < 8 ? 8 : (int)(nn < 64 ? nn : 64); }
+    }
 }
 
 SweepPlan make_sweep_plan(int64_t nx, int64_t ny, double dx, double gamma, int cons,
@@ -743,16 +863,17 @@ SweepPlan make_sweep_plan(int64_t nx, int64_t ny, double dx, double gamma, int c
     int64_t my = nx < (ny + 1) ? nx : (ny + 1);   // y-face grid (nx, ny+1)
     p.two_h_x = 2.0 * (1.0 / (double)(mx > 1 ? mx : 1));
     p.two_h_y = 2.0 * (1.0 / (double)(my > 1 ? my : 1));
-    p.chunk_x = pick_chunk(nx, ny);
-    p.chunk_y = pick_chunk(ny, nx);
+    pick_tiling(nx, ny, p.chunk_x, p.waves_x);
+    pick_tiling(ny, nx, p.chunk_y, p.waves_y);
     return p;
 }
 
 int64_t sweep_flag_count(const SweepPlan& p)
 {
     const int64_t c = WX_NS::OUT_COLS;
-    int64_t fx = ((p.ny + c - 1) / c) * ((p.nx + p.chunk_x - 1) / p.chunk_x);
-    int64_t fy = ((p.nx + c - 1) / c) * ((p.ny + p.chunk_y - 1) / p.chunk_y);
+    const int64_t rx = (int64_t)p.chunk_x * p.waves_x, ry = (int64_t)p.chunk_y * p.waves_y;
+    int64_t fx = ((p.ny + c - 1) / c) * ((p.nx + rx - 1) / rx) * p.waves_x;
+    int64_t fy = ((p.nx + c - 1) / c) * ((p.ny + ry - 1) / ry) * p.waves_y;
     return fx > fy ? fx : fy;
 }
 #endif

@@ -131,25 +131,48 @@ class Solver:
     def set_chunks(self, chunk_x: int, chunk_y: int):
         _lib.check(_lib.load().wenox_solver_set_chunks(self._h, int(chunk_x), int(chunk_y)))
 
+    @property
+    def tiling(self):
+        """(chunk_x, chunk_y, waves_x, waves_y): rows marched per wavefront and
+        wavefronts per workgroup of the x and y sweeps."""
+        v = [ctypes.c_int() for _ in range(4)]
+        _lib.check(_lib.load().wenox_solver_tiling(self._h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def set_tiling(self, chunk_x: int, chunk_y: int, waves_x: int, waves_y: int):
+        _lib.check(_lib.load().wenox_solver_set_tiling(self._h, int(chunk_x), int(chunk_y), int(waves_x),
+                                                        int(waves_y)))
+
     def set_cfl_rule(self, rho_floor: float = 1e-14, cmax_eps: float = 1e-16):
         """dt = CFL dx / (max(sqrt(g p / max(rho, rho_floor)) + |V|) + cmax_eps);
         set before upload() for the first step to follow it too."""
         _lib.check(_lib.load().wenox_solver_set_cfl_rule(self._h, float(rho_floor), float(cmax_eps)))
 
-    def autotune(self, candidates=(8, 12, 16, 24, 32, 48, 64), reps: int = 5) -> dict:  # noqa: D401
-        """Pick the x- and y-sweep tile lengths (rows marched per workgroup)
-        with the lowest measured kernel time on this device; keeps the best."""
+    TILINGS = ((1, 8), (1, 12), (1, 16), (1, 24), (1, 32), (1, 48), (1, 64),
+               (2, 6), (2, 8), (2, 12), (2, 16), (2, 24),
+               (4, 3), (4, 4), (4, 6), (4, 8), (4, 12), (4, 16))
+
+    def autotune(self, candidates=TILINGS, reps: int = 5) -> dict:
+        """Pick each sweep's tiling -- (wavefronts per workgroup, rows per
+        wavefront) -- with the lowest measured kernel time on this device
+        and keep it.  Returns {sweep: ((waves, chunk), {candidate: ms})}."""
         best = {}
         for d, key in ((0, "sweep_x"), (1, "sweep_y")):
             times = {}
-            cx0, cy0 = self.chunks
-            for c in candidates:
-                self.set_chunks(c if d == 0 else cx0, c if d == 1 else cy0)
-                times[c] = self.profile(reps)[key]
-            cbest = min(times, key=times.get)
-            cx0, cy0 = self.chunks
-            self.set_chunks(cbest if d == 0 else cx0, cbest if d == 1 else cy0)
-            best[key] = (cbest, times)
+            for w, c in candidates:
+                cx, cy, wx, wy = self.tiling
+                if d == 0:
+                    self.set_tiling(c, cy, w, wy)
+                else:
+                    self.set_tiling(cx, c, wx, w)
+                times[(w, c)] = self.profile(reps)[key]
+            wb, cb = min(times, key=times.get)
+            cx, cy, wx, wy = self.tiling
+            if d == 0:
+                self.set_tiling(cb, cy, wb, wy)
+            else:
+                self.set_tiling(cx, cb, wx, wb)
+            best[key] = ((wb, cb), times)
         return best
 
 
