@@ -128,6 +128,27 @@ def cpu_baseline(q, dx, w, budget_s=12.0):
                        f"(3 oracle l_local calls + stage combines each), {el:.1f} s")
 
 
+def dropin_rate(q, dx, w, calls=5):
+    """The drop-in path: weno_ext.l_local_py on the host array (pack, upload,
+    unfused kernels, download per call), fast and exact variants; cell
+    RHS evaluations per second including the PCIe copies."""
+    import weno_ext
+    out = {}
+    for fast in (True, False):
+        weno_ext.set_variant(fast)
+        weno_ext.l_local_py(q, dx, 1.4, w["force_hlle"], tuple(w["jp"]), w["cons"])
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            weno_ext.l_local_py(q, dx, 1.4, w["force_hlle"], tuple(w["jp"]), w["cons"])
+        el = (time.perf_counter() - t0) / calls
+        out["fast" if fast else "exact"] = {"ms_per_call": el * 1e3,
+                                            "cell_rhs_per_s": w["nx"] * w["ny"] / el}
+    weno_ext.set_variant(False)
+    out["note"] = ("weno_ext.l_local_py with host arrays: one RHS evaluation per call including the "
+                   "host->device and device->host copies (PCIe-inclusive), not a time step")
+    return out
+
+
 def kernel_bytes(w, split):
     """Algorithmic HBM bytes of each launch of a step (state and D arrays
     touched once each; halo re-reads counted 0)."""
@@ -280,6 +301,8 @@ def main():
                                "chunks_fp64": rq.get("chunks"), "chunks_fp32split": rq32.get("chunks"),
                                "roofline_fp64": roofline(rq, wq),
                                "roofline_fp32split": roofline(rq32, wq, split=True)}
+    if not args.no_extra and args.workload == "dmr":
+        res["dropin_l_local"] = dropin_rate(q, dx, w)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(q, dx, w)
     print(json.dumps(res), flush=True)

@@ -208,15 +208,18 @@ __device__ __forceinline__ void cfl_block(St U, int64_t nx, int64_t ny, double g
     __shared__ bool last;
     const int64_t n = ny + 6;
     double vmaxl = 0.0;
-    for (int64_t t = blk * ST_TB + threadIdx.x; t < nx * ny; t += nblk * ST_TB) {
-        int64_t i = t / ny + 3, j = t - (t / ny) * ny + 3;
-        int64_t o = i * n + j;
-        double q[4] = {U.ld(0, o), U.ld(1, o), U.ld(2, o), U.ld(3, o)};
-        Prim<double> r = con2primi_pt(q, gam);
-        double spd = sqrt(r.u * r.u + r.v * r.v);
-        double c = sqrt(gam * r.p / fmax(q[0], ctl->rho_floor));
-        double w = c + spd;
-        vmaxl = nan_max(vmaxl, w);
+    // rows blk, blk + nblk, ...; the block's threads stride along the row
+    for (int64_t i = blk; i < nx; i += nblk) {
+        const int64_t o0 = (i + 3) * n + 3;
+        for (int64_t j = threadIdx.x; j < ny; j += ST_TB) {
+            const int64_t o = o0 + j;
+            double q[4] = {U.ld(0, o), U.ld(1, o), U.ld(2, o), U.ld(3, o)};
+            Prim<double> r = con2primi_pt(q, gam);
+            double spd = sqrt(r.u * r.u + r.v * r.v);
+            double c = sqrt(gam * r.p / fmax(q[0], ctl->rho_floor));
+            double w = c + spd;
+            vmaxl = nan_max(vmaxl, w);
+        }
     }
     vmaxl = wave_max(vmaxl);
     int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -270,8 +273,7 @@ static void launch_bc(St U, St UT, bool has_t, int64_t nx, int64_t ny, BcParams 
 template <class St>
 static void launch_cfl(St U, int64_t nx, int64_t ny, double gam, StepCtl* ctl, hipStream_t s)
 {
-    int64_t nb = (nx * ny + ST_TB - 1) / ST_TB;
-    if (nb > CFL_BLOCKS) nb = CFL_BLOCKS;
+    int64_t nb = nx < CFL_BLOCKS ? nx : CFL_BLOCKS;   // whole rows per block
     if (nb < 1) nb = 1;
     k_cfl<St><<<(unsigned)nb, ST_TB, 0, s>>>(U, nx, ny, gam, ctl);
 }

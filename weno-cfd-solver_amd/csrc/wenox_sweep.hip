@@ -203,20 +203,26 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
 
     if (!empty) {
         // stencil window rows a..a+5 (padded), 4 conserved components
-        T w[6][4];
+        T w[6][4], nxt[4];
 #pragma unroll
         for (int s = 0; s < 5; ++s)
 #pragma unroll
             for (int k = 0; k < 4; ++k) w[s + 1][k] = ld_state<T, IN>(A.U, P, k, (fa + s) * A.N + pcl);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) nxt[k] = ld_state<T, IN>(A.U, P, k, (fa + 5) * A.N + pcl);
 
         for (int64_t a = fa; a <= fb; ++a) {
-            // slide the window and load row a+5
+            // slide the window; row a+5 was fetched one face ahead, fetch row a+6
 #pragma unroll
             for (int s = 0; s < 5; ++s)
 #pragma unroll
                 for (int k = 0; k < 4; ++k) w[s][k] = w[s + 1][k];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) w[5][k] = ld_state<T, IN>(A.U, P, k, (a + 5) * A.N + pcl);
+            for (int k = 0; k < 4; ++k) w[5][k] = nxt[k];
+            if (a < fb) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) nxt[k] = ld_state<T, IN>(A.U, P, k, (a + 6) * A.N + pcl);
+            }
 
             // pass 1 (normal reconstruction) on every lane
             {
@@ -387,53 +393,65 @@ __global__ void __launch_bounds__(WAVE * W) __attribute__((amdgpu_waves_per_eu(2
 // ------------------------------------------------------------------------
 constexpr int TT = 32;
 
-// One 32 x 32 tile: tile row ti (x), tile column tj (y).
+// One 32 x 32 tile: tile row ti (x), tile column tj (y).  The four
+// components' Dy^T tiles are staged in LDS together, so each thread has all
+// loads of its cells in flight at once and the tile needs three barriers.
 template <class St, typename DT>
 __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT* __restrict__ DyT, St Un,
                                              St Us, St out, St outT, int64_t nx, int64_t ny, int stage,
                                              const StepCtl* ctl, int64_t ti, int64_t tj)
 {
     if (ctl && !ctl->active) return;
-    __shared__ double tile[TT][TT + 1];
+    __shared__ double tile[4][TT][TT + 1];
     const int tx = threadIdx.x & (TT - 1);   // fast index
     const int ty = threadIdx.x / TT;         // 0..7
     const int64_t i0 = ti * TT, j0 = tj * TT;
     const int64_t m = nx + 6, n = ny + 6, PI = nx * ny;
     const double dt = ctl ? ctl->dt : 0.0;
-    for (int k = 0; k < 4; ++k) {
-        // Dy^T[k][j][i]: rows j, contiguous i
+    // Dy^T[k][j][i]: rows j, contiguous i
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
         for (int r = ty; r < TT; r += 8) {
-            int64_t j = j0 + r, i = i0 + tx;
-            if (j < ny && i < nx) tile[r][tx] = (double)DyT[k * PI + j * nx + i];
+            const int64_t j = j0 + r, i = i0 + tx;
+            if (j < ny && i < nx) tile[k][r][tx] = (double)DyT[k * PI + j * nx + i];
         }
-        __syncthreads();
-        double res[TT / 8];
-        for (int r = ty, q = 0; r < TT; r += 8, ++q) {
-            int64_t i = i0 + r, j = j0 + tx;
-            res[q] = 0.0;
-            if (i < nx && j < ny) {
-                double dgdy = tile[tx][r];
-                double l = -((double)Dx[k * PI + i * ny + j] + dgdy);
-                int64_t o = (i + 3) * n + (j + 3);
-                double qn = Un.ld(k, o);
+    __syncthreads();
+    double res[4][TT / 8];
+#pragma unroll
+    for (int r = ty, q = 0; r < TT; r += 8, ++q) {
+        const int64_t i = i0 + r, j = j0 + tx;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) res[k][q] = 0.0;
+        if (i < nx && j < ny) {
+            const int64_t o = (i + 3) * n + (j + 3);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const double l = -((double)Dx[k * PI + i * ny + j] + tile[k][tx][r]);
+                const double qn = Un.ld(k, o);
                 double v;
                 if (stage == 1) v = qn + dt * l;
                 else if (stage == 2) v = 0.75 * qn + 0.25 * Us.ld(k, o) + (0.25 * dt) * l;
                 else v = (1.0 / 3.0) * qn + (2.0 / 3.0) * Us.ld(k, o) + ((2.0 / 3.0) * dt) * l;
                 out.st(k, o, v);
-                res[q] = v;
+                res[k][q] = v;
             }
         }
-        __syncthreads();
-        for (int r = ty, q = 0; r < TT; r += 8, ++q) tile[r][tx] = res[q];
-        __syncthreads();
-        // outT[k][j+3][i+3], contiguous i
-        for (int r = ty; r < TT; r += 8) {
-            int64_t j = j0 + r, i = i0 + tx;
-            if (j < ny && i < nx) outT.st(k, (j + 3) * m + (i + 3), tile[tx][r]);
-        }
-        __syncthreads();
     }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int r = ty, q = 0; r < TT; r += 8, ++q) tile[k][r][tx] = res[k][q];
+    __syncthreads();
+    // outT[k][j+3][i+3], contiguous i
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int r = ty; r < TT; r += 8) {
+            const int64_t j = j0 + r, i = i0 + tx;
+            if (j < ny && i < nx) outT.st(k, (j + 3) * m + (i + 3), tile[k][tx][r]);
+        }
 }
 
 template <class St, typename DT>
