@@ -426,6 +426,7 @@ struct wenox_solver {
     double* work = nullptr;  // unfused face arrays, download scratch
     size_t work_n = 0;
     StepCtl* ctl = nullptr;  // device
+    double* cfl_part = nullptr;   // device, CFL_SLOTS per-block maxima
     BcParams bp{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -492,7 +493,9 @@ int read_ctl(wenox_solver* s, StepCtl* h)
 
 int write_ctl(wenox_solver* s, const StepCtl* h)
 {
-    HIPCHK(hipMemcpyAsync(s->ctl, h, sizeof(StepCtl), hipMemcpyHostToDevice, s->stream));
+    StepCtl c = *h;
+    c.part = s->cfl_part;   // per-block maxima of the CFL reduction
+    HIPCHK(hipMemcpyAsync(s->ctl, &c, sizeof(StepCtl), hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     return WENOX_OK;
 }
@@ -581,7 +584,8 @@ extern "C" int wenox_solver_create(const wenox_solver_desc* desc, wenox_solver**
               hipMalloc(&s->B, P * 8) == hipSuccess &&
               hipMalloc(&s->L, std::max<size_t>(I, 4) * 8) == hipSuccess &&
               hipMalloc(&s->work, s->work_n * 8) == hipSuccess &&
-              hipMalloc(&s->ctl, sizeof(StepCtl)) == hipSuccess;
+              hipMalloc(&s->ctl, sizeof(StepCtl)) == hipSuccess &&
+              hipMalloc(&s->cfl_part, CFL_SLOTS * sizeof(double)) == hipSuccess;
     if (ok && fused)
         ok = hipMalloc(&s->UT, P * 8) == hipSuccess && hipMalloc(&s->AT, P * 8) == hipSuccess &&
              hipMalloc(&s->BT, P * 8) == hipSuccess && hipMalloc(&s->DyT, std::max<size_t>(I, 4) * 8) == hipSuccess;
@@ -616,6 +620,7 @@ extern "C" void wenox_solver_destroy(wenox_solver* s)
     for (double* p : {s->U, s->A, s->B, s->UT, s->AT, s->BT, s->L, s->DyT, s->work})
         if (p) (void)hipFree(p);
     if (s->ctl) (void)hipFree(s->ctl);
+    if (s->cfl_part) (void)hipFree(s->cfl_part);
     if (s->flags) (void)hipFree(s->flags);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);

@@ -14,6 +14,9 @@
 namespace wenox {
 
 // Device-resident time control of one solve (read by every step kernel).
+// Slots of the CFL reduction's per-block maxima (StepCtl::part).
+constexpr int CFL_SLOTS = 1024;
+
 struct StepCtl {
     double T;              // simulated time
     double dt;             // dt of the current step
@@ -29,6 +32,7 @@ struct StepCtl {
     unsigned long long cmax_bits;   // running max of c+|V| (bit pattern, >= 0)
     unsigned int blocks_done;       // last-block detection of the reduction
     int pad;
+    double* part;                   // per-block maxima of the CFL reduction (nullptr: atomics)
 };
 
 enum BcKind : int {

@@ -172,11 +172,13 @@ __device__ __forceinline__ void bc_cell(St U, St UT, bool has_t, int64_t nx, int
 // Single-pass CFL reduction (Rie_prob_2d_rs.py:311-319):
 //   c_max = max over the interior of sqrt(g p / max(rho, 1e-14)) + sqrt(u^2+v^2)
 // with (rho, u, v, p) the synced primitives of con2primi (utils.rs:9-35).
-// Blocks reduce with wave shuffles, publish with one atomicMax on the bit
-// pattern (a non-negative double orders like its bits), and the last block
-// to finish turns the max into dt and re-arms the accumulator.
+// Blocks reduce with wave shuffles and store one value each; a one-wave
+// kernel takes the maximum and forms dt.  (The batched engine's few blocks per
+// instance instead publish with atomicMax on the bit pattern -- a
+// non-negative double orders like its bits -- and the last block finishes.)
+// max is exact and order-free, so the result does not depend on the split.
 // ------------------------------------------------------------------------
-constexpr int CFL_BLOCKS = 1024;
+constexpr int CFL_BLOCKS = CFL_SLOTS;
 
 // max that propagates NaN like numpy.max
 __device__ __forceinline__ double nan_max(double a, double b)
@@ -198,8 +200,31 @@ __global__ void k_bc(St U, St UT, bool has_t, int64_t nx, int64_t ny, BcParams b
     bc_cell<St>(U, UT, has_t, nx, ny, bp, ctl, stage, (int64_t)blockIdx.x * ST_TB + threadIdx.x);
 }
 
-// Block `blk` of `nblk` blocks reducing one state's interior; the last block
-// to finish writes dt into ctl.
+// dt and the loop condition from the step's wave-speed maximum
+// (Rie_prob_2d_rs.py:335-336 loop condition, then :315-318).
+__device__ __forceinline__ void cfl_finish(StepCtl* ctl, double cmax_state)
+{
+    double cmax = ctl->use_cmax0 ? ctl->cmax0 : cmax_state;
+    ctl->use_cmax0 = 0;
+    ctl->cmax_bits = 0ull;
+    ctl->blocks_done = 0u;
+    double T = ctl->T;
+    bool run = ctl->step < ctl->max_steps;
+    if (ctl->t_final >= 0.0) run = run && (T < ctl->t_final - 1e-12);
+    ctl->active = run ? 1 : 0;
+    double c_max = cmax + ctl->cmax_eps;
+    double dt = ctl->cfl_dx / c_max;
+    if (ctl->t_final >= 0.0) {
+        double rem = ctl->t_final - T;
+        dt = rem < dt ? rem : dt;
+    }
+    ctl->dt = dt;
+}
+
+// Block `blk` of `nblk` blocks reducing one state's interior.  With a
+// per-block array (ctl->part) the block stores its maximum there and
+// k_cfl_dt finishes; without one (the batched engine, a few blocks per
+// instance) blocks combine with atomics and the last one finishes.
 template <class St>
 __device__ __forceinline__ void cfl_block(St U, int64_t nx, int64_t ny, double gam, StepCtl* ctl,
                                           int64_t blk, int64_t nblk)
@@ -225,34 +250,25 @@ __device__ __forceinline__ void cfl_block(St U, int64_t nx, int64_t ny, double g
     int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (lane == 0) part[wv] = vmaxl;
     __syncthreads();
+    double* const slots = ctl->part;
     if (threadIdx.x == 0) {
         double b = part[0];
         for (int w = 1; w < ST_TB / 64; ++w) b = nan_max(b, part[w]);
-        atomicMax(&ctl->cmax_bits, (unsigned long long)__double_as_longlong(b));
-        __threadfence();
-        unsigned prev = atomicAdd(&ctl->blocks_done, 1u);
-        last = (prev == (unsigned)(nblk - 1));
+        if (slots) {
+            slots[blk] = b;
+        } else {
+            atomicMax(&ctl->cmax_bits, (unsigned long long)__double_as_longlong(b));
+            __threadfence();
+            unsigned prev = atomicAdd(&ctl->blocks_done, 1u);
+            last = (prev == (unsigned)(nblk - 1));
+        }
     }
+    if (slots) return;
     __syncthreads();
     if (!last || threadIdx.x != 0) return;
     __threadfence();
     unsigned long long bits = atomicAdd(&ctl->cmax_bits, 0ull);
-    double cmax = ctl->use_cmax0 ? ctl->cmax0 : __longlong_as_double((long long)bits);
-    ctl->use_cmax0 = 0;
-    ctl->cmax_bits = 0ull;
-    ctl->blocks_done = 0u;
-    // Rie_prob_2d_rs.py:335-336 loop condition, then :315-318
-    double T = ctl->T;
-    bool run = ctl->step < ctl->max_steps;
-    if (ctl->t_final >= 0.0) run = run && (T < ctl->t_final - 1e-12);
-    ctl->active = run ? 1 : 0;
-    double c_max = cmax + ctl->cmax_eps;
-    double dt = ctl->cfl_dx / c_max;
-    if (ctl->t_final >= 0.0) {
-        double rem = ctl->t_final - T;
-        dt = rem < dt ? rem : dt;
-    }
-    ctl->dt = dt;
+    cfl_finish(ctl, __longlong_as_double((long long)bits));
 }
 
 template <class St>
@@ -261,6 +277,14 @@ __global__ void __launch_bounds__(ST_TB) k_cfl(St U, int64_t nx, int64_t ny, dou
     cfl_block<St>(U, nx, ny, gam, ctl, blockIdx.x, gridDim.x);
 }
 
+// One wavefront: maximum of the nblk per-block values, then dt.
+__global__ void __launch_bounds__(64) k_cfl_dt(StepCtl* ctl, int nblk)
+{
+    double v = 0.0;
+    for (int b = (int)threadIdx.x; b < nblk; b += 64) v = nan_max(v, ctl->part[b]);
+    v = wave_max(v);
+    if (threadIdx.x == 0) cfl_finish(ctl, v);
+}
 
 template <class St>
 static void launch_bc(St U, St UT, bool has_t, int64_t nx, int64_t ny, BcParams bp, const StepCtl* ctl,
@@ -273,7 +297,9 @@ static void launch_bc(St U, St UT, bool has_t, int64_t nx, int64_t ny, BcParams 
 template <class St>
 static void launch_cfl(St U, int64_t nx, int64_t ny, double gam, StepCtl* ctl, hipStream_t s)
 {
-    int64_t nb = nx < CFL_BLOCKS ? nx : CFL_BLOCKS;   // whole rows per block
+    // whole rows per block; ctl->part holds CFL_BLOCKS values
+    int64_t nb = nx < CFL_BLOCKS ? nx : CFL_BLOCKS;
     if (nb < 1) nb = 1;
     k_cfl<St><<<(unsigned)nb, ST_TB, 0, s>>>(U, nx, ny, gam, ctl);
+    k_cfl_dt<<<1, 64, 0, s>>>(ctl, (int)nb);
 }
