@@ -174,7 +174,8 @@ def roofline(res, w, split=False):
     cells = w["nx"] * w["ny"]
     step_ms = km["step_sum"] if km else res["dev_ms_per_step"]
     achieved = cells * BYTES_PER_CELL_UPDATE / (step_ms * 1e-3)
-    tr = TRAFFIC.get((w["name"], w["nx"], w["ny"], split))
+    tiling = tuple(res.get("chunks") or ())
+    tr = TRAFFIC.get((w["name"], w["nx"], w["ny"], split, tiling))
     out = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
            "frac": achieved / HBM_PEAK, "traffic": tr["bytes_per_step"] if tr else None,
            "kernel": "one SSP-RK3 step (one hipGraph launch)",
@@ -193,7 +194,7 @@ def roofline(res, w, split=False):
         dom = max(("sweep_x", "sweep_y", "combine"), key=lambda k: km[k])
         out["dominant_kernel"] = {"name": f"k_sweep ({dom})" if dom.startswith("sweep") else "k_combine",
                                   "share_of_step": round(3 * km[dom] / step_ms, 4)}
-        v = VALU.get((w["name"], w["nx"], w["ny"], split, dom))
+        v = VALU.get((w["name"], w["nx"], w["ny"], split, tiling, dom))
         if v:
             rate = v["wave_instr_per_launch"] / (km[dom] * 1e-3)
             out["dominant_kernel"]["valu"] = {"wave_instr_per_launch": v["wave_instr_per_launch"],
@@ -203,11 +204,27 @@ def roofline(res, w, split=False):
     return out
 
 
-# HBM bytes per step and VALU instruction counts from rocprofv3 counter runs
-# (FETCH_SIZE x 2 and WRITE_SIZE x 1: the factors a plain 512 MiB device copy
-# reads back; SQ_INSTS_VALU per dispatch), keyed by workload and precision.
-TRAFFIC = {}
-VALU = {}
+# HBM bytes per step and VALU instructions per sweep launch from rocprofv3
+# counter runs (FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB: the factors a plain
+# 512 MiB device copy reads back; SQ_INSTS_VALU per dispatch), keyed by
+# workload, precision and sweep tiling (chunk_x, chunk_y, waves_x, waves_y):
+# the halo re-reads depend on the tiling, so other tilings report null.
+_DMR_SRC64 = "profiles/dmr1920x480_fp64.md"
+_DMR_SRC32 = "profiles/dmr1920x480_fp32split.md"
+TRAFFIC = {
+    ("double_mach_reflection", 1920, 480, False, (6, 8, 4, 4)): {"bytes_per_step": 1.1358e9, "source": _DMR_SRC64},
+    ("double_mach_reflection", 1920, 480, True, (3, 8, 4, 2)): {"bytes_per_step": 1.1580e9, "source": _DMR_SRC32},
+}
+VALU = {
+    ("double_mach_reflection", 1920, 480, False, (6, 8, 4, 4), "sweep_x"): {"wave_instr_per_launch": 3.2434e7,
+                                                                           "source": _DMR_SRC64},
+    ("double_mach_reflection", 1920, 480, False, (6, 8, 4, 4), "sweep_y"): {"wave_instr_per_launch": 2.9541e7,
+                                                                           "source": _DMR_SRC64},
+    ("double_mach_reflection", 1920, 480, True, (3, 8, 4, 2), "sweep_x"): {"wave_instr_per_launch": 4.3433e7,
+                                                                          "source": _DMR_SRC32},
+    ("double_mach_reflection", 1920, 480, True, (3, 8, 4, 2), "sweep_y"): {"wave_instr_per_launch": 3.8132e7,
+                                                                          "source": _DMR_SRC32},
+}
 
 
 def copy_calibration(mib=512, reps=5):
