@@ -853,21 +853,27 @@ const SweepOps WX_SWEEP_OPS = {WX_NS::h_stage, WX_NS::h_rhs, WX_NS::h_transpose,
                                nullptr,        nullptr,      nullptr,            nullptr, nullptr,
                                WX_NS::h_sweep1, WX_NS::h_combine1};
 
-// Default tiling: about two wavefronts per SIMD (2048 on the 1024 SIMDs).
-// Long grids march 8..64 rows per single-wave workgroup; where that would
-// leave fewer rows per wave than 24, four waves share one group of chunks
-// (two halo faces per group instead of per chunk).
+// Default tiling from a round model of the 256 CUs: two wavefronts per SIMD
+// are resident (2048 waves, 2048 / W workgroups of W waves); a sweep takes
+// rounds x (faces one wave computes) face-times, where a wave computes its
+// chunk plus its share of the group's two halo faces (and, with W > 1, about
+// half a face of deferred work).  The autotuner refines this on the device.
 static void pick_tiling(int64_t nn, int64_t nt, int& chunk, int& waves)
 {
-    int64_t cols = (nt + WX_NS::OUT_COLS - 1) / WX_NS::OUT_COLS;
-    int64_t r = nn * cols / 2048;
-    if (r >= 24) {
-        waves = 1;
-        chunk = (int)(r > 64 ? 64 : r);
-    } else {
-        waves = 4;
-        chunk = (int)(r < 4 ? 4 : r);
-        if (4 * chunk > nn) { waves = 1; chunk = nn < 8 ? 8 : (int)(nn < 64 ? nn : 64); }
+    const int64_t cols = (nt + WX_NS::OUT_COLS - 1) / WX_NS::OUT_COLS;
+    double best = 1e300;
+    chunk = 8;
+    waves = 1;
+    for (int w : {1, 2, 4}) {
+        const int cmin = w == 1 ? 8 : (w == 2 ? 4 : 3), cmax = w == 1 ? 64 : (w == 2 ? 24 : 16);
+        for (int c = cmin; c <= cmax; ++c) {
+            if (w > 1 && (int64_t)w * c > nn) continue;
+            const int64_t groups = (nn + (int64_t)w * c - 1) / ((int64_t)w * c);
+            const int64_t wg = groups * cols, slots = 2048 / w;
+            const int64_t rounds = (wg + slots - 1) / slots;
+            const double cost = (double)rounds * (c + 2.0 / w + (w > 1 ? 0.5 : 0.0));
+            if (cost < best - 1e-9) { best = cost; chunk = c; waves = w; }
+        }
     }
 }
 

@@ -148,11 +148,15 @@ class Solver:
         set before upload() for the first step to follow it too."""
         _lib.check(_lib.load().wenox_solver_set_cfl_rule(self._h, float(rho_floor), float(cmax_eps)))
 
-    TILINGS = ((1, 8), (1, 12), (1, 16), (1, 24), (1, 32), (1, 48), (1, 64),
-               (2, 6), (2, 8), (2, 12), (2, 16), (2, 24),
-               (4, 3), (4, 4), (4, 6), (4, 8), (4, 12), (4, 16))
+    # (wavefronts per workgroup, rows per wavefront).  Dense in the row count:
+    # the sweep time steps with the number of workgroup rounds on the 256 CUs
+    # (two resident waves per SIMD), so one row more or less per wave can
+    # move a grid across a round boundary.
+    TILINGS = (tuple((1, c) for c in (8, 12, 16, 24, 32, 40, 48, 56, 64))
+               + tuple((2, c) for c in range(4, 25, 2))
+               + tuple((4, c) for c in range(3, 17)))
 
-    def autotune(self, candidates=TILINGS, reps: int = 5) -> dict:
+    def autotune(self, candidates=TILINGS, reps: int = 3) -> dict:
         """Pick each sweep's tiling -- (wavefronts per workgroup, rows per
         wavefront) -- with the lowest measured kernel time on this device
         and keep it.  Returns {sweep: ((waves, chunk), {candidate: ms})}."""
