@@ -56,6 +56,20 @@ WX_HD double rcp(double x)
     return 1.0 / x;
 #endif
 }
+// Reciprocal for divisors that may be zero or infinite (the guarded Riemann
+// denominators): v_rcp_f64 returns +-inf for +-0 and 0 for +-inf exactly as
+// 1/x does; the Newton step is applied only to finite nonzero x.
+WX_HD double rcp_z(double x)
+{
+#if WX_FAST && defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    double rr = fma(r, e, r);
+    return (x == 0.0 || fabs(x) == __builtin_inf()) ? r : rr;
+#else
+    return 1.0 / x;
+#endif
+}
 WX_HD float rcp(float x)
 {
 #if WX_FAST && defined(__HIP_DEVICE_COMPILE__)
@@ -65,6 +79,7 @@ WX_HD float rcp(float x)
     return 1.0f / x;
 #endif
 }
+WX_HD float rcp_z(float x) { return 1.0f / x; }
 
 template <typename T> struct V4 { T v[4]; };
 
@@ -532,6 +547,7 @@ template <typename T> struct FaceState {
     T ql0, ql1, ql2, ql3, qr0, qr1, qr2, qr3;   // floored densities
     T u_l, v_l, u_r, v_r, p_l, p_r;
     T s_l, s_r;
+    T il, ir;                                   // 1 / floored densities (fast variant)
 };
 
 template <typename T> WX_HD FaceState<T> face_speeds(int dir, const T ql[4], const T qr[4], T gam)
@@ -542,6 +558,8 @@ template <typename T> WX_HD FaceState<T> face_speeds(int dir, const T ql[4], con
     s.qr0 = vmax(qr[0], eps); s.qr1 = qr[1]; s.qr2 = qr[2]; s.qr3 = qr[3];
 #if WX_FAST
     T il = rcp(s.ql0), ir = rcp(s.qr0);
+    s.il = il;
+    s.ir = ir;
     s.u_l = s.ql1 * il; s.v_l = s.ql2 * il;
     s.u_r = s.qr1 * ir; s.v_r = s.qr2 * ir;
 #else
@@ -591,13 +609,23 @@ template <typename T> WX_HD void hllc_from(int dir, const FaceState<T>& s, const
     T wl = dir == DIR_X ? s.u_l : s.v_l;
     T wr = dir == DIR_X ? s.u_r : s.v_r;
     T den = guard_denom(s.ql0 * (s.s_l - wl) - s.qr0 * (s.s_r - wr));
-    T s_st = (s.p_r - s.p_l + s.ql0 * wl * (s.s_l - wl) - s.qr0 * wr * (s.s_r - wr)) / den;
+#if WX_FAST
+    // reciprocals keep the quotients' zero-divisor results (inf / NaN)
+#define WX_QUOT(n, d) ((n) * rcp_z(d))
+#define WX_OVER_RHO(n, side) ((n) * (side == 0 ? s.il : s.ir))
+#define WX_P_TERM(p, rho, irho, sw) ((p) * (irho) * rcp_z(sw))
+#else
+#define WX_QUOT(n, d) ((n) / (d))
+#define WX_OVER_RHO(n, side) ((n) / (side == 0 ? s.ql0 : s.qr0))
+#define WX_P_TERM(p, rho, irho, sw) ((p) / ((rho) * (sw)))
+#endif
+    T s_st = WX_QUOT(s.p_r - s.p_l + s.ql0 * wl * (s.s_l - wl) - s.qr0 * wr * (s.s_r - wr), den);
     if (s.s_l >= T(0)) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) out[k] = fl[k];
     } else if (s.s_l < T(0) && s_st >= T(0)) {
-        T cf = s.ql0 * (s.s_l - wl) / (s.s_l - s_st);
-        T e = s.ql3 / s.ql0 + (s_st - wl) * (s_st + s.p_l / (s.ql0 * (s.s_l - wl)));
+        T cf = WX_QUOT(s.ql0 * (s.s_l - wl), s.s_l - s_st);
+        T e = WX_OVER_RHO(s.ql3, 0) + (s_st - wl) * (s_st + WX_P_TERM(s.p_l, s.ql0, s.il, s.s_l - wl));
         T st[4];
         st[0] = cf;
         st[1] = dir == DIR_X ? cf * s_st : cf * s.u_l;
@@ -606,8 +634,8 @@ template <typename T> WX_HD void hllc_from(int dir, const FaceState<T>& s, const
 #pragma unroll
         for (int k = 0; k < 4; ++k) out[k] = fl[k] + s.s_l * (st[k] - ql[k]);
     } else if (s_st < T(0) && s.s_r > T(0)) {
-        T cf = s.qr0 * (s.s_r - wr) / (s.s_r - s_st);
-        T e = s.qr3 / s.qr0 + (s_st - wr) * (s_st + s.p_r / (s.qr0 * (s.s_r - wr)));
+        T cf = WX_QUOT(s.qr0 * (s.s_r - wr), s.s_r - s_st);
+        T e = WX_OVER_RHO(s.qr3, 1) + (s_st - wr) * (s_st + WX_P_TERM(s.p_r, s.qr0, s.ir, s.s_r - wr));
         T st[4];
         st[0] = cf;
         st[1] = dir == DIR_X ? cf * s_st : cf * s.u_r;
@@ -619,6 +647,9 @@ template <typename T> WX_HD void hllc_from(int dir, const FaceState<T>& s, const
 #pragma unroll
         for (int k = 0; k < 4; ++k) out[k] = fr[k];
     }
+#undef WX_QUOT
+#undef WX_OVER_RHO
+#undef WX_P_TERM
 }
 
 // riemann.rs:191-269 (x) / :271-349 (y): HLLE with floored densities.
@@ -638,7 +669,7 @@ template <typename T> WX_HD void hlle_from(const FaceState<T>& s, const T fl[4],
         for (int k = 0; k < 4; ++k)
 #if WX_FAST
         {
-            const T rden = T(1) / den;   // one division for the four components
+            const T rden = rcp_z(den);   // one reciprocal for the four components
             out[k] = (s.s_r * fl[k] - s.s_l * fr[k] + s.s_r * s.s_l * (b[k] - a[k])) * rden;
         }
 #else
@@ -651,7 +682,11 @@ template <typename T> WX_HD void hlle_from(const FaceState<T>& s, const T fl[4],
 template <typename T> WX_HD T sensor_phi(T dpdx, T dpdy, T pbar, T lo, T hi)
 {
     T grad = sqrt(dpdx * dpdx + dpdy * dpdy);
+#if WX_FAST
+    T jp = grad * rcp(vmax(pbar, K<T>::eps));
+#else
     T jp = grad / vmax(pbar, K<T>::eps);
+#endif
     if (jp <= lo) return T(1);
     if (jp >= hi) return T(0);
     return T(1) - (jp - lo) / (hi - lo);
