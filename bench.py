@@ -47,10 +47,18 @@ WORKLOADS = {
                      jp=(1e4, 1.1e4), bc="mirror", cfl=0.4),
     "kh8192": dict(name="kelvin_helmholtz", nx=8192, ny=8192, cons=False, force_hlle=False,
                    jp=(1e4, 1.1e4), bc="periodic", cfl=0.4),
+    # 512 independent four-quadrant instances, sizes from {64,96,128,192,256,384}^2
+    "batch512": dict(name="batched_four_quadrant", nx=0, ny=0, cons=False, force_hlle=False,
+                     jp=(1e4, 1.1e4), bc="mirror", cfl=0.4, count=512,
+                     sizes=(64, 96, 128, 192, 256, 384)),
 }
 
 
 def describe(w):
+    if w["name"] == "batched_four_quadrant":
+        return (f"{w['count']} independent four-quadrant instances, sizes uniform from "
+                f"{{{','.join(str(n) for n in w['sizes'])}}}^2 (seed 20261203), characteristic WENO5 + HLLC, "
+                f"per-instance dt at CFL {w['cfl']}, bc {w['bc']}")
     return (f"{w['name']} {w['nx']}x{w['ny']}, characteristic WENO5 + HLLC (jp {w['jp'][0]:g},{w['jp'][1]:g}), "
             f"SSP-RK3, CFL {w['cfl']}, bc {w['bc']}")
 
@@ -96,6 +104,29 @@ def run_gpu(w, q, dx, steps, warmup, engine, variant, precision, profile=True, t
         return out
     finally:
         s.close()
+
+
+def run_batch(w, steps, warmup, precision, chunk=16):
+    """The batched engine: every phase of a step is one launch over all
+    instances (per-instance dt); value = sum of instance cells x steps / wall."""
+    import numpy as np
+    from wenox import Batch, problems
+    rng = np.random.default_rng(20261203)
+    sizes = [(int(n), int(n)) for n in rng.choice(w["sizes"], size=w["count"])]
+    inst = [problems.random_riemann_instance(rng, nx, ny) for nx, ny in sizes]
+    cells = sum(nx * ny for nx, ny in sizes)
+    with Batch(sizes, [d for _, d in inst], cfl=w["cfl"], cons_switch=w["cons"], force_hlle=w["force_hlle"],
+               jp_cri=w["jp"], bc=w["bc"], variant="fast", precision=precision, chunk=chunk) as b:
+        for i, (q, _) in enumerate(inst):
+            b.upload(i, q)
+        b.run(nstep=warmup)
+        t0 = time.perf_counter()
+        b.run(nstep=steps)
+        el = time.perf_counter() - t0
+        T, k = b.status()
+        fin = all(bool(np.isfinite(b.download(i)[3:-3, 3:-3, :4]).all()) for i in range(0, w["count"], 37))
+    return dict(value=cells * steps / el, seconds=el, cells=cells, steps_each=int(np.min(k)) - warmup,
+                ms_per_step=el / steps * 1e3, finite_sampled=fin, chunk=chunk)
 
 
 def cpu_baseline(q, dx, w, budget_s=12.0):
@@ -271,6 +302,23 @@ def main():
         return
 
     w = WORKLOADS[args.workload]
+    if args.workload == "batch512":
+        r = run_batch(w, args.steps, args.warmup, args.precision)
+        res = {"metric": "cell-updates/s FP64 vs FP32-split @<=1e-6 L1 err; % of HBM roofline",
+               "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None,
+               "dtype": "f64" if args.precision == "fp64" else "f32 bulk, f32 hi+lo state",
+               "data": "synthetic", "config": {"workload": describe(w), "cells": r["cells"], "chunk": r["chunk"]},
+               "state_finite_sampled": r["finite_sampled"],
+               "roofline": {"bound": "hbm", "achieved": r["value"] * BYTES_PER_CELL_UPDATE / 1e9,
+                            "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                            "frac": r["value"] * BYTES_PER_CELL_UPDATE / HBM_PEAK, "traffic": None}}
+        if not args.no_extra and args.precision == "fp64":
+            r32 = run_batch(w, args.steps, args.warmup, "fp32split")
+            res["fp32_split"] = {"value": r32["value"], "speedup_vs_fp64": r32["value"] / r["value"]}
+        print(json.dumps(res), flush=True)
+        return
     q, dx = make_state(w)
     r64 = run_gpu(w, q, dx, args.steps, args.warmup, args.engine, args.variant, args.precision,
                   chunks=args.chunks)
@@ -304,7 +352,7 @@ def main():
             "chunks": r32.get("chunks"),
             "roofline": roofline(r32, w, split=True),
         }
-        if args.workload != "quad4096":
+        if args.workload == "dmr":
             wq = WORKLOADS["quad4096"]
             qq, dq = make_state(wq)
             rq = run_gpu(wq, qq, dq, 20, 3, "fused", args.variant, "fp64")

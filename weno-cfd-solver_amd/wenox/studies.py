@@ -54,3 +54,39 @@ def vortex_study(sizes=(256, 512, 1024, 2048, 4096), t_end=2.0, cfl=0.4, variant
             b[prec]["order_L1"] = math.log(a[prec]["L1"] / b[prec]["L1"]) / math.log(r)
             b[prec]["order_Linf"] = math.log(a[prec]["Linf"] / b[prec]["Linf"]) / math.log(r)
     return rows
+
+
+def main(argv=None):
+    """python -m wenox.studies [--sizes 256 512 ...] [--t-end 2.0] [--out study.json]"""
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(description="isentropic vortex precision study on the GPU")
+    ap.add_argument("--sizes", type=int, nargs="+", default=[256, 512, 1024, 2048, 4096])
+    ap.add_argument("--t-end", type=float, default=2.0)
+    ap.add_argument("--cfl", type=float, default=0.4)
+    ap.add_argument("--out", default="")
+    args = ap.parse_args(argv)
+    rows = []
+    for n in args.sizes:
+        # one grid at a time so progress shows while the larger grids run
+        rows += vortex_study(sizes=(n,), t_end=args.t_end, cfl=args.cfl)
+        r = rows[-1]
+        print(f"n={n} fp64 L1={r['fp64']['L1']:.3e} Linf={r['fp64']['Linf']:.3e} steps={r['fp64']['steps']} "
+              f"fp32split L1={r['fp32split']['L1']:.3e} rel_vs_fp64={r['fp32split'].get('rel_L1_vs_fp64', 0):.2e}",
+              file=sys.stderr, flush=True)
+    for prec in ("fp64", "fp32split"):
+        for a, b in zip(rows, rows[1:]):
+            rr = b["n"] / a["n"]
+            b[prec]["order_L1"] = math.log(a[prec]["L1"] / b[prec]["L1"]) / math.log(rr)
+            b[prec]["order_Linf"] = math.log(a[prec]["Linf"] / b[prec]["Linf"]) / math.log(rr)
+    out = json.dumps({"study": "isentropic vortex, component-wise WENO5 + Gauss pass + HLLE, SSP-RK3",
+                      "t_end": args.t_end, "cfl": args.cfl, "rows": rows})
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(out + "\n")
+    print(out, flush=True)
+
+
+if __name__ == "__main__":
+    main()
