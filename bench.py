@@ -124,9 +124,9 @@ def run_batch(w, steps, warmup, precision, chunk=16):
         b.run(nstep=steps)
         el = time.perf_counter() - t0
         T, k = b.status()
-        fin = all(bool(np.isfinite(b.download(i)[3:-3, 3:-3, :4]).all()) for i in range(0, w["count"], 37))
+        bad = [i for i in range(w["count"]) if not np.isfinite(b.download(i)[3:-3, 3:-3, :4]).all()]
     return dict(value=cells * steps / el, seconds=el, cells=cells, steps_each=int(np.min(k)) - warmup,
-                ms_per_step=el / steps * 1e3, finite_sampled=fin, chunk=chunk)
+                ms_per_step=el / steps * 1e3, non_finite=len(bad), chunk=chunk)
 
 
 def cpu_baseline(q, dx, w, budget_s=12.0):
@@ -206,14 +206,16 @@ def roofline(res, w, split=False):
     step_ms = km["step_sum"] if km else res["dev_ms_per_step"]
     achieved = cells * BYTES_PER_CELL_UPDATE / (step_ms * 1e-3)
     tiling = tuple(res.get("chunks") or ())
-    tr = TRAFFIC.get((w["name"], w["nx"], w["ny"], split, tiling))
+    tr = TRAFFIC.get((w["name"], w["nx"], w["ny"], split))
     out = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
            "frac": achieved / HBM_PEAK, "traffic": tr["bytes_per_step"] if tr else None,
            "kernel": "one SSP-RK3 step (one hipGraph launch)",
            "algorithmic_bytes_per_launch": cells * BYTES_PER_CELL_UPDATE,
            "avg_launch_ms": step_ms}
     if tr:
+        # the profiled tiling; another tiling changes only the sweeps' halo re-reads
         out["traffic_source"] = tr["source"]
+        out["traffic_tiling"] = list(tr["tiling"])
     if km:
         kb = kernel_bytes(w, split)
         per = {}
@@ -225,8 +227,8 @@ def roofline(res, w, split=False):
         dom = max(("sweep_x", "sweep_y", "combine"), key=lambda k: km[k])
         out["dominant_kernel"] = {"name": f"k_sweep ({dom})" if dom.startswith("sweep") else "k_combine",
                                   "share_of_step": round(3 * km[dom] / step_ms, 4)}
-        v = VALU.get((w["name"], w["nx"], w["ny"], split, tiling, dom))
-        if v:
+        v = VALU.get((w["name"], w["nx"], w["ny"], split, dom))
+        if v and tuple(v["tiling"]) == tiling:
             rate = v["wave_instr_per_launch"] / (km[dom] * 1e-3)
             out["dominant_kernel"]["valu"] = {"wave_instr_per_launch": v["wave_instr_per_launch"],
                                               "achieved": rate, "peak": VALU_WAVE_INSTR_PEAK,
@@ -243,18 +245,20 @@ def roofline(res, w, split=False):
 _DMR_SRC64 = "profiles/dmr1920x480_fp64.md"
 _DMR_SRC32 = "profiles/dmr1920x480_fp32split.md"
 TRAFFIC = {
-    ("double_mach_reflection", 1920, 480, False, (6, 8, 4, 4)): {"bytes_per_step": 1.1358e9, "source": _DMR_SRC64},
-    ("double_mach_reflection", 1920, 480, True, (3, 8, 4, 2)): {"bytes_per_step": 1.1580e9, "source": _DMR_SRC32},
+    ("double_mach_reflection", 1920, 480, False): {"bytes_per_step": 1.1358e9, "tiling": (6, 8, 4, 4),
+                                                   "source": _DMR_SRC64},
+    ("double_mach_reflection", 1920, 480, True): {"bytes_per_step": 1.1580e9, "tiling": (3, 8, 4, 2),
+                                                  "source": _DMR_SRC32},
 }
 VALU = {
-    ("double_mach_reflection", 1920, 480, False, (6, 8, 4, 4), "sweep_x"): {"wave_instr_per_launch": 3.2434e7,
-                                                                           "source": _DMR_SRC64},
-    ("double_mach_reflection", 1920, 480, False, (6, 8, 4, 4), "sweep_y"): {"wave_instr_per_launch": 2.9541e7,
-                                                                           "source": _DMR_SRC64},
-    ("double_mach_reflection", 1920, 480, True, (3, 8, 4, 2), "sweep_x"): {"wave_instr_per_launch": 4.3433e7,
-                                                                          "source": _DMR_SRC32},
-    ("double_mach_reflection", 1920, 480, True, (3, 8, 4, 2), "sweep_y"): {"wave_instr_per_launch": 3.8132e7,
-                                                                          "source": _DMR_SRC32},
+    ("double_mach_reflection", 1920, 480, False, "sweep_x"): {"wave_instr_per_launch": 3.2434e7, "tiling": (6, 8, 4, 4),
+                                                              "source": _DMR_SRC64},
+    ("double_mach_reflection", 1920, 480, False, "sweep_y"): {"wave_instr_per_launch": 2.9541e7, "tiling": (6, 8, 4, 4),
+                                                              "source": _DMR_SRC64},
+    ("double_mach_reflection", 1920, 480, True, "sweep_x"): {"wave_instr_per_launch": 4.3433e7, "tiling": (3, 8, 4, 2),
+                                                             "source": _DMR_SRC32},
+    ("double_mach_reflection", 1920, 480, True, "sweep_y"): {"wave_instr_per_launch": 3.8132e7, "tiling": (3, 8, 4, 2),
+                                                             "source": _DMR_SRC32},
 }
 
 
@@ -310,7 +314,9 @@ def main():
                "scaling": "weak", "vs_baseline": None,
                "dtype": "f64" if args.precision == "fp64" else "f32 bulk, f32 hi+lo state",
                "data": "synthetic", "config": {"workload": describe(w), "cells": r["cells"], "chunk": r["chunk"]},
-               "state_finite_sampled": r["finite_sampled"],
+               # the reference scheme itself turns some instances non-finite (the oracle
+               # reproduces it); they keep stepping, so the count does not change the cost
+               "instances_non_finite": r["non_finite"],
                "roofline": {"bound": "hbm", "achieved": r["value"] * BYTES_PER_CELL_UPDATE / 1e9,
                             "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                             "frac": r["value"] * BYTES_PER_CELL_UPDATE / HBM_PEAK, "traffic": None}}
