@@ -454,12 +454,21 @@ __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT
         }
 }
 
+// Workgroups go to the 8 XCDs round-robin in launch order.  Workgroup
+// b = 8 k + x takes tile k of band x, where the row-major tile order is cut
+// into 8 contiguous bands (the first ntiles % 8 of them one tile longer): each
+// XCD's L2 then holds a contiguous band, and the cache lines two neighbouring
+// tiles share (rows are 32 doubles, 3 off a line boundary) are fetched once.
 template <class St, typename DT>
 __global__ void __launch_bounds__(256) k_combine(const DT* __restrict__ Dx, const DT* __restrict__ DyT,
                                                  St Un, St Us, St out, St outT, int64_t nx, int64_t ny,
                                                  int stage, const StepCtl* ctl)
 {
-    combine_tile<St, DT>(Dx, DyT, Un, Us, out, outT, nx, ny, stage, ctl, blockIdx.y, blockIdx.x);
+    const int64_t tiles_j = gridDim.x, ntiles = (int64_t)gridDim.x * gridDim.y;
+    const int64_t b = (int64_t)blockIdx.y * tiles_j + blockIdx.x;
+    const int64_t q = ntiles / 8, r = ntiles % 8, x = b % 8, k = b / 8;
+    const int64_t t = x * q + (x < r ? x : r) + k;
+    combine_tile<St, DT>(Dx, DyT, Un, Us, out, outT, nx, ny, stage, ctl, t / tiles_j, t % tiles_j);
 }
 
 // Plain RHS from the two sweeps (for tests): L = -(Dx + Dy^T), SoA4 interior.
