@@ -153,7 +153,9 @@ double wenox_solver_last_ms_per_step(wenox_solver *s);
 /* Per-kernel device times (ms, HIP events on the solver's stream, averaged
  * over reps launches of each kernel alone), fused engine only:
  * out_ms[0] CFL reduction, [1] x sweep, [2] y sweep, [3] RK combine,
- * [4] ghost fill, [5] their sum over one step (3 stages).  n_out >= 6. */
+ * [4] ghost fill, [5] their sum over one step (3 stages), and with
+ * n_out >= 7 [6] both sweeps launched as the step launches them ([5] then
+ * uses [6] for the sweeps).  n_out >= 6. */
 int wenox_solver_profile(wenox_solver *s, int reps, double *out_ms, int n_out);
 /* CFL rule dt = CFL dx / (max(sqrt(g p / max(rho, rho_floor)) + |V|) + cmax_eps):
  * (1e-14, 1e-16) by default as Rie_prob_2d_rs.py:311-319; the 1-D driver

@@ -55,6 +55,10 @@ struct SweepOps {
                    const StepCtl* ctl, hipStream_t s);
     void (*combine1)(const SweepPlan& p, int split, const void* Dx, const void* DyT, const void* Un,
                      const void* Us, void* out, void* outT, const StepCtl* ctl, hipStream_t s);
+    // Both sweeps of a stage as the step launches them (one kernel when the
+    // two tilings use the same number of wavefronts per workgroup).
+    void (*sweep2)(const SweepPlan& p, int split, const void* U, const void* UT, void* Dx, void* DyT,
+                   int* flags, const StepCtl* ctl, hipStream_t s);
 };
 
 // ---- batched engine: many independent problems in each launch ----------------
@@ -115,7 +119,7 @@ extern const SweepOps sweep_ops_fast;
 SweepPlan make_sweep_plan(int64_t nx, int64_t ny, double dx, double gamma, int cons,
                           int force_hlle, double jp_lo, double jp_hi);
 
-// Workgroups of the larger sweep grid (size of the FP32-split chunk flags).
+// Flag slots of the FP32-split sweeps: the workgroups of both sweeps of a stage.
 int64_t sweep_flag_count(const SweepPlan& p);
 
 }  // namespace wenox

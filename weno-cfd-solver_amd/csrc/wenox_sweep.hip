@@ -102,6 +102,16 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// LDS of one sweep workgroup (declared by the kernel, so a kernel running
+// both sweeps holds one copy).
+template <typename T, int W> struct SweepLds {
+    T p1[W][8][WAVE];     // pass-1 q_l (0..3), q_r (4..7) of the current face
+    T pb[W][2][WAVE];     // p̄ ring (face parity)
+    T xpb[W][2][WAVE];    // p̄ of the first [0] and last [1] owned face
+    T xf[W][12][WAVE];    // deferred first face: HLLC (0..3), HLLE (4..7), F of the next face (8..11)
+    T xF[W][4][WAVE];     // blended flux of the first owned face
+};
+
 // One workgroup: column block bx, row group by of W consecutive chunks, one
 // wavefront per chunk; `bid` indexes the flags (W per workgroup).
 //
@@ -117,7 +127,8 @@ __device__ __forceinline__ void wave_sync()
 // Halo cost per group: 2 faces for W * chunk cells.  Needs chunk >= 3 when
 // W > 1 (the host enforces it).
 template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND, int W>
-__device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64_t by, int64_t bid)
+__device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64_t by, int64_t bid,
+                                           SweepLds<T, W>& S)
 {
     if (A.ctl && !A.ctl->active) return;
     if constexpr (MODE == MODE_FIXUP) {
@@ -126,12 +137,11 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
         for (int i = 0; i < W; ++i) any = any || A.flags[bid * W + i] != 0;
         if (!any) return;
     }
-    constexpr int XW = W > 1 ? W : 1;
-    __shared__ T s_p1[W][8][WAVE];     // pass-1 q_l (0..3), q_r (4..7) of the current face
-    __shared__ T s_pb[W][2][WAVE];     // p̄ ring (face parity)
-    __shared__ T s_xpb[XW][2][WAVE];   // p̄ of the first [0] and last [1] owned face
-    __shared__ T s_xf[XW][12][WAVE];   // deferred first face: HLLC (0..3), HLLE (4..7), F of the next face (8..11)
-    __shared__ T s_xF[XW][4][WAVE];    // blended flux of the first owned face
+    auto& s_p1 = S.p1;
+    auto& s_pb = S.pb;
+    auto& s_xpb = S.xpb;
+    auto& s_xf = S.xf;
+    auto& s_xF = S.xF;
 
     const T gam = (T)A.gam, two_h = (T)A.two_h, jlo = (T)A.lo, jhi = (T)A.hi, dx = (T)A.dx;
 #if WX_FAST
@@ -382,8 +392,28 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
 template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND, int W>
 __global__ void __launch_bounds__(WAVE * W) __attribute__((amdgpu_waves_per_eu(2))) k_sweep(SweepArgs A)
 {
+    __shared__ SweepLds<T, W> S;
     sweep_body<T, IN, DT, MODE, DIR, CONS, BLEND, W>(A, blockIdx.x, blockIdx.y,
-                                                     (int64_t)blockIdx.y * gridDim.x + blockIdx.x);
+                                                     (int64_t)blockIdx.y * gridDim.x + blockIdx.x, S);
+}
+
+// Both sweeps of a stage in one launch: workgroups [0, nbx) run the x sweep,
+// the rest the y sweep (they read different planes and write different D
+// arrays).  The two halves fill each other's tail, which on grids of about
+// one wave round per sweep is most of the gain.  Flag slots follow the
+// workgroup number, so the x and y halves use disjoint ranges.
+template <typename T, int IN, typename DT, int MODE, bool CONS, bool BLEND, int W>
+__global__ void __launch_bounds__(WAVE * W) __attribute__((amdgpu_waves_per_eu(2)))
+k_sweep_xy(SweepArgs Ax, SweepArgs Ay, unsigned nbx, unsigned colsx, unsigned colsy)
+{
+    __shared__ SweepLds<T, W> S;
+    const unsigned b = blockIdx.x;
+    if (b < nbx) {
+        sweep_body<T, IN, DT, MODE, DIR_X, CONS, BLEND, W>(Ax, b % colsx, b / colsx, b, S);
+    } else {
+        const unsigned c = b - nbx;
+        sweep_body<T, IN, DT, MODE, DIR_Y, CONS, BLEND, W>(Ay, c % colsy, c / colsy, b, S);
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -526,6 +556,41 @@ static void launch_sweep(const SweepArgs& a, int cons, int blend, hipStream_t s)
     else launch_sweep_w<T, IN, DT, MODE, DIR, 1>(a, cons, blend, s);
 }
 
+template <typename T, int IN, typename DT, int MODE, int W>
+static void launch_sweep_xy_w(const SweepArgs& ax, const SweepArgs& ay, int cons, int blend, hipStream_t s)
+{
+    const unsigned colsx = (unsigned)((ax.nt + OUT_COLS - 1) / OUT_COLS);
+    const unsigned colsy = (unsigned)((ay.nt + OUT_COLS - 1) / OUT_COLS);
+    const int64_t rx = (int64_t)W * ax.chunk, ry = (int64_t)W * ay.chunk;
+    const unsigned nbx = colsx * (unsigned)((ax.nn + rx - 1) / rx);
+    const unsigned nby = colsy * (unsigned)((ay.nn + ry - 1) / ry);
+    const dim3 grid(nbx + nby);
+    if (cons) {
+        if (blend) k_sweep_xy<T, IN, DT, MODE, true, true, W><<<grid, WAVE * W, 0, s>>>(ax, ay, nbx, colsx, colsy);
+        else k_sweep_xy<T, IN, DT, MODE, true, false, W><<<grid, WAVE * W, 0, s>>>(ax, ay, nbx, colsx, colsy);
+    } else {
+        if (blend) k_sweep_xy<T, IN, DT, MODE, false, true, W><<<grid, WAVE * W, 0, s>>>(ax, ay, nbx, colsx, colsy);
+        else k_sweep_xy<T, IN, DT, MODE, false, false, W><<<grid, WAVE * W, 0, s>>>(ax, ay, nbx, colsx, colsy);
+    }
+}
+
+// x then y sweep of one stage: one launch when both use the same number of
+// wavefronts per workgroup, else two.
+template <typename T, int IN, typename DT, int MODE>
+static void launch_sweeps(const SweepArgs& ax, const SweepArgs& ay, int cons, int blend, hipStream_t s)
+{
+    if (ax.waves != ay.waves) {
+        launch_sweep<T, IN, DT, MODE, DIR_X>(ax, cons, blend, s);
+        launch_sweep<T, IN, DT, MODE, DIR_Y>(ay, cons, blend, s);
+    } else if (ax.waves == 4) {
+        launch_sweep_xy_w<T, IN, DT, MODE, 4>(ax, ay, cons, blend, s);
+    } else if (ax.waves == 2) {
+        launch_sweep_xy_w<T, IN, DT, MODE, 2>(ax, ay, cons, blend, s);
+    } else {
+        launch_sweep_xy_w<T, IN, DT, MODE, 1>(ax, ay, cons, blend, s);
+    }
+}
+
 static SweepArgs args_x(const SweepPlan& p, const void* U, void* D, int* flags, const StepCtl* ctl)
 {
     return SweepArgs{U, p.nx + 6, p.ny + 6, p.nx, p.ny, p.chunk_x, p.waves_x, p.gamma, p.dx, p.two_h_x,
@@ -543,8 +608,8 @@ static void h_sweeps(const SweepPlan& p, const double* Us, const double* UsT, do
                      const StepCtl* ctl, hipStream_t s)
 {
     const int blend = p.force_hlle ? 0 : 1;
-    launch_sweep<double, IN_F64, double, MODE_PLAIN, DIR_X>(args_x(p, Us, Dx, nullptr, ctl), p.cons, blend, s);
-    launch_sweep<double, IN_F64, double, MODE_PLAIN, DIR_Y>(args_y(p, UsT, DyT, nullptr, ctl), p.cons, blend, s);
+    launch_sweeps<double, IN_F64, double, MODE_PLAIN>(args_x(p, Us, Dx, nullptr, ctl),
+                                                      args_y(p, UsT, DyT, nullptr, ctl), p.cons, blend, s);
 }
 
 static dim3 combine_grid(const SweepPlan& p)
@@ -599,6 +664,22 @@ static void h_sweep1(const SweepPlan& p, int dir, int split, const void* U, void
         launch_sweep<double, IN_F64, double, MODE_PLAIN, DIR_Y>(args_y(p, U, D, nullptr, ctl), p.cons, blend, s);
 }
 
+static void h_sweep2(const SweepPlan& p, int split, const void* U, const void* UT, void* Dx, void* DyT,
+                     int* flags, const StepCtl* ctl, hipStream_t s)
+{
+    const int blend = p.force_hlle ? 0 : 1;
+#if WX_FAST
+    if (split) {
+        const SweepArgs ax = args_x(p, U, Dx, flags, ctl), ay = args_y(p, UT, DyT, flags, ctl);
+        launch_sweeps<double, IN_SPLIT, float, MODE_DETECT>(ax, ay, p.cons, blend, s);
+        launch_sweeps<double, IN_SPLIT, float, MODE_FIXUP>(ax, ay, p.cons, blend, s);
+        return;
+    }
+#endif
+    launch_sweeps<double, IN_F64, double, MODE_PLAIN>(args_x(p, U, Dx, nullptr, ctl), args_y(p, UT, DyT, nullptr, ctl),
+                                                      p.cons, blend, s);
+}
+
 static void h_combine1(const SweepPlan& p, int split, const void* Dx, const void* DyT, const void* Un,
                        const void* Us, void* out, void* outT, const StepCtl* ctl, hipStream_t s)
 {
@@ -624,10 +705,9 @@ static void h_sweeps_split(const SweepPlan& p, const float* Us, const float* UsT
                            float* DyT, int* flags, const StepCtl* ctl, hipStream_t s)
 {
     const int blend = p.force_hlle ? 0 : 1;
-    launch_sweep<double, IN_SPLIT, float, MODE_DETECT, DIR_X>(args_x(p, Us, Dx, flags, ctl), p.cons, blend, s);
-    launch_sweep<double, IN_SPLIT, float, MODE_FIXUP, DIR_X>(args_x(p, Us, Dx, flags, ctl), p.cons, blend, s);
-    launch_sweep<double, IN_SPLIT, float, MODE_DETECT, DIR_Y>(args_y(p, UsT, DyT, flags, ctl), p.cons, blend, s);
-    launch_sweep<double, IN_SPLIT, float, MODE_FIXUP, DIR_Y>(args_y(p, UsT, DyT, flags, ctl), p.cons, blend, s);
+    const SweepArgs ax = args_x(p, Us, Dx, flags, ctl), ay = args_y(p, UsT, DyT, flags, ctl);
+    launch_sweeps<double, IN_SPLIT, float, MODE_DETECT>(ax, ay, p.cons, blend, s);
+    launch_sweeps<double, IN_SPLIT, float, MODE_FIXUP>(ax, ay, p.cons, blend, s);
 }
 
 static void h_stage_split(const SweepPlan& p, const float* Us, const float* UsT, const float* Un,
@@ -732,7 +812,8 @@ __global__ void __launch_bounds__(WAVE) k_sweep_batch(BatchTables B, BatchCommon
     A.ctl = &ctls[i];
     const int local = b - (DIR == DIR_X ? I.blk0_x : I.blk0_y);
     const int gx = DIR == DIR_X ? I.gx_x : I.gx_y;
-    sweep_body<T, IN, DT, MODE, DIR, CONS, BLEND, 1>(A, local % gx, local / gx, b);
+    __shared__ SweepLds<T, 1> S;
+    sweep_body<T, IN, DT, MODE, DIR, CONS, BLEND, 1>(A, local % gx, local / gx, b, S);
 }
 
 template <class St, typename DT>
@@ -856,11 +937,12 @@ const BatchOps batch_ops_exact = {WX_NS::hb_stage, WX_NS::hb_bc, WX_NS::hb_cfl, 
 const SweepOps WX_SWEEP_OPS = {WX_NS::h_stage,       WX_NS::h_rhs,        WX_NS::h_transpose,
                                WX_NS::h_stage_split, WX_NS::h_rhs_split,  WX_NS::h_transpose_split,
                                WX_NS::h_to_split,    WX_NS::h_from_split, WX_NS::h_bc_split,
-                               WX_NS::h_cfl_split,   WX_NS::h_sweep1,     WX_NS::h_combine1};
+                               WX_NS::h_cfl_split,   WX_NS::h_sweep1,     WX_NS::h_combine1,
+                               WX_NS::h_sweep2};
 #else
 const SweepOps WX_SWEEP_OPS = {WX_NS::h_stage, WX_NS::h_rhs, WX_NS::h_transpose, nullptr, nullptr,
                                nullptr,        nullptr,      nullptr,            nullptr, nullptr,
-                               WX_NS::h_sweep1, WX_NS::h_combine1};
+                               WX_NS::h_sweep1, WX_NS::h_combine1, WX_NS::h_sweep2};
 
 // Default tiling from a round model of the 256 CUs: two wavefronts per SIMD
 // are resident (2048 waves, 2048 / W workgroups of W waves); a sweep takes
@@ -907,7 +989,8 @@ int64_t sweep_flag_count(const SweepPlan& p)
     const int64_t rx = (int64_t)p.chunk_x * p.waves_x, ry = (int64_t)p.chunk_y * p.waves_y;
     int64_t fx = ((p.ny + c - 1) / c) * ((p.nx + rx - 1) / rx) * p.waves_x;
     int64_t fy = ((p.nx + c - 1) / c) * ((p.ny + ry - 1) / ry) * p.waves_y;
-    return fx > fy ? fx : fy;
+    // the one-launch sweep pair numbers its workgroups across both halves
+    return fx + fy;
 }
 #endif
 #endif

@@ -113,13 +113,15 @@ class Solver:
     def ms_per_step(self) -> float:
         return _lib.load().wenox_solver_last_ms_per_step(self._h)
 
-    KERNELS = ("cfl", "sweep_x", "sweep_y", "combine", "ghost_fill", "step_sum")
+    KERNELS = ("cfl", "sweep_x", "sweep_y", "combine", "ghost_fill", "step_sum", "sweeps")
 
     def profile(self, reps: int = 10) -> dict:
         """Per-kernel device time (ms) of the fused engine, HIP events on the
-        solver's stream, each kernel launched `reps` times alone."""
-        out = np.zeros(6)
-        _lib.check(_lib.load().wenox_solver_profile(self._h, int(reps), out.ctypes.data, 6))
+        solver's stream, each kernel launched `reps` times alone.  "sweeps" is
+        the stage's x and y sweep as the step launches them (one kernel when
+        their tilings share a wave count); "step_sum" adds it up per step."""
+        out = np.zeros(7)
+        _lib.check(_lib.load().wenox_solver_profile(self._h, int(reps), out.ctypes.data, 7))
         return dict(zip(self.KERNELS, out.tolist()))
 
     @property
@@ -156,34 +158,44 @@ class Solver:
                + tuple((2, c) for c in range(4, 25, 2))
                + tuple((4, c) for c in range(3, 17)))
 
-    def autotune(self, candidates=TILINGS, reps: int = 3) -> dict:
-        """Pick each sweep's tiling -- (wavefronts per workgroup, rows per
-        wavefront) -- with the lowest measured kernel time on this device
-        and keep it.  Returns {sweep: ((waves, chunk), {candidate: ms})}."""
-        best = {}
+    def autotune(self, candidates=TILINGS, reps: int = 3, finalists: int = 4) -> dict:
+        """Pick the sweep tiling -- (wavefronts per workgroup, rows per
+        wavefront) for x and y -- with the lowest measured time on this
+        device and keep it.  Each direction is timed over `candidates`; the
+        `finalists` fastest of each (plus the model's default) are then timed
+        in pairs as the step launches them (one kernel when the wave counts
+        agree).  The default is kept unless a pair beats it by 3 %.
+        Returns {"sweep_x": {cand: ms}, "sweep_y": {...}, "pairs": {...},
+        "best": (cx, cy, wx, wy)}."""
+        cx0, cy0, wx0, wy0 = self.tiling
+        per = {}
         for d, key in ((0, "sweep_x"), (1, "sweep_y")):
             times = {}
-            cx, cy, wx, wy = self.tiling
-            start = (wx, cx) if d == 0 else (wy, cy)
+            start = (wx0, cx0) if d == 0 else (wy0, cy0)
             for w, c in tuple(candidates) + ((start,) if start not in candidates else ()):
-                cx, cy, wx, wy = self.tiling
                 if d == 0:
-                    self.set_tiling(c, cy, w, wy)
+                    self.set_tiling(c, cy0, w, wy0)
                 else:
-                    self.set_tiling(cx, c, wx, w)
+                    self.set_tiling(cx0, c, wx0, w)
                 times[(w, c)] = self.profile(reps)[key]
-            wb, cb = min(times, key=times.get)
-            # keep the model's default unless a candidate is clearly faster
-            # (timings of near-equal tilings swap from run to run)
-            if times[(wb, cb)] > 0.97 * times[start]:
-                wb, cb = start
-            cx, cy, wx, wy = self.tiling
-            if d == 0:
-                self.set_tiling(cb, cy, wb, wy)
-            else:
-                self.set_tiling(cx, cb, wx, wb)
-            best[key] = ((wb, cb), times)
-        return best
+            per[key] = times
+        self.set_tiling(cx0, cy0, wx0, wy0)
+        fx = sorted(per["sweep_x"], key=per["sweep_x"].get)[:finalists]
+        fy = sorted(per["sweep_y"], key=per["sweep_y"].get)[:finalists]
+        pairs = {}
+        for (wx, cx) in fx + [(wx0, cx0)]:
+            for (wy, cy) in fy + [(wy0, cy0)]:
+                t = (cx, cy, wx, wy)
+                if t in pairs:
+                    continue
+                self.set_tiling(*t)
+                pairs[t] = self.profile(reps)["sweeps"]
+        default = (cx0, cy0, wx0, wy0)
+        best = min(pairs, key=pairs.get)
+        if pairs[best] > 0.97 * pairs[default]:
+            best = default
+        self.set_tiling(*best)
+        return {"sweep_x": per["sweep_x"], "sweep_y": per["sweep_y"], "pairs": pairs, "best": best}
 
 
 class Batch:
