@@ -258,20 +258,18 @@ def roofline(res, w, split=False):
 _DMR_SRC64 = "profiles/dmr1920x480_fp64.md"
 _DMR_SRC32 = "profiles/dmr1920x480_fp32split.md"
 TRAFFIC = {
-    ("double_mach_reflection", 1920, 480, False): {"bytes_per_step": 1.1358e9, "tiling": (6, 8, 4, 4),
+    ("double_mach_reflection", 1920, 480, False): {"bytes_per_step": 1.0918e9, "tiling": (9, 8, 4, 4),
                                                    "source": _DMR_SRC64},
-    ("double_mach_reflection", 1920, 480, True): {"bytes_per_step": 1.1580e9, "tiling": (3, 8, 4, 2),
+    ("double_mach_reflection", 1920, 480, True): {"bytes_per_step": 0.9710e9, "tiling": (3, 4, 4, 4),
                                                   "source": _DMR_SRC32},
 }
+# VALU wave-instructions of one k_sweep_xy launch (for FP32-split: its FP32
+# pass and its FP64 recompute launch together)
 VALU = {
-    ("double_mach_reflection", 1920, 480, False, "sweep_x"): {"wave_instr_per_launch": 3.2434e7, "tiling": (6, 8, 4, 4),
-                                                              "source": _DMR_SRC64},
-    ("double_mach_reflection", 1920, 480, False, "sweep_y"): {"wave_instr_per_launch": 2.9541e7, "tiling": (6, 8, 4, 4),
-                                                              "source": _DMR_SRC64},
-    ("double_mach_reflection", 1920, 480, True, "sweep_x"): {"wave_instr_per_launch": 4.3433e7, "tiling": (3, 8, 4, 2),
-                                                             "source": _DMR_SRC32},
-    ("double_mach_reflection", 1920, 480, True, "sweep_y"): {"wave_instr_per_launch": 3.8132e7, "tiling": (3, 8, 4, 2),
-                                                             "source": _DMR_SRC32},
+    ("double_mach_reflection", 1920, 480, False, "sweeps"): {"wave_instr_per_launch": 5.9547e7,
+                                                             "tiling": (9, 8, 4, 4), "source": _DMR_SRC64},
+    ("double_mach_reflection", 1920, 480, True, "sweeps"): {"wave_instr_per_launch": 8.0803e7,
+                                                            "tiling": (3, 4, 4, 4), "source": _DMR_SRC32},
 }
 
 
