@@ -100,7 +100,10 @@ def run_gpu(w, q, dx, steps, warmup, engine, variant, precision, profile=True, t
                 out["autotune_ms"] = {
                     "sweep_x": {f"w{w}c{c}": round(v, 5) for (w, c), v in tuned["sweep_x"].items()},
                     "sweep_y": {f"w{w}c{c}": round(v, 5) for (w, c), v in tuned["sweep_y"].items()},
-                    "pairs": {"x{0}w{2}_y{1}w{3}".format(*t): round(v, 5) for t, v in tuned["pairs"].items()}}
+                    "pairs": {"x{0}w{2}_y{1}w{3}".format(*t): round(v, 5) for t, v in tuned["pairs"].items()},
+                    "combine_cols": {str(c): round(v, 5) for c, v in tuned["combine"].items()}}
+        if engine == "fused":
+            out["combine_cols"] = s.combine_cols
         if profile and engine == "fused":
             out["kernels_ms"] = s.profile(reps=10)
         return out
@@ -353,7 +356,8 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32 bulk, f32 hi+lo state",
         "data": "synthetic",
         "config": {"workload": describe(w), "engine": args.engine, "variant": args.variant,
-                   "chunks": r64.get("chunks"), "autotune_ms": r64.get("autotune_ms")},
+                   "chunks": r64.get("chunks"), "combine_cols": r64.get("combine_cols"),
+                   "autotune_ms": r64.get("autotune_ms")},
         "state_finite": r64["finite"],
         "roofline": roofline(r64, w, split=args.precision != "fp64"),
     }

@@ -169,6 +169,10 @@ int wenox_solver_plan(wenox_solver *s, int *chunk_x, int *chunk_y);
 int wenox_solver_set_chunks(wenox_solver *s, int chunk_x, int chunk_y);
 int wenox_solver_tiling(wenox_solver *s, int *chunk_x, int *chunk_y, int *waves_x, int *waves_y);
 int wenox_solver_set_tiling(wenox_solver *s, int chunk_x, int chunk_y, int waves_x, int waves_y);
+/* y extent (32 or 64 cells) of the fused engine's RK-combine tiles (32 rows
+ * in x).  No reference counterpart. */
+int wenox_solver_combine_cols(wenox_solver *s, int *cols);
+int wenox_solver_set_combine_cols(wenox_solver *s, int cols);
 
 /* ---- 3. batched solver: many independent problems of uneven sizes ---------- */
 /* The n instances share the numerics of desc (nx, ny, dx of desc are ignored;

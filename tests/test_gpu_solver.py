@@ -184,3 +184,20 @@ def test_tiling_arguments_checked():
         for bad in ((2, 8, 4, 1), (8, 8, 3, 1), (0, 8, 1, 1)):
             with pytest.raises((ValueError, WenoxError)):
                 s.set_tiling(*bad)
+
+
+def test_combine_tile_widths_bitwise(oracle_mod):
+    """The RK combine with 32- and 64-wide tiles (grids that are not a
+    multiple of either) gives the oracle's 5-step trajectory bit for bit."""
+    from wenox import Solver, problems
+    nx, ny = 70, 45
+    q0, dx = problems.four_quadrant(nx, ny=ny)
+    ref, _ = oracle_mod.run_riemann(q0.copy(), nx, ny, dx, 1.4, 0.4, False, False, (0.5, 2.0), nstep=5)
+    print("[solver] combine tile widths 70x45", flush=True)
+    for cols in (32, 64):
+        with Solver(nx, ny, dx, cfl=0.4, jp_cri=(0.5, 2.0), engine="fused") as s:
+            s.set_combine_cols(cols)
+            assert s.combine_cols == cols
+            s.upload(q0)
+            s.run(nstep=5)
+            np.testing.assert_array_equal(s.download()[..., :4], ref[..., :4], err_msg=f"combine cols {cols}")

@@ -935,6 +935,23 @@ extern "C" int wenox_solver_set_tiling(wenox_solver* s, int chunk_x, int chunk_y
     return WENOX_OK;
 }
 
+extern "C" int wenox_solver_combine_cols(wenox_solver* s, int* cols)
+{
+    if (!s || !cols) return fail(WENOX_E_ARG, "wenox_solver_combine_cols: null argument");
+    *cols = s->plan.combine_cols;
+    return WENOX_OK;
+}
+
+extern "C" int wenox_solver_set_combine_cols(wenox_solver* s, int cols)
+{
+    if (!s || (cols != 32 && cols != 64)) return fail(WENOX_E_ARG, "wenox_solver_set_combine_cols: cols is 32 or 64");
+    if (s->d.engine != WENOX_ENGINE_FUSED) return fail(WENOX_E_ARG, "the combine tile applies to the fused engine");
+    s->plan.combine_cols = cols;
+    if (s->exec) { (void)hipGraphExecDestroy(s->exec); s->exec = nullptr; }
+    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    return WENOX_OK;
+}
+
 extern "C" int wenox_solver_set_chunks(wenox_solver* s, int chunk_x, int chunk_y)
 {
     if (!s) return fail(WENOX_E_ARG, "wenox_solver_set_chunks: null solver");

@@ -21,6 +21,7 @@ struct SweepPlan {
     double two_h_x, two_h_y;   // 2 / max(min(F1, F2), 1) of the x and y face grids (riemann.rs:360-361)
     int chunk_x, chunk_y;      // interior rows marched per wavefront
     int waves_x, waves_y;      // wavefronts per workgroup along the normal (1, 2 or 4)
+    int combine_cols;          // y extent of an RK-combine tile (32 or 64)
 };
 
 struct SweepOps {

@@ -423,65 +423,68 @@ k_sweep_xy(SweepArgs Ax, SweepArgs Ay, unsigned nbx, unsigned colsx, unsigned co
 // ------------------------------------------------------------------------
 constexpr int TT = 32;
 
-// One 32 x 32 tile: tile row ti (x), tile column tj (y).  The four
-// components' Dy^T tiles are staged in LDS together, so each thread has all
-// loads of its cells in flight at once and the tile needs three barriers.
-template <class St, typename DT>
+// One tile of TT rows (x) by TJ columns (y), TJ = 32 or 64: tile row ti,
+// tile column tj.  The four components' Dy^T tiles are staged in LDS
+// together, so each thread has all loads of its cells in flight at once.
+// A thread overwrites in LDS exactly the Dy^T entries it read with the new
+// state, so the tile needs two barriers; the transposed state then leaves
+// LDS along x.  Wider tiles give the x-major streams 512 B rows.
+template <class St, typename DT, int TJ>
 __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT* __restrict__ DyT, St Un,
                                              St Us, St out, St outT, int64_t nx, int64_t ny, int stage,
                                              const StepCtl* ctl, int64_t ti, int64_t tj)
 {
     if (ctl && !ctl->active) return;
-    __shared__ double tile[4][TT][TT + 1];
-    const int tx = threadIdx.x & (TT - 1);   // fast index
-    const int ty = threadIdx.x / TT;         // 0..7
-    const int64_t i0 = ti * TT, j0 = tj * TT;
+    __shared__ double tile[4][TJ][TT + 1];   // [k][j - j0][i - i0]
+    const int64_t i0 = ti * TT, j0 = tj * TJ;
     const int64_t m = nx + 6, n = ny + 6, PI = nx * ny;
     const double dt = ctl ? ctl->dt : 0.0;
-    // Dy^T[k][j][i]: rows j, contiguous i
+    // Dy^T[k][j][i]: rows j, contiguous i (32 threads per row)
+    {
+        const int c = threadIdx.x & (TT - 1), r0 = threadIdx.x / TT;
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < 4; ++k)
 #pragma unroll
-        for (int r = ty; r < TT; r += 8) {
-            const int64_t j = j0 + r, i = i0 + tx;
-            if (j < ny && i < nx) tile[k][r][tx] = (double)DyT[k * PI + j * nx + i];
-        }
+            for (int r = r0; r < TJ; r += 256 / TT) {
+                const int64_t j = j0 + r, i = i0 + c;
+                if (j < ny && i < nx) tile[k][r][c] = (double)DyT[k * PI + j * nx + i];
+            }
+    }
     __syncthreads();
-    double res[4][TT / 8];
+    // x-major cells: TJ threads per row, contiguous j
+    {
+        const int tx = threadIdx.x & (TJ - 1), ty = threadIdx.x / TJ;
 #pragma unroll
-    for (int r = ty, q = 0; r < TT; r += 8, ++q) {
-        const int64_t i = i0 + r, j = j0 + tx;
+        for (int r = ty; r < TT; r += 256 / TJ) {
+            const int64_t i = i0 + r, j = j0 + tx;
+            if (i < nx && j < ny) {
+                const int64_t o = (i + 3) * n + (j + 3);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) res[k][q] = 0.0;
-        if (i < nx && j < ny) {
-            const int64_t o = (i + 3) * n + (j + 3);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const double l = -((double)Dx[k * PI + i * ny + j] + tile[k][tx][r]);
-                const double qn = Un.ld(k, o);
-                double v;
-                if (stage == 1) v = qn + dt * l;
-                else if (stage == 2) v = 0.75 * qn + 0.25 * Us.ld(k, o) + (0.25 * dt) * l;
-                else v = (1.0 / 3.0) * qn + (2.0 / 3.0) * Us.ld(k, o) + ((2.0 / 3.0) * dt) * l;
-                out.st(k, o, v);
-                res[k][q] = v;
+                for (int k = 0; k < 4; ++k) {
+                    const double l = -((double)Dx[k * PI + i * ny + j] + tile[k][tx][r]);
+                    const double qn = Un.ld(k, o);
+                    double v;
+                    if (stage == 1) v = qn + dt * l;
+                    else if (stage == 2) v = 0.75 * qn + 0.25 * Us.ld(k, o) + (0.25 * dt) * l;
+                    else v = (1.0 / 3.0) * qn + (2.0 / 3.0) * Us.ld(k, o) + ((2.0 / 3.0) * dt) * l;
+                    out.st(k, o, v);
+                    tile[k][tx][r] = v;
+                }
             }
         }
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-#pragma unroll
-        for (int r = ty, q = 0; r < TT; r += 8, ++q) tile[k][r][tx] = res[k][q];
-    __syncthreads();
     // outT[k][j+3][i+3], contiguous i
+    {
+        const int c = threadIdx.x & (TT - 1), r0 = threadIdx.x / TT;
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < 4; ++k)
 #pragma unroll
-        for (int r = ty; r < TT; r += 8) {
-            const int64_t j = j0 + r, i = i0 + tx;
-            if (j < ny && i < nx) outT.st(k, (j + 3) * m + (i + 3), tile[k][tx][r]);
-        }
+            for (int r = r0; r < TJ; r += 256 / TT) {
+                const int64_t j = j0 + r, i = i0 + c;
+                if (j < ny && i < nx) outT.st(k, (j + 3) * m + (i + 3), tile[k][r][c]);
+            }
+    }
 }
 
 // Workgroups go to the 8 XCDs round-robin in launch order.  Workgroup
@@ -489,7 +492,7 @@ __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT
 // into 8 contiguous bands (the first ntiles % 8 of them one tile longer): each
 // XCD's L2 then holds a contiguous band, and the cache lines two neighbouring
 // tiles share (rows are 32 doubles, 3 off a line boundary) are fetched once.
-template <class St, typename DT>
+template <class St, typename DT, int TJ>
 __global__ void __launch_bounds__(256) k_combine(const DT* __restrict__ Dx, const DT* __restrict__ DyT,
                                                  St Un, St Us, St out, St outT, int64_t nx, int64_t ny,
                                                  int stage, const StepCtl* ctl)
@@ -498,7 +501,21 @@ __global__ void __launch_bounds__(256) k_combine(const DT* __restrict__ Dx, cons
     const int64_t b = (int64_t)blockIdx.y * tiles_j + blockIdx.x;
     const int64_t q = ntiles / 8, r = ntiles % 8, x = b % 8, k = b / 8;
     const int64_t t = x * q + (x < r ? x : r) + k;
-    combine_tile<St, DT>(Dx, DyT, Un, Us, out, outT, nx, ny, stage, ctl, t / tiles_j, t % tiles_j);
+    combine_tile<St, DT, TJ>(Dx, DyT, Un, Us, out, outT, nx, ny, stage, ctl, t / tiles_j, t % tiles_j);
+}
+
+template <class St, typename DT>
+static void launch_combine(const SweepPlan& p, const DT* Dx, const DT* DyT, St Un, St Us, St out, St outT,
+                           int stage, const StepCtl* ctl, hipStream_t s)
+{
+    const unsigned ti = (unsigned)((p.nx + TT - 1) / TT);
+    if (p.combine_cols == 64) {
+        k_combine<St, DT, 64><<<dim3((unsigned)((p.ny + 63) / 64), ti), 256, 0, s>>>(Dx, DyT, Un, Us, out, outT,
+                                                                                       p.nx, p.ny, stage, ctl);
+    } else {
+        k_combine<St, DT, 32><<<dim3((unsigned)((p.ny + 31) / 32), ti), 256, 0, s>>>(Dx, DyT, Un, Us, out, outT,
+                                                                                       p.nx, p.ny, stage, ctl);
+    }
 }
 
 // Plain RHS from the two sweeps (for tests): L = -(Dx + Dy^T), SoA4 interior.
@@ -612,20 +629,14 @@ static void h_sweeps(const SweepPlan& p, const double* Us, const double* UsT, do
                                                       args_y(p, UsT, DyT, nullptr, ctl), p.cons, blend, s);
 }
 
-static dim3 combine_grid(const SweepPlan& p)
-{
-    return dim3((unsigned)((p.ny + TT - 1) / TT), (unsigned)((p.nx + TT - 1) / TT));
-}
-
 static void h_stage(const SweepPlan& p, const double* Us, const double* UsT, const double* Un,
                     double* Dx, double* DyT, double* out, double* outT, int stage, const StepCtl* ctl,
                     hipStream_t s)
 {
     h_sweeps(p, Us, UsT, Dx, DyT, ctl, s);
     const int64_t P = (p.nx + 6) * (p.ny + 6);
-    k_combine<StF64, double><<<combine_grid(p), 256, 0, s>>>(
-        Dx, DyT, StF64{const_cast<double*>(Un), P}, StF64{const_cast<double*>(Us), P}, StF64{out, P},
-        StF64{outT, P}, p.nx, p.ny, stage, ctl);
+    launch_combine<StF64, double>(p, Dx, DyT, StF64{const_cast<double*>(Un), P}, StF64{const_cast<double*>(Us), P},
+                                  StF64{out, P}, StF64{outT, P}, stage, ctl, s);
 }
 
 static void h_rhs(const SweepPlan& p, const double* U, const double* UT, double* Dx, double* DyT,
@@ -686,15 +697,14 @@ static void h_combine1(const SweepPlan& p, int split, const void* Dx, const void
     const int64_t P = (p.nx + 6) * (p.ny + 6);
 #if WX_FAST
     if (split) {
-        k_combine<StSplit, float><<<combine_grid(p), 256, 0, s>>>(
-            (const float*)Dx, (const float*)DyT, StSplit{(float*)Un, P}, StSplit{(float*)Us, P},
-            StSplit{(float*)out, P}, StSplit{(float*)outT, P}, p.nx, p.ny, 2, ctl);
+        launch_combine<StSplit, float>(p, (const float*)Dx, (const float*)DyT, StSplit{(float*)Un, P},
+                                       StSplit{(float*)Us, P}, StSplit{(float*)out, P}, StSplit{(float*)outT, P}, 2,
+                                       ctl, s);
         return;
     }
 #endif
-    k_combine<StF64, double><<<combine_grid(p), 256, 0, s>>>(
-        (const double*)Dx, (const double*)DyT, StF64{(double*)Un, P}, StF64{(double*)Us, P},
-        StF64{(double*)out, P}, StF64{(double*)outT, P}, p.nx, p.ny, 2, ctl);
+    launch_combine<StF64, double>(p, (const double*)Dx, (const double*)DyT, StF64{(double*)Un, P},
+                                  StF64{(double*)Us, P}, StF64{(double*)out, P}, StF64{(double*)outT, P}, 2, ctl, s);
 }
 
 #if WX_FAST
@@ -716,9 +726,8 @@ static void h_stage_split(const SweepPlan& p, const float* Us, const float* UsT,
 {
     h_sweeps_split(p, Us, UsT, Dx, DyT, flags, ctl, s);
     const int64_t P = (p.nx + 6) * (p.ny + 6);
-    k_combine<StSplit, float><<<combine_grid(p), 256, 0, s>>>(
-        Dx, DyT, StSplit{const_cast<float*>(Un), P}, StSplit{const_cast<float*>(Us), P},
-        StSplit{out, P}, StSplit{outT, P}, p.nx, p.ny, stage, ctl);
+    launch_combine<StSplit, float>(p, Dx, DyT, StSplit{const_cast<float*>(Un), P}, StSplit{const_cast<float*>(Us), P},
+                                   StSplit{out, P}, StSplit{outT, P}, stage, ctl, s);
 }
 
 static void h_rhs_split(const SweepPlan& p, const float* U, const float* UT, float* Dx, float* DyT,
@@ -825,7 +834,7 @@ __global__ void __launch_bounds__(256) k_combine_batch(BatchTables B, const DT* 
     const BatchInst I = B.inst[i];
     const int64_t P = (I.nx + 6) * (I.ny + 6);
     const int local = b - I.tile0;
-    combine_tile<St, DT>(Dx + I.ioff, DyT + I.ioff, make_st<St>(Un, I.soff, P), make_st<St>(Us, I.soff, P),
+    combine_tile<St, DT, TT>(Dx + I.ioff, DyT + I.ioff, make_st<St>(Un, I.soff, P), make_st<St>(Us, I.soff, P),
                          make_st<St>(out, I.soff, P), make_st<St>(outT, I.soff, P), I.nx, I.ny, stage,
                          &ctls[i], local / I.tiles_j, local % I.tiles_j);
 }
@@ -980,6 +989,7 @@ SweepPlan make_sweep_plan(int64_t nx, int64_t ny, double dx, double gamma, int c
     p.two_h_y = 2.0 * (1.0 / (double)(my > 1 ? my : 1));
     pick_tiling(nx, ny, p.chunk_x, p.waves_x);
     pick_tiling(ny, nx, p.chunk_y, p.waves_y);
+    p.combine_cols = ny >= 64 ? 64 : 32;
     return p;
 }
 

@@ -141,6 +141,16 @@ class Solver:
         _lib.check(_lib.load().wenox_solver_tiling(self._h, *[ctypes.byref(x) for x in v]))
         return tuple(x.value for x in v)
 
+    @property
+    def combine_cols(self) -> int:
+        """y extent (32 or 64) of the RK-combine tiles."""
+        v = ctypes.c_int()
+        _lib.check(_lib.load().wenox_solver_combine_cols(self._h, ctypes.byref(v)))
+        return v.value
+
+    def set_combine_cols(self, cols: int):
+        _lib.check(_lib.load().wenox_solver_set_combine_cols(self._h, int(cols)))
+
     def set_tiling(self, chunk_x: int, chunk_y: int, waves_x: int, waves_y: int):
         _lib.check(_lib.load().wenox_solver_set_tiling(self._h, int(chunk_x), int(chunk_y), int(waves_x),
                                                         int(waves_y)))
@@ -195,7 +205,16 @@ class Solver:
         if pairs[best] > 0.97 * pairs[default]:
             best = default
         self.set_tiling(*best)
-        return {"sweep_x": per["sweep_x"], "sweep_y": per["sweep_y"], "pairs": pairs, "best": best}
+        # RK-combine tile width
+        comb = {}
+        c0 = self.combine_cols
+        for cols in (32, 64):
+            self.set_combine_cols(cols)
+            comb[cols] = self.profile(reps)["combine"]
+        cb = min(comb, key=comb.get)
+        self.set_combine_cols(cb if comb[cb] < 0.97 * comb[c0] else c0)
+        return {"sweep_x": per["sweep_x"], "sweep_y": per["sweep_y"], "pairs": pairs, "best": best,
+                "combine": comb}
 
 
 class Batch:

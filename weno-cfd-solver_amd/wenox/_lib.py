@@ -89,6 +89,8 @@ def load():
         "wenox_solver_set_chunks": ([dp, i, i], i),
         "wenox_solver_tiling": ([dp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)], i),
         "wenox_solver_set_tiling": ([dp, i, i, i, i], i),
+        "wenox_solver_combine_cols": ([dp, ctypes.POINTER(i)], i),
+        "wenox_solver_set_combine_cols": ([dp, i], i),
         "wenox_solver_set_cfl_rule": ([dp, d, d], i),
         "wenox_batch_create": ([ctypes.POINTER(SolverDesc), i, dp, dp, dp, i,
                                 ctypes.POINTER(ctypes.c_void_p)], i),
