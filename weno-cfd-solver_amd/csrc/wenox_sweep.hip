@@ -989,7 +989,7 @@ SweepPlan make_sweep_plan(int64_t nx, int64_t ny, double dx, double gamma, int c
     p.two_h_y = 2.0 * (1.0 / (double)(my > 1 ? my : 1));
     pick_tiling(nx, ny, p.chunk_x, p.waves_x);
     pick_tiling(ny, nx, p.chunk_y, p.waves_y);
-    p.combine_cols = ny >= 64 ? 64 : 32;
+    p.combine_cols = 32;   // measured faster than 64-wide on 1920x480 and no slower at 4096^2
     return p;
 }
 
