@@ -178,7 +178,7 @@ __device__ __forceinline__ double sensor_at(const double* p, int64_t F1, int64_t
     if (j == 0) dpdy = (p[i * F2 + 1] - p[i * F2]) / two_h;
     else if (j == F2 - 1) dpdy = (p[i * F2 + j] - p[i * F2 + j - 1]) / two_h;
     else dpdy = (p[i * F2 + j + 1] - p[i * F2 + j - 1]) / two_h;
-    return sensor_phi(dpdx, dpdy, p[i * F2 + j], lo, hi);
+    return sensor_phi(dpdx, dpdy, p[i * F2 + j], lo, hi, 1.0 / (hi - lo));
 }
 
 __global__ void k_blend(Acc QL, Acc FL, Acc QR, Acc FR, const double* pbar, int64_t F1, int64_t F2,

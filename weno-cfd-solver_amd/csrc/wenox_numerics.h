@@ -81,21 +81,81 @@ WX_HD float rcp(float x)
 }
 WX_HD float rcp_z(float x) { return 1.0f / x; }
 
+// Square root and reciprocal square root of a positive finite x for the fast
+// variant: v_rsq_f64 followed by the coupled Goldschmidt iteration for
+// s ~ sqrt(x), h ~ 1/(2 sqrt(x)) (Markstein, "IA-64 and Elementary
+// Functions", 2000, ch. 9), a residual correction of s (s + (x - s^2) h) and
+// a second step on h.  Both results are within about one ulp; the sequence
+// costs one transcendental and nine VALU operations against about seventeen
+// for a correctly rounded sqrt plus a reciprocal.
+WX_HD void sqrt_rsqrt(double x, double& s, double& g)
+{
+#if WX_FAST && defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rsq(x);
+    double ss = x * y, h = 0.5 * y;
+    double r = fma(-ss, h, 0.5);
+    ss = fma(ss, r, ss);
+    h = fma(h, r, h);
+    double d = fma(-ss, ss, x);
+    ss = fma(d, h, ss);
+    r = fma(-ss, h, 0.5);
+    h = fma(h, r, h);
+    s = ss;
+    g = h + h;
+#else
+    s = sqrt(x);
+    g = 1.0 / s;
+#endif
+}
+WX_HD void sqrt_rsqrt(float x, float& s, float& g)
+{
+    s = sqrtf(x);
+    g = 1.0f / s;
+}
+// Square root of a finite x >= 0 (zero allowed) for the fast variant: the
+// same iteration without the second step on h.  Arguments below 1e-300
+// (zero, subnormals) return 0, NaN stays NaN, +inf gives NaN.
+WX_HD double sqrt_z(double x)
+{
+#if WX_FAST && defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rsq(x);
+    double ss = x * y, h = 0.5 * y;
+    double r = fma(-ss, h, 0.5);
+    ss = fma(ss, r, ss);
+    h = fma(h, r, h);
+    double d = fma(-ss, ss, x);
+    ss = fma(d, h, ss);
+    return x > 1e-300 ? ss : (x == x ? 0.0 : x);
+#else
+    return sqrt(x);
+#endif
+}
+WX_HD float sqrt_z(float x) { return sqrtf(x); }
+
+// Roe-average ingredients from two positive densities a (left), b (right):
+// k = sqrt(b / a), sqrt(a b), 1 / sqrt(a b), 1 / a and 1 / b, from one
+// sqrt_rsqrt of a b (fast variant; a, b >= 1e-12 keep a b in range).
+template <typename T> struct RoeW { T k, rt, irt, ia, ib; };
+template <typename T> WX_HD RoeW<T> roe_weights(T a, T b)
+{
+    RoeW<T> w;
+    T ab = a * b;
+    sqrt_rsqrt(ab, w.rt, w.irt);
+    T g2 = w.irt * w.irt;
+    w.k = b * w.irt;
+    w.ia = b * g2;
+    w.ib = a * g2;
+    return w;
+}
+
 template <typename T> struct V4 { T v[4]; };
 
 // ---------------------------------------------------------------------
 // utils.rs:74-108  Euler flux normal to x (dir 0) or y (dir 1).
 // Density is floored, pressure is not.
 // ---------------------------------------------------------------------
-template <typename T> WX_HD void flux_pt(int dir, const T q[4], T gam, T f[4])
+template <typename T> WX_HD void flux_uv(int dir, const T q[4], T rho, T u, T v, T gam, T f[4])
 {
-    T rho = vmax(q[0], K<T>::eps);
-#if WX_FAST
-    T ir = rcp(rho);
-    T u = q[1] * ir, v = q[2] * ir;
-#else
-    T u = q[1] / rho, v = q[2] / rho;
-#endif
     T p = (gam - T(1)) * (q[3] - T(0.5) * rho * (u * u + v * v));
     if (dir == DIR_X) {
         f[0] = q[1];
@@ -108,6 +168,31 @@ template <typename T> WX_HD void flux_pt(int dir, const T q[4], T gam, T f[4])
         f[2] = q[2] * v + p;
         f[3] = v * (q[3] + p);
     }
+}
+template <typename T> WX_HD void flux_pt(int dir, const T q[4], T gam, T f[4])
+{
+    T rho = vmax(q[0], K<T>::eps);
+#if WX_FAST
+    T ir = rcp(rho);
+    flux_uv(dir, q, rho, q[1] * ir, q[2] * ir, gam, f);
+#else
+    flux_uv(dir, q, rho, q[1] / rho, q[2] / rho, gam, f);
+#endif
+}
+// Two fluxes; the fast variant shares one reciprocal, 1/a = b / (a b) and
+// 1/b = a / (a b) (floored densities keep a b in range).
+template <typename T> WX_HD void flux_pt2(int dir, const T qa[4], const T qb[4], T gam, T fa[4], T fb[4])
+{
+#if WX_FAST
+    T ra = vmax(qa[0], K<T>::eps), rb = vmax(qb[0], K<T>::eps);
+    T r = rcp(ra * rb);
+    T ia = rb * r, ib = ra * r;
+    flux_uv(dir, qa, ra, qa[1] * ia, qa[2] * ia, gam, fa);
+    flux_uv(dir, qb, rb, qb[1] * ib, qb[2] * ib, gam, fb);
+#else
+    flux_pt(dir, qa, gam, fa);
+    flux_pt(dir, qb, gam, fb);
+#endif
 }
 
 // utils.rs:9-35  conservative -> (p, a, rho_safe, u, v, h); floors on rho and p.
@@ -140,16 +225,18 @@ template <typename T> WX_HD T pressure_floored(const T q[4], T gam)
 // ---------------------------------------------------------------------
 // weno.rs:13-63  Roe average for the characteristic basis (same for x, y).
 // ---------------------------------------------------------------------
-template <typename T> struct Roe { T rho, u, v, h, c; };
+// The fast variant writes the weights as sqrt(rho_l) / (sqrt(rho_l) +
+// sqrt(rho_r)) = 1 / (1 + k), k = sqrt(rho_r / rho_l), and takes rho~ =
+// sqrt(rho_l rho_r) and 1 / rho~ from roe_weights; it also returns 1 / c.
+template <typename T> struct Roe { T rho, u, v, h, c, ic, irho; };
 template <typename T> WX_HD Roe<T> roe_avg(const T ul[4], const T ur[4], T gam)
 {
     const T eps = K<T>::eps;
     T rho_l = vmax(ul[0], eps);
     T rho_r = vmax(ur[0], eps);
-    T sl = sqrt(rho_l), sr = sqrt(rho_r);
-    T den = sl + sr;
 #if WX_FAST
-    T il = rcp(rho_l), ir = rcp(rho_r), id = rcp(den);
+    RoeW<T> rw = roe_weights(rho_l, rho_r);
+    T il = rw.ia, ir = rw.ib, id = rcp(T(1) + rw.k);
     T u_l = ul[1] * il, v_l = ul[2] * il;
     T u_r = ur[1] * ir, v_r = ur[2] * ir;
     T p_l = vmax((gam - T(1)) * (ul[3] - T(0.5) * rho_l * (u_l * u_l + v_l * v_l)), eps);
@@ -157,12 +244,17 @@ template <typename T> WX_HD Roe<T> roe_avg(const T ul[4], const T ur[4], T gam)
     T h_l = (ul[3] + p_l) * il;
     T h_r = (ur[3] + p_r) * ir;
     Roe<T> r;
-    r.u = (sl * u_l + sr * u_r) * id;
-    r.v = (sl * v_l + sr * v_r) * id;
-    r.h = (sl * h_l + sr * h_r) * id;
+    r.u = (u_l + rw.k * u_r) * id;
+    r.v = (v_l + rw.k * v_r) * id;
+    r.h = (h_l + rw.k * h_r) * id;
     T kin = T(0.5) * (r.u * r.u + r.v * r.v);
-    r.c = sqrt(vmax((gam - T(1)) * (r.h - kin), eps * vmax(il, ir)));
+    sqrt_rsqrt(vmax((gam - T(1)) * (r.h - kin), eps * vmax(il, ir)), r.c, r.ic);
+    r.rho = rw.rt;
+    r.irho = rw.irt;
+    return r;
 #else
+    T sl = sqrt(rho_l), sr = sqrt(rho_r);
+    T den = sl + sr;
     T u_l = ul[1] / rho_l, v_l = ul[2] / rho_l;
     T u_r = ur[1] / rho_r, v_r = ur[2] / rho_r;
     T p_l = vmax((gam - T(1)) * (ul[3] - T(0.5) * rho_l * (u_l * u_l + v_l * v_l)), eps);
@@ -175,9 +267,11 @@ template <typename T> WX_HD Roe<T> roe_avg(const T ul[4], const T ur[4], T gam)
     r.h = (sl * h_l + sr * h_r) / den;
     T kin = T(0.5) * (r.u * r.u + r.v * r.v);
     r.c = sqrt(vmax((gam - T(1)) * (r.h - kin), eps / vmin(rho_l, rho_r)));
-#endif
     r.rho = sl * sr;
+    r.ic = T(0);     // not used by the exact variant
+    r.irho = T(0);
     return r;
+#endif
 }
 
 // weno.rs:76-189  left (rows: u-c, entropy, shear, u+c) and right
@@ -191,9 +285,9 @@ template <typename T> WX_HD void eig_left(int dir, const Roe<T>& s, T gam, T L[4
     T kin = T(0.5) * (u * u + v * v);
     T un = dir == DIR_X ? u : v;
 #if WX_FAST
-    T ic = rcp(c);
+    T ic = s.ic;
     T inv_c2 = ic * ic;
-    T irho = rcp(rho);
+    T irho = s.irho;
     T inv_rc = irho * ic;
     T a = g1 * kin * (T(0.5) * inv_c2);
     T b = un * (T(0.5) * ic);
@@ -275,6 +369,41 @@ template <typename T> WX_HD void mat4_vec4(const T M[4][4], const T x[4], T y[4]
         y[r] = acc;
 #endif
     }
+}
+
+// Products with the left / right eigenvector matrices.  The fast variant
+// skips their structural zeros (L row 2: columns 1 and 3 for x faces, 2 and
+// 3 for y faces; R row 0 column 2; R row 1 (x) or row 2 (y) column 2) and the
+// unit entries of R row 0; the exact variant keeps the reference's
+// four-term accumulation.
+template <typename T> WX_HD void mat4_vec4_L(int dir, const T M[4][4], const T x[4], T y[4])
+{
+#if WX_FAST
+    y[0] = M[0][0] * x[0] + M[0][1] * x[1] + M[0][2] * x[2] + M[0][3] * x[3];
+    y[1] = M[1][0] * x[0] + M[1][1] * x[1] + M[1][2] * x[2] + M[1][3] * x[3];
+    y[2] = dir == DIR_X ? M[2][0] * x[0] + M[2][2] * x[2] : M[2][0] * x[0] + M[2][1] * x[1];
+    y[3] = M[3][0] * x[0] + M[3][1] * x[1] + M[3][2] * x[2] + M[3][3] * x[3];
+#else
+    (void)dir;
+    mat4_vec4(M, x, y);
+#endif
+}
+template <typename T> WX_HD void mat4_vec4_R(int dir, const T M[4][4], const T x[4], T y[4])
+{
+#if WX_FAST
+    y[0] = x[0] + x[1] + x[3];
+    if (dir == DIR_X) {
+        y[1] = M[1][0] * x[0] + M[1][1] * x[1] + M[1][3] * x[3];
+        y[2] = M[2][0] * x[0] + M[2][1] * x[1] + M[2][2] * x[2] + M[2][3] * x[3];
+    } else {
+        y[1] = M[1][0] * x[0] + M[1][1] * x[1] + M[1][2] * x[2] + M[1][3] * x[3];
+        y[2] = M[2][0] * x[0] + M[2][1] * x[1] + M[2][3] * x[3];
+    }
+    y[3] = M[3][0] * x[0] + M[3][1] * x[1] + M[3][2] * x[2] + M[3][3] * x[3];
+#else
+    (void)dir;
+    mat4_vec4(M, x, y);
+#endif
 }
 
 // ---------------------------------------------------------------------
@@ -440,15 +569,15 @@ template <typename T> WX_HD void recon_normal(int dir, bool cons, const T c[6][4
     eig_right(dir, s, R);
     T w[6][4];
 #pragma unroll
-    for (int t = 0; t < 6; ++t) mat4_vec4(L, c[t], w[t]);
+    for (int t = 0; t < 6; ++t) mat4_vec4_L(dir, L, c[t], w[t]);
     T chl[4], chr[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         chl[k] = weno5_left(w[0][k], w[1][k], w[2][k], w[3][k], w[4][k]);
         chr[k] = weno5_right(w[1][k], w[2][k], w[3][k], w[4][k], w[5][k]);
     }
-    mat4_vec4(R, chl, ql);
-    mat4_vec4(R, chr, qr);
+    mat4_vec4_R(dir, R, chl, ql);
+    mat4_vec4_R(dir, R, chr, qr);
     pp_limit(ql, c[2], gam);
     pp_limit(qr, c[3], gam);
 }
@@ -507,7 +636,7 @@ WX_HD void recon_normal_dev(int dir, bool cons, const double c[6][4], double gam
         base[n] -= kap;
         base[3] -= kap * (dir == DIR_X ? s.u : s.v);
     }
-    Roe<float> sf{(float)s.rho, (float)s.u, (float)s.v, (float)s.h, (float)s.c};
+    Roe<float> sf{(float)s.rho, (float)s.u, (float)s.v, (float)s.h, (float)s.c, (float)s.ic, (float)s.irho};
     float L[4][4], R[4][4];
     eig_left<float>(dir, sf, (float)gam, L);
     eig_right<float>(dir, sf, R);
@@ -518,7 +647,7 @@ WX_HD void recon_normal_dev(int dir, bool cons, const double c[6][4], double gam
 #pragma unroll
             for (int k = 0; k < 4; ++k) w[t][k] = 0.0f;
         } else {
-            mat4_vec4<float>(L, d[t], w[t]);
+            mat4_vec4_L<float>(dir, L, d[t], w[t]);
         }
     }
     float chl[4], chr[4], el[4], er[4];
@@ -527,8 +656,8 @@ WX_HD void recon_normal_dev(int dir, bool cons, const double c[6][4], double gam
         chl[k] = weno5_left<float>(w[0][k], w[1][k], 0.0f, w[3][k], w[4][k]);
         chr[k] = weno5_right<float>(w[1][k], 0.0f, w[3][k], w[4][k], w[5][k]);
     }
-    mat4_vec4<float>(R, chl, el);
-    mat4_vec4<float>(R, chr, er);
+    mat4_vec4_R<float>(dir, R, chl, el);
+    mat4_vec4_R<float>(dir, R, chr, er);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         ql[k] = base[k] + (double)el[k];
@@ -557,7 +686,8 @@ template <typename T> WX_HD FaceState<T> face_speeds(int dir, const T ql[4], con
     s.ql0 = vmax(ql[0], eps); s.ql1 = ql[1]; s.ql2 = ql[2]; s.ql3 = ql[3];
     s.qr0 = vmax(qr[0], eps); s.qr1 = qr[1]; s.qr2 = qr[2]; s.qr3 = qr[3];
 #if WX_FAST
-    T il = rcp(s.ql0), ir = rcp(s.qr0);
+    RoeW<T> rw = roe_weights(s.ql0, s.qr0);
+    T il = rw.ia, ir = rw.ib;
     s.il = il;
     s.ir = ir;
     s.u_l = s.ql1 * il; s.v_l = s.ql2 * il;
@@ -570,24 +700,26 @@ template <typename T> WX_HD FaceState<T> face_speeds(int dir, const T ql[4], con
     T p_r = (gam - T(1)) * (s.qr3 - T(0.5) * s.qr0 * (s.u_r * s.u_r + s.v_r * s.v_r));
     s.p_l = vmax(p_l, eps);
     s.p_r = vmax(p_r, eps);
-    T sl = sqrt(s.ql0), sr = sqrt(s.qr0);
-    T dr = sl + sr;
 #if WX_FAST
     T h_l = (s.ql3 + s.p_l) * il;
     T h_r = (s.qr3 + s.p_r) * ir;
-    T idr = rcp(dr);
-    T ut = (sl * s.u_l + sr * s.u_r) * idr;
-    T vt = (sl * s.v_l + sr * s.v_r) * idr;
-    T ht = (sl * h_l + sr * h_r) * idr;
+    T idr = rcp(T(1) + rw.k);
+    T ut = (s.u_l + rw.k * s.u_r) * idr;
+    T vt = (s.v_l + rw.k * s.v_r) * idr;
+    T ht = (h_l + rw.k * h_r) * idr;
+    T tmp = ht - T(0.5) * (ut * ut + vt * vt);
+    T at = sqrt_z(vmax((gam - T(1)) * tmp, T(0)));
 #else
+    T sl = sqrt(s.ql0), sr = sqrt(s.qr0);
+    T dr = sl + sr;
     T h_l = (s.ql3 + s.p_l) / s.ql0;
     T h_r = (s.qr3 + s.p_r) / s.qr0;
     T ut = (sl * s.u_l + sr * s.u_r) / dr;
     T vt = (sl * s.v_l + sr * s.v_r) / dr;
     T ht = (sl * h_l + sr * h_r) / dr;
-#endif
     T tmp = ht - T(0.5) * (ut * ut + vt * vt);
     T at = sqrt(vmax((gam - T(1)) * tmp, T(0)));
+#endif
     T wt = dir == DIR_X ? ut : vt;
     s.s_l = wt - at;
     s.s_r = wt + at;
@@ -678,18 +810,25 @@ template <typename T> WX_HD void hlle_from(const FaceState<T>& s, const T fl[4],
     }
 }
 
-// riemann.rs:389-407  blend weight from the sensor value
-template <typename T> WX_HD T sensor_phi(T dpdx, T dpdy, T pbar, T lo, T hi)
+// riemann.rs:389-407  blend weight from the sensor value.  inv_span =
+// 1 / (hi - lo) is used by the fast variant only.
+template <typename T> WX_HD T sensor_phi(T dpdx, T dpdy, T pbar, T lo, T hi, T inv_span)
 {
-    T grad = sqrt(dpdx * dpdx + dpdy * dpdy);
 #if WX_FAST
+    T grad = sqrt_z(dpdx * dpdx + dpdy * dpdy);
     T jp = grad * rcp(vmax(pbar, K<T>::eps));
 #else
+    T grad = sqrt(dpdx * dpdx + dpdy * dpdy);
     T jp = grad / vmax(pbar, K<T>::eps);
 #endif
     if (jp <= lo) return T(1);
     if (jp >= hi) return T(0);
+#if WX_FAST
+    return T(1) - (jp - lo) * inv_span;
+#else
+    (void)inv_span;
     return T(1) - (jp - lo) / (hi - lo);
+#endif
 }
 
 }  // namespace wx

@@ -147,11 +147,13 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
 #if WX_FAST
     // the fast variant multiplies by host-formed reciprocals of the constant divisors
     const T inv_two_h = (T)(1.0 / A.two_h), inv_dx = (T)(1.0 / A.dx);
+    const T inv_span = (T)(1.0 / (A.hi - A.lo));
 #define WX_DIV_2H(x) ((x) * inv_two_h)
 #define WX_DIV_DX(x) ((x) * inv_dx)
 #else
 #define WX_DIV_2H(x) ((x) / two_h)
 #define WX_DIV_DX(x) ((x) / dx)
+    const T inv_span = T(0);   // unused by the exact variant
 #endif
     // the wave index is uniform across the wave: keep it (and every row bound
     // derived from it) in scalar registers
@@ -208,7 +210,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
         else if (t >= A.nt - 1) dt_ = WX_DIV_2H(ring[l] - ring[lm]);
         else dt_ = WX_DIV_2H(ring[lp] - ring[lm]);
         // the reference's dpdx is along x: normal for x faces, transverse for y faces
-        return DIR == DIR_X ? sensor_phi(dn, dt_, pbf, jlo, jhi) : sensor_phi(dt_, dn, pbf, jlo, jhi);
+        return DIR == DIR_X ? sensor_phi(dn, dt_, pbf, jlo, jhi, inv_span)
+                            : sensor_phi(dt_, dn, pbf, jlo, jhi, inv_span);
     };
 
     if (!empty) {
@@ -268,10 +271,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                     bad = bad || bad_state(l1, gam) || bad_state(l2, gam) || bad_state(q1, gam) ||
                           bad_state(q2, gam);
                 T fa_[4], fb_[4], fcc[4], fd[4];
-                flux_pt(DIR, l1, gam, fa_);
-                flux_pt(DIR, l2, gam, fb_);
-                flux_pt(DIR, q1, gam, fcc);
-                flux_pt(DIR, q2, gam, fd);
+                flux_pt2(DIR, l1, l2, gam, fa_, fb_);
+                flux_pt2(DIR, q1, q2, gam, fcc, fd);
                 T QL[4], FL[4], QR[4], FR[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
@@ -284,7 +285,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                 hlle_from(st, FL, FR, fe);
                 if (BLEND) {
                     hllc_from(DIR, st, QL, FL, QR, FR, fc);
-                    pb = T(0.5) * (pressure_floored(QL, gam) + pressure_floored(QR, gam));
+                    // floored pressures of QL and QR (riemann.rs:421-430), as formed by face_speeds
+                    pb = T(0.5) * (st.p_l + st.p_r);
                 }
             } else {
 #pragma unroll
