@@ -286,34 +286,6 @@ template <typename T> WX_HD void eig_left(int dir, const Roe<T>& s, T gam, T L[4
     T g1 = gam - T(1);
     T kin = T(0.5) * (u * u + v * v);
     T un = dir == DIR_X ? u : v;
-#if WX_FAST
-    T ic = s.ic;
-    T inv_c2 = ic * ic;
-    T irho = s.irho;
-    T inv_rc = irho * ic;
-    T a = g1 * kin * (T(0.5) * inv_c2);
-    T b = un * (T(0.5) * ic);
-    T hu = T(0.5) * g1 * u * inv_c2, hv = T(0.5) * g1 * v * inv_c2;
-    T h3 = T(0.5) * g1 * inv_c2;
-    T hr = T(0.5) * inv_rc;
-    L[0][0] = a + b;
-    L[3][0] = a - b;
-    L[1][0] = T(1) - g1 * kin * inv_c2;
-    L[1][1] = g1 * u * inv_c2;
-    L[1][2] = g1 * v * inv_c2;
-    L[1][3] = -g1 * inv_c2;
-    L[0][3] = h3;
-    L[3][3] = h3;
-    if (dir == DIR_X) {
-        L[0][1] = -hu - hr; L[0][2] = -hv;
-        L[3][1] = -hu + hr; L[3][2] = -hv;
-        L[2][0] = -v * irho; L[2][1] = T(0); L[2][2] = irho; L[2][3] = T(0);
-    } else {
-        L[0][1] = -hu; L[0][2] = -hv - hr;
-        L[3][1] = -hu; L[3][2] = -hv + hr;
-        L[2][0] = -u * irho; L[2][1] = irho; L[2][2] = T(0); L[2][3] = T(0);
-    }
-#else
     T rc = rho * c;
     T inv_rc = T(1) / rc;
     T inv_c2 = T(1) / (c * c);
@@ -338,7 +310,6 @@ template <typename T> WX_HD void eig_left(int dir, const Roe<T>& s, T gam, T L[4
         L[3][2] = -g1 * v * inv_c2 / T(2) + T(0.5) * inv_rc;
         L[2][0] = -u / rho; L[2][1] = T(1) / rho; L[2][2] = T(0); L[2][3] = T(0);
     }
-#endif
 }
 
 template <typename T> WX_HD void eig_right(int dir, const Roe<T>& s, T R[4][4])
@@ -373,39 +344,67 @@ template <typename T> WX_HD void mat4_vec4(const T M[4][4], const T x[4], T y[4]
     }
 }
 
-// Products with the left / right eigenvector matrices.  The fast variant
-// skips their structural zeros (L row 2: columns 1 and 3 for x faces, 2 and
-// 3 for y faces; R row 0 column 2; R row 1 (x) or row 2 (y) column 2) and the
-// unit entries of R row 0; the exact variant keeps the reference's
-// four-term accumulation.
-template <typename T> WX_HD void mat4_vec4_L(int dir, const T M[4][4], const T x[4], T y[4])
+// Fast-variant products with the reference's eigenvector matrices, from
+// their entries' structure instead of 4x4 products (weno.rs:76-189):
+//   rows 0 and 3 of L are A +- B with A = a c0 - hu c1 - hv c2 + h3 c3 and
+//   B = b c0 - hr c_n (c_n the normal momentum); row 1 equals
+//   e0 - 2 (a, -hu, -hv, h3), so L[1] c = c0 - 2 A; row 2 is
+//   (c_t - w_t c0) / rho (c_t, w_t transverse momentum and velocity).
+//   R c has y0 = c0 + c1 + c3, the normal-momentum row w_n y0 + c (c3 - c0),
+//   the transverse one w_t y0 + rho c2 and the energy row
+//   h y0 + (kin - h) c1 + w_n c (c3 - c0) + rho w_t c2.
+// The exact variant keeps the matrices and the four-term accumulation.
+template <typename T> struct LeftRows { T a, b, hu, hv, h3, hr, irho, wt; };
+template <typename T> WX_HD LeftRows<T> left_rows(int dir, const Roe<T>& s, T gam)
 {
-#if WX_FAST
-    y[0] = M[0][0] * x[0] + M[0][1] * x[1] + M[0][2] * x[2] + M[0][3] * x[3];
-    y[1] = M[1][0] * x[0] + M[1][1] * x[1] + M[1][2] * x[2] + M[1][3] * x[3];
-    y[2] = dir == DIR_X ? M[2][0] * x[0] + M[2][2] * x[2] : M[2][0] * x[0] + M[2][1] * x[1];
-    y[3] = M[3][0] * x[0] + M[3][1] * x[1] + M[3][2] * x[2] + M[3][3] * x[3];
-#else
-    (void)dir;
-    mat4_vec4(M, x, y);
-#endif
+    const T g1 = gam - T(1), u = s.u, v = s.v, ic = s.ic;
+    const T h3 = T(0.5) * g1 * (ic * ic);
+    LeftRows<T> L;
+    L.a = T(0.5) * (u * u + v * v) * h3;
+    L.b = (dir == DIR_X ? u : v) * (T(0.5) * ic);
+    L.hu = h3 * u;
+    L.hv = h3 * v;
+    L.h3 = h3;
+    L.hr = T(0.5) * (s.irho * ic);
+    L.irho = s.irho;
+    L.wt = dir == DIR_X ? v : u;
+    return L;
 }
-template <typename T> WX_HD void mat4_vec4_R(int dir, const T M[4][4], const T x[4], T y[4])
+template <typename T> WX_HD void left_project(int dir, const LeftRows<T>& L, const T x[4], T y[4])
 {
-#if WX_FAST
-    y[0] = x[0] + x[1] + x[3];
-    if (dir == DIR_X) {
-        y[1] = M[1][0] * x[0] + M[1][1] * x[1] + M[1][3] * x[3];
-        y[2] = M[2][0] * x[0] + M[2][1] * x[1] + M[2][2] * x[2] + M[2][3] * x[3];
-    } else {
-        y[1] = M[1][0] * x[0] + M[1][1] * x[1] + M[1][2] * x[2] + M[1][3] * x[3];
-        y[2] = M[2][0] * x[0] + M[2][1] * x[1] + M[2][3] * x[3];
-    }
-    y[3] = M[3][0] * x[0] + M[3][1] * x[1] + M[3][2] * x[2] + M[3][3] * x[3];
-#else
-    (void)dir;
-    mat4_vec4(M, x, y);
-#endif
+    const T cn = dir == DIR_X ? x[1] : x[2];
+    const T ct = dir == DIR_X ? x[2] : x[1];
+    T A = L.a * x[0] - L.hu * x[1] - L.hv * x[2] + L.h3 * x[3];
+    T B = L.b * x[0] - L.hr * cn;
+    y[0] = A + B;
+    y[1] = x[0] - T(2) * A;
+    y[2] = L.irho * (ct - L.wt * x[0]);
+    y[3] = A - B;
+}
+template <typename T> struct RightCols { T wn, wt, h, c, rho, kmh, wnc, rwt; };
+template <typename T> WX_HD RightCols<T> right_cols(int dir, const Roe<T>& s)
+{
+    RightCols<T> R;
+    R.wn = dir == DIR_X ? s.u : s.v;
+    R.wt = dir == DIR_X ? s.v : s.u;
+    R.h = s.h;
+    R.c = s.c;
+    R.rho = s.rho;
+    R.kmh = T(0.5) * (s.u * s.u + s.v * s.v) - s.h;
+    R.wnc = R.wn * s.c;
+    R.rwt = s.rho * R.wt;
+    return R;
+}
+template <typename T> WX_HD void right_project(int dir, const RightCols<T>& R, const T x[4], T y[4])
+{
+    T y0 = x[0] + x[1] + x[3];
+    T d30 = x[3] - x[0];
+    T yn = R.wn * y0 + R.c * d30;
+    T yt = R.wt * y0 + R.rho * x[2];
+    y[0] = y0;
+    y[1] = dir == DIR_X ? yn : yt;
+    y[2] = dir == DIR_X ? yt : yn;
+    y[3] = R.h * y0 + R.kmh * x[1] + R.wnc * d30 + R.rwt * x[2];
 }
 
 // ---------------------------------------------------------------------
@@ -589,20 +588,32 @@ template <typename T> WX_HD void recon_normal(int dir, bool cons, const T c[6][4
         return;
     }
     Roe<T> s = roe_avg(c[2], c[3], gam);
+    T w[6][4];
+#if WX_FAST
+    const LeftRows<T> L = left_rows(dir, s, gam);
+#pragma unroll
+    for (int t = 0; t < 6; ++t) left_project(dir, L, c[t], w[t]);
+#else
     T L[4][4], R[4][4];
     eig_left(dir, s, gam, L);
     eig_right(dir, s, R);
-    T w[6][4];
 #pragma unroll
-    for (int t = 0; t < 6; ++t) mat4_vec4_L(dir, L, c[t], w[t]);
+    for (int t = 0; t < 6; ++t) mat4_vec4(L, c[t], w[t]);
+#endif
     T chl[4], chr[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         chl[k] = weno5_left(w[0][k], w[1][k], w[2][k], w[3][k], w[4][k]);
         chr[k] = weno5_right(w[1][k], w[2][k], w[3][k], w[4][k], w[5][k]);
     }
-    mat4_vec4_R(dir, R, chl, ql);
-    mat4_vec4_R(dir, R, chr, qr);
+#if WX_FAST
+    const RightCols<T> R = right_cols(dir, s);
+    right_project(dir, R, chl, ql);
+    right_project(dir, R, chr, qr);
+#else
+    mat4_vec4(R, chl, ql);
+    mat4_vec4(R, chr, qr);
+#endif
     pp_limit(ql, c[2], gam);
     pp_limit(qr, c[3], gam);
 }
@@ -662,9 +673,8 @@ WX_HD void recon_normal_dev(int dir, bool cons, const double c[6][4], double gam
         base[3] -= kap * (dir == DIR_X ? s.u : s.v);
     }
     Roe<float> sf{(float)s.rho, (float)s.u, (float)s.v, (float)s.h, (float)s.c, (float)s.ic, (float)s.irho};
-    float L[4][4], R[4][4];
-    eig_left<float>(dir, sf, (float)gam, L);
-    eig_right<float>(dir, sf, R);
+    const LeftRows<float> L = left_rows<float>(dir, sf, (float)gam);
+    const RightCols<float> R = right_cols<float>(dir, sf);
     float w[6][4];
 #pragma unroll
     for (int t = 0; t < 6; ++t) {
@@ -672,7 +682,7 @@ WX_HD void recon_normal_dev(int dir, bool cons, const double c[6][4], double gam
 #pragma unroll
             for (int k = 0; k < 4; ++k) w[t][k] = 0.0f;
         } else {
-            mat4_vec4_L<float>(dir, L, d[t], w[t]);
+            left_project<float>(dir, L, d[t], w[t]);
         }
     }
     float chl[4], chr[4], el[4], er[4];
@@ -681,8 +691,8 @@ WX_HD void recon_normal_dev(int dir, bool cons, const double c[6][4], double gam
         chl[k] = weno5_left<float>(w[0][k], w[1][k], 0.0f, w[3][k], w[4][k]);
         chr[k] = weno5_right<float>(w[1][k], 0.0f, w[3][k], w[4][k], w[5][k]);
     }
-    mat4_vec4_R<float>(dir, R, chl, el);
-    mat4_vec4_R<float>(dir, R, chr, er);
+    right_project<float>(dir, R, chl, el);
+    right_project<float>(dir, R, chr, er);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         ql[k] = base[k] + (double)el[k];
