@@ -23,7 +23,8 @@ Before timing, the sweep tile lengths are picked by the solver's autotuner
 throughput on the same workload with its relative L1 density error against
 the FP64 run after the same steps, the 4096^2 four-quadrant grid, per-kernel
 device times from HIP events (the roofline block), and the CPU oracle timed
-on the host (cpu_baseline).
+on the host (cpu_baseline: all of the process's cores up to 16, with the
+single-thread rate of the reference's own threading beside it).
 """
 import argparse
 import json
@@ -134,14 +135,12 @@ def run_batch(w, steps, warmup, precision, chunk=16):
                 ms_per_step=el / steps * 1e3, non_finite=len(bad), chunk=chunk)
 
 
-def cpu_baseline(q, dx, w, budget_s=12.0):
-    """The oracle (C restatement of the reference Rust RHS, 1 thread) driving
-    the reference's SSP-RK3 stage combination on the same state; whole steps
-    until ~budget_s elapsed (at least one)."""
+def _cpu_steps(q, dx, w, budget_s):
+    """Whole SSP-RK3 steps of the oracle RHS (the reference's stage
+    combination in numpy) until ~budget_s elapsed (at least one)."""
     import numpy as np
 
     import oracle
-    oracle.lib()
     nx, ny = w["nx"], w["ny"]
     sl = np.s_[3:nx + 3, 3:ny + 3, :4]
     q = q.copy()
@@ -158,10 +157,45 @@ def cpu_baseline(q, dx, w, budget_s=12.0):
         steps += 1
         if time.perf_counter() - t0 > budget_s:
             break
-    el = time.perf_counter() - t0
-    return dict(value=nx * ny * steps / el, unit="cell-updates/s", cores=1, kind="port",
+    return steps, time.perf_counter() - t0
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(q, dx, w, budget_s=10.0):
+    """The oracle (C restatement of the reference Rust RHS) driving the
+    reference's SSP-RK3 stage combination on the same state, timed on the
+    host: on all of this process's cores (up to 16, OpenMP over the RHS
+    passes' outer loops; the reported figure) and on one thread as the
+    reference runs (reported beside it)."""
+    import oracle
+    oracle.lib()
+    nx, ny = w["nx"], w["ny"]
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    res = {}
+    try:
+        for t in sorted({cores, 1}, reverse=True):
+            oracle.set_threads(t)
+            res[t] = _cpu_steps(q, dx, w, budget_s)
+    finally:
+        oracle.set_threads(1)
+    steps, el = res[cores]
+    s1, e1 = res[1]
+    return dict(value=nx * ny * steps / el, unit="cell-updates/s", cores=cores, kind="port",
                 sample=f"{steps} SSP-RK3 step(s) of the {nx}x{ny} {w['name']} state "
-                       f"(3 oracle l_local calls + stage combines each), {el:.1f} s")
+                       f"(3 oracle l_local calls + numpy stage combines each) on {cores} thread(s), "
+                       f"{el:.1f} s",
+                one_thread=dict(value=nx * ny * s1 / e1, steps=s1, seconds=round(e1, 2)),
+                host=_cpu_model())
 
 
 def dropin_rate(q, dx, w, calls=5):
@@ -387,6 +421,18 @@ def main():
                                "chunks_fp64": rq.get("chunks"), "chunks_fp32split": rq32.get("chunks"),
                                "roofline_fp64": roofline(rq, wq),
                                "roofline_fp32split": roofline(rq32, wq, split=True)}
+            # the scheme variants of the same grid (FP64, the tiling tuned above):
+            # {characteristic, component-wise} x {HLLC (phi = 1), HLLC/HLLE blend jp (0.5, 2)}
+            variants = {}
+            for cons in (False, True):
+                for jp, tag in (((1e4, 1.1e4), "hllc"), ((0.5, 2.0), "blend_jp0.5_2")):
+                    wv = dict(wq, cons=cons, jp=jp)
+                    rv = run_gpu(wv, qq, dq, 10, 2, "fused", args.variant, "fp64", profile=False,
+                                 chunks=rq.get("chunks"))
+                    variants[("componentwise_" if cons else "characteristic_") + tag] = {
+                        "value": rv["value"], "ms_per_step": rv["seconds"] / rv["steps"] * 1e3,
+                        "state_finite": rv["finite"]}
+            res["quad4096"]["scheme_variants_fp64"] = variants
     if not args.no_extra and args.workload == "dmr":
         res["dropin_l_local"] = dropin_rate(q, dx, w)
     if not args.no_cpu:

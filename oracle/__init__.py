@@ -64,12 +64,19 @@ def lib():
         L.orc_test_weno5.argtypes = [dp, dp, dp]
         L.orc_test_gq.argtypes = [dp, dp, dp]
         L.orc_test_riemann_pt.argtypes = [i, i, dp, dp, dp, dp, d, dp]
-        for f in (L.orc_flux, L.orc_con2primi, L.orc_weno, L.orc_riemann,
+        L.orc_set_threads.argtypes = [i]
+        for f in (L.orc_set_threads, L.orc_flux, L.orc_con2primi, L.orc_weno, L.orc_riemann,
                   L.orc_sensor, L.orc_blend, L.orc_l_local, L.orc_test_roe_eig,
                   L.orc_test_pp_limit, L.orc_test_weno5, L.orc_test_gq, L.orc_test_riemann_pt):
             f.restype = None
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """Threads of the oracle's outer loops (1, the default, is the
+    reference's single thread; results do not depend on n)."""
+    lib().orc_set_threads(int(n))
 
 
 def _view(a: np.ndarray) -> _View3:

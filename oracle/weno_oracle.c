@@ -1,11 +1,16 @@
 /*
  * weno_oracle.c -- CPU oracle for the 2D WENO5 / HLLC right-hand side.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is a plain-C, single-threaded FP64
- * restatement of the reference Rust crate `weno_ext` (weno_ext/src/utils.rs,
- * weno_ext/src/weno.rs, weno_ext/src/riemann.rs).  It is the checker that the
- * GPU path is compared against and the "port" CPU baseline that bench.py times.
- * Nothing in the product path links, loads or calls it.
+ * TEST INFRASTRUCTURE ONLY.  This file is a plain-C FP64 restatement of the
+ * reference Rust crate `weno_ext` (weno_ext/src/utils.rs, weno_ext/src/weno.rs,
+ * weno_ext/src/riemann.rs).  It is the checker that the GPU path is compared
+ * against and the "port" CPU baseline that bench.py times.  Nothing in the
+ * product path links, loads or calls it.
+ *
+ * Single-threaded by default, like the reference.  orc_set_threads(n) splits
+ * the outer loops of the heavy passes over n OpenMP threads (a multi-core CPU
+ * comparator); every output element is computed by the same arithmetic in
+ * either case, so the results do not depend on n.
  *
  * Semantics that are kept on purpose (they decide bit-level parity):
  *   - same passes, loop bounds, temporaries and operation order as the Rust;
@@ -25,6 +30,15 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* threads of the outer loops (1 = the reference's single thread) */
+static int orc_nthreads = 1;
+void orc_set_threads(int n) { orc_nthreads = n > 0 ? n : 1; }
+#ifdef _OPENMP
+#define ORC_PAR _Pragma("omp parallel for num_threads(orc_nthreads) schedule(static)")
+#else
+#define ORC_PAR
+#endif
 
 typedef struct {
     const double *p;
@@ -91,6 +105,7 @@ void orc_flux(const orc_view3 *q, double gamma, int dir, double *out)
 void orc_con2primi(const orc_view3 *q, double gamma, double *p, double *a,
                    double *rho_s, double *u, double *v, double *h)
 {
+    ORC_PAR
     for (int64_t i = 0; i < q->d0; ++i)
         for (int64_t j = 0; j < q->d1; ++j) {
             int64_t o = i * q->d1 + j;
@@ -302,6 +317,7 @@ void orc_weno(const orc_view3 *u, double gamma, int dir, int cons,
     const int64_t on = dir == 0 ? n : n - 5;
     double *pl = zalloc(om * on * 4);
     double *pr = zalloc(om * on * 4);
+    ORC_PAR
     for (int64_t i = 0; i < om; ++i)
         for (int64_t j = 0; j < on; ++j) {
             double cell[6][4];
@@ -347,6 +363,7 @@ void orc_weno(const orc_view3 *u, double gamma, int dir, int cons,
     const int64_t n1 = dir == 0 ? on - 6 : on;
     double *l1 = zalloc(m1 * n1 * 4), *l2 = zalloc(m1 * n1 * 4);
     double *r1 = zalloc(m1 * n1 * 4), *r2 = zalloc(m1 * n1 * 4);
+    ORC_PAR
     for (int64_t i = 0; i < m1; ++i)
         for (int64_t j = 0; j < n1; ++j)
             for (int k = 0; k < 4; ++k) {
@@ -360,6 +377,7 @@ void orc_weno(const orc_view3 *u, double gamma, int dir, int cons,
             }
 
     /* fluxes at the Gauss points, averaged (weno.rs:492-501 / :622-631) */
+    ORC_PAR
     for (int64_t i = 0; i < m1; ++i)
         for (int64_t j = 0; j < n1; ++j) {
             double fa[4], fb[4], fc[4], fd[4];
@@ -477,6 +495,7 @@ static void hlle_pt(int dir, const double qlr[4], const double flv[4], const dou
 void orc_riemann(const orc_view3 *ql, const orc_view3 *fl, const orc_view3 *qr,
                  const orc_view3 *fr, double gamma, int dir, int kind, double *out)
 {
+    ORC_PAR
     for (int64_t i = 0; i < ql->d0; ++i)
         for (int64_t j = 0; j < ql->d1; ++j) {
             double a[4], b[4], c[4], d[4], o[4];
@@ -510,6 +529,7 @@ void orc_sensor(const double *p, int64_t m, int64_t n, double lo, double hi, dou
         dpdy[i * n] = (p[i * n + 1] - p[i * n]) / (2.0 * dx);
         dpdy[i * n + n - 1] = (p[i * n + n - 1] - p[i * n + n - 2]) / (2.0 * dx);
     }
+    ORC_PAR
     for (int64_t o = 0; o < m * n; ++o) {
         double grad = sqrt(sq(dpdx[o]) + sq(dpdy[o]));
         double jp = grad / fmax(p[o], ORC_EPS);
@@ -539,6 +559,7 @@ void orc_blend(const orc_view3 *ql, const orc_view3 *fl, const orc_view3 *qr,
     double *fc = zalloc(m * n * 4), *fe = zalloc(m * n * 4);
     orc_riemann(ql, fl, qr, fr, gamma, dir, 0, fc);
     orc_riemann(ql, fl, qr, fr, gamma, dir, 1, fe);
+    ORC_PAR
     for (int64_t o = 0; o < m * n; ++o) {
         double w = phi[o];
         for (int k = 0; k < 4; ++k) out[o * 4 + k] = w * fc[o * 4 + k] + (1.0 - w) * fe[o * 4 + k];
@@ -573,6 +594,7 @@ void orc_l_local(const orc_view3 *u, double dx, double gamma, int force_hlle,
     }
     const int64_t nx = ax - 1, ny = by - 1;
     double *dfdx = zalloc(nx * bx * 4), *dgdy = zalloc(ay * ny * 4);
+    ORC_PAR
     for (int64_t i = 0; i < nx; ++i)
         for (int64_t j = 0; j < bx; ++j)
             for (int k = 0; k < 4; ++k)

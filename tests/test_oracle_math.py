@@ -23,6 +23,7 @@ term, a wrong sign, index or weight, a transposed operand) fails it:
   density and pressure above the floors.
 * Whole RHS: free stream gives 0; y-uniform data gives no y-flux difference.
 * Drivers: one rk3_cc step on dU/dt = z U has amplification 1 + z + z^2/2 + z^3/6.
+* Threads: the OpenMP split of the oracle's loops leaves the RHS bitwise unchanged.
 """
 import numpy as np
 import pytest
@@ -444,3 +445,22 @@ def test_mirror_bc(oracle_mod):
         np.testing.assert_array_equal(q[11 - g], q[6 + g])
         np.testing.assert_array_equal(q[:, g], q[:, 5 - g])
         np.testing.assert_array_equal(q[:, 10 - g], q[:, 5 + g])
+
+
+def test_threads_do_not_change_results(oracle_mod):
+    """The multi-core comparator (orc_set_threads) splits outer loops only:
+    the whole RHS is bitwise the same on 1 and 4 threads."""
+    rng = np.random.default_rng(11)
+    q = np.zeros((40, 34, 8))
+    q[..., 0] = 1.0 + 0.5 * rng.random(q.shape[:2])
+    q[..., 1] = 0.3 * rng.standard_normal(q.shape[:2])
+    q[..., 2] = 0.3 * rng.standard_normal(q.shape[:2])
+    q[..., 3] = 2.5 + rng.random(q.shape[:2])
+    out = {}
+    try:
+        for t in (1, 4):
+            oracle_mod.set_threads(t)
+            out[t] = oracle_mod.l_local_py(q, 0.02, G, False, (0.5, 2.0), False)
+    finally:
+        oracle_mod.set_threads(1)
+    assert np.array_equal(out[1], out[4])
