@@ -1,5 +1,6 @@
-"""CPU checks of the host-side initial data and closed-form solutions."""
+"""CPU checks of the host-side initial data, closed-form solutions and driver helpers."""
 import numpy as np
+import pytest
 
 from wenox import problems
 
@@ -78,3 +79,19 @@ def test_reference_dmr_needs_its_own_cfl(oracle_mod):
     are made at CFL 0.1."""
     assert _reference_dmr_finite(oracle_mod, 0.1, 70)
     assert not _reference_dmr_finite(oracle_mod, 0.4, 70)
+
+
+def test_snapshot_writer(tmp_path):
+    """The drivers' background writer saves every frame it is given and
+    reports a failed write to the stepping thread."""
+    from wenox.drivers import _SnapshotWriter
+    w = _SnapshotWriter(depth=2)
+    for i in range(6):
+        w.put(str(tmp_path / f"{i}.npy"), np.full((4, 3, 8), float(i)))
+    w.close()
+    for i in range(6):
+        assert np.load(tmp_path / f"{i}.npy")[0, 0, 0] == float(i)
+    w = _SnapshotWriter()
+    w.put(str(tmp_path / "missing_dir" / "0.npy"), np.zeros(3))
+    with pytest.raises(OSError):
+        w.close()

@@ -693,27 +693,25 @@ extern "C" int wenox_solver_download(wenox_solver* s, double* q8)
     const int64_t m = d.nx + 6, n = d.ny + 6;
     const size_t P = (size_t)(m * n);
     HIPCHK(hipSetDevice(s->device));
-    double* w = s->work;
     const double* state = s->U;
     if (s->split) {
-        // FP64 value hi + lo of every cell, after the six primitive planes
-        s->sops->from_split((const float*)s->U, w + 6 * P, (int64_t)P, s->stream);
-        state = w + 6 * P;
+        // FP64 value hi + lo of every cell
+        s->sops->from_split((const float*)s->U, s->work + 6 * P, (int64_t)P, s->stream);
+        state = s->work + 6 * P;
     }
-    s->ops->con2primi(state, lay_soa4(m, n), m, n, d.gamma, w, w + P, w + 2 * P, w + 3 * P,
-                      w + 4 * P, w + 5 * P, s->stream);
-    std::vector<double> cons(P * 4), prim(P * 6);
-    HIPCHK(hipMemcpyAsync(cons.data(), state, P * 4 * 8, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(hipMemcpyAsync(prim.data(), w, P * 6 * 8, hipMemcpyDeviceToHost, s->stream));
+    // the (m, n, 8) array is formed on the device in row blocks staged in the
+    // stage-1 buffer A (4 P doubles of space, free between steps) and copied
+    // straight into the caller's array
+    double* stage = (double*)s->A;
+    const int64_t rows_per = std::max<int64_t>(1, (int64_t)(4 * P) / (8 * n));
+    for (int64_t r0 = 0; r0 < m; r0 += rows_per) {
+        const int64_t rows = std::min(rows_per, m - r0);
+        s->ops->snapshot8(state, m, n, d.gamma, r0, rows, stage, s->stream);
+        HIPCHK(hipMemcpyAsync(q8 + r0 * n * 8, stage, (size_t)(rows * n) * 8 * 8, hipMemcpyDeviceToHost,
+                              s->stream));
+    }
     HIPCHK(hipStreamSynchronize(s->stream));
-    for (size_t c = 0; c < P; ++c) {
-        double* o = q8 + c * 8;
-        for (int k = 0; k < 4; ++k) o[k] = cons[k * P + c];
-        o[4] = prim[2 * P + c];   // rho_safe
-        o[5] = prim[3 * P + c];   // u
-        o[6] = prim[4 * P + c];   // v
-        o[7] = prim[0 * P + c];   // p
-    }
+    HIPCHK(hipGetLastError());
     if (d.bc == WENOX_BC_DMR_REF) dmr_ref_fill8(q8, d.nx, d.ny, d.dmr_bound);
     return WENOX_OK;
 }

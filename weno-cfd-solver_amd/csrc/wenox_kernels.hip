@@ -72,6 +72,23 @@ __global__ void k_con2primi(Acc q, int64_t M, int64_t N, double gam, double* p, 
     p[t] = r.p; a[t] = r.a; rho[t] = r.rho; u[t] = r.u; v[t] = r.v; h[t] = r.h;
 }
 
+// Snapshot rows [row0, row0 + rows) of the padded SoA4 state as the reference
+// drivers' (m, n, 8) array: conserved channels, then rho_safe, u, v, p of
+// con2primi (Rie_prob_2d_rs.py:157-163), one cell (64 B) per thread.
+__global__ void k_snapshot8(const double* U, int64_t P, int64_t n, double gam, int64_t row0, int64_t cells,
+                            double* out)
+{
+    int64_t t = (int64_t)blockIdx.x * TB + threadIdx.x;
+    if (t >= cells) return;
+    const int64_t o = row0 * n + t;
+    double c[4];
+    for (int k = 0; k < 4; ++k) c[k] = U[k * P + o];
+    Prim<double> r = con2primi_pt(c, gam);
+    double* d = out + t * 8;
+    d[0] = c[0]; d[1] = c[1]; d[2] = c[2]; d[3] = c[3];
+    d[4] = r.rho; d[5] = r.u; d[6] = r.v; d[7] = r.p;
+}
+
 // ------------------------------------------------------------------------
 // weno.rs:385-454 (characteristic) / :654-717 (component-wise): pass 1.
 // x: face i between cells i+2 and i+3 of column j, (m-5, n) faces.
@@ -228,6 +245,14 @@ static void h_con2primi(const double* q, Lay lq, int64_t M, int64_t N, double ga
     k_con2primi<<<blocks_for(M * N), TB, 0, s>>>(Acc{q, lq}, M, N, gam, p, a, rho, u, v, h);
 }
 
+static void h_snapshot8(const double* U, int64_t m, int64_t n, double gam, int64_t row0, int64_t rows,
+                        double* out, hipStream_t s)
+{
+    const int64_t cells = rows * n;
+    if (cells <= 0) return;
+    k_snapshot8<<<blocks_for(cells), TB, 0, s>>>(U, m * n, n, gam, row0, cells, out);
+}
+
 static void h_weno(const double* u, Lay lu, int64_t m, int64_t n, int dir, int cons, double gam,
                    double* pl, double* pr, double* ql, double* fl, double* qr, double* fr, Lay lo,
                    hipStream_t s)
@@ -369,7 +394,7 @@ static void h_rk(const double* Un, const double* Us, const double* L, double* ou
 
 #ifndef __HIP_DEVICE_COMPILE__
 const Ops WX_OPS = {
-    WX_NS::h_flux, WX_NS::h_con2primi, WX_NS::h_weno, WX_NS::h_riemann, WX_NS::h_divergence,
+    WX_NS::h_flux, WX_NS::h_con2primi, WX_NS::h_snapshot8, WX_NS::h_weno, WX_NS::h_riemann, WX_NS::h_divergence,
     WX_NS::h_bc,   WX_NS::h_cfl,       WX_NS::h_advance, WX_NS::h_rhs_unfused, WX_NS::h_rk,
 };
 

@@ -65,6 +65,10 @@ struct Ops {
                  Lay lo, hipStream_t s);
     void (*con2primi)(const double* q, Lay lq, int64_t M, int64_t N, double gam, double* p,
                       double* a, double* rho, double* u, double* v, double* h, hipStream_t s);
+    // rows [row0, row0 + rows) of a padded SoA4 state (m x n planes) as
+    // (rows, n, 8): conserved channels, then rho_safe, u, v, p
+    void (*snapshot8)(const double* U, int64_t m, int64_t n, double gam, int64_t row0, int64_t rows,
+                      double* out, hipStream_t s);
     // pass 1 (normal) into pl/pr, then pass 2 (transverse Gauss points) and the
     // averaged fluxes into ql/fl/qr/fr.  u is (m, n); pl/pr are SoA4 of
     // (m-5, n) for x or (m, n-5) for y; outputs are (F1, F2) in layout lo.

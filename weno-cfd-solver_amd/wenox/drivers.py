@@ -6,16 +6,57 @@ same dictionaries, loop control (t_final primary, nstep as a cap; CFL dt
 clipped to the remaining time) and snapshot format
 (``data/<YYYY-mm-dd_HH-MM-SS>/{step}.npy`` of the (nx+6, ny+6, 8) array and
 ``time.npy``), so ``plot_utils.interactive_plot_keyboard`` reads the output.
-Every step runs on the GPU; snapshots are the only host copies.
+Every step runs on the GPU; snapshots are the only host copies: the (nx+6,
+ny+6, 8) array is formed on the device and written to disk by a background
+thread while the next steps run.
 """
 from __future__ import annotations
 
 import datetime
 import os
+import queue
+import threading
 
 import numpy as np
 
 from . import Solver, problems
+
+
+class _SnapshotWriter:
+    """Writes snapshot files on a background thread (numpy releases the GIL
+    while it writes), so the next GPU steps and downloads overlap the disk
+    writes.  At most `depth` frames wait in memory; errors surface on put()
+    or close()."""
+
+    def __init__(self, depth=2):
+        self._q = queue.Queue(maxsize=depth)
+        self._err = None
+        self._t = threading.Thread(target=self._work, daemon=True)
+        self._t.start()
+
+    def _work(self):
+        while True:
+            item = self._q.get()
+            if item is None:
+                return
+            if self._err is None:
+                try:
+                    np.save(*item)
+                except BaseException as e:  # reported to the stepping thread
+                    self._err = e
+
+    def _check(self):
+        if self._err is not None:
+            raise self._err
+
+    def put(self, path, array):
+        self._check()
+        self._q.put((path, array))
+
+    def close(self):
+        self._q.put(None)
+        self._t.join()
+        self._check()
 
 
 def _run(solver, q0, control_dict, nx, ny, snapshot_dir):
@@ -27,15 +68,18 @@ def _run(solver, q0, control_dict, nx, ny, snapshot_dir):
     solver.upload(q0)
     t_list = [0.0]
     if snapshot_dir:
-        np.save(os.path.join(snapshot_dir, "0.npy"), q0)
-    if snapshot_dir:
         # one step per call so every state can be written (Rie_prob_2d_rs.py:342-343)
-        for step in range(1, max_steps + 1):
-            T, k = solver.run(nstep=1, t_final=t_final)
-            if k == 0:
-                break
-            np.save(os.path.join(snapshot_dir, f"{step}.npy"), solver.download())
-            t_list.append(T)
+        writer = _SnapshotWriter()
+        try:
+            writer.put(os.path.join(snapshot_dir, "0.npy"), np.array(q0, dtype=np.float64, copy=True))
+            for step in range(1, max_steps + 1):
+                T, k = solver.run(nstep=1, t_final=t_final)
+                if k == 0:
+                    break
+                writer.put(os.path.join(snapshot_dir, f"{step}.npy"), solver.download())
+                t_list.append(T)
+        finally:
+            writer.close()
         np.save(os.path.join(snapshot_dir, "time.npy"), np.array(t_list))
     else:
         solver.run(nstep=max_steps, t_final=t_final)
