@@ -415,15 +415,20 @@ struct wenox_solver {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double* U = nullptr;     // U^n, padded SoA4
+    // State buffers: padded SoA4 planes with row pitch ld (U, A, B) or ldT
+    // (their transposed copies), each starting STATE_SHIFT elements into its
+    // allocation mem[] (see wenox_ops.h).
+    double* U = nullptr;     // U^n
     double* A = nullptr;     // stage 1 output
     double* B = nullptr;     // stage 2 output
     double* UT = nullptr;    // transposed copies (fused engine only)
     double* AT = nullptr;
     double* BT = nullptr;
-    double* L = nullptr;     // RHS (unfused) / Dx (fused), interior SoA4
-    double* DyT = nullptr;   // fused: y flux differences, transposed interior
-    double* work = nullptr;  // unfused face arrays, download scratch
+    void* mem[6] = {};
+    int64_t ld = 0, ldT = 0;
+    double* L = nullptr;     // RHS (unfused, nx*ny SoA4) / Dx (fused, row pitch plan.ldx)
+    double* DyT = nullptr;   // fused: y flux differences, transposed interior (row pitch plan.ldyT)
+    double* work = nullptr;  // unfused face arrays, RHS output, FP64 copy of a split state
     size_t work_n = 0;
     StepCtl* ctl = nullptr;  // device
     double* cfl_part = nullptr;   // device, CFL_SLOTS per-block maxima
@@ -441,7 +446,11 @@ struct wenox_solver {
 
 namespace {
 
-int64_t padded(const wenox_solver* s) { return (s->d.nx + 6) * (s->d.ny + 6); }
+// plane strides of the pitched state and of its transposed copy
+int64_t plane(const wenox_solver* s) { return (s->d.nx + 6) * s->ld; }
+int64_t plane_t(const wenox_solver* s) { return (s->d.ny + 6) * s->ldT; }
+// FP64 view of work with the state geometry (the split download's staging copy)
+double* work_state(const wenox_solver* s) { return s->work + STATE_SHIFT; }
 
 // One time step on the solver's stream: CFL -> 3 x (RHS, RK combine, ghost fill) -> advance.
 void enqueue_step(wenox_solver* s)
@@ -454,17 +463,17 @@ void enqueue_step(wenox_solver* s)
         float* dst[3] = {(float*)s->A, (float*)s->B, (float*)s->U};
         float* srcT[3] = {(float*)s->UT, (float*)s->AT, (float*)s->BT};
         float* dstT[3] = {(float*)s->AT, (float*)s->BT, (float*)s->UT};
-        s->sops->cfl_split((const float*)s->U, d.nx, d.ny, d.gamma, s->ctl, st);
+        s->sops->cfl_split((const float*)s->U, d.nx, d.ny, s->ld, d.gamma, s->ctl, st);
         for (int stage = 1; stage <= 3; ++stage) {
             const int i = stage - 1;
             s->sops->stage_split(s->plan, src[i], srcT[i], (const float*)s->U, (float*)s->L,
                                  (float*)s->DyT, dst[i], dstT[i], s->flags, stage, s->ctl, st);
-            s->sops->bc_split(dst[i], dstT[i], d.nx, d.ny, s->bp, s->ctl, stage, st);
+            s->sops->bc_split(dst[i], dstT[i], d.nx, d.ny, s->ld, s->ldT, s->bp, s->ctl, stage, st);
         }
         o->advance(s->ctl, st);
         return;
     }
-    o->cfl(s->U, d.nx, d.ny, d.gamma, s->ctl, st);
+    o->cfl(s->U, d.nx, d.ny, s->ld, d.gamma, s->ctl, st);
     double* src[3] = {s->U, s->A, s->B};
     double* dst[3] = {s->A, s->B, s->U};
     double* srcT[3] = {s->UT, s->AT, s->BT};
@@ -475,11 +484,11 @@ void enqueue_step(wenox_solver* s)
             s->sops->stage(s->plan, src[i], srcT[i], s->U, s->L, s->DyT, dst[i], dstT[i], stage,
                            s->ctl, st);
         } else {
-            o->rhs_unfused(src[i], d.nx, d.ny, d.dx, d.gamma, d.cons_switch, d.force_hlle,
+            o->rhs_unfused(src[i], d.nx, d.ny, s->ld, d.dx, d.gamma, d.cons_switch, d.force_hlle,
                            d.jp_lo, d.jp_hi, s->work, s->L, s->ctl, st);
-            o->rk(s->U, src[i], s->L, dst[i], d.nx, d.ny, stage, s->ctl, st);
+            o->rk(s->U, src[i], s->L, dst[i], d.nx, d.ny, s->ld, stage, s->ctl, st);
         }
-        o->bc(dst[i], dstT[i], d.nx, d.ny, s->bp, s->ctl, stage, st);
+        o->bc(dst[i], dstT[i], d.nx, d.ny, s->ld, s->ldT, s->bp, s->ctl, stage, st);
     }
     o->advance(s->ctl, st);
 }
@@ -566,34 +575,47 @@ extern "C" int wenox_solver_create(const wenox_solver_desc* desc, wenox_solver**
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
         return cleanup(fail(WENOX_E_DEVICE, "stream/event creation failed"));
-    const size_t P = (size_t)padded(s) * 4, I = (size_t)(d.nx * d.ny) * 4;
+    const size_t I = (size_t)(d.nx * d.ny) * 4;
     const bool fused = d.engine == WENOX_ENGINE_FUSED;
-    s->work_n = std::max(rhs_unfused_scratch(d.nx, d.ny), (size_t)padded(s) * 6);
+    s->ld = row_pitch(d.ny + 6);
+    s->ldT = row_pitch(d.nx + 6);
+    // doubles per state buffer: four pitched planes plus the shift and its tail
+    const size_t SP = (size_t)plane(s) * 4 + ROW_ALIGN, SPT = (size_t)plane_t(s) * 4 + ROW_ALIGN;
+    size_t nL = I, nDyT = 4;
+    s->work_n = std::max(rhs_unfused_scratch(d.nx, d.ny), I);
     if (fused) {
         s->plan = make_sweep_plan(d.nx, d.ny, d.dx, d.gamma, d.cons_switch, d.force_hlle, d.jp_lo, d.jp_hi);
         s->sops = d.variant == WENOX_FAST ? &sweep_ops_fast : &sweep_ops_exact;
-        s->work_n = (size_t)padded(s) * 6;
+        nL = std::max(I, (size_t)(s->plan.PDx() * 4));
+        nDyT = (size_t)(s->plan.PDyT() * 4);
+        s->work_n = I;
         if (d.precision == WENOX_PREC_FP32SPLIT) {
             // the FP32 kernels are built in the fast variant only
             s->split = true;
             s->sops = &sweep_ops_fast;
-            s->work_n = (size_t)padded(s) * 10;
+            s->work_n = std::max(I, SP);
         }
     }
-    bool ok = hipMalloc(&s->U, P * 8) == hipSuccess && hipMalloc(&s->A, P * 8) == hipSuccess &&
-              hipMalloc(&s->B, P * 8) == hipSuccess &&
-              hipMalloc(&s->L, std::max<size_t>(I, 4) * 8) == hipSuccess &&
-              hipMalloc(&s->work, s->work_n * 8) == hipSuccess &&
+    // the shift is STATE_SHIFT elements: doubles, or the floats of a split state
+    const size_t shift_bytes = (size_t)STATE_SHIFT * (s->split ? sizeof(float) : sizeof(double));
+    auto state_buf = [&](int slot, double** base, size_t n_doubles) {
+        if (hipMalloc(&s->mem[slot], n_doubles * 8) != hipSuccess) return false;
+        *base = (double*)((char*)s->mem[slot] + shift_bytes);
+        return true;
+    };
+    bool ok = state_buf(0, &s->U, SP) && state_buf(1, &s->A, SP) && state_buf(2, &s->B, SP) &&
+              hipMalloc(&s->L, std::max<size_t>(nL, 4) * 8) == hipSuccess &&
+              hipMalloc(&s->work, std::max<size_t>(s->work_n, 4) * 8) == hipSuccess &&
               hipMalloc(&s->ctl, sizeof(StepCtl)) == hipSuccess &&
               hipMalloc(&s->cfl_part, CFL_SLOTS * sizeof(double)) == hipSuccess;
     if (ok && fused)
-        ok = hipMalloc(&s->UT, P * 8) == hipSuccess && hipMalloc(&s->AT, P * 8) == hipSuccess &&
-             hipMalloc(&s->BT, P * 8) == hipSuccess && hipMalloc(&s->DyT, std::max<size_t>(I, 4) * 8) == hipSuccess;
+        ok = state_buf(3, &s->UT, SPT) && state_buf(4, &s->AT, SPT) && state_buf(5, &s->BT, SPT) &&
+             hipMalloc(&s->DyT, std::max<size_t>(nDyT, 4) * 8) == hipSuccess;
     if (ok && s->split)
         ok = hipMalloc(&s->flags, (size_t)std::max<int64_t>(sweep_flag_count(s->plan), 1) * sizeof(int)) == hipSuccess;
     if (!ok)
         return cleanup(fail(WENOX_E_DEVICE, "device allocation failed (state of " +
-                                                std::to_string(((fused ? 6 : 3) * P + 2 * I + s->work_n) * 8 >> 20) +
+                                                std::to_string(((fused ? 3 * (SP + SPT) : 3 * SP) + nL + nDyT + s->work_n) * 8 >> 20) +
                                                 " MiB)"));
     s->bp.kind = d.bc;
     s->bp.dmr_bound = d.dmr_bound;
@@ -617,7 +639,9 @@ extern "C" void wenox_solver_destroy(wenox_solver* s)
     if (!s) return;
     if (s->exec) (void)hipGraphExecDestroy(s->exec);
     if (s->graph) (void)hipGraphDestroy(s->graph);
-    for (double* p : {s->U, s->A, s->B, s->UT, s->AT, s->BT, s->L, s->DyT, s->work})
+    for (void* p : s->mem)
+        if (p) (void)hipFree(p);
+    for (double* p : {s->L, s->DyT, s->work})
         if (p) (void)hipFree(p);
     if (s->ctl) (void)hipFree(s->ctl);
     if (s->cfl_part) (void)hipFree(s->cfl_part);
@@ -636,15 +660,17 @@ extern "C" int wenox_solver_upload(wenox_solver* s, const wenox_view3* q, double
     if (q->dim[0] != m || q->dim[1] != n || q->dim[2] < 4)
         return fail(WENOX_E_SHAPE, "state must be (nx+6, ny+6, c>=4), got " + dims(q));
     HIPCHK(hipSetDevice(s->device));
-    // pack to SoA4 in pinned memory
-    const size_t P = (size_t)(m * n);
+    // pack to the pitched SoA4 planes in pinned memory
+    const int64_t ld = s->ld;
+    const size_t P = (size_t)plane(s);
     double* pin = nullptr;
     HIPCHK(hipHostMalloc(&pin, P * 4 * sizeof(double)));
+    std::memset(pin, 0, P * 4 * sizeof(double));
     const int64_t s0 = q->stride[0], s1 = q->stride[1], s2 = q->stride[2];
     for (int64_t i = 0; i < m; ++i)
         for (int64_t j = 0; j < n; ++j) {
             const double* src = q->data + i * s0 + j * s1;
-            for (int k = 0; k < 4; ++k) pin[k * P + i * n + j] = src[k * s2];
+            for (int k = 0; k < 4; ++k) pin[k * P + i * ld + j] = src[k * s2];
         }
     StepCtl h{};
     h.cfl_dx = d.cfl * d.dx;
@@ -668,18 +694,18 @@ extern "C" int wenox_solver_upload(wenox_solver* s, const wenox_view3* q, double
         h.cmax0 = cm;
         h.use_cmax0 = 1;
     }
-    hipError_t e = hipMemcpy(s->split ? s->work : s->U, pin, P * 4 * sizeof(double), hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpy(s->split ? work_state(s) : s->U, pin, P * 4 * sizeof(double), hipMemcpyHostToDevice);
     (void)hipHostFree(pin);
     if (e != hipSuccess) return fail(WENOX_E_DEVICE, std::string("upload: ") + hipGetErrorString(e));
     int rc = write_ctl(s, &h);
     if (rc) return rc;
     if (s->split) {
-        s->sops->to_split(s->work, (float*)s->U, (int64_t)P, s->stream);
-        s->sops->transpose_split((const float*)s->U, (float*)s->UT, m, n, s->stream);
-        s->sops->bc_split((float*)s->U, (float*)s->UT, d.nx, d.ny, s->bp, s->ctl, 0, s->stream);
+        s->sops->to_split(work_state(s), (float*)s->U, (int64_t)P, s->stream);
+        s->sops->transpose_split((const float*)s->U, (float*)s->UT, m, n, ld, s->ldT, s->stream);
+        s->sops->bc_split((float*)s->U, (float*)s->UT, d.nx, d.ny, ld, s->ldT, s->bp, s->ctl, 0, s->stream);
     } else {
-        if (s->UT) s->sops->transpose(s->U, s->UT, m, n, s->stream);
-        s->ops->bc(s->U, s->UT, d.nx, d.ny, s->bp, s->ctl, 0, s->stream);
+        if (s->UT) s->sops->transpose(s->U, s->UT, m, n, ld, s->ldT, s->stream);
+        s->ops->bc(s->U, s->UT, d.nx, d.ny, ld, s->ldT, s->bp, s->ctl, 0, s->stream);
     }
     HIPCHK(hipStreamSynchronize(s->stream));
     HIPCHK(hipGetLastError());
@@ -691,13 +717,13 @@ extern "C" int wenox_solver_download(wenox_solver* s, double* q8)
     if (!s || !q8) return fail(WENOX_E_ARG, "wenox_solver_download: null argument");
     const auto& d = s->d;
     const int64_t m = d.nx + 6, n = d.ny + 6;
-    const size_t P = (size_t)(m * n);
+    const size_t P = (size_t)plane(s);
     HIPCHK(hipSetDevice(s->device));
     const double* state = s->U;
     if (s->split) {
         // FP64 value hi + lo of every cell
-        s->sops->from_split((const float*)s->U, s->work + 6 * P, (int64_t)P, s->stream);
-        state = s->work + 6 * P;
+        s->sops->from_split((const float*)s->U, work_state(s), (int64_t)P, s->stream);
+        state = work_state(s);
     }
     // the (m, n, 8) array is formed on the device in row blocks staged in the
     // stage-1 buffer A (4 P doubles of space, free between steps) and copied
@@ -706,7 +732,7 @@ extern "C" int wenox_solver_download(wenox_solver* s, double* q8)
     const int64_t rows_per = std::max<int64_t>(1, (int64_t)(4 * P) / (8 * n));
     for (int64_t r0 = 0; r0 < m; r0 += rows_per) {
         const int64_t rows = std::min(rows_per, m - r0);
-        s->ops->snapshot8(state, m, n, d.gamma, r0, rows, stage, s->stream);
+        s->ops->snapshot8(state, m, n, s->ld, d.gamma, r0, rows, stage, s->stream);
         HIPCHK(hipMemcpyAsync(q8 + r0 * n * 8, stage, (size_t)(rows * n) * 8 * 8, hipMemcpyDeviceToHost,
                               s->stream));
     }
@@ -736,7 +762,7 @@ extern "C" int wenox_solver_rhs(wenox_solver* s, double* out)
         s->sops->rhs(s->plan, s->U, s->UT, s->L, s->DyT, s->work, s->ctl, s->stream);
         HIPCHK(hipMemcpyAsync(s->L, s->work, (size_t)(d.nx * d.ny) * 4 * 8, hipMemcpyDeviceToDevice, s->stream));
     } else {
-        s->ops->rhs_unfused(s->U, d.nx, d.ny, d.dx, d.gamma, d.cons_switch, d.force_hlle, d.jp_lo,
+        s->ops->rhs_unfused(s->U, d.nx, d.ny, s->ld, d.dx, d.gamma, d.cons_switch, d.force_hlle, d.jp_lo,
                             d.jp_hi, s->work, s->L, s->ctl, s->stream);
     }
     const size_t I = (size_t)(d.nx * d.ny);
@@ -845,8 +871,8 @@ extern "C" int wenox_solver_profile(wenox_solver* s, int reps, double* out_ms, i
     // scratch targets: the sweeps read the stage-1 input (U, U^T) and write Dx / Dy^T;
     // the combine writes into B / B^T, which the next step overwrites anyway.
     if ((rc = timed(0, [&] {
-             if (split) s->sops->cfl_split((const float*)s->U, d.nx, d.ny, d.gamma, s->ctl, st);
-             else s->ops->cfl(s->U, d.nx, d.ny, d.gamma, s->ctl, st);
+             if (split) s->sops->cfl_split((const float*)s->U, d.nx, d.ny, s->ld, d.gamma, s->ctl, st);
+             else s->ops->cfl(s->U, d.nx, d.ny, s->ld, d.gamma, s->ctl, st);
          })))
         return rc;
     if ((rc = write_ctl(s, &act))) return rc;
@@ -859,8 +885,8 @@ extern "C" int wenox_solver_profile(wenox_solver* s, int reps, double* out_ms, i
          })))
         return rc;
     if ((rc = timed(4, [&] {
-             if (split) s->sops->bc_split((float*)s->B, (float*)s->BT, d.nx, d.ny, s->bp, s->ctl, 2, st);
-             else s->ops->bc(s->B, s->BT, d.nx, d.ny, s->bp, s->ctl, 2, st);
+             if (split) s->sops->bc_split((float*)s->B, (float*)s->BT, d.nx, d.ny, s->ld, s->ldT, s->bp, s->ctl, 2, st);
+             else s->ops->bc(s->B, s->BT, d.nx, d.ny, s->ld, s->ldT, s->bp, s->ctl, 2, st);
          })))
         return rc;
     double pair = out_ms[1] + out_ms[2];

@@ -253,12 +253,12 @@ extern "C" int wenox_batch_upload(wenox_batch* b, int i, const wenox_view3* q, d
     if (b->split) {
         BCHK(hipMemcpy(b->work, pin.data(), pin.size() * 8, hipMemcpyHostToDevice));
         b->sops->to_split(b->work, (float*)Ui, P, b->stream);
-        b->sops->transpose_split((const float*)Ui, (float*)UTi, m, n, b->stream);
-        b->sops->bc_split((float*)Ui, (float*)UTi, I.nx, I.ny, bp, ci, 0, b->stream);
+        b->sops->transpose_split((const float*)Ui, (float*)UTi, m, n, n, m, b->stream);
+        b->sops->bc_split((float*)Ui, (float*)UTi, I.nx, I.ny, n, m, bp, ci, 0, b->stream);
     } else {
         BCHK(hipMemcpy(Ui, pin.data(), pin.size() * 8, hipMemcpyHostToDevice));
-        b->sops->transpose(Ui, UTi, m, n, b->stream);
-        b->ops->bc(Ui, UTi, I.nx, I.ny, bp, ci, 0, b->stream);
+        b->sops->transpose(Ui, UTi, m, n, n, m, b->stream);
+        b->ops->bc(Ui, UTi, I.nx, I.ny, n, m, bp, ci, 0, b->stream);
     }
     BCHK(hipStreamSynchronize(b->stream));
     BCHK(hipGetLastError());

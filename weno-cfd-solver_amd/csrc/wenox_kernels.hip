@@ -75,12 +75,13 @@ __global__ void k_con2primi(Acc q, int64_t M, int64_t N, double gam, double* p, 
 // Snapshot rows [row0, row0 + rows) of the padded SoA4 state as the reference
 // drivers' (m, n, 8) array: conserved channels, then rho_safe, u, v, p of
 // con2primi (Rie_prob_2d_rs.py:157-163), one cell (64 B) per thread.
-__global__ void k_snapshot8(const double* U, int64_t P, int64_t n, double gam, int64_t row0, int64_t cells,
-                            double* out)
+__global__ void k_snapshot8(const double* U, int64_t P, int64_t n, int64_t ld, double gam, int64_t row0,
+                            int64_t cells, double* out)
 {
     int64_t t = (int64_t)blockIdx.x * TB + threadIdx.x;
     if (t >= cells) return;
-    const int64_t o = row0 * n + t;
+    const int64_t row = t / n;
+    const int64_t o = (row0 + row) * ld + (t - row * n);
     double c[4];
     for (int k = 0; k < 4; ++k) c[k] = U[k * P + o];
     Prim<double> r = con2primi_pt(c, gam);
@@ -245,12 +246,12 @@ static void h_con2primi(const double* q, Lay lq, int64_t M, int64_t N, double ga
     k_con2primi<<<blocks_for(M * N), TB, 0, s>>>(Acc{q, lq}, M, N, gam, p, a, rho, u, v, h);
 }
 
-static void h_snapshot8(const double* U, int64_t m, int64_t n, double gam, int64_t row0, int64_t rows,
-                        double* out, hipStream_t s)
+static void h_snapshot8(const double* U, int64_t m, int64_t n, int64_t ld, double gam, int64_t row0,
+                        int64_t rows, double* out, hipStream_t s)
 {
     const int64_t cells = rows * n;
     if (cells <= 0) return;
-    k_snapshot8<<<blocks_for(cells), TB, 0, s>>>(U, m * n, n, gam, row0, cells, out);
+    k_snapshot8<<<blocks_for(cells), TB, 0, s>>>(U, m * ld, n, ld, gam, row0, cells, out);
 }
 
 static void h_weno(const double* u, Lay lu, int64_t m, int64_t n, int dir, int cons, double gam,
@@ -299,16 +300,16 @@ static void h_divergence(const double* fx, Lay lfx, const double* fy, Lay lfy, i
 
 #include "wenox_state.h"
 
-static void h_bc(double* U, double* UT, int64_t nx, int64_t ny, BcParams bp, const StepCtl* ctl,
-                 int stage, hipStream_t s)
+static void h_bc(double* U, double* UT, int64_t nx, int64_t ny, int64_t ld, int64_t ldT, BcParams bp,
+                 const StepCtl* ctl, int stage, hipStream_t s)
 {
-    const int64_t P = (nx + 6) * (ny + 6);
-    launch_bc(StF64{U, P}, StF64{UT, P}, UT != nullptr, nx, ny, bp, ctl, stage, s);
+    launch_bc(StF64{U, (nx + 6) * ld}, StF64{UT, (ny + 6) * ldT}, UT != nullptr, nx, ny, ld, ldT, bp, ctl,
+              stage, s);
 }
 
-static void h_cfl(const double* U, int64_t nx, int64_t ny, double gam, StepCtl* ctl, hipStream_t s)
+static void h_cfl(const double* U, int64_t nx, int64_t ny, int64_t ld, double gam, StepCtl* ctl, hipStream_t s)
 {
-    launch_cfl(StF64{const_cast<double*>(U), (nx + 6) * (ny + 6)}, nx, ny, gam, ctl, s);
+    launch_cfl(StF64{const_cast<double*>(U), (nx + 6) * ld}, nx, ny, ld, gam, ctl, s);
 }
 
 __global__ void k_advance(StepCtl* ctl)
@@ -326,12 +327,12 @@ static void h_advance(StepCtl* ctl, hipStream_t s) { k_advance<<<1, 64, 0, s>>>(
 // work layout: pl, pr (x pass 1), then ql, fl, qr, fr (x faces), fx; then the
 // same for y; then pbar.
 // ------------------------------------------------------------------------
-static void h_rhs_unfused(const double* U, int64_t nx, int64_t ny, double dx, double gam, int cons,
+static void h_rhs_unfused(const double* U, int64_t nx, int64_t ny, int64_t ld, double dx, double gam, int cons,
                           int force_hlle, double lo, double hi, double* work, double* L,
                           const StepCtl* /*ctl*/, hipStream_t s)
 {
     const int64_t m = nx + 6, n = ny + 6;
-    const Lay lu = lay_soa4(m, n);
+    const Lay lu = Lay{ld, 1, m * ld};
     const int64_t p1 = 4 * (m - 5) * n > 4 * m * (n - 5) ? 4 * (m - 5) * n : 4 * m * (n - 5);
     const int64_t fxs = 4 * (nx + 1) * ny, fys = 4 * nx * (ny + 1);
     double* pl = work;
@@ -358,14 +359,14 @@ static void h_rhs_unfused(const double* U, int64_t nx, int64_t ny, double dx, do
 //   3: ((1/3) q + (2/3) q2) + ((2/3) dt) l
 // ------------------------------------------------------------------------
 __global__ void k_rk(const double* Un, const double* Us, const double* L, double* out, int64_t nx,
-                     int64_t ny, int stage, const StepCtl* ctl)
+                     int64_t ny, int64_t ld, int stage, const StepCtl* ctl)
 {
     if (!ctl->active) return;
     int64_t t = (int64_t)blockIdx.x * TB + threadIdx.x;
     if (t >= nx * ny) return;
-    const int64_t m = nx + 6, n = ny + 6, P = m * n, PL = nx * ny;
+    const int64_t P = (nx + 6) * ld, PL = nx * ny;
     int64_t i = t / ny, j = t - i * ny;
-    int64_t o = (i + 3) * n + (j + 3);
+    int64_t o = (i + 3) * ld + (j + 3);
     const double dt = ctl->dt;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -384,10 +385,10 @@ __global__ void k_rk(const double* Un, const double* Us, const double* L, double
 }
 
 static void h_rk(const double* Un, const double* Us, const double* L, double* out, int64_t nx,
-                 int64_t ny, int stage, const StepCtl* ctl, hipStream_t s)
+                 int64_t ny, int64_t ld, int stage, const StepCtl* ctl, hipStream_t s)
 {
     if (nx * ny == 0) return;
-    k_rk<<<blocks_for(nx * ny), TB, 0, s>>>(Un, Us, L, out, nx, ny, stage, ctl);
+    k_rk<<<blocks_for(nx * ny), TB, 0, s>>>(Un, Us, L, out, nx, ny, ld, stage, ctl);
 }
 
 }  // namespace WX_NS

@@ -7,11 +7,24 @@
 // Array conventions:
 //   AoS4  (M, N, 4) C-contiguous, the numpy layout of the reference's arrays;
 //   SoA4  four planes [4][M][N], j (y) fastest -- the stepper's HBM layout.
+//
+// The stepper stores each padded plane (M = nx+6 rows of N = ny+6 cells)
+// with a row pitch ld = row_pitch(N) >= N and plane stride M * ld, and the
+// plane set begins STATE_SHIFT elements into its allocation: padded column 3
+// (the first interior cell) of every row then starts on a 256 B (FP64) or
+// 128 B (FP32 planes) boundary, so 32-cell row segments of the interior are
+// whole cache lines.  The transposed copy uses row_pitch(M) the same way, and
+// the interior flux-difference arrays (nx x ny, ny x nx) use row_pitch of
+// their row length without a shift.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace wenox {
+
+constexpr int64_t ROW_ALIGN = 32;                // elements
+constexpr int64_t STATE_SHIFT = ROW_ALIGN - 3;   // elements before padded column 0 of row 0
+inline int64_t row_pitch(int64_t n) { return (n + ROW_ALIGN - 1) / ROW_ALIGN * ROW_ALIGN; }
 
 // Device-resident time control of one solve (read by every step kernel).
 // Slots of the CFL reduction's per-block maxima (StepCtl::part).
@@ -65,10 +78,10 @@ struct Ops {
                  Lay lo, hipStream_t s);
     void (*con2primi)(const double* q, Lay lq, int64_t M, int64_t N, double gam, double* p,
                       double* a, double* rho, double* u, double* v, double* h, hipStream_t s);
-    // rows [row0, row0 + rows) of a padded SoA4 state (m x n planes) as
-    // (rows, n, 8): conserved channels, then rho_safe, u, v, p
-    void (*snapshot8)(const double* U, int64_t m, int64_t n, double gam, int64_t row0, int64_t rows,
-                      double* out, hipStream_t s);
+    // rows [row0, row0 + rows) of a padded SoA4 state (m x n planes, row
+    // pitch ld) as (rows, n, 8): conserved channels, then rho_safe, u, v, p
+    void (*snapshot8)(const double* U, int64_t m, int64_t n, int64_t ld, double gam, int64_t row0,
+                      int64_t rows, double* out, hipStream_t s);
     // pass 1 (normal) into pl/pr, then pass 2 (transverse Gauss points) and the
     // averaged fluxes into ql/fl/qr/fr.  u is (m, n); pl/pr are SoA4 of
     // (m-5, n) for x or (m, n-5) for y; outputs are (F1, F2) in layout lo.
@@ -84,19 +97,21 @@ struct Ops {
                        int64_t Ny, double dx, double* out, Lay lout, hipStream_t s);
 
     // ---- stepper (SoA4 padded state, (nx+6) x (ny+6) per plane) ---------
+    // Row pitches: ld of U's planes, ldT of UT's (plane strides (nx+6) ld and
+    // (ny+6) ldT).
     // ghost fill of U, and of its transposed copy UT when UT != nullptr
-    void (*bc)(double* U, double* UT, int64_t nx, int64_t ny, BcParams bp, const StepCtl* ctl,
-               int stage, hipStream_t s);
-    void (*cfl)(const double* U, int64_t nx, int64_t ny, double gam, StepCtl* ctl, hipStream_t s);
+    void (*bc)(double* U, double* UT, int64_t nx, int64_t ny, int64_t ld, int64_t ldT, BcParams bp,
+               const StepCtl* ctl, int stage, hipStream_t s);
+    void (*cfl)(const double* U, int64_t nx, int64_t ny, int64_t ld, double gam, StepCtl* ctl, hipStream_t s);
     void (*advance)(StepCtl* ctl, hipStream_t s);
     // Unfused RHS: L (interior Nx x Ny, SoA4) from U (padded SoA4), through
     // face arrays in `work` (rhs_unfused_scratch(nx, ny) doubles).
-    void (*rhs_unfused)(const double* U, int64_t nx, int64_t ny, double dx, double gam, int cons,
+    void (*rhs_unfused)(const double* U, int64_t nx, int64_t ny, int64_t ld, double dx, double gam, int cons,
                         int force_hlle, double lo, double hi, double* work, double* L,
                         const StepCtl* ctl, hipStream_t s);
     // RK stage combine on the SoA4 interior (stage 1..3), skipped when !ctl->active.
     void (*rk)(const double* Un, const double* Us, const double* L, double* out, int64_t nx,
-               int64_t ny, int stage, const StepCtl* ctl, hipStream_t s);
+               int64_t ny, int64_t ld, int stage, const StepCtl* ctl, hipStream_t s);
 };
 
 size_t rhs_unfused_scratch(int64_t nx, int64_t ny);

@@ -79,13 +79,14 @@ __device__ __forceinline__ int64_t periodic_src(int64_t g, int64_t N)
 // rows bound+1..bound+9 copy column 3, the rest mirror with rho*v negated.
 __constant__ double c_dmr_post[4] = {8.0, 33.0, -57.1576, 563.499};
 
-// Ghost cell number t (0 <= t < 6 (ny+6) + 6 nx) of one state.
+// Ghost cell number t (0 <= t < 6 (ny+6) + 6 nx) of one state; ld and ldT
+// are the row pitches of U and UT.
 template <class St>
-__device__ __forceinline__ void bc_cell(St U, St UT, bool has_t, int64_t nx, int64_t ny, BcParams bp,
-                                        const StepCtl* ctl, int stage, int64_t t)
+__device__ __forceinline__ void bc_cell(St U, St UT, bool has_t, int64_t nx, int64_t ny, int64_t ld, int64_t ldT,
+                                        BcParams bp, const StepCtl* ctl, int stage, int64_t t)
 {
     if (ctl && !ctl->active) return;
-    const int64_t m = nx + 6, n = ny + 6;
+    const int64_t n = ny + 6;
     const int64_t count = 6 * n + 6 * nx;
     if (t >= count) return;
     int64_t i, j;
@@ -96,12 +97,12 @@ __device__ __forceinline__ void bc_cell(St U, St UT, bool has_t, int64_t nx, int
         int64_t si = i < 3 ? 3 : (i > nx + 2 ? nx + 2 : i);
         int64_t sj = j < 3 ? 3 : (j > ny + 2 ? ny + 2 : j);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) v[k] = U.ld(k, si * n + sj);
+        for (int k = 0; k < 4; ++k) v[k] = U.ld(k, si * ld + sj);
     } else if (bp.kind == BC_MIRROR || bp.kind == BC_PERIODIC) {
         int64_t si = bp.kind == BC_MIRROR ? mirror_src(i, nx) : periodic_src(i, nx);
         int64_t sj = bp.kind == BC_MIRROR ? mirror_src(j, ny) : periodic_src(j, ny);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) v[k] = U.ld(k, si * n + sj);
+        for (int k = 0; k < 4; ++k) v[k] = U.ld(k, si * ld + sj);
     } else if (bp.kind == BC_DMR_REF) {
         const int64_t B = bp.dmr_bound;
         int64_t si = i, sj = j;
@@ -125,7 +126,7 @@ __device__ __forceinline__ void bc_cell(St U, St UT, bool has_t, int64_t nx, int
             }
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) v[k] = post ? c_dmr_post[k] : U.ld(k, si * n + sj);
+        for (int k = 0; k < 4; ++k) v[k] = post ? c_dmr_post[k] : U.ld(k, si * ld + sj);
         if (flip) v[2] = -v[2];
     } else {
         // Standard double Mach reflection (Woodward & Colella 1984): left
@@ -157,14 +158,14 @@ __device__ __forceinline__ void bc_cell(St U, St UT, bool has_t, int64_t nx, int
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            v[k] = src == 1 ? post_q[k] : (src == 2 ? pre_q[k] : U.ld(k, ie * n + sj));
+            v[k] = src == 1 ? post_q[k] : (src == 2 ? pre_q[k] : U.ld(k, ie * ld + sj));
         if (flip) v[2] = -v[2];
     }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) U.st(k, i * n + j, v[k]);
+    for (int k = 0; k < 4; ++k) U.st(k, i * ld + j, v[k]);
     if (has_t) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) UT.st(k, j * m + i, v[k]);
+        for (int k = 0; k < 4; ++k) UT.st(k, j * ldT + i, v[k]);
     }
 }
 
@@ -194,10 +195,10 @@ __device__ __forceinline__ double wave_max(double v)
 }
 
 template <class St>
-__global__ void k_bc(St U, St UT, bool has_t, int64_t nx, int64_t ny, BcParams bp, const StepCtl* ctl,
-                     int stage)
+__global__ void k_bc(St U, St UT, bool has_t, int64_t nx, int64_t ny, int64_t ld, int64_t ldT, BcParams bp,
+                     const StepCtl* ctl, int stage)
 {
-    bc_cell<St>(U, UT, has_t, nx, ny, bp, ctl, stage, (int64_t)blockIdx.x * ST_TB + threadIdx.x);
+    bc_cell<St>(U, UT, has_t, nx, ny, ld, ldT, bp, ctl, stage, (int64_t)blockIdx.x * ST_TB + threadIdx.x);
 }
 
 // dt and the loop condition from the step's wave-speed maximum
@@ -226,16 +227,15 @@ __device__ __forceinline__ void cfl_finish(StepCtl* ctl, double cmax_state)
 // k_cfl_dt finishes; without one (the batched engine, a few blocks per
 // instance) blocks combine with atomics and the last one finishes.
 template <class St>
-__device__ __forceinline__ void cfl_block(St U, int64_t nx, int64_t ny, double gam, StepCtl* ctl,
+__device__ __forceinline__ void cfl_block(St U, int64_t nx, int64_t ny, int64_t ld, double gam, StepCtl* ctl,
                                           int64_t blk, int64_t nblk)
 {
     __shared__ double part[ST_TB / 64];
     __shared__ bool last;
-    const int64_t n = ny + 6;
     double vmaxl = 0.0;
     // rows blk, blk + nblk, ...; the block's threads stride along the row
     for (int64_t i = blk; i < nx; i += nblk) {
-        const int64_t o0 = (i + 3) * n + 3;
+        const int64_t o0 = (i + 3) * ld + 3;
         for (int64_t j = threadIdx.x; j < ny; j += ST_TB) {
             const int64_t o = o0 + j;
             double q[4] = {U.ld(0, o), U.ld(1, o), U.ld(2, o), U.ld(3, o)};
@@ -272,9 +272,9 @@ __device__ __forceinline__ void cfl_block(St U, int64_t nx, int64_t ny, double g
 }
 
 template <class St>
-__global__ void __launch_bounds__(ST_TB) k_cfl(St U, int64_t nx, int64_t ny, double gam, StepCtl* ctl)
+__global__ void __launch_bounds__(ST_TB) k_cfl(St U, int64_t nx, int64_t ny, int64_t ld, double gam, StepCtl* ctl)
 {
-    cfl_block<St>(U, nx, ny, gam, ctl, blockIdx.x, gridDim.x);
+    cfl_block<St>(U, nx, ny, ld, gam, ctl, blockIdx.x, gridDim.x);
 }
 
 // One wavefront: maximum of the nblk per-block values, then dt.
@@ -287,19 +287,20 @@ __global__ void __launch_bounds__(64) k_cfl_dt(StepCtl* ctl, int nblk)
 }
 
 template <class St>
-static void launch_bc(St U, St UT, bool has_t, int64_t nx, int64_t ny, BcParams bp, const StepCtl* ctl,
-                      int stage, hipStream_t s)
+static void launch_bc(St U, St UT, bool has_t, int64_t nx, int64_t ny, int64_t ld, int64_t ldT, BcParams bp,
+                      const StepCtl* ctl, int stage, hipStream_t s)
 {
     int64_t count = 6 * (ny + 6) + 6 * nx;
-    k_bc<St><<<(unsigned)((count + ST_TB - 1) / ST_TB), ST_TB, 0, s>>>(U, UT, has_t, nx, ny, bp, ctl, stage);
+    k_bc<St><<<(unsigned)((count + ST_TB - 1) / ST_TB), ST_TB, 0, s>>>(U, UT, has_t, nx, ny, ld, ldT, bp, ctl,
+                                                                       stage);
 }
 
 template <class St>
-static void launch_cfl(St U, int64_t nx, int64_t ny, double gam, StepCtl* ctl, hipStream_t s)
+static void launch_cfl(St U, int64_t nx, int64_t ny, int64_t ld, double gam, StepCtl* ctl, hipStream_t s)
 {
     // whole rows per block; ctl->part holds CFL_BLOCKS values
     int64_t nb = nx < CFL_BLOCKS ? nx : CFL_BLOCKS;
     if (nb < 1) nb = 1;
-    k_cfl<St><<<(unsigned)nb, ST_TB, 0, s>>>(U, nx, ny, gam, ctl);
+    k_cfl<St><<<(unsigned)nb, ST_TB, 0, s>>>(U, nx, ny, ld, gam, ctl);
     k_cfl_dt<<<1, 64, 0, s>>>(ctl, (int)nb);
 }

@@ -22,19 +22,29 @@ struct SweepPlan {
     int chunk_x, chunk_y;      // interior rows marched per wavefront
     int waves_x, waves_y;      // wavefronts per workgroup along the normal (1, 2 or 4)
     int combine_cols;          // y extent of an RK-combine tile (32 or 64)
+    // row pitches (elements): U planes, U^T planes, Dx rows, Dy^T rows
+    int64_t ld, ldT, ldx, ldyT;
+    int64_t P() const { return (nx + 6) * ld; }       // plane stride of U
+    int64_t PT() const { return (ny + 6) * ldT; }     // plane stride of U^T
+    int64_t PDx() const { return nx * ldx; }          // plane stride of Dx
+    int64_t PDyT() const { return ny * ldyT; }        // plane stride of Dy^T
 };
 
 struct SweepOps {
-    // stage s: Us/UsT stage input (ghost-filled), Un step start; Dx (nx*ny*4),
-    // DyT (ny*nx*4) scratch; out/outT receive the next state's interior.
+    // stage s: Us/UsT stage input (ghost-filled), Un step start; Dx (4 planes
+    // of nx rows, pitch ldx), DyT (4 planes of ny rows, pitch ldyT) scratch;
+    // out/outT receive the next state's interior.  State buffers use the
+    // plan's pitches (ld, ldT).
     void (*stage)(const SweepPlan& p, const double* Us, const double* UsT, const double* Un,
                   double* Dx, double* DyT, double* out, double* outT, int stage,
                   const StepCtl* ctl, hipStream_t s);
     // L(U) (SoA4 interior) only.
     void (*rhs)(const SweepPlan& p, const double* U, const double* UT, double* Dx, double* DyT,
                 double* L, const StepCtl* ctl, hipStream_t s);
-    // out[k][j][i] = in[k][i][j] for an M x N padded plane set
-    void (*transpose)(const double* in, double* out, int64_t M, int64_t N, hipStream_t s);
+    // out[k][j][i] = in[k][i][j] for an M x N padded plane set (row pitch
+    // ld_in of in, ld_out of out)
+    void (*transpose)(const double* in, double* out, int64_t M, int64_t N, int64_t ld_in, int64_t ld_out,
+                      hipStream_t s);
 
     // ---- FP32-split state: float planes [8][M][N], value = hi[k] + lo[k+4] ----
     // (fast build only; nullptr in the exact build)
@@ -43,12 +53,13 @@ struct SweepOps {
                         const StepCtl* ctl, hipStream_t s);
     void (*rhs_split)(const SweepPlan& p, const float* U, const float* UT, float* Dx, float* DyT,
                       int* flags, double* L, const StepCtl* ctl, hipStream_t s);
-    void (*transpose_split)(const float* in, float* out, int64_t M, int64_t N, hipStream_t s);
+    void (*transpose_split)(const float* in, float* out, int64_t M, int64_t N, int64_t ld_in, int64_t ld_out,
+                            hipStream_t s);
     void (*to_split)(const double* in, float* out, int64_t P, hipStream_t s);
     void (*from_split)(const float* in, double* out, int64_t P, hipStream_t s);
-    void (*bc_split)(float* U, float* UT, int64_t nx, int64_t ny, BcParams bp, const StepCtl* ctl,
-                     int stage, hipStream_t s);
-    void (*cfl_split)(const float* U, int64_t nx, int64_t ny, double gam, StepCtl* ctl, hipStream_t s);
+    void (*bc_split)(float* U, float* UT, int64_t nx, int64_t ny, int64_t ld, int64_t ldT, BcParams bp,
+                     const StepCtl* ctl, int stage, hipStream_t s);
+    void (*cfl_split)(const float* U, int64_t nx, int64_t ny, int64_t ld, double gam, StepCtl* ctl, hipStream_t s);
 
     // Single kernels, for per-kernel timing: one direction's sweep of the
     // stage input U (U^T for dir 1) into D, and the stage-2 combine.

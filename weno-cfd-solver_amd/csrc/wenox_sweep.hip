@@ -62,11 +62,12 @@ enum { MODE_PLAIN = 0, MODE_DETECT = 1, MODE_FIXUP = 2 };
 struct SweepArgs {
     const void* U;        // padded planes, M x N each, normal = rows
     int64_t M, N;         // padded extents (nn + 6, nt + 6)
+    int64_t ld, ldd;      // row pitches (elements) of U's planes and of D's
     int64_t nn, nt;       // interior cells along normal / transverse
     int chunk;            // interior rows per wavefront
     int waves;            // wavefronts per workgroup (consecutive chunks)
     double gam, dx, two_h, lo, hi;
-    void* D;              // interior planes nn x nt (DT)
+    void* D;              // interior planes nn x nt (DT), row pitch ldd
     int* flags;           // one per workgroup (DETECT writes, FIXUP reads)
     const StepCtl* ctl;
 };
@@ -174,8 +175,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
     const bool dfirst = BLEND && wv > 0;                             // first face blended at the end
     const int64_t fa = (BLEND && wv == 0 && r0 > 0) ? r0 - 1 : r0;
     const int64_t fb = last ? ((BLEND && r1 < A.nn) ? r1 + 1 : r1) : r1 - 1;   // inclusive
-    const int64_t P = A.M * A.N;
-    const int64_t Pd = A.nn * A.nt;
+    const int64_t P = A.M * A.ld;
+    const int64_t Pd = A.nn * A.ldd;
     DT* Dout = (DT*)A.D;
     bool bad = false;
 
@@ -186,7 +187,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
 
     auto put_d = [&](int64_t cell, const T Fr[4], const T Fl[4]) {
         if (out_lane) {
-            DT* d = Dout + cell * A.nt + t;
+            DT* d = Dout + cell * A.ldd + t;
 #pragma unroll
             for (int k = 0; k < 4; ++k) d[k * Pd] = (DT)WX_DIV_DX(Fr[k] - Fl[k]);
         }
@@ -220,9 +221,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
 #pragma unroll
         for (int s = 0; s < 5; ++s)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) w[s + 1][k] = ld_state<T, IN>(A.U, P, k, (fa + s) * A.N + pcl);
+            for (int k = 0; k < 4; ++k) w[s + 1][k] = ld_state<T, IN>(A.U, P, k, (fa + s) * A.ld + pcl);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) nxt[k] = ld_state<T, IN>(A.U, P, k, (fa + 5) * A.N + pcl);
+        for (int k = 0; k < 4; ++k) nxt[k] = ld_state<T, IN>(A.U, P, k, (fa + 5) * A.ld + pcl);
 
         for (int64_t a = fa; a <= fb; ++a) {
             // slide the window; row a+5 was fetched one face ahead, fetch row a+6
@@ -234,7 +235,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
             for (int k = 0; k < 4; ++k) w[5][k] = nxt[k];
             if (a < fb) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) nxt[k] = ld_state<T, IN>(A.U, P, k, (a + 6) * A.N + pcl);
+                for (int k = 0; k < 4; ++k) nxt[k] = ld_state<T, IN>(A.U, P, k, (a + 6) * A.ld + pcl);
             }
 
             // pass 1 (normal reconstruction) on every lane
@@ -431,15 +432,17 @@ constexpr int TT = 32;
 // A thread overwrites in LDS exactly the Dy^T entries it read with the new
 // state, so the tile needs two barriers; the transposed state then leaves
 // LDS along x.  Wider tiles give the x-major streams 512 B rows.
+// Row pitches: ld of Un, Us, out; ldT of outT; ldx of Dx; ldyT of DyT.
 template <class St, typename DT, int TJ>
 __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT* __restrict__ DyT, St Un,
-                                             St Us, St out, St outT, int64_t nx, int64_t ny, int stage,
+                                             St Us, St out, St outT, int64_t nx, int64_t ny, int64_t ld,
+                                             int64_t ldT, int64_t ldx, int64_t ldyT, int stage,
                                              const StepCtl* ctl, int64_t ti, int64_t tj)
 {
     if (ctl && !ctl->active) return;
     __shared__ double tile[4][TJ][TT + 1];   // [k][j - j0][i - i0]
     const int64_t i0 = ti * TT, j0 = tj * TJ;
-    const int64_t m = nx + 6, n = ny + 6, PI = nx * ny;
+    const int64_t PX = nx * ldx, PY = ny * ldyT;
     const double dt = ctl ? ctl->dt : 0.0;
     // Dy^T[k][j][i]: rows j, contiguous i (32 threads per row)
     {
@@ -449,7 +452,7 @@ __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT
 #pragma unroll
             for (int r = r0; r < TJ; r += 256 / TT) {
                 const int64_t j = j0 + r, i = i0 + c;
-                if (j < ny && i < nx) tile[k][r][c] = (double)DyT[k * PI + j * nx + i];
+                if (j < ny && i < nx) tile[k][r][c] = (double)DyT[k * PY + j * ldyT + i];
             }
     }
     __syncthreads();
@@ -460,10 +463,10 @@ __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT
         for (int r = ty; r < TT; r += 256 / TJ) {
             const int64_t i = i0 + r, j = j0 + tx;
             if (i < nx && j < ny) {
-                const int64_t o = (i + 3) * n + (j + 3);
+                const int64_t o = (i + 3) * ld + (j + 3);
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    const double l = -((double)Dx[k * PI + i * ny + j] + tile[k][tx][r]);
+                    const double l = -((double)Dx[k * PX + i * ldx + j] + tile[k][tx][r]);
                     const double qn = Un.ld(k, o);
                     double v;
                     if (stage == 1) v = qn + dt * l;
@@ -484,7 +487,7 @@ __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT
 #pragma unroll
             for (int r = r0; r < TJ; r += 256 / TT) {
                 const int64_t j = j0 + r, i = i0 + c;
-                if (j < ny && i < nx) outT.st(k, (j + 3) * m + (i + 3), tile[k][r][c]);
+                if (j < ny && i < nx) outT.st(k, (j + 3) * ldT + (i + 3), tile[k][r][c]);
             }
     }
 }
@@ -497,13 +500,15 @@ __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT
 template <class St, typename DT, int TJ>
 __global__ void __launch_bounds__(256) k_combine(const DT* __restrict__ Dx, const DT* __restrict__ DyT,
                                                  St Un, St Us, St out, St outT, int64_t nx, int64_t ny,
+                                                 int64_t ld, int64_t ldT, int64_t ldx, int64_t ldyT,
                                                  int stage, const StepCtl* ctl)
 {
     const int64_t tiles_j = gridDim.x, ntiles = (int64_t)gridDim.x * gridDim.y;
     const int64_t b = (int64_t)blockIdx.y * tiles_j + blockIdx.x;
     const int64_t q = ntiles / 8, r = ntiles % 8, x = b % 8, k = b / 8;
     const int64_t t = x * q + (x < r ? x : r) + k;
-    combine_tile<St, DT, TJ>(Dx, DyT, Un, Us, out, outT, nx, ny, stage, ctl, t / tiles_j, t % tiles_j);
+    combine_tile<St, DT, TJ>(Dx, DyT, Un, Us, out, outT, nx, ny, ld, ldT, ldx, ldyT, stage, ctl, t / tiles_j,
+                             t % tiles_j);
 }
 
 template <class St, typename DT>
@@ -512,42 +517,47 @@ static void launch_combine(const SweepPlan& p, const DT* Dx, const DT* DyT, St U
 {
     const unsigned ti = (unsigned)((p.nx + TT - 1) / TT);
     if (p.combine_cols == 64) {
-        k_combine<St, DT, 64><<<dim3((unsigned)((p.ny + 63) / 64), ti), 256, 0, s>>>(Dx, DyT, Un, Us, out, outT,
-                                                                                       p.nx, p.ny, stage, ctl);
+        k_combine<St, DT, 64><<<dim3((unsigned)((p.ny + 63) / 64), ti), 256, 0, s>>>(
+            Dx, DyT, Un, Us, out, outT, p.nx, p.ny, p.ld, p.ldT, p.ldx, p.ldyT, stage, ctl);
     } else {
-        k_combine<St, DT, 32><<<dim3((unsigned)((p.ny + 31) / 32), ti), 256, 0, s>>>(Dx, DyT, Un, Us, out, outT,
-                                                                                       p.nx, p.ny, stage, ctl);
+        k_combine<St, DT, 32><<<dim3((unsigned)((p.ny + 31) / 32), ti), 256, 0, s>>>(
+            Dx, DyT, Un, Us, out, outT, p.nx, p.ny, p.ld, p.ldT, p.ldx, p.ldyT, stage, ctl);
     }
 }
 
-// Plain RHS from the two sweeps (for tests): L = -(Dx + Dy^T), SoA4 interior.
+// Plain RHS from the two sweeps (for tests): L = -(Dx + Dy^T), SoA4 interior
+// (unpitched) from Dx (row pitch ldx) and Dy^T (row pitch ldyT).
 template <typename DT>
-__global__ void k_rhs_out(const DT* Dx, const DT* DyT, double* L, int64_t nx, int64_t ny)
+__global__ void k_rhs_out(const DT* Dx, const DT* DyT, double* L, int64_t nx, int64_t ny, int64_t ldx,
+                          int64_t ldyT)
 {
     int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= nx * ny) return;
     int64_t i = t / ny, j = t - i * ny;
     for (int k = 0; k < 4; ++k)
-        L[k * nx * ny + t] = -((double)Dx[k * nx * ny + t] + (double)DyT[k * nx * ny + j * nx + i]);
+        L[k * nx * ny + t] = -((double)Dx[k * nx * ldx + i * ldx + j] + (double)DyT[k * ny * ldyT + j * ldyT + i]);
 }
 
-// Padded plane-set transpose: out[k][j][i] = in[k][i][j] for NP planes of M x N.
+// Padded plane-set transpose: out[k][j][i] = in[k][i][j] for NP planes of
+// M x N (in: row pitch ld_in, plane M ld_in; out: row pitch ld_out, plane
+// N ld_out).
 template <typename E, int NP>
-__global__ void __launch_bounds__(256) k_transpose(const E* in, E* out, int64_t M, int64_t N)
+__global__ void __launch_bounds__(256) k_transpose(const E* in, E* out, int64_t M, int64_t N, int64_t ld_in,
+                                                   int64_t ld_out)
 {
     __shared__ E tile[TT][TT + 1];
     const int tx = threadIdx.x & (TT - 1), ty = threadIdx.x / TT;
     const int64_t i0 = (int64_t)blockIdx.y * TT, j0 = (int64_t)blockIdx.x * TT;
-    const int64_t P = M * N;
+    const int64_t Pi = M * ld_in, Po = N * ld_out;
     for (int k = 0; k < NP; ++k) {
         for (int r = ty; r < TT; r += 8) {
             int64_t i = i0 + r, j = j0 + tx;
-            if (i < M && j < N) tile[r][tx] = in[k * P + i * N + j];
+            if (i < M && j < N) tile[r][tx] = in[k * Pi + i * ld_in + j];
         }
         __syncthreads();
         for (int r = ty; r < TT; r += 8) {
             int64_t j = j0 + r, i = i0 + tx;
-            if (i < M && j < N) out[k * P + j * M + i] = tile[tx][r];
+            if (i < M && j < N) out[k * Po + j * ld_out + i] = tile[tx][r];
         }
         __syncthreads();
     }
@@ -612,14 +622,14 @@ static void launch_sweeps(const SweepArgs& ax, const SweepArgs& ay, int cons, in
 
 static SweepArgs args_x(const SweepPlan& p, const void* U, void* D, int* flags, const StepCtl* ctl)
 {
-    return SweepArgs{U, p.nx + 6, p.ny + 6, p.nx, p.ny, p.chunk_x, p.waves_x, p.gamma, p.dx, p.two_h_x,
-                     p.jp_lo, p.jp_hi, D, flags, ctl};
+    return SweepArgs{U,     p.nx + 6,   p.ny + 6, p.ld,    p.ldx,     p.nx,    p.ny,  p.chunk_x,
+                     p.waves_x, p.gamma, p.dx,     p.two_h_x, p.jp_lo, p.jp_hi, D,     flags, ctl};
 }
 
 static SweepArgs args_y(const SweepPlan& p, const void* UT, void* D, int* flags, const StepCtl* ctl)
 {
-    return SweepArgs{UT, p.ny + 6, p.nx + 6, p.ny, p.nx, p.chunk_y, p.waves_y, p.gamma, p.dx, p.two_h_y,
-                     p.jp_lo, p.jp_hi, D, flags, ctl};
+    return SweepArgs{UT,    p.ny + 6,   p.nx + 6, p.ldT,   p.ldyT,    p.ny,    p.nx,  p.chunk_y,
+                     p.waves_y, p.gamma, p.dx,     p.two_h_y, p.jp_lo, p.jp_hi, D,     flags, ctl};
 }
 
 // ---- FP64 ---------------------------------------------------------------
@@ -636,9 +646,9 @@ static void h_stage(const SweepPlan& p, const double* Us, const double* UsT, con
                     hipStream_t s)
 {
     h_sweeps(p, Us, UsT, Dx, DyT, ctl, s);
-    const int64_t P = (p.nx + 6) * (p.ny + 6);
+    const int64_t P = p.P();
     launch_combine<StF64, double>(p, Dx, DyT, StF64{const_cast<double*>(Un), P}, StF64{const_cast<double*>(Us), P},
-                                  StF64{out, P}, StF64{outT, P}, stage, ctl, s);
+                                  StF64{out, P}, StF64{outT, p.PT()}, stage, ctl, s);
 }
 
 static void h_rhs(const SweepPlan& p, const double* U, const double* UT, double* Dx, double* DyT,
@@ -646,13 +656,14 @@ static void h_rhs(const SweepPlan& p, const double* U, const double* UT, double*
 {
     h_sweeps(p, U, UT, Dx, DyT, ctl, s);
     int64_t nc = p.nx * p.ny;
-    k_rhs_out<double><<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(Dx, DyT, L, p.nx, p.ny);
+    k_rhs_out<double><<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(Dx, DyT, L, p.nx, p.ny, p.ldx, p.ldyT);
 }
 
-static void h_transpose(const double* in, double* out, int64_t M, int64_t N, hipStream_t s)
+static void h_transpose(const double* in, double* out, int64_t M, int64_t N, int64_t ld_in, int64_t ld_out,
+                        hipStream_t s)
 {
     dim3 grid((unsigned)((N + TT - 1) / TT), (unsigned)((M + TT - 1) / TT));
-    k_transpose<double, 4><<<grid, 256, 0, s>>>(in, out, M, N);
+    k_transpose<double, 4><<<grid, 256, 0, s>>>(in, out, M, N, ld_in, ld_out);
 }
 
 static void h_sweep1(const SweepPlan& p, int dir, int split, const void* U, void* D, int* flags,
@@ -696,17 +707,18 @@ static void h_sweep2(const SweepPlan& p, int split, const void* U, const void* U
 static void h_combine1(const SweepPlan& p, int split, const void* Dx, const void* DyT, const void* Un,
                        const void* Us, void* out, void* outT, const StepCtl* ctl, hipStream_t s)
 {
-    const int64_t P = (p.nx + 6) * (p.ny + 6);
+    const int64_t P = p.P(), PT = p.PT();
 #if WX_FAST
     if (split) {
         launch_combine<StSplit, float>(p, (const float*)Dx, (const float*)DyT, StSplit{(float*)Un, P},
-                                       StSplit{(float*)Us, P}, StSplit{(float*)out, P}, StSplit{(float*)outT, P}, 2,
-                                       ctl, s);
+                                       StSplit{(float*)Us, P}, StSplit{(float*)out, P}, StSplit{(float*)outT, PT},
+                                       2, ctl, s);
         return;
     }
 #endif
     launch_combine<StF64, double>(p, (const double*)Dx, (const double*)DyT, StF64{(double*)Un, P},
-                                  StF64{(double*)Us, P}, StF64{(double*)out, P}, StF64{(double*)outT, P}, 2, ctl, s);
+                                  StF64{(double*)Us, P}, StF64{(double*)out, P}, StF64{(double*)outT, PT}, 2, ctl,
+                                  s);
 }
 
 #if WX_FAST
@@ -727,9 +739,9 @@ static void h_stage_split(const SweepPlan& p, const float* Us, const float* UsT,
                           const StepCtl* ctl, hipStream_t s)
 {
     h_sweeps_split(p, Us, UsT, Dx, DyT, flags, ctl, s);
-    const int64_t P = (p.nx + 6) * (p.ny + 6);
+    const int64_t P = p.P();
     launch_combine<StSplit, float>(p, Dx, DyT, StSplit{const_cast<float*>(Un), P}, StSplit{const_cast<float*>(Us), P},
-                                   StSplit{out, P}, StSplit{outT, P}, stage, ctl, s);
+                                   StSplit{out, P}, StSplit{outT, p.PT()}, stage, ctl, s);
 }
 
 static void h_rhs_split(const SweepPlan& p, const float* U, const float* UT, float* Dx, float* DyT,
@@ -737,13 +749,14 @@ static void h_rhs_split(const SweepPlan& p, const float* U, const float* UT, flo
 {
     h_sweeps_split(p, U, UT, Dx, DyT, flags, ctl, s);
     int64_t nc = p.nx * p.ny;
-    k_rhs_out<float><<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(Dx, DyT, L, p.nx, p.ny);
+    k_rhs_out<float><<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(Dx, DyT, L, p.nx, p.ny, p.ldx, p.ldyT);
 }
 
-static void h_transpose_split(const float* in, float* out, int64_t M, int64_t N, hipStream_t s)
+static void h_transpose_split(const float* in, float* out, int64_t M, int64_t N, int64_t ld_in, int64_t ld_out,
+                              hipStream_t s)
 {
     dim3 grid((unsigned)((N + TT - 1) / TT), (unsigned)((M + TT - 1) / TT));
-    k_transpose<float, 8><<<grid, 256, 0, s>>>(in, out, M, N);
+    k_transpose<float, 8><<<grid, 256, 0, s>>>(in, out, M, N, ld_in, ld_out);
 }
 
 __global__ void k_to_split(const double* in, float* out, int64_t P)
@@ -772,16 +785,17 @@ static void h_from_split(const float* in, double* out, int64_t P, hipStream_t s)
     k_from_split<<<(unsigned)((P + 255) / 256), 256, 0, s>>>(in, out, P);
 }
 
-static void h_bc_split(float* U, float* UT, int64_t nx, int64_t ny, BcParams bp, const StepCtl* ctl,
-                       int stage, hipStream_t s)
+static void h_bc_split(float* U, float* UT, int64_t nx, int64_t ny, int64_t ld, int64_t ldT, BcParams bp,
+                       const StepCtl* ctl, int stage, hipStream_t s)
 {
-    const int64_t P = (nx + 6) * (ny + 6);
-    launch_bc(StSplit{U, P}, StSplit{UT, P}, UT != nullptr, nx, ny, bp, ctl, stage, s);
+    launch_bc(StSplit{U, (nx + 6) * ld}, StSplit{UT, (ny + 6) * ldT}, UT != nullptr, nx, ny, ld, ldT, bp, ctl,
+              stage, s);
 }
 
-static void h_cfl_split(const float* U, int64_t nx, int64_t ny, double gam, StepCtl* ctl, hipStream_t s)
+static void h_cfl_split(const float* U, int64_t nx, int64_t ny, int64_t ld, double gam, StepCtl* ctl,
+                        hipStream_t s)
 {
-    launch_cfl(StSplit{const_cast<float*>(U), (nx + 6) * (ny + 6)}, nx, ny, gam, ctl, s);
+    launch_cfl(StSplit{const_cast<float*>(U), (nx + 6) * ld}, nx, ny, ld, gam, ctl, s);
 }
 #endif
 
@@ -809,6 +823,8 @@ __global__ void __launch_bounds__(WAVE) k_sweep_batch(BatchTables B, BatchCommon
     A.U = (const char*)Ubase + I.soff * 8;
     A.M = DIR == DIR_X ? I.nx + 6 : I.ny + 6;
     A.N = DIR == DIR_X ? I.ny + 6 : I.nx + 6;
+    A.ld = A.N;                                   // instances are stored unpitched
+    A.ldd = DIR == DIR_X ? I.ny : I.nx;
     A.nn = DIR == DIR_X ? I.nx : I.ny;
     A.nt = DIR == DIR_X ? I.ny : I.nx;
     A.chunk = DIR == DIR_X ? I.chunk_x : I.chunk_y;
@@ -837,8 +853,8 @@ __global__ void __launch_bounds__(256) k_combine_batch(BatchTables B, const DT* 
     const int64_t P = (I.nx + 6) * (I.ny + 6);
     const int local = b - I.tile0;
     combine_tile<St, DT, TT>(Dx + I.ioff, DyT + I.ioff, make_st<St>(Un, I.soff, P), make_st<St>(Us, I.soff, P),
-                         make_st<St>(out, I.soff, P), make_st<St>(outT, I.soff, P), I.nx, I.ny, stage,
-                         &ctls[i], local / I.tiles_j, local % I.tiles_j);
+                             make_st<St>(out, I.soff, P), make_st<St>(outT, I.soff, P), I.nx, I.ny, I.ny + 6,
+                             I.nx + 6, I.ny, I.nx, stage, &ctls[i], local / I.tiles_j, local % I.tiles_j);
 }
 
 template <class St>
@@ -848,8 +864,8 @@ __global__ void k_bc_batch(BatchTables B, BcParams bp, void* U, void* UT, StepCt
     const int i = B.blk_g[b];
     const BatchInst I = B.inst[i];
     const int64_t P = (I.nx + 6) * (I.ny + 6);
-    bc_cell<St>(make_st<St>(U, I.soff, P), make_st<St>(UT, I.soff, P), true, I.nx, I.ny, bp, &ctls[i],
-                stage, (int64_t)(b - I.ghost_blk0) * ST_TB + threadIdx.x);
+    bc_cell<St>(make_st<St>(U, I.soff, P), make_st<St>(UT, I.soff, P), true, I.nx, I.ny, I.ny + 6, I.nx + 6, bp,
+                &ctls[i], stage, (int64_t)(b - I.ghost_blk0) * ST_TB + threadIdx.x);
 }
 
 template <class St>
@@ -859,7 +875,7 @@ __global__ void __launch_bounds__(ST_TB) k_cfl_batch(BatchTables B, void* U, dou
     const int i = B.blk_f[b];
     const BatchInst I = B.inst[i];
     const int64_t P = (I.nx + 6) * (I.ny + 6);
-    cfl_block<St>(make_st<St>(U, I.soff, P), I.nx, I.ny, gam, &ctls[i], b - I.cfl_blk0, I.cfl_nblk);
+    cfl_block<St>(make_st<St>(U, I.soff, P), I.nx, I.ny, I.ny + 6, gam, &ctls[i], b - I.cfl_blk0, I.cfl_nblk);
 }
 
 __global__ void k_advance_batch(StepCtl* ctls, int n)
@@ -992,6 +1008,10 @@ SweepPlan make_sweep_plan(int64_t nx, int64_t ny, double dx, double gamma, int c
     pick_tiling(nx, ny, p.chunk_x, p.waves_x);
     pick_tiling(ny, nx, p.chunk_y, p.waves_y);
     p.combine_cols = 32;   // measured faster than 64-wide on 1920x480 and no slower at 4096^2
+    p.ld = row_pitch(ny + 6);
+    p.ldT = row_pitch(nx + 6);
+    p.ldx = row_pitch(ny);
+    p.ldyT = row_pitch(nx);
     return p;
 }
 
