@@ -27,9 +27,9 @@
 //
 // Template parameters: T compute type; IN the state encoding read (FP64
 // planes, FP32 hi planes, FP32 hi+lo planes read as FP64); DT the type of D;
-// MODE plain, DETECT (FP32: flags the chunk when a Gauss-point state has
-// rho <= 0 or p <= 0 or is not finite) or FIXUP (FP64 recompute of flagged
-// chunks only, overwriting their D).
+// MODE plain or DETECT (FP32 arithmetic on increments; a face row where a
+// Gauss-point state has rho <= 0 or p <= 0 or is not finite is recomputed
+// in FP64 by its wavefront on the spot).
 //
 // k_combine applies the SSP-RK3 stage combination with L = -(Dx + Dy^T)
 // (riemann.rs:536; Rie_prob_2d_rs.py:183-202) through an LDS tile transpose,
@@ -57,7 +57,7 @@ constexpr int WAVE = 64;
 constexpr int OUT_COLS = WAVE - 6;   // lanes 3..60
 
 enum { IN_F64 = 0, IN_F32 = 1, IN_SPLIT = 2 };
-enum { MODE_PLAIN = 0, MODE_DETECT = 1, MODE_FIXUP = 2 };
+enum { MODE_PLAIN = 0, MODE_DETECT = 1 };
 
 struct SweepArgs {
     const void* U;        // padded planes, M x N each, normal = rows
@@ -68,7 +68,7 @@ struct SweepArgs {
     int waves;            // wavefronts per workgroup (consecutive chunks)
     double gam, dx, two_h, lo, hi;
     void* D;              // interior planes nn x nt (DT), row pitch ldd
-    int* flags;           // one per workgroup (DETECT writes, FIXUP reads)
+    int* flags;           // one per wavefront: DETECT records whether it recomputed a face row in FP64
     const StepCtl* ctl;
 };
 
@@ -113,6 +113,33 @@ template <typename T, int W> struct SweepLds {
     T xF[W][4][WAVE];     // blended flux of the first owned face
 };
 
+// FP64 recompute of one face row of a wavefront (the FP32-split sweep's
+// fallback): pass 1 from the lane's stencil c, through the wavefront's pass-1
+// LDS rows p1, then pass 2 on the Gauss lanes; g receives the Gauss-point
+// states l1, l2, q1, q2 (4 each).
+template <typename T, int DIR, bool CONS>
+__device__ __attribute__((noinline)) void face_row_fp64(const T (*c)[4], T gam, T (*p1)[WAVE], int l,
+                                                        bool gauss_lane, T* g)
+{
+    wave_sync();
+    T ql[4], qr[4];
+    recon_normal(DIR, CONS, c, gam, ql, qr);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        p1[k][l] = ql[k];
+        p1[4 + k][l] = qr[k];
+    }
+    wave_sync();
+    if (gauss_lane) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            weno5_gq(p1[k][l - 2], p1[k][l - 1], p1[k][l], p1[k][l + 1], p1[k][l + 2], g[k], g[4 + k]);
+            weno5_gq(p1[4 + k][l - 2], p1[4 + k][l - 1], p1[4 + k][l], p1[4 + k][l + 1], p1[4 + k][l + 2], g[8 + k],
+                     g[12 + k]);
+        }
+    }
+}
+
 // One workgroup: column block bx, row group by of W consecutive chunks, one
 // wavefront per chunk; `bid` indexes the flags (W per workgroup).
 //
@@ -132,12 +159,6 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                                            SweepLds<T, W>& S)
 {
     if (A.ctl && !A.ctl->active) return;
-    if constexpr (MODE == MODE_FIXUP) {
-        bool any = false;
-#pragma unroll
-        for (int i = 0; i < W; ++i) any = any || A.flags[bid * W + i] != 0;
-        if (!any) return;
-    }
     auto& s_p1 = S.p1;
     auto& s_pb = S.pb;
     auto& s_xpb = S.xpb;
@@ -252,10 +273,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
             wave_sync();
 
             T fc[4], fe[4], pb = T(1);
+            T l1[4], l2[4], q1[4], q2[4];
             if (gauss_lane) {
                 // pass 2: Gauss points along the face from lanes l-2..l+2
                 const T(*sp)[WAVE] = s_p1[wv];
-                T l1[4], l2[4], q1[4], q2[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     if constexpr (MODE == MODE_DETECT) {
@@ -268,9 +289,31 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                                  sp[4 + k][l + 2], q1[k], q2[k]);
                     }
                 }
-                if (MODE == MODE_DETECT)
-                    bad = bad || bad_state(l1, gam) || bad_state(l2, gam) || bad_state(q1, gam) ||
-                          bad_state(q2, gam);
+            }
+            if constexpr (MODE == MODE_DETECT) {
+                // A Gauss-point state with rho <= 0, p <= 0 or not finite on any
+                // lane: the wavefront recomputes this face row in FP64 (pass 1
+                // from the registers' stencil, then pass 2).
+                const bool badf = gauss_lane && (bad_state(l1, gam) || bad_state(l2, gam) || bad_state(q1, gam) ||
+                                                 bad_state(q2, gam));
+                if (__ballot(badf ? 1 : 0) != 0ull) {
+                    bad = true;
+                    // an out-of-line call with copies of the stencil and the
+                    // results, so the FP32 path's registers are not shared
+                    // with the FP64 recompute
+                    T wc[6][4], g[16];
+#pragma unroll
+                    for (int r = 0; r < 6; ++r)
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) wc[r][k] = w[r][k];
+                    face_row_fp64<T, DIR, CONS>(wc, gam, s_p1[wv], l, gauss_lane, g);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        l1[k] = g[k]; l2[k] = g[4 + k]; q1[k] = g[8 + k]; q2[k] = g[12 + k];
+                    }
+                }
+            }
+            if (gauss_lane) {
                 T fa_[4], fb_[4], fcc[4], fd[4];
                 flux_pt2(DIR, l1, l2, gam, fa_, fb_);
                 flux_pt2(DIR, q1, q2, gam, fcc, fd);
@@ -382,8 +425,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
         }
     }
     if (MODE == MODE_DETECT) {
-        unsigned long long m = __ballot(bad ? 1 : 0);
-        if (l == 0) A.flags[bid * W + wv] = m != 0ull ? 1 : 0;
+        if (l == 0) A.flags[bid * W + wv] = bad ? 1 : 0;
     }
 }
 
@@ -674,10 +716,8 @@ static void h_sweep1(const SweepPlan& p, int dir, int split, const void* U, void
     if (split) {
         if (dir == 0) {
             launch_sweep<double, IN_SPLIT, float, MODE_DETECT, DIR_X>(args_x(p, U, D, flags, ctl), p.cons, blend, s);
-            launch_sweep<double, IN_SPLIT, float, MODE_FIXUP, DIR_X>(args_x(p, U, D, flags, ctl), p.cons, blend, s);
         } else {
             launch_sweep<double, IN_SPLIT, float, MODE_DETECT, DIR_Y>(args_y(p, U, D, flags, ctl), p.cons, blend, s);
-            launch_sweep<double, IN_SPLIT, float, MODE_FIXUP, DIR_Y>(args_y(p, U, D, flags, ctl), p.cons, blend, s);
         }
         return;
     }
@@ -696,7 +736,6 @@ static void h_sweep2(const SweepPlan& p, int split, const void* U, const void* U
     if (split) {
         const SweepArgs ax = args_x(p, U, Dx, flags, ctl), ay = args_y(p, UT, DyT, flags, ctl);
         launch_sweeps<double, IN_SPLIT, float, MODE_DETECT>(ax, ay, p.cons, blend, s);
-        launch_sweeps<double, IN_SPLIT, float, MODE_FIXUP>(ax, ay, p.cons, blend, s);
         return;
     }
 #endif
@@ -723,15 +762,14 @@ static void h_combine1(const SweepPlan& p, int split, const void* Dx, const void
 
 #if WX_FAST
 // ---- FP32-split -----------------------------------------------------------
-// Bulk sweep in FP32 from the hi planes; chunks with a non-positive or
-// non-finite Gauss-point state are recomputed in FP64 from hi+lo.
+// Bulk sweep in FP32 on increments of the hi+lo values; face rows with a
+// non-positive or non-finite Gauss-point state are recomputed in FP64.
 static void h_sweeps_split(const SweepPlan& p, const float* Us, const float* UsT, float* Dx,
                            float* DyT, int* flags, const StepCtl* ctl, hipStream_t s)
 {
     const int blend = p.force_hlle ? 0 : 1;
     const SweepArgs ax = args_x(p, Us, Dx, flags, ctl), ay = args_y(p, UsT, DyT, flags, ctl);
     launch_sweeps<double, IN_SPLIT, float, MODE_DETECT>(ax, ay, p.cons, blend, s);
-    launch_sweeps<double, IN_SPLIT, float, MODE_FIXUP>(ax, ay, p.cons, blend, s);
 }
 
 static void h_stage_split(const SweepPlan& p, const float* Us, const float* UsT, const float* Un,
@@ -910,9 +948,7 @@ static void hb_stage(const BatchTables& t, const BatchCommon& c, int split, cons
 #if WX_FAST
     if (split) {
         launch_sweep_batch<double, IN_SPLIT, float, MODE_DETECT, DIR_X>(t, c, Us, Dx, flags, ctls, s);
-        launch_sweep_batch<double, IN_SPLIT, float, MODE_FIXUP, DIR_X>(t, c, Us, Dx, flags, ctls, s);
         launch_sweep_batch<double, IN_SPLIT, float, MODE_DETECT, DIR_Y>(t, c, UsT, DyT, flags, ctls, s);
-        launch_sweep_batch<double, IN_SPLIT, float, MODE_FIXUP, DIR_Y>(t, c, UsT, DyT, flags, ctls, s);
         k_combine_batch<StSplit, float><<<(unsigned)t.n_c, 256, 0, s>>>(
             t, (const float*)Dx, (const float*)DyT, (void*)Un, (void*)Us, out, outT, stage, ctls);
         return;
