@@ -294,19 +294,29 @@ def roofline(res, w, split=False):
 # the halo re-reads depend on the tiling, so other tilings report null.
 _DMR_SRC64 = "profiles/dmr1920x480_fp64.md"
 _DMR_SRC32 = "profiles/dmr1920x480_fp32split.md"
+_QUAD_SRC64 = "profiles/quad4096_fp64.md"
+_QUAD_SRC32 = "profiles/quad4096_fp32split.md"
 TRAFFIC = {
-    ("double_mach_reflection", 1920, 480, False): {"bytes_per_step": 1.0918e9, "tiling": (9, 8, 4, 4),
+    ("double_mach_reflection", 1920, 480, False): {"bytes_per_step": 1.0935e9, "tiling": (9, 8, 4, 4),
                                                    "source": _DMR_SRC64},
-    ("double_mach_reflection", 1920, 480, True): {"bytes_per_step": 0.9710e9, "tiling": (3, 4, 4, 4),
+    ("double_mach_reflection", 1920, 480, True): {"bytes_per_step": 1.1132e9, "tiling": (9, 8, 4, 4),
                                                   "source": _DMR_SRC32},
+    ("four_quadrant_riemann", 4096, 4096, False): {"bytes_per_step": 17.6371e9, "tiling": (48, 48, 1, 1),
+                                                   "source": _QUAD_SRC64},
+    ("four_quadrant_riemann", 4096, 4096, True): {"bytes_per_step": 15.6186e9, "tiling": (48, 48, 1, 1),
+                                                  "source": _QUAD_SRC32},
 }
-# VALU wave-instructions of one k_sweep_xy launch (for FP32-split: its FP32
-# pass and its FP64 recompute launch together)
+# VALU wave-instructions of one k_sweep_xy launch (FP32-split: including its
+# in-launch FP64 recompute of flagged face rows)
 VALU = {
-    ("double_mach_reflection", 1920, 480, False, "sweeps"): {"wave_instr_per_launch": 5.9547e7,
+    ("double_mach_reflection", 1920, 480, False, "sweeps"): {"wave_instr_per_launch": 4.990e7,
                                                              "tiling": (9, 8, 4, 4), "source": _DMR_SRC64},
-    ("double_mach_reflection", 1920, 480, True, "sweeps"): {"wave_instr_per_launch": 8.0803e7,
-                                                            "tiling": (3, 4, 4, 4), "source": _DMR_SRC32},
+    ("double_mach_reflection", 1920, 480, True, "sweeps"): {"wave_instr_per_launch": 6.203e7,
+                                                            "tiling": (9, 8, 4, 4), "source": _DMR_SRC32},
+    ("four_quadrant_riemann", 4096, 4096, False, "sweeps"): {"wave_instr_per_launch": 8.097e8,
+                                                             "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64},
+    ("four_quadrant_riemann", 4096, 4096, True, "sweeps"): {"wave_instr_per_launch": 1.013e9,
+                                                            "tiling": (48, 48, 1, 1), "source": _QUAD_SRC32},
 }
 
 
