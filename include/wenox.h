@@ -41,6 +41,9 @@ extern "C" {
 /* Arithmetic variant of the device kernels. */
 #define WENOX_EXACT 0  /* the reference's operation order, no FMA contraction */
 #define WENOX_FAST 1   /* FMA contraction and shared reciprocals (rounding-level differences) */
+#define WENOX_TWIN 2   /* the reference's NumPy twin WENO_HLLC_2d.py where it differs from the Rust:
+                          exact sqrt(3)/12, its HLLC right-star coefficient and wave speeds, its HLLE
+                          middle state (drop-in functions and the unfused engine only) */
 
 /* A read-only host array view: extents and element (not byte) strides. */
 typedef struct wenox_view3 {
@@ -52,7 +55,8 @@ typedef struct wenox_view3 {
 const char *wenox_last_error(void);
 /* Library version string. */
 const char *wenox_version(void);
-/* Selects the arithmetic variant used by the drop-in functions (default WENOX_EXACT). */
+/* Selects the arithmetic variant used by the drop-in functions (default WENOX_EXACT;
+ * WENOX_TWIN gives the NumPy twin's flux functions, WENO_HLLC_2d.py:435-757). */
 int wenox_set_variant(int variant);
 /* Number of visible HIP devices (0 when none); does not create a context. */
 int wenox_device_count(void);
@@ -125,7 +129,7 @@ typedef struct wenox_solver_desc {
     int dmr_bound;       /* WENOX_BC_DMR_REF: `bound` from unitest.init_refac */
     double dmr_x0;       /* WENOX_BC_DMR_STD: wall start (1/6) */
     int engine;          /* WENOX_ENGINE_* */
-    int variant;         /* WENOX_EXACT / WENOX_FAST */
+    int variant;         /* WENOX_EXACT / WENOX_FAST / WENOX_TWIN (unfused engine, FP64) */
     int precision;       /* WENOX_PREC_* */
     int use_graph;       /* capture one time step in a hipGraph and replay it */
 } wenox_solver_desc;
@@ -161,6 +165,11 @@ int wenox_solver_profile(wenox_solver *s, int reps, double *out_ms, int n_out);
  * (1e-14, 1e-16) by default as Rie_prob_2d_rs.py:311-319; the 1-D driver
  * Rie_prob_1D_rs.py:165-168 uses (1e-10, 0). */
 int wenox_solver_set_cfl_rule(wenox_solver *s, double rho_floor, double cmax_eps);
+/* Pressure floor after every step (pmin > 0; 0 switches it off): where a
+ * cell's pressure is below pmin its total energy is raised to give p = pmin,
+ * over every padded cell.  Replaces con_var2primi_pfloor of the NumPy twin's
+ * time loop (WENO_HLLC_2d.py:813-837, called at :903); FP64 states only. */
+int wenox_solver_set_pfloor(wenox_solver *s, double pmin);
 /* Tiling of the fused engine's sweeps: rows marched per wavefront (chunk) and
  * wavefronts per workgroup along the normal (waves: 1, 2 or 4; consecutive
  * chunks of one workgroup share their sensor halo through LDS, chunk >= 3

@@ -65,7 +65,8 @@ def lib():
         L.orc_test_gq.argtypes = [dp, dp, dp]
         L.orc_test_riemann_pt.argtypes = [i, i, dp, dp, dp, dp, d, dp]
         L.orc_set_threads.argtypes = [i]
-        for f in (L.orc_set_threads, L.orc_flux, L.orc_con2primi, L.orc_weno, L.orc_riemann,
+        L.orc_set_twin.argtypes = [i]
+        for f in (L.orc_set_threads, L.orc_set_twin, L.orc_flux, L.orc_con2primi, L.orc_weno, L.orc_riemann,
                   L.orc_sensor, L.orc_blend, L.orc_l_local, L.orc_test_roe_eig,
                   L.orc_test_pp_limit, L.orc_test_weno5, L.orc_test_gq, L.orc_test_riemann_pt):
             f.restype = None
@@ -77,6 +78,19 @@ def set_threads(n: int) -> None:
     """Threads of the oracle's outer loops (1, the default, is the
     reference's single thread; results do not depend on n)."""
     lib().orc_set_threads(int(n))
+
+
+class twin:
+    """Context manager: the C functions follow the reference's pure-NumPy
+    twin WENO_HLLC_2d.py (exact sqrt(3)/12, the twin's HLLC and HLLE) inside
+    the block, the Rust crate outside it."""
+
+    def __enter__(self):
+        lib().orc_set_twin(1)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_twin(0)
 
 
 def _view(a: np.ndarray) -> _View3:
@@ -439,3 +453,85 @@ def run_riemann(q, nx, ny, dx, gamma, cfl, force_hlle, weno_switch, jp_cri,
             step += 1
             ts.append(T)
     return q, ts
+
+
+# --------------------------------------------------------------------------
+# The NumPy twin's time loop (WENO_HLLC_2d.py:786-905)
+# --------------------------------------------------------------------------
+def twin_quadrants(n, gamma=1.4):
+    """Initial data of WENO_HLLC_2d.main (:846-880): four constant quadrants
+    split at padded index int((n+6)/2), ghost cells included, then the
+    constant-extrapolation ghost fill."""
+    u = np.zeros((n + 6, n + 6, 8))
+    h = int((n + 6) / 2)
+    for (si, sj), (r, vx, vy, p) in (((slice(None, h), slice(None, h)), (0.125, 1, 1, 0.125)),
+                                     ((slice(h, None), slice(None, h)), (0.5, 0, 1, 0.25)),
+                                     ((slice(None, h), slice(h, None)), (0.5, 1, 0, 0.25)),
+                                     ((slice(h, None), slice(h, None)), (1, 0, 0, 1))):
+        u[si, sj, 4], u[si, sj, 5], u[si, sj, 6], u[si, sj, 7] = r, vx, vy, p
+    u[:, :, 0] = u[:, :, 4]
+    u[:, :, 1] = u[:, :, 4] * u[:, :, 5]
+    u[:, :, 2] = u[:, :, 4] * u[:, :, 6]
+    u[:, :, 3] = u[:, :, 7] / (gamma - 1) + 0.5 * u[:, :, 4] * (u[:, :, 5] ** 2 + u[:, :, 6] ** 2)
+    return apply_bc_copy(u, n, n)
+
+
+def twin_pfloor(u, gamma, pmin=1e-8):
+    """con_var2primi_pfloor (:813-837): raise E where p < pmin, then write
+    the primitive channels (rho floored at 1e-12, u, v, p) on every cell."""
+    rho = np.maximum(u[:, :, 0], 1e-12)
+    uvel = u[:, :, 1] / rho
+    vvel = u[:, :, 2] / rho
+    E = u[:, :, 3]
+    kinetic = 0.5 * rho * (uvel ** 2 + vvel ** 2)
+    e_int = E - kinetic
+    p = (gamma - 1) * e_int
+    mask = p < pmin
+    if np.any(mask):
+        e_int[mask] = pmin / (gamma - 1)
+        E[mask] = kinetic[mask] + e_int[mask]
+        p[mask] = pmin
+    u[:, :, 4] = rho
+    u[:, :, 5] = uvel
+    u[:, :, 6] = vvel
+    u[:, :, 7] = p
+    return u
+
+
+def twin_rk3(u, n, dt, gamma, force_hlle, jp_cri):
+    """RK3_correct (:786-811): component-wise L of the twin semantics, ghost
+    copy after every stage; the caller holds the twin context."""
+    dx = 1 / n
+    sl = np.s_[3:n + 3, 3:n + 3, :4]
+    l0 = l_local_py(u, dx, gamma, force_hlle, jp_cri, True)
+    u1 = u.copy()
+    u1[sl] = u[sl] + dt * l0
+    u1 = apply_bc_copy(u1, n, n)
+    l1 = l_local_py(u1, dx, gamma, force_hlle, jp_cri, True)
+    u2 = u.copy()
+    u2[sl] = 0.75 * u[sl] + 0.25 * u1[sl] + 0.25 * dt * l1
+    u2 = apply_bc_copy(u2, n, n)
+    l2 = l_local_py(u2, dx, gamma, force_hlle, jp_cri, True)
+    un = u.copy()
+    un[sl] = 1 / 3 * u[sl] + 2 / 3 * u2[sl] + 2 / 3 * dt * l2
+    return apply_bc_copy(un, n, n)
+
+
+def run_twin(u, n, n_step, force_hlle=False, jp_cri=(1.0, 4.0), gamma=1.4, cfl=0.2):
+    """The loop of WENO_HLLC_2d.main (:895-905) without I/O: for i in
+    range(1, n_step), dt = CFL dx / max(sqrt(g p / max(rho, 1e-10)) + |V|)
+    from the primitive channels, RK3_correct, pressure floor.  Returns (u,
+    list of T)."""
+    dx = 1 / n
+    T, ts = 0, [0.0]
+    with twin():
+        for _ in range(1, n_step):
+            speed = np.sqrt(u[3:n + 3, 3:n + 3, 5] ** 2 + u[3:n + 3, 3:n + 3, 6] ** 2)
+            c_max = np.max(np.sqrt(gamma * u[3:n + 3, 3:n + 3, 7] / np.maximum(u[3:n + 3, 3:n + 3, 0], 1e-10))
+                           + speed)
+            dt = cfl * dx / c_max
+            u = twin_rk3(u, n, dt, gamma, force_hlle, jp_cri)
+            u = twin_pfloor(u, gamma)
+            T += dt
+            ts.append(float(T))
+    return u, ts

@@ -20,6 +20,13 @@
  *   - x.powi(2) is x*x;
  *   - the truncated constant SQRT3_12 = 0.1443375673 (utils.rs:4).
  *
+ * orc_set_twin(1) switches to the reference's pure-NumPy twin
+ * WENO_HLLC_2d.py where the two differ: the exact sqrt(3)/12, and the
+ * twin's HLLC and HLLE (hllc_twin_pt / hlle_twin_pt below).  Its
+ * component-wise reconstruction, fluxes, sensor and blend are those of the
+ * Rust crate's component-wise path operation for operation (for finite
+ * inputs: numpy's maximum propagates NaN where fmax does not).
+ *
  * Each function names the reference lines it follows.  Functions whose result
  * no independent check pins are listed as "parity unpinned" in the test files.
  *
@@ -54,6 +61,18 @@ static const double ORC_EPS = 1e-12;
 static const double ORC_SQRT3_12 = 0.1443375673;
 static const double ORC_RHO_MIN = 1e-12;
 static const double ORC_P_MIN = 1e-12;
+
+/* Twin semantics (orc_set_twin(1)): the reference's pure-NumPy twin
+ * WENO_HLLC_2d.py where it differs from the Rust crate -- the exact
+ * np.sqrt(3)/12 in the Gauss-point pass (:5), and its own HLLC (:435-574)
+ * and HLLE (:576-669).  Off by default. */
+static int orc_twin = 0;
+static double orc_sqrt3_12 = 0.1443375673;
+void orc_set_twin(int on)
+{
+    orc_twin = on != 0;
+    orc_sqrt3_12 = orc_twin ? sqrt(3.0) / 12.0 : ORC_SQRT3_12;
+}
 
 static inline double sq(double x) { return x * x; }
 
@@ -254,9 +273,9 @@ static void weno5_gq(double q0, double q1, double q2, double q3, double q4, doub
         double a2 = 0.1960481952 / sq(ORC_EPS + b2);
         double s = a0 + a1 + a2;
         double w0 = a0 / s, w1 = a1 / s, w2 = a2 / s;
-        double p0 = q2 + (3.0 * q2 - 4.0 * q3 + q4) * ORC_SQRT3_12;
-        double p1 = q2 - (-q1 + q3) * ORC_SQRT3_12;
-        double p2 = q2 - (3.0 * q2 - 4.0 * q1 + q0) * ORC_SQRT3_12;
+        double p0 = q2 + (3.0 * q2 - 4.0 * q3 + q4) * orc_sqrt3_12;
+        double p1 = q2 - (-q1 + q3) * orc_sqrt3_12;
+        double p2 = q2 - (3.0 * q2 - 4.0 * q1 + q0) * orc_sqrt3_12;
         *ga = w0 * p0 + w1 * p1 + w2 * p2;
     }
     {
@@ -265,9 +284,9 @@ static void weno5_gq(double q0, double q1, double q2, double q3, double q4, doub
         double a2 = 0.1928406937 / sq(ORC_EPS + b2);
         double s = a0 + a1 + a2;
         double w0 = a0 / s, w1 = a1 / s, w2 = a2 / s;
-        double p0 = q2 - (3.0 * q2 - 4.0 * q3 + q4) * ORC_SQRT3_12;
-        double p1 = q2 - (q1 - q3) * ORC_SQRT3_12;
-        double p2 = q2 + (3.0 * q2 - 4.0 * q1 + q0) * ORC_SQRT3_12;
+        double p0 = q2 - (3.0 * q2 - 4.0 * q3 + q4) * orc_sqrt3_12;
+        double p1 = q2 - (q1 - q3) * orc_sqrt3_12;
+        double p2 = q2 + (3.0 * q2 - 4.0 * q1 + q0) * orc_sqrt3_12;
         *gb = w0 * p0 + w1 * p1 + w2 * p2;
     }
 }
@@ -401,11 +420,109 @@ void orc_weno(const orc_view3 *u, double gamma, int dir, int cons,
 }
 
 /* ------------------------------------------------------------------ */
+/* Twin semantics: WENO_HLLC_2d.py's Riemann solvers, one face.        */
+/* ------------------------------------------------------------------ */
+/* con2primi (WENO_HLLC_2d.py:400-432) of one face state plus the
+ * enthalpy the solvers form from it (:445-446) */
+typedef struct { double rho, u, v, p, h; } orc_twin_prim;
+
+static orc_twin_prim twin_prim(const double q[4], double g)
+{
+    orc_twin_prim s;
+    s.rho = fmax(q[0], ORC_EPS);
+    s.u = q[1] / s.rho;
+    s.v = q[2] / s.rho;
+    s.p = fmax((g - 1.0) * (q[3] - 0.5 * s.rho * (s.u * s.u + s.v * s.v)), ORC_EPS);
+    s.h = (q[3] + s.p) / s.rho;
+    return s;
+}
+
+/* Roe-weighted velocity and enthalpy (:450-452 / :524-526); the sound
+ * speed is left to the caller (HLLC takes the root unguarded, HLLE of
+ * max(0, .)) */
+static void twin_roe(const orc_twin_prim *l, const orc_twin_prim *r, double *ut, double *vt, double *ht)
+{
+    double a = sqrt(l->rho), b = sqrt(r->rho);
+    *ut = (a * l->u + b * r->u) / (a + b);
+    *vt = (a * l->v + b * r->v) / (a + b);
+    *ht = (a * l->h + b * r->h) / (a + b);
+}
+
+/* numpy guard of a denominator: eps * sign(d) + eps where |d| < eps, with
+ * sign(+-0) = 0 (so a small negative d becomes 0) */
+static double twin_guard(double d)
+{
+    if (fabs(d) < ORC_EPS) d = ORC_EPS * (d > 0.0 ? 1.0 : (d < 0.0 ? -1.0 : 0.0)) + ORC_EPS;
+    return d;
+}
+
+/* HLLC_x / HLLC_y (:435-508 / :510-574).  The star states use the twin's
+ * coefficients -- right: rho_R (S* - w_R) / (S_R - S*) -- and the four
+ * masks are assigned in order, later ones overwriting earlier ones.  Where
+ * no mask holds (NaN wave speeds) the twin leaves np.empty memory; NaN here. */
+static void hllc_twin_pt(int dir, const double ql[4], const double fl[4], const double qr[4],
+                         const double fr[4], double g, double out[4])
+{
+    orc_twin_prim L = twin_prim(ql, g), R = twin_prim(qr, g);
+    double ut, vt, ht;
+    twin_roe(&L, &R, &ut, &vt, &ht);
+    double at = sqrt((g - 1.0) * (ht - 0.5 * (ut * ut + vt * vt)));
+    double wl = dir == 0 ? L.u : L.v, wr = dir == 0 ? R.u : R.v;
+    double SL = (dir == 0 ? ut : vt) - at, SR = (dir == 0 ? ut : vt) + at;
+    double den = twin_guard(L.rho * (SL - wl) - R.rho * (SR - wr));
+    double Ss = (R.p - L.p + L.rho * wl * (SL - wl) - R.rho * wr * (SR - wr)) / den;
+    double cl = L.rho * (SL - wl) / (SL - Ss);
+    double el = ql[3] / L.rho + (Ss - wl) * (Ss + L.p / (L.rho * (SL - wl)));
+    double cr = R.rho * (Ss - wr) / (SR - Ss);
+    double er = qr[3] / R.rho + (Ss - wr) * (Ss + R.p / (R.rho * (SR - wr)));
+    double sl[4], sr[4];
+    sl[0] = cl;
+    sl[1] = dir == 0 ? cl * Ss : cl * L.u;
+    sl[2] = dir == 0 ? cl * L.v : cl * Ss;
+    sl[3] = cl * el;
+    sr[0] = cr;
+    sr[1] = dir == 0 ? cr * Ss : cr * R.u;
+    sr[2] = dir == 0 ? cr * R.v : cr * Ss;
+    sr[3] = cr * er;
+    for (int k = 0; k < 4; ++k) out[k] = NAN;
+    if (SL >= 0.0)
+        for (int k = 0; k < 4; ++k) out[k] = fl[k];
+    if (SL < 0.0 && Ss >= 0.0)
+        for (int k = 0; k < 4; ++k) out[k] = fl[k] + SL * (sl[k] - ql[k]);
+    if (Ss < 0.0 && SR > 0.0)
+        for (int k = 0; k < 4; ++k) out[k] = fr[k] + SR * (sr[k] - qr[k]);
+    if (SR <= 0.0)
+        for (int k = 0; k < 4; ++k) out[k] = fr[k];
+}
+
+/* HLLE_x / HLLE_y (:576-623 / :625-669): masks SL >= 0 then SR <= 0; the
+ * middle state divides by the raw SR - SL (the guarded denominator of
+ * :602-603 is never used) and differences the unfloored face states */
+static void hlle_twin_pt(int dir, const double ql[4], const double fl[4], const double qr[4],
+                         const double fr[4], double g, double out[4])
+{
+    orc_twin_prim L = twin_prim(ql, g), R = twin_prim(qr, g);
+    double ut, vt, ht;
+    twin_roe(&L, &R, &ut, &vt, &ht);
+    double at = sqrt(fmax(0.0, (g - 1.0) * (ht - 0.5 * (ut * ut + vt * vt))));
+    double SL = (dir == 0 ? ut : vt) - at, SR = (dir == 0 ? ut : vt) + at;
+    int ml = SL >= 0.0, mr = SR <= 0.0;
+    if (ml)
+        for (int k = 0; k < 4; ++k) out[k] = fl[k];
+    if (mr)
+        for (int k = 0; k < 4; ++k) out[k] = fr[k];
+    if (!ml && !mr)
+        for (int k = 0; k < 4; ++k)
+            out[k] = (SR * fl[k] - SL * fr[k] + SR * SL * (qr[k] - ql[k])) / (SR - SL);
+}
+
+/* ------------------------------------------------------------------ */
 /* riemann.rs:6-189  HLLC (Toro) with Roe-average wave speeds          */
 /* ------------------------------------------------------------------ */
 static void hllc_pt(int dir, const double qlr[4], const double flv[4], const double qrr[4],
                     const double frv[4], double g, double out[4])
 {
+    if (orc_twin) { hllc_twin_pt(dir, qlr, flv, qrr, frv, g, out); return; }
     double ql0 = fmax(qlr[0], ORC_EPS), ql1 = qlr[1], ql2 = qlr[2], ql3 = qlr[3];
     double qr0 = fmax(qrr[0], ORC_EPS), qr1 = qrr[1], qr2 = qrr[2], qr3 = qrr[3];
     double u_l = ql1 / ql0, v_l = ql2 / ql0;
@@ -459,6 +576,7 @@ static void hllc_pt(int dir, const double qlr[4], const double flv[4], const dou
 static void hlle_pt(int dir, const double qlr[4], const double flv[4], const double qrr[4],
                     const double frv[4], double g, double out[4])
 {
+    if (orc_twin) { hlle_twin_pt(dir, qlr, flv, qrr, frv, g, out); return; }
     double ql0 = fmax(qlr[0], ORC_EPS), ql1 = qlr[1], ql2 = qlr[2], ql3 = qlr[3];
     double qr0 = fmax(qrr[0], ORC_EPS), qr1 = qrr[1], qr2 = qrr[2], qr3 = qrr[3];
     double u_l = ql1 / ql0, v_l = ql2 / ql0;

@@ -39,7 +39,10 @@ int fail(int code, const std::string& msg)
 
 int g_variant = WENOX_EXACT;
 
-const Ops* ops_for(int variant) { return variant == WENOX_FAST ? &ops_fast : &ops_exact; }
+const Ops* ops_for(int variant)
+{
+    return variant == WENOX_FAST ? &ops_fast : (variant == WENOX_TWIN ? &ops_twin : &ops_exact);
+}
 
 // Checks that a gfx950 device is current; the code objects are built for it only.
 int require_device()
@@ -257,7 +260,8 @@ extern "C" const char* wenox_version(void) { return "wenox 0.1.0 (gfx950)"; }
 
 extern "C" int wenox_set_variant(int variant)
 {
-    if (variant != WENOX_EXACT && variant != WENOX_FAST) return fail(WENOX_E_ARG, "variant must be 0 or 1");
+    if (variant != WENOX_EXACT && variant != WENOX_FAST && variant != WENOX_TWIN)
+        return fail(WENOX_E_ARG, "variant must be 0 (exact), 1 (fast) or 2 (twin)");
     g_variant = variant;
     return WENOX_OK;
 }
@@ -442,6 +446,7 @@ struct wenox_solver {
     int* flags = nullptr;    // FP32-split: per-workgroup FP64-recompute flags
     double cfl_rho_floor = 1e-14;   // Rie_prob_2d_rs.py:314
     double cfl_cmax_eps = 1e-16;    // Rie_prob_2d_rs.py:315
+    double pfloor = 0.0;            // > 0: the twin's pressure floor after every step
 };
 
 namespace {
@@ -489,6 +494,11 @@ void enqueue_step(wenox_solver* s)
             o->rk(s->U, src[i], s->L, dst[i], d.nx, d.ny, s->ld, stage, s->ctl, st);
         }
         o->bc(dst[i], dstT[i], d.nx, d.ny, s->ld, s->ldT, s->bp, s->ctl, stage, st);
+    }
+    if (s->pfloor > 0.0) {
+        // WENO_HLLC_2d.py:903 (main): con_var2primi_pfloor after each step, every padded cell
+        o->pfloor(s->U, d.nx + 6, d.ny + 6, s->ld, d.gamma, s->pfloor, s->ctl, 1, st);
+        if (s->UT) o->pfloor(s->UT, d.ny + 6, d.nx + 6, s->ldT, d.gamma, s->pfloor, s->ctl, 0, st);
     }
     o->advance(s->ctl, st);
 }
@@ -554,7 +564,12 @@ extern "C" int wenox_solver_create(const wenox_solver_desc* desc, wenox_solver**
     if (d.bc < 0 || d.bc > 4) return fail(WENOX_E_ARG, "unknown boundary kind");
     if (d.bc == WENOX_BC_DMR_REF && d.nx + 2 < d.dmr_bound + 10)
         return fail(WENOX_E_SHAPE, "DMR reference ghost fill needs nx + 2 >= bound + 10");
-    if (d.variant != WENOX_EXACT && d.variant != WENOX_FAST) return fail(WENOX_E_ARG, "unknown variant");
+    if (d.variant != WENOX_EXACT && d.variant != WENOX_FAST && d.variant != WENOX_TWIN)
+        return fail(WENOX_E_ARG, "unknown variant");
+    if (d.variant == WENOX_TWIN && (d.engine != WENOX_ENGINE_UNFUSED || d.precision != WENOX_PREC_FP64))
+        return fail(WENOX_E_ARG, "the twin variant runs on the unfused engine in FP64");
+    if (d.variant == WENOX_TWIN && !d.cons_switch)
+        return fail(WENOX_E_ARG, "the twin variant reconstructs component-wise (cons_switch = 1)");
     if (d.engine != WENOX_ENGINE_UNFUSED && d.engine != WENOX_ENGINE_FUSED)
         return fail(WENOX_E_ARG, "unknown engine");
     if (d.precision != WENOX_PREC_FP64 && d.precision != WENOX_PREC_FP32SPLIT)
@@ -913,6 +928,17 @@ extern "C" int wenox_solver_set_cfl_rule(wenox_solver* s, double rho_floor, doub
     h.rho_floor = rho_floor;
     h.cmax_eps = cmax_eps;
     return write_ctl(s, &h);
+}
+
+extern "C" int wenox_solver_set_pfloor(wenox_solver* s, double pmin)
+{
+    if (!s || !(pmin >= 0.0)) return fail(WENOX_E_ARG, "wenox_solver_set_pfloor: pmin >= 0");
+    if (pmin > 0.0 && s->split) return fail(WENOX_E_ARG, "the pressure floor applies to FP64 states");
+    s->pfloor = pmin;
+    // the captured step graph lacks (or holds) the floor launch
+    if (s->exec) { (void)hipGraphExecDestroy(s->exec); s->exec = nullptr; }
+    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    return WENOX_OK;
 }
 
 extern "C" int wenox_solver_plan(wenox_solver* s, int* chunk_x, int* chunk_y)

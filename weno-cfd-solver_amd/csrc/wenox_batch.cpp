@@ -120,6 +120,7 @@ extern "C" int wenox_batch_create(const wenox_solver_desc* desc, int n, const in
     for (int i = 0; i < n; ++i)
         if (nx[i] < 3 || ny[i] < 3 || !(dx[i] > 0)) return abi_fail(WENOX_E_SHAPE, "each instance needs nx, ny >= 3, dx > 0");
     if (chunk < 1) chunk = 16;
+    if (d.variant == WENOX_TWIN) return abi_fail(WENOX_E_ARG, "the twin variant runs on the single-problem unfused engine");
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
         return abi_fail(WENOX_E_DEVICE, "no HIP device visible: the wenox kernels need an MI355X (gfx950)");

@@ -1,7 +1,9 @@
 // wenox_kernels.hip -- gfx950 kernels of the 2D WENO5 / HLLC hot path.
 //
-// Compiled twice: -DWX_FAST=0 -ffp-contract=off (exports wenox::ops_exact) and
-// -DWX_FAST=1 -ffp-contract=fast (exports wenox::ops_fast).
+// Compiled three times: -DWX_FAST=0 -ffp-contract=off (exports
+// wenox::ops_exact), -DWX_FAST=1 -ffp-contract=fast (wenox::ops_fast) and
+// -DWX_FAST=0 -DWX_TWIN=1 -ffp-contract=off (wenox::ops_twin, the NumPy twin's
+// semantics, see wenox_numerics.h).
 //
 // This file holds the layer-by-layer kernels that mirror the reference's
 // passes (weno_ext/src/weno.rs pass 1 / pass 2, riemann.rs solvers, sensor and
@@ -12,7 +14,10 @@
 #include "wenox_numerics.h"
 #include "wenox_ops.h"
 
-#if WX_FAST
+#if WX_TWIN
+#define WX_NS twink
+#define WX_OPS ops_twin
+#elif WX_FAST
 #define WX_NS fastk
 #define WX_OPS ops_fast
 #else
@@ -168,7 +173,7 @@ __global__ void k_riemann(Acc QL, Acc FL, Acc QR, Acc FR, int64_t F1, int64_t F2
     load4(QL, i, j, ql); load4(FL, i, j, fl); load4(QR, i, j, qr); load4(FR, i, j, fr);
     FaceState<double> s = face_speeds(dir, ql, qr, gam);
     if (kind == 0) hllc_from(dir, s, ql, fl, qr, fr, o);
-    else hlle_from(s, fl, fr, o);
+    else hlle_from(s, ql, fl, qr, fr, o);
 #pragma unroll
     for (int k = 0; k < 4; ++k) out(i, j, k) = o[k];
 }
@@ -209,7 +214,7 @@ __global__ void k_blend(Acc QL, Acc FL, Acc QR, Acc FR, const double* pbar, int6
     load4(QL, i, j, ql); load4(FL, i, j, fl); load4(QR, i, j, qr); load4(FR, i, j, fr);
     FaceState<double> s = face_speeds(dir, ql, qr, gam);
     hllc_from(dir, s, ql, fl, qr, fr, fc);
-    hlle_from(s, fl, fr, fe);
+    hlle_from(s, ql, fl, qr, fr, fe);
     double w = sensor_at(pbar, F1, F2, i, j, two_h, lo, hi);
 #pragma unroll
     for (int k = 0; k < 4; ++k) out(i, j, k) = w * fc[k] + (1.0 - w) * fe[k];
@@ -322,6 +327,47 @@ __global__ void k_advance(StepCtl* ctl)
 
 static void h_advance(StepCtl* ctl, hipStream_t s) { k_advance<<<1, 64, 0, s>>>(ctl); }
 
+// WENO_HLLC_2d.py:813-837 (con_var2primi_pfloor): where the pressure of a cell
+// (density floored at 1e-12 for the velocities and the kinetic energy) falls
+// below pmin, raise its total energy so that p = pmin.  Every padded cell of
+// the M x N planes (row pitch ld), after an active step.  With cfl != 0 the
+// interior cells also give the next step's wave speed as the twin's loop
+// forms it from the primitive channels this function writes (:895-897):
+// sqrt(g p / max(rho, rho_floor)) + sqrt(u^2 + v^2) with p = pmin where the
+// floor acted; the maximum (NaN-propagating, as np.max) goes to
+// ctl->floor_bits by a bit-pattern atomicMax (values are >= 0).
+__global__ void k_pfloor(double* U, int64_t M, int64_t N, int64_t ld, double gam, double pmin, StepCtl* ctl,
+                         int cfl)
+{
+    if (!ctl->active) return;
+    const int64_t t = (int64_t)blockIdx.x * TB + threadIdx.x;
+    double w = 0.0;
+    if (t < M * N) {
+        const int64_t i = t / N, j = t - i * N, o = i * ld + j, P = M * ld;
+        const double rho = fmax(U[o], 1e-12);
+        const double u = U[P + o] / rho, v = U[2 * P + o] / rho;
+        const double kinetic = 0.5 * rho * (u * u + v * v);
+        double p = (gam - 1.0) * (U[3 * P + o] - kinetic);
+        if (p < pmin) {
+            U[3 * P + o] = kinetic + pmin / (gam - 1.0);
+            p = pmin;
+        }
+        if (cfl && i >= 3 && i < M - 3 && j >= 3 && j < N - 3)
+            w = sqrt(gam * p / fmax(U[o], ctl->rho_floor)) + sqrt(u * u + v * v);
+    }
+    if (!cfl) return;
+    w = wave_max(w);
+    if ((threadIdx.x & 63) == 0) atomicMax(&ctl->floor_bits, (unsigned long long)__double_as_longlong(w));
+    if (t == 0) ctl->floor_valid = 1;
+}
+
+static void h_pfloor(double* U, int64_t M, int64_t N, int64_t ld, double gam, double pmin, StepCtl* ctl, int cfl,
+                     hipStream_t s)
+{
+    if (M * N == 0) return;
+    k_pfloor<<<blocks_for(M * N), TB, 0, s>>>(U, M, N, ld, gam, pmin, ctl, cfl);
+}
+
 // ------------------------------------------------------------------------
 // Unfused RHS for the stepper: the reference's pass structure on SoA4.
 // work layout: pl, pr (x pass 1), then ql, fl, qr, fr (x faces), fx; then the
@@ -397,9 +443,10 @@ static void h_rk(const double* Un, const double* Us, const double* L, double* ou
 const Ops WX_OPS = {
     WX_NS::h_flux, WX_NS::h_con2primi, WX_NS::h_snapshot8, WX_NS::h_weno, WX_NS::h_riemann, WX_NS::h_divergence,
     WX_NS::h_bc,   WX_NS::h_cfl,       WX_NS::h_advance, WX_NS::h_rhs_unfused, WX_NS::h_rk,
+    WX_NS::h_pfloor,
 };
 
-#if !WX_FAST
+#if !WX_FAST && !WX_TWIN
 size_t rhs_unfused_scratch(int64_t nx, int64_t ny)
 {
     const int64_t m = nx + 6, n = ny + 6;

@@ -14,6 +14,15 @@
 //                 seven, reciprocals shared across a face) and FMA
 //                 contraction.  Results differ from "exact" by rounding only.
 //
+// A third build, WX_TWIN == 1 (with WX_FAST == 0), follows the reference's
+// pure-NumPy twin WENO_HLLC_2d.py instead of the Rust crate where the two
+// differ: the exact sqrt(3)/12 in the Gauss-point pass (:5), HLLC's
+// right-star coefficient rho_R (S* - u_R) / (S_R - S*) and wave speeds
+// without the max(0, .) under the root (:453,476,528,547) with its masks
+// assigned in order (later regions overwrite earlier ones, :493-506), and
+// HLLE's middle state over the raw S_R - S_L with the unfloored face states,
+// S_R <= 0 taking F_R (:601-621,647-667).
+//
 // T is double (FP64 path) or float (the bulk arithmetic of the FP32 path).
 #pragma once
 #include <hip/hip_runtime.h>
@@ -22,6 +31,12 @@
 
 #ifndef WX_FAST
 #error "WX_FAST must be defined to 0 or 1"
+#endif
+#ifndef WX_TWIN
+#define WX_TWIN 0
+#endif
+#if WX_TWIN && WX_FAST
+#error "the twin semantics are built with the exact operation order only"
 #endif
 
 #define WX_HD __host__ __device__ __forceinline__
@@ -33,7 +48,11 @@ enum Dir : int { DIR_X = 0, DIR_Y = 1 };
 // utils.rs:3-6
 template <typename T> struct K {
     static constexpr T eps = T(1e-12);        // DEFAULT_EPS = RHO_MIN = P_MIN
+#if WX_TWIN
+    static constexpr T sqrt3_12 = T(0.14433756729740643);   // np.sqrt(3)/12 (WENO_HLLC_2d.py:5)
+#else
     static constexpr T sqrt3_12 = T(0.1443375673);
+#endif
 };
 
 template <typename T> WX_HD T sq(T x) { return x * x; }
@@ -712,6 +731,7 @@ template <typename T> struct FaceState {
     T u_l, v_l, u_r, v_r, p_l, p_r;
     T s_l, s_r;
     T il, ir;                                   // 1 / floored densities (fast variant)
+    T s_lc, s_rc;                               // HLLC's speeds (twin: no max(0, .) under the root)
 };
 
 template <typename T> WX_HD FaceState<T> face_speeds(int dir, const T ql[4], const T qr[4], T gam)
@@ -758,6 +778,14 @@ template <typename T> WX_HD FaceState<T> face_speeds(int dir, const T ql[4], con
     T wt = dir == DIR_X ? ut : vt;
     s.s_l = wt - at;
     s.s_r = wt + at;
+#if WX_TWIN
+    const T atc = sqrt((gam - T(1)) * tmp);
+    s.s_lc = wt - atc;
+    s.s_rc = wt + atc;
+#else
+    s.s_lc = s.s_l;
+    s.s_rc = s.s_r;
+#endif
     return s;
 }
 
@@ -768,6 +796,44 @@ template <typename T> WX_HD T guard_denom(T d)
     return d;
 }
 
+#if WX_TWIN
+// WENO_HLLC_2d.py:435-574: HLLC of the NumPy twin (see the file header).
+template <typename T> WX_HD void hllc_from(int dir, const FaceState<T>& s, const T ql[4], const T fl[4],
+                                           const T qr[4], const T fr[4], T out[4])
+{
+    const T wl = dir == DIR_X ? s.u_l : s.v_l;
+    const T wr = dir == DIR_X ? s.u_r : s.v_r;
+    const T sl = s.s_lc, sr = s.s_rc;
+    T den = s.ql0 * (sl - wl) - s.qr0 * (sr - wr);
+    // numpy's guard: eps * sign(d) + eps, sign(+-0) = 0
+    if (fabs(den) < K<T>::eps) den = (den > T(0) ? K<T>::eps : (den < T(0) ? -K<T>::eps : T(0))) + K<T>::eps;
+    const T s_st = (s.p_r - s.p_l + s.ql0 * wl * (sl - wl) - s.qr0 * wr * (sr - wr)) / den;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[k] = T(NAN);   // no region: the twin leaves F unset
+    if (sl >= T(0)) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = fl[k];
+    }
+    if (sl < T(0) && s_st >= T(0)) {
+        T cf = s.ql0 * (sl - wl) / (sl - s_st);
+        T e = s.ql3 / s.ql0 + (s_st - wl) * (s_st + s.p_l / (s.ql0 * (sl - wl)));
+        T st[4] = {cf, dir == DIR_X ? cf * s_st : cf * s.u_l, dir == DIR_X ? cf * s.v_l : cf * s_st, cf * e};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = fl[k] + sl * (st[k] - ql[k]);
+    }
+    if (s_st < T(0) && sr > T(0)) {
+        T cf = s.qr0 * (s_st - wr) / (sr - s_st);   // the twin's right-star coefficient
+        T e = s.qr3 / s.qr0 + (s_st - wr) * (s_st + s.p_r / (s.qr0 * (sr - wr)));
+        T st[4] = {cf, dir == DIR_X ? cf * s_st : cf * s.u_r, dir == DIR_X ? cf * s.v_r : cf * s_st, cf * e};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = fr[k] + sr * (st[k] - qr[k]);
+    }
+    if (sr <= T(0)) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = fr[k];
+    }
+}
+#else
 // riemann.rs:6-96 (x) / :99-189 (y): HLLC.  ql/qr are the raw (unfloored)
 // face states; the star-state difference uses them as given.
 template <typename T> WX_HD void hllc_from(int dir, const FaceState<T>& s, const T ql[4], const T fl[4],
@@ -819,9 +885,31 @@ template <typename T> WX_HD void hllc_from(int dir, const FaceState<T>& s, const
 #undef WX_P_TERM
 }
 
+#endif
+
 // riemann.rs:191-269 (x) / :271-349 (y): HLLE with floored densities.
-template <typename T> WX_HD void hlle_from(const FaceState<T>& s, const T fl[4], const T fr[4], T out[4])
+// ql / qr are the raw face states (used by the twin build only).
+template <typename T> WX_HD void hlle_from(const FaceState<T>& s, const T ql[4], const T fl[4], const T qr[4],
+                                           const T fr[4], T out[4])
 {
+#if WX_TWIN
+    // WENO_HLLC_2d.py:576-669: masks SL >= 0, then SR <= 0 (assigned later,
+    // so it wins where both hold), middle over the raw SR - SL with the
+    // unfloored states
+    if (s.s_r <= T(0)) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = fr[k];
+    } else if (s.s_l >= T(0)) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = fl[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            out[k] = (s.s_r * fl[k] - s.s_l * fr[k] + s.s_r * s.s_l * (qr[k] - ql[k])) / (s.s_r - s.s_l);
+    }
+#else
+    (void)ql;
+    (void)qr;
     T den = guard_denom(s.s_r - s.s_l);
     if (s.s_l >= T(0)) {
 #pragma unroll
@@ -843,6 +931,7 @@ template <typename T> WX_HD void hlle_from(const FaceState<T>& s, const T fl[4],
             out[k] = (s.s_r * fl[k] - s.s_l * fr[k] + s.s_r * s.s_l * (b[k] - a[k])) / den;
 #endif
     }
+#endif
 }
 
 // riemann.rs:389-407  blend weight from the sensor value.  inv_span =

@@ -44,7 +44,8 @@ struct StepCtl {
     int use_cmax0;         // next CFL uses cmax0 instead of the state
     unsigned long long cmax_bits;   // running max of c+|V| (bit pattern, >= 0)
     unsigned int blocks_done;       // last-block detection of the reduction
-    int pad;
+    int floor_valid;                // floor_bits holds the next step's c_max (pressure floor ran)
+    unsigned long long floor_bits;  // max of c+|V| from the floored primitives (bit pattern)
     double* part;                   // per-block maxima of the CFL reduction (nullptr: atomics)
 };
 
@@ -112,11 +113,19 @@ struct Ops {
     // RK stage combine on the SoA4 interior (stage 1..3), skipped when !ctl->active.
     void (*rk)(const double* Un, const double* Us, const double* L, double* out, int64_t nx,
                int64_t ny, int64_t ld, int stage, const StepCtl* ctl, hipStream_t s);
+    // pressure floor of the NumPy twin's time loop on every cell of M x N
+    // planes (row pitch ld), skipped when !ctl->active.  With cfl != 0 it also
+    // reduces max(sqrt(g p / max(rho, rho_floor)) + |V|) over the interior
+    // (three ghost layers) from the floored primitives into ctl->floor_bits,
+    // which the next step's dt then uses (the twin's CFL reads them).
+    void (*pfloor)(double* U, int64_t M, int64_t N, int64_t ld, double gam, double pmin, StepCtl* ctl,
+                   int cfl, hipStream_t s);
 };
 
 size_t rhs_unfused_scratch(int64_t nx, int64_t ny);
 
 extern const Ops ops_exact;
 extern const Ops ops_fast;
+extern const Ops ops_twin;
 
 }  // namespace wenox

@@ -202,17 +202,25 @@ __global__ void k_bc(St U, St UT, bool has_t, int64_t nx, int64_t ny, int64_t ld
 }
 
 // dt and the loop condition from the step's wave-speed maximum
-// (Rie_prob_2d_rs.py:335-336 loop condition, then :315-318).
+// (Rie_prob_2d_rs.py:335-336 loop condition, then :315-318).  The wave speed
+// is, in order of precedence: the caller's primitives at upload (first
+// step), the pressure floor's reduction after the previous step, the state's
+// own reduction.  A step that does not run leaves the first two for the next
+// one that does.
 __device__ __forceinline__ void cfl_finish(StepCtl* ctl, double cmax_state)
 {
-    double cmax = ctl->use_cmax0 ? ctl->cmax0 : cmax_state;
-    ctl->use_cmax0 = 0;
     ctl->cmax_bits = 0ull;
     ctl->blocks_done = 0u;
     double T = ctl->T;
     bool run = ctl->step < ctl->max_steps;
     if (ctl->t_final >= 0.0) run = run && (T < ctl->t_final - 1e-12);
     ctl->active = run ? 1 : 0;
+    if (!run) return;
+    double cmax = ctl->use_cmax0 ? ctl->cmax0
+                : (ctl->floor_valid ? __longlong_as_double((long long)ctl->floor_bits) : cmax_state);
+    ctl->use_cmax0 = 0;
+    ctl->floor_valid = 0;
+    ctl->floor_bits = 0ull;
     double c_max = cmax + ctl->cmax_eps;
     double dt = ctl->cfl_dx / c_max;
     if (ctl->t_final >= 0.0) {

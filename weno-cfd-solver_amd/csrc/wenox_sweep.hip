@@ -326,7 +326,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                     QR[k] = T(0.5) * (q1[k] + q2[k]);
                 }
                 FaceState<T> st = face_speeds(DIR, QL, QR, gam);
-                hlle_from(st, FL, FR, fe);
+                hlle_from(st, QL, FL, QR, FR, fe);
                 if (BLEND) {
                     hllc_from(DIR, st, QL, FL, QR, FR, fc);
                     // floored pressures of QL and QR (riemann.rs:421-430), as formed by face_speeds

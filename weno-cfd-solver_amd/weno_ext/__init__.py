@@ -189,8 +189,16 @@ def l_local_py(u, dx, gamma, force_hlle, jp_cri, cons_switch=False):
     return out
 
 
-def set_variant(fast: bool) -> None:
-    """Choose the arithmetic variant of the drop-in kernels: False (default) is
-    the reference's operation order; True uses FMA contraction and shared
-    reciprocals (rounding-level differences)."""
-    _lib.check(_lib.load().wenox_set_variant(1 if fast else 0))
+def set_variant(variant) -> None:
+    """Choose the semantics of the drop-in kernels.  "exact" (or False, the
+    default) is the Rust crate's operation order; "fast" (or True) uses FMA
+    contraction and shared reciprocals (rounding-level differences); "twin"
+    follows the pure-NumPy WENO_HLLC_2d.py where it differs from the crate:
+    the exact sqrt(3)/12 in the Gauss-point pass and its own HLLC and HLLE
+    (WENO_HLLC_2d.py:5,435-669).  The twin's weno_x/weno_y and L reconstruct
+    component-wise: call with cons_switch=True to reproduce them."""
+    if isinstance(variant, str):
+        code = {"exact": 0, "fast": 1, "twin": 2}[variant]
+    else:
+        code = 1 if variant else 0
+    _lib.check(_lib.load().wenox_set_variant(code))

@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import EXACT, FAST, PanicException, WenoxError  # noqa: F401
+from ._lib import EXACT, FAST, TWIN, PanicException, WenoxError  # noqa: F401
 
 BC_MIRROR, BC_PERIODIC, BC_DMR_REF, BC_DMR_STD, BC_COPY = 0, 1, 2, 3, 4
 ENGINE_UNFUSED, ENGINE_FUSED = 0, 1
@@ -21,6 +21,13 @@ PREC_FP64, PREC_FP32SPLIT = 0, 1
 
 _BC_NAMES = {"mirror": BC_MIRROR, "transmissive": BC_MIRROR, "zero_gradient": BC_MIRROR,
              "periodic": BC_PERIODIC, "dmr_ref": BC_DMR_REF, "dmr": BC_DMR_STD, "copy": BC_COPY}
+
+
+# "exact": the Rust crate's operation order (bitwise the oracle); "fast":
+# FMA contraction and shared reciprocals (rounding-level differences);
+# "twin": the pure-NumPy WENO_HLLC_2d.py's semantics (exact sqrt(3)/12, its
+# HLLC and HLLE), unfused FP64 engine, component-wise reconstruction.
+_VARIANTS = {"exact": EXACT, "fast": FAST, "twin": TWIN}
 
 
 def device_count() -> int:
@@ -47,7 +54,7 @@ class Solver:
         d.dmr_bound = int(dmr_bound)
         d.dmr_x0 = float(dmr_x0)
         d.engine = {"unfused": ENGINE_UNFUSED, "fused": ENGINE_FUSED}[engine] if isinstance(engine, str) else int(engine)
-        d.variant = {"exact": EXACT, "fast": FAST}[variant] if isinstance(variant, str) else int(variant)
+        d.variant = _VARIANTS[variant] if isinstance(variant, str) else int(variant)
         d.precision = {"fp64": PREC_FP64, "fp32split": PREC_FP32SPLIT}[precision] if isinstance(precision, str) else int(precision)
         d.use_graph = int(bool(use_graph))
         self.desc = d
@@ -160,6 +167,12 @@ class Solver:
         set before upload() for the first step to follow it too."""
         _lib.check(_lib.load().wenox_solver_set_cfl_rule(self._h, float(rho_floor), float(cmax_eps)))
 
+    def set_pfloor(self, pmin: float = 1e-8):
+        """After every step raise the total energy of each cell (ghosts
+        included) whose pressure is below ``pmin`` so that p = pmin, as
+        WENO_HLLC_2d.con_var2primi_pfloor does (:813-837); 0 switches it off."""
+        _lib.check(_lib.load().wenox_solver_set_pfloor(self._h, float(pmin)))
+
     # (wavefronts per workgroup, rows per wavefront).  Dense in the row count:
     # the sweep time steps with the number of workgroup rounds on the 256 CUs
     # (two resident waves per SIMD), so one row more or less per wave can
@@ -231,7 +244,7 @@ class Batch:
         d.jp_lo, d.jp_hi = float(jp_cri[0]), float(jp_cri[1])
         d.bc = _BC_NAMES[bc] if isinstance(bc, str) else int(bc)
         d.engine = ENGINE_FUSED
-        d.variant = {"exact": EXACT, "fast": FAST}[variant] if isinstance(variant, str) else int(variant)
+        d.variant = _VARIANTS[variant] if isinstance(variant, str) else int(variant)
         d.precision = {"fp64": PREC_FP64, "fp32split": PREC_FP32SPLIT}[precision] if isinstance(precision, str) else int(precision)
         d.use_graph = int(bool(use_graph))
         self.sizes = [(int(a), int(b)) for a, b in sizes]

@@ -10,7 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libwenox.so")
 
 OK, E_ARG, E_SHAPE, E_DEVICE = 0, -1, -2, 1
-EXACT, FAST = 0, 1
+EXACT, FAST, TWIN = 0, 1, 2
 
 
 class WenoxError(RuntimeError):
@@ -92,6 +92,7 @@ def load():
         "wenox_solver_combine_cols": ([dp, ctypes.POINTER(i)], i),
         "wenox_solver_set_combine_cols": ([dp, i], i),
         "wenox_solver_set_cfl_rule": ([dp, d, d], i),
+        "wenox_solver_set_pfloor": ([dp, d], i),
         "wenox_batch_create": ([ctypes.POINTER(SolverDesc), i, dp, dp, dp, i,
                                 ctypes.POINTER(ctypes.c_void_p)], i),
         "wenox_batch_destroy": ([dp], None),

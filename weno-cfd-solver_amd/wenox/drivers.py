@@ -1,7 +1,9 @@
 """Device-resident counterparts of the reference drivers.
 
-``main_rs`` follows ``Rie_prob_2d_rs.main_rs`` (:269-376) and
-``main_rs_dmr`` follows ``double_mach_reflection.main_rs`` (:177-303): the
+``main_rs`` follows ``Rie_prob_2d_rs.main_rs`` (:269-376),
+``main_rs_dmr`` follows ``double_mach_reflection.main_rs`` (:177-303),
+``main_1d`` follows ``Rie_prob_1D_rs.main`` (:110-181) and ``main_twin``
+follows the pure-NumPy ``WENO_HLLC_2d.main`` (:840-908), each with the
 same dictionaries, loop control (t_final primary, nstep as a cap; CFL dt
 clipped to the remaining time) and snapshot format
 (``data/<YYYY-mm-dd_HH-MM-SS>/{step}.npy`` of the (nx+6, ny+6, 8) array and
@@ -157,3 +159,23 @@ def main_rs_dmr(grid_dict: dict, control_dict: dict, phys_dict: dict, engine="fu
     if control_dict.get("mode", "") == "opt":
         return q
     return None
+
+
+def main_twin(Ngrid: int, N_STEP: int, fileStorage: bool = True, force_HLLE: bool = False,
+              Jp_cri=(1.0, 4.0), gamma: float = 1.4):
+    """WENO_HLLC_2d.main (:840-908) on the device, in the twin's semantics
+    (exact sqrt(3)/12, its HLLC and HLLE; component-wise reconstruction):
+    the quadrant initial data of problems.twin_quadrants, dx = 1/Ngrid,
+    CFL 0.2 with dt = CFL dx / max(sqrt(g p / max(rho, 1e-10)) + |V|),
+    RK3_correct with the constant-extrapolation ghost fill, the pressure floor
+    1e-8 after every step, and N_STEP - 1 steps.  Snapshots (fileStorage) as
+    the twin writes them; its closing interactive plot is left to the caller.
+    Returns the final (Ngrid+6, Ngrid+6, 8) state."""
+    n = int(Ngrid)
+    q0 = problems.twin_quadrants(n, gamma)
+    ctrl = {"nstep": max(int(N_STEP) - 1, 0), "file_storage": bool(fileStorage)}
+    with Solver(n, n, 1 / n, gamma=gamma, cfl=0.2, cons_switch=True, force_hlle=bool(force_HLLE),
+                jp_cri=tuple(Jp_cri), bc="copy", engine="unfused", variant="twin") as s:
+        s.set_cfl_rule(1e-10, 0.0)
+        s.set_pfloor(1e-8)
+        return _run(s, q0, ctrl, n, n, _snapshot_dir(ctrl))

@@ -43,6 +43,33 @@ def _mirror_fill(q):
     return q
 
 
+def _copy_fill(q):
+    """Constant extrapolation: each ghost row / column copies the nearest
+    interior one, x then y (WENO_HLLC_2d.py:759-774, Rie_prob_1D_rs.py:59-76)."""
+    nx, ny = q.shape[0] - 6, q.shape[1] - 6
+    q[:3] = q[3]
+    q[nx + 3:] = q[nx + 2]
+    q[:, :3] = q[:, 3:4]
+    q[:, ny + 3:] = q[:, ny + 2:ny + 3]
+    return q
+
+
+def twin_quadrants(n: int, gamma: float = 1.4):
+    """Initial data of the NumPy twin's driver (WENO_HLLC_2d.main, :846-880):
+    four constant states split at padded index int((n+6)/2), ghost cells
+    included -- (rho, u, v, p) = (0.125, 1, 1, 0.125) below/left of it in both
+    indices, (0.5, 0, 1, 0.25) past it in x only, (0.5, 1, 0, 0.25) past it in
+    y only, (1, 0, 0, 1) past it in both -- then the constant-extrapolation
+    ghost fill.  The grid is n x n with dx = 1/n."""
+    q = np.zeros((n + 6, n + 6, 8))
+    h = int((n + 6) / 2)
+    q[:h, :h, 4:] = (0.125, 1, 1, 0.125)
+    q[h:, :h, 4:] = (0.5, 0, 1, 0.25)
+    q[:h, h:, 4:] = (0.5, 1, 0, 0.25)
+    q[h:, h:, 4:] = (1, 0, 0, 1)
+    return _copy_fill(_from_prims(q, gamma))
+
+
 def four_quadrant(n: int, gamma: float = 1.4, ny: int | None = None):
     """Four-quadrant Riemann problem on [0,1]^2 (Lax-Liu configuration 3):
     (rho,u,v,p) = (1.5,0,0,1.5) for x>0.5,y>0.5; (0.5323,1.206,0,0.3) for x<0.5,y>0.5; (0.138,1.206,1.206,0.029) for
