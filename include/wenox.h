@@ -202,6 +202,35 @@ int wenox_batch_run(wenox_batch *b, int64_t max_steps, double t_final, int64_t *
 int wenox_batch_status(wenox_batch *b, double *T, int64_t *steps);
 double wenox_batch_last_ms_per_step(wenox_batch *b);
 
+/* ---- 4. jp_cri tuning objective --------------------------------------------- */
+/* A read-only 2-D host array view: extents and element strides. */
+typedef struct wenox_view2 {
+    const double *data;
+    int64_t dim[2];
+    int64_t stride[2];
+} wenox_view2;
+
+/* metrics.py:1-77 compute_2d_objective(rho, u, v, p, dx, dy, weights), the
+ * objective opt_jp_cro.py:41-47 minimises, on the GPU.  rho, u, v: (nx, ny)
+ * views of one shape (p is not read, as in the reference).  weights: {I, E,
+ * O, TV} (NULL: {1, 1, 50, 1}).  *J receives the objective; metrics (8
+ * values, may be NULL) receives I_curve, E_perp, O, TV, cI, cE, cO, cTV.
+ * The shock mask is thinned with the Zhang-Suen algorithm, which
+ * skimage.morphology.skeletonize implements for 2-D images. */
+int wenox_objective_2d(const wenox_view2 *rho, const wenox_view2 *u, const wenox_view2 *v, double dx,
+                       double dy, const double *weights, double *J, double *metrics);
+/* The same objective of a solver's current state (FP64 states): rho the
+ * conserved density, u and v the con2primi velocities of the interior, as
+ * opt_jp_cro.py:31-35 takes them from main_rs's final array, with no copy of
+ * the state to the host. */
+int wenox_solver_objective(wenox_solver *s, double dx, double dy, const double *weights, double *J,
+                           double *metrics);
+/* Intermediates of the objective of rho (metrics.py:13,16,22,49): shock mask
+ * S, skeleton C, band (nx*ny bytes each, 0 or 1) and the 7 x 7 median of rho
+ * (nx*ny doubles); any output may be NULL.  For inspection and tests. */
+int wenox_objective_fields(const wenox_view2 *rho, double dx, double dy, uint8_t *S, uint8_t *C,
+                           uint8_t *band, double *median);
+
 #ifdef __cplusplus
 }
 #endif

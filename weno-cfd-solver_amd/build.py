@@ -3,7 +3,8 @@
 The kernel translation units are compiled once per arithmetic variant
 ("exact": reference operation order, no FMA contraction; "fast": FMA
 contraction and shared reciprocals; the layer-by-layer kernels also "twin":
-the NumPy twin's semantics) and linked with the C-ABI host code into
+the NumPy twin's semantics), the tuning objective once, and linked with the
+C-ABI host code into
 wenox/libwenox.so next to the Python package that loads it.
 """
 import os
@@ -31,6 +32,7 @@ UNITS = [
     ("wenox_kernels.hip", "kernels_twin.o", ["-DWX_FAST=0", "-DWX_TWIN=1", "-ffp-contract=off"]),
     ("wenox_sweep.hip", "sweep_exact.o", ["-DWX_FAST=0", "-ffp-contract=off"]),
     ("wenox_sweep.hip", "sweep_fast.o", ["-DWX_FAST=1", "-ffp-contract=fast"]),
+    ("wenox_objective.hip", "objective.o", ["-ffp-contract=off"]),
     ("wenox_abi.cpp", "abi.o", ["-x", "hip", "-ffp-contract=off"]),
     ("wenox_batch.cpp", "batch.o", ["-x", "hip", "-ffp-contract=off"]),
 ]

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/wenox.h"
+#include "wenox_objective.h"
 #include "wenox_ops.h"
 #include "wenox_sweep.h"
 
@@ -1006,4 +1007,81 @@ extern "C" int wenox_solver_set_chunks(wenox_solver* s, int chunk_x, int chunk_y
 {
     if (!s) return fail(WENOX_E_ARG, "wenox_solver_set_chunks: null solver");
     return wenox_solver_set_tiling(s, chunk_x, chunk_y, s->plan.waves_x, s->plan.waves_y);
+}
+
+// ---------------------------------------------------------------------------
+// extern "C": jp_cri tuning objective (metrics.py:1-77)
+// ---------------------------------------------------------------------------
+namespace {
+
+const double kDefaultWeights[4] = {1.0, 1.0, 50.0, 1.0};   // metrics.py:69-70
+
+int check_view2(const char* fn, const wenox_view2* v, int64_t nx, int64_t ny)
+{
+    if (!v || !v->data) return fail(WENOX_E_ARG, std::string(fn) + ": null array");
+    if (v->dim[0] != nx || v->dim[1] != ny) return fail(WENOX_E_ARG, std::string(fn) + ": arrays differ in shape");
+    return WENOX_OK;
+}
+
+void pack2(const wenox_view2* v, std::vector<double>& out)
+{
+    const int64_t nx = v->dim[0], ny = v->dim[1];
+    out.resize((size_t)(nx * ny));
+    for (int64_t i = 0; i < nx; ++i)
+        for (int64_t j = 0; j < ny; ++j) out[(size_t)(i * ny + j)] = v->data[i * v->stride[0] + j * v->stride[1]];
+}
+
+int objective_host(const char* fn, const wenox_view2* rho, const wenox_view2* u, const wenox_view2* v, double dx,
+                   double dy, const double* weights, double* J, double* metrics, uint8_t* S, uint8_t* C,
+                   uint8_t* band, double* median)
+{
+    if (!rho || !J) return fail(WENOX_E_ARG, std::string(fn) + ": null argument");
+    const int64_t nx = rho->dim[0], ny = rho->dim[1];
+    if (nx < 1 || ny < 1) return fail(WENOX_E_SHAPE, std::string(fn) + ": empty array");
+    int rc;
+    if ((rc = check_view2(fn, rho, nx, ny)) || (rc = check_view2(fn, u, nx, ny)) || (rc = check_view2(fn, v, nx, ny)))
+        return rc;
+    if ((rc = require_device())) return rc;
+    std::vector<double> r, uu, vv;
+    pack2(rho, r);
+    pack2(u, uu);
+    pack2(v, vv);
+    ObjSource src;
+    src.rho = r.data();
+    src.u = uu.data();
+    src.v = vv.data();
+    std::lock_guard<std::mutex> lk(g_drop.mu);
+    if (!g_drop.stream) HIPCHK(hipStreamCreateWithFlags(&g_drop.stream, hipStreamNonBlocking));
+    return objective_eval(src, nx, ny, dx, dy, weights ? weights : kDefaultWeights, g_drop.stream, J, metrics, S, C,
+                          band, median);
+}
+
+}  // namespace
+
+extern "C" int wenox_objective_2d(const wenox_view2* rho, const wenox_view2* u, const wenox_view2* v, double dx,
+                                  double dy, const double* weights, double* J, double* metrics)
+{
+    return objective_host("wenox_objective_2d", rho, u, v, dx, dy, weights, J, metrics, nullptr, nullptr, nullptr,
+                          nullptr);
+}
+
+extern "C" int wenox_objective_fields(const wenox_view2* rho, double dx, double dy, uint8_t* S, uint8_t* C,
+                                      uint8_t* band, double* median)
+{
+    double J = 0.0;
+    return objective_host("wenox_objective_fields", rho, rho, rho, dx, dy, nullptr, &J, nullptr, S, C, band, median);
+}
+
+extern "C" int wenox_solver_objective(wenox_solver* s, double dx, double dy, const double* weights, double* J,
+                                      double* metrics)
+{
+    if (!s || !J) return fail(WENOX_E_ARG, "wenox_solver_objective: null argument");
+    if (s->split) return fail(WENOX_E_ARG, "wenox_solver_objective: FP64 states only");
+    HIPCHK(hipSetDevice(s->device));
+    ObjSource src;
+    src.U = s->U;
+    src.ld = s->ld;
+    src.plane = plane(s);
+    return objective_eval(src, s->d.nx, s->d.ny, dx, dy, weights ? weights : kDefaultWeights, s->stream, J, metrics,
+                          nullptr, nullptr, nullptr, nullptr);
 }

@@ -29,6 +29,12 @@ class View3(ctypes.Structure):
                 ("stride", ctypes.c_int64 * 3)]
 
 
+class View2(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p),
+                ("dim", ctypes.c_int64 * 2),
+                ("stride", ctypes.c_int64 * 2)]
+
+
 class SolverDesc(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int64), ("ny", ctypes.c_int64),
                 ("dx", ctypes.c_double), ("gamma", ctypes.c_double), ("cfl", ctypes.c_double),
@@ -93,6 +99,9 @@ def load():
         "wenox_solver_set_combine_cols": ([dp, i], i),
         "wenox_solver_set_cfl_rule": ([dp, d, d], i),
         "wenox_solver_set_pfloor": ([dp, d], i),
+        "wenox_objective_2d": ([ctypes.POINTER(View2)] * 3 + [d, d, dp, dp, dp], i),
+        "wenox_solver_objective": ([dp, d, d, dp, dp, dp], i),
+        "wenox_objective_fields": ([ctypes.POINTER(View2), d, d, dp, dp, dp, dp], i),
         "wenox_batch_create": ([ctypes.POINTER(SolverDesc), i, dp, dp, dp, i,
                                 ctypes.POINTER(ctypes.c_void_p)], i),
         "wenox_batch_destroy": ([dp], None),
@@ -123,6 +132,19 @@ def check(rc: int) -> None:
     if rc == E_ARG:
         raise ValueError(msg)
     raise WenoxError(msg)
+
+
+def view2(a: np.ndarray) -> View2:
+    """Describe a 2-D float64 array (non-multiple-of-8 strides get a
+    contiguous copy, kept alive through the View2's _keep)."""
+    a = np.asarray(a, dtype=np.float64)
+    if a.ndim != 2:
+        raise TypeError("expected a 2-D array")
+    if any(s % 8 for s in a.strides):
+        a = np.ascontiguousarray(a)
+    v = View2(a.ctypes.data, (ctypes.c_int64 * 2)(*a.shape), (ctypes.c_int64 * 2)(*[s // 8 for s in a.strides]))
+    v._keep = a
+    return v
 
 
 def view(a: np.ndarray) -> View3:
