@@ -219,6 +219,39 @@ def dropin_rate(q, dx, w, calls=5):
     return out
 
 
+def objective_rate(w, q, dx, steps, reps=5):
+    """The jp_cri tuning objective (metrics.compute_2d_objective) of the state
+    after `steps` steps: on the GPU from the solver's resident state (wall
+    time per evaluation, allocation and the thinning loop's host checks
+    included), and the oracle (numpy + scipy.ndimage, one host thread) on the
+    downloaded fields."""
+    import numpy as np
+    from wenox import Solver, tuning
+    with Solver(w["nx"], w["ny"], dx, gamma=1.4, cfl=w["cfl"], cons_switch=w["cons"],
+                force_hlle=w["force_hlle"], jp_cri=w["jp"], bc=w["bc"], engine="fused", variant="fast") as s:
+        s.upload(q)
+        s.run(nstep=steps)
+        J, _ = tuning.solver_objective(s)
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            tuning.solver_objective(s)
+            times.append(time.perf_counter() - t0)
+        st = s.download()
+    out = {"workload": f"{describe(w)} state after {steps} steps", "J": J,
+           "gpu_ms": float(np.median(times)) * 1e3}
+    try:
+        from oracle import objective as ob
+    except ImportError:
+        return out
+    rho, u, v = st[3:-3, 3:-3, 0], st[3:-3, 3:-3, 5], st[3:-3, 3:-3, 6]
+    t0 = time.perf_counter()
+    Jc, _ = ob.compute_2d_objective(rho, u, v, None)
+    out.update(cpu_ms=(time.perf_counter() - t0) * 1e3, cpu_kind="port", cpu_cores=1,
+               rel_diff_J=abs(J - Jc) / abs(Jc) if Jc else abs(J - Jc))
+    return out
+
+
 def kernel_bytes(w, split):
     """Algorithmic HBM bytes of each launch of a step (state and D arrays
     touched once each; halo re-reads counted 0)."""
@@ -354,6 +387,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 512 MiB device copy (counter calibration) and exit")
+    ap.add_argument("--objective", action="store_true",
+                    help="measure only the jp_cri tuning objective of the workload's state after "
+                         "warmup + steps steps and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -364,6 +400,13 @@ def main():
         return
 
     w = WORKLOADS[args.workload]
+    if args.objective:
+        q, dx = make_state(w)
+        obj = objective_rate(w, q, dx, args.warmup + args.steps)
+        if args.no_cpu:
+            obj = {k: v for k, v in obj.items() if not k.startswith(("cpu", "rel_diff"))}
+        print(json.dumps({"tuning_objective": obj}), flush=True)
+        return
     if args.workload == "batch512":
         r = run_batch(w, args.steps, args.warmup, args.precision)
         res = {"metric": "cell-updates/s FP64 vs FP32-split @<=1e-6 L1 err; % of HBM roofline",
@@ -445,6 +488,7 @@ def main():
             res["quad4096"]["scheme_variants_fp64"] = variants
     if not args.no_extra and args.workload == "dmr":
         res["dropin_l_local"] = dropin_rate(q, dx, w)
+        res["tuning_objective"] = objective_rate(w, q, dx, args.warmup + args.steps)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(q, dx, w)
     print(json.dumps(res), flush=True)

@@ -266,19 +266,19 @@ __device__ double median7x7(const double* R, Grid g, int64_t i, int64_t j)
     return b[24];
 }
 
-// :37-55, :63  over the band: max |rho - median|, sum vperp^2, count; keys
-// |v| (band) or +inf for the median of |v| over the band.  all_median:
-// write the median of every cell to med.
+// :37-55, :63  over the band: max |rho - median|, sum vperp^2, count, and
+// |v| appended to keys (slots from *nkeys, in no particular order: only its
+// median is used).  med != nullptr: write the median of every cell to med.
 __global__ void __launch_bounds__(OT) k_band(const double* R, const double* Uv, const double* Vv,
                                              const uint8_t* band, Grid g, double dx2, double dy2, double* part,
-                                             double* keys, double* med)
+                                             double* keys, unsigned int* nkeys, double* med)
 {
     __shared__ double sh[OT];
     double dmax = 0.0, e = 0.0, cnt = 0.0;
     for (int64_t t = (int64_t)blockIdx.x * OT + threadIdx.x; t < g.nx * g.ny; t += (int64_t)gridDim.x * OT) {
         const int64_t i = t / g.ny, j = t - i * g.ny;
         const bool in = band[t] != 0;
-        keys[t] = in ? fabs(Vv[t]) : INFINITY;
+        if (in) keys[atomicAdd(nkeys, 1u)] = fabs(Vv[t]);
         if (!in && !med) continue;
         const double m = median7x7(R, g, i, j);
         if (med) med[t] = m;
@@ -345,6 +345,7 @@ int objective_eval(const ObjSource& src, int64_t nx, int64_t ny, double dx, doub
     uint8_t *S, *C, *C2, *band;
     long long* lpart;
     int* removed;
+    unsigned int* nkeys;
     OCHK(buf.get(&R, N));
     OCHK(buf.get(&Uv, N));
     OCHK(buf.get(&Vv, N));
@@ -357,6 +358,7 @@ int objective_eval(const ObjSource& src, int64_t nx, int64_t ny, double dx, doub
     OCHK(buf.get(&C2, N));
     OCHK(buf.get(&band, N));
     OCHK(buf.get(&removed, 1));
+    OCHK(buf.get(&nkeys, 1));
     if (median_out) OCHK(buf.get(&med, N));
 
     if (src.U) {
@@ -429,12 +431,8 @@ int objective_eval(const ObjSource& src, int64_t nx, int64_t ny, double dx, doub
     }
 
     // band terms and the median of |v| over the band
-    k_band<<<nb, OT, 0, s>>>(R, Uv, Vv, band, g, dx2, dy2, dpart, keys, med);
-    size_t tmp_bytes = 0;
-    OCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, keys, sorted, (int)N, 0, 64, s));
-    void* tmp = nullptr;
-    OCHK(buf.get((uint8_t**)&tmp, tmp_bytes));
-    OCHK(hipcub::DeviceRadixSort::SortKeys(tmp, tmp_bytes, keys, sorted, (int)N, 0, 64, s));
+    OCHK(hipMemsetAsync(nkeys, 0, sizeof(unsigned int), s));
+    k_band<<<nb, OT, 0, s>>>(R, Uv, Vv, band, g, dx2, dy2, dpart, keys, nkeys, med);
     OCHK(hipMemcpyAsync(hp.data(), dpart, (size_t)nb * 3 * sizeof(double), hipMemcpyDeviceToHost, s));
     OCHK(hipStreamSynchronize(s));
     double dmax = hp[0], E_perp = 0.0, cnt = 0.0;
@@ -446,8 +444,13 @@ int objective_eval(const ObjSource& src, int64_t nx, int64_t ny, double dx, doub
     const int64_t K = (int64_t)cnt;
     double med_v = 0.0;
     if (K > 0) {
+        size_t tmp_bytes = 0;
+        OCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, keys, sorted, (int)K, 0, 64, s));
+        void* tmp = nullptr;
+        OCHK(buf.get((uint8_t**)&tmp, tmp_bytes));
+        OCHK(hipcub::DeviceRadixSort::SortKeys(tmp, tmp_bytes, keys, sorted, (int)K, 0, 64, s));
         double two[2] = {0.0, 0.0};
-        const int64_t lo = (K - 1) / 2, take = std::min<int64_t>(2, N - lo);
+        const int64_t lo = (K - 1) / 2, take = std::min<int64_t>(2, K - lo);
         OCHK(hipMemcpyAsync(two, sorted + lo, (size_t)take * sizeof(double), hipMemcpyDeviceToHost, s));
         OCHK(hipStreamSynchronize(s));
         med_v = (K % 2) ? two[0] : (two[0] + two[1]) / 2;
