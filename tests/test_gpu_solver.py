@@ -201,3 +201,48 @@ def test_combine_tile_widths_bitwise(oracle_mod):
             s.upload(q0)
             s.run(nstep=5)
             np.testing.assert_array_equal(s.download()[..., :4], ref[..., :4], err_msg=f"combine cols {cols}")
+
+
+def test_fused_step_control_across_run_calls(oracle_mod):
+    """The fused engine forms the next step's CFL maxima in the last RK
+    combine and applies a step's time advance at the next step's start (or
+    when run() returns).  Whatever the split of the steps over run() calls
+    -- one call, one step per call (the snapshot drivers), a t_final run,
+    multi-step graph replays -- T, the step count and the state equal the
+    oracle's bit for bit (exact variant); so does a CFL rule changed after
+    upload."""
+    from wenox import Solver, problems
+    n, steps = 40, 19
+    q0, dx = problems.four_quadrant(n)
+    ref, ts = oracle_mod.run_riemann(q0.copy(), n, n, dx, 1.4, 0.4, False, False, (0.5, 2.0), nstep=steps)
+    print("[solver] fused step control: one call, per-step calls, t_final, cfl rule", flush=True)
+    for split in ([steps], [1] * steps, [8, 3, 8], [17, 2]):
+        with Solver(n, n, dx, cfl=0.4, jp_cri=(0.5, 2.0), engine="fused") as s:
+            s.upload(q0)
+            total = 0
+            for k in split:
+                T, got = s.run(nstep=k)
+                total += got
+                assert T == ts[total], (split, total)
+            assert total == steps
+            np.testing.assert_array_equal(s.download()[..., :4], ref[..., :4], err_msg=str(split))
+    t_end = ts[12] + 0.5 * (ts[13] - ts[12])
+    ref_t, ts_t = oracle_mod.run_riemann(q0.copy(), n, n, dx, 1.4, 0.4, False, False, (0.5, 2.0), t_final=t_end)
+    with Solver(n, n, dx, cfl=0.4, jp_cri=(0.5, 2.0), engine="fused") as s:
+        s.upload(q0)
+        T, k = s.run(t_final=t_end)
+        assert k == len(ts_t) - 1 and T == ts_t[-1]
+        np.testing.assert_array_equal(s.download()[..., :4], ref_t[..., :4])
+        # stepping on after the end changes nothing
+        T2, k2 = s.run(nstep=3, t_final=t_end)
+        assert k2 == 0 and T2 == T
+    # a CFL rule set after upload (the 1-D drivers' floor and no epsilon)
+    with Solver(n, n, dx, cfl=0.4, jp_cri=(0.5, 2.0), engine="fused") as s:
+        s.upload(q0[..., :4].copy())
+        s.set_cfl_rule(0.3, 0.0)
+        T, k = s.run(nstep=4)
+    with Solver(n, n, dx, cfl=0.4, jp_cri=(0.5, 2.0), engine="unfused") as s:
+        s.upload(q0[..., :4].copy())
+        s.set_cfl_rule(0.3, 0.0)
+        T_ref, _ = s.run(nstep=4)
+    assert T == T_ref

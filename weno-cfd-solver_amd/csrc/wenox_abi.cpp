@@ -438,8 +438,10 @@ struct wenox_solver {
     StepCtl* ctl = nullptr;  // device
     double* cfl_part = nullptr;   // device, CFL_SLOTS per-block maxima
     BcParams bp{};
-    hipGraph_t graph = nullptr;
+    hipGraph_t graph = nullptr;      // one step
     hipGraphExec_t exec = nullptr;
+    hipGraph_t graph_n = nullptr;    // GRAPH_STEPS steps (fewer graph launches on long runs)
+    hipGraphExec_t exec_n = nullptr;
     double last_ms = 0.0;
     SweepPlan plan{};
     const SweepOps* sops = nullptr;
@@ -455,6 +457,28 @@ namespace {
 // plane strides of the pitched state and of its transposed copy
 int64_t plane(const wenox_solver* s) { return (s->d.nx + 6) * s->ld; }
 int64_t plane_t(const wenox_solver* s) { return (s->d.ny + 6) * s->ldT; }
+
+// Entries of the CFL maxima array: CFL_SLOTS row blocks of k_cfl, or one per
+// 32 x 32 tile of the fused engine's last RK combine.
+size_t cfl_slots(const wenox_solver_desc& d)
+{
+    const size_t tiles = (size_t)((d.nx + 31) / 32) * (size_t)((d.ny + 31) / 32);
+    return std::max<size_t>(CFL_SLOTS, tiles);
+}
+
+// Steps per replay of the multi-step graph: consecutive graph launches leave
+// a gap of several microseconds between them, consecutive kernels of one
+// graph do not.
+constexpr int64_t GRAPH_STEPS = 8;
+
+// Forget the captured step graphs (after a change of what a step launches).
+void drop_graphs(wenox_solver* s)
+{
+    if (s->exec) { (void)hipGraphExecDestroy(s->exec); s->exec = nullptr; }
+    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    if (s->exec_n) { (void)hipGraphExecDestroy(s->exec_n); s->exec_n = nullptr; }
+    if (s->graph_n) { (void)hipGraphDestroy(s->graph_n); s->graph_n = nullptr; }
+}
 // FP64 view of work with the state geometry (the split download's staging copy)
 double* work_state(const wenox_solver* s) { return s->work + STATE_SHIFT; }
 
@@ -479,7 +503,11 @@ void enqueue_step(wenox_solver* s)
         o->advance(s->ctl, st);
         return;
     }
-    o->cfl(s->U, d.nx, d.ny, s->ld, d.gamma, s->ctl, st);
+    const bool fused = d.engine == WENOX_ENGINE_FUSED;
+    // fused: the previous step's last combine (or the upload) left the CFL
+    // maxima; unfused: reduce the state here
+    if (fused) o->step_dt(s->ctl, st);
+    else o->cfl(s->U, d.nx, d.ny, s->ld, d.gamma, s->ctl, st);
     double* src[3] = {s->U, s->A, s->B};
     double* dst[3] = {s->A, s->B, s->U};
     double* srcT[3] = {s->UT, s->AT, s->BT};
@@ -501,7 +529,9 @@ void enqueue_step(wenox_solver* s)
         o->pfloor(s->U, d.nx + 6, d.ny + 6, s->ld, d.gamma, s->pfloor, s->ctl, 1, st);
         if (s->UT) o->pfloor(s->UT, d.ny + 6, d.nx + 6, s->ldT, d.gamma, s->pfloor, s->ctl, 0, st);
     }
-    o->advance(s->ctl, st);
+    // fused: the last combine marked the advance pending (the next step_dt or
+    // a settle applies it)
+    if (!fused) o->advance(s->ctl, st);
 }
 
 int read_ctl(wenox_solver* s, StepCtl* h)
@@ -623,7 +653,7 @@ extern "C" int wenox_solver_create(const wenox_solver_desc* desc, wenox_solver**
               hipMalloc(&s->L, std::max<size_t>(nL, 4) * 8) == hipSuccess &&
               hipMalloc(&s->work, std::max<size_t>(s->work_n, 4) * 8) == hipSuccess &&
               hipMalloc(&s->ctl, sizeof(StepCtl)) == hipSuccess &&
-              hipMalloc(&s->cfl_part, CFL_SLOTS * sizeof(double)) == hipSuccess;
+              hipMalloc(&s->cfl_part, cfl_slots(d) * sizeof(double)) == hipSuccess;
     if (ok && fused)
         ok = state_buf(3, &s->UT, SPT) && state_buf(4, &s->AT, SPT) && state_buf(5, &s->BT, SPT) &&
              hipMalloc(&s->DyT, std::max<size_t>(nDyT, 4) * 8) == hipSuccess;
@@ -653,8 +683,7 @@ extern "C" int wenox_solver_create(const wenox_solver_desc* desc, wenox_solver**
 extern "C" void wenox_solver_destroy(wenox_solver* s)
 {
     if (!s) return;
-    if (s->exec) (void)hipGraphExecDestroy(s->exec);
-    if (s->graph) (void)hipGraphDestroy(s->graph);
+    drop_graphs(s);
     for (void* p : s->mem)
         if (p) (void)hipFree(p);
     for (double* p : {s->L, s->DyT, s->work})
@@ -722,6 +751,8 @@ extern "C" int wenox_solver_upload(wenox_solver* s, const wenox_view3* q, double
     } else {
         if (s->UT) s->sops->transpose(s->U, s->UT, m, n, ld, s->ldT, s->stream);
         s->ops->bc(s->U, s->UT, d.nx, d.ny, ld, s->ldT, s->bp, s->ctl, 0, s->stream);
+        // the fused engine's first step reads the CFL maxima of this state
+        if (d.engine == WENOX_ENGINE_FUSED) s->ops->cfl_part(s->U, d.nx, d.ny, ld, d.gamma, s->ctl, s->stream);
     }
     HIPCHK(hipStreamSynchronize(s->stream));
     HIPCHK(hipGetLastError());
@@ -811,7 +842,15 @@ extern "C" int wenox_solver_run(wenox_solver* s, int64_t max_steps, double t_fin
         HIPCHK(hipStreamEndCapture(s->stream, &s->graph));
         HIPCHK(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
     }
+    if (s->d.use_graph && !s->exec_n && max_steps >= GRAPH_STEPS) {
+        HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        for (int64_t i = 0; i < GRAPH_STEPS; ++i) enqueue_step(s);
+        HIPCHK(hipStreamEndCapture(s->stream, &s->graph_n));
+        HIPCHK(hipGraphInstantiate(&s->exec_n, s->graph_n, nullptr, nullptr, 0));
+    }
     auto launch = [&](int64_t k) -> int {
+        if (s->exec_n)
+            for (; k >= GRAPH_STEPS; k -= GRAPH_STEPS) HIPCHK(hipGraphLaunch(s->exec_n, s->stream));
         for (int64_t i = 0; i < k; ++i) {
             if (s->exec) {
                 HIPCHK(hipGraphLaunch(s->exec, s->stream));
@@ -829,6 +868,7 @@ extern "C" int wenox_solver_run(wenox_solver* s, int64_t max_steps, double t_fin
     } else {
         // Time-controlled: launch chunks sized by the remaining time / dt.
         while (launched < max_steps) {
+            s->ops->settle(s->ctl, s->stream);
             if ((rc = read_ctl(s, &h))) return rc;
             if (h.step >= h.max_steps) break;
             if (launched > 0 && !h.active) break;
@@ -844,6 +884,7 @@ extern "C" int wenox_solver_run(wenox_solver* s, int64_t max_steps, double t_fin
         }
     }
     HIPCHK(hipEventRecord(s->ev1, s->stream));
+    s->ops->settle(s->ctl, s->stream);   // the last step's pending advance
     HIPCHK(hipStreamSynchronize(s->stream));
     HIPCHK(hipGetLastError());
     float ms = 0.f;
@@ -886,9 +927,12 @@ extern "C" int wenox_solver_profile(wenox_solver* s, int reps, double* out_ms, i
     };
     // scratch targets: the sweeps read the stage-1 input (U, U^T) and write Dx / Dy^T;
     // the combine writes into B / B^T, which the next step overwrites anyway.
+    // slot 0: the step's CFL launches -- FP32-split: the reduction over the
+    // state; FP64: the one-wave dt launch (the maxima come from the previous
+    // step's last combine)
     if ((rc = timed(0, [&] {
              if (split) s->sops->cfl_split((const float*)s->U, d.nx, d.ny, s->ld, d.gamma, s->ctl, st);
-             else s->ops->cfl(s->U, d.nx, d.ny, s->ld, d.gamma, s->ctl, st);
+             else s->ops->step_dt(s->ctl, st);
          })))
         return rc;
     if ((rc = write_ctl(s, &act))) return rc;
@@ -928,7 +972,14 @@ extern "C" int wenox_solver_set_cfl_rule(wenox_solver* s, double rho_floor, doub
     if (rc) return rc;
     h.rho_floor = rho_floor;
     h.cmax_eps = cmax_eps;
-    return write_ctl(s, &h);
+    rc = write_ctl(s, &h);
+    if (rc) return rc;
+    if (s->d.engine == WENOX_ENGINE_FUSED && !s->split) {
+        // the stored CFL maxima were formed with the previous floor
+        s->ops->cfl_part(s->U, s->d.nx, s->d.ny, s->ld, s->d.gamma, s->ctl, s->stream);
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    return WENOX_OK;
 }
 
 extern "C" int wenox_solver_set_pfloor(wenox_solver* s, double pmin)
@@ -937,8 +988,7 @@ extern "C" int wenox_solver_set_pfloor(wenox_solver* s, double pmin)
     if (pmin > 0.0 && s->split) return fail(WENOX_E_ARG, "the pressure floor applies to FP64 states");
     s->pfloor = pmin;
     // the captured step graph lacks (or holds) the floor launch
-    if (s->exec) { (void)hipGraphExecDestroy(s->exec); s->exec = nullptr; }
-    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    drop_graphs(s);
     return WENOX_OK;
 }
 
@@ -981,8 +1031,7 @@ extern "C" int wenox_solver_set_tiling(wenox_solver* s, int chunk_x, int chunk_y
     }
     s->plan = p;
     // the captured step graph holds the old launch shapes
-    if (s->exec) { (void)hipGraphExecDestroy(s->exec); s->exec = nullptr; }
-    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    drop_graphs(s);
     return WENOX_OK;
 }
 
@@ -998,8 +1047,7 @@ extern "C" int wenox_solver_set_combine_cols(wenox_solver* s, int cols)
     if (!s || (cols != 32 && cols != 64)) return fail(WENOX_E_ARG, "wenox_solver_set_combine_cols: cols is 32 or 64");
     if (s->d.engine != WENOX_ENGINE_FUSED) return fail(WENOX_E_ARG, "the combine tile applies to the fused engine");
     s->plan.combine_cols = cols;
-    if (s->exec) { (void)hipGraphExecDestroy(s->exec); s->exec = nullptr; }
-    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    drop_graphs(s);
     return WENOX_OK;
 }
 

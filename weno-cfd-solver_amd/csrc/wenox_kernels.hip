@@ -327,6 +327,16 @@ __global__ void k_advance(StepCtl* ctl)
 
 static void h_advance(StepCtl* ctl, hipStream_t s) { k_advance<<<1, 64, 0, s>>>(ctl); }
 
+static void h_cfl_part(const double* U, int64_t nx, int64_t ny, int64_t ld, double gam, StepCtl* ctl,
+                       hipStream_t s)
+{
+    launch_cfl(StF64{const_cast<double*>(U), (nx + 6) * ld}, nx, ny, ld, gam, ctl, s, false);
+}
+
+static void h_step_dt(StepCtl* ctl, hipStream_t s) { k_step_dt<<<1, 256, 0, s>>>(ctl); }
+
+static void h_settle(StepCtl* ctl, hipStream_t s) { k_settle<<<1, 64, 0, s>>>(ctl); }
+
 // WENO_HLLC_2d.py:813-837 (con_var2primi_pfloor): where the pressure of a cell
 // (density floored at 1e-12 for the velocities and the kinetic energy) falls
 // below pmin, raise its total energy so that p = pmin.  Every padded cell of
@@ -442,8 +452,8 @@ static void h_rk(const double* Un, const double* Us, const double* L, double* ou
 #ifndef __HIP_DEVICE_COMPILE__
 const Ops WX_OPS = {
     WX_NS::h_flux, WX_NS::h_con2primi, WX_NS::h_snapshot8, WX_NS::h_weno, WX_NS::h_riemann, WX_NS::h_divergence,
-    WX_NS::h_bc,   WX_NS::h_cfl,       WX_NS::h_advance, WX_NS::h_rhs_unfused, WX_NS::h_rk,
-    WX_NS::h_pfloor,
+    WX_NS::h_bc,   WX_NS::h_cfl,       WX_NS::h_advance, WX_NS::h_cfl_part, WX_NS::h_step_dt,
+    WX_NS::h_settle, WX_NS::h_rhs_unfused, WX_NS::h_rk, WX_NS::h_pfloor,
 };
 
 #if !WX_FAST && !WX_TWIN

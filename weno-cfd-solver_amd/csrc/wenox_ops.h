@@ -46,6 +46,8 @@ struct StepCtl {
     unsigned int blocks_done;       // last-block detection of the reduction
     int floor_valid;                // floor_bits holds the next step's c_max (pressure floor ran)
     unsigned long long floor_bits;  // max of c+|V| from the floored primitives (bit pattern)
+    int pending;                    // a finished step's T += dt, step += 1 not yet applied
+    int nparts;                     // valid entries of part (the last writer's block count)
     double* part;                   // per-block maxima of the CFL reduction (nullptr: atomics)
 };
 
@@ -105,6 +107,15 @@ struct Ops {
                const StepCtl* ctl, int stage, hipStream_t s);
     void (*cfl)(const double* U, int64_t nx, int64_t ny, int64_t ld, double gam, StepCtl* ctl, hipStream_t s);
     void (*advance)(StepCtl* ctl, hipStream_t s);
+    // the per-block maxima of the CFL reduction only (into ctl->part)
+    void (*cfl_part)(const double* U, int64_t nx, int64_t ny, int64_t ld, double gam, StepCtl* ctl,
+                     hipStream_t s);
+    // a step's first launch on the fused engine: apply the previous step's
+    // pending advance, reduce ctl->part (written by the previous step's last
+    // RK combine, or by cfl_part), form dt and the loop condition
+    void (*step_dt)(StepCtl* ctl, hipStream_t s);
+    // apply a pending advance (before the host reads T and the step count)
+    void (*settle)(StepCtl* ctl, hipStream_t s);
     // Unfused RHS: L (interior Nx x Ny, SoA4) from U (padded SoA4), through
     // face arrays in `work` (rhs_unfused_scratch(nx, ny) doubles).
     void (*rhs_unfused)(const double* U, int64_t nx, int64_t ny, int64_t ld, double dx, double gam, int cons,

@@ -207,8 +207,28 @@ __global__ void k_bc(St U, St UT, bool has_t, int64_t nx, int64_t ny, int64_t ld
 // step), the pressure floor's reduction after the previous step, the state's
 // own reduction.  A step that does not run leaves the first two for the next
 // one that does.
+// c + |V| of one cell as the CFL rule forms it (Rie_prob_2d_rs.py:312-315):
+// con2primi velocities and pressure, sound speed over max(rho, rho_floor)
+__device__ __forceinline__ double cfl_speed(const double q[4], double gam, double rho_floor)
+{
+    Prim<double> r = con2primi_pt(q, gam);
+    double spd = sqrt(r.u * r.u + r.v * r.v);
+    double c = sqrt(gam * r.p / fmax(q[0], rho_floor));
+    return c + spd;
+}
+
+__device__ __forceinline__ void apply_pending(StepCtl* ctl)
+{
+    if (ctl->pending) {
+        ctl->T += ctl->dt;
+        ctl->step += 1;
+        ctl->pending = 0;
+    }
+}
+
 __device__ __forceinline__ void cfl_finish(StepCtl* ctl, double cmax_state)
 {
+    apply_pending(ctl);
     ctl->cmax_bits = 0ull;
     ctl->blocks_done = 0u;
     double T = ctl->T;
@@ -247,11 +267,7 @@ __device__ __forceinline__ void cfl_block(St U, int64_t nx, int64_t ny, int64_t 
         for (int64_t j = threadIdx.x; j < ny; j += ST_TB) {
             const int64_t o = o0 + j;
             double q[4] = {U.ld(0, o), U.ld(1, o), U.ld(2, o), U.ld(3, o)};
-            Prim<double> r = con2primi_pt(q, gam);
-            double spd = sqrt(r.u * r.u + r.v * r.v);
-            double c = sqrt(gam * r.p / fmax(q[0], ctl->rho_floor));
-            double w = c + spd;
-            vmaxl = nan_max(vmaxl, w);
+            vmaxl = nan_max(vmaxl, cfl_speed(q, gam, ctl->rho_floor));
         }
     }
     vmaxl = wave_max(vmaxl);
@@ -264,6 +280,7 @@ __device__ __forceinline__ void cfl_block(St U, int64_t nx, int64_t ny, int64_t 
         for (int w = 1; w < ST_TB / 64; ++w) b = nan_max(b, part[w]);
         if (slots) {
             slots[blk] = b;
+            if (blk == 0) ctl->nparts = (int)nblk;
         } else {
             atomicMax(&ctl->cmax_bits, (unsigned long long)__double_as_longlong(b));
             __threadfence();
@@ -294,6 +311,24 @@ __global__ void __launch_bounds__(64) k_cfl_dt(StepCtl* ctl, int nblk)
     if (threadIdx.x == 0) cfl_finish(ctl, v);
 }
 
+// The same over ctl->nparts values (up to one per RK-combine tile).
+__global__ void __launch_bounds__(256) k_step_dt(StepCtl* ctl)
+{
+    __shared__ double part[4];
+    const int n = ctl->nparts;
+    double v = 0.0;
+    for (int b = (int)threadIdx.x; b < n; b += 256) v = nan_max(v, ctl->part[b]);
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) cfl_finish(ctl, nan_max(nan_max(part[0], part[1]), nan_max(part[2], part[3])));
+}
+
+__global__ void k_settle(StepCtl* ctl)
+{
+    if (threadIdx.x == 0) apply_pending(ctl);
+}
+
 template <class St>
 static void launch_bc(St U, St UT, bool has_t, int64_t nx, int64_t ny, int64_t ld, int64_t ldT, BcParams bp,
                       const StepCtl* ctl, int stage, hipStream_t s)
@@ -304,11 +339,12 @@ static void launch_bc(St U, St UT, bool has_t, int64_t nx, int64_t ny, int64_t l
 }
 
 template <class St>
-static void launch_cfl(St U, int64_t nx, int64_t ny, int64_t ld, double gam, StepCtl* ctl, hipStream_t s)
+static void launch_cfl(St U, int64_t nx, int64_t ny, int64_t ld, double gam, StepCtl* ctl, hipStream_t s,
+                       bool finish = true)
 {
     // whole rows per block; ctl->part holds CFL_BLOCKS values
     int64_t nb = nx < CFL_BLOCKS ? nx : CFL_BLOCKS;
     if (nb < 1) nb = 1;
     k_cfl<St><<<(unsigned)nb, ST_TB, 0, s>>>(U, nx, ny, ld, gam, ctl);
-    k_cfl_dt<<<1, 64, 0, s>>>(ctl, (int)nb);
+    if (finish) k_cfl_dt<<<1, 64, 0, s>>>(ctl, (int)nb);
 }

@@ -140,6 +140,19 @@ __device__ __attribute__((noinline)) void face_row_fp64(const T (*c)[4], T gam, 
     }
 }
 
+// Cells of one workgroup of W waves (see sweep_body), and the first cell of
+// wave k of the group (k = W: the group's end).
+__host__ __device__ inline int64_t group_cells(int W, int chunk, bool blend)
+{
+    return W > 1 ? (int64_t)W * chunk - (blend ? 3 : 1) : (int64_t)chunk;
+}
+__host__ __device__ inline int64_t wave_start(int W, int chunk, bool blend, int k)
+{
+    if (k <= 0) return 0;
+    if (k >= W) return group_cells(W, chunk, blend);
+    return (int64_t)k * chunk - (blend ? 1 : 0);
+}
+
 // One workgroup: column block bx, row group by of W consecutive chunks, one
 // wavefront per chunk; `bid` indexes the flags (W per workgroup).
 //
@@ -152,8 +165,14 @@ __device__ __attribute__((noinline)) void face_row_fp64(const T (*c)[4], T gam, 
 // defers the blend of its first face (it needs the previous wave's last p̄)
 // and of its last face (it needs the next wave's first p̄) to the end of the
 // march, and the flux differences of its first and last cell follow them.
-// Halo cost per group: 2 faces for W * chunk cells.  Needs chunk >= 3 when
-// W > 1 (the host enforces it).
+// Halo cost per group: 2 faces.  With W > 1 the cells are split so that
+// every wave marches the same number of faces: `chunk` cells for the middle
+// waves, one fewer for the first (it adds the halo face r0 - 1) and, for the
+// last, as many fewer as the faces it adds (r1, and the halo face r1 + 1 with
+// the blend), so a group covers W * chunk - 3 cells (W * chunk - 1 without
+// the blend); group_cells gives the count.  The waves of a group run on
+// separate SIMDs of one CU, and equal marches keep each of them busy until
+// the group's end.  Needs chunk >= 3 when W > 1 (the host enforces it).
 template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND, int W>
 __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64_t by, int64_t bid,
                                            SweepLds<T, W>& S)
@@ -189,8 +208,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
     const bool out_lane = l >= 3 && l < WAVE - 3 && t >= 0 && t < A.nt;
     const int lm = l > 0 ? l - 1 : 0, lp = l < WAVE - 1 ? l + 1 : WAVE - 1;   // sensor neighbours
 
-    const int64_t r0 = (by * W + wv) * (int64_t)A.chunk;
-    const int64_t r1 = r0 + A.chunk < A.nn ? r0 + A.chunk : A.nn;   // cells [r0, r1)
+    const int64_t g0 = by * group_cells(W, A.chunk, BLEND);
+    const int64_t r0 = g0 + wave_start(W, A.chunk, BLEND, wv);
+    const int64_t e1 = g0 + wave_start(W, A.chunk, BLEND, wv + 1);
+    const int64_t r1 = e1 < A.nn ? e1 : A.nn;                         // cells [r0, r1)
     const bool empty = r0 >= A.nn;
     const bool last = wv == W - 1 || r1 == A.nn;                    // owns face r1
     const bool dfirst = BLEND && wv > 0;                             // first face blended at the end
@@ -475,13 +496,20 @@ constexpr int TT = 32;
 // state, so the tile needs two barriers; the transposed state then leaves
 // LDS along x.  Wider tiles give the x-major streams 512 B rows.
 // Row pitches: ld of Un, Us, out; ldT of outT; ldx of Dx; ldyT of DyT.
-template <class St, typename DT, int TJ>
+// CFL (the final stage of a step on the fused FP64 engine): also reduce the
+// next step's wave speed c + |V| over the tile's new cells into ctl->part[t]
+// (t the tile's index among ntiles) and mark the step's advance pending; the
+// next step's first launch finishes both.
+template <class St, typename DT, int TJ, bool CFL = false>
 __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT* __restrict__ DyT, St Un,
                                              St Us, St out, St outT, int64_t nx, int64_t ny, int64_t ld,
                                              int64_t ldT, int64_t ldx, int64_t ldyT, int stage,
-                                             const StepCtl* ctl, int64_t ti, int64_t tj)
+                                             const StepCtl* ctl, int64_t ti, int64_t tj, double gam = 0.0,
+                                             int64_t t = 0, int64_t ntiles = 0)
 {
     if (ctl && !ctl->active) return;
+    double wmax = 0.0;
+    __shared__ double wpart[4];   // per-wave CFL maxima (CFL only)
     __shared__ double tile[4][TJ][TT + 1];   // [k][j - j0][i - i0]
     const int64_t i0 = ti * TT, j0 = tj * TJ;
     const int64_t PX = nx * ldx, PY = ny * ldyT;
@@ -506,6 +534,7 @@ __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT
             const int64_t i = i0 + r, j = j0 + tx;
             if (i < nx && j < ny) {
                 const int64_t o = (i + 3) * ld + (j + 3);
+                double qv[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const double l = -((double)Dx[k * PX + i * ldx + j] + tile[k][tx][r]);
@@ -516,11 +545,27 @@ __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT
                     else v = (1.0 / 3.0) * qn + (2.0 / 3.0) * Us.ld(k, o) + ((2.0 / 3.0) * dt) * l;
                     out.st(k, o, v);
                     tile[k][tx][r] = v;
+                    qv[k] = v;
                 }
+                if constexpr (CFL) wmax = nan_max(wmax, cfl_speed(qv, gam, ctl->rho_floor));
             }
         }
     }
+    if constexpr (CFL) {
+        wmax = wave_max(wmax);
+        if ((threadIdx.x & 63) == 0) wpart[threadIdx.x >> 6] = wmax;
+    }
     __syncthreads();
+    if constexpr (CFL) {
+        if (threadIdx.x == 0) {
+            StepCtl* c = const_cast<StepCtl*>(ctl);
+            c->part[t] = nan_max(nan_max(wpart[0], wpart[1]), nan_max(wpart[2], wpart[3]));
+            if (t == 0) {
+                c->nparts = (int)ntiles;
+                c->pending = 1;
+            }
+        }
+    }
     // outT[k][j+3][i+3], contiguous i
     {
         const int c = threadIdx.x & (TT - 1), r0 = threadIdx.x / TT;
@@ -539,31 +584,43 @@ __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT
 // into 8 contiguous bands (the first ntiles % 8 of them one tile longer): each
 // XCD's L2 then holds a contiguous band, and the cache lines two neighbouring
 // tiles share (rows are 32 doubles, 3 off a line boundary) are fetched once.
-template <class St, typename DT, int TJ>
+template <class St, typename DT, int TJ, bool CFL = false>
 __global__ void __launch_bounds__(256) k_combine(const DT* __restrict__ Dx, const DT* __restrict__ DyT,
                                                  St Un, St Us, St out, St outT, int64_t nx, int64_t ny,
                                                  int64_t ld, int64_t ldT, int64_t ldx, int64_t ldyT,
-                                                 int stage, const StepCtl* ctl)
+                                                 int stage, const StepCtl* ctl, double gam)
 {
     const int64_t tiles_j = gridDim.x, ntiles = (int64_t)gridDim.x * gridDim.y;
     const int64_t b = (int64_t)blockIdx.y * tiles_j + blockIdx.x;
     const int64_t q = ntiles / 8, r = ntiles % 8, x = b % 8, k = b / 8;
     const int64_t t = x * q + (x < r ? x : r) + k;
-    combine_tile<St, DT, TJ>(Dx, DyT, Un, Us, out, outT, nx, ny, ld, ldT, ldx, ldyT, stage, ctl, t / tiles_j,
-                             t % tiles_j);
+    combine_tile<St, DT, TJ, CFL>(Dx, DyT, Un, Us, out, outT, nx, ny, ld, ldT, ldx, ldyT, stage, ctl,
+                                  t / tiles_j, t % tiles_j, gam, t, ntiles);
 }
 
+// cfl: the step's last combine also leaves the next step's CFL maxima (one
+// per tile, combine_tiles(p) of them) in ctl->part and the advance pending.
 template <class St, typename DT>
 static void launch_combine(const SweepPlan& p, const DT* Dx, const DT* DyT, St Un, St Us, St out, St outT,
-                           int stage, const StepCtl* ctl, hipStream_t s)
+                           int stage, const StepCtl* ctl, hipStream_t s, bool cfl = false)
 {
     const unsigned ti = (unsigned)((p.nx + TT - 1) / TT);
     if (p.combine_cols == 64) {
-        k_combine<St, DT, 64><<<dim3((unsigned)((p.ny + 63) / 64), ti), 256, 0, s>>>(
-            Dx, DyT, Un, Us, out, outT, p.nx, p.ny, p.ld, p.ldT, p.ldx, p.ldyT, stage, ctl);
+        const dim3 g((unsigned)((p.ny + 63) / 64), ti);
+        if (cfl)
+            k_combine<St, DT, 64, true><<<g, 256, 0, s>>>(Dx, DyT, Un, Us, out, outT, p.nx, p.ny, p.ld, p.ldT,
+                                                           p.ldx, p.ldyT, stage, ctl, p.gamma);
+        else
+            k_combine<St, DT, 64><<<g, 256, 0, s>>>(Dx, DyT, Un, Us, out, outT, p.nx, p.ny, p.ld, p.ldT, p.ldx,
+                                                     p.ldyT, stage, ctl, p.gamma);
     } else {
-        k_combine<St, DT, 32><<<dim3((unsigned)((p.ny + 31) / 32), ti), 256, 0, s>>>(
-            Dx, DyT, Un, Us, out, outT, p.nx, p.ny, p.ld, p.ldT, p.ldx, p.ldyT, stage, ctl);
+        const dim3 g((unsigned)((p.ny + 31) / 32), ti);
+        if (cfl)
+            k_combine<St, DT, 32, true><<<g, 256, 0, s>>>(Dx, DyT, Un, Us, out, outT, p.nx, p.ny, p.ld, p.ldT,
+                                                           p.ldx, p.ldyT, stage, ctl, p.gamma);
+        else
+            k_combine<St, DT, 32><<<g, 256, 0, s>>>(Dx, DyT, Un, Us, out, outT, p.nx, p.ny, p.ld, p.ldT, p.ldx,
+                                                     p.ldyT, stage, ctl, p.gamma);
     }
 }
 
@@ -608,7 +665,7 @@ __global__ void __launch_bounds__(256) k_transpose(const E* in, E* out, int64_t 
 template <typename T, int IN, typename DT, int MODE, int DIR, int W>
 static void launch_sweep_w(const SweepArgs& a, int cons, int blend, hipStream_t s)
 {
-    const int64_t rows = (int64_t)W * a.chunk;
+    const int64_t rows = group_cells(W, a.chunk, blend != 0);
     dim3 grid((unsigned)((a.nt + OUT_COLS - 1) / OUT_COLS), (unsigned)((a.nn + rows - 1) / rows));
     if (cons) {
         if (blend) k_sweep<T, IN, DT, MODE, DIR, true, true, W><<<grid, WAVE * W, 0, s>>>(a);
@@ -632,7 +689,7 @@ static void launch_sweep_xy_w(const SweepArgs& ax, const SweepArgs& ay, int cons
 {
     const unsigned colsx = (unsigned)((ax.nt + OUT_COLS - 1) / OUT_COLS);
     const unsigned colsy = (unsigned)((ay.nt + OUT_COLS - 1) / OUT_COLS);
-    const int64_t rx = (int64_t)W * ax.chunk, ry = (int64_t)W * ay.chunk;
+    const int64_t rx = group_cells(W, ax.chunk, blend != 0), ry = group_cells(W, ay.chunk, blend != 0);
     const unsigned nbx = colsx * (unsigned)((ax.nn + rx - 1) / rx);
     const unsigned nby = colsy * (unsigned)((ay.nn + ry - 1) / ry);
     const dim3 grid(nbx + nby);
@@ -690,7 +747,7 @@ static void h_stage(const SweepPlan& p, const double* Us, const double* UsT, con
     h_sweeps(p, Us, UsT, Dx, DyT, ctl, s);
     const int64_t P = p.P();
     launch_combine<StF64, double>(p, Dx, DyT, StF64{const_cast<double*>(Un), P}, StF64{const_cast<double*>(Us), P},
-                                  StF64{out, P}, StF64{outT, p.PT()}, stage, ctl, s);
+                                  StF64{out, P}, StF64{outT, p.PT()}, stage, ctl, s, stage == 3);
 }
 
 static void h_rhs(const SweepPlan& p, const double* U, const double* UT, double* Dx, double* DyT,
@@ -1021,11 +1078,13 @@ static void pick_tiling(int64_t nn, int64_t nt, int& chunk, int& waves)
     for (int w : {1, 2, 4}) {
         const int cmin = w == 1 ? 8 : (w == 2 ? 4 : 3), cmax = w == 1 ? 64 : (w == 2 ? 24 : 16);
         for (int c = cmin; c <= cmax; ++c) {
-            if (w > 1 && (int64_t)w * c > nn) continue;
-            const int64_t groups = (nn + (int64_t)w * c - 1) / ((int64_t)w * c);
+            if (w > 1 && WX_NS::group_cells(w, c, true) > nn) continue;
+            const int64_t gc = WX_NS::group_cells(w, c, true);
+            const int64_t groups = (nn + gc - 1) / gc;
             const int64_t wg = groups * cols, slots = 2048 / w;
             const int64_t rounds = (wg + slots - 1) / slots;
-            const double cost = (double)rounds * (c + 2.0 / w + (w > 1 ? 0.5 : 0.0));
+            // faces one wave marches: c + 3 alone; c (balanced) + deferred work in a group
+            const double cost = (double)rounds * (w > 1 ? c + 0.5 : c + 3.0);
             if (cost < best - 1e-9) { best = cost; chunk = c; waves = w; }
         }
     }
@@ -1054,7 +1113,9 @@ SweepPlan make_sweep_plan(int64_t nx, int64_t ny, double dx, double gamma, int c
 int64_t sweep_flag_count(const SweepPlan& p)
 {
     const int64_t c = WX_NS::OUT_COLS;
-    const int64_t rx = (int64_t)p.chunk_x * p.waves_x, ry = (int64_t)p.chunk_y * p.waves_y;
+    // the blend's groups are the smaller ones: enough flags either way
+    const int64_t rx = WX_NS::group_cells(p.waves_x, p.chunk_x, true);
+    const int64_t ry = WX_NS::group_cells(p.waves_y, p.chunk_y, true);
     int64_t fx = ((p.ny + c - 1) / c) * ((p.nx + rx - 1) / rx) * p.waves_x;
     int64_t fy = ((p.nx + c - 1) / c) * ((p.ny + ry - 1) / ry) * p.waves_y;
     // the one-launch sweep pair numbers its workgroups across both halves

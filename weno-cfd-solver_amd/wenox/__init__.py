@@ -126,7 +126,10 @@ class Solver:
         """Per-kernel device time (ms) of the fused engine, HIP events on the
         solver's stream, each kernel launched `reps` times alone.  "sweeps" is
         the stage's x and y sweep as the step launches them (one kernel when
-        their tilings share a wave count); "step_sum" adds it up per step."""
+        their tilings share a wave count); "step_sum" adds it up per step.
+        "cfl" is the step's CFL launch: with FP64 the one-wave dt reduction
+        (the last RK combine of the previous step forms the maxima), with
+        FP32-split the reduction over the state."""
         out = np.zeros(7)
         _lib.check(_lib.load().wenox_solver_profile(self._h, int(reps), out.ctypes.data, 7))
         return dict(zip(self.KERNELS, out.tolist()))
