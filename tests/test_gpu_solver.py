@@ -166,6 +166,7 @@ def test_sweep_tilings_bitwise(oracle_mod, cons, force_hlle):
     nx, ny = 45, 37
     q0, dx = problems.four_quadrant(nx, ny=ny)
     ref = oracle_mod.l_local_py(q0, dx, 1.4, force_hlle, (0.5, 2.0), cons)
+    assert np.isfinite(ref).all()   # (NaN would compare equal to NaN below)
     print(f"[solver] tilings 45x37 cons={cons} hlle={force_hlle}", flush=True)
     with Solver(nx, ny, dx, cons_switch=cons, force_hlle=force_hlle, jp_cri=(0.5, 2.0), engine="fused") as s:
         s.upload(q0)
@@ -193,6 +194,7 @@ def test_combine_tile_widths_bitwise(oracle_mod):
     nx, ny = 70, 45
     q0, dx = problems.four_quadrant(nx, ny=ny)
     ref, _ = oracle_mod.run_riemann(q0.copy(), nx, ny, dx, 1.4, 0.4, False, False, (0.5, 2.0), nstep=5)
+    assert np.isfinite(ref).all()
     print("[solver] combine tile widths 70x45", flush=True)
     for cols in (32, 64):
         with Solver(nx, ny, dx, cfl=0.4, jp_cri=(0.5, 2.0), engine="fused") as s:
@@ -246,3 +248,38 @@ def test_fused_step_control_across_run_calls(oracle_mod):
         s.set_cfl_rule(0.3, 0.0)
         T_ref, _ = s.run(nstep=4)
     assert T == T_ref
+
+
+@pytest.mark.parametrize("bc", ["mirror", "periodic", "copy", "dmr_ref", "dmr"])
+def test_fused_matches_unfused_for_every_boundary(bc):
+    """The fused engine (sweeps on the state and its transposed copy, ghost
+    fill of both orientations, CFL maxima from the last RK combine) and the
+    unfused one (the reference's pass structure) agree bit for bit (exact
+    variant) after 7 steps for every boundary kind: the full downloaded
+    arrays, ghost cells included, the time, and rhs() of the final state."""
+    from wenox import Solver, problems
+    kw = dict(cfl=0.3, jp_cri=(0.5, 2.0), bc=bc, variant="exact")
+    if bc == "dmr_ref":
+        nx, ny = 80, 20
+        q0, bound = problems.dmr_reference(nx, ny)
+        dx = 1.0 / ny
+        kw.update(dmr_bound=bound, cfl=0.1, jp_cri=(10000, 11000))
+    elif bc == "dmr":
+        nx, ny = 96, 24
+        q0, dx = problems.double_mach(nx, ny)
+        kw.update(cfl=0.1)
+    else:
+        nx, ny = 45, 37
+        q0, dx = problems.four_quadrant(nx, ny=ny)
+    print(f"[solver] fused vs unfused, bc={bc} {nx}x{ny}", flush=True)
+    out = {}
+    for engine in ("unfused", "fused"):
+        with Solver(nx, ny, dx, engine=engine, **kw) as s:
+            s.upload(q0)
+            T, k = s.run(nstep=7)
+            assert k == 7
+            out[engine] = (s.download(), T, s.rhs())
+    assert np.isfinite(out["unfused"][0]).all() and np.isfinite(out["unfused"][2]).all()
+    assert out["fused"][1] == out["unfused"][1]
+    np.testing.assert_array_equal(out["fused"][0], out["unfused"][0])
+    np.testing.assert_array_equal(out["fused"][2], out["unfused"][2])

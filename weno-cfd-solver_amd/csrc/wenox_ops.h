@@ -65,6 +65,7 @@ struct BcParams {
     double dx, dy;      // BC_DMR_STD geometry
     double x0;          // BC_DMR_STD wall start (1/6)
     double gamma;
+    double t0;          // BC_DMR_STD ghost time of a fill launched without StepCtl
 };
 
 // Strides (in elements) of a 3-index array (i, j, k): element at

@@ -79,6 +79,102 @@ __device__ __forceinline__ int64_t periodic_src(int64_t g, int64_t N)
 // rows bound+1..bound+9 copy column 3, the rest mirror with rho*v negated.
 __constant__ double c_dmr_post[4] = {8.0, 33.0, -57.1576, 563.499};
 
+// Where the value of ghost cell (i, j) (padded indices) comes from: the
+// interior cell (si, sj) (src 0, rho v negated when flip) or the post-shock
+// (src 1) or pre-shock (src 2) state of the DMR rules.  ts is the stage time
+// of the standard DMR's moving shock.
+struct BcSrc {
+    int64_t si, sj;
+    int src;
+    bool flip;
+};
+
+__device__ __forceinline__ BcSrc bc_source(int64_t i, int64_t j, int64_t nx, int64_t ny, const BcParams& bp,
+                                           double ts)
+{
+    BcSrc r{i, j, 0, false};
+    if (bp.kind == BC_COPY) {
+        // x rows first, then y columns over all rows: ghost (i, j) <- (clamp i, clamp j)
+        r.si = i < 3 ? 3 : (i > nx + 2 ? nx + 2 : i);
+        r.sj = j < 3 ? 3 : (j > ny + 2 ? ny + 2 : j);
+    } else if (bp.kind == BC_MIRROR || bp.kind == BC_PERIODIC) {
+        r.si = bp.kind == BC_MIRROR ? mirror_src(i, nx) : periodic_src(i, nx);
+        r.sj = bp.kind == BC_MIRROR ? mirror_src(j, ny) : periodic_src(j, ny);
+    } else if (bp.kind == BC_DMR_REF) {
+        const int64_t B = bp.dmr_bound;
+        if (i < 3) {
+            r.src = 1;
+        } else {
+            if (i >= nx + 3) r.si = nx + 2;
+            if (j >= ny + 3) {
+                r.sj = ny + 2;
+            } else if (j < 3) {
+                if (i >= nx + 3) {
+                    r.sj = 5 - j; r.flip = true;
+                } else if (i <= B) {
+                    r.src = 1;
+                } else if (i <= B + 9) {
+                    r.sj = 3;
+                } else {
+                    r.sj = 5 - j; r.flip = true;
+                }
+            }
+        }
+    } else {
+        // Standard double Mach reflection (Woodward & Colella 1984): left
+        // inflow at the post-shock state; right outflow copies the last
+        // interior row; bottom: post-shock for x < x0, reflecting wall
+        // (rho*v negated) beyond; top: post-shock where x lies behind the
+        // exact shock trace x0 + (1 + 20 t) / sqrt(3) at the stage time, else
+        // the resting pre-shock gas.  Ghost columns of the right ghost rows
+        // follow the same rules at the last interior row.
+        const int64_t ie = i > nx + 2 ? nx + 2 : i;
+        const double xc = ((double)(ie - 3) + 0.5) * bp.dx;
+        r.si = ie;
+        if (i < 3) {
+            r.src = 1;
+        } else if (j >= ny + 3) {
+            r.src = xc < bp.x0 + (1.0 + 20.0 * ts) / 1.7320508075688772935 ? 1 : 2;
+        } else if (j < 3) {
+            if (xc < bp.x0) r.src = 1;
+            else { r.sj = 5 - j; r.flip = true; }
+        }
+    }
+    return r;
+}
+
+// The value of a ghost cell from its source: rd(idx of the interior cell in
+// the (si, sj) frame, k) reads the state.
+template <class Rd>
+__device__ __forceinline__ void bc_value(const BcSrc& b, const BcParams& bp, Rd rd, double v[4])
+{
+    if (b.src == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = rd(k);
+        if (b.flip) v[2] = -v[2];
+    } else if (bp.kind == BC_DMR_REF) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = c_dmr_post[k];
+    } else {
+        const double g = bp.gamma;
+        const double rho1 = 8.0, u1 = 8.25 * 0.86602540378443864676, v1 = -8.25 * 0.5, p1 = 116.5;
+        if (b.src == 1) {
+            v[0] = rho1; v[1] = rho1 * u1; v[2] = rho1 * v1;
+            v[3] = p1 / (g - 1.0) + 0.5 * rho1 * (u1 * u1 + v1 * v1);
+        } else {
+            v[0] = 1.4; v[1] = 0.0; v[2] = 0.0; v[3] = 1.0 / (g - 1.0);
+        }
+    }
+}
+
+// Ghost time of a state written by RK stage `stage` (0: the step's input):
+// T, T + dt, T + dt/2, T + dt.  Without ctl: bp.t0.
+__device__ __forceinline__ double bc_time(const StepCtl* ctl, const BcParams& bp, int stage)
+{
+    if (!ctl) return bp.t0;
+    return ctl->T + (stage == 2 ? 0.5 * ctl->dt : (stage == 0 ? 0.0 : ctl->dt));
+}
+
 // Ghost cell number t (0 <= t < 6 (ny+6) + 6 nx) of one state; ld and ldT
 // are the row pitches of U and UT.
 template <class St>
@@ -91,76 +187,10 @@ __device__ __forceinline__ void bc_cell(St U, St UT, bool has_t, int64_t nx, int
     if (t >= count) return;
     int64_t i, j;
     ghost_index(t, nx, ny, i, j);
+    const double ts = bp.kind == BC_DMR_STD ? bc_time(ctl, bp, stage) : 0.0;
+    const BcSrc b = bc_source(i, j, nx, ny, bp, ts);
     double v[4];
-    if (bp.kind == BC_COPY) {
-        // x rows first, then y columns over all rows: ghost (i, j) <- (clamp i, clamp j)
-        int64_t si = i < 3 ? 3 : (i > nx + 2 ? nx + 2 : i);
-        int64_t sj = j < 3 ? 3 : (j > ny + 2 ? ny + 2 : j);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) v[k] = U.ld(k, si * ld + sj);
-    } else if (bp.kind == BC_MIRROR || bp.kind == BC_PERIODIC) {
-        int64_t si = bp.kind == BC_MIRROR ? mirror_src(i, nx) : periodic_src(i, nx);
-        int64_t sj = bp.kind == BC_MIRROR ? mirror_src(j, ny) : periodic_src(j, ny);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) v[k] = U.ld(k, si * ld + sj);
-    } else if (bp.kind == BC_DMR_REF) {
-        const int64_t B = bp.dmr_bound;
-        int64_t si = i, sj = j;
-        bool post = false, flip = false;
-        if (i < 3) {
-            post = true;
-        } else {
-            if (i >= nx + 3) si = nx + 2;
-            if (j >= ny + 3) {
-                sj = ny + 2;
-            } else if (j < 3) {
-                if (i >= nx + 3) {
-                    sj = 5 - j; flip = true;
-                } else if (i <= B) {
-                    post = true;
-                } else if (i <= B + 9) {
-                    sj = 3;
-                } else {
-                    sj = 5 - j; flip = true;
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) v[k] = post ? c_dmr_post[k] : U.ld(k, si * ld + sj);
-        if (flip) v[2] = -v[2];
-    } else {
-        // Standard double Mach reflection (Woodward & Colella 1984): left
-        // inflow at the post-shock state; right outflow copies the last
-        // interior row; bottom: post-shock for x < x0, reflecting wall
-        // (rho*v negated) beyond; top: post-shock where x lies behind the
-        // exact shock trace x0 + (1 + 20 t) / sqrt(3) at the stage time, else
-        // the resting pre-shock gas.  Ghost columns of the right ghost rows
-        // follow the same rules at the last interior row.
-        const double g = bp.gamma;
-        const double rho1 = 8.0, u1 = 8.25 * 0.86602540378443864676, v1 = -8.25 * 0.5, p1 = 116.5;
-        const double post_q[4] = {rho1, rho1 * u1, rho1 * v1,
-                                  p1 / (g - 1.0) + 0.5 * rho1 * (u1 * u1 + v1 * v1)};
-        const double pre_q[4] = {1.4, 0.0, 0.0, 1.0 / (g - 1.0)};
-        double ts = 0.0;
-        if (ctl) ts = ctl->T + (stage == 2 ? 0.5 * ctl->dt : (stage == 0 ? 0.0 : ctl->dt));
-        const int64_t ie = i > nx + 2 ? nx + 2 : i;
-        const double xc = ((double)(ie - 3) + 0.5) * bp.dx;
-        int src = 0;            // 0 copy (ie, sj), 1 post, 2 pre
-        int64_t sj = j;
-        bool flip = false;
-        if (i < 3) {
-            src = 1;
-        } else if (j >= ny + 3) {
-            src = xc < bp.x0 + (1.0 + 20.0 * ts) / 1.7320508075688772935 ? 1 : 2;
-        } else if (j < 3) {
-            if (xc < bp.x0) src = 1;
-            else { sj = 5 - j; flip = true; }
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            v[k] = src == 1 ? post_q[k] : (src == 2 ? pre_q[k] : U.ld(k, ie * ld + sj));
-        if (flip) v[2] = -v[2];
-    }
+    bc_value(b, bp, [&](int k) { return U.ld(k, b.si * ld + b.sj); }, v);
 #pragma unroll
     for (int k = 0; k < 4; ++k) U.st(k, i * ld + j, v[k]);
     if (has_t) {

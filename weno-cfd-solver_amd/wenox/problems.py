@@ -72,18 +72,19 @@ def twin_quadrants(n: int, gamma: float = 1.4):
 
 def four_quadrant(n: int, gamma: float = 1.4, ny: int | None = None):
     """Four-quadrant Riemann problem on [0,1]^2 (Lax-Liu configuration 3):
-    (rho,u,v,p) = (1.5,0,0,1.5) for x>0.5,y>0.5; (0.5323,1.206,0,0.3) for x<0.5,y>0.5; (0.138,1.206,1.206,0.029) for
-    x<0.5,y<0.5; (0.5323,0,1.206,0.3) for x>0.5,y<0.5.  Returns (q, dx)."""
+    (rho,u,v,p) = (1.5,0,0,1.5) for x>=0.5,y>=0.5; (0.5323,1.206,0,0.3) for x<0.5,y>=0.5; (0.138,1.206,1.206,0.029)
+    for x<0.5,y<0.5; (0.5323,0,1.206,0.3) for x>=0.5,y<0.5 (cell centres; with an odd cell count the middle
+    cell lies on 0.5 and joins the upper side).  Returns (q, dx)."""
     nx = n
     ny = n if ny is None else ny
     q = np.zeros((nx + 6, ny + 6, 8))
     x = (np.arange(nx + 6) - 3 + 0.5) / nx
     y = (np.arange(ny + 6) - 3 + 0.5) / ny
     X, Y = np.meshgrid(x, y, indexing="ij")
-    states = [((X > 0.5) & (Y > 0.5), (1.5, 0.0, 0.0, 1.5)),
-              ((X < 0.5) & (Y > 0.5), (0.5323, 1.206, 0.0, 0.3)),
+    states = [((X >= 0.5) & (Y >= 0.5), (1.5, 0.0, 0.0, 1.5)),
+              ((X < 0.5) & (Y >= 0.5), (0.5323, 1.206, 0.0, 0.3)),
               ((X < 0.5) & (Y < 0.5), (0.138, 1.206, 1.206, 0.029)),
-              ((X > 0.5) & (Y < 0.5), (0.5323, 0.0, 1.206, 0.3))]
+              ((X >= 0.5) & (Y < 0.5), (0.5323, 0.0, 1.206, 0.3))]
     for mask, st in states:
         for k in range(4):
             q[:, :, 4 + k][mask] = st[k]
