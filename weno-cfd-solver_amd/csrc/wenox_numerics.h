@@ -498,6 +498,41 @@ template <typename T> WX_HD T weno5_right(T u1, T u2, T u3, T u4, T u5)
     return weno5_core(b0, b1, b2, T(0.1), T(0.6), T(0.3), p0, p1, p2);
 }
 
+// Both WENO5 values of one face: left from u0..u4 (weno5_left), right from
+// u1..u5 (weno5_right).  The fast FP64 variant normalises the two with one
+// reciprocal: left = nL / dL = nL dR / (dL dR), right = nR dL / (dL dR).  dL
+// and dR are 6 sums of d_k times products of two (eps+beta)^2 factors (about
+// beta^4 each), so dL dR stays in range while the indicators stay below
+// about 1e37 (differences of the stencil values below about 1e18); at the
+// other end it is at least about 1e-90.
+template <typename T> WX_HD void weno5_pair(T u0, T u1, T u2, T u3, T u4, T u5, T& left, T& right)
+{
+#if WX_FAST
+    if constexpr (sizeof(T) == 8) {
+        const T eps = T(4) * K<T>::eps;
+        auto nd = [&](T b0, T b1, T b2, T d0, T d1, T d2, T p0, T p1, T p2, T& num, T& den) {
+            T t0 = sq(eps + b0), t1 = sq(eps + b1), t2 = sq(eps + b2);
+            T a0 = d0 * (t1 * t2), a1 = d1 * (t0 * t2), a2 = d2 * (t0 * t1);
+            num = a0 * p0 + a1 * p1 + a2 * p2;
+            den = T(6) * (a0 + a1 + a2);
+        };
+        T nl, dl, nr, dr;
+        nd(beta4(u2 - T(2) * u3 + u4, T(3) * u2 - T(4) * u3 + u4), beta4(u1 - T(2) * u2 + u3, u1 - u3),
+           beta4(u0 - T(2) * u1 + u2, u0 - T(4) * u1 + T(3) * u2), T(0.3), T(0.6), T(0.1),
+           -u4 + T(5) * u3 + T(2) * u2, -u1 + T(5) * u2 + T(2) * u3, T(2) * u0 - T(7) * u1 + T(11) * u2, nl, dl);
+        nd(beta4(u3 - T(2) * u4 + u5, T(3) * u3 - T(4) * u4 + u5), beta4(u2 - T(2) * u3 + u4, u2 - u4),
+           beta4(u1 - T(2) * u2 + u3, u1 - T(4) * u2 + T(3) * u3), T(0.1), T(0.6), T(0.3),
+           T(2) * u5 - T(7) * u4 + T(11) * u3, -u4 + T(5) * u3 + T(2) * u2, -u1 + T(5) * u2 + T(2) * u3, nr, dr);
+        const T r = rcp(dl * dr);
+        left = nl * (dr * r);
+        right = nr * (dl * r);
+        return;
+    }
+#endif
+    left = weno5_left(u0, u1, u2, u3, u4);
+    right = weno5_right(u1, u2, u3, u4, u5);
+}
+
 // weno.rs:262-303  Gauss-point values (ga at -h/(2 sqrt 3), gb at +h/(2 sqrt 3))
 // from five face values, sharing the smoothness indicators.
 template <typename T> WX_HD void weno5_gq(T q0, T q1, T q2, T q3, T q4, T& ga, T& gb)
@@ -529,8 +564,15 @@ template <typename T> WX_HD void weno5_gq(T q0, T q1, T q2, T q3, T q4, T& ga, T
     T nb = SC * u0 + SB * u1 - SA * u2;
     T da = dA * e0 + dB * e1 + dC * e2;
     T db = dC * e0 + dB * e1 + dA * e2;
-    ga = q2 + na * rcp(da);
-    gb = q2 - nb * rcp(db);
+    if constexpr (sizeof(T) == 8) {
+        // one reciprocal for both normalisations (da db stays in range as in weno5_pair)
+        const T r = rcp(da * db);
+        ga = q2 + na * (db * r);
+        gb = q2 - nb * (da * r);
+    } else {
+        ga = q2 + na * rcp(da);
+        gb = q2 - nb * rcp(db);
+    }
 #else
     const T c13 = T(13.0 / 12.0);
     const T eps = K<T>::eps;
@@ -600,10 +642,7 @@ template <typename T> WX_HD void recon_normal(int dir, bool cons, const T c[6][4
 {
     if (cons) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            ql[k] = weno5_left(c[0][k], c[1][k], c[2][k], c[3][k], c[4][k]);
-            qr[k] = weno5_right(c[1][k], c[2][k], c[3][k], c[4][k], c[5][k]);
-        }
+        for (int k = 0; k < 4; ++k) weno5_pair(c[0][k], c[1][k], c[2][k], c[3][k], c[4][k], c[5][k], ql[k], qr[k]);
         return;
     }
     Roe<T> s = roe_avg(c[2], c[3], gam);
@@ -621,10 +660,7 @@ template <typename T> WX_HD void recon_normal(int dir, bool cons, const T c[6][4
 #endif
     T chl[4], chr[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        chl[k] = weno5_left(w[0][k], w[1][k], w[2][k], w[3][k], w[4][k]);
-        chr[k] = weno5_right(w[1][k], w[2][k], w[3][k], w[4][k], w[5][k]);
-    }
+    for (int k = 0; k < 4; ++k) weno5_pair(w[0][k], w[1][k], w[2][k], w[3][k], w[4][k], w[5][k], chl[k], chr[k]);
 #if WX_FAST
     const RightCols<T> R = right_cols(dir, s);
     right_project(dir, R, chl, ql);
