@@ -333,7 +333,7 @@ static void h_cfl_part(const double* U, int64_t nx, int64_t ny, int64_t ld, doub
     launch_cfl(StF64{const_cast<double*>(U), (nx + 6) * ld}, nx, ny, ld, gam, ctl, s, false);
 }
 
-static void h_step_dt(StepCtl* ctl, hipStream_t s) { k_step_dt<<<1, 256, 0, s>>>(ctl); }
+static void h_step_dt(StepCtl* ctl, hipStream_t s) { k_step_dt<<<1, 1024, 0, s>>>(ctl); }
 
 static void h_settle(StepCtl* ctl, hipStream_t s) { k_settle<<<1, 64, 0, s>>>(ctl); }
 

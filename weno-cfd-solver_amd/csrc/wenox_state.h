@@ -241,10 +241,20 @@ __global__ void k_bc(St U, St UT, bool has_t, int64_t nx, int64_t ny, int64_t ld
 // con2primi velocities and pressure, sound speed over max(rho, rho_floor)
 __device__ __forceinline__ double cfl_speed(const double q[4], double gam, double rho_floor)
 {
+#if WX_FAST
+    // the fast variant: reciprocals and roots from v_rcp_f64 / v_rsq_f64 with
+    // their Newton / Goldschmidt steps (rounding-level differences)
+    const double rho = fmax(q[0], K<double>::eps);
+    const double ir = rcp(rho);
+    const double u = q[1] * ir, v = q[2] * ir;
+    const double p = fmax((gam - 1.0) * (q[3] - 0.5 * rho * (u * u + v * v)), K<double>::eps);
+    return sqrt_z(gam * p * rcp(fmax(q[0], rho_floor))) + sqrt_z(u * u + v * v);
+#else
     Prim<double> r = con2primi_pt(q, gam);
     double spd = sqrt(r.u * r.u + r.v * r.v);
     double c = sqrt(gam * r.p / fmax(q[0], rho_floor));
     return c + spd;
+#endif
 }
 
 __device__ __forceinline__ void apply_pending(StepCtl* ctl)
@@ -341,17 +351,22 @@ __global__ void __launch_bounds__(64) k_cfl_dt(StepCtl* ctl, int nblk)
     if (threadIdx.x == 0) cfl_finish(ctl, v);
 }
 
-// The same over ctl->nparts values (up to one per RK-combine tile).
-__global__ void __launch_bounds__(256) k_step_dt(StepCtl* ctl)
+// The same over ctl->nparts values (up to one per RK-combine tile: 16384 at
+// 4096^2), 1024 threads.
+__global__ void __launch_bounds__(1024) k_step_dt(StepCtl* ctl)
 {
-    __shared__ double part[4];
+    __shared__ double part[16];
     const int n = ctl->nparts;
     double v = 0.0;
-    for (int b = (int)threadIdx.x; b < n; b += 256) v = nan_max(v, ctl->part[b]);
+    for (int b = (int)threadIdx.x; b < n; b += 1024) v = nan_max(v, ctl->part[b]);
     v = wave_max(v);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) cfl_finish(ctl, nan_max(nan_max(part[0], part[1]), nan_max(part[2], part[3])));
+    if (threadIdx.x == 0) {
+        double m = part[0];
+        for (int w = 1; w < 16; ++w) m = nan_max(m, part[w]);
+        cfl_finish(ctl, m);
+    }
 }
 
 __global__ void k_settle(StepCtl* ctl)

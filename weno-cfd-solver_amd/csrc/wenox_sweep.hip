@@ -534,7 +534,6 @@ __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT
             const int64_t i = i0 + r, j = j0 + tx;
             if (i < nx && j < ny) {
                 const int64_t o = (i + 3) * ld + (j + 3);
-                double qv[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const double l = -((double)Dx[k * PX + i * ldx + j] + tile[k][tx][r]);
@@ -545,37 +544,50 @@ __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT
                     else v = (1.0 / 3.0) * qn + (2.0 / 3.0) * Us.ld(k, o) + ((2.0 / 3.0) * dt) * l;
                     out.st(k, o, v);
                     tile[k][tx][r] = v;
-                    qv[k] = v;
                 }
-                if constexpr (CFL) wmax = nan_max(wmax, cfl_speed(qv, gam, ctl->rho_floor));
             }
         }
-    }
-    if constexpr (CFL) {
-        wmax = wave_max(wmax);
-        if ((threadIdx.x & 63) == 0) wpart[threadIdx.x >> 6] = wmax;
     }
     __syncthreads();
-    if constexpr (CFL) {
-        if (threadIdx.x == 0) {
-            StepCtl* c = const_cast<StepCtl*>(ctl);
-            c->part[t] = nan_max(nan_max(wpart[0], wpart[1]), nan_max(wpart[2], wpart[3]));
-            if (t == 0) {
-                c->nparts = (int)ntiles;
-                c->pending = 1;
-            }
-        }
-    }
     // outT[k][j+3][i+3], contiguous i
     {
         const int c = threadIdx.x & (TT - 1), r0 = threadIdx.x / TT;
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
+        if constexpr (CFL) {
+            // the transposed stores visit every new cell once: take its c + |V| here,
+            // away from the x-major loop's loads and stores
 #pragma unroll
             for (int r = r0; r < TJ; r += 256 / TT) {
                 const int64_t j = j0 + r, i = i0 + c;
-                if (j < ny && i < nx) outT.st(k, (j + 3) * ldT + (i + 3), tile[k][r][c]);
+                if (j < ny && i < nx) {
+                    double qv[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        qv[k] = tile[k][r][c];
+                        outT.st(k, (j + 3) * ldT + (i + 3), qv[k]);
+                    }
+                    wmax = nan_max(wmax, cfl_speed(qv, gam, ctl->rho_floor));
+                }
             }
+            wmax = wave_max(wmax);
+            if ((threadIdx.x & 63) == 0) wpart[threadIdx.x >> 6] = wmax;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                StepCtl* cw = const_cast<StepCtl*>(ctl);
+                cw->part[t] = nan_max(nan_max(wpart[0], wpart[1]), nan_max(wpart[2], wpart[3]));
+                if (t == 0) {
+                    cw->nparts = (int)ntiles;
+                    cw->pending = 1;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int r = r0; r < TJ; r += 256 / TT) {
+                    const int64_t j = j0 + r, i = i0 + c;
+                    if (j < ny && i < nx) outT.st(k, (j + 3) * ldT + (i + 3), tile[k][r][c]);
+                }
+        }
     }
 }
 
