@@ -333,25 +333,27 @@ _DMR_SRC32 = "profiles/dmr1920x480_fp32split.md"
 _QUAD_SRC64 = "profiles/quad4096_fp64.md"
 _QUAD_SRC32 = "profiles/quad4096_fp32split.md"
 TRAFFIC = {
-    ("double_mach_reflection", 1920, 480, False): {"bytes_per_step": 1.0935e9, "tiling": (9, 8, 4, 4),
+    ("double_mach_reflection", 1920, 480, False): {"bytes_per_step": 1.0838e9, "tiling": (10, 9, 4, 4),
                                                    "source": _DMR_SRC64},
-    ("double_mach_reflection", 1920, 480, True): {"bytes_per_step": 1.1132e9, "tiling": (9, 8, 4, 4),
+    ("double_mach_reflection", 1920, 480, True): {"bytes_per_step": 1.1491e9, "tiling": (10, 8, 2, 1),
                                                   "source": _DMR_SRC32},
-    ("four_quadrant_riemann", 4096, 4096, False): {"bytes_per_step": 17.6371e9, "tiling": (48, 48, 1, 1),
+    ("four_quadrant_riemann", 4096, 4096, False): {"bytes_per_step": 17.3434e9, "tiling": (48, 48, 1, 1),
                                                    "source": _QUAD_SRC64},
-    ("four_quadrant_riemann", 4096, 4096, True): {"bytes_per_step": 15.6186e9, "tiling": (48, 48, 1, 1),
+    ("four_quadrant_riemann", 4096, 4096, True): {"bytes_per_step": 15.6968e9, "tiling": (48, 48, 1, 1),
                                                   "source": _QUAD_SRC32},
 }
-# VALU wave-instructions of one k_sweep_xy launch (FP32-split: including its
-# in-launch FP64 recompute of flagged face rows)
+# VALU wave-instructions of a stage's sweeps as the step launches them (one
+# k_sweep_xy, or the x and y k_sweep launches together when their wave counts
+# differ; FP32-split: including the in-launch FP64 recompute of flagged face
+# rows)
 VALU = {
-    ("double_mach_reflection", 1920, 480, False, "sweeps"): {"wave_instr_per_launch": 4.990e7,
-                                                             "tiling": (9, 8, 4, 4), "source": _DMR_SRC64},
-    ("double_mach_reflection", 1920, 480, True, "sweeps"): {"wave_instr_per_launch": 6.203e7,
-                                                            "tiling": (9, 8, 4, 4), "source": _DMR_SRC32},
-    ("four_quadrant_riemann", 4096, 4096, False, "sweeps"): {"wave_instr_per_launch": 8.097e8,
+    ("double_mach_reflection", 1920, 480, False, "sweeps"): {"wave_instr_per_launch": 4.980e7,
+                                                             "tiling": (10, 9, 4, 4), "source": _DMR_SRC64},
+    ("double_mach_reflection", 1920, 480, True, "sweeps"): {"wave_instr_per_launch": 3.442e7 + 3.757e7,
+                                                            "tiling": (10, 8, 2, 1), "source": _DMR_SRC32},
+    ("four_quadrant_riemann", 4096, 4096, False, "sweeps"): {"wave_instr_per_launch": 8.092e8,
                                                              "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64},
-    ("four_quadrant_riemann", 4096, 4096, True, "sweeps"): {"wave_instr_per_launch": 1.013e9,
+    ("four_quadrant_riemann", 4096, 4096, True, "sweeps"): {"wave_instr_per_launch": 1.014e9,
                                                             "tiling": (48, 48, 1, 1), "source": _QUAD_SRC32},
 }
 
