@@ -155,3 +155,25 @@ def test_twin_solver_arguments_checked():
         args.update(kw)
         with pytest.raises((ValueError, WenoxError)):
             Solver(16, 16, 1 / 16, **args)
+
+
+def test_main_twin_snapshots(oracle_mod, tmp_path, monkeypatch):
+    """main_twin with fileStorage writes the twin's files: data/<stamp>/0.npy
+    (the initial state), {i}.npy for i in 1 .. N_STEP-1 and time.npy, the
+    conserved channels and times equal to the oracle's twin loop."""
+    import glob
+    from wenox import drivers, problems
+    monkeypatch.chdir(tmp_path)
+    n, n_step = 24, 6
+    print(f"[twin] main_twin snapshots {n}^2 {n_step - 1} steps", flush=True)
+    q = drivers.main_twin(n, n_step, fileStorage=True)
+    ref, ts = oracle_mod.run_twin(problems.twin_quadrants(n), n, n_step, False, (1.0, 4.0))
+    folders = glob.glob(os.path.join("data", "*"))
+    assert len(folders) == 1
+    files = set(os.listdir(folders[0]))
+    assert files == {f"{i}.npy" for i in range(n_step)} | {"time.npy"}
+    np.testing.assert_array_equal(np.load(os.path.join(folders[0], "time.npy")), ts)
+    np.testing.assert_array_equal(np.load(os.path.join(folders[0], "0.npy")), problems.twin_quadrants(n))
+    last = np.load(os.path.join(folders[0], f"{n_step - 1}.npy"))
+    np.testing.assert_array_equal(last, q)
+    np.testing.assert_array_equal(last[..., :4], ref[..., :4])

@@ -158,3 +158,24 @@ def test_objective_without_shock_and_weights():
     # G = 0 everywhere, so S, band and the skeleton are empty
     assert m["I_curve"] == 0 and m["E_perp"] == 0 and m["O"] == 0 and m["TV"] == 0
     assert J == 0.0
+
+
+def test_fallback_search_finds_the_minimum_of_a_known_function():
+    """The built-in Bayesian search (used when scikit-optimize is absent) on
+    the reference's box and constraint: a smooth bowl with its minimum at
+    (0.3, 4.0) and the 1e6 penalty for jp_lo >= jp_hi; 30 evaluations get
+    within 0.02 of the minimum value, deterministic for a seed."""
+    from wenox import tuning
+
+    def f(x):
+        if x[0] >= x[1]:
+            return tuning.PENALTY
+        return (x[0] - 0.3) ** 2 + 0.01 * (x[1] - 4.0) ** 2
+
+    x1, f1, xs, fs = tuning.gp_ei_minimize(f, tuning.SPACE, 30, 8, 0)
+    x2, f2, _, _ = tuning.gp_ei_minimize(f, tuning.SPACE, 30, 8, 0)
+    assert len(xs) == 30 and f1 == min(fs)
+    assert f1 == f2 and np.array_equal(x1, x2)
+    assert f1 < 0.02
+    for (lo, hi), v in zip(tuning.SPACE, x1):
+        assert lo <= v <= hi
