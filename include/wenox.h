@@ -15,7 +15,7 @@
  *  2. Device-resident solver.  Holds the padded conserved state in HBM and
  *     advances it with the reference's SSP-RK3 + ghost fill + CFL loop
  *     (Rie_prob_2d_rs.py:169-207,311-345; double_mach_reflection.py:97-134,
- *     229-260), one hipGraph replay per time step, dt computed on the device.
+ *     229-260), hipGraph replays of one or eight time steps, dt computed on the device.
  *
  * Every function returns WENOX_OK (0) on success, a negative value for an
  * argument or shape error (the cases where the reference raises), or a

@@ -482,7 +482,9 @@ void drop_graphs(wenox_solver* s)
 // FP64 view of work with the state geometry (the split download's staging copy)
 double* work_state(const wenox_solver* s) { return s->work + STATE_SHIFT; }
 
-// One time step on the solver's stream: CFL -> 3 x (RHS, RK combine, ghost fill) -> advance.
+// One time step on the solver's stream: CFL -> 3 x (RHS, RK combine, ghost fill) -> advance
+// (fused FP64 engine: dt from the previous step's tile maxima, the advance
+// applied at the next step's start or by a settle launch).
 void enqueue_step(wenox_solver* s)
 {
     const auto& d = s->d;

@@ -1,8 +1,8 @@
 """wenox: MI355X-native (gfx950) 2D WENO5 / HLLC Euler hot path.
 
 ``Solver`` keeps the padded conserved state in HBM and advances it with the
-reference's SSP-RK3 + ghost fill + CFL time loop, one hipGraph replay per
-step (``Rie_prob_2d_rs.py:169-207,311-345``; ``double_mach_reflection.py:
+reference's SSP-RK3 + ghost fill + CFL time loop in hipGraph replays of one or
+eight steps (``Rie_prob_2d_rs.py:169-207,311-345``; ``double_mach_reflection.py:
 97-134,229-260``).  The drop-in module ``weno_ext`` (sibling package) keeps
 the reference's per-call surface.
 """
