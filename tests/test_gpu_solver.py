@@ -177,6 +177,40 @@ def test_sweep_tilings_bitwise(oracle_mod, cons, force_hlle):
                 np.testing.assert_array_equal(s.rhs(), ref, err_msg=f"tiling {(cx, cy, wx, wy)}")
 
 
+@pytest.mark.parametrize("cons,force_hlle", [(False, False), (True, False), (False, True)])
+def test_fast_step_same_for_every_tiling(oracle_mod, cons, force_hlle):
+    """The fast FP64 step lets the x sweep form its part of each Shu-Osher
+    stage, a Un + b Us - (c dt) Dx, inside the march or -- for the cells next
+    to the faces a workgroup's wavefronts share -- after it, from reloaded
+    states.  Its state after 6 steps is the same bit for bit for every tiling
+    (x tilings with 1, 2 and 4 wavefronts, with both merged and separate
+    launches), and within 1e-12 relative L1 of the oracle's time loop; a
+    wrong coefficient or a cell finished with another cell's state would
+    give O(dt) differences."""
+    from wenox import Solver, problems
+    nx, ny = 45, 37
+    q0, dx = problems.four_quadrant(nx, ny=ny)
+    ref, ts = oracle_mod.run_riemann(q0.copy(), nx, ny, dx, 1.4, 0.4, force_hlle, cons, (0.5, 2.0), nstep=6)
+    assert np.isfinite(ref).all()
+    print(f"[solver] fast step, tilings 45x37 cons={cons} hlle={force_hlle}", flush=True)
+    first = None
+    for wx, cx in ((1, 8), (2, 3), (2, 7), (4, 3), (4, 4), (4, 11)):
+        for wy, cy in ((1, 5), (4, 3)):
+            with Solver(nx, ny, dx, cfl=0.4, cons_switch=cons, force_hlle=force_hlle, jp_cri=(0.5, 2.0),
+                        engine="fused", variant="fast") as s:
+                s.set_tiling(cx, cy, wx, wy)
+                s.upload(q0)
+                T, k = s.run(nstep=6)
+                got = s.download()[..., :4]
+            assert k == 6 and T == pytest.approx(ts[6], rel=1e-12)
+            if first is None:
+                first = got
+                err = rel_l1(got, ref, nx, ny)
+                assert max(err) <= 1e-12, err
+            else:
+                np.testing.assert_array_equal(got, first, err_msg=f"tiling {(cx, cy, wx, wy)}")
+
+
 def test_tiling_arguments_checked():
     from wenox import Solver, problems
     from wenox._lib import WenoxError

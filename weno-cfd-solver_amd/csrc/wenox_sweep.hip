@@ -33,7 +33,10 @@
 //
 // k_combine applies the SSP-RK3 stage combination with L = -(Dx + Dy^T)
 // (riemann.rs:536; Rie_prob_2d_rs.py:183-202) through an LDS tile transpose,
-// writing the next state in both orientations.
+// writing the next state in both orientations.  The exact build keeps the
+// reference's operation order; the fast build's FP64 step lets the x sweep
+// form a Un + b Us - (c dt) Dx in place of Dx (FOLD), so the combine reads
+// two arrays instead of four.
 #include "wenox_numerics.h"
 #include "wenox_ops.h"
 #include "wenox_sweep.h"
@@ -70,7 +73,18 @@ struct SweepArgs {
     void* D;              // interior planes nn x nt (DT), row pitch ldd
     int* flags;           // one per wavefront: DETECT records whether it recomputed a face row in FP64
     const StepCtl* ctl;
+    const void* Un = nullptr;   // step-start state, planes like U (FOLD)
+    int stage = 0;              // RK stage 1..3 (FOLD)
 };
+
+// Shu-Osher coefficients of stage s: U_{s+1} = a Un + b Us + (c dt) L
+// (Rie_prob_2d_rs.py:183-202; in stage 1 Us = Un, so a = 0, b = 1).
+__host__ __device__ inline void stage_coeffs(int stage, double& a, double& b, double& c)
+{
+    if (stage == 1) { a = 0.0; b = 1.0; c = 1.0; }
+    else if (stage == 2) { a = 0.75; b = 0.25; c = 0.25; }
+    else { a = 1.0 / 3.0; b = 2.0 / 3.0; c = 2.0 / 3.0; }
+}
 
 template <typename T, int IN>
 __device__ __forceinline__ T ld_state(const void* U, int64_t P, int k, int64_t idx)
@@ -173,10 +187,18 @@ __host__ __device__ inline int64_t wave_start(int W, int chunk, bool blend, int 
 // the blend); group_cells gives the count.  The waves of a group run on
 // separate SIMDs of one CU, and equal marches keep each of them busy until
 // the group's end.  Needs chunk >= 3 when W > 1 (the host enforces it).
-template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND, int W>
+//
+// FOLD (x sweep of the fast FP64 step, which needs A.ctl for dt): D receives,
+// in place of Dx, the part of the stage's combination this sweep can form,
+// E = a Un + b Us - (c dt) Dx (stage_coeffs), so the combine reads E and
+// Dy^T instead of Dx, Dy^T, Un and Us.  Us of a cell finished inside the
+// march is a row of the stencil window and Un is fetched one face ahead; the
+// cells finished after the march reload both.
+template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND, int W, bool FOLD = false>
 __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64_t by, int64_t bid,
                                            SweepLds<T, W>& S)
 {
+    static_assert(!FOLD || (IN == IN_F64 && MODE == MODE_PLAIN && DIR == DIR_X), "fold: FP64 x sweep only");
     if (A.ctl && !A.ctl->active) return;
     auto& s_p1 = S.p1;
     auto& s_pb = S.pb;
@@ -227,21 +249,45 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
 #pragma unroll
     for (int k = 0; k < 4; ++k) { fc_prev[k] = T(0); fe_prev[k] = T(0); F_prev[k] = T(0); }
 
-    auto put_d = [&](int64_t cell, const T Fr[4], const T Fl[4]) {
+    double fa_c = 0.0, fb_c = 0.0, fc_c = 0.0;   // FOLD: a, b and c dt of the stage
+    if constexpr (FOLD) {
+        stage_coeffs(A.stage, fa_c, fb_c, fc_c);
+        fc_c *= A.ctl->dt;
+    }
+    const bool fold_un = FOLD && A.stage > 1;   // stage 1: a = 0, Un unread
+    // FOLD: Us and Un of interior cell `cell` from memory
+    auto load_cell = [&](int64_t cell, T us[4], T un[4]) {
+        if constexpr (FOLD) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                us[k] = ld_state<T, IN>(A.U, P, k, (cell + 3) * A.ld + pcl);
+                un[k] = fold_un ? ld_state<T, IN>(A.Un, P, k, (cell + 3) * A.ld + pcl) : T(0);
+            }
+        }
+    };
+    // cell's flux difference; with FOLD its part of the stage combination
+    // (us, un: the cell's Us and Un), in explicit fused operations so that
+    // every cell rounds alike wherever the march finishes it
+    auto put_d = [&](int64_t cell, const T Fr[4], const T Fl[4], const T* us, const T* un) {
         if (out_lane) {
             DT* d = Dout + cell * A.ldd + t;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) d[k * Pd] = (DT)WX_DIV_DX(Fr[k] - Fl[k]);
+            for (int k = 0; k < 4; ++k) {
+                const T dv = WX_DIV_DX(Fr[k] - Fl[k]);
+                if constexpr (!FOLD) d[k * Pd] = (DT)dv;
+                else d[k * Pd] = (DT)fma(-T(fc_c), dv, fma(T(fa_c), un[k], T(fb_c) * us[k]));
+            }
         }
     };
     // face f finished with flux F: cell f-1 lies between faces f-1 and f
-    auto finish_face = [&](int64_t f, const T F[4]) {
+    // (us, un: that cell's Us and Un when it is finished here)
+    auto finish_face = [&](int64_t f, const T F[4], const T* us, const T* un) {
         if (f - 1 >= r0 && f - 1 < r1) {
             if (dfirst && f - 1 == r0) {
 #pragma unroll
                 for (int k = 0; k < 4; ++k) s_xf[wv][8 + k][l] = F[k];
             } else {
-                put_d(f - 1, F, F_prev);
+                put_d(f - 1, F, F_prev, us, un);
             }
         }
 #pragma unroll
@@ -266,6 +312,17 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
             for (int k = 0; k < 4; ++k) w[s + 1][k] = ld_state<T, IN>(A.U, P, k, (fa + s) * A.ld + pcl);
 #pragma unroll
         for (int k = 0; k < 4; ++k) nxt[k] = ld_state<T, IN>(A.U, P, k, (fa + 5) * A.ld + pcl);
+        // FOLD: Un of the cell finished at face a, a - 2 with the blend (one
+        // face behind) and a - 1 without, fetched one face ahead like the
+        // window's next row (measured 0.4 % faster than at the top of the same
+        // face); rows fa - 2 .. fb - 1 lie inside the padded planes
+        constexpr int FIN = BLEND ? 2 : 1;
+        T un_cur[4], un_nxt[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            un_cur[k] = T(0);
+            un_nxt[k] = fold_un ? ld_state<T, IN>(A.Un, P, k, (fa - FIN + 3) * A.ld + pcl) : T(0);
+        }
 
         for (int64_t a = fa; a <= fb; ++a) {
             // slide the window; row a+5 was fetched one face ahead, fetch row a+6
@@ -278,6 +335,15 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
             if (a < fb) {
 #pragma unroll
                 for (int k = 0; k < 4; ++k) nxt[k] = ld_state<T, IN>(A.U, P, k, (a + 6) * A.ld + pcl);
+            }
+            if (fold_un) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) un_cur[k] = un_nxt[k];
+                if (a < fb) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        un_nxt[k] = ld_state<T, IN>(A.Un, P, k, (a + 1 - FIN + 3) * A.ld + pcl);
+                }
             }
 
             // pass 1 (normal reconstruction) on every lane
@@ -363,7 +429,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
 #pragma unroll
                     for (int k = 0; k < 4; ++k) s_xF[wv][k][l] = fe[k];
                 }
-                finish_face(a, fe);
+                finish_face(a, fe, w[2], un_cur);
             } else {
                 s_pb[wv][a & 1][l] = pb;
                 if (dfirst && a == r0) {
@@ -382,7 +448,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                         T F[4];
 #pragma unroll
                         for (int k = 0; k < 4; ++k) F[k] = phi * fc_prev[k] + (T(1) - phi) * fe_prev[k];
-                        if (f >= r0) finish_face(f, F);
+                        if (f >= r0) finish_face(f, F, w[1], un_cur);
                     }
                 }
                 pb_m2 = pb_m1;
@@ -401,12 +467,22 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
             T F[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) F[k] = phi * fc_prev[k] + (T(1) - phi) * fe_prev[k];
-            finish_face(f, F);
+            T us[4], un[4];
+            load_cell(f - 1, us, un);
+            finish_face(f, F, us, un);
         }
         if (BLEND && !last) s_xpb[wv][1][l] = pb_m1;     // p̄ of face r1 - 1
     }
 
     if constexpr (W > 1) {
+        // FOLD: Us and Un of the cells finished below, fetched before the
+        // barrier so that the wait covers their latency
+        T us0[4], un0[4], us1[4], un1[4], us2[4], un2[4];
+        if (!empty) {
+            if (dfirst) load_cell(r0, us0, un0);
+            if (BLEND && !last) load_cell(r1 - 2, us1, un1);
+            if (!last) load_cell(r1 - 1, us2, un2);
+        }
         __syncthreads();
         T FLast[4];
         if (!empty && BLEND) {
@@ -422,7 +498,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                     F1[k] = s_xf[wv][8 + k][l];
                     s_xF[wv][k][l] = F0[k];
                 }
-                put_d(r0, F1, F0);
+                put_d(r0, F1, F0, us0, un0);
             }
             if (!last) {
                 // face r1 - 1: p̄(r1 - 2) below, next wave's first p̄ above
@@ -430,7 +506,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                 T phi = sensor(s_pb[wv][(r1 - 1) & 1], dn, pb_m1);
 #pragma unroll
                 for (int k = 0; k < 4; ++k) FLast[k] = phi * fc_prev[k] + (T(1) - phi) * fe_prev[k];
-                put_d(r1 - 2, FLast, F_prev);
+                put_d(r1 - 2, FLast, F_prev, us1, un1);
             }
         } else {
 #pragma unroll
@@ -442,7 +518,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
             T Fn[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) Fn[k] = s_xF[wv + 1][k][l];
-            put_d(r1 - 1, Fn, FLast);
+            put_d(r1 - 1, Fn, FLast, us2, un2);
         }
     }
     if (MODE == MODE_DETECT) {
@@ -455,27 +531,28 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
 
 // Two wavefronts per SIMD (at most 256 VGPRs): the sweeps are VALU-bound and
 // need the second wave to cover LDS and memory latency.
-template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND, int W>
+template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND, int W, bool FOLD = false>
 __global__ void __launch_bounds__(WAVE * W) __attribute__((amdgpu_waves_per_eu(2))) k_sweep(SweepArgs A)
 {
     __shared__ SweepLds<T, W> S;
-    sweep_body<T, IN, DT, MODE, DIR, CONS, BLEND, W>(A, blockIdx.x, blockIdx.y,
-                                                     (int64_t)blockIdx.y * gridDim.x + blockIdx.x, S);
+    sweep_body<T, IN, DT, MODE, DIR, CONS, BLEND, W, FOLD>(A, blockIdx.x, blockIdx.y,
+                                                           (int64_t)blockIdx.y * gridDim.x + blockIdx.x, S);
 }
 
 // Both sweeps of a stage in one launch: workgroups [0, nbx) run the x sweep,
 // the rest the y sweep (they read different planes and write different D
 // arrays).  The two halves fill each other's tail, which on grids of about
 // one wave round per sweep is most of the gain.  Flag slots follow the
-// workgroup number, so the x and y halves use disjoint ranges.
-template <typename T, int IN, typename DT, int MODE, bool CONS, bool BLEND, int W>
+// workgroup number, so the x and y halves use disjoint ranges.  FOLD applies
+// to the x half.
+template <typename T, int IN, typename DT, int MODE, bool CONS, bool BLEND, int W, bool FOLD = false>
 __global__ void __launch_bounds__(WAVE * W) __attribute__((amdgpu_waves_per_eu(2)))
 k_sweep_xy(SweepArgs Ax, SweepArgs Ay, unsigned nbx, unsigned colsx, unsigned colsy)
 {
     __shared__ SweepLds<T, W> S;
     const unsigned b = blockIdx.x;
     if (b < nbx) {
-        sweep_body<T, IN, DT, MODE, DIR_X, CONS, BLEND, W>(Ax, b % colsx, b / colsx, b, S);
+        sweep_body<T, IN, DT, MODE, DIR_X, CONS, BLEND, W, FOLD>(Ax, b % colsx, b / colsx, b, S);
     } else {
         const unsigned c = b - nbx;
         sweep_body<T, IN, DT, MODE, DIR_Y, CONS, BLEND, W>(Ay, c % colsy, c / colsy, b, S);
@@ -500,7 +577,9 @@ constexpr int TT = 32;
 // next step's wave speed c + |V| over the tile's new cells into ctl->part[t]
 // (t the tile's index among ntiles) and mark the step's advance pending; the
 // next step's first launch finishes both.
-template <class St, typename DT, int TJ, bool CFL = false>
+// FOLD: Dx holds the x sweep's part E of the combination (the sweep's FOLD);
+// the new state is E - (c dt) Dy.
+template <class St, typename DT, int TJ, bool CFL = false, bool FOLD = false>
 __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT* __restrict__ DyT, St Un,
                                              St Us, St out, St outT, int64_t nx, int64_t ny, int64_t ld,
                                              int64_t ldT, int64_t ldx, int64_t ldyT, int stage,
@@ -514,6 +593,9 @@ __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT
     const int64_t i0 = ti * TT, j0 = tj * TJ;
     const int64_t PX = nx * ldx, PY = ny * ldyT;
     const double dt = ctl ? ctl->dt : 0.0;
+    double fa_c, fb_c, fc_c;   // FOLD: c dt of the stage
+    stage_coeffs(stage, fa_c, fb_c, fc_c);
+    fc_c *= dt;
     // Dy^T[k][j][i]: rows j, contiguous i (32 threads per row)
     {
         const int c = threadIdx.x & (TT - 1), r0 = threadIdx.x / TT;
@@ -536,12 +618,16 @@ __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT
                 const int64_t o = (i + 3) * ld + (j + 3);
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    const double l = -((double)Dx[k * PX + i * ldx + j] + tile[k][tx][r]);
-                    const double qn = Un.ld(k, o);
                     double v;
-                    if (stage == 1) v = qn + dt * l;
-                    else if (stage == 2) v = 0.75 * qn + 0.25 * Us.ld(k, o) + (0.25 * dt) * l;
-                    else v = (1.0 / 3.0) * qn + (2.0 / 3.0) * Us.ld(k, o) + ((2.0 / 3.0) * dt) * l;
+                    if constexpr (!FOLD) {
+                        const double l = -((double)Dx[k * PX + i * ldx + j] + tile[k][tx][r]);
+                        const double qn = Un.ld(k, o);
+                        if (stage == 1) v = qn + dt * l;
+                        else if (stage == 2) v = 0.75 * qn + 0.25 * Us.ld(k, o) + (0.25 * dt) * l;
+                        else v = (1.0 / 3.0) * qn + (2.0 / 3.0) * Us.ld(k, o) + ((2.0 / 3.0) * dt) * l;
+                    } else {
+                        v = fma(-fc_c, tile[k][tx][r], (double)Dx[k * PX + i * ldx + j]);
+                    }
                     out.st(k, o, v);
                     tile[k][tx][r] = v;
                 }
@@ -596,7 +682,7 @@ __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT
 // into 8 contiguous bands (the first ntiles % 8 of them one tile longer): each
 // XCD's L2 then holds a contiguous band, and the cache lines two neighbouring
 // tiles share (rows are 32 doubles, 3 off a line boundary) are fetched once.
-template <class St, typename DT, int TJ, bool CFL = false>
+template <class St, typename DT, int TJ, bool CFL = false, bool FOLD = false>
 __global__ void __launch_bounds__(256) k_combine(const DT* __restrict__ Dx, const DT* __restrict__ DyT,
                                                  St Un, St Us, St out, St outT, int64_t nx, int64_t ny,
                                                  int64_t ld, int64_t ldT, int64_t ldx, int64_t ldyT,
@@ -606,34 +692,36 @@ __global__ void __launch_bounds__(256) k_combine(const DT* __restrict__ Dx, cons
     const int64_t b = (int64_t)blockIdx.y * tiles_j + blockIdx.x;
     const int64_t q = ntiles / 8, r = ntiles % 8, x = b % 8, k = b / 8;
     const int64_t t = x * q + (x < r ? x : r) + k;
-    combine_tile<St, DT, TJ, CFL>(Dx, DyT, Un, Us, out, outT, nx, ny, ld, ldT, ldx, ldyT, stage, ctl,
-                                  t / tiles_j, t % tiles_j, gam, t, ntiles);
+    combine_tile<St, DT, TJ, CFL, FOLD>(Dx, DyT, Un, Us, out, outT, nx, ny, ld, ldT, ldx, ldyT, stage, ctl,
+                                        t / tiles_j, t % tiles_j, gam, t, ntiles);
 }
 
 // cfl: the step's last combine also leaves the next step's CFL maxima (one
 // per tile, combine_tiles(p) of them) in ctl->part and the advance pending.
-template <class St, typename DT>
+// FOLD: Dx holds the x sweep's part of the combination (the sweep's FOLD).
+template <class St, typename DT, int TJ, bool CFL, bool FOLD>
+static void launch_combine_t(const SweepPlan& p, const DT* Dx, const DT* DyT, St Un, St Us, St out, St outT,
+                             int stage, const StepCtl* ctl, hipStream_t s)
+{
+    const dim3 g((unsigned)((p.ny + TJ - 1) / TJ), (unsigned)((p.nx + TT - 1) / TT));
+    k_combine<St, DT, TJ, CFL, FOLD><<<g, 256, 0, s>>>(Dx, DyT, Un, Us, out, outT, p.nx, p.ny, p.ld, p.ldT, p.ldx,
+                                                      p.ldyT, stage, ctl, p.gamma);
+}
+
+template <class St, typename DT, int TJ, bool FOLD>
+static void launch_combine_f(const SweepPlan& p, const DT* Dx, const DT* DyT, St Un, St Us, St out, St outT,
+                             int stage, const StepCtl* ctl, hipStream_t s, bool cfl)
+{
+    if (cfl) launch_combine_t<St, DT, TJ, true, FOLD>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s);
+    else launch_combine_t<St, DT, TJ, false, FOLD>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s);
+}
+
+template <class St, typename DT, bool FOLD = false>
 static void launch_combine(const SweepPlan& p, const DT* Dx, const DT* DyT, St Un, St Us, St out, St outT,
                            int stage, const StepCtl* ctl, hipStream_t s, bool cfl = false)
 {
-    const unsigned ti = (unsigned)((p.nx + TT - 1) / TT);
-    if (p.combine_cols == 64) {
-        const dim3 g((unsigned)((p.ny + 63) / 64), ti);
-        if (cfl)
-            k_combine<St, DT, 64, true><<<g, 256, 0, s>>>(Dx, DyT, Un, Us, out, outT, p.nx, p.ny, p.ld, p.ldT,
-                                                           p.ldx, p.ldyT, stage, ctl, p.gamma);
-        else
-            k_combine<St, DT, 64><<<g, 256, 0, s>>>(Dx, DyT, Un, Us, out, outT, p.nx, p.ny, p.ld, p.ldT, p.ldx,
-                                                     p.ldyT, stage, ctl, p.gamma);
-    } else {
-        const dim3 g((unsigned)((p.ny + 31) / 32), ti);
-        if (cfl)
-            k_combine<St, DT, 32, true><<<g, 256, 0, s>>>(Dx, DyT, Un, Us, out, outT, p.nx, p.ny, p.ld, p.ldT,
-                                                           p.ldx, p.ldyT, stage, ctl, p.gamma);
-        else
-            k_combine<St, DT, 32><<<g, 256, 0, s>>>(Dx, DyT, Un, Us, out, outT, p.nx, p.ny, p.ld, p.ldT, p.ldx,
-                                                     p.ldyT, stage, ctl, p.gamma);
-    }
+    if (p.combine_cols == 64) launch_combine_f<St, DT, 64, FOLD>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s, cfl);
+    else launch_combine_f<St, DT, 32, FOLD>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s, cfl);
 }
 
 // Plain RHS from the two sweeps (for tests): L = -(Dx + Dy^T), SoA4 interior
@@ -674,29 +762,29 @@ __global__ void __launch_bounds__(256) k_transpose(const E* in, E* out, int64_t 
     }
 }
 
-template <typename T, int IN, typename DT, int MODE, int DIR, int W>
+template <typename T, int IN, typename DT, int MODE, int DIR, int W, bool FOLD = false>
 static void launch_sweep_w(const SweepArgs& a, int cons, int blend, hipStream_t s)
 {
     const int64_t rows = group_cells(W, a.chunk, blend != 0);
     dim3 grid((unsigned)((a.nt + OUT_COLS - 1) / OUT_COLS), (unsigned)((a.nn + rows - 1) / rows));
     if (cons) {
-        if (blend) k_sweep<T, IN, DT, MODE, DIR, true, true, W><<<grid, WAVE * W, 0, s>>>(a);
-        else k_sweep<T, IN, DT, MODE, DIR, true, false, W><<<grid, WAVE * W, 0, s>>>(a);
+        if (blend) k_sweep<T, IN, DT, MODE, DIR, true, true, W, FOLD><<<grid, WAVE * W, 0, s>>>(a);
+        else k_sweep<T, IN, DT, MODE, DIR, true, false, W, FOLD><<<grid, WAVE * W, 0, s>>>(a);
     } else {
-        if (blend) k_sweep<T, IN, DT, MODE, DIR, false, true, W><<<grid, WAVE * W, 0, s>>>(a);
-        else k_sweep<T, IN, DT, MODE, DIR, false, false, W><<<grid, WAVE * W, 0, s>>>(a);
+        if (blend) k_sweep<T, IN, DT, MODE, DIR, false, true, W, FOLD><<<grid, WAVE * W, 0, s>>>(a);
+        else k_sweep<T, IN, DT, MODE, DIR, false, false, W, FOLD><<<grid, WAVE * W, 0, s>>>(a);
     }
 }
 
-template <typename T, int IN, typename DT, int MODE, int DIR>
+template <typename T, int IN, typename DT, int MODE, int DIR, bool FOLD = false>
 static void launch_sweep(const SweepArgs& a, int cons, int blend, hipStream_t s)
 {
-    if (a.waves == 4) launch_sweep_w<T, IN, DT, MODE, DIR, 4>(a, cons, blend, s);
-    else if (a.waves == 2) launch_sweep_w<T, IN, DT, MODE, DIR, 2>(a, cons, blend, s);
-    else launch_sweep_w<T, IN, DT, MODE, DIR, 1>(a, cons, blend, s);
+    if (a.waves == 4) launch_sweep_w<T, IN, DT, MODE, DIR, 4, FOLD>(a, cons, blend, s);
+    else if (a.waves == 2) launch_sweep_w<T, IN, DT, MODE, DIR, 2, FOLD>(a, cons, blend, s);
+    else launch_sweep_w<T, IN, DT, MODE, DIR, 1, FOLD>(a, cons, blend, s);
 }
 
-template <typename T, int IN, typename DT, int MODE, int W>
+template <typename T, int IN, typename DT, int MODE, int W, bool FOLD = false>
 static void launch_sweep_xy_w(const SweepArgs& ax, const SweepArgs& ay, int cons, int blend, hipStream_t s)
 {
     const unsigned colsx = (unsigned)((ax.nt + OUT_COLS - 1) / OUT_COLS);
@@ -706,28 +794,32 @@ static void launch_sweep_xy_w(const SweepArgs& ax, const SweepArgs& ay, int cons
     const unsigned nby = colsy * (unsigned)((ay.nn + ry - 1) / ry);
     const dim3 grid(nbx + nby);
     if (cons) {
-        if (blend) k_sweep_xy<T, IN, DT, MODE, true, true, W><<<grid, WAVE * W, 0, s>>>(ax, ay, nbx, colsx, colsy);
-        else k_sweep_xy<T, IN, DT, MODE, true, false, W><<<grid, WAVE * W, 0, s>>>(ax, ay, nbx, colsx, colsy);
+        if (blend)
+            k_sweep_xy<T, IN, DT, MODE, true, true, W, FOLD><<<grid, WAVE * W, 0, s>>>(ax, ay, nbx, colsx, colsy);
+        else
+            k_sweep_xy<T, IN, DT, MODE, true, false, W, FOLD><<<grid, WAVE * W, 0, s>>>(ax, ay, nbx, colsx, colsy);
     } else {
-        if (blend) k_sweep_xy<T, IN, DT, MODE, false, true, W><<<grid, WAVE * W, 0, s>>>(ax, ay, nbx, colsx, colsy);
-        else k_sweep_xy<T, IN, DT, MODE, false, false, W><<<grid, WAVE * W, 0, s>>>(ax, ay, nbx, colsx, colsy);
+        if (blend)
+            k_sweep_xy<T, IN, DT, MODE, false, true, W, FOLD><<<grid, WAVE * W, 0, s>>>(ax, ay, nbx, colsx, colsy);
+        else
+            k_sweep_xy<T, IN, DT, MODE, false, false, W, FOLD><<<grid, WAVE * W, 0, s>>>(ax, ay, nbx, colsx, colsy);
     }
 }
 
 // x then y sweep of one stage: one launch when both use the same number of
-// wavefronts per workgroup, else two.
-template <typename T, int IN, typename DT, int MODE>
+// wavefronts per workgroup, else two.  FOLD applies to the x sweep.
+template <typename T, int IN, typename DT, int MODE, bool FOLD = false>
 static void launch_sweeps(const SweepArgs& ax, const SweepArgs& ay, int cons, int blend, hipStream_t s)
 {
     if (ax.waves != ay.waves) {
-        launch_sweep<T, IN, DT, MODE, DIR_X>(ax, cons, blend, s);
+        launch_sweep<T, IN, DT, MODE, DIR_X, FOLD>(ax, cons, blend, s);
         launch_sweep<T, IN, DT, MODE, DIR_Y>(ay, cons, blend, s);
     } else if (ax.waves == 4) {
-        launch_sweep_xy_w<T, IN, DT, MODE, 4>(ax, ay, cons, blend, s);
+        launch_sweep_xy_w<T, IN, DT, MODE, 4, FOLD>(ax, ay, cons, blend, s);
     } else if (ax.waves == 2) {
-        launch_sweep_xy_w<T, IN, DT, MODE, 2>(ax, ay, cons, blend, s);
+        launch_sweep_xy_w<T, IN, DT, MODE, 2, FOLD>(ax, ay, cons, blend, s);
     } else {
-        launch_sweep_xy_w<T, IN, DT, MODE, 1>(ax, ay, cons, blend, s);
+        launch_sweep_xy_w<T, IN, DT, MODE, 1, FOLD>(ax, ay, cons, blend, s);
     }
 }
 
@@ -752,10 +844,35 @@ static void h_sweeps(const SweepPlan& p, const double* Us, const double* UsT, do
                                                       args_y(p, UsT, DyT, nullptr, ctl), p.cons, blend, s);
 }
 
+#if WX_FAST
+// Both sweeps of a stage with the x sweep's fold (needs ctl for dt), then the
+// combine that completes the stage.
+static void h_stage_fold(const SweepPlan& p, const double* Us, const double* UsT, const double* Un, double* Dx,
+                         double* DyT, double* out, double* outT, int stage, const StepCtl* ctl, hipStream_t s,
+                         bool sweeps, bool combine)
+{
+    const int blend = p.force_hlle ? 0 : 1;
+    const int64_t P = p.P();
+    SweepArgs ax = args_x(p, Us, Dx, nullptr, ctl);
+    ax.Un = Un;
+    ax.stage = stage;
+    const SweepArgs ay = args_y(p, UsT, DyT, nullptr, ctl);
+    const StF64 un{const_cast<double*>(Un), P}, us{const_cast<double*>(Us), P}, o{out, P}, oT{outT, p.PT()};
+    if (sweeps) launch_sweeps<double, IN_F64, double, MODE_PLAIN, true>(ax, ay, p.cons, blend, s);
+    if (combine) launch_combine<StF64, double, true>(p, Dx, DyT, un, us, o, oT, stage, ctl, s, stage == 3);
+}
+#endif
+
 static void h_stage(const SweepPlan& p, const double* Us, const double* UsT, const double* Un,
                     double* Dx, double* DyT, double* out, double* outT, int stage, const StepCtl* ctl,
                     hipStream_t s)
 {
+#if WX_FAST
+    if (ctl) {
+        h_stage_fold(p, Us, UsT, Un, Dx, DyT, out, outT, stage, ctl, s, true, true);
+        return;
+    }
+#endif
     h_sweeps(p, Us, UsT, Dx, DyT, ctl, s);
     const int64_t P = p.P();
     launch_combine<StF64, double>(p, Dx, DyT, StF64{const_cast<double*>(Un), P}, StF64{const_cast<double*>(Us), P},
@@ -807,6 +924,12 @@ static void h_sweep2(const SweepPlan& p, int split, const void* U, const void* U
         launch_sweeps<double, IN_SPLIT, float, MODE_DETECT>(ax, ay, p.cons, blend, s);
         return;
     }
+    if (ctl) {
+        // as a stage-2 step launches them (U standing in for Un)
+        h_stage_fold(p, (const double*)U, (const double*)UT, (const double*)U, (double*)Dx, (double*)DyT, nullptr,
+                     nullptr, 2, ctl, s, true, false);
+        return;
+    }
 #endif
     launch_sweeps<double, IN_F64, double, MODE_PLAIN>(args_x(p, U, Dx, nullptr, ctl), args_y(p, UT, DyT, nullptr, ctl),
                                                       p.cons, blend, s);
@@ -821,6 +944,11 @@ static void h_combine1(const SweepPlan& p, int split, const void* Dx, const void
         launch_combine<StSplit, float>(p, (const float*)Dx, (const float*)DyT, StSplit{(float*)Un, P},
                                        StSplit{(float*)Us, P}, StSplit{(float*)out, P}, StSplit{(float*)outT, PT},
                                        2, ctl, s);
+        return;
+    }
+    if (ctl) {
+        h_stage_fold(p, (const double*)Us, nullptr, (const double*)Un, (double*)Dx, (double*)DyT, (double*)out,
+                     (double*)outT, 2, ctl, s, false, true);
         return;
     }
 #endif
@@ -919,9 +1047,10 @@ template <> __device__ __forceinline__ StSplit make_st<StSplit>(void* base, int6
     return StSplit{(float*)base + 2 * soff, P};
 }
 
-template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND>
+template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND, bool FOLD = false>
 __global__ void __launch_bounds__(WAVE) k_sweep_batch(BatchTables B, BatchCommon C, const void* Ubase,
-                                                      void* Dbase, int* flags, StepCtl* ctls)
+                                                      void* Dbase, int* flags, StepCtl* ctls, const void* Unbase,
+                                                      int stage)
 {
     const int b = blockIdx.x;
     const int i = (DIR == DIR_X ? B.blk_x : B.blk_y)[b];
@@ -944,13 +1073,15 @@ __global__ void __launch_bounds__(WAVE) k_sweep_batch(BatchTables B, BatchCommon
     A.D = (DT*)Dbase + I.ioff;
     A.flags = flags;
     A.ctl = &ctls[i];
+    A.Un = FOLD ? (const char*)Unbase + I.soff * 8 : nullptr;
+    A.stage = stage;
     const int local = b - (DIR == DIR_X ? I.blk0_x : I.blk0_y);
     const int gx = DIR == DIR_X ? I.gx_x : I.gx_y;
     __shared__ SweepLds<T, 1> S;
-    sweep_body<T, IN, DT, MODE, DIR, CONS, BLEND, 1>(A, local % gx, local / gx, b, S);
+    sweep_body<T, IN, DT, MODE, DIR, CONS, BLEND, 1, FOLD>(A, local % gx, local / gx, b, S);
 }
 
-template <class St, typename DT>
+template <class St, typename DT, bool FOLD = false>
 __global__ void __launch_bounds__(256) k_combine_batch(BatchTables B, const DT* Dx, const DT* DyT, void* Un,
                                                        void* Us, void* out, void* outT, int stage, StepCtl* ctls)
 {
@@ -959,9 +1090,10 @@ __global__ void __launch_bounds__(256) k_combine_batch(BatchTables B, const DT* 
     const BatchInst I = B.inst[i];
     const int64_t P = (I.nx + 6) * (I.ny + 6);
     const int local = b - I.tile0;
-    combine_tile<St, DT, TT>(Dx + I.ioff, DyT + I.ioff, make_st<St>(Un, I.soff, P), make_st<St>(Us, I.soff, P),
-                             make_st<St>(out, I.soff, P), make_st<St>(outT, I.soff, P), I.nx, I.ny, I.ny + 6,
-                             I.nx + 6, I.ny, I.nx, stage, &ctls[i], local / I.tiles_j, local % I.tiles_j);
+    combine_tile<St, DT, TT, false, FOLD>(Dx + I.ioff, DyT + I.ioff, make_st<St>(Un, I.soff, P),
+                                          make_st<St>(Us, I.soff, P), make_st<St>(out, I.soff, P),
+                                          make_st<St>(outT, I.soff, P), I.nx, I.ny, I.ny + 6, I.nx + 6, I.ny, I.nx,
+                                          stage, &ctls[i], local / I.tiles_j, local % I.tiles_j);
 }
 
 template <class St>
@@ -994,19 +1126,26 @@ __global__ void k_advance_batch(StepCtl* ctls, int n)
     }
 }
 
-template <typename T, int IN, typename DT, int MODE, int DIR>
+template <typename T, int IN, typename DT, int MODE, int DIR, bool FOLD = false>
 static void launch_sweep_batch(const BatchTables& t, const BatchCommon& c, const void* U, void* D,
-                               int* flags, StepCtl* ctls, hipStream_t s)
+                               int* flags, StepCtl* ctls, hipStream_t s, const void* Un = nullptr, int stage = 0)
 {
     const unsigned n = (unsigned)(DIR == DIR_X ? t.n_x : t.n_y);
     if (!n) return;
     const bool blend = !c.force_hlle;
     if (c.cons) {
-        if (blend) k_sweep_batch<T, IN, DT, MODE, DIR, true, true><<<n, WAVE, 0, s>>>(t, c, U, D, flags, ctls);
-        else k_sweep_batch<T, IN, DT, MODE, DIR, true, false><<<n, WAVE, 0, s>>>(t, c, U, D, flags, ctls);
+        if (blend)
+            k_sweep_batch<T, IN, DT, MODE, DIR, true, true, FOLD><<<n, WAVE, 0, s>>>(t, c, U, D, flags, ctls, Un, stage);
+        else
+            k_sweep_batch<T, IN, DT, MODE, DIR, true, false, FOLD><<<n, WAVE, 0, s>>>(t, c, U, D, flags, ctls, Un,
+                                                                                     stage);
     } else {
-        if (blend) k_sweep_batch<T, IN, DT, MODE, DIR, false, true><<<n, WAVE, 0, s>>>(t, c, U, D, flags, ctls);
-        else k_sweep_batch<T, IN, DT, MODE, DIR, false, false><<<n, WAVE, 0, s>>>(t, c, U, D, flags, ctls);
+        if (blend)
+            k_sweep_batch<T, IN, DT, MODE, DIR, false, true, FOLD><<<n, WAVE, 0, s>>>(t, c, U, D, flags, ctls, Un,
+                                                                                     stage);
+        else
+            k_sweep_batch<T, IN, DT, MODE, DIR, false, false, FOLD><<<n, WAVE, 0, s>>>(t, c, U, D, flags, ctls, Un,
+                                                                                      stage);
     }
 }
 
@@ -1023,10 +1162,18 @@ static void hb_stage(const BatchTables& t, const BatchCommon& c, int split, cons
         return;
     }
 #endif
+#if WX_FAST
+    // the x sweep forms part of the stage combination, as in the single-grid step
+    launch_sweep_batch<double, IN_F64, double, MODE_PLAIN, DIR_X, true>(t, c, Us, Dx, nullptr, ctls, s, Un, stage);
+    launch_sweep_batch<double, IN_F64, double, MODE_PLAIN, DIR_Y>(t, c, UsT, DyT, nullptr, ctls, s);
+    k_combine_batch<StF64, double, true><<<(unsigned)t.n_c, 256, 0, s>>>(
+        t, (const double*)Dx, (const double*)DyT, (void*)Un, (void*)Us, out, outT, stage, ctls);
+#else
     launch_sweep_batch<double, IN_F64, double, MODE_PLAIN, DIR_X>(t, c, Us, Dx, nullptr, ctls, s);
     launch_sweep_batch<double, IN_F64, double, MODE_PLAIN, DIR_Y>(t, c, UsT, DyT, nullptr, ctls, s);
     k_combine_batch<StF64, double><<<(unsigned)t.n_c, 256, 0, s>>>(
         t, (const double*)Dx, (const double*)DyT, (void*)Un, (void*)Us, out, outT, stage, ctls);
+#endif
 }
 
 static void hb_bc(const BatchTables& t, BcParams bp, int split, void* U, void* UT, StepCtl* ctls, int stage,
