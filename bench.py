@@ -94,7 +94,7 @@ def run_gpu(w, q, dx, steps, warmup, engine, variant, precision, profile=True, t
         assert k == steps, (k, steps)
         state = s.download()[3:-3, 3:-3, :4]
         out = dict(seconds=el, steps=k, value=w["nx"] * w["ny"] * k / el, dev_ms_per_step=s.ms_per_step,
-                   state=state, finite=bool(np.isfinite(state).all()), t=T)
+                   state=state, finite=bool(np.isfinite(state).all()), t=T, variant=variant)
         if engine == "fused":
             out["chunks"] = list(s.tiling)
             if tuned:
@@ -252,13 +252,19 @@ def objective_rate(w, q, dx, steps, reps=5):
     return out
 
 
-def kernel_bytes(w, split):
+def kernel_bytes(w, split, fold=False):
     """Algorithmic HBM bytes of each launch of a step (state and D arrays
-    touched once each; halo re-reads counted 0)."""
+    touched once each; halo re-reads counted 0), averaged over the three
+    stages.  fold: the fast FP64 step, whose x sweep also reads U^n in stages
+    2 and 3 and writes its part of the stage combination, so that the combine
+    reads only that and Dy^T."""
     cells = w["nx"] * w["ny"]
     padded = (w["nx"] + 6) * (w["ny"] + 6)
     d = 16 if split else 32            # D planes: 4 x FP32 or 4 x FP64 per cell
     st = 32                            # state: 4 x FP64, or FP32 hi + lo
+    if fold:
+        return {"sweep_x": st * padded + (2 * st / 3) * cells + d * cells, "sweep_y": st * padded + d * cells,
+                "combine": 2 * d * cells + 2 * st * cells, "cfl": st * cells}
     return {"sweep_x": st * padded + d * cells, "sweep_y": st * padded + d * cells,
             # stage 1 reads U^n, stages 2 and 3 also U_s; both orientations written
             "combine": 2 * d * cells + (st + 2 * st / 3) * cells + 2 * st * cells,
@@ -279,7 +285,10 @@ def roofline(res, w, split=False):
     step_ms = km["step_sum"] if km else res["dev_ms_per_step"]
     achieved = cells * BYTES_PER_CELL_UPDATE / (step_ms * 1e-3)
     tiling = tuple(res.get("chunks") or ())
-    tr = TRAFFIC.get((w["name"], w["nx"], w["ny"], split))
+    fold = not split and res.get("variant") == "fast"
+    # the counter tables hold the fused engine's fast-variant kernels
+    profiled = res.get("variant") == "fast" and bool(tiling)
+    tr = TRAFFIC.get((w["name"], w["nx"], w["ny"], split)) if profiled else None
     out = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
            "frac": achieved / HBM_PEAK, "traffic": tr["bytes_per_step"] if tr else None,
            "kernel": "one SSP-RK3 step (its kernels, back to back)",
@@ -290,7 +299,7 @@ def roofline(res, w, split=False):
         out["traffic_source"] = tr["source"]
         out["traffic_tiling"] = list(tr["tiling"])
     if km:
-        kb = kernel_bytes(w, split)
+        kb = kernel_bytes(w, split, fold)
         per = {}
         for k, launches in (("sweep_x", 3), ("sweep_y", 3), ("combine", 3), ("cfl", 1)):
             # FP64: the CFL maxima come out of the last combine's registers;
@@ -314,7 +323,7 @@ def roofline(res, w, split=False):
                                   "achieved_GBps": round(sw_bytes / (km["sweeps"] * 1e-3) / 1e9, 1),
                                   "share_of_step": round(3 * km["sweeps"] / step_ms, 4)}
         v = VALU.get((w["name"], w["nx"], w["ny"], split, "sweeps"))
-        if v and tuple(v["tiling"]) == tiling:
+        if v and tuple(v["tiling"]) == tiling and profiled:
             rate = v["wave_instr_per_launch"] / (km["sweeps"] * 1e-3)
             out["dominant_kernel"]["valu"] = {"wave_instr_per_launch": v["wave_instr_per_launch"],
                                               "achieved": rate, "peak": VALU_WAVE_INSTR_PEAK,
@@ -323,23 +332,24 @@ def roofline(res, w, split=False):
     return out
 
 
-# HBM bytes per step and VALU instructions per sweep launch from rocprofv3
-# counter runs (FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB: the factors a plain
-# 512 MiB device copy reads back; SQ_INSTS_VALU per dispatch), keyed by
-# workload, precision and sweep tiling (chunk_x, chunk_y, waves_x, waves_y):
-# the halo re-reads depend on the tiling, so other tilings report null.
+# HBM bytes per step and VALU instructions per sweep launch of the fused
+# engine's fast variant from rocprofv3 counter runs (FETCH_SIZE x 2 +
+# WRITE_SIZE x 1 KiB: the factors a plain 512 MiB device copy reads back;
+# SQ_INSTS_VALU per dispatch), keyed by workload, precision and sweep tiling
+# (chunk_x, chunk_y, waves_x, waves_y): the halo re-reads depend on the
+# tiling, so other tilings report null.
 _DMR_SRC64 = "profiles/dmr1920x480_fp64.md"
 _DMR_SRC32 = "profiles/dmr1920x480_fp32split.md"
 _QUAD_SRC64 = "profiles/quad4096_fp64.md"
 _QUAD_SRC32 = "profiles/quad4096_fp32split.md"
 TRAFFIC = {
-    ("double_mach_reflection", 1920, 480, False): {"bytes_per_step": 1.0838e9, "tiling": (10, 9, 4, 4),
+    ("double_mach_reflection", 1920, 480, False): {"bytes_per_step": 1.0712e9, "tiling": (10, 9, 4, 4),
                                                    "source": _DMR_SRC64},
-    ("double_mach_reflection", 1920, 480, True): {"bytes_per_step": 1.1491e9, "tiling": (10, 8, 2, 1),
+    ("double_mach_reflection", 1920, 480, True): {"bytes_per_step": 1.1495e9, "tiling": (10, 8, 2, 1),
                                                   "source": _DMR_SRC32},
-    ("four_quadrant_riemann", 4096, 4096, False): {"bytes_per_step": 17.3434e9, "tiling": (48, 48, 1, 1),
+    ("four_quadrant_riemann", 4096, 4096, False): {"bytes_per_step": 16.7160e9, "tiling": (48, 48, 1, 1),
                                                    "source": _QUAD_SRC64},
-    ("four_quadrant_riemann", 4096, 4096, True): {"bytes_per_step": 15.6968e9, "tiling": (48, 48, 1, 1),
+    ("four_quadrant_riemann", 4096, 4096, True): {"bytes_per_step": 15.6964e9, "tiling": (48, 48, 1, 1),
                                                   "source": _QUAD_SRC32},
 }
 # VALU wave-instructions of a stage's sweeps as the step launches them (one
@@ -347,11 +357,11 @@ TRAFFIC = {
 # differ; FP32-split: including the in-launch FP64 recompute of flagged face
 # rows)
 VALU = {
-    ("double_mach_reflection", 1920, 480, False, "sweeps"): {"wave_instr_per_launch": 4.980e7,
+    ("double_mach_reflection", 1920, 480, False, "sweeps"): {"wave_instr_per_launch": 5.081e7,
                                                              "tiling": (10, 9, 4, 4), "source": _DMR_SRC64},
     ("double_mach_reflection", 1920, 480, True, "sweeps"): {"wave_instr_per_launch": 3.442e7 + 3.757e7,
                                                             "tiling": (10, 8, 2, 1), "source": _DMR_SRC32},
-    ("four_quadrant_riemann", 4096, 4096, False, "sweeps"): {"wave_instr_per_launch": 8.092e8,
+    ("four_quadrant_riemann", 4096, 4096, False, "sweeps"): {"wave_instr_per_launch": 8.205e8,
                                                              "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64},
     ("four_quadrant_riemann", 4096, 4096, True, "sweeps"): {"wave_instr_per_launch": 1.014e9,
                                                             "tiling": (48, 48, 1, 1), "source": _QUAD_SRC32},
