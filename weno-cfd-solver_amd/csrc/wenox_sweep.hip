@@ -1048,9 +1048,9 @@ template <> __device__ __forceinline__ StSplit make_st<StSplit>(void* base, int6
 }
 
 template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND, bool FOLD = false>
-__global__ void __launch_bounds__(WAVE) k_sweep_batch(BatchTables B, BatchCommon C, const void* Ubase,
-                                                      void* Dbase, int* flags, StepCtl* ctls, const void* Unbase,
-                                                      int stage)
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2)))
+k_sweep_batch(BatchTables B, BatchCommon C, const void* Ubase, void* Dbase, int* flags, StepCtl* ctls,
+              const void* Unbase, int stage)
 {
     const int b = blockIdx.x;
     const int i = (DIR == DIR_X ? B.blk_x : B.blk_y)[b];
