@@ -317,3 +317,28 @@ def test_fused_matches_unfused_for_every_boundary(bc):
     assert out["fused"][1] == out["unfused"][1]
     np.testing.assert_array_equal(out["fused"][0], out["unfused"][0])
     np.testing.assert_array_equal(out["fused"][2], out["unfused"][2])
+
+
+@pytest.mark.parametrize("nx,ny,cols", [(33, 40, 32), (40, 34, 32), (66, 65, 64), (64, 35, 32), (35, 64, 64)])
+def test_fused_ghost_cells_for_every_edge_tile_width(nx, ny, cols):
+    """The fused FP64 step lets the RK combine write the ghost cells where
+    each ghost's source lies in the combine tile of its nearest interior
+    cell, and keeps the ghost-fill kernel where an edge tile is 1 or 2 cells
+    wide (33, 34, 65 wide grids) or the rule is periodic.  Either way the full
+    arrays, ghost cells included, equal the unfused engine's bit for bit
+    after 5 steps (exact variant)."""
+    from wenox import Solver, problems
+    q0, dx = problems.four_quadrant(nx, ny=ny)
+    print(f"[solver] ghost cells {nx}x{ny} combine cols {cols}", flush=True)
+    for bc in ("mirror", "periodic"):
+        out = {}
+        for engine in ("unfused", "fused"):
+            with Solver(nx, ny, dx, cfl=0.3, jp_cri=(0.5, 2.0), bc=bc, engine=engine, variant="exact") as s:
+                if engine == "fused":
+                    s.set_combine_cols(cols)
+                s.upload(q0)
+                T, k = s.run(nstep=5)
+                out[engine] = (s.download(), T)
+        assert np.isfinite(out["unfused"][0]).all()
+        assert out["fused"][1] == out["unfused"][1]
+        np.testing.assert_array_equal(out["fused"][0], out["unfused"][0], err_msg=bc)

@@ -22,6 +22,8 @@ struct SweepPlan {
     int chunk_x, chunk_y;      // interior rows marched per wavefront
     int waves_x, waves_y;      // wavefronts per workgroup along the normal (1, 2 or 4)
     int combine_cols;          // y extent of an RK-combine tile (32 or 64)
+    int ghost;                 // the step may let the combine write the ghost cells (ghost_on_store)
+    BcParams bp;               // ghost-fill rule of the solver
     // row pitches (elements): U planes, U^T planes, Dx rows, Dy^T rows
     int64_t ld, ldT, ldx, ldyT;
     int64_t P() const { return (nx + 6) * ld; }       // plane stride of U
@@ -124,6 +126,18 @@ extern const BatchOps batch_ops_fast;
 constexpr int SWEEP_OUT_COLS = 58;
 constexpr int COMBINE_TILE = 32;
 constexpr int GHOST_TB = 256;
+
+// The RK combine writes the ghost cells of the state it forms, in place of a
+// ghost-fill launch, when every ghost cell's source lies in the combine tile
+// that holds the ghost's nearest interior cell: the local rules (mirror,
+// copy, both double-Mach-reflection fills; periodic takes cells from the far
+// side) on grids whose edge tiles are at least 3 cells (the ghost depth) wide.
+inline bool ghost_on_store(const SweepPlan& p)
+{
+    if (!p.ghost || p.bp.kind == BC_PERIODIC) return false;
+    const int64_t rx = p.nx % COMBINE_TILE, ry = p.ny % p.combine_cols;
+    return p.nx >= 3 && p.ny >= 3 && rx != 1 && rx != 2 && ry != 1 && ry != 2;
+}
 
 extern const SweepOps sweep_ops_exact;
 extern const SweepOps sweep_ops_fast;

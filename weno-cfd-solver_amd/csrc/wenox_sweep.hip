@@ -579,12 +579,16 @@ constexpr int TT = 32;
 // next step's first launch finishes both.
 // FOLD: Dx holds the x sweep's part E of the combination (the sweep's FOLD);
 // the new state is E - (c dt) Dy.
-template <class St, typename DT, int TJ, bool CFL = false, bool FOLD = false>
+// GHOST (ghost_on_store): a tile on the grid's edge also writes, in both
+// orientations, the ghost cells whose nearest interior cell it holds, from
+// the new states in LDS by the solver's rule bp (bc_source / bc_value, as
+// the ghost-fill kernel).
+template <class St, typename DT, int TJ, bool CFL = false, bool FOLD = false, bool GHOST = false>
 __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT* __restrict__ DyT, St Un,
                                              St Us, St out, St outT, int64_t nx, int64_t ny, int64_t ld,
                                              int64_t ldT, int64_t ldx, int64_t ldyT, int stage,
                                              const StepCtl* ctl, int64_t ti, int64_t tj, double gam = 0.0,
-                                             int64_t t = 0, int64_t ntiles = 0)
+                                             int64_t t = 0, int64_t ntiles = 0, BcParams bp = BcParams{})
 {
     if (ctl && !ctl->active) return;
     double wmax = 0.0;
@@ -675,6 +679,45 @@ __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT
                 }
         }
     }
+    if constexpr (GHOST) {
+        // padded rows / columns of the ghost cells anchored in this tile: its
+        // own, widened by the 3 ghost layers on the grid's edges (the tile is
+        // read-only in LDS from the barrier above on)
+        const bool e0 = ti == 0, e1 = i0 + TT >= nx, f0 = tj == 0, f1 = j0 + TJ >= ny;
+        if (e0 || e1 || f0 || f1) {
+            const int ia = e0 ? 0 : (int)i0 + 3, ib = e1 ? (int)nx + 5 : (int)i0 + TT + 2;
+            const int ja = f0 ? 0 : (int)j0 + 3, jb = f1 ? (int)ny + 5 : (int)j0 + TJ + 2;
+            // the ghost cells only: ghost rows over the full column range
+            // (top, bottom), then ghost columns over the interior rows (left,
+            // right)
+            const int ci = ib - ia + 1, cj = jb - ja + 1;
+            const int ri = ((int)i0 + 3 > ia ? 3 : 0), rb = (ib > (int)nx + 2 ? 3 : 0);   // ghost rows above / below
+            const int gj = (f0 ? 3 : 0) + (f1 ? 3 : 0);                                    // ghost columns
+            const int nrow = (ri + rb) * cj, n = nrow + (ci - ri - rb) * gj;
+            const double ts = bp.kind == BC_DMR_STD ? bc_time(ctl, bp, stage) : 0.0;
+            for (int q = threadIdx.x; q < n; q += 256) {
+                int64_t i, j;
+                if (q < nrow) {
+                    const int r = q / cj;
+                    i = r < ri ? ia + r : (int64_t)nx + 3 + (r - ri);
+                    j = ja + q % cj;
+                } else {
+                    const int gs = gj > 0 ? gj : 1;   // (gj = 0 leaves this branch unused)
+                    const int u = q - nrow, c = u % gs;
+                    i = ia + ri + u / gs;
+                    j = (f0 && c < 3) ? c : (int64_t)ny + 3 + (c - (f0 ? 3 : 0));
+                }
+                const BcSrc b = bc_source(i, j, nx, ny, bp, ts);
+                double v[4];
+                bc_value(b, bp, [&](int k) { return tile[k][b.sj - 3 - j0][b.si - 3 - i0]; }, v);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    out.st(k, i * ld + j, v[k]);
+                    outT.st(k, j * ldT + i, v[k]);
+                }
+            }
+        }
+    }
 }
 
 // Workgroups go to the 8 XCDs round-robin in launch order.  Workgroup
@@ -682,46 +725,54 @@ __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT
 // into 8 contiguous bands (the first ntiles % 8 of them one tile longer): each
 // XCD's L2 then holds a contiguous band, and the cache lines two neighbouring
 // tiles share (rows are 32 doubles, 3 off a line boundary) are fetched once.
-template <class St, typename DT, int TJ, bool CFL = false, bool FOLD = false>
+template <class St, typename DT, int TJ, bool CFL = false, bool FOLD = false, bool GHOST = false>
 __global__ void __launch_bounds__(256) k_combine(const DT* __restrict__ Dx, const DT* __restrict__ DyT,
                                                  St Un, St Us, St out, St outT, int64_t nx, int64_t ny,
                                                  int64_t ld, int64_t ldT, int64_t ldx, int64_t ldyT,
-                                                 int stage, const StepCtl* ctl, double gam)
+                                                 int stage, const StepCtl* ctl, double gam, BcParams bp)
 {
     const int64_t tiles_j = gridDim.x, ntiles = (int64_t)gridDim.x * gridDim.y;
     const int64_t b = (int64_t)blockIdx.y * tiles_j + blockIdx.x;
     const int64_t q = ntiles / 8, r = ntiles % 8, x = b % 8, k = b / 8;
     const int64_t t = x * q + (x < r ? x : r) + k;
-    combine_tile<St, DT, TJ, CFL, FOLD>(Dx, DyT, Un, Us, out, outT, nx, ny, ld, ldT, ldx, ldyT, stage, ctl,
-                                        t / tiles_j, t % tiles_j, gam, t, ntiles);
+    combine_tile<St, DT, TJ, CFL, FOLD, GHOST>(Dx, DyT, Un, Us, out, outT, nx, ny, ld, ldT, ldx, ldyT, stage, ctl,
+                                               t / tiles_j, t % tiles_j, gam, t, ntiles, bp);
 }
 
 // cfl: the step's last combine also leaves the next step's CFL maxima (one
 // per tile, combine_tiles(p) of them) in ctl->part and the advance pending.
 // FOLD: Dx holds the x sweep's part of the combination (the sweep's FOLD).
-template <class St, typename DT, int TJ, bool CFL, bool FOLD>
+// ghost: the combine also writes the ghost cells (ghost_on_store(p)).
+template <class St, typename DT, int TJ, bool CFL, bool FOLD, bool GHOST>
 static void launch_combine_t(const SweepPlan& p, const DT* Dx, const DT* DyT, St Un, St Us, St out, St outT,
                              int stage, const StepCtl* ctl, hipStream_t s)
 {
     const dim3 g((unsigned)((p.ny + TJ - 1) / TJ), (unsigned)((p.nx + TT - 1) / TT));
-    k_combine<St, DT, TJ, CFL, FOLD><<<g, 256, 0, s>>>(Dx, DyT, Un, Us, out, outT, p.nx, p.ny, p.ld, p.ldT, p.ldx,
-                                                      p.ldyT, stage, ctl, p.gamma);
+    k_combine<St, DT, TJ, CFL, FOLD, GHOST><<<g, 256, 0, s>>>(Dx, DyT, Un, Us, out, outT, p.nx, p.ny, p.ld, p.ldT,
+                                                             p.ldx, p.ldyT, stage, ctl, p.gamma, p.bp);
 }
 
 template <class St, typename DT, int TJ, bool FOLD>
 static void launch_combine_f(const SweepPlan& p, const DT* Dx, const DT* DyT, St Un, St Us, St out, St outT,
-                             int stage, const StepCtl* ctl, hipStream_t s, bool cfl)
+                             int stage, const StepCtl* ctl, hipStream_t s, bool cfl, bool ghost)
 {
-    if (cfl) launch_combine_t<St, DT, TJ, true, FOLD>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s);
-    else launch_combine_t<St, DT, TJ, false, FOLD>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s);
+    if (cfl) {
+        if (ghost) launch_combine_t<St, DT, TJ, true, FOLD, true>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s);
+        else launch_combine_t<St, DT, TJ, true, FOLD, false>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s);
+    } else {
+        if (ghost) launch_combine_t<St, DT, TJ, false, FOLD, true>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s);
+        else launch_combine_t<St, DT, TJ, false, FOLD, false>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s);
+    }
 }
 
 template <class St, typename DT, bool FOLD = false>
 static void launch_combine(const SweepPlan& p, const DT* Dx, const DT* DyT, St Un, St Us, St out, St outT,
-                           int stage, const StepCtl* ctl, hipStream_t s, bool cfl = false)
+                           int stage, const StepCtl* ctl, hipStream_t s, bool cfl = false, bool ghost = false)
 {
-    if (p.combine_cols == 64) launch_combine_f<St, DT, 64, FOLD>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s, cfl);
-    else launch_combine_f<St, DT, 32, FOLD>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s, cfl);
+    if (p.combine_cols == 64)
+        launch_combine_f<St, DT, 64, FOLD>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s, cfl, ghost);
+    else
+        launch_combine_f<St, DT, 32, FOLD>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s, cfl, ghost);
 }
 
 // Plain RHS from the two sweeps (for tests): L = -(Dx + Dy^T), SoA4 interior
@@ -859,7 +910,8 @@ static void h_stage_fold(const SweepPlan& p, const double* Us, const double* UsT
     const SweepArgs ay = args_y(p, UsT, DyT, nullptr, ctl);
     const StF64 un{const_cast<double*>(Un), P}, us{const_cast<double*>(Us), P}, o{out, P}, oT{outT, p.PT()};
     if (sweeps) launch_sweeps<double, IN_F64, double, MODE_PLAIN, true>(ax, ay, p.cons, blend, s);
-    if (combine) launch_combine<StF64, double, true>(p, Dx, DyT, un, us, o, oT, stage, ctl, s, stage == 3);
+    if (combine)
+        launch_combine<StF64, double, true>(p, Dx, DyT, un, us, o, oT, stage, ctl, s, stage == 3, ghost_on_store(p));
 }
 #endif
 
@@ -876,7 +928,7 @@ static void h_stage(const SweepPlan& p, const double* Us, const double* UsT, con
     h_sweeps(p, Us, UsT, Dx, DyT, ctl, s);
     const int64_t P = p.P();
     launch_combine<StF64, double>(p, Dx, DyT, StF64{const_cast<double*>(Un), P}, StF64{const_cast<double*>(Us), P},
-                                  StF64{out, P}, StF64{outT, p.PT()}, stage, ctl, s, stage == 3);
+                                  StF64{out, P}, StF64{outT, p.PT()}, stage, ctl, s, stage == 3, ghost_on_store(p));
 }
 
 static void h_rhs(const SweepPlan& p, const double* U, const double* UT, double* Dx, double* DyT,
@@ -954,7 +1006,7 @@ static void h_combine1(const SweepPlan& p, int split, const void* Dx, const void
 #endif
     launch_combine<StF64, double>(p, (const double*)Dx, (const double*)DyT, StF64{(double*)Un, P},
                                   StF64{(double*)Us, P}, StF64{(double*)out, P}, StF64{(double*)outT, PT}, 2, ctl,
-                                  s);
+                                  s, false, ghost_on_store(p));
 }
 
 #if WX_FAST
