@@ -129,7 +129,10 @@ class Solver:
         their tilings share a wave count); "step_sum" adds it up per step.
         "cfl" is the step's CFL launch: with FP64 the one-wave dt reduction
         (the last RK combine of the previous step forms the maxima), with
-        FP32-split the reduction over the state."""
+        FP32-split the reduction over the state.  "ghost_fill" is 0 where the
+        FP64 step's RK combine writes the ghost cells itself (mirror, copy and
+        the double-Mach-reflection fills on grids whose edge tiles are at
+        least 3 cells wide)."""
         out = np.zeros(7)
         _lib.check(_lib.load().wenox_solver_profile(self._h, int(reps), out.ctypes.data, 7))
         return dict(zip(self.KERNELS, out.tolist()))
