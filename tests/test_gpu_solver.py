@@ -342,3 +342,31 @@ def test_fused_ghost_cells_for_every_edge_tile_width(nx, ny, cols):
         assert np.isfinite(out["unfused"][0]).all()
         assert out["fused"][1] == out["unfused"][1]
         np.testing.assert_array_equal(out["fused"][0], out["unfused"][0], err_msg=bc)
+
+
+@pytest.mark.parametrize("bc", ["mirror", "copy", "dmr"])
+def test_fp32_split_ghost_cells_on_store(bc):
+    """The FP32-split step's combine writes the ghost cells too where
+    ghost_on_store allows it.  On a 40 x 34 grid 32-wide combine tiles leave
+    a 2-cell edge tile (the fill kernel runs) and 64-wide tiles do not (the
+    combine writes the ghosts); the tile width changes nothing else, so the
+    two runs agree bit for bit, ghost cells included."""
+    from wenox import Solver, problems
+    nx, ny = 40, 34
+    if bc == "dmr":
+        q0, dx = problems.double_mach(nx, ny)
+        kw = dict(cfl=0.1, jp_cri=(1e4, 1.1e4))
+    else:
+        q0, dx = problems.four_quadrant(nx, ny=ny)
+        kw = dict(cfl=0.3, jp_cri=(0.5, 2.0))
+    print(f"[solver] fp32-split ghost cells {nx}x{ny} bc={bc}", flush=True)
+    out = {}
+    for cols in (32, 64):
+        with Solver(nx, ny, dx, bc=bc, engine="fused", variant="fast", precision="fp32split", **kw) as s:
+            s.set_combine_cols(cols)
+            s.upload(q0)
+            T, k = s.run(nstep=6)
+            out[cols] = (s.download(), T)
+    assert np.isfinite(out[32][0]).all()
+    assert out[32][1] == out[64][1]
+    np.testing.assert_array_equal(out[64][0], out[32][0])

@@ -500,7 +500,8 @@ void enqueue_step(wenox_solver* s)
             const int i = stage - 1;
             s->sops->stage_split(s->plan, src[i], srcT[i], (const float*)s->U, (float*)s->L,
                                  (float*)s->DyT, dst[i], dstT[i], s->flags, stage, s->ctl, st);
-            s->sops->bc_split(dst[i], dstT[i], d.nx, d.ny, s->ld, s->ldT, s->bp, s->ctl, stage, st);
+            if (!ghost_on_store(s->plan))
+                s->sops->bc_split(dst[i], dstT[i], d.nx, d.ny, s->ld, s->ldT, s->bp, s->ctl, stage, st);
         }
         o->advance(s->ctl, st);
         return;
@@ -673,10 +674,10 @@ extern "C" int wenox_solver_create(const wenox_solver_desc* desc, wenox_solver**
     s->bp.x0 = d.dmr_x0;
     s->bp.gamma = d.gamma;
     if (fused) {
-        // the FP64 step lets the RK combine write the ghost cells where the
-        // rule allows it (ghost_on_store); FP32-split keeps the fill kernel
+        // the step lets the RK combine write the ghost cells where the rule
+        // allows it (ghost_on_store)
         s->plan.bp = s->bp;
-        s->plan.ghost = s->split ? 0 : 1;
+        s->plan.ghost = 1;
     }
     StepCtl h{};
     h.cfl_dx = d.cfl * d.dx;
@@ -954,7 +955,7 @@ extern "C" int wenox_solver_profile(wenox_solver* s, int reps, double* out_ms, i
          })))
         return rc;
     // slot 4: the ghost fill, 0 where the combine writes the ghost cells
-    if (!split && ghost_on_store(s->plan)) {
+    if (ghost_on_store(s->plan)) {
         out_ms[4] = 0.0;
     } else if ((rc = timed(4, [&] {
                     if (split)

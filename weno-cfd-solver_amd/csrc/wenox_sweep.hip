@@ -995,7 +995,7 @@ static void h_combine1(const SweepPlan& p, int split, const void* Dx, const void
     if (split) {
         launch_combine<StSplit, float>(p, (const float*)Dx, (const float*)DyT, StSplit{(float*)Un, P},
                                        StSplit{(float*)Us, P}, StSplit{(float*)out, P}, StSplit{(float*)outT, PT},
-                                       2, ctl, s);
+                                       2, ctl, s, false, ghost_on_store(p));
         return;
     }
     if (ctl) {
@@ -1028,7 +1028,7 @@ static void h_stage_split(const SweepPlan& p, const float* Us, const float* UsT,
     h_sweeps_split(p, Us, UsT, Dx, DyT, flags, ctl, s);
     const int64_t P = p.P();
     launch_combine<StSplit, float>(p, Dx, DyT, StSplit{const_cast<float*>(Un), P}, StSplit{const_cast<float*>(Us), P},
-                                   StSplit{out, P}, StSplit{outT, p.PT()}, stage, ctl, s);
+                                   StSplit{out, P}, StSplit{outT, p.PT()}, stage, ctl, s, false, ghost_on_store(p));
 }
 
 static void h_rhs_split(const SweepPlan& p, const float* U, const float* UT, float* Dx, float* DyT,

@@ -130,7 +130,7 @@ class Solver:
         "cfl" is the step's CFL launch: with FP64 the one-wave dt reduction
         (the last RK combine of the previous step forms the maxima), with
         FP32-split the reduction over the state.  "ghost_fill" is 0 where the
-        FP64 step's RK combine writes the ghost cells itself (mirror, copy and
+        step's RK combine writes the ghost cells itself (mirror, copy and
         the double-Mach-reflection fills on grids whose edge tiles are at
         least 3 cells wide)."""
         out = np.zeros(7)
