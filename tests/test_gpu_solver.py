@@ -370,3 +370,36 @@ def test_fp32_split_ghost_cells_on_store(bc):
     assert np.isfinite(out[32][0]).all()
     assert out[32][1] == out[64][1]
     np.testing.assert_array_equal(out[64][0], out[32][0])
+
+
+def test_fp32_split_step_control_across_run_calls():
+    """The FP32-split step takes dt from the previous step's last RK combine
+    (or the upload) and applies its time advance at the next step's start or
+    when run() returns, like FP64: one call, one step per call, graph-sized
+    and odd splits give bitwise the same state, time and step count, and a
+    t_final run stops exactly at t_final with the last dt clipped."""
+    from wenox import Solver, problems
+    n, steps = 48, 13
+    q0, dx = problems.four_quadrant(n)
+    print("[solver] fp32-split step control across run calls", flush=True)
+    res = []
+    for split in ([steps], [1] * steps, [8, 5], [3, 9, 1]):
+        with Solver(n, n, dx, cfl=0.4, jp_cri=(0.5, 2.0), engine="fused", variant="fast",
+                    precision="fp32split") as s:
+            s.upload(q0)
+            total, T = 0, 0.0
+            for k in split:
+                T, got = s.run(nstep=k)
+                total += got
+            assert total == steps
+            res.append((T, s.download()))
+    for T, q in res[1:]:
+        assert T == res[0][0]
+        np.testing.assert_array_equal(q, res[0][1])
+    t_end = 0.5 * res[0][0]
+    with Solver(n, n, dx, cfl=0.4, jp_cri=(0.5, 2.0), engine="fused", variant="fast", precision="fp32split") as s:
+        s.upload(q0)
+        T, k = s.run(t_final=t_end)
+        assert T == pytest.approx(t_end, rel=0, abs=1e-15) and 0 < k < steps
+        T2, k2 = s.run(nstep=3, t_final=t_end)
+        assert k2 == 0 and T2 == T

@@ -483,8 +483,9 @@ void drop_graphs(wenox_solver* s)
 double* work_state(const wenox_solver* s) { return s->work + STATE_SHIFT; }
 
 // One time step on the solver's stream: CFL -> 3 x (RHS, RK combine, ghost fill) -> advance
-// (fused FP64 engine: dt from the previous step's tile maxima, the advance
-// applied at the next step's start or by a settle launch).
+// (fused engine: dt from the previous step's tile maxima, the advance applied
+// at the next step's start or by a settle launch; the combine writes the
+// ghost cells where ghost_on_store allows it).
 void enqueue_step(wenox_solver* s)
 {
     const auto& d = s->d;
@@ -495,7 +496,8 @@ void enqueue_step(wenox_solver* s)
         float* dst[3] = {(float*)s->A, (float*)s->B, (float*)s->U};
         float* srcT[3] = {(float*)s->UT, (float*)s->AT, (float*)s->BT};
         float* dstT[3] = {(float*)s->AT, (float*)s->BT, (float*)s->UT};
-        s->sops->cfl_split((const float*)s->U, d.nx, d.ny, s->ld, d.gamma, s->ctl, st);
+        // dt from the previous step's last combine (or the upload), as FP64
+        o->step_dt(s->ctl, st);
         for (int stage = 1; stage <= 3; ++stage) {
             const int i = stage - 1;
             s->sops->stage_split(s->plan, src[i], srcT[i], (const float*)s->U, (float*)s->L,
@@ -503,7 +505,7 @@ void enqueue_step(wenox_solver* s)
             if (!ghost_on_store(s->plan))
                 s->sops->bc_split(dst[i], dstT[i], d.nx, d.ny, s->ld, s->ldT, s->bp, s->ctl, stage, st);
         }
-        o->advance(s->ctl, st);
+        // the last combine left the next step's CFL maxima and the advance pending
         return;
     }
     const bool fused = d.engine == WENOX_ENGINE_FUSED;
@@ -758,6 +760,8 @@ extern "C" int wenox_solver_upload(wenox_solver* s, const wenox_view3* q, double
         s->sops->to_split(work_state(s), (float*)s->U, (int64_t)P, s->stream);
         s->sops->transpose_split((const float*)s->U, (float*)s->UT, m, n, ld, s->ldT, s->stream);
         s->sops->bc_split((float*)s->U, (float*)s->UT, d.nx, d.ny, ld, s->ldT, s->bp, s->ctl, 0, s->stream);
+        // the first step reads the CFL maxima of this state
+        s->sops->cfl_split((const float*)s->U, d.nx, d.ny, ld, d.gamma, s->ctl, s->stream, 0);
     } else {
         if (s->UT) s->sops->transpose(s->U, s->UT, m, n, ld, s->ldT, s->stream);
         s->ops->bc(s->U, s->UT, d.nx, d.ny, ld, s->ldT, s->bp, s->ctl, 0, s->stream);
@@ -937,12 +941,10 @@ extern "C" int wenox_solver_profile(wenox_solver* s, int reps, double* out_ms, i
     };
     // scratch targets: the sweeps read the stage-1 input (U, U^T) and write Dx / Dy^T;
     // the combine writes into B / B^T, which the next step overwrites anyway.
-    // slot 0: the step's CFL launches -- FP32-split: the reduction over the
-    // state; FP64: the one-wave dt launch (the maxima come from the previous
-    // step's last combine)
+    // slot 0: the step's CFL launch, the one-block dt reduction (the maxima
+    // come from the previous step's last combine)
     if ((rc = timed(0, [&] {
-             if (split) s->sops->cfl_split((const float*)s->U, d.nx, d.ny, s->ld, d.gamma, s->ctl, st);
-             else s->ops->step_dt(s->ctl, st);
+             s->ops->step_dt(s->ctl, st);
          })))
         return rc;
     if ((rc = write_ctl(s, &act))) return rc;
@@ -988,9 +990,11 @@ extern "C" int wenox_solver_set_cfl_rule(wenox_solver* s, double rho_floor, doub
     h.cmax_eps = cmax_eps;
     rc = write_ctl(s, &h);
     if (rc) return rc;
-    if (s->d.engine == WENOX_ENGINE_FUSED && !s->split) {
+    if (s->d.engine == WENOX_ENGINE_FUSED) {
         // the stored CFL maxima were formed with the previous floor
-        s->ops->cfl_part(s->U, s->d.nx, s->d.ny, s->ld, s->d.gamma, s->ctl, s->stream);
+        if (s->split)
+            s->sops->cfl_split((const float*)s->U, s->d.nx, s->d.ny, s->ld, s->d.gamma, s->ctl, s->stream, 0);
+        else s->ops->cfl_part(s->U, s->d.nx, s->d.ny, s->ld, s->d.gamma, s->ctl, s->stream);
         HIPCHK(hipStreamSynchronize(s->stream));
     }
     return WENOX_OK;

@@ -14,6 +14,8 @@ struct StF64 {
     int64_t P;
     __device__ __forceinline__ double ld(int k, int64_t idx) const { return p[k * P + idx]; }
     __device__ __forceinline__ void st(int k, int64_t idx, double v) const { p[k * P + idx] = v; }
+    // the value st(v) leaves in memory, as ld reads it back
+    static __device__ __forceinline__ double stored(double v) { return v; }
 };
 
 struct StSplit {
@@ -28,6 +30,11 @@ struct StSplit {
         float h = (float)v;
         p[k * P + idx] = h;
         p[(k + 4) * P + idx] = (float)(v - (double)h);
+    }
+    static __device__ __forceinline__ double stored(double v)
+    {
+        const float h = (float)v;
+        return (double)h + (double)(float)(v - (double)h);
     }
 };
 

@@ -61,7 +61,10 @@ struct SweepOps {
     void (*from_split)(const float* in, double* out, int64_t P, hipStream_t s);
     void (*bc_split)(float* U, float* UT, int64_t nx, int64_t ny, int64_t ld, int64_t ldT, BcParams bp,
                      const StepCtl* ctl, int stage, hipStream_t s);
-    void (*cfl_split)(const float* U, int64_t nx, int64_t ny, int64_t ld, double gam, StepCtl* ctl, hipStream_t s);
+    // CFL maxima of a split state into ctl->part; finish: also form dt (as
+    // the step's first launch does from the last combine's maxima)
+    void (*cfl_split)(const float* U, int64_t nx, int64_t ny, int64_t ld, double gam, StepCtl* ctl, hipStream_t s,
+                      int finish);
 
     // Single kernels, for per-kernel timing: one direction's sweep of the
     // stage input U (U^T for dir 1) into D, and the stage-2 combine.

@@ -652,8 +652,8 @@ __device__ __forceinline__ void combine_tile(const DT* __restrict__ Dx, const DT
                     double qv[4];
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
-                        qv[k] = tile[k][r][c];
-                        outT.st(k, (j + 3) * ldT + (i + 3), qv[k]);
+                        outT.st(k, (j + 3) * ldT + (i + 3), tile[k][r][c]);
+                        qv[k] = St::stored(tile[k][r][c]);   // the state as stored (FP32-split: hi + lo)
                     }
                     wmax = nan_max(wmax, cfl_speed(qv, gam, ctl->rho_floor));
                 }
@@ -1028,7 +1028,7 @@ static void h_stage_split(const SweepPlan& p, const float* Us, const float* UsT,
     h_sweeps_split(p, Us, UsT, Dx, DyT, flags, ctl, s);
     const int64_t P = p.P();
     launch_combine<StSplit, float>(p, Dx, DyT, StSplit{const_cast<float*>(Un), P}, StSplit{const_cast<float*>(Us), P},
-                                   StSplit{out, P}, StSplit{outT, p.PT()}, stage, ctl, s, false, ghost_on_store(p));
+                                   StSplit{out, P}, StSplit{outT, p.PT()}, stage, ctl, s, stage == 3, ghost_on_store(p));
 }
 
 static void h_rhs_split(const SweepPlan& p, const float* U, const float* UT, float* Dx, float* DyT,
@@ -1080,9 +1080,9 @@ static void h_bc_split(float* U, float* UT, int64_t nx, int64_t ny, int64_t ld, 
 }
 
 static void h_cfl_split(const float* U, int64_t nx, int64_t ny, int64_t ld, double gam, StepCtl* ctl,
-                        hipStream_t s)
+                        hipStream_t s, int finish)
 {
-    launch_cfl(StSplit{const_cast<float*>(U), (nx + 6) * ld}, nx, ny, ld, gam, ctl, s);
+    launch_cfl(StSplit{const_cast<float*>(U), (nx + 6) * ld}, nx, ny, ld, gam, ctl, s, finish != 0);
 }
 #endif
 
