@@ -343,13 +343,13 @@ _DMR_SRC32 = "profiles/dmr1920x480_fp32split.md"
 _QUAD_SRC64 = "profiles/quad4096_fp64.md"
 _QUAD_SRC32 = "profiles/quad4096_fp32split.md"
 TRAFFIC = {
-    ("double_mach_reflection", 1920, 480, False): {"bytes_per_step": 1.0668e9, "tiling": (10, 9, 4, 4),
+    ("double_mach_reflection", 1920, 480, False): {"bytes_per_step": 1.0667e9, "tiling": (10, 9, 4, 4),
                                                    "source": _DMR_SRC64},
-    ("double_mach_reflection", 1920, 480, True): {"bytes_per_step": 1.1491e9, "tiling": (10, 8, 2, 1),
+    ("double_mach_reflection", 1920, 480, True): {"bytes_per_step": 1.1225e9, "tiling": (10, 8, 2, 1),
                                                   "source": _DMR_SRC32},
-    ("four_quadrant_riemann", 4096, 4096, False): {"bytes_per_step": 16.6974e9, "tiling": (48, 48, 1, 1),
+    ("four_quadrant_riemann", 4096, 4096, False): {"bytes_per_step": 16.6967e9, "tiling": (48, 48, 1, 1),
                                                    "source": _QUAD_SRC64},
-    ("four_quadrant_riemann", 4096, 4096, True): {"bytes_per_step": 15.6970e9, "tiling": (48, 48, 1, 1),
+    ("four_quadrant_riemann", 4096, 4096, True): {"bytes_per_step": 15.3719e9, "tiling": (48, 48, 1, 1),
                                                   "source": _QUAD_SRC32},
 }
 # VALU wave-instructions of a stage's sweeps as the step launches them (one
