@@ -342,6 +342,8 @@ _DMR_SRC64 = "profiles/dmr1920x480_fp64.md"
 _DMR_SRC32 = "profiles/dmr1920x480_fp32split.md"
 _QUAD_SRC64 = "profiles/quad4096_fp64.md"
 _QUAD_SRC32 = "profiles/quad4096_fp32split.md"
+_KH_SRC64 = "profiles/kh8192_fp64.md"
+_KH_SRC32 = "profiles/kh8192_fp32split.md"
 TRAFFIC = {
     ("double_mach_reflection", 1920, 480, False): {"bytes_per_step": 1.0667e9, "tiling": (10, 9, 4, 4),
                                                    "source": _DMR_SRC64},
@@ -351,6 +353,10 @@ TRAFFIC = {
                                                    "source": _QUAD_SRC64},
     ("four_quadrant_riemann", 4096, 4096, True): {"bytes_per_step": 15.3719e9, "tiling": (48, 48, 1, 1),
                                                   "source": _QUAD_SRC32},
+    ("kelvin_helmholtz", 8192, 8192, False): {"bytes_per_step": 66.4381e9, "tiling": (57, 57, 1, 1),
+                                              "source": _KH_SRC64},
+    ("kelvin_helmholtz", 8192, 8192, True): {"bytes_per_step": 60.8258e9, "tiling": (57, 57, 1, 1),
+                                             "source": _KH_SRC32},
 }
 # VALU wave-instructions of a stage's sweeps as the step launches them (one
 # k_sweep_xy, or the x and y k_sweep launches together when their wave counts
@@ -365,6 +371,10 @@ VALU = {
                                                              "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64},
     ("four_quadrant_riemann", 4096, 4096, True, "sweeps"): {"wave_instr_per_launch": 1.014e9,
                                                             "tiling": (48, 48, 1, 1), "source": _QUAD_SRC32},
+    ("kelvin_helmholtz", 8192, 8192, False, "sweeps"): {"wave_instr_per_launch": 3.362e9,
+                                                        "tiling": (57, 57, 1, 1), "source": _KH_SRC64},
+    ("kelvin_helmholtz", 8192, 8192, True, "sweeps"): {"wave_instr_per_launch": 4.129e9,
+                                                       "tiling": (57, 57, 1, 1), "source": _KH_SRC32},
 }
 
 
