@@ -150,6 +150,14 @@ int wenox_solver_download(wenox_solver *s, double *q8);
  * t_final >= 0 (Rie_prob_2d_rs.py:335).  Reports the time and steps taken. */
 int wenox_solver_run(wenox_solver *s, int64_t max_steps, double t_final, double *t_out,
                      int64_t *steps_out);
+/* Compensated global sums of the current state over the interior cells:
+ * out[0..3] = sum of rho, rho u, rho v, E (the conserved channels 0-3 of the
+ * reference's state array, Rie_prob_2d_rs.py:28-32), in FP64 with TwoSum error
+ * compensation (Ogita-Rump-Oishi Sum2) for FP64 and FP32-split states, in an
+ * order fixed by the grid size; with n_out = 8, out[4..7] = the compensated
+ * sums of their magnitudes.  Multiply by dx^2 for the totals.  No reference
+ * counterpart (the reference forms no global sums); conservation checks. */
+int wenox_solver_conserved_sums(wenox_solver *s, double *out, int n_out);
 /* Evaluate L(U) of the current state into out (nx, ny, 4) host array. */
 int wenox_solver_rhs(wenox_solver *s, double *out);
 /* Average device time per step of the last run (ms), from HIP events. */

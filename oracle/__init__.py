@@ -65,13 +65,33 @@ def lib():
         L.orc_test_gq.argtypes = [dp, dp, dp]
         L.orc_test_riemann_pt.argtypes = [i, i, dp, dp, dp, dp, d, dp]
         L.orc_set_threads.argtypes = [i]
+        L.orc_pp_counts.argtypes = [dp]
+        L.orc_pp_reset.argtypes = []
         L.orc_set_twin.argtypes = [i]
-        for f in (L.orc_set_threads, L.orc_set_twin, L.orc_flux, L.orc_con2primi, L.orc_weno, L.orc_riemann,
+        for f in (L.orc_pp_counts, L.orc_pp_reset, L.orc_set_threads, L.orc_set_twin, L.orc_flux,
+                  L.orc_con2primi, L.orc_weno, L.orc_riemann,
                   L.orc_sensor, L.orc_blend, L.orc_l_local, L.orc_test_roe_eig,
                   L.orc_test_pp_limit, L.orc_test_weno5, L.orc_test_gq, L.orc_test_riemann_pt):
             f.restype = None
         _lib = L
     return _lib
+
+
+class count_limiter:
+    """Context manager counting the faces where the positivity limiter's
+    density pass and pressure pass acted (weno.rs:327-352) in the oracle
+    calls inside the block: ``with count_limiter() as c: ...; c.density,
+    c.pressure``."""
+
+    def __enter__(self):
+        lib().orc_pp_reset()
+        self.density = self.pressure = 0
+        return self
+
+    def __exit__(self, *exc):
+        out = np.zeros(2, dtype=np.int64)
+        lib().orc_pp_counts(out.ctypes.data)
+        self.density, self.pressure = int(out[0]), int(out[1])
 
 
 def set_threads(n: int) -> None:
@@ -431,6 +451,21 @@ def cfl_dt(q, nx, ny, gamma, cfl, dx, remaining=None):
     if remaining is not None:
         dt = min(dt, remaining)
     return dt
+
+
+def run_dmr_reference(q, nx, ny, dx, bound, gamma, cfl, t_final, nstep, force_hlle=False,
+                      jp_cri=(1e4, 1.1e4)):
+    """Time-controlled loop of double_mach_reflection.main_rs (:229-260)
+    without I/O: CFL dt clamped to the remaining time, rk3_refac, T += dt,
+    while T < t_final - 1e-12 and step < nstep.  Returns (q, list of T)."""
+    T, step, ts = 0.0, 0, [0.0]
+    while T < t_final - 1e-12 and step < int(nstep):
+        dt = cfl_dt(q, nx, ny, gamma, cfl, dx, remaining=t_final - T)
+        q = rk3_refac(q, nx, ny, dx, dt, gamma, force_hlle, bound, jp_cri)
+        T += dt
+        step += 1
+        ts.append(T)
+    return q, ts
 
 
 def run_riemann(q, nx, ny, dx, gamma, cfl, force_hlle, weno_switch, jp_cri,

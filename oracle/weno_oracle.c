@@ -298,16 +298,30 @@ static double pressure_pp(const double q[4], double g)
     return (g - 1.0) * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) / q[0]);
 }
 
+/* faces where the limiter's density / pressure pass acted since the last
+   orc_pp_reset() (test instrumentation: shows that a case exercises them) */
+static long orc_pp_fired[2];
+void orc_pp_reset(void) { orc_pp_fired[0] = orc_pp_fired[1] = 0; }
+void orc_pp_counts(long out[2]) { out[0] = orc_pp_fired[0]; out[1] = orc_pp_fired[1]; }
+
 /* weno.rs:327-352  density pass, then pressure pass, toward the cell average */
 static void pp_limit(double q[4], const double ub[4], double g)
 {
     if (q[0] < ORC_RHO_MIN) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+        orc_pp_fired[0]++;
         double rho_bar = fmax(ub[0], ORC_RHO_MIN);
         double th = fmax(fmin((rho_bar - ORC_RHO_MIN) / (rho_bar - q[0]), 1.0), 0.0);
         for (int k = 0; k < 4; ++k) q[k] = th * (q[k] - ub[k]) + ub[k];
     }
     double pq = pressure_pp(q, g);
     if (pq < ORC_P_MIN) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+        orc_pp_fired[1]++;
         double p_bar = fmax(pressure_pp(ub, g), ORC_P_MIN);
         double th = fmax(fmin((p_bar - ORC_P_MIN) / (p_bar - pq), 1.0), 0.0);
         for (int k = 0; k < 4; ++k) q[k] = th * (q[k] - ub[k]) + ub[k];

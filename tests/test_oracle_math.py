@@ -397,9 +397,39 @@ def test_pp_limit(oracle_mod):
     p = (G - 1) * (r[3] - 0.5 * (r[1] ** 2 + r[2] ** 2) / r[0])
     assert p >= 1e-12 * (1 - 1e-6)
     # the limited state lies on the segment between q and the average
-    t = (r - ub) / (q - ub)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        t = (r - ub) / (q - ub)
     t = t[np.isfinite(t)]
     assert np.allclose(t, t[0]) and 0 <= t[0] <= 1
+
+
+def test_pp_limit_closed_forms(oracle_mod):
+    """weno.rs:327-352.  Density is linear along the segment q -> u_bar, so the
+    density pass lands on RHO_MIN = 1e-12 up to the rounding of O(1) values;
+    pressure is linear along it when q and u_bar share density and momentum,
+    and then the pressure pass lands on P_MIN = 1e-12.  A theta that drops the
+    floor, over-limits or limits toward the wrong end misses these by ~1e-12."""
+    tol = 4e-16 * 4.0
+    # density pass only (no momentum, positive energy: the pressure stays positive)
+    ub = np.array([1.0, 0.0, 0.0, 2.5])
+    for rho_q in (-0.5, 0.0, 5e-13):
+        r = oracle_mod.pp_limit(np.array([rho_q, 0.0, 0.0, 2.0]), ub, G)
+        assert abs(r[0] - 1e-12) <= tol, r
+        assert r[1] == 0.0 and r[2] == 0.0 and 2.0 <= r[3] <= 2.5
+    # an average below the floor gives theta = 0: the face takes the average
+    ub_low = np.array([1e-13, 0.0, 0.0, 1.0])
+    np.testing.assert_array_equal(oracle_mod.pp_limit(np.array([-0.3, 0.2, 0.1, 0.5]), ub_low, G), ub_low)
+    # pressure pass with the same density and momentum: p lands on 1e-12
+    for (rho, u, v, p) in ((1.0, 0.3, -0.2, 1.0), (0.2, -1.5, 0.7, 0.05), (3.0, 0.0, 2.0, 10.0)):
+        ub = cons(rho, u, v, p)
+        kin = 0.5 * (ub[1] ** 2 + ub[2] ** 2) / ub[0]
+        for p_q in (-0.5 * p, -1e-3, 0.0, 1e-13):
+            q = ub.copy()
+            q[3] = kin + p_q / (G - 1)
+            r = oracle_mod.pp_limit(q, ub, G)
+            pr = (G - 1) * (r[3] - 0.5 * (r[1] ** 2 + r[2] ** 2) / r[0])
+            assert abs(pr - 1e-12) <= tol * max(p, kin) * 4, (pr, p_q)
+            assert r[0] == ub[0]
 
 
 # ----------------------------------------------------------- full RHS ----

@@ -33,6 +33,7 @@ UNITS = [
     ("wenox_sweep.hip", "sweep_exact.o", ["-DWX_FAST=0", "-ffp-contract=off"]),
     ("wenox_sweep.hip", "sweep_fast.o", ["-DWX_FAST=1", "-ffp-contract=fast"]),
     ("wenox_objective.hip", "objective.o", ["-ffp-contract=off"]),
+    ("wenox_sums.hip", "sums.o", ["-ffp-contract=off"]),
     ("wenox_abi.cpp", "abi.o", ["-x", "hip", "-ffp-contract=off"]),
     ("wenox_batch.cpp", "batch.o", ["-x", "hip", "-ffp-contract=off"]),
 ]

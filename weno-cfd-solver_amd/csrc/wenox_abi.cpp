@@ -16,6 +16,7 @@
 
 #include "../../include/wenox.h"
 #include "wenox_objective.h"
+#include "wenox_sums.h"
 #include "wenox_ops.h"
 #include "wenox_sweep.h"
 
@@ -450,6 +451,7 @@ struct wenox_solver {
     double cfl_rho_floor = 1e-14;   // Rie_prob_2d_rs.py:314
     double cfl_cmax_eps = 1e-16;    // Rie_prob_2d_rs.py:315
     double pfloor = 0.0;            // > 0: the twin's pressure floor after every step
+    double* sums = nullptr;  // compensated sums: SUM_BLOCKS x 16 block partials, then 8 results
 };
 
 namespace {
@@ -659,7 +661,8 @@ extern "C" int wenox_solver_create(const wenox_solver_desc* desc, wenox_solver**
               hipMalloc(&s->L, std::max<size_t>(nL, 4) * 8) == hipSuccess &&
               hipMalloc(&s->work, std::max<size_t>(s->work_n, 4) * 8) == hipSuccess &&
               hipMalloc(&s->ctl, sizeof(StepCtl)) == hipSuccess &&
-              hipMalloc(&s->cfl_part, cfl_slots(d) * sizeof(double)) == hipSuccess;
+              hipMalloc(&s->cfl_part, cfl_slots(d) * sizeof(double)) == hipSuccess &&
+              hipMalloc(&s->sums, ((size_t)SUM_BLOCKS * 16 + 8) * sizeof(double)) == hipSuccess;
     if (ok && fused)
         ok = state_buf(3, &s->UT, SPT) && state_buf(4, &s->AT, SPT) && state_buf(5, &s->BT, SPT) &&
              hipMalloc(&s->DyT, std::max<size_t>(nDyT, 4) * 8) == hipSuccess;
@@ -702,6 +705,7 @@ extern "C" void wenox_solver_destroy(wenox_solver* s)
         if (p) (void)hipFree(p);
     if (s->ctl) (void)hipFree(s->ctl);
     if (s->cfl_part) (void)hipFree(s->cfl_part);
+    if (s->sums) (void)hipFree(s->sums);
     if (s->flags) (void)hipFree(s->flags);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -770,6 +774,21 @@ extern "C" int wenox_solver_upload(wenox_solver* s, const wenox_view3* q, double
     }
     HIPCHK(hipStreamSynchronize(s->stream));
     HIPCHK(hipGetLastError());
+    return WENOX_OK;
+}
+
+extern "C" int wenox_solver_conserved_sums(wenox_solver* s, double* out, int n_out)
+{
+    if (!s || !out) return fail(WENOX_E_ARG, "wenox_solver_conserved_sums: null argument");
+    if (n_out != 4 && n_out != 8) return fail(WENOX_E_ARG, "wenox_solver_conserved_sums: n_out must be 4 or 8");
+    HIPCHK(hipSetDevice(s->device));
+    double* res = s->sums + (size_t)SUM_BLOCKS * 16;
+    conserved_sums(s->U, s->split, s->d.nx, s->d.ny, s->ld, plane(s), s->sums, res, s->stream);
+    double h[8];
+    HIPCHK(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipGetLastError());
+    for (int k = 0; k < n_out; ++k) out[k] = h[k];
     return WENOX_OK;
 }
 
@@ -856,7 +875,9 @@ extern "C" int wenox_solver_run(wenox_solver* s, int64_t max_steps, double t_fin
         HIPCHK(hipStreamEndCapture(s->stream, &s->graph));
         HIPCHK(hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0));
     }
-    if (s->d.use_graph && !s->exec_n && max_steps >= GRAPH_STEPS) {
+    // both graphs are captured on the first run() call, whatever its length,
+    // so that no later call pays for a capture
+    if (s->d.use_graph && !s->exec_n) {
         HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         for (int64_t i = 0; i < GRAPH_STEPS; ++i) enqueue_step(s);
         HIPCHK(hipStreamEndCapture(s->stream, &s->graph_n));

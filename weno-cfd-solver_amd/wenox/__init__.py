@@ -116,6 +116,15 @@ class Solver:
         _lib.check(_lib.load().wenox_solver_rhs(self._h, out.ctypes.data))
         return out
 
+    def conserved_sums(self, magnitudes: bool = False) -> np.ndarray:
+        """Compensated FP64 sums of rho, rho u, rho v, E over the interior
+        cells of the current state (times dx**2: the totals); with
+        magnitudes=True also the sums of their absolute values (8 values)."""
+        n = 8 if magnitudes else 4
+        out = np.empty(n)
+        _lib.check(_lib.load().wenox_solver_conserved_sums(self._h, out.ctypes.data, n))
+        return out
+
     @property
     def ms_per_step(self) -> float:
         return _lib.load().wenox_solver_last_ms_per_step(self._h)

@@ -89,6 +89,7 @@ def load():
         "wenox_solver_run": ([dp, ctypes.c_int64, d, ctypes.POINTER(d),
                               ctypes.POINTER(ctypes.c_int64)], i),
         "wenox_solver_rhs": ([dp, dp], i),
+        "wenox_solver_conserved_sums": ([dp, dp, i], i),
         "wenox_solver_last_ms_per_step": ([dp], d),
         "wenox_solver_profile": ([dp, i, dp, i], i),
         "wenox_solver_plan": ([dp, ctypes.POINTER(i), ctypes.POINTER(i)], i),
