@@ -7,6 +7,13 @@ pass and HLLE, SSP-RK3 at CFL 0.4 to t_end.  For each grid it reports the
 L1 and L-infinity errors of the density against the exact solution (cell
 averages by 4x4 Gauss-Legendre quadrature), the FP32-split path's errors and
 its distance from the FP64 run, and the observed order between grids.
+
+The exact solution is the free-space vortex placed by nearest periodic image,
+so it is not smooth across the lines halfway between images (the velocity
+perturbation is ~2e-5 there): besides the whole-domain errors the study
+reports ``L1_core``, the mean error over the cells within distance 2.5 of
+the vortex centre: the seam lies 5 away, and sound (c ~ 1.2 in the frame
+moving with the flow) from it does not reach that disc before t = 2.
 """
 from __future__ import annotations
 
@@ -30,18 +37,38 @@ def _run(n, t_end, precision, variant, cfl):
     return q, T, k, el
 
 
+def _core_mask(n, t, L=10.0, radius=2.5):
+    """Cells whose centre lies within `radius` of the vortex centre at time t
+    (nearest periodic image)."""
+    h = L / n
+    c = (np.arange(n) + 0.5) * h
+    xc = (5.0 + t) % L
+    d = (c - xc + 0.5 * L) % L - 0.5 * L
+    return (d[:, None] ** 2 + d[None, :] ** 2) <= radius ** 2
+
+
+def _orders(rows, prec):
+    for a, b in zip(rows, rows[1:]):
+        r = b["n"] / a["n"]
+        for key in ("L1", "Linf", "L1_core"):
+            b[prec][f"order_{key}"] = math.log(a[prec][key] / b[prec][key]) / math.log(r)
+
+
 def vortex_study(sizes=(256, 512, 1024, 2048, 4096), t_end=2.0, cfl=0.4, variant="fast",
                  precisions=("fp64", "fp32split")):
     rows = []
     for n in sizes:
         exact = problems.vortex_cell_average(n, t_end)[..., 0]
+        core = _core_mask(n, t_end)
         row = {"n": n}
         ref = None
         for prec in precisions:
             q, T, k, el = _run(n, t_end, prec, variant, cfl)
             rho = q[..., 0]
             err = np.abs(rho - exact)
-            row[prec] = {"L1": float(err.mean()), "Linf": float(err.max()), "T": T, "steps": k,
+            row[prec] = {"L1": float(err.mean()), "Linf": float(err.max()),
+                         "L1_core": float(err[core].mean()), "Linf_core": float(err[core].max()),
+                         "T": T, "steps": k,
                          "seconds": el, "cell_updates_per_s": n * n * k / el}
             if prec == "fp64":
                 ref = rho
@@ -49,10 +76,7 @@ def vortex_study(sizes=(256, 512, 1024, 2048, 4096), t_end=2.0, cfl=0.4, variant
                 row[prec]["rel_L1_vs_fp64"] = float(np.abs(rho - ref).sum() / np.abs(ref).sum())
         rows.append(row)
     for prec in precisions:
-        for a, b in zip(rows, rows[1:]):
-            r = b["n"] / a["n"]
-            b[prec]["order_L1"] = math.log(a[prec]["L1"] / b[prec]["L1"]) / math.log(r)
-            b[prec]["order_Linf"] = math.log(a[prec]["Linf"] / b[prec]["Linf"]) / math.log(r)
+        _orders(rows, prec)
     return rows
 
 
@@ -76,10 +100,7 @@ def main(argv=None):
               f"fp32split L1={r['fp32split']['L1']:.3e} rel_vs_fp64={r['fp32split'].get('rel_L1_vs_fp64', 0):.2e}",
               file=sys.stderr, flush=True)
     for prec in ("fp64", "fp32split"):
-        for a, b in zip(rows, rows[1:]):
-            rr = b["n"] / a["n"]
-            b[prec]["order_L1"] = math.log(a[prec]["L1"] / b[prec]["L1"]) / math.log(rr)
-            b[prec]["order_Linf"] = math.log(a[prec]["Linf"] / b[prec]["Linf"]) / math.log(rr)
+        _orders(rows, prec)
     out = json.dumps({"study": "isentropic vortex, component-wise WENO5 + Gauss pass + HLLE, SSP-RK3",
                       "t_end": args.t_end, "cfl": args.cfl, "rows": rows})
     if args.out:
