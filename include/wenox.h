@@ -103,6 +103,24 @@ int wenox_hllc_hlle_y(const wenox_view3 *q_l, const wenox_view3 *f_l, const weno
 int wenox_l_local(const wenox_view3 *u, double dx, double gamma, int force_hlle, double jp_lo,
                   double jp_hi, int cons_switch, double *out);
 
+/* l_local on device arrays, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * u: a view of a (m, n, c>=4) array in device memory (any element strides);
+ * out: a C-contiguous (m-6, n-6, 4) device array; stream: a hipStream_t (NULL:
+ * the default stream).  Same results as wenox_l_local (the variant set by
+ * wenox_set_variant).  The kernels are enqueued on `stream` and the call
+ * returns without waiting.  The face arrays live in one 64 MiB device array
+ * of the library (per device, held until the process ends), so grids are
+ * limited to those whose face arrays fit (about 500 x 500 interior cells;
+ * larger ones fail with WENOX_E_SHAPE).  Every call on a device shares that
+ * array: calls that overlap on one device (other streams, other threads)
+ * race.  The first call on a device looks the array up
+ * (hipGetSymbolAddress); make one call before a graph capture if the capture
+ * must not include it. */
+int wenox_l_local_noalloc(const wenox_view3 *u, double dx, double gamma, int force_hlle, double jp_lo,
+                          double jp_hi, int cons_switch, double *out, void *stream);
+
 /* ---- 2. device-resident solver --------------------------------------------- */
 
 #define WENOX_BC_MIRROR 0    /* Rie_prob_2d_rs.py:123-152 apply_bc_zero_gradient (even mirror) */

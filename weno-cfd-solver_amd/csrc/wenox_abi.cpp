@@ -413,6 +413,92 @@ extern "C" int wenox_l_local(const wenox_view3* u, double dx, double gamma, int 
 }
 
 // ==========================================================================
+// l_local on device arrays with no allocation
+// ==========================================================================
+// The face arrays of the reference pass structure live in one file-scope
+// device array of fixed size instead of the drop-in arena, so that the call
+// allocates nothing (for callers under hipGraph capture or in loops that
+// must not allocate).  Every device holds its own copy for the life of the
+// process; host code reaches it only through the address hipGetSymbolAddress
+// returns for the current device.
+namespace {
+
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = ((size_t)64 << 20) / sizeof(double);   // 64 MiB
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (looked up once per device).
+int scratch_for_current_device(double** out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(WENOX_E_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int rc = require_device();
+        if (rc) return rc;
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(WENOX_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return WENOX_OK;
+}
+
+}  // namespace
+
+extern "C" int wenox_l_local_noalloc(const wenox_view3* u, double dx, double gamma, int force_hlle,
+                                     double jp_lo, double jp_hi, int cons, double* out, void* stream)
+{
+    const char* fn = "l_local_noalloc";
+    if (bad_view(u) || !u->data) return fail(WENOX_E_ARG, "l_local_noalloc: null array");
+    const int64_t m = u->dim[0], n = u->dim[1];
+    if (u->dim[2] < 4) return shape_err(fn, "u must have at least 4 channels, got " + dims(u));
+    if (m < 6 || n < 6) return shape_err(fn, "u needs m >= 6 and n >= 6 (3 ghost cells per side), got " + dims(u));
+    if (!force_hlle && (m < 8 || n < 8))
+        return shape_err(fn, "the HLLC/HLLE blend's sensor needs at least 2 x 2 interior cells, got " + dims(u));
+    const int64_t nx = m - 6, ny = n - 6;
+    if (nx * ny > 0 && !out) return fail(WENOX_E_ARG, "l_local_noalloc: null output");
+    const size_t need = rhs_unfused_scratch(nx, ny);
+    if (need > NOALLOC_SCRATCH_DOUBLES)
+        return shape_err(fn, "grid " + dims(u) + " needs " + std::to_string(need * 8 >> 20) +
+                                 " MiB of scratch; this entry point has " +
+                                 std::to_string(NOALLOC_SCRATCH_DOUBLES * 8 >> 20) +
+                                 " MiB (use wenox_l_local or the solver for larger grids)");
+    double* work = nullptr;
+    int rc = scratch_for_current_device(&work);
+    if (rc) return rc;
+    if (nx * ny == 0) return WENOX_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const Ops* ops = ops_for(g_variant);
+    const Lay lu{u->stride[0], u->stride[1], u->stride[2]};
+    const int64_t p1 = std::max((m - 5) * n, m * (n - 5)) * 4;
+    const int64_t fxs = 4 * (nx + 1) * ny, fys = 4 * nx * (ny + 1), fm = std::max(fxs, fys);
+    double* pl = work;
+    double* pr = pl + p1;
+    double* f0 = pr + p1;
+    double* fx = f0 + 4 * fm;
+    double* fy = fx + fxs;
+    double* pbar = fy + fys;
+    const int kind = force_hlle ? 1 : 2;
+    Lay lx = lay_aos4(nx + 1, ny), ly = lay_aos4(nx, ny + 1);
+    ops->weno(u->data, lu, m, n, 0, cons, gamma, pl, pr, f0, f0 + fm, f0 + 2 * fm, f0 + 3 * fm, lx, s);
+    ops->riemann(f0, f0 + fm, f0 + 2 * fm, f0 + 3 * fm, lx, nx + 1, ny, 0, kind, gamma, jp_lo, jp_hi, pbar,
+                 fx, lx, s);
+    ops->weno(u->data, lu, m, n, 1, cons, gamma, pl, pr, f0, f0 + fm, f0 + 2 * fm, f0 + 3 * fm, ly, s);
+    ops->riemann(f0, f0 + fm, f0 + 2 * fm, f0 + 3 * fm, ly, nx, ny + 1, 1, kind, gamma, jp_lo, jp_hi, pbar,
+                 fy, ly, s);
+    ops->divergence(fx, lx, fy, ly, nx, ny, dx, out, lay_aos4(nx, ny), s);
+    HIPCHK(hipGetLastError());
+    return WENOX_OK;
+}
+
+// ==========================================================================
 // Device-resident solver
 // ==========================================================================
 struct wenox_solver {

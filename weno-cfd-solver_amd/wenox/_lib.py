@@ -82,6 +82,7 @@ def load():
         "wenox_hllc_hlle_x": ([V, V, V, V, d, d, d, i, dp], i),
         "wenox_hllc_hlle_y": ([V, V, V, V, d, d, d, i, dp], i),
         "wenox_l_local": ([V, d, d, i, d, d, i, dp], i),
+        "wenox_l_local_noalloc": ([V, d, d, i, d, d, i, dp, dp], i),
         "wenox_solver_create": ([ctypes.POINTER(SolverDesc), ctypes.POINTER(ctypes.c_void_p)], i),
         "wenox_solver_destroy": ([dp], None),
         "wenox_solver_upload": ([dp, V, d], i),
