@@ -1,30 +1,32 @@
 """Throughput of the device-resident WENO5/HLLC SSP-RK3 step on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dmr|quad4096|kh8192]
-                [--engine fused|unfused] [--variant fast|exact] [--no-cpu] [--no-extra]
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+                [--workload quad4096|dmr|kh8192|batch512] [--engine fused|unfused]
+                [--variant fast|exact] [--precision fp64|fp32split] [--no-cpu] [--no-extra]
 
-A step is one full SSP-RK3 time step over the whole grid -- CFL reduction,
-3 x (x sweep, y sweep, RK combine, ghost fill), time advance -- replayed as one
-hipGraph with the state resident in HBM.
+A step is one full SSP-RK3 time step over the whole grid -- CFL dt, 3 x (x and
+y sweeps, RK combine with the ghost cells), time advance -- replayed from a
+hipGraph with the state resident in HBM; both step graphs (one step, eight
+steps) are captured by the first, untimed run() call.
 
-Default workload: double Mach reflection, 1920 x 480 cells on [0,4]x[0,1],
-characteristic WENO5 + HLLC (sensor window (1e4, 1.1e4): phi = 1), SSP-RK3,
-FP64, synthetic initial data built on the host.  It runs at CFL 0.1, the
-value of the reference's own DMR driver (double_mach_reflection.py:17): at
-CFL 0.4 the reference scheme turns non-finite near the top boundary within
-~60 steps (the CPU tests show it on the oracle), and a throughput measured on
-a non-finite state would be meaningless; the cost of a step does not depend
-on the CFL number.  Prints one JSON line.
+Default workload (the one the throughput target is quoted on): the
+four-quadrant Riemann problem on [0,1]^2 at 4096 x 4096 cells, FP64,
+characteristic WENO5 + HLLC (sensor window (1e4, 1.1e4): phi = 1), SSP-RK3 at
+CFL 0.4, mirror ("transmissive") ghost cells, synthetic initial data built on
+the host.  Before timing, the sweep tiling is picked by the solver's
+autotuner.  Prints one JSON line.
 
-Before timing, the sweep tile lengths are picked by the solver's autotuner
-(measured kernel times over a set of rows-per-workgroup candidates).
-
-`value` is the FP64 throughput.  The same line carries the FP32-split
-throughput on the same workload with its relative L1 density error against
-the FP64 run after the same steps, the 4096^2 four-quadrant grid, per-kernel
-device times from HIP events (the roofline block), and the CPU oracle timed
-on the host (cpu_baseline: all of the process's cores up to 16, with the
-single-thread rate of the reference's own threading beside it).
+`value` is the FP64 fast-variant throughput.  Beside it the line carries: the
+exact variant (the reference's operation order, bitwise equal to the oracle)
+on the same grid; FP32-split on the same grid with its relative L1 and
+L-infinity density distance from the FP64 run; the scheme variants
+{characteristic, component-wise} x {HLLC, HLLC/HLLE blend}; the double Mach
+reflection 1920 x 480, Kelvin-Helmholtz 8192^2 and the 512-instance batched
+workload (FP64 and FP32-split each); the drop-in l_local_py call; the jp_cri
+tuning objective; the roofline blocks (per-kernel device times from HIP
+events on the solver's stream, HBM traffic and VALU counts from the counter
+profiles under profiles/); and the CPU oracle timed on the host's cores
+(cpu_baseline).
 """
 import argparse
 import json
@@ -44,8 +46,12 @@ VALU_WAVE_INSTR_PEAK = 256 * 4 * 2.4e9 / 4   # CUs x SIMDs x clock / 4 cycles pe
 WORKLOADS = {
     "dmr": dict(name="double_mach_reflection", nx=1920, ny=480, cons=False, force_hlle=False,
                 jp=(1e4, 1.1e4), bc="dmr", cfl=0.1),
+    # the reference scheme (characteristic WENO5 + HLLC) turns this problem
+    # non-finite between steps 130 and 140 at CFL 0.4 (the exact variant, bitwise
+    # the oracle's arithmetic, at the same step): timed runs restart from the
+    # initial state at most every `window` steps (see run_gpu)
     "quad4096": dict(name="four_quadrant_riemann", nx=4096, ny=4096, cons=False, force_hlle=False,
-                     jp=(1e4, 1.1e4), bc="mirror", cfl=0.4),
+                     jp=(1e4, 1.1e4), bc="mirror", cfl=0.4, window=120),
     "kh8192": dict(name="kelvin_helmholtz", nx=8192, ny=8192, cons=False, force_hlle=False,
                    jp=(1e4, 1.1e4), bc="periodic", cfl=0.4),
     # 512 independent four-quadrant instances, sizes from {64,96,128,192,256,384}^2
@@ -88,13 +94,25 @@ def run_gpu(w, q, dx, steps, warmup, engine, variant, precision, profile=True, t
             elif tune:
                 tuned = s.autotune()
         s.run(nstep=warmup)
-        t0 = time.perf_counter()
-        T, k = s.run(nstep=steps)
-        el = time.perf_counter() - t0
-        assert k == steps, (k, steps)
+        # A workload with a `window` is timed in segments of at most that many
+        # steps, each from the initial state (re-uploaded between segments,
+        # outside the timed segments): its scheme turns the state non-finite
+        # after about that many steps.
+        window = w.get("window")
+        el, k, T = 0.0, 0, 0.0
+        while k < steps:
+            seg = steps - k if not window else min(window, steps - k)
+            if window:
+                s.upload(q)
+            t0 = time.perf_counter()
+            T, kk = s.run(nstep=seg)
+            el += time.perf_counter() - t0
+            k += kk
+            assert kk == seg, (kk, seg)
         state = s.download()[3:-3, 3:-3, :4]
         out = dict(seconds=el, steps=k, value=w["nx"] * w["ny"] * k / el, dev_ms_per_step=s.ms_per_step,
-                   state=state, finite=bool(np.isfinite(state).all()), t=T, variant=variant)
+                   state=state, finite=bool(np.isfinite(state).all()), t=T, variant=variant,
+                   segments=-(-steps // window) if window else 1)
         if engine == "fused":
             out["chunks"] = list(s.tiling)
             if tuned:
@@ -112,14 +130,20 @@ def run_gpu(w, q, dx, steps, warmup, engine, variant, precision, profile=True, t
         s.close()
 
 
-def run_batch(w, steps, warmup, precision, chunk=16):
+# the batched workload's configured state ranges: rho~U[0.1,2], p~U[0.02,2], u,v~U[-1.5,1.5]
+CONFIG_RANGES = dict(rho=(0.1, 2.0), p=(0.02, 2.0), vel=1.5)
+
+
+def run_batch(w, steps, warmup, precision, chunk=16, ranges=None):
     """The batched engine: every phase of a step is one launch over all
-    instances (per-instance dt); value = sum of instance cells x steps / wall."""
+    instances (per-instance dt); value = sum of instance cells x steps / wall.
+    ranges: the instances' state ranges (default: the narrowed ranges of
+    problems.random_riemann_instance)."""
     import numpy as np
     from wenox import Batch, problems
     rng = np.random.default_rng(20261203)
     sizes = [(int(n), int(n)) for n in rng.choice(w["sizes"], size=w["count"])]
-    inst = [problems.random_riemann_instance(rng, nx, ny) for nx, ny in sizes]
+    inst = [problems.random_riemann_instance(rng, nx, ny, **(ranges or {})) for nx, ny in sizes]
     cells = sum(nx * ny for nx, ny in sizes)
     with Batch(sizes, [d for _, d in inst], cfl=w["cfl"], cons_switch=w["cons"], force_hlle=w["force_hlle"],
                jp_cri=w["jp"], bc=w["bc"], variant="fast", precision=precision, chunk=chunk) as b:
@@ -171,31 +195,51 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(q, dx, w, budget_s=10.0):
-    """The oracle (C restatement of the reference Rust RHS) driving the
-    reference's SSP-RK3 stage combination on the same state, timed on the
-    host: on all of this process's cores (up to 16, OpenMP over the RHS
-    passes' outer loops; the reported figure) and on one thread as the
-    reference runs (reported beside it)."""
+def _host_cores():
+    """CPUs this process may run on: its affinity set, capped by a cgroup
+    CPU quota when one is set.  Returns (cores, affinity, quota or None)."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = max(1, int(int(q) // int(period)))
+    except (OSError, ValueError):
+        pass
+    return (min(aff, quota) if quota else aff), aff, quota
+
+
+def cpu_baseline(q, dx, w, budget_s=10.0, slab=256):
+    """The oracle (C restatement of the reference Rust RHS, OpenMP over the
+    outer loops of its passes) driving the reference's SSP-RK3 stage
+    combination, timed on the host: on every core this process may use (the
+    reported figure) and on one thread as the reference runs (beside it).
+    The sample is whole steps on a slab of `slab` x-rows through the middle
+    of the workload's state, with its 3-cell halo and the full y extent."""
     import oracle
     oracle.lib()
     nx, ny = w["nx"], w["ny"]
-    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    rows = min(slab, nx)
+    i0 = 3 + (nx - rows) // 2
+    sub = q[i0 - 3:i0 + rows + 3].copy()
+    ws = dict(w, nx=rows)
+    cores, aff, quota = _host_cores()
     res = {}
     try:
         for t in sorted({cores, 1}, reverse=True):
             oracle.set_threads(t)
-            res[t] = _cpu_steps(q, dx, w, budget_s)
+            res[t] = _cpu_steps(sub, dx, ws, budget_s)
     finally:
         oracle.set_threads(1)
     steps, el = res[cores]
     s1, e1 = res[1]
-    return dict(value=nx * ny * steps / el, unit="cell-updates/s", cores=cores, kind="port",
-                sample=f"{steps} SSP-RK3 step(s) of the {nx}x{ny} {w['name']} state "
-                       f"(3 oracle l_local calls + numpy stage combines each) on {cores} thread(s), "
-                       f"{el:.1f} s",
-                one_thread=dict(value=nx * ny * s1 / e1, steps=s1, seconds=round(e1, 2)),
-                host=_cpu_model())
+    return dict(value=rows * ny * steps / el, unit="cell-updates/s", cores=cores, kind="port",
+                sample=f"{steps} SSP-RK3 step(s) (3 oracle l_local calls + numpy stage combines each) of the "
+                       f"{rows} x {ny} slab at x-rows {i0 - 3}..{i0 + rows - 4} of the {nx}x{ny} {w['name']} "
+                       f"state on {cores} thread(s), {el:.1f} s",
+                one_thread=dict(value=rows * ny * s1 / e1, steps=s1, seconds=round(e1, 2)),
+                host=_cpu_model(), cpus_affinity=aff, cpu_quota=quota)
 
 
 def dropin_rate(q, dx, w, calls=5):
@@ -394,13 +438,71 @@ def copy_calibration(mib=512, reps=5):
           flush=True)
 
 
+def _dist(a, b):
+    """Relative L1 and L-infinity distance of a from b."""
+    import numpy as np
+    d = np.abs(a - b)
+    return {"rel_l1": float(d.sum() / np.abs(b).sum()), "rel_linf": float(d.max() / np.abs(b).max())}
+
+
+METRIC = "cell-updates/s FP64 vs FP32-split @<=1e-6 L1 err; % of HBM roofline"
+SPLIT_DTYPE = "f32 bulk, f32 hi+lo state"
+
+
+def split_block(w, q, dx, steps, warmup, r64):
+    """FP32-split on the same state and steps as the FP64 run r64."""
+    r32 = run_gpu(w, q, dx, steps, warmup, "fused", "fast", "fp32split")
+    return {"value": r32["value"], "unit": "cell-updates/s", "dtype": SPLIT_DTYPE,
+            "speedup_vs_fp64": r32["value"] / r64["value"],
+            "ms_per_step": r32["seconds"] / r32["steps"] * 1e3,
+            "density_vs_fp64": _dist(r32["state"][..., 0], r64["state"][..., 0]),
+            "after_steps": warmup + steps, "state_finite": r32["finite"], "chunks": r32.get("chunks"),
+            "roofline": roofline(r32, w, split=True)}
+
+
+def workload_block(key, steps, warmup):
+    """FP64 (fast, autotuned; exact on the same tiling) and FP32-split on one
+    more workload."""
+    w = WORKLOADS[key]
+    q, dx = make_state(w)
+    r64 = run_gpu(w, q, dx, steps, warmup, "fused", "fast", "fp64")
+    rex = run_gpu(w, q, dx, steps, warmup, "fused", "exact", "fp64", profile=False, chunks=r64["chunks"])
+    out = {"workload": describe(w), "steps": steps, "warmup": warmup,
+           "value_fp64": r64["value"], "ms_per_step_fp64": r64["seconds"] / r64["steps"] * 1e3,
+           "value_fp64_exact": rex["value"], "chunks": r64["chunks"], "state_finite": r64["finite"] and rex["finite"],
+           "roofline_fp64": roofline(r64, w)}
+    out["fp32_split"] = split_block(w, q, dx, steps, warmup, r64)
+    return out
+
+
+def batch_block(w, steps, warmup):
+    r = run_batch(w, steps, warmup, "fp64")
+    r32 = run_batch(w, steps, warmup, "fp32split")
+    # the configured (wider) state ranges, for the instability evidence: the
+    # reference scheme turns most of those instances non-finite
+    rc = run_batch(w, steps, warmup, "fp64", ranges=CONFIG_RANGES)
+    return {"workload": describe(w), "cells": r["cells"], "steps": steps, "warmup": warmup,
+            "value_fp64": r["value"], "ms_per_step_fp64": r["ms_per_step"], "chunk": r["chunk"],
+            # the reference scheme itself turns some instances non-finite (the oracle
+            # reproduces it); they keep stepping, so the count does not change the cost
+            "instances_non_finite": r["non_finite"],
+            "roofline_fp64": {"bound": "hbm", "achieved": r["value"] * BYTES_PER_CELL_UPDATE / 1e9,
+                              "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                              "frac": r["value"] * BYTES_PER_CELL_UPDATE / HBM_PEAK, "traffic": None},
+            "state_ranges": "rho~U[1,2], p~U[1,2], u,v~U[-0.5,0.5] per quadrant",
+            "configured_ranges_fp64": {"state_ranges": "rho~U[0.1,2], p~U[0.02,2], u,v~U[-1.5,1.5]",
+                                       "value": rc["value"], "instances_non_finite": rc["non_finite"]},
+            "fp32_split": {"value": r32["value"], "speedup_vs_fp64": r32["value"] / r["value"],
+                           "instances_non_finite": r32["non_finite"]}}
+
+
 def main():
     import numpy as np
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="dmr", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="quad4096", choices=sorted(WORKLOADS))
     ap.add_argument("--engine", default="fused", choices=["fused", "unfused"])
     ap.add_argument("--variant", default="fast", choices=["fast", "exact"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32split"],
@@ -433,29 +535,20 @@ def main():
         print(json.dumps({"tuning_objective": obj}), flush=True)
         return
     if args.workload == "batch512":
-        r = run_batch(w, args.steps, args.warmup, args.precision)
-        res = {"metric": "cell-updates/s FP64 vs FP32-split @<=1e-6 L1 err; % of HBM roofline",
-               "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
-               "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
-               "scaling": "weak", "vs_baseline": None,
-               "dtype": "f64" if args.precision == "fp64" else "f32 bulk, f32 hi+lo state",
-               "data": "synthetic", "config": {"workload": describe(w), "cells": r["cells"], "chunk": r["chunk"]},
-               # the reference scheme itself turns some instances non-finite (the oracle
-               # reproduces it); they keep stepping, so the count does not change the cost
-               "instances_non_finite": r["non_finite"],
-               "roofline": {"bound": "hbm", "achieved": r["value"] * BYTES_PER_CELL_UPDATE / 1e9,
-                            "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                            "frac": r["value"] * BYTES_PER_CELL_UPDATE / HBM_PEAK, "traffic": None}}
-        if not args.no_extra and args.precision == "fp64":
-            r32 = run_batch(w, args.steps, args.warmup, "fp32split")
-            res["fp32_split"] = {"value": r32["value"], "speedup_vs_fp64": r32["value"] / r["value"]}
+        b = batch_block(w, args.steps, args.warmup)
+        res = {"metric": METRIC, "value": b["value_fp64"], "unit": "cell-updates/s", "n_gpus": 1,
+               "steps": args.steps, "warmup": args.warmup, "ms_per_step": b["ms_per_step_fp64"],
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+               "data": "synthetic", "config": {"workload": describe(w), "cells": b["cells"], "chunk": b["chunk"]},
+               "instances_non_finite": b["instances_non_finite"], "roofline": b["roofline_fp64"],
+               "fp32_split": b["fp32_split"]}
         print(json.dumps(res), flush=True)
         return
     q, dx = make_state(w)
     r64 = run_gpu(w, q, dx, args.steps, args.warmup, args.engine, args.variant, args.precision,
                   chunks=args.chunks)
     res = {
-        "metric": "cell-updates/s FP64 vs FP32-split @<=1e-6 L1 err; % of HBM roofline",
+        "metric": METRIC,
         "value": r64["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -465,55 +558,47 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32 bulk, f32 hi+lo state",
+        "dtype": "f64" if args.precision == "fp64" else SPLIT_DTYPE,
         "data": "synthetic",
         "config": {"workload": describe(w), "engine": args.engine, "variant": args.variant,
                    "chunks": r64.get("chunks"), "combine_cols": r64.get("combine_cols"),
                    "autotune_ms": r64.get("autotune_ms")},
         "state_finite": r64["finite"],
+        "timed_segments": r64["segments"],
+        "dev_ms_per_step": r64["dev_ms_per_step"],
         "roofline": roofline(r64, w, split=args.precision != "fp64"),
     }
     if not args.no_extra and args.engine == "fused" and args.precision == "fp64":
-        r32 = run_gpu(w, q, dx, args.steps, args.warmup, "fused", "fast", "fp32split")
-        rho64, rho32 = r64["state"][..., 0], r32["state"][..., 0]
-        res["fp32_split"] = {
-            "value": r32["value"], "unit": "cell-updates/s", "dtype": "f32 bulk, f32 hi+lo state",
-            "speedup_vs_fp64": r32["value"] / r64["value"],
-            "rel_l1_density_vs_fp64": float(np.abs(rho32 - rho64).sum() / np.abs(rho64).sum()),
-            "after_steps": args.warmup + args.steps,
-            "state_finite": r32["finite"],
-            "chunks": r32.get("chunks"),
-            "roofline": roofline(r32, w, split=True),
-        }
-        if args.workload == "dmr":
-            wq = WORKLOADS["quad4096"]
-            qq, dq = make_state(wq)
-            rq = run_gpu(wq, qq, dq, 20, 3, "fused", args.variant, "fp64")
-            rq32 = run_gpu(wq, qq, dq, 20, 3, "fused", "fast", "fp32split")
-            res["quad4096"] = {"workload": describe(wq), "value_fp64": rq["value"],
-                               "value_fp32split": rq32["value"],
-                               "rel_l1_density_fp32split_vs_fp64": float(
-                                   np.abs(rq32["state"][..., 0] - rq["state"][..., 0]).sum()
-                                   / np.abs(rq["state"][..., 0]).sum()),
-                               "state_finite": rq["finite"] and rq32["finite"],
-                               "chunks_fp64": rq.get("chunks"), "chunks_fp32split": rq32.get("chunks"),
-                               "roofline_fp64": roofline(rq, wq),
-                               "roofline_fp32split": roofline(rq32, wq, split=True)}
-            # the scheme variants of the same grid (FP64, the tiling tuned above):
-            # {characteristic, component-wise} x {HLLC (phi = 1), HLLC/HLLE blend jp (0.5, 2)}
+        # the parity variant on the same grid and tiling
+        rex = run_gpu(w, q, dx, args.steps, args.warmup, "fused", "exact", "fp64", chunks=r64.get("chunks"))
+        res["exact_variant"] = {"value": rex["value"], "ms_per_step": rex["seconds"] / rex["steps"] * 1e3,
+                                "speed_vs_fast": rex["value"] / r64["value"],
+                                "density_fast_vs_exact": _dist(r64["state"][..., 0], rex["state"][..., 0]),
+                                "state_finite": rex["finite"], "roofline": roofline(rex, w)}
+        del rex
+        res["fp32_split"] = split_block(w, q, dx, args.steps, args.warmup, r64)
+        if args.workload == "quad4096":
+            # {characteristic, component-wise} x {HLLC (phi = 1), HLLC/HLLE blend jp (0.5, 2)},
+            # FP64 fast on the tiling tuned above
             variants = {}
             for cons in (False, True):
                 for jp, tag in (((1e4, 1.1e4), "hllc"), ((0.5, 2.0), "blend_jp0.5_2")):
-                    wv = dict(wq, cons=cons, jp=jp)
-                    rv = run_gpu(wv, qq, dq, 10, 2, "fused", args.variant, "fp64", profile=False,
-                                 chunks=rq.get("chunks"))
+                    wv = dict(w, cons=cons, jp=jp)
+                    rv = run_gpu(wv, q, dx, 16, 8, "fused", "fast", "fp64", profile=False,
+                                 chunks=r64.get("chunks"))
                     variants[("componentwise_" if cons else "characteristic_") + tag] = {
                         "value": rv["value"], "ms_per_step": rv["seconds"] / rv["steps"] * 1e3,
                         "state_finite": rv["finite"]}
-            res["quad4096"]["scheme_variants_fp64"] = variants
-    if not args.no_extra and args.workload == "dmr":
-        res["dropin_l_local"] = dropin_rate(q, dx, w)
-        res["tuning_objective"] = objective_rate(w, q, dx, args.warmup + args.steps)
+            res["scheme_variants_fp64"] = variants
+        r64.pop("state", None)
+        for key, steps, warmup in (("dmr", 200, 20), ("kh8192", 100, 10)):
+            if key != args.workload:
+                res[key] = workload_block(key, steps, warmup)
+        res["batch512"] = batch_block(WORKLOADS["batch512"], 200, 10)
+        wd = WORKLOADS["dmr"]
+        qd, dd = make_state(wd)
+        res["dropin_l_local"] = dropin_rate(qd, dd, wd)
+        res["tuning_objective"] = objective_rate(wd, qd, dd, 220)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(q, dx, w)
     print(json.dumps(res), flush=True)

@@ -108,3 +108,43 @@ def test_four_quadrant_4096_keeps_diagonal_symmetry(variant):
     assert max(asym) <= 1e-12, asym
     # the run moved the state: the check is not on the initial data
     assert np.abs(q[..., 0] - q0[3:-3, 3:-3, 0]).max() > 1e-3
+
+
+def test_reference_scheme_blowup_reproduced(oracle_mod):
+    """The reference scheme (characteristic WENO5 + HLLC, phi = 1, CFL 0.4)
+    turns the four-quadrant problem non-finite by itself: on the oracle at
+    128^2 the state first holds a non-finite value after step 293.  The exact
+    variant on the GPU follows the oracle bit for bit up to step 290 and turns
+    non-finite at the same step (at 4096^2 the same scheme does so between
+    steps 130 and 140, which is why the benchmark restarts that workload)."""
+    from wenox import Solver, problems
+    n = 128
+    q0, dx = problems.four_quadrant(n)
+    print("[blowup] oracle 128^2, characteristic + HLLC, CFL 0.4", flush=True)
+    oracle_mod.set_threads(min(16, len(os.sched_getaffinity(0))))
+    try:
+        q, bad, q290 = q0.copy(), None, None
+        for step in range(1, 301):
+            dt = oracle_mod.cfl_dt(q, n, n, 1.4, 0.4, dx)
+            q = oracle_mod.rk3_cc(q, n, n, dx, dt, 1.4, False, False, (1e4, 1.1e4))
+            if step == 290:
+                q290 = q.copy()
+            if not np.isfinite(q[3:-3, 3:-3, :4]).all():
+                bad = step
+                break
+    finally:
+        oracle_mod.set_threads(1)
+    assert bad is not None and bad > 290, bad
+    print(f"[blowup] oracle first non-finite after step {bad}", flush=True)
+    with Solver(n, n, dx, cfl=0.4, jp_cri=(1e4, 1.1e4), bc="mirror", engine="fused", variant="exact") as s:
+        s.upload(q0)
+        s.run(nstep=290)
+        got = s.download()
+        np.testing.assert_array_equal(got[..., :4], q290[..., :4])
+        gbad = None
+        for step in range(291, 301):
+            s.run(nstep=1)
+            if not np.isfinite(s.conserved_sums()).all():
+                gbad = step
+                break
+    assert gbad == bad, (gbad, bad)
