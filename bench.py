@@ -120,9 +120,11 @@ def run_gpu(w, q, dx, steps, warmup, engine, variant, precision, profile=True, t
                     "sweep_x": {f"w{w}c{c}": round(v, 5) for (w, c), v in tuned["sweep_x"].items()},
                     "sweep_y": {f"w{w}c{c}": round(v, 5) for (w, c), v in tuned["sweep_y"].items()},
                     "pairs": {"x{0}w{2}_y{1}w{3}".format(*t): round(v, 5) for t, v in tuned["pairs"].items()},
-                    "combine_cols": {str(c): round(v, 5) for c, v in tuned["combine"].items()}}
+                    "combine_cols": {str(c): round(v, 5) for c, v in tuned["combine"].items()},
+                    "pipeline_stage": {str(c): round(v, 5) for c, v in tuned.get("pipeline", {}).items()}}
         if engine == "fused":
             out["combine_cols"] = s.combine_cols
+            out["pipeline"] = s.pipeline
         if profile and engine == "fused":
             out["kernels_ms"] = s.profile(reps=10)
         return out
@@ -149,6 +151,7 @@ def run_batch(w, steps, warmup, precision, chunk=16, ranges=None):
                jp_cri=w["jp"], bc=w["bc"], variant="fast", precision=precision, chunk=chunk) as b:
         for i, (q, _) in enumerate(inst):
             b.upload(i, q)
+        tuned = b.autotune()
         b.run(nstep=warmup)
         t0 = time.perf_counter()
         b.run(nstep=steps)
@@ -156,7 +159,8 @@ def run_batch(w, steps, warmup, precision, chunk=16, ranges=None):
         T, k = b.status()
         bad = [i for i in range(w["count"]) if not np.isfinite(b.download(i)[3:-3, 3:-3, :4]).all()]
     return dict(value=cells * steps / el, seconds=el, cells=cells, steps_each=int(np.min(k)) - warmup,
-                ms_per_step=el / steps * 1e3, non_finite=len(bad), chunk=chunk)
+                ms_per_step=el / steps * 1e3, non_finite=len(bad), chunk=tuned["best"],
+                autotune_stage_ms={str(c): round(v, 5) for c, v in tuned.items() if c != "best"})
 
 
 def _cpu_steps(q, dx, w, budget_s):
@@ -483,6 +487,7 @@ def batch_block(w, steps, warmup):
     rc = run_batch(w, steps, warmup, "fp64", ranges=CONFIG_RANGES)
     return {"workload": describe(w), "cells": r["cells"], "steps": steps, "warmup": warmup,
             "value_fp64": r["value"], "ms_per_step_fp64": r["ms_per_step"], "chunk": r["chunk"],
+            "autotune_stage_ms": r["autotune_stage_ms"],
             # the reference scheme itself turns some instances non-finite (the oracle
             # reproduces it); they keep stepping, so the count does not change the cost
             "instances_non_finite": r["non_finite"],
@@ -493,7 +498,7 @@ def batch_block(w, steps, warmup):
             "configured_ranges_fp64": {"state_ranges": "rho~U[0.1,2], p~U[0.02,2], u,v~U[-1.5,1.5]",
                                        "value": rc["value"], "instances_non_finite": rc["non_finite"]},
             "fp32_split": {"value": r32["value"], "speedup_vs_fp64": r32["value"] / r["value"],
-                           "instances_non_finite": r32["non_finite"]}}
+                           "chunk": r32["chunk"], "instances_non_finite": r32["non_finite"]}}
 
 
 def main():
@@ -562,6 +567,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": describe(w), "engine": args.engine, "variant": args.variant,
                    "chunks": r64.get("chunks"), "combine_cols": r64.get("combine_cols"),
+                   "pipeline": r64.get("pipeline"),
                    "autotune_ms": r64.get("autotune_ms")},
         "state_finite": r64["finite"],
         "timed_segments": r64["segments"],

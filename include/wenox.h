@@ -185,7 +185,9 @@ double wenox_solver_last_ms_per_step(wenox_solver *s);
  * out_ms[0] CFL reduction, [1] x sweep, [2] y sweep, [3] RK combine,
  * [4] ghost fill, [5] their sum over one step (3 stages), and with
  * n_out >= 7 [6] both sweeps launched as the step launches them ([5] then
- * uses [6] for the sweeps).  n_out >= 6. */
+ * uses [6] for the sweeps), with n_out >= 8 [7] one whole RK stage as the
+ * step launches it (sweeps and combine, pipelined slabs included; [5] then
+ * uses [7]).  n_out >= 6. */
 int wenox_solver_profile(wenox_solver *s, int reps, double *out_ms, int n_out);
 /* CFL rule dt = CFL dx / (max(sqrt(g p / max(rho, rho_floor)) + |V|) + cmax_eps):
  * (1e-14, 1e-16) by default as Rie_prob_2d_rs.py:311-319; the 1-D driver
@@ -209,12 +211,21 @@ int wenox_solver_set_tiling(wenox_solver *s, int chunk_x, int chunk_y, int waves
 int wenox_solver_combine_cols(wenox_solver *s, int *cols);
 int wenox_solver_set_combine_cols(wenox_solver *s, int cols);
 
+/* Slab pipelining of the fused engine's FP64 fast step: with parts > 1 (at
+ * most 8) each RK stage runs its sweeps in `parts` slabs of x rows on the
+ * solver's stream while a second stream runs the RK combine of every slab
+ * already swept (the sweeps are bound by the FP64 VALU, the combine by HBM).
+ * Results do not depend on parts.  1 (the default) launches each stage's
+ * sweeps and combine once.  No reference counterpart. */
+int wenox_solver_pipeline(wenox_solver *s, int *parts);
+int wenox_solver_set_pipeline(wenox_solver *s, int parts);
+
 /* ---- 3. batched solver: many independent problems of uneven sizes ---------- */
 /* The n instances share the numerics of desc (nx, ny, dx of desc are ignored;
  * bc must be MIRROR or PERIODIC) and each has its own grid, dx, time and CFL
  * dt.  Every phase of a step (CFL, x and y sweeps, RK combine, ghost fill) is
- * one launch over all instances; a step is one hipGraph replay.  chunk: rows
- * marched per sweep workgroup (<= 0: 16).  No counterpart in the reference,
+ * one launch over all instances (both sweeps of a stage share one); a step is
+ * one hipGraph replay.  chunk: rows marched per sweep workgroup (<= 0: 16).  No counterpart in the reference,
  * which runs one problem per process; this serves its batched workload. */
 typedef struct wenox_batch wenox_batch;
 int wenox_batch_create(const wenox_solver_desc *desc, int n, const int64_t *nx, const int64_t *ny,
@@ -227,6 +238,13 @@ int wenox_batch_download(wenox_batch *b, int i, double *q8);
 int wenox_batch_run(wenox_batch *b, int64_t max_steps, double t_final, int64_t *steps_out);
 int wenox_batch_status(wenox_batch *b, double *T, int64_t *steps);
 double wenox_batch_last_ms_per_step(wenox_batch *b);
+/* Rows marched per sweep workgroup (x and y sweeps) of every instance; set
+ * rebuilds the launch tables.  No reference counterpart (autotuning). */
+int wenox_batch_chunks(wenox_batch *b, int *chunk_x, int *chunk_y);
+int wenox_batch_set_chunks(wenox_batch *b, int chunk_x, int chunk_y);
+/* Device time (ms, HIP events) of one RK stage of all instances as a step
+ * launches it, averaged over reps; the state is left as it was. */
+int wenox_batch_profile(wenox_batch *b, int reps, double *stage_ms);
 
 /* ---- 4. jp_cri tuning objective --------------------------------------------- */
 /* A read-only 2-D host array view: extents and element strides. */

@@ -83,3 +83,32 @@ def test_batch_t_final():
         T, k = b.status()
     assert np.all(np.abs(T - 0.02) <= 1e-12)
     assert len(set(k.tolist())) > 1           # instances needed different step counts
+
+
+@pytest.mark.parametrize("variant", ["exact", "fast"])
+def test_batch_chunks_and_autotune(variant):
+    """Every sweep chunk (rows per workgroup), set by hand or picked by the
+    autotuner, gives the same instance states bit for bit; the stage profile
+    leaves the state unchanged."""
+    from wenox import Batch
+    inst = _instances(5)
+    out = {}
+    for chunks in ((16, 16), (5, 9), (64, 64), "auto"):
+        print(f"[batch] chunks {chunks} variant={variant}", flush=True)
+        with Batch(SIZES, [q[1] for q in inst], cfl=0.4, jp_cri=(0.5, 2.0), variant=variant) as b:
+            for i, (q, dx) in enumerate(inst):
+                b.upload(i, q)
+            if chunks == "auto":
+                tuned = b.autotune(reps=2)
+                assert tuned["best"] == b.chunks[0]
+            else:
+                b.set_chunks(*chunks)
+                assert b.chunks == chunks
+                assert b.profile(2) > 0.0
+            b.run(nstep=6)
+            out[chunks] = ([b.download(i) for i in range(len(inst))], b.status())
+    ref, (T0, k0) = out[(16, 16)]
+    for chunks, (got, (T, k)) in out.items():
+        assert np.array_equal(T, T0) and np.array_equal(k, k0), chunks
+        for i in range(len(inst)):
+            np.testing.assert_array_equal(got[i], ref[i], err_msg=f"chunks {chunks} instance {i}")

@@ -538,6 +538,9 @@ struct wenox_solver {
     double cfl_cmax_eps = 1e-16;    // Rie_prob_2d_rs.py:315
     double pfloor = 0.0;            // > 0: the twin's pressure floor after every step
     double* sums = nullptr;  // compensated sums: SUM_BLOCKS x 16 block partials, then 8 results
+    // fused engine: side stream and events of the slab-pipelined stage (plan.parts)
+    hipStream_t side = nullptr;
+    hipEvent_t pipe_ev[PIPE_MAX_PARTS + 1] = {};
 };
 
 namespace {
@@ -769,6 +772,12 @@ extern "C" int wenox_solver_create(const wenox_solver_desc* desc, wenox_solver**
         // allows it (ghost_on_store)
         s->plan.bp = s->bp;
         s->plan.ghost = 1;
+        // default stream priority (as the solver's own stream)
+        bool ev_ok = hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking) == hipSuccess;
+        for (auto& e : s->pipe_ev) ev_ok = ev_ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+        if (!ev_ok) return cleanup(fail(WENOX_E_DEVICE, "side stream/event creation failed"));
+        s->plan.side = s->side;
+        s->plan.ev = s->pipe_ev;
     }
     StepCtl h{};
     h.cfl_dx = d.cfl * d.dx;
@@ -793,6 +802,9 @@ extern "C" void wenox_solver_destroy(wenox_solver* s)
     if (s->cfl_part) (void)hipFree(s->cfl_part);
     if (s->sums) (void)hipFree(s->sums);
     if (s->flags) (void)hipFree(s->flags);
+    for (hipEvent_t e : s->pipe_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (s->side) (void)hipStreamDestroy(s->side);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -1082,6 +1094,20 @@ extern "C" int wenox_solver_profile(wenox_solver* s, int reps, double* out_ms, i
         pair = out_ms[6];
     }
     out_ms[5] = 3.0 * (pair + out_ms[3] + out_ms[4]) + out_ms[0];
+    if (n_out >= 8) {
+        // slot 7: one whole RK stage as the step launches it (sweeps and
+        // combine, the slab pipeline's side stream included), stage 2 into B
+        if ((rc = timed(7, [&] {
+                 if (split)
+                     s->sops->stage_split(s->plan, (const float*)s->U, (const float*)s->UT, (const float*)s->U,
+                                          (float*)s->L, (float*)s->DyT, (float*)s->B, (float*)s->BT, s->flags, 2,
+                                          s->ctl, st);
+                 else
+                     s->sops->stage(s->plan, s->U, s->UT, s->U, s->L, s->DyT, s->B, s->BT, 2, s->ctl, st);
+             })))
+            return rc;
+        out_ms[5] = 3.0 * (out_ms[7] + out_ms[4]) + out_ms[0];
+    }
     return write_ctl(s, &h);
 }
 
@@ -1172,6 +1198,23 @@ extern "C" int wenox_solver_set_combine_cols(wenox_solver* s, int cols)
     if (!s || (cols != 32 && cols != 64)) return fail(WENOX_E_ARG, "wenox_solver_set_combine_cols: cols is 32 or 64");
     if (s->d.engine != WENOX_ENGINE_FUSED) return fail(WENOX_E_ARG, "the combine tile applies to the fused engine");
     s->plan.combine_cols = cols;
+    drop_graphs(s);
+    return WENOX_OK;
+}
+
+extern "C" int wenox_solver_pipeline(wenox_solver* s, int* parts)
+{
+    if (!s || !parts) return fail(WENOX_E_ARG, "wenox_solver_pipeline: null argument");
+    *parts = s->d.engine == WENOX_ENGINE_FUSED ? s->plan.parts : 1;
+    return WENOX_OK;
+}
+
+extern "C" int wenox_solver_set_pipeline(wenox_solver* s, int parts)
+{
+    if (!s || parts < 1 || parts > PIPE_MAX_PARTS)
+        return fail(WENOX_E_ARG, "wenox_solver_set_pipeline: parts is 1 .. " + std::to_string(PIPE_MAX_PARTS));
+    if (s->d.engine != WENOX_ENGINE_FUSED) return fail(WENOX_E_ARG, "the pipeline applies to the fused engine");
+    s->plan.parts = parts;
     drop_graphs(s);
     return WENOX_OK;
 }

@@ -54,6 +54,8 @@ struct wenox_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     double last_ms = 0.0;
+    int chunk_x = 16, chunk_y = 16;
+    size_t flags_n = 0;
 };
 
 namespace {
@@ -72,6 +74,72 @@ void batch_step(wenox_batch* b)
         b->bops->bc(b->tab, b->bp, b->split, dst[s - 1], dstT[s - 1], b->ctls, s, st);
     }
     b->bops->advance(b->ctls, b->n, st);
+}
+
+// Instance table and workgroup maps for sweep chunks (chunk_x, chunk_y),
+// uploaded to the device (the previous tables are freed); the instances'
+// offsets and sizes stay.
+int build_tables(wenox_batch* b, int chunk_x, int chunk_y)
+{
+    std::vector<int> bx, by, bc, bg, bf;
+    for (int i = 0; i < b->n; ++i) {
+        BatchInst& I = b->inst[i];
+        I.chunk_x = chunk_x;
+        I.chunk_y = chunk_y;
+        I.gx_x = ceil_div(I.ny, SWEEP_OUT_COLS);
+        I.gx_y = ceil_div(I.nx, SWEEP_OUT_COLS);
+        I.blk0_x = (int)bx.size();
+        for (int k = 0; k < I.gx_x * ceil_div(I.nx, chunk_x); ++k) bx.push_back(i);
+        I.blk0_y = (int)by.size();
+        for (int k = 0; k < I.gx_y * ceil_div(I.ny, chunk_y); ++k) by.push_back(i);
+        I.tiles_j = ceil_div(I.ny, COMBINE_TILE);
+        I.tile0 = (int)bc.size();
+        for (int k = 0; k < I.tiles_j * ceil_div(I.nx, COMBINE_TILE); ++k) bc.push_back(i);
+        I.ghost_blk0 = (int)bg.size();
+        for (int k = 0; k < ceil_div(6 * (I.ny + 6) + 6 * I.nx, GHOST_TB); ++k) bg.push_back(i);
+        I.cfl_blk0 = (int)bf.size();
+        I.cfl_nblk = std::min(ceil_div(I.nx * I.ny, GHOST_TB), 64);
+        for (int k = 0; k < I.cfl_nblk; ++k) bf.push_back(i);
+    }
+    const size_t tbytes = sizeof(BatchInst) * b->n +
+                          sizeof(int) * (bx.size() + by.size() + bc.size() + bg.size() + bf.size());
+    if (b->dev_tables) BCHK(hipFree(b->dev_tables));
+    b->dev_tables = nullptr;
+    BCHK(hipMalloc(&b->dev_tables, tbytes));
+    // one flag per workgroup of a stage's sweep launch (x and y together)
+    const size_t nf = bx.size() + by.size();
+    if (nf > b->flags_n) {
+        if (b->flags) BCHK(hipFree(b->flags));
+        b->flags = nullptr;
+        BCHK(hipMalloc(&b->flags, sizeof(int) * nf));
+        b->flags_n = nf;
+    }
+    std::vector<char> host(tbytes);
+    char* h = host.data();
+    char* dbase = (char*)b->dev_tables;
+    auto put = [&](const void* src, size_t bytes) {
+        std::memcpy(h, src, bytes);
+        char* dptr = dbase + (h - host.data());
+        h += bytes;
+        return dptr;
+    };
+    b->tab.inst = (const BatchInst*)put(b->inst.data(), sizeof(BatchInst) * b->n);
+    b->tab.blk_x = (const int*)put(bx.data(), sizeof(int) * bx.size());
+    b->tab.blk_y = (const int*)put(by.data(), sizeof(int) * by.size());
+    b->tab.blk_c = (const int*)put(bc.data(), sizeof(int) * bc.size());
+    b->tab.blk_g = (const int*)put(bg.data(), sizeof(int) * bg.size());
+    b->tab.blk_f = (const int*)put(bf.data(), sizeof(int) * bf.size());
+    b->tab.n_x = (int)bx.size(); b->tab.n_y = (int)by.size(); b->tab.n_c = (int)bc.size();
+    b->tab.n_g = (int)bg.size(); b->tab.n_f = (int)bf.size(); b->tab.n_inst = b->n;
+    BCHK(hipMemcpy(b->dev_tables, host.data(), tbytes, hipMemcpyHostToDevice));
+    b->chunk_x = chunk_x;
+    b->chunk_y = chunk_y;
+    // the captured step holds the old grids
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    return WENOX_OK;
 }
 
 int read_ctls(wenox_batch* b, std::vector<StepCtl>& h)
@@ -137,12 +205,12 @@ extern "C" int wenox_batch_create(const wenox_solver_desc* desc, int n, const in
         wenox_batch_destroy(b);
         return code;
     };
-    // instance table and block maps
-    std::vector<int> bx, by, bc, bg, bf;
+    // instance layout (back to back, unpitched)
     int64_t soff = 0, ioff = 0, maxP = 0;
     b->inst.resize(n);
     for (int i = 0; i < n; ++i) {
         BatchInst& I = b->inst[i];
+        I = BatchInst{};
         I.nx = nx[i]; I.ny = ny[i]; I.dx = dx[i];
         I.soff = soff; I.ioff = ioff;
         const int64_t P = (I.nx + 6) * (I.ny + 6);
@@ -151,53 +219,17 @@ extern "C" int wenox_batch_create(const wenox_solver_desc* desc, int n, const in
         maxP = std::max(maxP, P);
         SweepPlan p = make_sweep_plan(I.nx, I.ny, I.dx, d.gamma, d.cons_switch, d.force_hlle, d.jp_lo, d.jp_hi);
         I.two_h_x = p.two_h_x; I.two_h_y = p.two_h_y;
-        I.chunk_x = chunk; I.chunk_y = chunk;
-        I.gx_x = ceil_div(I.ny, SWEEP_OUT_COLS);
-        I.gx_y = ceil_div(I.nx, SWEEP_OUT_COLS);
-        I.blk0_x = (int)bx.size();
-        for (int k = 0; k < I.gx_x * ceil_div(I.nx, chunk); ++k) bx.push_back(i);
-        I.blk0_y = (int)by.size();
-        for (int k = 0; k < I.gx_y * ceil_div(I.ny, chunk); ++k) by.push_back(i);
-        I.tiles_j = ceil_div(I.ny, COMBINE_TILE);
-        I.tile0 = (int)bc.size();
-        for (int k = 0; k < I.tiles_j * ceil_div(I.nx, COMBINE_TILE); ++k) bc.push_back(i);
-        I.ghost_blk0 = (int)bg.size();
-        for (int k = 0; k < ceil_div(6 * (I.ny + 6) + 6 * I.nx, GHOST_TB); ++k) bg.push_back(i);
-        I.cfl_blk0 = (int)bf.size();
-        I.cfl_nblk = std::min(ceil_div(I.nx * I.ny, GHOST_TB), 64);
-        for (int k = 0; k < I.cfl_nblk; ++k) bf.push_back(i);
     }
-    size_t tbytes = sizeof(BatchInst) * n + sizeof(int) * (bx.size() + by.size() + bc.size() + bg.size() + bf.size());
     bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreate(&b->ev0) == hipSuccess && hipEventCreate(&b->ev1) == hipSuccess &&
-              hipMalloc(&b->dev_tables, tbytes) == hipSuccess;
+              hipEventCreate(&b->ev0) == hipSuccess && hipEventCreate(&b->ev1) == hipSuccess;
     for (double** p : {&b->U, &b->A, &b->B, &b->UT, &b->AT, &b->BT})
         ok = ok && hipMalloc(p, (size_t)soff * 8) == hipSuccess;
     ok = ok && hipMalloc(&b->Dx, (size_t)ioff * 8) == hipSuccess && hipMalloc(&b->DyT, (size_t)ioff * 8) == hipSuccess &&
          hipMalloc(&b->work, (size_t)maxP * 10 * 8) == hipSuccess &&
-         hipMalloc(&b->flags, sizeof(int) * std::max(bx.size(), by.size())) == hipSuccess &&
          hipMalloc(&b->ctls, sizeof(StepCtl) * n) == hipSuccess;
     if (!ok) return cleanup(abi_fail(WENOX_E_DEVICE, "batch allocation failed (" + std::to_string((6 * soff + 2 * ioff) * 8 >> 20) + " MiB)"));
-    // tables to the device
-    std::vector<char> host(tbytes);
-    char* h = host.data();
-    char* dbase = (char*)b->dev_tables;
-    auto put = [&](const void* src, size_t bytes) {
-        std::memcpy(h, src, bytes);
-        char* dptr = dbase + (h - host.data());
-        h += bytes;
-        return dptr;
-    };
-    b->tab.inst = (const BatchInst*)put(b->inst.data(), sizeof(BatchInst) * n);
-    b->tab.blk_x = (const int*)put(bx.data(), sizeof(int) * bx.size());
-    b->tab.blk_y = (const int*)put(by.data(), sizeof(int) * by.size());
-    b->tab.blk_c = (const int*)put(bc.data(), sizeof(int) * bc.size());
-    b->tab.blk_g = (const int*)put(bg.data(), sizeof(int) * bg.size());
-    b->tab.blk_f = (const int*)put(bf.data(), sizeof(int) * bf.size());
-    b->tab.n_x = (int)bx.size(); b->tab.n_y = (int)by.size(); b->tab.n_c = (int)bc.size();
-    b->tab.n_g = (int)bg.size(); b->tab.n_f = (int)bf.size(); b->tab.n_inst = n;
-    if (hipMemcpy(b->dev_tables, host.data(), tbytes, hipMemcpyHostToDevice) != hipSuccess)
-        return cleanup(abi_fail(WENOX_E_DEVICE, "batch table upload failed"));
+    int rct = build_tables(b, chunk, chunk);
+    if (rct) return cleanup(rct);
     b->com = BatchCommon{d.gamma, d.jp_lo, d.jp_hi, d.cons_switch, d.force_hlle};
     b->bp.kind = d.bc;
     b->bp.gamma = d.gamma;
@@ -368,3 +400,48 @@ extern "C" int wenox_batch_status(wenox_batch* b, double* T, int64_t* steps)
 }
 
 extern "C" double wenox_batch_last_ms_per_step(wenox_batch* b) { return b ? b->last_ms : 0.0; }
+
+extern "C" int wenox_batch_chunks(wenox_batch* b, int* chunk_x, int* chunk_y)
+{
+    if (!b) return abi_fail(WENOX_E_ARG, "wenox_batch_chunks: null batch");
+    if (chunk_x) *chunk_x = b->chunk_x;
+    if (chunk_y) *chunk_y = b->chunk_y;
+    return WENOX_OK;
+}
+
+extern "C" int wenox_batch_set_chunks(wenox_batch* b, int chunk_x, int chunk_y)
+{
+    if (!b || chunk_x < 1 || chunk_y < 1) return abi_fail(WENOX_E_ARG, "wenox_batch_set_chunks: chunks must be >= 1");
+    BCHK(hipSetDevice(b->device));
+    BCHK(hipStreamSynchronize(b->stream));
+    return build_tables(b, chunk_x, chunk_y);
+}
+
+extern "C" int wenox_batch_profile(wenox_batch* b, int reps, double* stage_ms)
+{
+    if (!b || reps < 1 || !stage_ms) return abi_fail(WENOX_E_ARG, "wenox_batch_profile: bad argument");
+    BCHK(hipSetDevice(b->device));
+    std::vector<StepCtl> h;
+    int rc = read_ctls(b, h);
+    if (rc) return rc;
+    std::vector<StepCtl> act = h;
+    for (auto& c : act) {
+        c.active = 1;
+        c.dt = 0.0;
+    }
+    if ((rc = write_ctls(b, act))) return rc;
+    // one RK stage (stage 2, from U into B / B^T: the step overwrites B) as a step launches it
+    auto stage = [&] {
+        b->bops->stage(b->tab, b->com, b->split, b->U, b->UT, b->U, b->Dx, b->DyT, b->B, b->BT, b->flags, 2,
+                       b->ctls, b->stream);
+    };
+    stage();
+    BCHK(hipEventRecord(b->ev0, b->stream));
+    for (int r = 0; r < reps; ++r) stage();
+    BCHK(hipEventRecord(b->ev1, b->stream));
+    BCHK(hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    BCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    *stage_ms = ms / reps;
+    return write_ctls(b, h);
+}

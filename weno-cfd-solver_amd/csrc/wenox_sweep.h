@@ -26,6 +26,13 @@ struct SweepPlan {
     BcParams bp;               // ghost-fill rule of the solver
     // row pitches (elements): U planes, U^T planes, Dx rows, Dy^T rows
     int64_t ld, ldT, ldx, ldyT;
+    // Slab pipelining of the fast FP64 stage: with parts > 1 the sweeps run in
+    // `parts` slabs of x rows on the stage's stream and each finished slab's
+    // RK combine on `side`, ordered through the events ev[0 .. parts]
+    // (PIPE_MAX_PARTS + 1 of them).
+    int parts = 1;
+    hipStream_t side = nullptr;
+    hipEvent_t* ev = nullptr;
     int64_t P() const { return (nx + 6) * ld; }       // plane stride of U
     int64_t PT() const { return (ny + 6) * ldT; }     // plane stride of U^T
     int64_t PDx() const { return nx * ldx; }          // plane stride of Dx
@@ -125,6 +132,7 @@ struct BatchOps {
 
 extern const BatchOps batch_ops_exact;
 extern const BatchOps batch_ops_fast;
+constexpr int PIPE_MAX_PARTS = 8;
 // workgroups per sweep column block (the wave's output columns)
 constexpr int SWEEP_OUT_COLS = 58;
 constexpr int COMBINE_TILE = 32;

@@ -75,6 +75,9 @@ struct SweepArgs {
     const StepCtl* ctl;
     const void* Un = nullptr;   // step-start state, planes like U (FOLD)
     int stage = 0;              // RK stage 1..3 (FOLD)
+    // the launch's workgroups cover column blocks [bx0, bx0 + ncols) and row
+    // groups [by0, by0 + nrows) (ncols, nrows = 0: to the grid's end)
+    int64_t bx0 = 0, by0 = 0, ncols = 0, nrows = 0;
 };
 
 // Shu-Osher coefficients of stage s: U_{s+1} = a Un + b Us + (c dt) L
@@ -535,7 +538,7 @@ template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BL
 __global__ void __launch_bounds__(WAVE * W) __attribute__((amdgpu_waves_per_eu(2))) k_sweep(SweepArgs A)
 {
     __shared__ SweepLds<T, W> S;
-    sweep_body<T, IN, DT, MODE, DIR, CONS, BLEND, W, FOLD>(A, blockIdx.x, blockIdx.y,
+    sweep_body<T, IN, DT, MODE, DIR, CONS, BLEND, W, FOLD>(A, A.bx0 + blockIdx.x, A.by0 + blockIdx.y,
                                                            (int64_t)blockIdx.y * gridDim.x + blockIdx.x, S);
 }
 
@@ -552,10 +555,10 @@ k_sweep_xy(SweepArgs Ax, SweepArgs Ay, unsigned nbx, unsigned colsx, unsigned co
     __shared__ SweepLds<T, W> S;
     const unsigned b = blockIdx.x;
     if (b < nbx) {
-        sweep_body<T, IN, DT, MODE, DIR_X, CONS, BLEND, W, FOLD>(Ax, b % colsx, b / colsx, b, S);
+        sweep_body<T, IN, DT, MODE, DIR_X, CONS, BLEND, W, FOLD>(Ax, Ax.bx0 + b % colsx, Ax.by0 + b / colsx, b, S);
     } else {
         const unsigned c = b - nbx;
-        sweep_body<T, IN, DT, MODE, DIR_Y, CONS, BLEND, W>(Ay, c % colsy, c / colsy, b, S);
+        sweep_body<T, IN, DT, MODE, DIR_Y, CONS, BLEND, W>(Ay, Ay.bx0 + c % colsy, Ay.by0 + c / colsy, b, S);
     }
 }
 
@@ -729,50 +732,59 @@ template <class St, typename DT, int TJ, bool CFL = false, bool FOLD = false, bo
 __global__ void __launch_bounds__(256) k_combine(const DT* __restrict__ Dx, const DT* __restrict__ DyT,
                                                  St Un, St Us, St out, St outT, int64_t nx, int64_t ny,
                                                  int64_t ld, int64_t ldT, int64_t ldx, int64_t ldyT,
-                                                 int stage, const StepCtl* ctl, double gam, BcParams bp)
+                                                 int stage, const StepCtl* ctl, double gam, BcParams bp,
+                                                 int64_t ti0, int64_t tiles_i)
 {
-    const int64_t tiles_j = gridDim.x, ntiles = (int64_t)gridDim.x * gridDim.y;
+    // this launch holds tile rows [ti0, ti0 + gridDim.y) of the tiles_i rows
+    const int64_t tiles_j = gridDim.x, nloc = (int64_t)gridDim.x * gridDim.y;
     const int64_t b = (int64_t)blockIdx.y * tiles_j + blockIdx.x;
-    const int64_t q = ntiles / 8, r = ntiles % 8, x = b % 8, k = b / 8;
-    const int64_t t = x * q + (x < r ? x : r) + k;
+    const int64_t q = nloc / 8, r = nloc % 8, x = b % 8, k = b / 8;
+    const int64_t t = ti0 * tiles_j + x * q + (x < r ? x : r) + k;
     combine_tile<St, DT, TJ, CFL, FOLD, GHOST>(Dx, DyT, Un, Us, out, outT, nx, ny, ld, ldT, ldx, ldyT, stage, ctl,
-                                               t / tiles_j, t % tiles_j, gam, t, ntiles, bp);
+                                               t / tiles_j, t % tiles_j, gam, t, tiles_i * tiles_j, bp);
 }
 
 // cfl: the step's last combine also leaves the next step's CFL maxima (one
 // per tile, combine_tiles(p) of them) in ctl->part and the advance pending.
 // FOLD: Dx holds the x sweep's part of the combination (the sweep's FOLD).
 // ghost: the combine also writes the ghost cells (ghost_on_store(p)).
+// Tile rows [ti0, ti1) (ti1 < 0: to the last row).
 template <class St, typename DT, int TJ, bool CFL, bool FOLD, bool GHOST>
 static void launch_combine_t(const SweepPlan& p, const DT* Dx, const DT* DyT, St Un, St Us, St out, St outT,
-                             int stage, const StepCtl* ctl, hipStream_t s)
+                             int stage, const StepCtl* ctl, hipStream_t s, int64_t ti0, int64_t ti1)
 {
-    const dim3 g((unsigned)((p.ny + TJ - 1) / TJ), (unsigned)((p.nx + TT - 1) / TT));
+    const int64_t tiles_i = (p.nx + TT - 1) / TT;
+    if (ti1 < 0) ti1 = tiles_i;
+    if (ti1 <= ti0) return;
+    const dim3 g((unsigned)((p.ny + TJ - 1) / TJ), (unsigned)(ti1 - ti0));
     k_combine<St, DT, TJ, CFL, FOLD, GHOST><<<g, 256, 0, s>>>(Dx, DyT, Un, Us, out, outT, p.nx, p.ny, p.ld, p.ldT,
-                                                             p.ldx, p.ldyT, stage, ctl, p.gamma, p.bp);
+                                                             p.ldx, p.ldyT, stage, ctl, p.gamma, p.bp, ti0,
+                                                             tiles_i);
 }
 
 template <class St, typename DT, int TJ, bool FOLD>
 static void launch_combine_f(const SweepPlan& p, const DT* Dx, const DT* DyT, St Un, St Us, St out, St outT,
-                             int stage, const StepCtl* ctl, hipStream_t s, bool cfl, bool ghost)
+                             int stage, const StepCtl* ctl, hipStream_t s, bool cfl, bool ghost, int64_t ti0,
+                             int64_t ti1)
 {
     if (cfl) {
-        if (ghost) launch_combine_t<St, DT, TJ, true, FOLD, true>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s);
-        else launch_combine_t<St, DT, TJ, true, FOLD, false>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s);
+        if (ghost) launch_combine_t<St, DT, TJ, true, FOLD, true>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s, ti0, ti1);
+        else launch_combine_t<St, DT, TJ, true, FOLD, false>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s, ti0, ti1);
     } else {
-        if (ghost) launch_combine_t<St, DT, TJ, false, FOLD, true>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s);
-        else launch_combine_t<St, DT, TJ, false, FOLD, false>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s);
+        if (ghost) launch_combine_t<St, DT, TJ, false, FOLD, true>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s, ti0, ti1);
+        else launch_combine_t<St, DT, TJ, false, FOLD, false>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s, ti0, ti1);
     }
 }
 
 template <class St, typename DT, bool FOLD = false>
 static void launch_combine(const SweepPlan& p, const DT* Dx, const DT* DyT, St Un, St Us, St out, St outT,
-                           int stage, const StepCtl* ctl, hipStream_t s, bool cfl = false, bool ghost = false)
+                           int stage, const StepCtl* ctl, hipStream_t s, bool cfl = false, bool ghost = false,
+                           int64_t ti0 = 0, int64_t ti1 = -1)
 {
     if (p.combine_cols == 64)
-        launch_combine_f<St, DT, 64, FOLD>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s, cfl, ghost);
+        launch_combine_f<St, DT, 64, FOLD>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s, cfl, ghost, ti0, ti1);
     else
-        launch_combine_f<St, DT, 32, FOLD>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s, cfl, ghost);
+        launch_combine_f<St, DT, 32, FOLD>(p, Dx, DyT, Un, Us, out, outT, stage, ctl, s, cfl, ghost, ti0, ti1);
 }
 
 // Plain RHS from the two sweeps (for tests): L = -(Dx + Dy^T), SoA4 interior
@@ -813,11 +825,22 @@ __global__ void __launch_bounds__(256) k_transpose(const E* in, E* out, int64_t 
     }
 }
 
+// Column blocks and row groups of a sweep launch (its ncols / nrows, else
+// the rest of the grid from bx0 / by0).
+static int64_t sweep_cols(const SweepArgs& a)
+{
+    return a.ncols ? a.ncols : (a.nt + OUT_COLS - 1) / OUT_COLS - a.bx0;
+}
+static int64_t sweep_rows(const SweepArgs& a, int W, bool blend)
+{
+    const int64_t rows = group_cells(W, a.chunk, blend);
+    return a.nrows ? a.nrows : (a.nn + rows - 1) / rows - a.by0;
+}
+
 template <typename T, int IN, typename DT, int MODE, int DIR, int W, bool FOLD = false>
 static void launch_sweep_w(const SweepArgs& a, int cons, int blend, hipStream_t s)
 {
-    const int64_t rows = group_cells(W, a.chunk, blend != 0);
-    dim3 grid((unsigned)((a.nt + OUT_COLS - 1) / OUT_COLS), (unsigned)((a.nn + rows - 1) / rows));
+    dim3 grid((unsigned)sweep_cols(a), (unsigned)sweep_rows(a, W, blend != 0));
     if (cons) {
         if (blend) k_sweep<T, IN, DT, MODE, DIR, true, true, W, FOLD><<<grid, WAVE * W, 0, s>>>(a);
         else k_sweep<T, IN, DT, MODE, DIR, true, false, W, FOLD><<<grid, WAVE * W, 0, s>>>(a);
@@ -838,11 +861,9 @@ static void launch_sweep(const SweepArgs& a, int cons, int blend, hipStream_t s)
 template <typename T, int IN, typename DT, int MODE, int W, bool FOLD = false>
 static void launch_sweep_xy_w(const SweepArgs& ax, const SweepArgs& ay, int cons, int blend, hipStream_t s)
 {
-    const unsigned colsx = (unsigned)((ax.nt + OUT_COLS - 1) / OUT_COLS);
-    const unsigned colsy = (unsigned)((ay.nt + OUT_COLS - 1) / OUT_COLS);
-    const int64_t rx = group_cells(W, ax.chunk, blend != 0), ry = group_cells(W, ay.chunk, blend != 0);
-    const unsigned nbx = colsx * (unsigned)((ax.nn + rx - 1) / rx);
-    const unsigned nby = colsy * (unsigned)((ay.nn + ry - 1) / ry);
+    const unsigned colsx = (unsigned)sweep_cols(ax), colsy = (unsigned)sweep_cols(ay);
+    const unsigned nbx = colsx * (unsigned)sweep_rows(ax, W, blend != 0);
+    const unsigned nby = colsy * (unsigned)sweep_rows(ay, W, blend != 0);
     const dim3 grid(nbx + nby);
     if (cons) {
         if (blend)
@@ -909,6 +930,39 @@ static void h_stage_fold(const SweepPlan& p, const double* Us, const double* UsT
     ax.stage = stage;
     const SweepArgs ay = args_y(p, UsT, DyT, nullptr, ctl);
     const StF64 un{const_cast<double*>(Un), P}, us{const_cast<double*>(Us), P}, o{out, P}, oT{outT, p.PT()};
+    if (sweeps && combine && p.parts > 1 && p.side) {
+        // Slabs of x rows: the sweeps of slab k run on s while the combine of
+        // the slabs finished before runs on the side stream (the sweeps are
+        // VALU-bound, the combine HBM-bound).  Slab k's sweeps cover x-sweep
+        // row groups [gx0, gx1) and y-sweep column blocks [cy0, cy1); the
+        // combine takes the tile rows whose cells both have finished.
+        const int wx = ax.waves;
+        const int64_t gcx = group_cells(wx, ax.chunk, blend != 0);
+        const int64_t Gx = (p.nx + gcx - 1) / gcx, Cy = (p.nx + OUT_COLS - 1) / OUT_COLS;
+        const int K = (int)std::min<int64_t>(std::min<int64_t>(p.parts, Gx), Cy);
+        int64_t t_done = 0;
+        for (int k = 0; k < K; ++k) {
+            SweepArgs bx = ax, by = ay;
+            bx.by0 = k * Gx / K;
+            bx.nrows = (k + 1) * Gx / K - bx.by0;
+            by.bx0 = k * Cy / K;
+            by.ncols = (k + 1) * Cy / K - by.bx0;
+            launch_sweeps<double, IN_F64, double, MODE_PLAIN, true>(bx, by, p.cons, blend, s);
+            const int64_t x_done = std::min((bx.by0 + bx.nrows) * gcx, (by.bx0 + by.ncols) * OUT_COLS);
+            const int64_t t1 = k == K - 1 ? (p.nx + TT - 1) / TT : std::min(x_done, p.nx) / TT;
+            if (t1 > t_done) {
+                (void)hipEventRecord(p.ev[k], s);
+                (void)hipStreamWaitEvent(p.side, p.ev[k], 0);
+                launch_combine<StF64, double, true>(p, Dx, DyT, un, us, o, oT, stage, ctl, p.side, stage == 3,
+                                                    ghost_on_store(p), t_done, t1);
+                t_done = t1;
+            }
+        }
+        // the next launch on s needs every slab's combine
+        (void)hipEventRecord(p.ev[K], p.side);
+        (void)hipStreamWaitEvent(s, p.ev[K], 0);
+        return;
+    }
     if (sweeps) launch_sweeps<double, IN_F64, double, MODE_PLAIN, true>(ax, ay, p.cons, blend, s);
     if (combine)
         launch_combine<StF64, double, true>(p, Dx, DyT, un, us, o, oT, stage, ctl, s, stage == 3, ghost_on_store(p));
@@ -1099,38 +1153,65 @@ template <> __device__ __forceinline__ StSplit make_st<StSplit>(void* base, int6
     return StSplit{(float*)base + 2 * soff, P};
 }
 
-template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND, bool FOLD = false>
+// Both sweeps of a stage in one launch: workgroups [0, n_x) run the x sweep,
+// the rest the y sweep (their tails fill each other, as k_sweep_xy).
+template <typename T, int IN, typename DT, int MODE, bool CONS, bool BLEND, bool FOLD = false>
 __global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2)))
-k_sweep_batch(BatchTables B, BatchCommon C, const void* Ubase, void* Dbase, int* flags, StepCtl* ctls,
-              const void* Unbase, int stage)
+k_sweep_batch_xy(BatchTables B, BatchCommon C, const void* Ubase, const void* UTbase, void* Dxbase, void* DyTbase,
+                 int* flags, StepCtl* ctls, const void* Unbase, int stage)
 {
     const int b = blockIdx.x;
-    const int i = (DIR == DIR_X ? B.blk_x : B.blk_y)[b];
+    const bool xdir = b < B.n_x;
+    const int i = xdir ? B.blk_x[b] : B.blk_y[b - B.n_x];
     const BatchInst I = B.inst[i];
     SweepArgs A;
-    A.U = (const char*)Ubase + I.soff * 8;
-    A.M = DIR == DIR_X ? I.nx + 6 : I.ny + 6;
-    A.N = DIR == DIR_X ? I.ny + 6 : I.nx + 6;
+    A.U = (const char*)(xdir ? Ubase : UTbase) + I.soff * 8;
+    A.M = xdir ? I.nx + 6 : I.ny + 6;
+    A.N = xdir ? I.ny + 6 : I.nx + 6;
     A.ld = A.N;                                   // instances are stored unpitched
-    A.ldd = DIR == DIR_X ? I.ny : I.nx;
-    A.nn = DIR == DIR_X ? I.nx : I.ny;
-    A.nt = DIR == DIR_X ? I.ny : I.nx;
-    A.chunk = DIR == DIR_X ? I.chunk_x : I.chunk_y;
+    A.ldd = xdir ? I.ny : I.nx;
+    A.nn = xdir ? I.nx : I.ny;
+    A.nt = xdir ? I.ny : I.nx;
+    A.chunk = xdir ? I.chunk_x : I.chunk_y;
     A.waves = 1;
     A.gam = C.gam;
     A.dx = I.dx;
-    A.two_h = DIR == DIR_X ? I.two_h_x : I.two_h_y;
+    A.two_h = xdir ? I.two_h_x : I.two_h_y;
     A.lo = C.jp_lo;
     A.hi = C.jp_hi;
-    A.D = (DT*)Dbase + I.ioff;
+    A.D = (DT*)(xdir ? Dxbase : DyTbase) + I.ioff;
     A.flags = flags;
     A.ctl = &ctls[i];
-    A.Un = FOLD ? (const char*)Unbase + I.soff * 8 : nullptr;
+    A.Un = FOLD && xdir ? (const char*)Unbase + I.soff * 8 : nullptr;
     A.stage = stage;
-    const int local = b - (DIR == DIR_X ? I.blk0_x : I.blk0_y);
-    const int gx = DIR == DIR_X ? I.gx_x : I.gx_y;
     __shared__ SweepLds<T, 1> S;
-    sweep_body<T, IN, DT, MODE, DIR, CONS, BLEND, 1, FOLD>(A, local % gx, local / gx, b, S);
+    if (xdir) {
+        const int local = b - I.blk0_x;
+        sweep_body<T, IN, DT, MODE, DIR_X, CONS, BLEND, 1, FOLD>(A, local % I.gx_x, local / I.gx_x, b, S);
+    } else {
+        const int local = b - B.n_x - I.blk0_y;
+        sweep_body<T, IN, DT, MODE, DIR_Y, CONS, BLEND, 1>(A, local % I.gx_y, local / I.gx_y, b, S);
+    }
+}
+
+template <typename T, int IN, typename DT, int MODE, bool FOLD = false>
+static void launch_sweep_batch_xy(const BatchTables& t, const BatchCommon& c, const void* U, const void* UT,
+                                  void* Dx, void* DyT, int* flags, StepCtl* ctls, hipStream_t s,
+                                  const void* Un = nullptr, int stage = 0)
+{
+    const unsigned n = (unsigned)(t.n_x + t.n_y);
+    if (!n) return;
+    const bool blend = !c.force_hlle;
+#define WX_KB(CO, BL) k_sweep_batch_xy<T, IN, DT, MODE, CO, BL, FOLD><<<n, WAVE, 0, s>>>(t, c, U, UT, Dx, DyT, flags, \
+                                                                                      ctls, Un, stage)
+    if (c.cons) {
+        if (blend) WX_KB(true, true);
+        else WX_KB(true, false);
+    } else {
+        if (blend) WX_KB(false, true);
+        else WX_KB(false, false);
+    }
+#undef WX_KB
 }
 
 template <class St, typename DT, bool FOLD = false>
@@ -1178,51 +1259,27 @@ __global__ void k_advance_batch(StepCtl* ctls, int n)
     }
 }
 
-template <typename T, int IN, typename DT, int MODE, int DIR, bool FOLD = false>
-static void launch_sweep_batch(const BatchTables& t, const BatchCommon& c, const void* U, void* D,
-                               int* flags, StepCtl* ctls, hipStream_t s, const void* Un = nullptr, int stage = 0)
-{
-    const unsigned n = (unsigned)(DIR == DIR_X ? t.n_x : t.n_y);
-    if (!n) return;
-    const bool blend = !c.force_hlle;
-    if (c.cons) {
-        if (blend)
-            k_sweep_batch<T, IN, DT, MODE, DIR, true, true, FOLD><<<n, WAVE, 0, s>>>(t, c, U, D, flags, ctls, Un, stage);
-        else
-            k_sweep_batch<T, IN, DT, MODE, DIR, true, false, FOLD><<<n, WAVE, 0, s>>>(t, c, U, D, flags, ctls, Un,
-                                                                                     stage);
-    } else {
-        if (blend)
-            k_sweep_batch<T, IN, DT, MODE, DIR, false, true, FOLD><<<n, WAVE, 0, s>>>(t, c, U, D, flags, ctls, Un,
-                                                                                     stage);
-        else
-            k_sweep_batch<T, IN, DT, MODE, DIR, false, false, FOLD><<<n, WAVE, 0, s>>>(t, c, U, D, flags, ctls, Un,
-                                                                                      stage);
-    }
-}
-
 static void hb_stage(const BatchTables& t, const BatchCommon& c, int split, const void* Us, const void* UsT,
                      const void* Un, void* Dx, void* DyT, void* out, void* outT, int* flags, int stage,
                      StepCtl* ctls, hipStream_t s)
 {
 #if WX_FAST
     if (split) {
-        launch_sweep_batch<double, IN_SPLIT, float, MODE_DETECT, DIR_X>(t, c, Us, Dx, flags, ctls, s);
-        launch_sweep_batch<double, IN_SPLIT, float, MODE_DETECT, DIR_Y>(t, c, UsT, DyT, flags, ctls, s);
+        launch_sweep_batch_xy<double, IN_SPLIT, float, MODE_DETECT>(t, c, Us, UsT, Dx, DyT, flags, ctls, s);
         k_combine_batch<StSplit, float><<<(unsigned)t.n_c, 256, 0, s>>>(
             t, (const float*)Dx, (const float*)DyT, (void*)Un, (void*)Us, out, outT, stage, ctls);
         return;
     }
 #endif
 #if WX_FAST
-    // the x sweep forms part of the stage combination, as in the single-grid step
-    launch_sweep_batch<double, IN_F64, double, MODE_PLAIN, DIR_X, true>(t, c, Us, Dx, nullptr, ctls, s, Un, stage);
-    launch_sweep_batch<double, IN_F64, double, MODE_PLAIN, DIR_Y>(t, c, UsT, DyT, nullptr, ctls, s);
+    // the x sweep forms part of the stage combination, as in the single-grid
+    // step; both sweeps in one launch
+    launch_sweep_batch_xy<double, IN_F64, double, MODE_PLAIN, true>(t, c, Us, UsT, Dx, DyT, nullptr, ctls, s, Un,
+                                                                     stage);
     k_combine_batch<StF64, double, true><<<(unsigned)t.n_c, 256, 0, s>>>(
         t, (const double*)Dx, (const double*)DyT, (void*)Un, (void*)Us, out, outT, stage, ctls);
 #else
-    launch_sweep_batch<double, IN_F64, double, MODE_PLAIN, DIR_X>(t, c, Us, Dx, nullptr, ctls, s);
-    launch_sweep_batch<double, IN_F64, double, MODE_PLAIN, DIR_Y>(t, c, UsT, DyT, nullptr, ctls, s);
+    launch_sweep_batch_xy<double, IN_F64, double, MODE_PLAIN>(t, c, Us, UsT, Dx, DyT, nullptr, ctls, s);
     k_combine_batch<StF64, double><<<(unsigned)t.n_c, 256, 0, s>>>(
         t, (const double*)Dx, (const double*)DyT, (void*)Un, (void*)Us, out, outT, stage, ctls);
 #endif

@@ -129,7 +129,7 @@ class Solver:
     def ms_per_step(self) -> float:
         return _lib.load().wenox_solver_last_ms_per_step(self._h)
 
-    KERNELS = ("cfl", "sweep_x", "sweep_y", "combine", "ghost_fill", "step_sum", "sweeps")
+    KERNELS = ("cfl", "sweep_x", "sweep_y", "combine", "ghost_fill", "step_sum", "sweeps", "stage")
 
     def profile(self, reps: int = 10) -> dict:
         """Per-kernel device time (ms) of the fused engine, HIP events on the
@@ -141,9 +141,11 @@ class Solver:
         FP32-split the reduction over the state.  "ghost_fill" is 0 where the
         step's RK combine writes the ghost cells itself (mirror, copy and
         the double-Mach-reflection fills on grids whose edge tiles are at
-        least 3 cells wide)."""
-        out = np.zeros(7)
-        _lib.check(_lib.load().wenox_solver_profile(self._h, int(reps), out.ctypes.data, 7))
+        least 3 cells wide).  "stage" is one whole RK stage as the step
+        launches it (sweeps, combine and the slab pipeline), and "step_sum"
+        is then 3 x (stage + ghost_fill) + cfl."""
+        out = np.zeros(8)
+        _lib.check(_lib.load().wenox_solver_profile(self._h, int(reps), out.ctypes.data, 8))
         return dict(zip(self.KERNELS, out.tolist()))
 
     @property
@@ -173,6 +175,17 @@ class Solver:
     def set_combine_cols(self, cols: int):
         _lib.check(_lib.load().wenox_solver_set_combine_cols(self._h, int(cols)))
 
+    @property
+    def pipeline(self) -> int:
+        """Slabs of the fused FP64 fast stage whose RK combines overlap the
+        next slab's sweeps (1: none)."""
+        v = ctypes.c_int()
+        _lib.check(_lib.load().wenox_solver_pipeline(self._h, ctypes.byref(v)))
+        return v.value
+
+    def set_pipeline(self, parts: int):
+        _lib.check(_lib.load().wenox_solver_set_pipeline(self._h, int(parts)))
+
     def set_tiling(self, chunk_x: int, chunk_y: int, waves_x: int, waves_y: int):
         _lib.check(_lib.load().wenox_solver_set_tiling(self._h, int(chunk_x), int(chunk_y), int(waves_x),
                                                         int(waves_y)))
@@ -192,6 +205,8 @@ class Solver:
     # the sweep time steps with the number of workgroup rounds on the 256 CUs
     # (two resident waves per SIMD), so one row more or less per wave can
     # move a grid across a round boundary.
+    # slab counts of the pipelined FP64 stage the autotuner tries
+    PIPELINE_PARTS = (1, 2, 3, 4, 6, 8)
     TILINGS = (tuple((1, c) for c in (8, 12, 16, 24, 32, 40, 48, 56, 64))
                + tuple((2, c) for c in range(4, 25, 2))
                + tuple((4, c) for c in range(3, 17)))
@@ -241,14 +256,23 @@ class Solver:
             comb[cols] = self.profile(reps)["combine"]
         cb = min(comb, key=comb.get)
         self.set_combine_cols(cb if comb[cb] < 0.97 * comb[c0] else c0)
+        # slab pipeline of the FP64 stage (its combines overlap the next slab's sweeps)
+        pipe = {}
+        if self.desc.precision == PREC_FP64 and self.desc.variant == FAST:
+            for parts in self.PIPELINE_PARTS:
+                self.set_pipeline(parts)
+                pipe[parts] = self.profile(reps)["stage"]
+            bp = min(pipe, key=pipe.get)
+            self.set_pipeline(bp if pipe[bp] < 0.97 * pipe[1] else 1)
         return {"sweep_x": per["sweep_x"], "sweep_y": per["sweep_y"], "pairs": pairs, "best": best,
-                "combine": comb}
+                "combine": comb, "pipeline": pipe}
 
 
 class Batch:
     """Many independent problems of uneven sizes advanced together on the
     device, each with its own dx, time and CFL dt; every phase of a step is
-    one launch over all instances and a step is one hipGraph replay."""
+    one launch over all instances (the x and y sweeps of a stage share one)
+    and a step is one hipGraph replay."""
 
     def __init__(self, sizes, dx, gamma=1.4, cfl=0.4, cons_switch=False, force_hlle=False,
                  jp_cri=(0.2, 1.0), bc="mirror", variant="fast", precision="fp64", chunk=16,
@@ -316,3 +340,40 @@ class Batch:
     @property
     def ms_per_step(self) -> float:
         return _lib.load().wenox_batch_last_ms_per_step(self._h)
+
+    @property
+    def chunks(self):
+        cx, cy = ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.load().wenox_batch_chunks(self._h, ctypes.byref(cx), ctypes.byref(cy)))
+        return cx.value, cy.value
+
+    def set_chunks(self, chunk_x: int, chunk_y: int):
+        _lib.check(_lib.load().wenox_batch_set_chunks(self._h, int(chunk_x), int(chunk_y)))
+
+    def profile(self, reps: int = 5) -> float:
+        """Device time (ms) of one RK stage of every instance as a step
+        launches it; the state is left unchanged."""
+        v = ctypes.c_double()
+        _lib.check(_lib.load().wenox_batch_profile(self._h, int(reps), ctypes.byref(v)))
+        return v.value
+
+    CHUNKS = (8, 12, 16, 24, 32, 48, 64, 96)
+
+    def autotune(self, candidates=CHUNKS, reps: int = 3) -> dict:
+        """Pick the rows marched per sweep workgroup (one value for the x and
+        y sweeps of every instance) with the lowest measured stage time and
+        keep it; the current chunks stay unless another beats them by 3 %.
+        Returns {chunk: stage ms, "best": chunk}."""
+        c0 = self.chunks
+        times = {}
+        for c in candidates:
+            self.set_chunks(c, c)
+            times[c] = self.profile(reps)
+        self.set_chunks(*c0)
+        t0 = self.profile(reps)
+        best = min(times, key=times.get)
+        if times[best] < 0.97 * t0:
+            self.set_chunks(best, best)
+        out = dict(times)
+        out["best"] = self.chunks[0]
+        return out

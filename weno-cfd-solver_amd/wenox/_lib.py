@@ -99,6 +99,8 @@ def load():
         "wenox_solver_set_tiling": ([dp, i, i, i, i], i),
         "wenox_solver_combine_cols": ([dp, ctypes.POINTER(i)], i),
         "wenox_solver_set_combine_cols": ([dp, i], i),
+        "wenox_solver_pipeline": ([dp, ctypes.POINTER(i)], i),
+        "wenox_solver_set_pipeline": ([dp, i], i),
         "wenox_solver_set_cfl_rule": ([dp, d, d], i),
         "wenox_solver_set_pfloor": ([dp, d], i),
         "wenox_objective_2d": ([ctypes.POINTER(View2)] * 3 + [d, d, dp, dp, dp], i),
@@ -110,6 +112,9 @@ def load():
         "wenox_batch_upload": ([dp, i, V, d], i),
         "wenox_batch_download": ([dp, i, dp], i),
         "wenox_batch_run": ([dp, ctypes.c_int64, d, ctypes.POINTER(ctypes.c_int64)], i),
+        "wenox_batch_chunks": ([dp, ctypes.POINTER(i), ctypes.POINTER(i)], i),
+        "wenox_batch_set_chunks": ([dp, i, i], i),
+        "wenox_batch_profile": ([dp, i, dp], i),
         "wenox_batch_status": ([dp, dp, dp], i),
         "wenox_batch_last_ms_per_step": ([dp], d),
     }
