@@ -126,7 +126,8 @@ int wenox_l_local_noalloc(const wenox_view3 *u, double dx, double gamma, int for
 #define WENOX_BC_MIRROR 0    /* Rie_prob_2d_rs.py:123-152 apply_bc_zero_gradient (even mirror) */
 #define WENOX_BC_PERIODIC 1
 #define WENOX_BC_DMR_REF 2   /* unitest.py:207-227 apply_bc_refac */
-#define WENOX_BC_DMR_STD 3   /* standard double Mach reflection, exact moving shock on top */
+#define WENOX_BC_DMR_STD 3   /* standard double Mach reflection, exact moving shock on top
+                                (domain [0, 4] x [0, 1]: ny * dx must be 1) */
 #define WENOX_BC_COPY 4      /* Rie_prob_1D_rs.py:59-76 apply_bc_corrected (constant extrapolation) */
 
 #define WENOX_ENGINE_UNFUSED 0  /* the reference's pass structure, face arrays in HBM */

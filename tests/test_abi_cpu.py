@@ -122,3 +122,18 @@ def test_empty_arrays_follow_the_reference(wx):
         pytest.skip("needs the device for the (empty) call path")
     e = np.zeros((0, 5, 4))
     assert wx.flux_x_py(e, 1.4).shape == (0, 5, 4)
+
+
+def test_solver_argument_checks_before_device_use():
+    """Descriptor errors are reported before any device is touched (here and
+    on the GPU alike): the standard double Mach reflection's domain height
+    must be 1; the no-allocation l_local rejects null and undersized views."""
+    import ctypes
+    import wenox
+    from wenox import _lib
+    with pytest.raises(ValueError, match="domain height"):
+        wenox.Solver(40, 34, 4.0 / 40, bc="dmr", engine="fused")
+    L = _lib.load()
+    assert L.wenox_l_local_noalloc(None, 0.1, 1.4, 0, 0.5, 2.0, 0, None, None) == _lib.E_ARG
+    v = _lib.View3(8, (ctypes.c_int64 * 3)(5, 10, 4), (ctypes.c_int64 * 3)(40, 4, 1))
+    assert L.wenox_l_local_noalloc(ctypes.byref(v), 0.1, 1.4, 1, 0.5, 2.0, 0, None, None) == _lib.E_SHAPE

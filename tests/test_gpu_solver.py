@@ -319,43 +319,69 @@ def test_fused_matches_unfused_for_every_boundary(bc):
     np.testing.assert_array_equal(out["fused"][2], out["unfused"][2])
 
 
-@pytest.mark.parametrize("nx,ny,cols", [(33, 40, 32), (40, 34, 32), (66, 65, 64), (64, 35, 32), (35, 64, 64)])
-def test_fused_ghost_cells_for_every_edge_tile_width(nx, ny, cols):
+GHOST_GRIDS = [(33, 40, 32), (40, 34, 32), (66, 65, 64), (64, 35, 32), (35, 64, 64)]
+# double Mach reflection grids are 4:1; their edge tiles: 33 -> 1 cell (32-wide
+# tiles), 34 -> 2 cells, 65 -> 1 cell (64-wide), 132 x 33 -> 4 x 1, 140 x 35 -> 12 x 3
+DMR_GHOST_GRIDS = [(132, 33, 32), (136, 34, 32), (260, 65, 64), (140, 35, 32), (136, 34, 64)]
+
+
+@pytest.mark.parametrize("bc", ["mirror", "periodic", "copy", "dmr_ref", "dmr"])
+@pytest.mark.parametrize("grid", range(5))
+def test_fused_ghost_cells_for_every_edge_tile_width(bc, grid):
     """The fused FP64 step lets the RK combine write the ghost cells where
     each ghost's source lies in the combine tile of its nearest interior
     cell, and keeps the ghost-fill kernel where an edge tile is 1 or 2 cells
     wide (33, 34, 65 wide grids) or the rule is periodic.  Either way the full
     arrays, ghost cells included, equal the unfused engine's bit for bit
-    after 5 steps (exact variant)."""
+    after 5 steps (exact variant), for every boundary rule, with 32- and
+    64-wide combine tiles."""
     from wenox import Solver, problems
-    q0, dx = problems.four_quadrant(nx, ny=ny)
-    print(f"[solver] ghost cells {nx}x{ny} combine cols {cols}", flush=True)
-    for bc in ("mirror", "periodic"):
-        out = {}
-        for engine in ("unfused", "fused"):
-            with Solver(nx, ny, dx, cfl=0.3, jp_cri=(0.5, 2.0), bc=bc, engine=engine, variant="exact") as s:
-                if engine == "fused":
-                    s.set_combine_cols(cols)
-                s.upload(q0)
-                T, k = s.run(nstep=5)
-                out[engine] = (s.download(), T)
-        assert np.isfinite(out["unfused"][0]).all()
-        assert out["fused"][1] == out["unfused"][1]
-        np.testing.assert_array_equal(out["fused"][0], out["unfused"][0], err_msg=bc)
+    kw = dict(cfl=0.3, jp_cri=(0.5, 2.0))
+    if bc in ("dmr", "dmr_ref"):
+        nx, ny, cols = DMR_GHOST_GRIDS[grid]
+        if bc == "dmr":
+            q0, dx = problems.double_mach(nx, ny)
+        else:
+            q0, bound = problems.dmr_reference(nx, ny)
+            dx = 1.0 / ny
+            kw.update(dmr_bound=bound)
+        kw.update(cfl=0.1, jp_cri=(1e4, 1.1e4))
+    else:
+        nx, ny, cols = GHOST_GRIDS[grid]
+        q0, dx = problems.four_quadrant(nx, ny=ny)
+    print(f"[solver] ghost cells {bc} {nx}x{ny} combine cols {cols}", flush=True)
+    out = {}
+    for engine in ("unfused", "fused"):
+        with Solver(nx, ny, dx, bc=bc, engine=engine, variant="exact", **kw) as s:
+            if engine == "fused":
+                s.set_combine_cols(cols)
+            s.upload(q0)
+            T, k = s.run(nstep=5)
+            out[engine] = (s.download(), T)
+    assert np.isfinite(out["unfused"][0]).all()
+    assert out["fused"][1] == out["unfused"][1]
+    np.testing.assert_array_equal(out["fused"][0], out["unfused"][0], err_msg=bc)
 
 
-@pytest.mark.parametrize("bc", ["mirror", "copy", "dmr"])
+@pytest.mark.parametrize("bc", ["mirror", "copy", "dmr", "dmr_ref"])
 def test_fp32_split_ghost_cells_on_store(bc):
     """The FP32-split step's combine writes the ghost cells too where
-    ghost_on_store allows it.  On a 40 x 34 grid 32-wide combine tiles leave
-    a 2-cell edge tile (the fill kernel runs) and 64-wide tiles do not (the
-    combine writes the ghosts); the tile width changes nothing else, so the
-    two runs agree bit for bit, ghost cells included."""
+    ghost_on_store allows it.  On a 40 x 34 grid (136 x 34 for the double
+    Mach reflection rules) 32-wide combine tiles leave a 2-cell edge tile (the
+    fill kernel runs) and 64-wide tiles do not (the combine writes the
+    ghosts); the tile width changes nothing else, so the two runs agree bit
+    for bit, ghost cells included."""
     from wenox import Solver, problems
     nx, ny = 40, 34
     if bc == "dmr":
+        nx = 4 * ny
         q0, dx = problems.double_mach(nx, ny)
         kw = dict(cfl=0.1, jp_cri=(1e4, 1.1e4))
+    elif bc == "dmr_ref":
+        nx = 4 * ny
+        q0, bound = problems.dmr_reference(nx, ny)
+        dx = 1.0 / ny
+        kw = dict(cfl=0.1, jp_cri=(1e4, 1.1e4), dmr_bound=bound)
     else:
         q0, dx = problems.four_quadrant(nx, ny=ny)
         kw = dict(cfl=0.3, jp_cri=(0.5, 2.0))

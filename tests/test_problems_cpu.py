@@ -95,3 +95,15 @@ def test_snapshot_writer(tmp_path):
     w.put(str(tmp_path / "missing_dir" / "0.npy"), np.zeros(3))
     with pytest.raises(OSError):
         w.close()
+
+
+def test_double_mach_needs_a_4_to_1_grid():
+    """The solver places the exact shock trace on the top boundary for
+    [0,4] x [0,1] with one cell size, so the standard double Mach reflection
+    rejects other aspect ratios."""
+    import pytest
+    from wenox import problems
+    with pytest.raises(ValueError):
+        problems.double_mach(40, 34)
+    q, dx = problems.double_mach(136, 34)
+    assert q.shape == (142, 40, 8) and dx == 4.0 / 136

@@ -690,6 +690,9 @@ extern "C" int wenox_solver_create(const wenox_solver_desc* desc, wenox_solver**
     if (d.nx < 3 || d.ny < 3) return fail(WENOX_E_SHAPE, "solver needs nx >= 3 and ny >= 3");
     if (!d.force_hlle && (d.nx < 2 || d.ny < 2)) return fail(WENOX_E_SHAPE, "blend needs nx, ny >= 2");
     if (d.bc < 0 || d.bc > 4) return fail(WENOX_E_ARG, "unknown boundary kind");
+    if (d.bc == WENOX_BC_DMR_STD && !(std::fabs((double)d.ny * d.dx - 1.0) <= 1e-9))
+        return fail(WENOX_E_ARG, "standard double Mach reflection: the domain height ny * dx must be 1 "
+                                 "(the shock trace on the top boundary is placed for [0, 4] x [0, 1])");
     if (d.bc == WENOX_BC_DMR_REF && d.nx + 2 < d.dmr_bound + 10)
         return fail(WENOX_E_SHAPE, "DMR reference ghost fill needs nx + 2 >= bound + 10");
     if (d.variant != WENOX_EXACT && d.variant != WENOX_FAST && d.variant != WENOX_TWIN)

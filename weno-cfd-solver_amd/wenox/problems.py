@@ -256,7 +256,11 @@ def dmr_reference_fill(q, bound, nx, ny):
 def double_mach(nx: int = 1920, ny: int = 480, gamma: float = 1.4):
     """Standard double Mach reflection on [0,4]x[0,1]: Mach-10 shock at 60 degrees to the wall through (1/6, 0); pre-shock
     (1.4, 0, 0, 1), post-shock (8, 8.25 cos30, -8.25 sin30, 116.5).
+    The solver uses one cell size for both directions and places the shock's
+    trace on the top boundary at height 1, so the grid must be 4:1.
     Returns (q, dx)."""
+    if nx != 4 * ny:
+        raise ValueError(f"double Mach reflection on [0,4]x[0,1] needs nx = 4 ny, got {nx}x{ny}")
     dx = 4.0 / nx
     q = np.zeros((nx + 6, ny + 6, 8))
     x = (np.arange(nx + 6) - 3 + 0.5) * dx
