@@ -27,8 +27,11 @@ from . import Solver, problems
 
 def _run(n, t_end, precision, variant, cfl):
     q0, dx = problems.isentropic_vortex(n)
+    # the twin variant (the NumPy twin's exact sqrt(3)/12 Gauss-point offset,
+    # WENO_HLLC_2d.py:5) runs on the unfused engine
+    engine = "unfused" if variant == "twin" else "fused"
     with Solver(n, n, dx, cfl=cfl, cons_switch=True, force_hlle=True, jp_cri=(0.2, 1.0),
-                bc="periodic", engine="fused", variant=variant, precision=precision) as s:
+                bc="periodic", engine=engine, variant=variant, precision=precision) as s:
         s.upload(q0)
         t0 = time.perf_counter()
         T, k = s.run(t_final=t_end)
@@ -89,20 +92,23 @@ def main(argv=None):
     ap.add_argument("--sizes", type=int, nargs="+", default=[256, 512, 1024, 2048, 4096])
     ap.add_argument("--t-end", type=float, default=2.0)
     ap.add_argument("--cfl", type=float, default=0.4)
+    ap.add_argument("--variant", default="fast", choices=["fast", "exact", "twin"],
+                    help="twin: the exact sqrt(3)/12 of the NumPy twin in place of the Rust crate's 0.1443375673")
+    ap.add_argument("--precisions", nargs="+", default=["fp64", "fp32split"])
     ap.add_argument("--out", default="")
     args = ap.parse_args(argv)
+    precs = tuple(args.precisions)
     rows = []
     for n in args.sizes:
         # one grid at a time so progress shows while the larger grids run
-        rows += vortex_study(sizes=(n,), t_end=args.t_end, cfl=args.cfl)
+        rows += vortex_study(sizes=(n,), t_end=args.t_end, cfl=args.cfl, variant=args.variant, precisions=precs)
         r = rows[-1]
-        print(f"n={n} fp64 L1={r['fp64']['L1']:.3e} Linf={r['fp64']['Linf']:.3e} steps={r['fp64']['steps']} "
-              f"fp32split L1={r['fp32split']['L1']:.3e} rel_vs_fp64={r['fp32split'].get('rel_L1_vs_fp64', 0):.2e}",
-              file=sys.stderr, flush=True)
-    for prec in ("fp64", "fp32split"):
+        print(f"n={n} " + " ".join(f"{p} L1={r[p]['L1']:.3e} Linf={r[p]['Linf']:.3e} L1_core={r[p]['L1_core']:.3e} "
+                                    f"steps={r[p]['steps']}" for p in precs), file=sys.stderr, flush=True)
+    for prec in precs:
         _orders(rows, prec)
     out = json.dumps({"study": "isentropic vortex, component-wise WENO5 + Gauss pass + HLLE, SSP-RK3",
-                      "t_end": args.t_end, "cfl": args.cfl, "rows": rows})
+                      "t_end": args.t_end, "cfl": args.cfl, "variant": args.variant, "rows": rows})
     if args.out:
         with open(args.out, "w") as f:
             f.write(out + "\n")
