@@ -49,6 +49,21 @@
 #define WX_SWEEP_OPS sweep_ops_exact
 #endif
 
+// Phase markers for static instruction counts (phase_counts.py builds with
+// -DWX_PHASE_MARKS=1): a scheduling barrier and an assembly comment at each
+// phase boundary of the sweep's march.  Off in the library build.
+#if WX_PHASE_MARKS
+#define WX_PHASE(name)                                 \
+    do {                                               \
+        __builtin_amdgcn_sched_barrier(0);             \
+        asm volatile(";@@phase " #name ::: "memory");  \
+    } while (0)
+#else
+#define WX_PHASE(name) \
+    do {               \
+    } while (0)
+#endif
+
 namespace wenox {
 namespace WX_NS {
 
@@ -328,6 +343,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
         }
 
         for (int64_t a = fa; a <= fb; ++a) {
+            WX_PHASE(window);
             // slide the window; row a+5 was fetched one face ahead, fetch row a+6
 #pragma unroll
             for (int s = 0; s < 5; ++s)
@@ -350,6 +366,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
             }
 
             // pass 1 (normal reconstruction) on every lane
+            WX_PHASE(pass1);
             {
                 T ql[4], qr[4];
                 if constexpr (MODE == MODE_DETECT) recon_normal_dev(DIR, CONS, w, gam, ql, qr);
@@ -364,6 +381,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
 
             T fc[4], fe[4], pb = T(1);
             T l1[4], l2[4], q1[4], q2[4];
+            WX_PHASE(pass2);
             if (gauss_lane) {
                 // pass 2: Gauss points along the face from lanes l-2..l+2
                 const T(*sp)[WAVE] = s_p1[wv];
@@ -380,6 +398,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                     }
                 }
             }
+            WX_PHASE(detect);
             if constexpr (MODE == MODE_DETECT) {
                 // A Gauss-point state with rho <= 0, p <= 0 or not finite on any
                 // lane: the wavefront recomputes this face row in FP64 (pass 1
@@ -403,6 +422,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                     }
                 }
             }
+            WX_PHASE(fluxes);
             if (gauss_lane) {
                 T fa_[4], fb_[4], fcc[4], fd[4];
                 flux_pt2(DIR, l1, l2, gam, fa_, fb_);
@@ -415,6 +435,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                     QL[k] = T(0.5) * (l1[k] + l2[k]);
                     QR[k] = T(0.5) * (q1[k] + q2[k]);
                 }
+                WX_PHASE(riemann);
                 FaceState<T> st = face_speeds(DIR, QL, QR, gam);
                 hlle_from(st, QL, FL, QR, FR, fe);
                 if (BLEND) {
@@ -427,6 +448,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                 for (int k = 0; k < 4; ++k) { fc[k] = T(0); fe[k] = T(0); }
             }
 
+            WX_PHASE(sensor_blend);
             if (!BLEND) {
                 if (W > 1 && a == r0 && wv > 0) {
 #pragma unroll
@@ -460,6 +482,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                 for (int k = 0; k < 4; ++k) { fc_prev[k] = fc[k]; fe_prev[k] = fe[k]; }
             }
             wave_sync();
+            WX_PHASE(end);
         }
 
         if (BLEND && fb == A.nn) {
