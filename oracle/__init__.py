@@ -77,6 +77,43 @@ def lib():
     return _lib
 
 
+_lib32 = None
+
+
+def lib32():
+    """libweno_oracle_f32.so: the same restatement built with every
+    floating-point value in FP32 (test instrumentation; float32 arrays)."""
+    global _lib32
+    if _lib32 is None:
+        path = os.path.join(_HERE, "libweno_oracle_f32.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        P = ctypes.POINTER(_View3)
+        f, dp, i = ctypes.c_float, ctypes.c_void_p, ctypes.c_int
+        L.orc_weno.argtypes = [P, f, i, i, dp, dp, dp, dp]
+        L.orc_blend.argtypes = [P, P, P, P, f, i, f, f, dp]
+        L.orc_riemann.argtypes = [P, P, P, P, f, i, i, dp]
+        for fn in (L.orc_weno, L.orc_blend, L.orc_riemann):
+            fn.restype = None
+        _lib32 = L
+    return _lib32
+
+
+def _view32(a):
+    s = [st // 4 for st in a.strides]
+    return _View3(a.ctypes.data, a.shape[0], a.shape[1], a.shape[2], s[0], s[1], s[2])
+
+
+def blend_f32(q_l, f_l, q_r, f_r, gamma, jp_lo, jp_hi, direction):
+    """hllc_hlle_{x,y}_rs_py (riemann.rs:412-486) in FP32 arithmetic on the
+    face states rounded to FP32; returns float64."""
+    fs = [np.ascontiguousarray(x, dtype=np.float32) for x in (q_l, f_l, q_r, f_r)]
+    o = np.zeros(fs[0].shape, dtype=np.float32)
+    lib32().orc_blend(*[ctypes.byref(_view32(x)) for x in fs], gamma, direction, jp_lo, jp_hi, o.ctypes.data)
+    return o.astype(np.float64)
+
+
 class count_limiter:
     """Context manager counting the faces where the positivity limiter's
     density pass and pressure pass acted (weno.rs:327-352) in the oracle

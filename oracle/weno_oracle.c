@@ -38,6 +38,19 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifdef ORC_FP32
+/* The FP32 build (libweno_oracle_f32.so, test instrumentation): every
+ * floating-point value and operation in single precision, so a test can
+ * measure how far FP32 arithmetic in one stage of the scheme moves the
+ * results.  Arrays passed to it are float32. */
+#define double float
+#define fmax fmaxf
+#define fmin fminf
+#define sqrt sqrtf
+#define fabs fabsf
+#define copysign copysignf
+#endif
+
 /* threads of the outer loops (1 = the reference's single thread) */
 static int orc_nthreads = 1;
 void orc_set_threads(int n) { orc_nthreads = n > 0 ? n : 1; }

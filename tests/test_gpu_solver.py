@@ -122,28 +122,35 @@ def test_solver_rhs_matches_l_local(oracle_mod):
 
 def test_fp32_split_accuracy(oracle_mod):
     """FP32-split (FP32 WENO arithmetic on increments, hi+lo state, FP64
-    fluxes, Riemann solver, combine and CFL) after 50 steps on the coarse 128^2
-    four-quadrant grid: relative L1 density error against the FP64 oracle
-    <= 5e-5.  The error comes from the FP32 characteristic projection of the
-    jumps (about 3e-6 of the local variation per face), so it scales with the
-    share of cells next to a discontinuity and with how many of them fall in
-    workgroups recomputed in FP64: 128^2 after 50 steps measures 2e-6 .. 1.2e-5
-    depending on the tiling, while the 4096^2 grid and the 1920x480 double Mach
-    reflection of the benchmark stay near 2e-9."""
+    fluxes, Riemann solver, combine and CFL; faces with an inadmissible FP32
+    Gauss-point state recomputed in FP64) after 50 steps on the coarse 128^2
+    four-quadrant grid: the same state bit for bit for every sweep tiling (the
+    FP64 fallback is chosen per face), and relative L1 density error against
+    the FP64 oracle <= 1e-5 (measured 5.0e-6).  This grid is the hard case:
+    the reference scheme itself turns it non-finite at step 293, and
+    perturbations of FP32 size grow by about 50x within 50 steps (an FP32
+    Riemann stage alone, on the oracle, gives 1.6e-5: test_fp32_sensitivity);
+    the 4096^2 grid and the 1920x480 double Mach reflection of the benchmark
+    stay near 1e-7 and 1e-9."""
     from wenox import Solver, problems
     n = 128
     q0, dx = problems.four_quadrant(n)
     print(f"[solver] fp32-split four-quadrant {n}^2 50 steps", flush=True)
     ref, ts = oracle_mod.run_riemann(q0.copy(), n, n, dx, 1.4, 0.4, False, False, (1e4, 1.1e4), nstep=50)
-    with Solver(n, n, dx, cfl=0.4, jp_cri=(1e4, 1.1e4), engine="fused", variant="fast",
-                precision="fp32split") as s:
-        s.upload(q0)
-        T, k = s.run(nstep=50)
-        got = s.download()
-    err = rel_l1(got, ref, n, n)
+    outs = []
+    for tiling in ((8, 8, 1, 1), (48, 48, 1, 1), (10, 9, 4, 4), (13, 7, 2, 1)):
+        with Solver(n, n, dx, cfl=0.4, jp_cri=(1e4, 1.1e4), engine="fused", variant="fast",
+                    precision="fp32split") as s:
+            s.set_tiling(*tiling)
+            s.upload(q0)
+            T, k = s.run(nstep=50)
+            assert k == 50
+            outs.append(s.download())
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
+    err = rel_l1(outs[0], ref, n, n)
     print("fp32-split rel L1 per variable", err)
-    assert k == 50
-    assert err[0] <= 5e-5, err
+    assert err[0] <= 1e-5, err
 
 
 def test_fp32_split_rhs(oracle_mod):

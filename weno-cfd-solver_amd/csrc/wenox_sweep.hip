@@ -146,12 +146,13 @@ template <typename T, int W> struct SweepLds {
 };
 
 // FP64 recompute of one face row of a wavefront (the FP32-split sweep's
-// fallback): pass 1 from the lane's stencil c, through the wavefront's pass-1
-// LDS rows p1, then pass 2 on the Gauss lanes; g receives the Gauss-point
-// states l1, l2, q1, q2 (4 each).
+// fallback): pass 1 from the lane's stencil c on every lane, through the
+// wavefront's pass-1 LDS rows p1, then pass 2 on the Gauss lanes whose face
+// is flagged (take); g holds the lane's Gauss-point states l1, l2, q1, q2 (4
+// each) and receives the FP64 ones where take is set.
 template <typename T, int DIR, bool CONS>
 __device__ __attribute__((noinline)) void face_row_fp64(const T (*c)[4], T gam, T (*p1)[WAVE], int l,
-                                                        bool gauss_lane, T* g)
+                                                        bool gauss_lane, bool take, T* g)
 {
     wave_sync();
     T ql[4], qr[4];
@@ -162,7 +163,7 @@ __device__ __attribute__((noinline)) void face_row_fp64(const T (*c)[4], T gam, 
         p1[4 + k][l] = qr[k];
     }
     wave_sync();
-    if (gauss_lane) {
+    if (gauss_lane && take) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             weno5_gq(p1[k][l - 2], p1[k][l - 1], p1[k][l], p1[k][l + 1], p1[k][l + 2], g[k], g[4 + k]);
@@ -402,7 +403,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
             if constexpr (MODE == MODE_DETECT) {
                 // A Gauss-point state with rho <= 0, p <= 0 or not finite on any
                 // lane: the wavefront recomputes this face row in FP64 (pass 1
-                // from the registers' stencil, then pass 2).
+                // from the registers' stencil, then pass 2), and the flagged
+                // faces take the FP64 states, so that every face's result is
+                // the same whatever tiling puts it in a row with others.
                 const bool badf = gauss_lane && (bad_state(l1, gam) || bad_state(l2, gam) || bad_state(q1, gam) ||
                                                  bad_state(q2, gam));
                 if (__ballot(badf ? 1 : 0) != 0ull) {
@@ -415,7 +418,11 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                     for (int r = 0; r < 6; ++r)
 #pragma unroll
                         for (int k = 0; k < 4; ++k) wc[r][k] = w[r][k];
-                    face_row_fp64<T, DIR, CONS>(wc, gam, s_p1[wv], l, gauss_lane, g);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        g[k] = l1[k]; g[4 + k] = l2[k]; g[8 + k] = q1[k]; g[12 + k] = q2[k];
+                    }
+                    face_row_fp64<T, DIR, CONS>(wc, gam, s_p1[wv], l, gauss_lane, badf, g);
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
                         l1[k] = g[k]; l2[k] = g[4 + k]; q1[k] = g[8 + k]; q2[k] = g[12 + k];
