@@ -126,7 +126,7 @@ def test_fp32_split_accuracy(oracle_mod):
     Gauss-point state recomputed in FP64) after 50 steps on the coarse 128^2
     four-quadrant grid: the same state bit for bit for every sweep tiling (the
     FP64 fallback is chosen per face), and relative L1 density error against
-    the FP64 oracle <= 1e-5 (measured 5.0e-6).  This grid is the hard case:
+    the FP64 oracle <= 1e-5 (measured 2.7e-6).  This grid is the hard case:
     the reference scheme itself turns it non-finite at step 293, and
     perturbations of FP32 size grow by about 50x within 50 steps (an FP32
     Riemann stage alone, on the oracle, gives 1.6e-5: test_fp32_sensitivity);

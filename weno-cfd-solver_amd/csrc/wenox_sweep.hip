@@ -408,8 +408,14 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                 // the same whatever tiling puts it in a row with others.
                 const bool badf = gauss_lane && (bad_state(l1, gam) || bad_state(l2, gam) || bad_state(q1, gam) ||
                                                  bad_state(q2, gam));
-                if (__ballot(badf ? 1 : 0) != 0ull) {
+                const unsigned long long bm = __ballot(badf ? 1 : 0);
+                if (bm != 0ull) {
                     bad = true;
+                    // the flagged faces and their neighbours across the face
+                    // (lanes l +- 1, Gauss lanes of the same wave for every
+                    // output lane 3..60) take the FP64 states
+                    const unsigned long long nb = bm | (bm << 1) | (bm >> 1);
+                    const bool take = gauss_lane && ((nb >> l) & 1ull);
                     // an out-of-line call with copies of the stencil and the
                     // results, so the FP32 path's registers are not shared
                     // with the FP64 recompute
@@ -422,7 +428,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                     for (int k = 0; k < 4; ++k) {
                         g[k] = l1[k]; g[4 + k] = l2[k]; g[8 + k] = q1[k]; g[12 + k] = q2[k];
                     }
-                    face_row_fp64<T, DIR, CONS>(wc, gam, s_p1[wv], l, gauss_lane, badf, g);
+                    face_row_fp64<T, DIR, CONS>(wc, gam, s_p1[wv], l, gauss_lane, take, g);
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
                         l1[k] = g[k]; l2[k] = g[4 + k]; q1[k] = g[8 + k]; q2[k] = g[12 + k];
