@@ -408,14 +408,13 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                 // the same whatever tiling puts it in a row with others.
                 const bool badf = gauss_lane && (bad_state(l1, gam) || bad_state(l2, gam) || bad_state(q1, gam) ||
                                                  bad_state(q2, gam));
-                const unsigned long long bm = __ballot(badf ? 1 : 0);
-                if (bm != 0ull) {
+                if (__ballot(badf ? 1 : 0) != 0ull) {
                     bad = true;
-                    // the flagged faces and their neighbours across the face
-                    // (lanes l +- 1, Gauss lanes of the same wave for every
-                    // output lane 3..60) take the FP64 states
-                    const unsigned long long nb = bm | (bm << 1) | (bm >> 1);
-                    const bool take = gauss_lane && ((nb >> l) & 1ull);
+                    // only the flagged faces take the FP64 states: a face's
+                    // choice depends on its own states alone (a neighbour's
+                    // flag at the wave's edge lanes is not known to the wave,
+                    // and the sensor reads the edge lanes' p̄)
+                    const bool take = badf;
                     // an out-of-line call with copies of the stencil and the
                     // results, so the FP32 path's registers are not shared
                     // with the FP64 recompute

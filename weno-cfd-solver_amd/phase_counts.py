@@ -35,7 +35,7 @@ def build_asm(out_dir):
     asm = os.path.join(out_dir, "sweep.s")
     cmd = [hipcc, "-O3", "-std=c++17", "--offload-arch=gfx950", "--cuda-device-only", "-S",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument",
-           "-DWX_FAST=1", "-DWX_PHASE_MARKS=1", "-ffp-contract=fast", SRC, "-o", asm]
+           "-DWX_FAST=1", "-DWX_PHASE_MARKS=1", "-ffp-contract=on", SRC, "-o", asm]
     subprocess.run(cmd, check=True, capture_output=True)
     return asm
 
