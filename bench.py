@@ -394,17 +394,17 @@ _QUAD_SRC32 = "profiles/quad4096_fp32split.md"
 _KH_SRC64 = "profiles/kh8192_fp64.md"
 _KH_SRC32 = "profiles/kh8192_fp32split.md"
 TRAFFIC = {
-    ("double_mach_reflection", 1920, 480, False): {"bytes_per_step": 1.0611e9, "tiling": (10, 9, 4, 4),
+    ("double_mach_reflection", 1920, 480, False): {"bytes_per_step": 1.0506e9, "tiling": (10, 9, 4, 4),
                                                    "source": _DMR_SRC64},
-    ("double_mach_reflection", 1920, 480, True): {"bytes_per_step": 1.1225e9, "tiling": (10, 8, 2, 1),
+    ("double_mach_reflection", 1920, 480, True): {"bytes_per_step": 1.1188e9, "tiling": (10, 8, 2, 1),
                                                   "source": _DMR_SRC32},
-    ("four_quadrant_riemann", 4096, 4096, False): {"bytes_per_step": 16.7539e9, "tiling": (48, 48, 1, 1),
+    ("four_quadrant_riemann", 4096, 4096, False): {"bytes_per_step": 16.7482e9, "tiling": (48, 48, 1, 1),
                                                    "source": _QUAD_SRC64},
-    ("four_quadrant_riemann", 4096, 4096, True): {"bytes_per_step": 15.4477e9, "tiling": (48, 48, 1, 1),
+    ("four_quadrant_riemann", 4096, 4096, True): {"bytes_per_step": 15.4459e9, "tiling": (48, 48, 1, 1),
                                                   "source": _QUAD_SRC32},
-    ("kelvin_helmholtz", 8192, 8192, False): {"bytes_per_step": 66.7026e9, "tiling": (57, 57, 1, 1),
+    ("kelvin_helmholtz", 8192, 8192, False): {"bytes_per_step": 66.6840e9, "tiling": (57, 57, 1, 1),
                                               "source": _KH_SRC64},
-    ("kelvin_helmholtz", 8192, 8192, True): {"bytes_per_step": 60.8258e9, "tiling": (57, 57, 1, 1),
+    ("kelvin_helmholtz", 8192, 8192, True): {"bytes_per_step": 61.1511e9, "tiling": (57, 57, 1, 1),
                                              "source": _KH_SRC32},
 }
 # VALU wave-instructions of a stage's sweeps as the step launches them (one
@@ -412,17 +412,17 @@ TRAFFIC = {
 # differ; FP32-split: including the in-launch FP64 recompute of flagged face
 # rows)
 VALU = {
-    ("double_mach_reflection", 1920, 480, False, "sweeps"): {"wave_instr_per_launch": 5.092e7,
+    ("double_mach_reflection", 1920, 480, False, "sweeps"): {"wave_instr_per_launch": 5.17e7,
                                                              "tiling": (10, 9, 4, 4), "source": _DMR_SRC64},
-    ("double_mach_reflection", 1920, 480, True, "sweeps"): {"wave_instr_per_launch": 3.442e7 + 3.757e7,
+    ("double_mach_reflection", 1920, 480, True, "sweeps"): {"wave_instr_per_launch": 3.479e7 + 3.795e7,
                                                             "tiling": (10, 8, 2, 1), "source": _DMR_SRC32},
-    ("four_quadrant_riemann", 4096, 4096, False, "sweeps"): {"wave_instr_per_launch": 8.207e8,
+    ("four_quadrant_riemann", 4096, 4096, False, "sweeps"): {"wave_instr_per_launch": 8.335e8,
                                                              "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64},
-    ("four_quadrant_riemann", 4096, 4096, True, "sweeps"): {"wave_instr_per_launch": 1.014e9,
+    ("four_quadrant_riemann", 4096, 4096, True, "sweeps"): {"wave_instr_per_launch": 1.023e9,
                                                             "tiling": (48, 48, 1, 1), "source": _QUAD_SRC32},
-    ("kelvin_helmholtz", 8192, 8192, False, "sweeps"): {"wave_instr_per_launch": 3.362e9,
+    ("kelvin_helmholtz", 8192, 8192, False, "sweeps"): {"wave_instr_per_launch": 3.422e9,
                                                         "tiling": (57, 57, 1, 1), "source": _KH_SRC64},
-    ("kelvin_helmholtz", 8192, 8192, True, "sweeps"): {"wave_instr_per_launch": 4.129e9,
+    ("kelvin_helmholtz", 8192, 8192, True, "sweeps"): {"wave_instr_per_launch": 4.178e9,
                                                        "tiling": (57, 57, 1, 1), "source": _KH_SRC32},
 }
 
