@@ -80,7 +80,10 @@ def make_state(w):
     return problems.kelvin_helmholtz(w["nx"])
 
 
-def run_gpu(w, q, dx, steps, warmup, engine, variant, precision, profile=True, tune=True, chunks=None):
+def run_gpu(w, q, dx, steps, warmup, engine, variant, precision, profile=True, tune=True, chunks=None,
+            layout=None):
+    """layout: "row" or "blocked" state layout of the fused FP64 fast step;
+    None: the autotuner picks it (with `chunks` given: row-major)."""
     import numpy as np
     from wenox import Solver
     s = Solver(w["nx"], w["ny"], dx, gamma=1.4, cfl=w["cfl"], cons_switch=w["cons"],
@@ -92,8 +95,9 @@ def run_gpu(w, q, dx, steps, warmup, engine, variant, precision, profile=True, t
         if engine == "fused":
             if chunks:
                 s.set_tiling(*chunks)
+                s.set_blocked(layout == "blocked")
             elif tune:
-                tuned = s.autotune()
+                tuned = s.autotune(layouts=None if layout is None else (layout == "blocked",))
         s.run(nstep=warmup)
         # A workload with a `window` is timed in segments of at most that many
         # steps, each from the initial state (re-uploaded between segments,
@@ -122,8 +126,10 @@ def run_gpu(w, q, dx, steps, warmup, engine, variant, precision, profile=True, t
                     "sweep_y": {f"w{w}c{c}": round(v, 5) for (w, c), v in tuned["sweep_y"].items()},
                     "pairs": {"x{0}w{2}_y{1}w{3}".format(*t): round(v, 5) for t, v in tuned["pairs"].items()},
                     "combine_cols": {str(c): round(v, 5) for c, v in tuned["combine"].items()},
-                    "pipeline_stage": {str(c): round(v, 5) for c, v in tuned.get("pipeline", {}).items()}}
+                    "pipeline_stage": {str(c): round(v, 5) for c, v in tuned.get("pipeline", {}).items()},
+                    "layout_step_ms": {k: round(v, 5) for k, v in tuned["step_ms"].items()}}
         if engine == "fused":
+            out["layout"] = "blocked" if s.blocked else "row"
             out["combine_cols"] = s.combine_cols
             out["pipeline"] = s.pipeline
         if profile and engine == "fused":
@@ -301,16 +307,20 @@ def objective_rate(w, q, dx, steps, reps=5):
     return out
 
 
-def kernel_bytes(w, split, fold=False):
+def kernel_bytes(w, split, fold=False, blocked=False):
     """Algorithmic HBM bytes of each launch of a step (state and D arrays
     touched once each; halo re-reads counted 0), averaged over the three
     stages.  fold: the fast FP64 step, whose x sweep also reads U^n in stages
     2 and 3 and writes its part of the stage combination, so that the combine
-    reads only that and Dy^T."""
+    reads only that and Dy^T.  blocked: the blocked fast FP64 step, whose y
+    sweep reads that part and writes the new state (no combine launch)."""
     cells = w["nx"] * w["ny"]
     padded = (w["nx"] + 6) * (w["ny"] + 6)
     d = 16 if split else 32            # D planes: 4 x FP32 or 4 x FP64 per cell
     st = 32                            # state: 4 x FP64, or FP32 hi + lo
+    if blocked:
+        return {"sweep_x": st * padded + (2 * st / 3) * cells + d * cells,
+                "sweep_y": st * padded + d * cells + st * cells, "cfl": st * cells}
     if fold:
         return {"sweep_x": st * padded + (2 * st / 3) * cells + d * cells, "sweep_y": st * padded + d * cells,
                 "combine": 2 * d * cells + 2 * st * cells, "cfl": st * cells}
@@ -335,9 +345,11 @@ def roofline(res, w, split=False):
     achieved = cells * BYTES_PER_CELL_UPDATE / (step_ms * 1e-3)
     tiling = tuple(res.get("chunks") or ())
     fold = not split and res.get("variant") == "fast"
-    # the counter tables hold the fused engine's fast-variant kernels
+    blocked = res.get("layout") == "blocked"
+    # the counter tables hold the fused engine's fast-variant kernels, per layout
     profiled = res.get("variant") == "fast" and bool(tiling)
-    tr = TRAFFIC.get((w["name"], w["nx"], w["ny"], split)) if profiled else None
+    layout = res.get("layout") or "row"
+    tr = TRAFFIC.get((w["name"], w["nx"], w["ny"], split, layout)) if profiled else None
     out = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
            "frac": achieved / HBM_PEAK, "traffic": tr["bytes_per_step"] if tr else None,
            "kernel": "one SSP-RK3 step (its kernels, back to back)",
@@ -348,7 +360,7 @@ def roofline(res, w, split=False):
         out["traffic_source"] = tr["source"]
         out["traffic_tiling"] = list(tr["tiling"])
     if km:
-        kb = kernel_bytes(w, split, fold)
+        kb = kernel_bytes(w, split, fold, blocked)
         per = {}
         for k, launches in (("sweep_x", 3), ("sweep_y", 3), ("combine", 3), ("cfl", 1)):
             # FP64: the CFL maxima come out of the last combine's registers;
@@ -365,13 +377,15 @@ def roofline(res, w, split=False):
         # when their tilings share a wave count; that launch dominates the step
         sw_bytes = kb["sweep_x"] + kb["sweep_y"]
         one = len(tiling) == 4 and tiling[2] == tiling[3]
-        out["dominant_kernel"] = {"name": "k_sweep_xy (x and y sweeps of a stage, one launch)" if one
-                                  else "k_sweep x then y (two launches per stage)",
+        name = ("k_sweep x (part E of the stage) then k_sweep y (the new state): two launches per stage"
+                if blocked else "k_sweep_xy (x and y sweeps of a stage, one launch)" if one
+                else "k_sweep x then y (two launches per stage)")
+        out["dominant_kernel"] = {"name": name,
                                   "avg_ms": round(km["sweeps"], 5), "launches_per_step": 3,
                                   "algorithmic_bytes": sw_bytes,
                                   "achieved_GBps": round(sw_bytes / (km["sweeps"] * 1e-3) / 1e9, 1),
                                   "share_of_step": round(3 * km["sweeps"] / step_ms, 4)}
-        v = VALU.get((w["name"], w["nx"], w["ny"], split, "sweeps"))
+        v = VALU.get((w["name"], w["nx"], w["ny"], split, layout, "sweeps"))
         if v and tuple(v["tiling"]) == tiling and profiled:
             rate = v["wave_instr_per_launch"] / (km["sweeps"] * 1e-3)
             out["dominant_kernel"]["valu"] = {"wave_instr_per_launch": v["wave_instr_per_launch"],
@@ -390,40 +404,45 @@ def roofline(res, w, split=False):
 _DMR_SRC64 = "profiles/dmr1920x480_fp64.md"
 _DMR_SRC32 = "profiles/dmr1920x480_fp32split.md"
 _QUAD_SRC64 = "profiles/quad4096_fp64.md"
+_QUAD_SRC64B = "profiles/quad4096_fp64_blocked.md"
 _QUAD_SRC32 = "profiles/quad4096_fp32split.md"
 _KH_SRC64 = "profiles/kh8192_fp64.md"
 _KH_SRC32 = "profiles/kh8192_fp32split.md"
 TRAFFIC = {
-    ("double_mach_reflection", 1920, 480, False): {"bytes_per_step": 1.0506e9, "tiling": (10, 9, 4, 4),
-                                                   "source": _DMR_SRC64},
-    ("double_mach_reflection", 1920, 480, True): {"bytes_per_step": 1.1188e9, "tiling": (10, 8, 2, 1),
-                                                  "source": _DMR_SRC32},
-    ("four_quadrant_riemann", 4096, 4096, False): {"bytes_per_step": 16.7482e9, "tiling": (48, 48, 1, 1),
-                                                   "source": _QUAD_SRC64},
-    ("four_quadrant_riemann", 4096, 4096, True): {"bytes_per_step": 15.4459e9, "tiling": (48, 48, 1, 1),
-                                                  "source": _QUAD_SRC32},
-    ("kelvin_helmholtz", 8192, 8192, False): {"bytes_per_step": 66.6840e9, "tiling": (57, 57, 1, 1),
-                                              "source": _KH_SRC64},
-    ("kelvin_helmholtz", 8192, 8192, True): {"bytes_per_step": 61.1511e9, "tiling": (57, 57, 1, 1),
-                                             "source": _KH_SRC32},
+    ("double_mach_reflection", 1920, 480, False, "row"):
+        {"bytes_per_step": 1.0506e9, "tiling": (10, 9, 4, 4), "source": _DMR_SRC64},
+    ("double_mach_reflection", 1920, 480, True, "row"):
+        {"bytes_per_step": 1.1188e9, "tiling": (10, 8, 2, 1), "source": _DMR_SRC32},
+    ("four_quadrant_riemann", 4096, 4096, False, "row"):
+        {"bytes_per_step": 16.7482e9, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64},
+    ("four_quadrant_riemann", 4096, 4096, False, "blocked"):
+        {"bytes_per_step": 30.0182e9, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64B},
+    ("four_quadrant_riemann", 4096, 4096, True, "row"):
+        {"bytes_per_step": 15.4459e9, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC32},
+    ("kelvin_helmholtz", 8192, 8192, False, "row"):
+        {"bytes_per_step": 66.6840e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC64},
+    ("kelvin_helmholtz", 8192, 8192, True, "row"):
+        {"bytes_per_step": 61.1511e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC32},
 }
 # VALU wave-instructions of a stage's sweeps as the step launches them (one
 # k_sweep_xy, or the x and y k_sweep launches together when their wave counts
-# differ; FP32-split: including the in-launch FP64 recompute of flagged face
-# rows)
+# differ or the layout is blocked; FP32-split: including the in-launch FP64
+# recompute of flagged face rows)
 VALU = {
-    ("double_mach_reflection", 1920, 480, False, "sweeps"): {"wave_instr_per_launch": 5.17e7,
-                                                             "tiling": (10, 9, 4, 4), "source": _DMR_SRC64},
-    ("double_mach_reflection", 1920, 480, True, "sweeps"): {"wave_instr_per_launch": 3.479e7 + 3.795e7,
-                                                            "tiling": (10, 8, 2, 1), "source": _DMR_SRC32},
-    ("four_quadrant_riemann", 4096, 4096, False, "sweeps"): {"wave_instr_per_launch": 8.335e8,
-                                                             "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64},
-    ("four_quadrant_riemann", 4096, 4096, True, "sweeps"): {"wave_instr_per_launch": 1.023e9,
-                                                            "tiling": (48, 48, 1, 1), "source": _QUAD_SRC32},
-    ("kelvin_helmholtz", 8192, 8192, False, "sweeps"): {"wave_instr_per_launch": 3.422e9,
-                                                        "tiling": (57, 57, 1, 1), "source": _KH_SRC64},
-    ("kelvin_helmholtz", 8192, 8192, True, "sweeps"): {"wave_instr_per_launch": 4.178e9,
-                                                       "tiling": (57, 57, 1, 1), "source": _KH_SRC32},
+    ("double_mach_reflection", 1920, 480, False, "row", "sweeps"):
+        {"wave_instr_per_launch": 5.17e7, "tiling": (10, 9, 4, 4), "source": _DMR_SRC64},
+    ("double_mach_reflection", 1920, 480, True, "row", "sweeps"):
+        {"wave_instr_per_launch": 3.479e7 + 3.795e7, "tiling": (10, 8, 2, 1), "source": _DMR_SRC32},
+    ("four_quadrant_riemann", 4096, 4096, False, "row", "sweeps"):
+        {"wave_instr_per_launch": 8.335e8, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64},
+    ("four_quadrant_riemann", 4096, 4096, False, "blocked", "sweeps"):
+        {"wave_instr_per_launch": 4.187e8 + 4.171e8, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64B},
+    ("four_quadrant_riemann", 4096, 4096, True, "row", "sweeps"):
+        {"wave_instr_per_launch": 1.023e9, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC32},
+    ("kelvin_helmholtz", 8192, 8192, False, "row", "sweeps"):
+        {"wave_instr_per_launch": 3.422e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC64},
+    ("kelvin_helmholtz", 8192, 8192, True, "row", "sweeps"):
+        {"wave_instr_per_launch": 4.178e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC32},
 }
 
 
@@ -471,10 +490,13 @@ def workload_block(key, steps, warmup):
     w = WORKLOADS[key]
     q, dx = make_state(w)
     r64 = run_gpu(w, q, dx, steps, warmup, "fused", "fast", "fp64")
-    rex = run_gpu(w, q, dx, steps, warmup, "fused", "exact", "fp64", profile=False, chunks=r64["chunks"])
+    rex = run_gpu(w, q, dx, steps, warmup, "fused", "exact", "fp64", profile=False, chunks=r64["chunks"],
+                  layout="row")
     out = {"workload": describe(w), "steps": steps, "warmup": warmup,
            "value_fp64": r64["value"], "ms_per_step_fp64": r64["seconds"] / r64["steps"] * 1e3,
-           "value_fp64_exact": rex["value"], "chunks": r64["chunks"], "state_finite": r64["finite"] and rex["finite"],
+           "value_fp64_exact": rex["value"], "chunks": r64["chunks"], "layout": r64["layout"],
+           "autotune_layout_step_ms": (r64.get("autotune_ms") or {}).get("layout_step_ms"),
+           "state_finite": r64["finite"] and rex["finite"],
            "roofline_fp64": roofline(r64, w)}
     out["fp32_split"] = split_block(w, q, dx, steps, warmup, r64)
     return out
@@ -516,6 +538,8 @@ def main():
     ap.add_argument("--chunks", type=int, nargs=4, metavar=("CX", "CY", "WX", "WY"),
                     help="sweep tiling: rows per wavefront and wavefronts per workgroup of the x / y "
                          "sweeps (skips the autotuner)")
+    ap.add_argument("--layout", default="auto", choices=["auto", "row", "blocked"],
+                    help="state layout of the FP64 fast step (auto: the autotuner's choice; with --chunks: row)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -551,8 +575,14 @@ def main():
         print(json.dumps(res), flush=True)
         return
     q, dx = make_state(w)
+    fast64 = args.engine == "fused" and args.variant == "fast" and args.precision == "fp64"
+    layout = None if args.layout == "auto" else args.layout
+    if not fast64:
+        layout = "row"
+    elif layout is None and args.chunks:
+        layout = "row"
     r64 = run_gpu(w, q, dx, args.steps, args.warmup, args.engine, args.variant, args.precision,
-                  chunks=args.chunks)
+                  chunks=args.chunks, layout=layout)
     res = {
         "metric": METRIC,
         "value": r64["value"],
@@ -567,8 +597,8 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else SPLIT_DTYPE,
         "data": "synthetic",
         "config": {"workload": describe(w), "engine": args.engine, "variant": args.variant,
-                   "chunks": r64.get("chunks"), "combine_cols": r64.get("combine_cols"),
-                   "pipeline": r64.get("pipeline"),
+                   "chunks": r64.get("chunks"), "layout": r64.get("layout"),
+                   "combine_cols": r64.get("combine_cols"), "pipeline": r64.get("pipeline"),
                    "autotune_ms": r64.get("autotune_ms")},
         "state_finite": r64["finite"],
         "timed_segments": r64["segments"],
@@ -577,7 +607,8 @@ def main():
     }
     if not args.no_extra and args.engine == "fused" and args.precision == "fp64":
         # the parity variant on the same grid and tiling
-        rex = run_gpu(w, q, dx, args.steps, args.warmup, "fused", "exact", "fp64", chunks=r64.get("chunks"))
+        rex = run_gpu(w, q, dx, args.steps, args.warmup, "fused", "exact", "fp64", chunks=r64.get("chunks"),
+                      layout="row")
         res["exact_variant"] = {"value": rex["value"], "ms_per_step": rex["seconds"] / rex["steps"] * 1e3,
                                 "speed_vs_fast": rex["value"] / r64["value"],
                                 "density_fast_vs_exact": _dist(r64["state"][..., 0], rex["state"][..., 0]),
@@ -592,7 +623,7 @@ def main():
                 for jp, tag in (((1e4, 1.1e4), "hllc"), ((0.5, 2.0), "blend_jp0.5_2")):
                     wv = dict(w, cons=cons, jp=jp)
                     rv = run_gpu(wv, q, dx, 16, 8, "fused", "fast", "fp64", profile=False,
-                                 chunks=r64.get("chunks"))
+                                 chunks=r64.get("chunks"), layout=r64.get("layout"))
                     variants[("componentwise_" if cons else "characteristic_") + tag] = {
                         "value": rv["value"], "ms_per_step": rv["seconds"] / rv["steps"] * 1e3,
                         "state_finite": rv["finite"]}

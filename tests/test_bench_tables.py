@@ -48,9 +48,10 @@ def test_traffic_table_matches_profiles(bench):
         m = re.search(r"= \*\*([0-9.]+) MB\*\*", text)
         assert m, entry["source"]
         assert abs(float(m.group(1)) * 1e6 - entry["bytes_per_step"]) <= 1e5, (key, m.group(1))
-        # the summary was made on the tiling the table names
+        # the summary was made on the tiling and in the layout the table names
         cx, cy, wx, wy = entry["tiling"]
         assert f"--chunks {cx} {cy} {wx} {wy}" in text, key
+        assert ("--layout blocked" in text) == (key[4] == "blocked"), key
 
 
 def test_valu_table_matches_profiles(bench):
@@ -62,6 +63,7 @@ def test_valu_table_matches_profiles(bench):
         assert abs(got - entry["wave_instr_per_launch"]) <= 5e-4 * got, (key, got, entry["wave_instr_per_launch"])
         cx, cy, wx, wy = entry["tiling"]
         assert f"--chunks {cx} {cy} {wx} {wy}" in text, key
+        assert ("--layout blocked" in text) == (key[4] == "blocked"), key
 
 
 def test_profile_commands_name_their_workload():

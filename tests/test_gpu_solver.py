@@ -492,3 +492,132 @@ def test_pipeline_parts_bitwise(bc, nx, ny):
     for parts, (q, T, k) in out.items():
         assert k == k1 and T == T1, (parts, T, T1)
         np.testing.assert_array_equal(q, q1, err_msg=f"parts {parts}")
+
+
+def _layout_case(bc):
+    from wenox import problems
+    kw = dict(cfl=0.4, jp_cri=(0.5, 2.0), bc=bc, engine="fused", variant="fast")
+    if bc == "dmr_ref":
+        nx, ny = 80, 20
+        q0, bound = problems.dmr_reference(nx, ny)
+        dx = 1.0 / ny
+        kw.update(dmr_bound=bound, cfl=0.1, jp_cri=(10000, 11000))
+    elif bc == "dmr":
+        nx, ny = 136, 34
+        q0, dx = problems.double_mach(nx, ny)
+        kw.update(cfl=0.1)
+    elif bc == "periodic":
+        nx, ny = 96, 70
+        q0, dx = problems.kelvin_helmholtz(nx)
+        q0 = q0[:, :ny + 6].copy()
+    else:
+        nx, ny = 45, 37
+        q0, dx = problems.four_quadrant(nx, ny=ny)
+    return nx, ny, dx, q0, kw
+
+
+@pytest.mark.parametrize("bc", ["mirror", "periodic", "copy", "dmr_ref", "dmr"])
+@pytest.mark.parametrize("cons,force_hlle", [(False, False), (True, False), (False, True)])
+def test_blocked_layout_same_states(bc, cons, force_hlle):
+    """The blocked FP64 fast step (both sweeps on one blocked copy of the
+    state, the y sweep finishing each stage, the CFL maxima from its stage-3
+    workgroups) gives the same states as the row-major step bit for bit:
+    the downloaded arrays (ghost cells included), the time and the step
+    count after 9 steps, for every boundary rule, both reconstructions,
+    HLLE alone, and tilings with 1, 2 and 4 wavefronts per workgroup."""
+    from wenox import Solver
+    nx, ny, dx, q0, kw = _layout_case(bc)
+    kw.update(cons_switch=cons, force_hlle=force_hlle)
+    print(f"[blocked] bc={bc} {nx}x{ny} cons={cons} hlle={force_hlle}", flush=True)
+    ref = None
+    for blocked, tiling in ((False, (9, 7, 1, 1)), (True, (9, 7, 1, 1)), (True, (5, 6, 2, 2)),
+                            (True, (4, 3, 4, 4)), (True, (11, 4, 1, 4))):
+        with Solver(nx, ny, dx, **kw) as s:
+            s.set_tiling(*tiling)
+            s.set_blocked(blocked)
+            assert s.blocked == blocked
+            s.upload(q0)
+            T, k = s.run(nstep=9)
+            got = (s.download(), T, k)
+        if ref is None:
+            ref = got
+            # component-wise reconstruction has no positivity limiter, and the
+            # double Mach reflection turns non-finite with it: there the
+            # comparison takes NaN for NaN
+            if not (cons and bc.startswith("dmr")):
+                assert np.isfinite(ref[0]).all()
+            continue
+        assert got[2] == ref[2], (tiling, got[2], ref[2])
+        assert got[1] == ref[1] or (got[1] != got[1] and ref[1] != ref[1]), (tiling, got[1], ref[1])
+        np.testing.assert_array_equal(got[0], ref[0], err_msg=f"tiling {tiling}")
+
+
+def test_blocked_layout_switches_and_reads():
+    """Switching the layout between runs, and reading the state of a blocked
+    run (download, conserved sums, RHS) in between, leaves the trajectory
+    that of the row-major step; t_final runs end at the same time."""
+    from wenox import Solver
+    nx, ny, dx, q0, kw = _layout_case("mirror")
+    print("[blocked] switching layouts between runs", flush=True)
+    with Solver(nx, ny, dx, **kw) as s:
+        s.upload(q0)
+        s.run(nstep=5)
+        s.run(t_final=0.05)
+        ref = (s.download(), s.conserved_sums(), s.rhs(), s.run(nstep=4))
+        ref_q = s.download()
+    with Solver(nx, ny, dx, **kw) as s:
+        s.set_blocked(True)
+        s.upload(q0)
+        s.run(nstep=2)
+        s.set_blocked(False)
+        s.run(nstep=3)
+        s.set_blocked(True)
+        s.run(t_final=0.05)
+        got = (s.download(), s.conserved_sums(), s.rhs(), s.run(nstep=4))
+        got_q = s.download()
+    np.testing.assert_array_equal(got[0], ref[0])
+    np.testing.assert_array_equal(got[1], ref[1])
+    np.testing.assert_array_equal(got[2], ref[2])
+    assert got[3] == ref[3]
+    np.testing.assert_array_equal(got_q, ref_q)
+
+
+def test_blocked_layout_argument_checks():
+    from wenox import Solver, problems
+    q0, dx = problems.four_quadrant(16)
+    for kw in (dict(engine="unfused"), dict(engine="fused", variant="exact"),
+               dict(engine="fused", precision="fp32split")):
+        with Solver(16, 16, dx, **kw) as s:
+            with pytest.raises(ValueError, match="blocked layout"):
+                s.set_blocked(True)
+            assert not s.blocked
+
+
+def test_autotune_picks_a_layout():
+    """The autotuner tunes the tiling of both state layouts of the FP64 fast
+    step, keeps the one with the faster tuned step, and the run that follows
+    gives the row-major step's states (both layouts and every tiling round
+    alike)."""
+    from wenox import Solver, problems
+    n = 128
+    q0, dx = problems.four_quadrant(n)
+    print(f"[autotune] layouts, four-quadrant {n}^2", flush=True)
+    kw = dict(cfl=0.4, jp_cri=(0.5, 2.0), bc="mirror", engine="fused", variant="fast")
+    with Solver(n, n, dx, **kw) as s:
+        s.upload(q0)
+        r = s.autotune(candidates=((1, 8), (1, 16), (2, 6), (4, 4)), reps=1)
+        assert set(r["step_ms"]) == {"row", "blocked"}
+        assert s.blocked == r["blocked"]
+        assert all(v > 0 for v in r["step_ms"].values())
+        s.run(nstep=6)
+        got = s.download()
+    with Solver(n, n, dx, **kw) as s:
+        s.upload(q0)
+        s.run(nstep=6)
+        ref = s.download()
+    np.testing.assert_array_equal(got, ref)
+    # FP32-split and the exact variant have one layout to tune
+    with Solver(n, n, dx, **dict(kw, precision="fp32split")) as s:
+        s.upload(q0)
+        r = s.autotune(candidates=((1, 8), (2, 6)), reps=1)
+        assert set(r["step_ms"]) == {"row"} and not r["blocked"]

@@ -541,6 +541,14 @@ struct wenox_solver {
     // fused engine: side stream and events of the slab-pipelined stage (plan.parts)
     hipStream_t side = nullptr;
     hipEvent_t pipe_ev[PIPE_MAX_PARTS + 1] = {};
+    // blocked layout of the fast FP64 step (wenox_solver_set_layout): Ub holds
+    // the state, Ab and Bb the stage states, Eb the x sweep's part E of each
+    // stage, all blocked planes (plan.nb, plan.Pb) in one allocation; the
+    // row-major U / U^T are brought up to date by sync_rows after steps ran
+    bool blk = false;
+    bool rows_stale = false;
+    double* blk_mem = nullptr;
+    double *Ub = nullptr, *Ab = nullptr, *Bb = nullptr, *Eb = nullptr;
 };
 
 namespace {
@@ -550,11 +558,13 @@ int64_t plane(const wenox_solver* s) { return (s->d.nx + 6) * s->ld; }
 int64_t plane_t(const wenox_solver* s) { return (s->d.ny + 6) * s->ldT; }
 
 // Entries of the CFL maxima array: CFL_SLOTS row blocks of k_cfl, or one per
-// 32 x 32 tile of the fused engine's last RK combine.
-size_t cfl_slots(const wenox_solver_desc& d)
+// 32 x 32 tile of the fused engine's last RK combine, or (FP64 fused engine)
+// one per workgroup of the blocked step's stage-3 y sweep.
+size_t cfl_slots(const wenox_solver_desc& d, const SweepPlan* blk_plan)
 {
     const size_t tiles = (size_t)((d.nx + 31) / 32) * (size_t)((d.ny + 31) / 32);
-    return std::max<size_t>(CFL_SLOTS, tiles);
+    const size_t blk = blk_plan ? (size_t)blk_part_slots(*blk_plan) : 0;
+    return std::max<size_t>(std::max<size_t>(CFL_SLOTS, tiles), blk);
 }
 
 // Steps per replay of the multi-step graph: consecutive graph launches leave
@@ -573,6 +583,16 @@ void drop_graphs(wenox_solver* s)
 // FP64 view of work with the state geometry (the split download's staging copy)
 double* work_state(const wenox_solver* s) { return s->work + STATE_SHIFT; }
 
+// After blocked steps: the row-major state U and its transposed copy from the
+// blocked one (ghost cells included), for the calls that read U.
+void sync_rows(wenox_solver* s)
+{
+    if (!s->blk || !s->rows_stale) return;
+    s->sops->from_blk(s->plan, s->Ub, s->U, s->stream);
+    s->sops->transpose(s->U, s->UT, s->d.nx + 6, s->d.ny + 6, s->ld, s->ldT, s->stream);
+    s->rows_stale = false;
+}
+
 // One time step on the solver's stream: CFL -> 3 x (RHS, RK combine, ghost fill) -> advance
 // (fused engine: dt from the previous step's tile maxima, the advance applied
 // at the next step's start or by a settle launch; the combine writes the
@@ -582,6 +602,18 @@ void enqueue_step(wenox_solver* s)
     const auto& d = s->d;
     const Ops* o = s->ops;
     hipStream_t st = s->stream;
+    if (s->blk) {
+        // blocked: per stage the x sweep (E) and the y sweep (the new state's
+        // interior; stage 3 also the next CFL maxima), then the ghost fill
+        double* src[3] = {s->Ub, s->Ab, s->Bb};
+        double* dst[3] = {s->Ab, s->Bb, s->Ub};
+        o->step_dt(s->ctl, st);
+        for (int stage = 1; stage <= 3; ++stage) {
+            s->sops->stage_blk(s->plan, src[stage - 1], s->Ub, s->Eb, dst[stage - 1], stage, s->ctl, st);
+            s->sops->bc_blk(s->plan, dst[stage - 1], s->ctl, stage, st);
+        }
+        return;
+    }
     if (s->split) {
         float* src[3] = {(float*)s->U, (float*)s->A, (float*)s->B};
         float* dst[3] = {(float*)s->A, (float*)s->B, (float*)s->U};
@@ -753,7 +785,8 @@ extern "C" int wenox_solver_create(const wenox_solver_desc* desc, wenox_solver**
               hipMalloc(&s->L, std::max<size_t>(nL, 4) * 8) == hipSuccess &&
               hipMalloc(&s->work, std::max<size_t>(s->work_n, 4) * 8) == hipSuccess &&
               hipMalloc(&s->ctl, sizeof(StepCtl)) == hipSuccess &&
-              hipMalloc(&s->cfl_part, cfl_slots(d) * sizeof(double)) == hipSuccess &&
+              hipMalloc(&s->cfl_part, cfl_slots(d, fused && !s->split ? &s->plan : nullptr) * sizeof(double)) ==
+                  hipSuccess &&
               hipMalloc(&s->sums, ((size_t)SUM_BLOCKS * 16 + 8) * sizeof(double)) == hipSuccess;
     if (ok && fused)
         ok = state_buf(3, &s->UT, SPT) && state_buf(4, &s->AT, SPT) && state_buf(5, &s->BT, SPT) &&
@@ -805,6 +838,7 @@ extern "C" void wenox_solver_destroy(wenox_solver* s)
     if (s->cfl_part) (void)hipFree(s->cfl_part);
     if (s->sums) (void)hipFree(s->sums);
     if (s->flags) (void)hipFree(s->flags);
+    if (s->blk_mem) (void)hipFree(s->blk_mem);
     for (hipEvent_t e : s->pipe_ev)
         if (e) (void)hipEventDestroy(e);
     if (s->side) (void)hipStreamDestroy(s->side);
@@ -872,6 +906,10 @@ extern "C" int wenox_solver_upload(wenox_solver* s, const wenox_view3* q, double
         s->ops->bc(s->U, s->UT, d.nx, d.ny, ld, s->ldT, s->bp, s->ctl, 0, s->stream);
         // the fused engine's first step reads the CFL maxima of this state
         if (d.engine == WENOX_ENGINE_FUSED) s->ops->cfl_part(s->U, d.nx, d.ny, ld, d.gamma, s->ctl, s->stream);
+        if (s->blk) {
+            s->sops->to_blk(s->plan, s->U, s->Ub, s->stream);
+            s->rows_stale = false;
+        }
     }
     HIPCHK(hipStreamSynchronize(s->stream));
     HIPCHK(hipGetLastError());
@@ -883,6 +921,7 @@ extern "C" int wenox_solver_conserved_sums(wenox_solver* s, double* out, int n_o
     if (!s || !out) return fail(WENOX_E_ARG, "wenox_solver_conserved_sums: null argument");
     if (n_out != 4 && n_out != 8) return fail(WENOX_E_ARG, "wenox_solver_conserved_sums: n_out must be 4 or 8");
     HIPCHK(hipSetDevice(s->device));
+    sync_rows(s);
     double* res = s->sums + (size_t)SUM_BLOCKS * 16;
     conserved_sums(s->U, s->split, s->d.nx, s->d.ny, s->ld, plane(s), s->sums, res, s->stream);
     double h[8];
@@ -900,6 +939,7 @@ extern "C" int wenox_solver_download(wenox_solver* s, double* q8)
     const int64_t m = d.nx + 6, n = d.ny + 6;
     const size_t P = (size_t)plane(s);
     HIPCHK(hipSetDevice(s->device));
+    sync_rows(s);
     const double* state = s->U;
     if (s->split) {
         // FP64 value hi + lo of every cell
@@ -928,6 +968,7 @@ extern "C" int wenox_solver_rhs(wenox_solver* s, double* out)
     if (!s || !out) return fail(WENOX_E_ARG, "wenox_solver_rhs: null argument");
     const auto& d = s->d;
     HIPCHK(hipSetDevice(s->device));
+    sync_rows(s);
     StepCtl h;
     int rc = read_ctl(s, &h);
     if (rc) return rc;
@@ -970,6 +1011,8 @@ extern "C" int wenox_solver_run(wenox_solver* s, int64_t max_steps, double t_fin
     h.t_final = t_final;
     h.active = 1;
     if ((rc = write_ctl(s, &h))) return rc;
+    // the steps write the blocked state only
+    if (s->blk) s->rows_stale = true;
     if (s->d.use_graph && !s->exec) {
         HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         enqueue_step(s);
@@ -1070,6 +1113,23 @@ extern "C" int wenox_solver_profile(wenox_solver* s, int reps, double* out_ms, i
          })))
         return rc;
     if ((rc = write_ctl(s, &act))) return rc;
+    if (s->blk) {
+        // blocked step, stage 2 from Ub into the scratch Eb / Bb: the x sweep
+        // (E), the y sweep (new state), no combine, the ghost fill
+        auto sweep = [&](int dir) {
+            s->sops->sweep1_blk(s->plan, dir, s->Ub, s->Ub, s->Eb, s->Bb, 2, s->ctl, st);
+        };
+        if ((rc = timed(1, [&] { sweep(0); }))) return rc;
+        if ((rc = timed(2, [&] { sweep(1); }))) return rc;
+        out_ms[3] = 0.0;
+        if ((rc = timed(4, [&] { s->sops->bc_blk(s->plan, s->Bb, s->ctl, 2, st); }))) return rc;
+        auto stage = [&] { s->sops->stage_blk(s->plan, s->Ub, s->Ub, s->Eb, s->Bb, 2, s->ctl, st); };
+        if (n_out >= 7 && (rc = timed(6, stage))) return rc;
+        if (n_out >= 8 && (rc = timed(7, stage))) return rc;
+        const double st_ms = n_out >= 8 ? out_ms[7] : (n_out >= 7 ? out_ms[6] : out_ms[1] + out_ms[2]);
+        out_ms[5] = 3.0 * (st_ms + out_ms[4]) + out_ms[0];
+        return write_ctl(s, &h);
+    }
     if ((rc = timed(1, [&] { s->sops->sweep1(s->plan, 0, split, s->U, s->L, s->flags, s->ctl, st); })))
         return rc;
     if ((rc = timed(2, [&] { s->sops->sweep1(s->plan, 1, split, s->UT, s->DyT, s->flags, s->ctl, st); })))
@@ -1127,6 +1187,7 @@ extern "C" int wenox_solver_set_cfl_rule(wenox_solver* s, double rho_floor, doub
     rc = write_ctl(s, &h);
     if (rc) return rc;
     if (s->d.engine == WENOX_ENGINE_FUSED) {
+        sync_rows(s);
         // the stored CFL maxima were formed with the previous floor
         if (s->split)
             s->sops->cfl_split((const float*)s->U, s->d.nx, s->d.ny, s->ld, s->d.gamma, s->ctl, s->stream, 0);
@@ -1136,10 +1197,52 @@ extern "C" int wenox_solver_set_cfl_rule(wenox_solver* s, double rho_floor, doub
     return WENOX_OK;
 }
 
+extern "C" int wenox_solver_layout(wenox_solver* s, int* blocked)
+{
+    if (!s || !blocked) return fail(WENOX_E_ARG, "wenox_solver_layout: null argument");
+    *blocked = s->blk ? 1 : 0;
+    return WENOX_OK;
+}
+
+extern "C" int wenox_solver_set_layout(wenox_solver* s, int blocked)
+{
+    if (!s || (blocked != 0 && blocked != 1)) return fail(WENOX_E_ARG, "wenox_solver_set_layout: blocked is 0 or 1");
+    if ((blocked != 0) == s->blk) return WENOX_OK;
+    HIPCHK(hipSetDevice(s->device));
+    if (blocked) {
+        if (s->d.engine != WENOX_ENGINE_FUSED || s->split || !s->sops || !s->sops->stage_blk)
+            return fail(WENOX_E_ARG, "the blocked layout serves the fused engine's fast FP64 step");
+        if (s->pfloor > 0.0) return fail(WENOX_E_ARG, "the pressure floor runs on the row-major layout");
+        if (!s->blk_mem) {
+            const size_t n = (size_t)s->plan.Pb * 4;
+            if (hipMalloc(&s->blk_mem, n * 4 * sizeof(double)) != hipSuccess)
+                return fail(WENOX_E_DEVICE, "device allocation failed (blocked state of " +
+                                                std::to_string(n * 4 * 8 >> 20) + " MiB)");
+            s->Ub = s->blk_mem;
+            s->Ab = s->Ub + n;
+            s->Bb = s->Ab + n;
+            s->Eb = s->Bb + n;
+        }
+        // the row-major U holds the state (its ghost cells included)
+        s->sops->to_blk(s->plan, s->U, s->Ub, s->stream);
+        s->blk = true;
+        s->rows_stale = false;
+    } else {
+        sync_rows(s);
+        s->blk = false;
+    }
+    // the captured step graphs launch the other layout's kernels
+    drop_graphs(s);
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipGetLastError());
+    return WENOX_OK;
+}
+
 extern "C" int wenox_solver_set_pfloor(wenox_solver* s, double pmin)
 {
     if (!s || !(pmin >= 0.0)) return fail(WENOX_E_ARG, "wenox_solver_set_pfloor: pmin >= 0");
     if (pmin > 0.0 && s->split) return fail(WENOX_E_ARG, "the pressure floor applies to FP64 states");
+    if (pmin > 0.0 && s->blk) return fail(WENOX_E_ARG, "the pressure floor runs on the row-major layout");
     s->pfloor = pmin;
     // the captured step graph lacks (or holds) the floor launch
     drop_graphs(s);
@@ -1297,6 +1400,7 @@ extern "C" int wenox_solver_objective(wenox_solver* s, double dx, double dy, con
     if (!s || !J) return fail(WENOX_E_ARG, "wenox_solver_objective: null argument");
     if (s->split) return fail(WENOX_E_ARG, "wenox_solver_objective: FP64 states only");
     HIPCHK(hipSetDevice(s->device));
+    sync_rows(s);
     ObjSource src;
     src.U = s->U;
     src.ld = s->ld;

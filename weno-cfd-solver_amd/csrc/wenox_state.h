@@ -182,28 +182,41 @@ __device__ __forceinline__ double bc_time(const StepCtl* ctl, const BcParams& bp
     return ctl->T + (stage == 2 ? 0.5 * ctl->dt : (stage == 0 ? 0.0 : ctl->dt));
 }
 
-// Ghost cell number t (0 <= t < 6 (ny+6) + 6 nx) of one state; ld and ldT
-// are the row pitches of U and UT.
-template <class St>
-__device__ __forceinline__ void bc_cell(St U, St UT, bool has_t, int64_t nx, int64_t ny, int64_t ld, int64_t ldT,
-                                        BcParams bp, const StepCtl* ctl, int stage, int64_t t)
+// Ghost cell number t (0 <= t < 6 (ny+6) + 6 nx) of one state U, whose
+// padded cell (i, j) is element at(i, j) of its planes: the value by the rule
+// bp, handed to put(i, j, v).
+template <class St, class At, class Put>
+__device__ __forceinline__ void bc_cell_at(St U, int64_t nx, int64_t ny, BcParams bp, const StepCtl* ctl,
+                                           int stage, int64_t t, At at, Put put)
 {
     if (ctl && !ctl->active) return;
-    const int64_t n = ny + 6;
-    const int64_t count = 6 * n + 6 * nx;
+    const int64_t count = 6 * (ny + 6) + 6 * nx;
     if (t >= count) return;
     int64_t i, j;
     ghost_index(t, nx, ny, i, j);
     const double ts = bp.kind == BC_DMR_STD ? bc_time(ctl, bp, stage) : 0.0;
     const BcSrc b = bc_source(i, j, nx, ny, bp, ts);
     double v[4];
-    bc_value(b, bp, [&](int k) { return U.ld(k, b.si * ld + b.sj); }, v);
+    bc_value(b, bp, [&](int k) { return U.ld(k, at(b.si, b.sj)); }, v);
+    put(i, j, v);
+}
+
+// Row-major planes: ld and ldT are the row pitches of U and its transposed
+// copy UT (written too when has_t).
+template <class St>
+__device__ __forceinline__ void bc_cell(St U, St UT, bool has_t, int64_t nx, int64_t ny, int64_t ld, int64_t ldT,
+                                        BcParams bp, const StepCtl* ctl, int stage, int64_t t)
+{
+    bc_cell_at(
+        U, nx, ny, bp, ctl, stage, t, [&](int64_t i, int64_t j) { return i * ld + j; },
+        [&](int64_t i, int64_t j, const double* v) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) U.st(k, i * ld + j, v[k]);
-    if (has_t) {
+            for (int k = 0; k < 4; ++k) U.st(k, i * ld + j, v[k]);
+            if (has_t) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) UT.st(k, j * ldT + i, v[k]);
-    }
+                for (int k = 0; k < 4; ++k) UT.st(k, j * ldT + i, v[k]);
+            }
+        });
 }
 
 // ------------------------------------------------------------------------

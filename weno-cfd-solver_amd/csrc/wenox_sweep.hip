@@ -76,6 +76,12 @@ constexpr int OUT_COLS = WAVE - 6;   // lanes 3..60
 
 enum { IN_F64 = 0, IN_F32 = 1, IN_SPLIT = 2 };
 enum { MODE_PLAIN = 0, MODE_DETECT = 1 };
+// What a sweep writes (its FOLD): the flux differences D of its cells
+// (FOLD_NONE); E = a Un + b Us - (c dt) D, the x sweep's part of the stage
+// combination (FOLD_E: row layout, interior planes; FOLD_BE: blocked padded
+// planes); or, as the y sweep of the blocked step, the stage's new state
+// E - (c dt) D (FOLD_BO: blocked padded planes, E read where Un is).
+enum { FOLD_NONE = 0, FOLD_E = 1, FOLD_BE = 2, FOLD_BO = 3 };
 
 struct SweepArgs {
     const void* U;        // padded planes, M x N each, normal = rows
@@ -93,6 +99,10 @@ struct SweepArgs {
     // the launch's workgroups cover column blocks [bx0, bx0 + ncols) and row
     // groups [by0, by0 + nrows) (ncols, nrows = 0: to the grid's end)
     int64_t bx0 = 0, by0 = 0, ncols = 0, nrows = 0;
+    // FOLD_BE / FOLD_BO: U, Un and D are blocked planes of the padded grid
+    // (blk_row / blk_col with nb blocks per block row, Pb elements per plane;
+    // for the y sweep rows are y and lanes x, as in the transposed copy)
+    int64_t nb = 0, Pb = 0;
 };
 
 // Shu-Osher coefficients of stage s: U_{s+1} = a Un + b Us + (c dt) L
@@ -143,6 +153,7 @@ template <typename T, int W> struct SweepLds {
     T xpb[W][2][WAVE];    // p̄ of the first [0] and last [1] owned face
     T xf[W][12][WAVE];    // deferred first face: HLLC (0..3), HLLE (4..7), F of the next face (8..11)
     T xF[W][4][WAVE];     // blended flux of the first owned face
+    double wm[W];         // FOLD_BO, stage 3: per-wave CFL maxima
 };
 
 // FP64 recompute of one face row of a wavefront (the FP32-split sweep's
@@ -207,17 +218,24 @@ __host__ __device__ inline int64_t wave_start(int W, int chunk, bool blend, int 
 // separate SIMDs of one CU, and equal marches keep each of them busy until
 // the group's end.  Needs chunk >= 3 when W > 1 (the host enforces it).
 //
-// FOLD (x sweep of the fast FP64 step, which needs A.ctl for dt): D receives,
-// in place of Dx, the part of the stage's combination this sweep can form,
-// E = a Un + b Us - (c dt) Dx (stage_coeffs), so the combine reads E and
-// Dy^T instead of Dx, Dy^T, Un and Us.  Us of a cell finished inside the
-// march is a row of the stencil window and Un is fetched one face ahead; the
-// cells finished after the march reload both.
-template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND, int W, bool FOLD = false>
+// FOLD_E / FOLD_BE (x sweep of the fast FP64 step, which needs A.ctl for
+// dt): D receives, in place of Dx, the part of the stage's combination this
+// sweep can form, E = a Un + b Us - (c dt) Dx (stage_coeffs), so the combine
+// (or the blocked y sweep) reads E instead of Dx, Un and Us.  Us of a cell
+// finished inside the march is a row of the stencil window and Un is fetched
+// one face ahead; the cells finished after the march reload both.
+// FOLD_BO (y sweep of the blocked step): E comes through the Un path (A.Un)
+// and D receives the new state fma(-(c dt), Dy, E), the operation the
+// combine performs, so both steps give the same state bit for bit; in stage
+// 3 the workgroup also leaves the maximum of c + |V| over its new cells in
+// ctl->part and marks the step's advance pending, as the stage-3 combine.
+template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND, int W, int FOLD = FOLD_NONE>
 __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64_t by, int64_t bid,
                                            SweepLds<T, W>& S)
 {
-    static_assert(!FOLD || (IN == IN_F64 && MODE == MODE_PLAIN && DIR == DIR_X), "fold: FP64 x sweep only");
+    static_assert(FOLD == FOLD_NONE || (IN == IN_F64 && MODE == MODE_PLAIN), "fold: FP64 sweeps only");
+    static_assert(FOLD == FOLD_NONE || (DIR == DIR_X) == (FOLD != FOLD_BO), "fold: E from the x sweep, state from the y sweep");
+    constexpr bool BLK = FOLD == FOLD_BE || FOLD == FOLD_BO;
     if (A.ctl && !A.ctl->active) return;
     auto& s_p1 = S.p1;
     auto& s_pb = S.pb;
@@ -258,10 +276,19 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
     const bool dfirst = BLEND && wv > 0;                             // first face blended at the end
     const int64_t fa = (BLEND && wv == 0 && r0 > 0) ? r0 - 1 : r0;
     const int64_t fb = last ? ((BLEND && r1 < A.nn) ? r1 + 1 : r1) : r1 - 1;   // inclusive
-    const int64_t P = A.M * A.ld;
+    const int64_t P = BLK ? A.Pb : A.M * A.ld;
     const int64_t Pd = A.nn * A.ldd;
     DT* Dout = (DT*)A.D;
     bool bad = false;
+    // element index of padded row r (normal) at this lane's (clamped) column:
+    // row layout r ld + pcl; blocked, with rows x (x sweep) or y (y sweep)
+    const int64_t lane_off = !BLK ? pcl : (DIR == DIR_X ? blk_col(pcl) : blk_row(pcl, A.nb));
+    auto at = [&](int64_t r) -> int64_t {
+        if constexpr (!BLK) return r * A.ld + pcl;
+        else return (DIR == DIR_X ? blk_row(r, A.nb) : blk_col(r)) + lane_off;
+    };
+    double wmax = 0.0;   // FOLD_BO, stage 3: max of c + |V| over this lane's new cells
+    const bool cfl_max = FOLD == FOLD_BO && A.stage == 3;
 
     T F_prev[4], fc_prev[4], fe_prev[4];
     T pb_m2 = T(0), pb_m1 = T(0), pb_r0n = T(0);   // p̄ of faces a-2, a-1; p̄(r0 + 1)
@@ -269,18 +296,19 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
     for (int k = 0; k < 4; ++k) { fc_prev[k] = T(0); fe_prev[k] = T(0); F_prev[k] = T(0); }
 
     double fa_c = 0.0, fb_c = 0.0, fc_c = 0.0;   // FOLD: a, b and c dt of the stage
-    if constexpr (FOLD) {
+    if constexpr (FOLD != FOLD_NONE) {
         stage_coeffs(A.stage, fa_c, fb_c, fc_c);
         fc_c *= A.ctl->dt;
     }
-    const bool fold_un = FOLD && A.stage > 1;   // stage 1: a = 0, Un unread
-    // FOLD: Us and Un of interior cell `cell` from memory
+    // Un read: the x fold from stage 2 on (stage 1: a = 0), E always
+    const bool fold_un = FOLD == FOLD_BO || (FOLD != FOLD_NONE && A.stage > 1);
+    // FOLD: Us and Un (FOLD_BO: E) of interior cell `cell` from memory
     auto load_cell = [&](int64_t cell, T us[4], T un[4]) {
-        if constexpr (FOLD) {
+        if constexpr (FOLD != FOLD_NONE) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                us[k] = ld_state<T, IN>(A.U, P, k, (cell + 3) * A.ld + pcl);
-                un[k] = fold_un ? ld_state<T, IN>(A.Un, P, k, (cell + 3) * A.ld + pcl) : T(0);
+                us[k] = FOLD == FOLD_BO ? T(0) : ld_state<T, IN>(A.U, P, k, at(cell + 3));
+                un[k] = fold_un ? ld_state<T, IN>(A.Un, P, k, at(cell + 3)) : T(0);
             }
         }
     };
@@ -289,12 +317,28 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
     // every cell rounds alike wherever the march finishes it
     auto put_d = [&](int64_t cell, const T Fr[4], const T Fl[4], const T* us, const T* un) {
         if (out_lane) {
-            DT* d = Dout + cell * A.ldd + t;
+            if constexpr (BLK) {
+                // the cell's padded element (out lanes are never clamped)
+                DT* d = Dout + at(cell + 3);
+                double v[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const T dv = WX_DIV_DX(Fr[k] - Fl[k]);
-                if constexpr (!FOLD) d[k * Pd] = (DT)dv;
-                else d[k * Pd] = (DT)fma(-T(fc_c), dv, fma(T(fa_c), un[k], T(fb_c) * us[k]));
+                for (int k = 0; k < 4; ++k) {
+                    const T dv = WX_DIV_DX(Fr[k] - Fl[k]);
+                    if constexpr (FOLD == FOLD_BE) v[k] = fma(-T(fc_c), dv, fma(T(fa_c), un[k], T(fb_c) * us[k]));
+                    else v[k] = fma(-T(fc_c), dv, un[k]);
+                    d[k * P] = (DT)v[k];
+                }
+                if constexpr (FOLD == FOLD_BO) {
+                    if (cfl_max) wmax = nan_max(wmax, cfl_speed(v, A.gam, A.ctl->rho_floor));
+                }
+            } else {
+                DT* d = Dout + cell * A.ldd + t;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const T dv = WX_DIV_DX(Fr[k] - Fl[k]);
+                    if constexpr (FOLD == FOLD_NONE) d[k * Pd] = (DT)dv;
+                    else d[k * Pd] = (DT)fma(-T(fc_c), dv, fma(T(fa_c), un[k], T(fb_c) * us[k]));
+                }
             }
         }
     };
@@ -328,9 +372,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
 #pragma unroll
         for (int s = 0; s < 5; ++s)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) w[s + 1][k] = ld_state<T, IN>(A.U, P, k, (fa + s) * A.ld + pcl);
+            for (int k = 0; k < 4; ++k) w[s + 1][k] = ld_state<T, IN>(A.U, P, k, at(fa + s));
 #pragma unroll
-        for (int k = 0; k < 4; ++k) nxt[k] = ld_state<T, IN>(A.U, P, k, (fa + 5) * A.ld + pcl);
+        for (int k = 0; k < 4; ++k) nxt[k] = ld_state<T, IN>(A.U, P, k, at(fa + 5));
         // FOLD: Un of the cell finished at face a, a - 2 with the blend (one
         // face behind) and a - 1 without, fetched one face ahead like the
         // window's next row (measured 0.4 % faster than at the top of the same
@@ -340,7 +384,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             un_cur[k] = T(0);
-            un_nxt[k] = fold_un ? ld_state<T, IN>(A.Un, P, k, (fa - FIN + 3) * A.ld + pcl) : T(0);
+            un_nxt[k] = fold_un ? ld_state<T, IN>(A.Un, P, k, at(fa - FIN + 3)) : T(0);
         }
 
         for (int64_t a = fa; a <= fb; ++a) {
@@ -354,7 +398,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
             for (int k = 0; k < 4; ++k) w[5][k] = nxt[k];
             if (a < fb) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) nxt[k] = ld_state<T, IN>(A.U, P, k, (a + 6) * A.ld + pcl);
+                for (int k = 0; k < 4; ++k) nxt[k] = ld_state<T, IN>(A.U, P, k, at(a + 6));
             }
             if (fold_un) {
 #pragma unroll
@@ -362,7 +406,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                 if (a < fb) {
 #pragma unroll
                     for (int k = 0; k < 4; ++k)
-                        un_nxt[k] = ld_state<T, IN>(A.Un, P, k, (a + 1 - FIN + 3) * A.ld + pcl);
+                        un_nxt[k] = ld_state<T, IN>(A.Un, P, k, at(a + 1 - FIN + 3));
                 }
             }
 
@@ -559,6 +603,30 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
             put_d(r1 - 1, Fn, FLast, us2, un2);
         }
     }
+    if constexpr (FOLD == FOLD_BO) {
+        if (cfl_max) {
+            // the workgroup's maximum into ctl->part[bid] (the launch's grid
+            // has gridDim.x * gridDim.y workgroups, bid their row-major index)
+            const double m = wave_max(wmax);
+            StepCtl* cw = const_cast<StepCtl*>(A.ctl);
+            if constexpr (W > 1) {
+                if (l == 0) S.wm[wv] = m;
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) {
+                double g = m;
+                if constexpr (W > 1) {
+#pragma unroll
+                    for (int v = 1; v < W; ++v) g = nan_max(g, S.wm[v]);
+                }
+                cw->part[bid] = g;
+                if (bid == 0) {
+                    cw->nparts = (int)(gridDim.x * gridDim.y);
+                    cw->pending = 1;
+                }
+            }
+        }
+    }
     if (MODE == MODE_DETECT) {
         if (l == 0) A.flags[bid * W + wv] = bad ? 1 : 0;
     }
@@ -569,7 +637,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
 
 // Two wavefronts per SIMD (at most 256 VGPRs): the sweeps are VALU-bound and
 // need the second wave to cover LDS and memory latency.
-template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND, int W, bool FOLD = false>
+template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND, int W, int FOLD = FOLD_NONE>
 __global__ void __launch_bounds__(WAVE * W) __attribute__((amdgpu_waves_per_eu(2))) k_sweep(SweepArgs A)
 {
     __shared__ SweepLds<T, W> S;
@@ -872,7 +940,7 @@ static int64_t sweep_rows(const SweepArgs& a, int W, bool blend)
     return a.nrows ? a.nrows : (a.nn + rows - 1) / rows - a.by0;
 }
 
-template <typename T, int IN, typename DT, int MODE, int DIR, int W, bool FOLD = false>
+template <typename T, int IN, typename DT, int MODE, int DIR, int W, int FOLD = FOLD_NONE>
 static void launch_sweep_w(const SweepArgs& a, int cons, int blend, hipStream_t s)
 {
     dim3 grid((unsigned)sweep_cols(a), (unsigned)sweep_rows(a, W, blend != 0));
@@ -885,7 +953,7 @@ static void launch_sweep_w(const SweepArgs& a, int cons, int blend, hipStream_t 
     }
 }
 
-template <typename T, int IN, typename DT, int MODE, int DIR, bool FOLD = false>
+template <typename T, int IN, typename DT, int MODE, int DIR, int FOLD = FOLD_NONE>
 static void launch_sweep(const SweepArgs& a, int cons, int blend, hipStream_t s)
 {
     if (a.waves == 4) launch_sweep_w<T, IN, DT, MODE, DIR, 4, FOLD>(a, cons, blend, s);
@@ -1097,6 +1165,86 @@ static void h_combine1(const SweepPlan& p, int split, const void* Dx, const void
                                   StF64{(double*)Us, P}, StF64{(double*)out, P}, StF64{(double*)outT, PT}, 2, ctl,
                                   s, false, ghost_on_store(p));
 }
+
+#if WX_FAST
+// ---- blocked FP64 step ------------------------------------------------------
+// The x sweep writes E, the y sweep reads the same blocked planes with lanes
+// along x and finishes the stage; no transposed copy, no combine launch.
+static SweepArgs args_blk(const SweepPlan& p, int dir, const double* U, const double* Un, double* D, int stage,
+                          const StepCtl* ctl)
+{
+    SweepArgs a = dir == 0 ? args_x(p, U, D, nullptr, ctl) : args_y(p, U, D, nullptr, ctl);
+    a.Un = Un;
+    a.stage = stage;
+    a.nb = p.nb;
+    a.Pb = p.Pb;
+    return a;
+}
+
+static void h_sweep1_blk(const SweepPlan& p, int dir, const double* Us, const double* Un, double* E, double* out,
+                         int stage, const StepCtl* ctl, hipStream_t s)
+{
+    const int blend = p.force_hlle ? 0 : 1;
+    if (dir == 0)
+        launch_sweep<double, IN_F64, double, MODE_PLAIN, DIR_X, FOLD_BE>(args_blk(p, 0, Us, Un, E, stage, ctl),
+                                                                       p.cons, blend, s);
+    else
+        launch_sweep<double, IN_F64, double, MODE_PLAIN, DIR_Y, FOLD_BO>(args_blk(p, 1, Us, E, out, stage, ctl),
+                                                                       p.cons, blend, s);
+}
+
+static void h_stage_blk(const SweepPlan& p, const double* Us, const double* Un, double* E, double* out, int stage,
+                        const StepCtl* ctl, hipStream_t s)
+{
+    h_sweep1_blk(p, 0, Us, Un, E, out, stage, ctl, s);
+    h_sweep1_blk(p, 1, Us, Un, E, out, stage, ctl, s);
+}
+
+__global__ void k_bc_blk(StF64 U, int64_t nx, int64_t ny, int64_t nb, BcParams bp, const StepCtl* ctl, int stage)
+{
+    auto at = [&](int64_t i, int64_t j) { return blk_row(i, nb) + blk_col(j); };
+    bc_cell_at(U, nx, ny, bp, ctl, stage, (int64_t)blockIdx.x * ST_TB + threadIdx.x, at,
+               [&](int64_t i, int64_t j, const double* v) {
+#pragma unroll
+                   for (int k = 0; k < 4; ++k) U.st(k, at(i, j), v[k]);
+               });
+}
+
+static void h_bc_blk(const SweepPlan& p, double* U, const StepCtl* ctl, int stage, hipStream_t s)
+{
+    const int64_t count = 6 * (p.ny + 6) + 6 * p.nx;
+    k_bc_blk<<<(unsigned)((count + ST_TB - 1) / ST_TB), ST_TB, 0, s>>>(StF64{U, p.Pb}, p.nx, p.ny, p.nb, p.bp, ctl,
+                                                                      stage);
+}
+
+// Row-major padded planes (row pitch ld, plane P) to blocked ones or back.
+template <bool TO_BLK>
+__global__ void k_blk_copy(const double* in, double* out, int64_t M, int64_t N, int64_t ld, int64_t P, int64_t nb,
+                           int64_t Pb)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= M * N) return;
+    const int64_t i = t / N, j = t - i * N;
+    const int64_t r = i * ld + j, b = blk_row(i, nb) + blk_col(j);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if constexpr (TO_BLK) out[k * Pb + b] = in[k * P + r];
+        else out[k * P + r] = in[k * Pb + b];
+    }
+}
+
+static void h_to_blk(const SweepPlan& p, const double* U, double* Ub, hipStream_t s)
+{
+    const int64_t n = (p.nx + 6) * (p.ny + 6);
+    k_blk_copy<true><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(U, Ub, p.nx + 6, p.ny + 6, p.ld, p.P(), p.nb, p.Pb);
+}
+
+static void h_from_blk(const SweepPlan& p, const double* Ub, double* U, hipStream_t s)
+{
+    const int64_t n = (p.nx + 6) * (p.ny + 6);
+    k_blk_copy<false><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(Ub, U, p.nx + 6, p.ny + 6, p.ld, p.P(), p.nb, p.Pb);
+}
+#endif
 
 #if WX_FAST
 // ---- FP32-split -----------------------------------------------------------
@@ -1361,11 +1509,13 @@ const SweepOps WX_SWEEP_OPS = {WX_NS::h_stage,       WX_NS::h_rhs,        WX_NS:
                                WX_NS::h_stage_split, WX_NS::h_rhs_split,  WX_NS::h_transpose_split,
                                WX_NS::h_to_split,    WX_NS::h_from_split, WX_NS::h_bc_split,
                                WX_NS::h_cfl_split,   WX_NS::h_sweep1,     WX_NS::h_combine1,
-                               WX_NS::h_sweep2};
+                               WX_NS::h_sweep2,      WX_NS::h_stage_blk,  WX_NS::h_sweep1_blk,
+                               WX_NS::h_bc_blk,      WX_NS::h_to_blk,     WX_NS::h_from_blk};
 #else
-const SweepOps WX_SWEEP_OPS = {WX_NS::h_stage, WX_NS::h_rhs, WX_NS::h_transpose, nullptr, nullptr,
-                               nullptr,        nullptr,      nullptr,            nullptr, nullptr,
-                               WX_NS::h_sweep1, WX_NS::h_combine1, WX_NS::h_sweep2};
+const SweepOps WX_SWEEP_OPS = {WX_NS::h_stage,  WX_NS::h_rhs,      WX_NS::h_transpose, nullptr, nullptr,
+                               nullptr,         nullptr,           nullptr,            nullptr, nullptr,
+                               WX_NS::h_sweep1, WX_NS::h_combine1, WX_NS::h_sweep2,    nullptr, nullptr,
+                               nullptr,         nullptr,           nullptr};
 
 // Default tiling from a round model of the 256 CUs: two wavefronts per SIMD
 // are resident (2048 waves, 2048 / W workgroups of W waves); a sweep takes
@@ -1410,6 +1560,8 @@ SweepPlan make_sweep_plan(int64_t nx, int64_t ny, double dx, double gamma, int c
     p.ldT = row_pitch(nx + 6);
     p.ldx = row_pitch(ny);
     p.ldyT = row_pitch(nx);
+    p.nb = (ny + 6 + 3) / 4;
+    p.Pb = (nx + 6 + 3) / 4 * p.nb * 16;
     return p;
 }
 

@@ -186,6 +186,19 @@ class Solver:
     def set_pipeline(self, parts: int):
         _lib.check(_lib.load().wenox_solver_set_pipeline(self._h, int(parts)))
 
+    @property
+    def blocked(self) -> bool:
+        """Blocked state layout of the fused FP64 fast step: both sweeps read
+        one blocked copy of the state and the y sweep finishes each RK stage
+        (no transposed copy, no combine launch); the same states bit for bit
+        as the row-major step."""
+        v = ctypes.c_int()
+        _lib.check(_lib.load().wenox_solver_layout(self._h, ctypes.byref(v)))
+        return bool(v.value)
+
+    def set_blocked(self, on: bool = True):
+        _lib.check(_lib.load().wenox_solver_set_layout(self._h, int(bool(on))))
+
     def set_tiling(self, chunk_x: int, chunk_y: int, waves_x: int, waves_y: int):
         _lib.check(_lib.load().wenox_solver_set_tiling(self._h, int(chunk_x), int(chunk_y), int(waves_x),
                                                         int(waves_y)))
@@ -211,15 +224,47 @@ class Solver:
                + tuple((2, c) for c in range(4, 25, 2))
                + tuple((4, c) for c in range(3, 17)))
 
-    def autotune(self, candidates=TILINGS, reps: int = 3, finalists: int = 4) -> dict:
+    def autotune(self, candidates=TILINGS, reps: int = 3, finalists: int = 4, layouts=None) -> dict:
         """Pick the sweep tiling -- (wavefronts per workgroup, rows per
         wavefront) for x and y -- with the lowest measured time on this
         device and keep it.  Each direction is timed over `candidates`; the
         `finalists` fastest of each (plus the model's default) are then timed
         in pairs as the step launches them (one kernel when the wave counts
-        agree).  The default is kept unless a pair beats it by 3 %.
-        Returns {"sweep_x": {cand: ms}, "sweep_y": {...}, "pairs": {...},
-        "best": (cx, cy, wx, wy)}."""
+        agree).  The default is kept unless a pair beats it by 3 %.  Then
+        the RK-combine tile width and the slab pipeline (row-major layout).
+        `layouts`: the state layouts to tune (False row-major, True blocked;
+        default both where the blocked layout applies: fused engine, fast
+        variant, FP64); the one whose tuned step (profile()["step_sum"]) is
+        faster is kept, the current one unless the other wins by 1 %.
+        Returns the kept layout's {"sweep_x": {cand: ms}, "sweep_y": {...},
+        "pairs": {...}, "best": (cx, cy, wx, wy), "combine": {...},
+        "pipeline": {...}} with "blocked" and "step_ms" {layout: ms}."""
+        if layouts is None:
+            d = self.desc
+            ok = d.engine == ENGINE_FUSED and d.variant == FAST and d.precision == PREC_FP64
+            layouts = (False, True) if ok else (self.blocked,)
+        start = self.blocked
+        runs = {}
+        for lay in layouts:
+            self.set_blocked(lay)
+            res = self._autotune_layout(candidates, reps, finalists)
+            res["step"] = self.profile(reps)["step_sum"]
+            res["state"] = (self.tiling, self.combine_cols, self.pipeline)
+            runs[lay] = res
+        keep = min(runs, key=lambda l: runs[l]["step"])
+        if start in runs and keep != start and runs[keep]["step"] > 0.99 * runs[start]["step"]:
+            keep = start
+        self.set_blocked(keep)
+        tiling, cols, parts = runs[keep]["state"]
+        self.set_tiling(*tiling)
+        self.set_combine_cols(cols)
+        self.set_pipeline(parts)
+        out = {k: v for k, v in runs[keep].items() if k not in ("state", "step")}
+        out["blocked"] = keep
+        out["step_ms"] = {("blocked" if l else "row"): r["step"] for l, r in runs.items()}
+        return out
+
+    def _autotune_layout(self, candidates, reps, finalists) -> dict:
         cx0, cy0, wx0, wy0 = self.tiling
         per = {}
         for d, key in ((0, "sweep_x"), (1, "sweep_y")):
@@ -248,22 +293,22 @@ class Solver:
         if pairs[best] > 0.97 * pairs[default]:
             best = default
         self.set_tiling(*best)
-        # RK-combine tile width
-        comb = {}
-        c0 = self.combine_cols
-        for cols in (32, 64):
-            self.set_combine_cols(cols)
-            comb[cols] = self.profile(reps)["combine"]
-        cb = min(comb, key=comb.get)
-        self.set_combine_cols(cb if comb[cb] < 0.97 * comb[c0] else c0)
-        # slab pipeline of the FP64 stage (its combines overlap the next slab's sweeps)
-        pipe = {}
-        if self.desc.precision == PREC_FP64 and self.desc.variant == FAST:
-            for parts in self.PIPELINE_PARTS:
-                self.set_pipeline(parts)
-                pipe[parts] = self.profile(reps)["stage"]
-            bp = min(pipe, key=pipe.get)
-            self.set_pipeline(bp if pipe[bp] < 0.97 * pipe[1] else 1)
+        comb, pipe = {}, {}
+        if not self.blocked:
+            # RK-combine tile width (the blocked step has no combine launch)
+            c0 = self.combine_cols
+            for cols in (32, 64):
+                self.set_combine_cols(cols)
+                comb[cols] = self.profile(reps)["combine"]
+            cb = min(comb, key=comb.get)
+            self.set_combine_cols(cb if comb[cb] < 0.97 * comb[c0] else c0)
+            # slab pipeline of the FP64 stage (its combines overlap the next slab's sweeps)
+            if self.desc.precision == PREC_FP64 and self.desc.variant == FAST:
+                for parts in self.PIPELINE_PARTS:
+                    self.set_pipeline(parts)
+                    pipe[parts] = self.profile(reps)["stage"]
+                bp = min(pipe, key=pipe.get)
+                self.set_pipeline(bp if pipe[bp] < 0.97 * pipe[1] else 1)
         return {"sweep_x": per["sweep_x"], "sweep_y": per["sweep_y"], "pairs": pairs, "best": best,
                 "combine": comb, "pipeline": pipe}
 

@@ -101,6 +101,8 @@ def load():
         "wenox_solver_set_combine_cols": ([dp, i], i),
         "wenox_solver_pipeline": ([dp, ctypes.POINTER(i)], i),
         "wenox_solver_set_pipeline": ([dp, i], i),
+        "wenox_solver_layout": ([dp, ctypes.POINTER(i)], i),
+        "wenox_solver_set_layout": ([dp, i], i),
         "wenox_solver_set_cfl_rule": ([dp, d, d], i),
         "wenox_solver_set_pfloor": ([dp, d], i),
         "wenox_objective_2d": ([ctypes.POINTER(View2)] * 3 + [d, d, dp, dp, dp], i),
