@@ -221,10 +221,11 @@ int wenox_solver_set_combine_cols(wenox_solver *s, int cols);
 int wenox_solver_pipeline(wenox_solver *s, int *parts);
 int wenox_solver_set_pipeline(wenox_solver *s, int parts);
 
-/* State layout of the fused engine's fast FP64 step.  0 (the default):
- * row-major planes with a transposed copy for the y sweep, and per RK stage
- * the x and y sweeps and an RK-combine launch.  1: blocked planes (4 x 4
- * cells per 128 B line) that both sweeps read; per stage the x sweep writes
+/* State layout of the fused engine's fast FP64 step.  0: row-major planes
+ * with a transposed copy for the y sweep, and per RK stage the x and y
+ * sweeps and an RK-combine launch.  1 (the default): a blocked state (a
+ * cell's four components side by side, 2 x 2 cells per 128 B line) that
+ * both sweeps read; per stage the x sweep writes
  * its part of the stage combination and the y sweep finishes the stage
  * (no transposed copy, no combine launch), then a ghost-fill launch.  The
  * states are the same bit for bit.  The other calls (upload, download,

@@ -206,8 +206,9 @@ def test_sweep_tilings_bitwise(oracle_mod, cons, force_hlle):
                 np.testing.assert_array_equal(s.rhs(), ref, err_msg=f"tiling {(cx, cy, wx, wy)}")
 
 
+@pytest.mark.parametrize("blocked", [False, True])
 @pytest.mark.parametrize("cons,force_hlle", [(False, False), (True, False), (False, True)])
-def test_fast_step_same_for_every_tiling(oracle_mod, cons, force_hlle):
+def test_fast_step_same_for_every_tiling(oracle_mod, cons, force_hlle, blocked):
     """The fast FP64 step lets the x sweep form its part of each Shu-Osher
     stage, a Un + b Us - (c dt) Dx, inside the march or -- for the cells next
     to the faces a workgroup's wavefronts share -- after it, from reloaded
@@ -221,12 +222,13 @@ def test_fast_step_same_for_every_tiling(oracle_mod, cons, force_hlle):
     q0, dx = problems.four_quadrant(nx, ny=ny)
     ref, ts = oracle_mod.run_riemann(q0.copy(), nx, ny, dx, 1.4, 0.4, force_hlle, cons, (0.5, 2.0), nstep=6)
     assert np.isfinite(ref).all()
-    print(f"[solver] fast step, tilings 45x37 cons={cons} hlle={force_hlle}", flush=True)
+    print(f"[solver] fast step, tilings 45x37 cons={cons} hlle={force_hlle} blocked={blocked}", flush=True)
     first = None
     for wx, cx in ((1, 8), (2, 3), (2, 7), (4, 3), (4, 4), (4, 11)):
         for wy, cy in ((1, 5), (4, 3)):
             with Solver(nx, ny, dx, cfl=0.4, cons_switch=cons, force_hlle=force_hlle, jp_cri=(0.5, 2.0),
                         engine="fused", variant="fast") as s:
+                s.set_blocked(blocked)
                 s.set_tiling(cx, cy, wx, wy)
                 s.upload(q0)
                 T, k = s.run(nstep=6)
@@ -482,6 +484,7 @@ def test_pipeline_parts_bitwise(bc, nx, ny):
         with Solver(nx, ny, dx, cfl=0.4 if bc != "dmr" else 0.1, jp_cri=(0.5, 2.0), bc=bc, engine="fused",
                     variant="fast") as s:
             s.set_tiling(7, 9, 1, 1)
+            s.set_blocked(False)   # the pipeline stages the row-major step's combine
             s.set_pipeline(parts)
             assert s.pipeline == parts
             s.upload(q0)
@@ -560,6 +563,8 @@ def test_blocked_layout_switches_and_reads():
     nx, ny, dx, q0, kw = _layout_case("mirror")
     print("[blocked] switching layouts between runs", flush=True)
     with Solver(nx, ny, dx, **kw) as s:
+        assert s.blocked   # the fast FP64 step's default
+        s.set_blocked(False)
         s.upload(q0)
         s.run(nstep=5)
         s.run(t_final=0.05)

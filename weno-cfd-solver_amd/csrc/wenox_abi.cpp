@@ -543,7 +543,7 @@ struct wenox_solver {
     hipEvent_t pipe_ev[PIPE_MAX_PARTS + 1] = {};
     // blocked layout of the fast FP64 step (wenox_solver_set_layout): Ub holds
     // the state, Ab and Bb the stage states, Eb the x sweep's part E of each
-    // stage, all blocked planes (plan.nb, plan.Pb) in one allocation; the
+    // stage, all blocked states (plan.nb, plan.Pb) in one allocation; the
     // row-major U / U^T are brought up to date by sync_rows after steps ran
     bool blk = false;
     bool rows_stale = false;
@@ -822,6 +822,8 @@ extern "C" int wenox_solver_create(const wenox_solver_desc* desc, wenox_solver**
     h.t_final = -1.0;
     h.active = 1;
     if ((rc = write_ctl(s, &h))) return cleanup(rc);
+    // the fast FP64 step runs on the blocked state by default
+    if (fused && d.variant == WENOX_FAST && !s->split && (rc = wenox_solver_set_layout(s, 1))) return cleanup(rc);
     *out = s;
     return WENOX_OK;
 }
@@ -1214,7 +1216,7 @@ extern "C" int wenox_solver_set_layout(wenox_solver* s, int blocked)
             return fail(WENOX_E_ARG, "the blocked layout serves the fused engine's fast FP64 step");
         if (s->pfloor > 0.0) return fail(WENOX_E_ARG, "the pressure floor runs on the row-major layout");
         if (!s->blk_mem) {
-            const size_t n = (size_t)s->plan.Pb * 4;
+            const size_t n = (size_t)s->plan.Pb;
             if (hipMalloc(&s->blk_mem, n * 4 * sizeof(double)) != hipSuccess)
                 return fail(WENOX_E_DEVICE, "device allocation failed (blocked state of " +
                                                 std::to_string(n * 4 * 8 >> 20) + " MiB)");

@@ -34,9 +34,9 @@ struct SweepPlan {
     hipStream_t side = nullptr;
     hipEvent_t* ev = nullptr;
     // Blocked layout of the fast FP64 step without transposed copies
-    // (SweepOps::stage_blk): each plane of the padded grid in 4 x 4 blocks of
-    // one 128 B line, blocks row-major; nb blocks per block row, Pb elements
-    // per plane (blk_row / blk_col).
+    // (SweepOps::stage_blk): the four conserved components of a cell side by
+    // side (32 B), 2 x 2 cells per 128 B line, lines row-major; nb lines per
+    // line row, Pb doubles per state (blk_row / blk_col).
     int64_t nb = 0, Pb = 0;
     int64_t P() const { return (nx + 6) * ld; }       // plane stride of U
     int64_t PT() const { return (ny + 6) * ldT; }     // plane stride of U^T
@@ -44,18 +44,17 @@ struct SweepPlan {
     int64_t PDyT() const { return ny * ldyT; }        // plane stride of Dy^T
 };
 
-// Element index of padded cell (i, j) in a blocked plane: blk_row(i, nb) +
-// blk_col(j).  A 4 x 4 block is one 128 B line, its 2 x 2 quarters one 32 B
-// sector each, so a wavefront's 64 lanes along either axis touch 16 lines,
-// 16 B of each sector they touch, and the next three rows of its march use
-// the same lines.  (Blocks whose sectors were rows of 4 cells left the y
-// sweep's lanes 8 B per sector: at 4096^2 its sweep took 1.17 ms against
-// 1.03 ms in this layout, the x sweep 0.89 ms in both.)
-__host__ __device__ inline int64_t blk_row(int64_t i, int64_t nb)
-{
-    return (i >> 2) * nb * 16 + (((i >> 1) & 1) << 3) + (i & 1);
-}
-__host__ __device__ inline int64_t blk_col(int64_t j) { return ((j >> 2) << 4) + (((j >> 1) & 1) << 2) + ((j & 1) << 1); }
+// Index of component 0 of padded cell (i, j) in a blocked state:
+// blk_row(i, nb) + blk_col(j); component k follows at + k.  A 2 x 2 group
+// of cells is one 128 B line and a cell one 32 B sector of it, so a lane
+// reads or writes its cell's four components as 32 contiguous bytes, a
+// wavefront's 64 lanes along either axis touch 32 lines, and the next row of
+// its march uses the same lines.  (With each component in its own plane of
+// 4 x 4-cell lines, the lines of a wave stayed in use over four rows of its
+// march and the L2 did not hold them: the 4096^2 step read 30 GB per step
+// from beyond the L2, against 16.7 GB for the row-major step.)
+__host__ __device__ inline int64_t blk_row(int64_t i, int64_t nb) { return (i >> 1) * nb * 16 + ((i & 1) << 3); }
+__host__ __device__ inline int64_t blk_col(int64_t j) { return ((j >> 1) << 4) + ((j & 1) << 2); }
 
 struct SweepOps {
     // stage s: Us/UsT stage input (ghost-filled), Un step start; Dx (4 planes
@@ -103,7 +102,7 @@ struct SweepOps {
                    int* flags, const StepCtl* ctl, hipStream_t s);
 
     // ---- blocked FP64 step (fast build only; nullptr in the exact build) ----
-    // Stage s on blocked planes (p.nb, p.Pb): the x sweep of Us writes E = a Un
+    // Stage s on blocked states (p.nb, p.Pb): the x sweep of Us writes E = a Un
     // + b Us - (c dt) Dx, then the y sweep of Us finishes the stage, out = E -
     // (c dt) Dy, and in stage 3 leaves the next step's CFL maxima (one per
     // workgroup, at most blk_part_slots(p)) and the advance pending.  out
@@ -114,7 +113,7 @@ struct SweepOps {
     void (*sweep1_blk)(const SweepPlan& p, int dir, const double* Us, const double* Un, double* E,
                        double* out, int stage, const StepCtl* ctl, hipStream_t s);
     void (*bc_blk)(const SweepPlan& p, double* U, const StepCtl* ctl, int stage, hipStream_t s);
-    // row-major padded planes (row pitch p.ld) <-> blocked planes
+    // row-major padded planes (row pitch p.ld) <-> blocked state
     void (*to_blk)(const SweepPlan& p, const double* U, double* Ub, hipStream_t s);
     void (*from_blk)(const SweepPlan& p, const double* Ub, double* U, hipStream_t s);
 };

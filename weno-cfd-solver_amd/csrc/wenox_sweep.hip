@@ -99,10 +99,10 @@ struct SweepArgs {
     // the launch's workgroups cover column blocks [bx0, bx0 + ncols) and row
     // groups [by0, by0 + nrows) (ncols, nrows = 0: to the grid's end)
     int64_t bx0 = 0, by0 = 0, ncols = 0, nrows = 0;
-    // FOLD_BE / FOLD_BO: U, Un and D are blocked planes of the padded grid
-    // (blk_row / blk_col with nb blocks per block row, Pb elements per plane;
+    // FOLD_BE / FOLD_BO: U, Un and D are blocked states of the padded grid
+    // (blk_row / blk_col with nb lines per line row, components adjacent;
     // for the y sweep rows are y and lanes x, as in the transposed copy)
-    int64_t nb = 0, Pb = 0;
+    int64_t nb = 0;
 };
 
 // Shu-Osher coefficients of stage s: U_{s+1} = a Un + b Us + (c dt) L
@@ -276,7 +276,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
     const bool dfirst = BLEND && wv > 0;                             // first face blended at the end
     const int64_t fa = (BLEND && wv == 0 && r0 > 0) ? r0 - 1 : r0;
     const int64_t fb = last ? ((BLEND && r1 < A.nn) ? r1 + 1 : r1) : r1 - 1;   // inclusive
-    const int64_t P = BLK ? A.Pb : A.M * A.ld;
+    // component stride: planes, or adjacent components in the blocked layout
+    const int64_t P = BLK ? 1 : A.M * A.ld;
     const int64_t Pd = A.nn * A.ldd;
     DT* Dout = (DT*)A.D;
     bool bad = false;
@@ -1168,7 +1169,7 @@ static void h_combine1(const SweepPlan& p, int split, const void* Dx, const void
 
 #if WX_FAST
 // ---- blocked FP64 step ------------------------------------------------------
-// The x sweep writes E, the y sweep reads the same blocked planes with lanes
+// The x sweep writes E, the y sweep reads the same blocked states with lanes
 // along x and finishes the stage; no transposed copy, no combine launch.
 static SweepArgs args_blk(const SweepPlan& p, int dir, const double* U, const double* Un, double* D, int stage,
                           const StepCtl* ctl)
@@ -1177,7 +1178,6 @@ static SweepArgs args_blk(const SweepPlan& p, int dir, const double* U, const do
     a.Un = Un;
     a.stage = stage;
     a.nb = p.nb;
-    a.Pb = p.Pb;
     return a;
 }
 
@@ -1213,14 +1213,13 @@ __global__ void k_bc_blk(StF64 U, int64_t nx, int64_t ny, int64_t nb, BcParams b
 static void h_bc_blk(const SweepPlan& p, double* U, const StepCtl* ctl, int stage, hipStream_t s)
 {
     const int64_t count = 6 * (p.ny + 6) + 6 * p.nx;
-    k_bc_blk<<<(unsigned)((count + ST_TB - 1) / ST_TB), ST_TB, 0, s>>>(StF64{U, p.Pb}, p.nx, p.ny, p.nb, p.bp, ctl,
+    k_bc_blk<<<(unsigned)((count + ST_TB - 1) / ST_TB), ST_TB, 0, s>>>(StF64{U, 1}, p.nx, p.ny, p.nb, p.bp, ctl,
                                                                       stage);
 }
 
-// Row-major padded planes (row pitch ld, plane P) to blocked ones or back.
+// Row-major padded planes (row pitch ld, plane P) to the blocked state or back.
 template <bool TO_BLK>
-__global__ void k_blk_copy(const double* in, double* out, int64_t M, int64_t N, int64_t ld, int64_t P, int64_t nb,
-                           int64_t Pb)
+__global__ void k_blk_copy(const double* in, double* out, int64_t M, int64_t N, int64_t ld, int64_t P, int64_t nb)
 {
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= M * N) return;
@@ -1228,21 +1227,21 @@ __global__ void k_blk_copy(const double* in, double* out, int64_t M, int64_t N, 
     const int64_t r = i * ld + j, b = blk_row(i, nb) + blk_col(j);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if constexpr (TO_BLK) out[k * Pb + b] = in[k * P + r];
-        else out[k * P + r] = in[k * Pb + b];
+        if constexpr (TO_BLK) out[b + k] = in[k * P + r];
+        else out[k * P + r] = in[b + k];
     }
 }
 
 static void h_to_blk(const SweepPlan& p, const double* U, double* Ub, hipStream_t s)
 {
     const int64_t n = (p.nx + 6) * (p.ny + 6);
-    k_blk_copy<true><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(U, Ub, p.nx + 6, p.ny + 6, p.ld, p.P(), p.nb, p.Pb);
+    k_blk_copy<true><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(U, Ub, p.nx + 6, p.ny + 6, p.ld, p.P(), p.nb);
 }
 
 static void h_from_blk(const SweepPlan& p, const double* Ub, double* U, hipStream_t s)
 {
     const int64_t n = (p.nx + 6) * (p.ny + 6);
-    k_blk_copy<false><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(Ub, U, p.nx + 6, p.ny + 6, p.ld, p.P(), p.nb, p.Pb);
+    k_blk_copy<false><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(Ub, U, p.nx + 6, p.ny + 6, p.ld, p.P(), p.nb);
 }
 #endif
 
@@ -1560,8 +1559,8 @@ SweepPlan make_sweep_plan(int64_t nx, int64_t ny, double dx, double gamma, int c
     p.ldT = row_pitch(nx + 6);
     p.ldx = row_pitch(ny);
     p.ldyT = row_pitch(nx);
-    p.nb = (ny + 6 + 3) / 4;
-    p.Pb = (nx + 6 + 3) / 4 * p.nb * 16;
+    p.nb = (ny + 6 + 1) / 2;
+    p.Pb = (nx + 6 + 1) / 2 * p.nb * 16;
     return p;
 }
 

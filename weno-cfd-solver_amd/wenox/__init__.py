@@ -188,10 +188,12 @@ class Solver:
 
     @property
     def blocked(self) -> bool:
-        """Blocked state layout of the fused FP64 fast step: both sweeps read
-        one blocked copy of the state and the y sweep finishes each RK stage
-        (no transposed copy, no combine launch); the same states bit for bit
-        as the row-major step."""
+        """Blocked state layout of the fused FP64 fast step (its default):
+        both sweeps read one blocked copy of the state and the y sweep
+        finishes each RK stage (no transposed copy, no combine launch); the
+        same states bit for bit as the row-major step (set_blocked(False)),
+        whose RK combine, combine tile width and slab pipeline apply only
+        there."""
         v = ctypes.c_int()
         _lib.check(_lib.load().wenox_solver_layout(self._h, ctypes.byref(v)))
         return bool(v.value)
