@@ -408,19 +408,25 @@ _QUAD_SRC64B = "profiles/quad4096_fp64_blocked.md"
 _QUAD_SRC32 = "profiles/quad4096_fp32split.md"
 _KH_SRC64 = "profiles/kh8192_fp64.md"
 _KH_SRC32 = "profiles/kh8192_fp32split.md"
+_KH_SRC64B = "profiles/kh8192_fp64_blocked.md"
+_DMR_SRC64B = "profiles/dmr1920x480_fp64_blocked.md"
 TRAFFIC = {
     ("double_mach_reflection", 1920, 480, False, "row"):
         {"bytes_per_step": 1.0506e9, "tiling": (10, 9, 4, 4), "source": _DMR_SRC64},
+    ("double_mach_reflection", 1920, 480, False, "blocked"):
+        {"bytes_per_step": 0.7440e9, "tiling": (10, 9, 4, 4), "source": _DMR_SRC64B},
     ("double_mach_reflection", 1920, 480, True, "row"):
         {"bytes_per_step": 1.1188e9, "tiling": (10, 8, 2, 1), "source": _DMR_SRC32},
     ("four_quadrant_riemann", 4096, 4096, False, "row"):
         {"bytes_per_step": 16.7482e9, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64},
     ("four_quadrant_riemann", 4096, 4096, False, "blocked"):
-        {"bytes_per_step": 30.0182e9, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64B},
+        {"bytes_per_step": 12.3849e9, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64B},
     ("four_quadrant_riemann", 4096, 4096, True, "row"):
         {"bytes_per_step": 15.4459e9, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC32},
     ("kelvin_helmholtz", 8192, 8192, False, "row"):
         {"bytes_per_step": 66.6840e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC64},
+    ("kelvin_helmholtz", 8192, 8192, False, "blocked"):
+        {"bytes_per_step": 49.6295e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC64B},
     ("kelvin_helmholtz", 8192, 8192, True, "row"):
         {"bytes_per_step": 61.1511e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC32},
 }
@@ -431,16 +437,20 @@ TRAFFIC = {
 VALU = {
     ("double_mach_reflection", 1920, 480, False, "row", "sweeps"):
         {"wave_instr_per_launch": 5.17e7, "tiling": (10, 9, 4, 4), "source": _DMR_SRC64},
+    ("double_mach_reflection", 1920, 480, False, "blocked", "sweeps"):
+        {"wave_instr_per_launch": 2.629e7 + 2.501e7, "tiling": (10, 9, 4, 4), "source": _DMR_SRC64B},
     ("double_mach_reflection", 1920, 480, True, "row", "sweeps"):
         {"wave_instr_per_launch": 3.479e7 + 3.795e7, "tiling": (10, 8, 2, 1), "source": _DMR_SRC32},
     ("four_quadrant_riemann", 4096, 4096, False, "row", "sweeps"):
         {"wave_instr_per_launch": 8.335e8, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64},
     ("four_quadrant_riemann", 4096, 4096, False, "blocked", "sweeps"):
-        {"wave_instr_per_launch": 4.187e8 + 4.171e8, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64B},
+        {"wave_instr_per_launch": 4.165e8 + 4.143e8, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64B},
     ("four_quadrant_riemann", 4096, 4096, True, "row", "sweeps"):
         {"wave_instr_per_launch": 1.023e9, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC32},
     ("kelvin_helmholtz", 8192, 8192, False, "row", "sweeps"):
         {"wave_instr_per_launch": 3.422e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC64},
+    ("kelvin_helmholtz", 8192, 8192, False, "blocked", "sweeps"):
+        {"wave_instr_per_launch": 1.69e9 + 1.722e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC64B},
     ("kelvin_helmholtz", 8192, 8192, True, "row", "sweeps"):
         {"wave_instr_per_launch": 4.178e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC32},
 }

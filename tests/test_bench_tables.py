@@ -70,8 +70,9 @@ def test_profile_commands_name_their_workload():
     """A summary's commands select the workload its title names (the bench
     default is the 4096^2 four-quadrant problem)."""
     for name, flag in (("kh8192", "--workload kh8192"), ("dmr1920x480", "--workload dmr")):
-        for prec in ("fp64", "fp32split"):
+        for prec in ("fp64", "fp32split", "fp64_blocked"):
             text = _read(f"profiles/{name}_{prec}.md")
             cmds = [ln for ln in text.splitlines() if ln.startswith("cd /tmp") and "bench.py" in ln]
             assert cmds and all(flag in ln for ln in cmds), (name, prec)
             assert all(("--precision fp32split" in ln) == (prec == "fp32split") for ln in cmds), (name, prec)
+            assert all(("--layout blocked" in ln) == prec.endswith("blocked") for ln in cmds), (name, prec)
