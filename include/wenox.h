@@ -260,6 +260,13 @@ int wenox_batch_set_chunks(wenox_batch *b, int chunk_x, int chunk_y);
 /* Device time (ms, HIP events) of one RK stage of all instances as a step
  * launches it, averaged over reps; the state is left as it was. */
 int wenox_batch_profile(wenox_batch *b, int reps, double *stage_ms);
+/* State layout of the batched fast FP64 step, as wenox_solver_set_layout:
+ * 1 (the default for the fast FP64 batch): each instance's blocked state,
+ * the y sweeps finishing every stage; 0: row-major planes with transposed
+ * copies and an RK-combine launch.  The states are the same bit for bit.
+ * No reference counterpart. */
+int wenox_batch_layout(wenox_batch *b, int *blocked);
+int wenox_batch_set_layout(wenox_batch *b, int blocked);
 
 /* ---- 4. jp_cri tuning objective --------------------------------------------- */
 /* A read-only 2-D host array view: extents and element strides. */

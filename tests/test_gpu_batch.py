@@ -112,3 +112,45 @@ def test_batch_chunks_and_autotune(variant):
         assert np.array_equal(T, T0) and np.array_equal(k, k0), chunks
         for i in range(len(inst)):
             np.testing.assert_array_equal(got[i], ref[i], err_msg=f"chunks {chunks} instance {i}")
+
+
+@pytest.mark.parametrize("bc,cons", [("mirror", False), ("periodic", False), ("copy", True)])
+def test_batch_blocked_layout_same_states(bc, cons):
+    """The batched fast FP64 step on blocked states (the default) and on
+    row-major states give every instance the same state bit for bit, ghost
+    cells included, and the same times, also when the layout is switched
+    between runs and an instance is uploaded again in between."""
+    from wenox import Batch
+    inst = _instances(11)
+    kw = dict(cfl=0.4, jp_cri=(0.5, 2.0), bc=bc, cons_switch=cons, variant="fast")
+    print(f"[batch] blocked vs row-major, bc={bc} cons={cons}", flush=True)
+    out = {}
+    for mode in ("row", "blocked", "switch"):
+        with Batch(SIZES, [q[1] for q in inst], **kw) as b:
+            assert b.blocked
+            if mode == "row":
+                b.set_blocked(False)
+                assert not b.blocked
+            for i, (q, dx) in enumerate(inst):
+                b.upload(i, q)
+            if mode == "switch":
+                b.run(nstep=3)
+                b.set_blocked(False)
+                b.run(nstep=2)
+                b.set_blocked(True)
+                b.upload(2, inst[2][0])
+                b.run(nstep=4)
+            else:
+                b.run(nstep=5)
+                b.upload(2, inst[2][0])
+                b.run(nstep=4)
+            T, k = b.status()
+            out[mode] = (T, k, [b.download(i) for i in range(len(inst))])
+    T0, k0, q0 = out["row"]
+    assert all(np.isfinite(q).all() for q in q0)
+    for mode in ("blocked", "switch"):
+        T1, k1, q1 = out[mode]
+        np.testing.assert_array_equal(k1, k0)
+        np.testing.assert_array_equal(T1, T0)
+        for i in range(len(inst)):
+            np.testing.assert_array_equal(q1[i], q0[i], err_msg=f"{mode} instance {i}")

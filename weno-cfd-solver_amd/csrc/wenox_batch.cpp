@@ -56,6 +56,14 @@ struct wenox_batch {
     double last_ms = 0.0;
     int chunk_x = 16, chunk_y = 16;
     size_t flags_n = 0;
+    // blocked FP64 fast step (the default where it applies): the instances'
+    // blocked states (BatchInst::boff, nb) in Ub (U^n), Ab, Bb (stage states)
+    // and Eb (the x sweeps' part of each stage), one allocation; the
+    // row-major U holds an instance's state only after sync_rows
+    bool blk = false;
+    bool rows_stale = false;
+    double* blk_mem = nullptr;
+    double *Ub = nullptr, *Ab = nullptr, *Bb = nullptr, *Eb = nullptr;
 };
 
 namespace {
@@ -63,6 +71,17 @@ namespace {
 void batch_step(wenox_batch* b)
 {
     hipStream_t st = b->stream;
+    if (b->blk) {
+        double* src[3] = {b->Ub, b->Ab, b->Bb};
+        double* dst[3] = {b->Ab, b->Bb, b->Ub};
+        b->bops->cfl_blk(b->tab, b->Ub, b->d.gamma, b->ctls, st);
+        for (int s = 1; s <= 3; ++s) {
+            b->bops->stage_blk(b->tab, b->com, src[s - 1], b->Ub, b->Eb, dst[s - 1], s, b->ctls, st);
+            b->bops->bc_blk(b->tab, b->bp, dst[s - 1], b->ctls, s, st);
+        }
+        b->bops->advance(b->ctls, b->n, st);
+        return;
+    }
     void* src[3] = {b->U, b->A, b->B};
     void* dst[3] = {b->A, b->B, b->U};
     void* srcT[3] = {b->UT, b->AT, b->BT};
@@ -142,6 +161,27 @@ int build_tables(wenox_batch* b, int chunk_x, int chunk_y)
     return WENOX_OK;
 }
 
+// Geometry of instance i for the layout conversions (unpitched row-major
+// planes, the instance's blocked geometry).
+SweepPlan inst_plan(const BatchInst& I)
+{
+    SweepPlan p{};
+    p.nx = I.nx;
+    p.ny = I.ny;
+    p.ld = I.ny + 6;
+    p.nb = I.nb;
+    return p;
+}
+
+// After blocked steps: every instance's row-major state from its blocked one
+// (a download converts its own instance only).
+void sync_rows(wenox_batch* b)
+{
+    if (!b->blk || !b->rows_stale) return;
+    for (const BatchInst& I : b->inst) b->sops->from_blk(inst_plan(I), b->Ub + I.boff, b->U + I.soff, b->stream);
+    b->rows_stale = false;
+}
+
 int read_ctls(wenox_batch* b, std::vector<StepCtl>& h)
 {
     h.resize(b->n);
@@ -164,7 +204,7 @@ extern "C" void wenox_batch_destroy(wenox_batch* b)
     if (!b) return;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    for (double* p : {b->U, b->A, b->B, b->UT, b->AT, b->BT, b->Dx, b->DyT, b->work})
+    for (double* p : {b->U, b->A, b->B, b->UT, b->AT, b->BT, b->Dx, b->DyT, b->work, b->blk_mem})
         if (p) (void)hipFree(p);
     if (b->flags) (void)hipFree(b->flags);
     if (b->ctls) (void)hipFree(b->ctls);
@@ -205,8 +245,8 @@ extern "C" int wenox_batch_create(const wenox_solver_desc* desc, int n, const in
         wenox_batch_destroy(b);
         return code;
     };
-    // instance layout (back to back, unpitched)
-    int64_t soff = 0, ioff = 0, maxP = 0;
+    // instance layout (back to back, unpitched; blocked states back to back)
+    int64_t soff = 0, ioff = 0, maxP = 0, boff = 0;
     b->inst.resize(n);
     for (int i = 0; i < n; ++i) {
         BatchInst& I = b->inst[i];
@@ -219,7 +259,12 @@ extern "C" int wenox_batch_create(const wenox_solver_desc* desc, int n, const in
         maxP = std::max(maxP, P);
         SweepPlan p = make_sweep_plan(I.nx, I.ny, I.dx, d.gamma, d.cons_switch, d.force_hlle, d.jp_lo, d.jp_hi);
         I.two_h_x = p.two_h_x; I.two_h_y = p.two_h_y;
+        I.boff = boff;
+        I.nb = p.nb;
+        boff += p.Pb;
     }
+    // the fast FP64 step runs on the blocked states
+    b->blk = !b->split && d.variant == WENOX_FAST;
     bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreate(&b->ev0) == hipSuccess && hipEventCreate(&b->ev1) == hipSuccess;
     for (double** p : {&b->U, &b->A, &b->B, &b->UT, &b->AT, &b->BT})
@@ -227,7 +272,16 @@ extern "C" int wenox_batch_create(const wenox_solver_desc* desc, int n, const in
     ok = ok && hipMalloc(&b->Dx, (size_t)ioff * 8) == hipSuccess && hipMalloc(&b->DyT, (size_t)ioff * 8) == hipSuccess &&
          hipMalloc(&b->work, (size_t)maxP * 10 * 8) == hipSuccess &&
          hipMalloc(&b->ctls, sizeof(StepCtl) * n) == hipSuccess;
-    if (!ok) return cleanup(abi_fail(WENOX_E_DEVICE, "batch allocation failed (" + std::to_string((6 * soff + 2 * ioff) * 8 >> 20) + " MiB)"));
+    if (ok && b->blk) {
+        ok = hipMalloc(&b->blk_mem, (size_t)boff * 4 * 8) == hipSuccess;
+        if (ok) {
+            b->Ub = b->blk_mem;
+            b->Ab = b->Ub + boff;
+            b->Bb = b->Ab + boff;
+            b->Eb = b->Bb + boff;
+        }
+    }
+    if (!ok) return cleanup(abi_fail(WENOX_E_DEVICE, "batch allocation failed (" + std::to_string((6 * soff + 2 * ioff + (b->blk ? 4 * boff : 0)) * 8 >> 20) + " MiB)"));
     int rct = build_tables(b, chunk, chunk);
     if (rct) return cleanup(rct);
     b->com = BatchCommon{d.gamma, d.jp_lo, d.jp_hi, d.cons_switch, d.force_hlle};
@@ -292,6 +346,7 @@ extern "C" int wenox_batch_upload(wenox_batch* b, int i, const wenox_view3* q, d
         BCHK(hipMemcpy(Ui, pin.data(), pin.size() * 8, hipMemcpyHostToDevice));
         b->sops->transpose(Ui, UTi, m, n, n, m, b->stream);
         b->ops->bc(Ui, UTi, I.nx, I.ny, n, m, bp, ci, 0, b->stream);
+        if (b->blk) b->sops->to_blk(inst_plan(I), Ui, b->Ub + I.boff, b->stream);
     }
     BCHK(hipStreamSynchronize(b->stream));
     BCHK(hipGetLastError());
@@ -304,6 +359,8 @@ extern "C" int wenox_batch_download(wenox_batch* b, int i, double* q8)
     const BatchInst& I = b->inst[i];
     const int64_t m = I.nx + 6, n = I.ny + 6, P = m * n;
     BCHK(hipSetDevice(b->device));
+    // this instance's row-major state from its blocked one
+    if (b->blk) b->sops->from_blk(inst_plan(I), b->Ub + I.boff, b->U + I.soff, b->stream);
     double* w = b->work;
     const double* state = b->U + I.soff;
     if (b->split) {
@@ -342,6 +399,8 @@ extern "C" int wenox_batch_run(wenox_batch* b, int64_t max_steps, double t_final
         c[i].active = 1;
     }
     if ((rc = write_ctls(b, c))) return rc;
+    // the steps write the blocked states only
+    if (b->blk) b->rows_stale = true;
     if (b->d.use_graph && !b->exec) {
         BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         batch_step(b);
@@ -432,8 +491,11 @@ extern "C" int wenox_batch_profile(wenox_batch* b, int reps, double* stage_ms)
     if ((rc = write_ctls(b, act))) return rc;
     // one RK stage (stage 2, from U into B / B^T: the step overwrites B) as a step launches it
     auto stage = [&] {
-        b->bops->stage(b->tab, b->com, b->split, b->U, b->UT, b->U, b->Dx, b->DyT, b->B, b->BT, b->flags, 2,
-                       b->ctls, b->stream);
+        if (b->blk)
+            b->bops->stage_blk(b->tab, b->com, b->Ub, b->Ub, b->Eb, b->Bb, 2, b->ctls, b->stream);
+        else
+            b->bops->stage(b->tab, b->com, b->split, b->U, b->UT, b->U, b->Dx, b->DyT, b->B, b->BT, b->flags, 2,
+                           b->ctls, b->stream);
     };
     stage();
     BCHK(hipEventRecord(b->ev0, b->stream));
@@ -444,4 +506,39 @@ extern "C" int wenox_batch_profile(wenox_batch* b, int reps, double* stage_ms)
     BCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     *stage_ms = ms / reps;
     return write_ctls(b, h);
+}
+
+extern "C" int wenox_batch_layout(wenox_batch* b, int* blocked)
+{
+    if (!b || !blocked) return abi_fail(WENOX_E_ARG, "wenox_batch_layout: null argument");
+    *blocked = b->blk ? 1 : 0;
+    return WENOX_OK;
+}
+
+extern "C" int wenox_batch_set_layout(wenox_batch* b, int blocked)
+{
+    if (!b || (blocked != 0 && blocked != 1)) return abi_fail(WENOX_E_ARG, "wenox_batch_set_layout: blocked is 0 or 1");
+    if ((blocked != 0) == b->blk) return WENOX_OK;
+    if (blocked && !b->blk_mem)
+        return abi_fail(WENOX_E_ARG, "the blocked layout serves the batched fast FP64 step");
+    BCHK(hipSetDevice(b->device));
+    if (blocked) {
+        // the row-major states hold every instance (rows are current when not blocked)
+        for (const BatchInst& I : b->inst) b->sops->to_blk(inst_plan(I), b->U + I.soff, b->Ub + I.boff, b->stream);
+        b->rows_stale = false;
+        b->blk = true;
+    } else {
+        sync_rows(b);
+        // the row-major step also reads the transposed copies
+        for (const BatchInst& I : b->inst)
+            b->sops->transpose(b->U + I.soff, b->UT + I.soff, I.nx + 6, I.ny + 6, I.ny + 6, I.nx + 6, b->stream);
+        b->blk = false;
+    }
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    BCHK(hipStreamSynchronize(b->stream));
+    BCHK(hipGetLastError());
+    return WENOX_OK;
 }

@@ -310,22 +310,22 @@ __device__ __forceinline__ void cfl_finish(StepCtl* ctl, double cmax_state)
     ctl->dt = dt;
 }
 
-// Block `blk` of `nblk` blocks reducing one state's interior.  With a
-// per-block array (ctl->part) the block stores its maximum there and
-// k_cfl_dt finishes; without one (the batched engine, a few blocks per
-// instance) blocks combine with atomics and the last one finishes.
-template <class St>
-__device__ __forceinline__ void cfl_block(St U, int64_t nx, int64_t ny, int64_t ld, double gam, StepCtl* ctl,
-                                          int64_t blk, int64_t nblk)
+// Block `blk` of `nblk` blocks reducing one state's interior, whose padded
+// cell (i, j) is element at(i, j) of its planes.  With a per-block array
+// (ctl->part) the block stores its maximum there and k_cfl_dt finishes;
+// without one (the batched engine, a few blocks per instance) blocks combine
+// with atomics and the last one finishes.
+template <class St, class At>
+__device__ __forceinline__ void cfl_block_at(St U, int64_t nx, int64_t ny, At at, double gam, StepCtl* ctl,
+                                             int64_t blk, int64_t nblk)
 {
     __shared__ double part[ST_TB / 64];
     __shared__ bool last;
     double vmaxl = 0.0;
     // rows blk, blk + nblk, ...; the block's threads stride along the row
     for (int64_t i = blk; i < nx; i += nblk) {
-        const int64_t o0 = (i + 3) * ld + 3;
         for (int64_t j = threadIdx.x; j < ny; j += ST_TB) {
-            const int64_t o = o0 + j;
+            const int64_t o = at(i + 3, j + 3);
             double q[4] = {U.ld(0, o), U.ld(1, o), U.ld(2, o), U.ld(3, o)};
             vmaxl = nan_max(vmaxl, cfl_speed(q, gam, ctl->rho_floor));
         }
@@ -354,6 +354,14 @@ __device__ __forceinline__ void cfl_block(St U, int64_t nx, int64_t ny, int64_t 
     __threadfence();
     unsigned long long bits = atomicAdd(&ctl->cmax_bits, 0ull);
     cfl_finish(ctl, __longlong_as_double((long long)bits));
+}
+
+// Row-major planes with row pitch ld.
+template <class St>
+__device__ __forceinline__ void cfl_block(St U, int64_t nx, int64_t ny, int64_t ld, double gam, StepCtl* ctl,
+                                          int64_t blk, int64_t nblk)
+{
+    cfl_block_at(U, nx, ny, [&](int64_t i, int64_t j) { return i * ld + j; }, gam, ctl, blk, nblk);
 }
 
 template <class St>

@@ -135,6 +135,7 @@ struct BatchInst {
     int tiles_j, tile0;    // combine: tile columns, first tile
     int ghost_blk0;        // first ghost-fill block
     int cfl_blk0, cfl_nblk;
+    int64_t boff, nb;      // blocked step: offset (doubles) of the instance's blocked state, its nb
 };
 
 struct BatchTables {       // device pointers and grid sizes
@@ -162,6 +163,15 @@ struct BatchOps {
     void (*cfl)(const BatchTables& t, int split, const void* U, double gam, StepCtl* ctls,
                 hipStream_t s);
     void (*advance)(StepCtl* ctls, int n, hipStream_t s);
+    // ---- blocked FP64 fast step (fast build only; nullptr in the exact build) ----
+    // the instances' blocked states start at BatchInst::boff: per stage the x
+    // sweeps write E, the y sweeps finish the stage (SweepOps::stage_blk, the
+    // CFL maxima left to cfl_blk); the ghost fill and the CFL reduction of the
+    // blocked states
+    void (*stage_blk)(const BatchTables& t, const BatchCommon& c, const double* Us, const double* Un, double* E,
+                      double* out, int stage, StepCtl* ctls, hipStream_t s);
+    void (*bc_blk)(const BatchTables& t, BcParams bp, double* U, StepCtl* ctls, int stage, hipStream_t s);
+    void (*cfl_blk)(const BatchTables& t, const double* U, double gam, StepCtl* ctls, hipStream_t s);
 };
 
 extern const BatchOps batch_ops_exact;

@@ -103,6 +103,7 @@ struct SweepArgs {
     // (blk_row / blk_col with nb lines per line row, components adjacent;
     // for the y sweep rows are y and lanes x, as in the transposed copy)
     int64_t nb = 0;
+    int cfl_out = 0;            // FOLD_BO, stage 3: leave the CFL maxima in ctl->part
 };
 
 // Shu-Osher coefficients of stage s: U_{s+1} = a Un + b Us + (c dt) L
@@ -289,7 +290,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
         else return (DIR == DIR_X ? blk_row(r, A.nb) : blk_col(r)) + lane_off;
     };
     double wmax = 0.0;   // FOLD_BO, stage 3: max of c + |V| over this lane's new cells
-    const bool cfl_max = FOLD == FOLD_BO && A.stage == 3;
+    const bool cfl_max = FOLD == FOLD_BO && A.stage == 3 && A.cfl_out;
 
     T F_prev[4], fc_prev[4], fe_prev[4];
     T pb_m2 = T(0), pb_m1 = T(0), pb_r0n = T(0);   // p̄ of faces a-2, a-1; p̄(r0 + 1)
@@ -1178,6 +1179,7 @@ static SweepArgs args_blk(const SweepPlan& p, int dir, const double* U, const do
     a.Un = Un;
     a.stage = stage;
     a.nb = p.nb;
+    a.cfl_out = 1;
     return a;
 }
 
@@ -1495,13 +1497,113 @@ static void hb_advance(StepCtl* ctls, int n, hipStream_t s)
     k_advance_batch<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(ctls, n);
 }
 
+#if WX_FAST
+// ---- batched blocked FP64 step ----------------------------------------------
+// One direction's sweeps of every instance: workgroup b of the launch is
+// block b - blk0 of instance blk_x[b] (x) or blk_y[b] (y), one wavefront each.
+template <int DIR, int FOLD, bool CONS, bool BLEND>
+__global__ void __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(2)))
+k_sweep_batch_blk(BatchTables B, BatchCommon C, const double* Ub, const double* Unb, double* Db, StepCtl* ctls,
+                  int stage)
+{
+    const int b = blockIdx.x;
+    const int i = DIR == DIR_X ? B.blk_x[b] : B.blk_y[b];
+    const BatchInst I = B.inst[i];
+    SweepArgs A;
+    A.U = Ub + I.boff;
+    A.Un = Unb + I.boff;
+    A.D = Db + I.boff;
+    A.M = DIR == DIR_X ? I.nx + 6 : I.ny + 6;
+    A.N = DIR == DIR_X ? I.ny + 6 : I.nx + 6;
+    A.ld = A.N;
+    A.ldd = DIR == DIR_X ? I.ny : I.nx;
+    A.nn = DIR == DIR_X ? I.nx : I.ny;
+    A.nt = DIR == DIR_X ? I.ny : I.nx;
+    A.chunk = DIR == DIR_X ? I.chunk_x : I.chunk_y;
+    A.waves = 1;
+    A.gam = C.gam;
+    A.dx = I.dx;
+    A.two_h = DIR == DIR_X ? I.two_h_x : I.two_h_y;
+    A.lo = C.jp_lo;
+    A.hi = C.jp_hi;
+    A.flags = nullptr;
+    A.ctl = &ctls[i];
+    A.stage = stage;
+    A.nb = I.nb;
+    __shared__ SweepLds<double, 1> S;
+    const int local = b - (DIR == DIR_X ? I.blk0_x : I.blk0_y);
+    const int gx = DIR == DIR_X ? I.gx_x : I.gx_y;
+    sweep_body<double, IN_F64, double, MODE_PLAIN, DIR, CONS, BLEND, 1, FOLD>(A, local % gx, local / gx, b, S);
+}
+
+template <int DIR, int FOLD>
+static void launch_sweep_batch_blk(const BatchTables& t, const BatchCommon& c, const double* U, const double* Un,
+                                   double* D, StepCtl* ctls, int stage, hipStream_t s)
+{
+    const unsigned n = (unsigned)(DIR == DIR_X ? t.n_x : t.n_y);
+    if (!n) return;
+#define WX_KB(CO, BL) k_sweep_batch_blk<DIR, FOLD, CO, BL><<<n, WAVE, 0, s>>>(t, c, U, Un, D, ctls, stage)
+    if (c.cons) {
+        if (!c.force_hlle) WX_KB(true, true);
+        else WX_KB(true, false);
+    } else {
+        if (!c.force_hlle) WX_KB(false, true);
+        else WX_KB(false, false);
+    }
+#undef WX_KB
+}
+
+static void hb_stage_blk(const BatchTables& t, const BatchCommon& c, const double* Us, const double* Un, double* E,
+                         double* out, int stage, StepCtl* ctls, hipStream_t s)
+{
+    launch_sweep_batch_blk<DIR_X, FOLD_BE>(t, c, Us, Un, E, ctls, stage, s);
+    launch_sweep_batch_blk<DIR_Y, FOLD_BO>(t, c, Us, E, out, ctls, stage, s);
+}
+
+__global__ void k_bc_batch_blk(BatchTables B, BcParams bp, double* U, StepCtl* ctls, int stage)
+{
+    const int b = blockIdx.x;
+    const int i = B.blk_g[b];
+    const BatchInst I = B.inst[i];
+    const StF64 Ui{U + I.boff, 1};
+    auto at = [&](int64_t r, int64_t c) { return blk_row(r, I.nb) + blk_col(c); };
+    bc_cell_at(Ui, I.nx, I.ny, bp, &ctls[i], stage, (int64_t)(b - I.ghost_blk0) * ST_TB + threadIdx.x, at,
+               [&](int64_t r, int64_t c, const double* v) {
+#pragma unroll
+                   for (int k = 0; k < 4; ++k) Ui.st(k, at(r, c), v[k]);
+               });
+}
+
+__global__ void __launch_bounds__(ST_TB) k_cfl_batch_blk(BatchTables B, const double* U, double gam, StepCtl* ctls)
+{
+    const int b = blockIdx.x;
+    const int i = B.blk_f[b];
+    const BatchInst I = B.inst[i];
+    cfl_block_at(StF64{const_cast<double*>(U) + I.boff, 1}, I.nx, I.ny,
+                 [&](int64_t r, int64_t c) { return blk_row(r, I.nb) + blk_col(c); }, gam, &ctls[i],
+                 b - I.cfl_blk0, I.cfl_nblk);
+}
+
+static void hb_bc_blk(const BatchTables& t, BcParams bp, double* U, StepCtl* ctls, int stage, hipStream_t s)
+{
+    k_bc_batch_blk<<<(unsigned)t.n_g, ST_TB, 0, s>>>(t, bp, U, ctls, stage);
+}
+
+static void hb_cfl_blk(const BatchTables& t, const double* U, double gam, StepCtl* ctls, hipStream_t s)
+{
+    k_cfl_batch_blk<<<(unsigned)t.n_f, ST_TB, 0, s>>>(t, U, gam, ctls);
+}
+#endif
+
 }  // namespace WX_NS
 
 #ifndef __HIP_DEVICE_COMPILE__
 #if WX_FAST
-const BatchOps batch_ops_fast = {WX_NS::hb_stage, WX_NS::hb_bc, WX_NS::hb_cfl, WX_NS::hb_advance};
+const BatchOps batch_ops_fast = {WX_NS::hb_stage,     WX_NS::hb_bc,     WX_NS::hb_cfl,    WX_NS::hb_advance,
+                                 WX_NS::hb_stage_blk, WX_NS::hb_bc_blk, WX_NS::hb_cfl_blk};
 #else
-const BatchOps batch_ops_exact = {WX_NS::hb_stage, WX_NS::hb_bc, WX_NS::hb_cfl, WX_NS::hb_advance};
+const BatchOps batch_ops_exact = {WX_NS::hb_stage, WX_NS::hb_bc, WX_NS::hb_cfl, WX_NS::hb_advance,
+                                  nullptr,         nullptr,      nullptr};
 #endif
 #if WX_FAST
 const SweepOps WX_SWEEP_OPS = {WX_NS::h_stage,       WX_NS::h_rhs,        WX_NS::h_transpose,

@@ -397,6 +397,18 @@ class Batch:
     def set_chunks(self, chunk_x: int, chunk_y: int):
         _lib.check(_lib.load().wenox_batch_set_chunks(self._h, int(chunk_x), int(chunk_y)))
 
+    @property
+    def blocked(self) -> bool:
+        """Blocked state layout of the batched FP64 fast step (its default),
+        as Solver.blocked; the same states bit for bit as the row-major
+        step (set_blocked(False))."""
+        v = ctypes.c_int()
+        _lib.check(_lib.load().wenox_batch_layout(self._h, ctypes.byref(v)))
+        return bool(v.value)
+
+    def set_blocked(self, on: bool = True):
+        _lib.check(_lib.load().wenox_batch_set_layout(self._h, int(bool(on))))
+
     def profile(self, reps: int = 5) -> float:
         """Device time (ms) of one RK stage of every instance as a step
         launches it; the state is left unchanged."""

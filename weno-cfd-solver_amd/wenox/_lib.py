@@ -117,6 +117,8 @@ def load():
         "wenox_batch_chunks": ([dp, ctypes.POINTER(i), ctypes.POINTER(i)], i),
         "wenox_batch_set_chunks": ([dp, i, i], i),
         "wenox_batch_profile": ([dp, i, dp], i),
+        "wenox_batch_layout": ([dp, ctypes.POINTER(i)], i),
+        "wenox_batch_set_layout": ([dp, i], i),
         "wenox_batch_status": ([dp, dp, dp], i),
         "wenox_batch_last_ms_per_step": ([dp], d),
     }
