@@ -78,6 +78,29 @@ def test_four_quadrant_256_50_steps(oracle_mod, precision, variant):
         assert np.all(err <= 1e-6), err
 
 
+def test_blocked_layout_at_4096():
+    """The benchmark grid at its full size: the blocked fast FP64 step (the
+    default) and the row-major one give the same state bit for bit after 12
+    steps, on the tilings the autotuner picks for 4096^2."""
+    from wenox import Solver, problems
+    n, steps = 4096, 12
+    q0, dx = problems.four_quadrant(n)
+    out = {}
+    for blocked, tiling in ((True, (48, 48, 1, 1)), (False, (48, 48, 1, 1)), (True, (57, 57, 1, 1))):
+        print(f"[fullsize] 4096^2 blocked={blocked} tiling={tiling}", flush=True)
+        with Solver(n, n, dx, cfl=0.4, jp_cri=(1e4, 1.1e4), bc="mirror", engine="fused", variant="fast") as s:
+            s.set_blocked(blocked)
+            s.set_tiling(*tiling)
+            s.upload(q0)
+            T, k = s.run(nstep=steps)
+            out[(blocked, tiling)] = (s.download()[..., :4], T, k)
+    ref = out[(False, (48, 48, 1, 1))]
+    assert ref[2] == steps and np.isfinite(ref[0]).all()
+    for key, got in out.items():
+        assert got[1] == ref[1] and got[2] == ref[2], key
+        np.testing.assert_array_equal(got[0], ref[0], err_msg=str(key))
+
+
 def test_dmr_production_size_matches_oracle(oracle_mod):
     from wenox import Solver, problems
     variant = "exact"
