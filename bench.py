@@ -144,8 +144,10 @@ CONFIG_RANGES = dict(rho=(0.1, 2.0), p=(0.02, 2.0), vel=1.5)
 
 
 def run_batch(w, steps, warmup, precision, chunk=16, ranges=None):
-    """The batched engine: every phase of a step is one launch over all
-    instances (per-instance dt); value = sum of instance cells x steps / wall.
+    """The batched engine (per-instance dt), with the step structure and the
+    sweep chunk its autotuner picks: the persistent step (one launch per
+    eight steps, work items from a device queue) or one launch per phase over
+    all instances; value = sum of instance cells x steps / wall.
     ranges: the instances' state ranges (default: the narrowed ranges of
     problems.random_riemann_instance)."""
     import numpy as np
@@ -167,7 +169,9 @@ def run_batch(w, steps, warmup, precision, chunk=16, ranges=None):
         bad = [i for i in range(w["count"]) if not np.isfinite(b.download(i)[3:-3, 3:-3, :4]).all()]
     return dict(value=cells * steps / el, seconds=el, cells=cells, steps_each=int(np.min(k)) - warmup,
                 ms_per_step=el / steps * 1e3, non_finite=len(bad), chunk=tuned["best"],
-                autotune_stage_ms={str(c): round(v, 5) for c, v in tuned.items() if c != "best"})
+                autotune_stage_ms={str(c): round(v, 5) for c, v in tuned.items() if isinstance(c, int)},
+                step_structure="persistent" if tuned.get("persistent") else "launch per phase",
+                autotune_step_ms={m: {str(c): v for c, v in d.items()} for m, d in tuned.get("step_ms", {}).items()})
 
 
 def _cpu_steps(q, dx, w, budget_s):
@@ -520,6 +524,7 @@ def batch_block(w, steps, warmup):
     rc = run_batch(w, steps, warmup, "fp64", ranges=CONFIG_RANGES)
     return {"workload": describe(w), "cells": r["cells"], "steps": steps, "warmup": warmup,
             "value_fp64": r["value"], "ms_per_step_fp64": r["ms_per_step"], "chunk": r["chunk"],
+            "step_structure": r["step_structure"], "autotune_step_ms": r["autotune_step_ms"],
             "autotune_stage_ms": r["autotune_stage_ms"],
             # the reference scheme itself turns some instances non-finite (the oracle
             # reproduces it); they keep stepping, so the count does not change the cost

@@ -267,6 +267,27 @@ int wenox_batch_profile(wenox_batch *b, int reps, double *stage_ms);
  * No reference counterpart. */
 int wenox_batch_layout(wenox_batch *b, int *blocked);
 int wenox_batch_set_layout(wenox_batch *b, int blocked);
+/* Persistent step of the blocked fast FP64 batch (on = 1): one launch advances
+ * every instance one step, or eight in the graph that wenox_batch_run replays
+ * while at least eight steps remain.  Its workgroups take work items (CFL,
+ * x-sweep, y-sweep and ghost-fill blocks of each instance and stage) in
+ * order from a device counter; an item waits only for the earlier items of
+ * its own instance whose results it reads, so instances move through the
+ * phases of a step independently.  Same states bit for bit as the
+ * launch-per-phase step (on = 0).  A wait that passes its spin limit (about a
+ * second) makes wenox_batch_run return WENOX_E_DEVICE.  No reference
+ * counterpart. */
+int wenox_batch_persistent(wenox_batch *b, int *on);
+int wenox_batch_set_persistent(wenox_batch *b, int on);
+/* Polls of a persistent step's dependency wait before it gives up and sets
+ * the error flag (default 4194304, about a second).  A small value forces the
+ * error path, for its tests; the launch still drains.  No reference
+ * counterpart. */
+int wenox_batch_set_spin_limit(wenox_batch *b, int spins);
+/* Device time (ms, HIP events) per step of `steps` whole steps of every
+ * instance as wenox_batch_run launches them (for the autotuner); the states
+ * and step controls are restored afterwards.  No reference counterpart. */
+int wenox_batch_profile_steps(wenox_batch *b, int steps, double *ms_per_step);
 
 /* ---- 4. jp_cri tuning objective --------------------------------------------- */
 /* A read-only 2-D host array view: extents and element strides. */

@@ -154,3 +154,129 @@ def test_batch_blocked_layout_same_states(bc, cons):
         np.testing.assert_array_equal(T1, T0)
         for i in range(len(inst)):
             np.testing.assert_array_equal(q1[i], q0[i], err_msg=f"{mode} instance {i}")
+
+
+def _mixed_instances(count, seed):
+    """count instances of sizes from {16, 24, ..., 96} x {16, ..., 96}."""
+    from wenox import problems
+    rng = np.random.default_rng(seed)
+    sizes = [(int(a), int(b)) for a, b in rng.choice(np.arange(16, 97, 8), size=(count, 2))]
+    return sizes, [problems.random_riemann_instance(rng, nx, ny) for nx, ny in sizes]
+
+
+def _run_modes(sizes, inst, modes, script, **kw):
+    from wenox import Batch
+    out = {}
+    for mode, graph in modes:
+        print(f"[batch] {len(sizes)} instances, step structure {mode}, graphs {graph}, {kw}", flush=True)
+        with Batch(sizes, [q[1] for q in inst], use_graph=graph, variant="fast", **kw) as b:
+            b.set_persistent(mode == "persistent")
+            assert b.persistent == (mode == "persistent")
+            for i, (q, dx) in enumerate(inst):
+                b.upload(i, q)
+            script(b)
+            T, k = b.status()
+            out[(mode, graph)] = (T, k, [b.download(i) for i in range(len(sizes))])
+    return out
+
+
+def _assert_same(out):
+    keys = list(out)
+    T0, k0, q0 = out[keys[0]]
+    assert all(np.isfinite(q[3:-3, 3:-3, :4]).all() for q in q0)
+    for key in keys[1:]:
+        T1, k1, q1 = out[key]
+        np.testing.assert_array_equal(k1, k0, err_msg=str(key))
+        np.testing.assert_array_equal(T1, T0, err_msg=str(key))
+        for i in range(len(q0)):
+            np.testing.assert_array_equal(q1[i], q0[i], err_msg=f"{key} instance {i}")
+
+
+@pytest.mark.parametrize("cons,force_hlle,bc", [(False, False, "mirror"), (False, True, "periodic"),
+                                                (True, False, "copy"), (True, True, "mirror")])
+def test_batch_persistent_same_states(cons, force_hlle, bc):
+    """The persistent step (work items from a device queue, each waiting for
+    its own instance's earlier phases) gives every instance the same state,
+    time and step count bit for bit as the launch-per-phase step: single
+    steps, eight-step launches with a remainder, a t_final run in which the
+    instances stop at different steps inside one launch, and without graphs."""
+    inst = _instances(13)
+
+    def script(b):
+        b.run(nstep=3)
+        b.run(nstep=11)
+        T, _ = b.status()
+        b.run(t_final=float(T.max()) * 1.7)
+
+    out = _run_modes(SIZES, inst, (("phases", True), ("persistent", True), ("persistent", False)), script,
+                     cfl=0.4, jp_cri=(0.5, 2.0), bc=bc, cons_switch=cons, force_hlle=force_hlle)
+    T, k, _ = out[("phases", True)]
+    assert len(set(k.tolist())) > 1        # the t_final run stopped instances at different steps
+    _assert_same(out)
+
+
+@pytest.mark.parametrize("chunk", [5, 48])
+def test_batch_persistent_many_items(chunk):
+    """Forty uneven instances with short chunks (thousands of work items per
+    step) and long ones: the persistent step equals the launch-per-phase step
+    bit for bit over 17 steps (two eight-step launches and one single)."""
+    sizes, inst = _mixed_instances(40, 29)
+    out = _run_modes(sizes, inst, (("phases", True), ("persistent", True)), lambda b: b.run(nstep=17),
+                     cfl=0.4, jp_cri=(1e4, 1.1e4), chunk=chunk)
+    _assert_same(out)
+
+
+def test_batch_persistent_wait_limit_error_path():
+    """A dependency wait that passes its spin limit (forced here with a limit
+    of zero polls, so waits give up at once) ends in an error from run(); the
+    launch still drains, the flag is cleared, and with the normal limit the
+    next run is correct again."""
+    from wenox import Batch, WenoxError
+    sizes, inst = _mixed_instances(40, 31)
+    ref = _run_modes(sizes, inst, (("phases", True),), lambda b: b.run(nstep=8), cfl=0.4, jp_cri=(1e4, 1.1e4))
+    with Batch(sizes, [q[1] for q in inst], cfl=0.4, jp_cri=(1e4, 1.1e4), variant="fast") as b:
+        b.set_persistent(True)
+        for i, (q, dx) in enumerate(inst):
+            b.upload(i, q)
+        b.set_spin_limit(0)
+        print("[batch] persistent step with a zero spin limit (error path)", flush=True)
+        with pytest.raises(WenoxError, match="spin limit"):
+            b.run(nstep=8)
+        b.set_spin_limit(1 << 22)
+        for i, (q, dx) in enumerate(inst):
+            b.upload(i, q)
+        b.run(nstep=8)
+        T, k = b.status()
+        got = [b.download(i) for i in range(len(sizes))]
+    T0, k0, q0 = ref[("phases", True)]
+    np.testing.assert_array_equal(k, k0)
+    np.testing.assert_array_equal(T, T0)
+    for i in range(len(sizes)):
+        np.testing.assert_array_equal(got[i], q0[i], err_msg=f"instance {i}")
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+def test_batch_profile_steps_restores_state(persistent):
+    """profile_steps times whole steps and leaves states and times as they
+    were; autotune picks a (structure, chunk) pair that gives the same states."""
+    from wenox import Batch
+    inst = _instances(17)
+
+    def script(b):
+        b.set_persistent(persistent)
+        b.run(nstep=4)
+        assert b.profile_steps(9) > 0.0
+        b.run(nstep=5)
+
+    out = _run_modes(SIZES, inst, (("phases", True),), script, cfl=0.4, jp_cri=(0.5, 2.0))
+    ref = _run_modes(SIZES, inst, (("phases", True),), lambda b: b.run(nstep=9), cfl=0.4, jp_cri=(0.5, 2.0))
+    _assert_same({"profiled": out[("phases", True)], "plain": ref[("phases", True)]})
+    with Batch(SIZES, [q[1] for q in inst], cfl=0.4, jp_cri=(0.5, 2.0), variant="fast") as b:
+        for i, (q, dx) in enumerate(inst):
+            b.upload(i, q)
+        tuned = b.autotune(candidates=(8, 16, 48), reps=1, steps=2)
+        assert set(tuned["step_ms"]) == {"phases", "persistent"} and tuned["persistent"] == b.persistent
+        b.run(nstep=9)
+        T, k = b.status()
+        got = [b.download(i) for i in range(len(SIZES))]
+    _assert_same({"plain": ref[("phases", True)], "tuned": (T, k, got)})

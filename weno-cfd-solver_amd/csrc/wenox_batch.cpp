@@ -64,13 +64,47 @@ struct wenox_batch {
     bool rows_stale = false;
     double* blk_mem = nullptr;
     double *Ub = nullptr, *Ab = nullptr, *Bb = nullptr, *Eb = nullptr;
+    // persistent blocked step (BatchOps::persist): one launch per step, or per
+    // PERSIST_STEPS steps (graph8); pcnt: n * PC_N counters, ticket, error flag
+    bool persist = false;
+    int pgrid = 0;
+    int pspin = PERSIST_SPIN_LIMIT;
+    int* pcnt = nullptr;
+    PersistQueue pq[PERSIST_QUEUES] = {};
+    int* pitems = nullptr;   // the queues' item lists (device)
+    hipGraph_t graph8 = nullptr;
+    hipGraphExec_t exec8 = nullptr;
 };
 
 namespace {
 
+bool persistent(const wenox_batch* b) { return b->blk && b->persist; }
+
+void persist_steps(wenox_batch* b, int steps)
+{
+    b->bops->persist(b->tab, b->com, b->bp, b->Ub, b->Ab, b->Bb, b->Eb, b->ctls, b->pq, b->pitems, b->pcnt, steps,
+                     b->pgrid, b->pspin, b->stream);
+}
+
+void drop_graphs(wenox_batch* b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->exec8) (void)hipGraphExecDestroy(b->exec8);
+    if (b->graph8) (void)hipGraphDestroy(b->graph8);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->exec8 = nullptr;
+    b->graph8 = nullptr;
+}
+
 void batch_step(wenox_batch* b)
 {
     hipStream_t st = b->stream;
+    if (persistent(b)) {
+        persist_steps(b, 1);
+        return;
+    }
     if (b->blk) {
         double* src[3] = {b->Ub, b->Ab, b->Bb};
         double* dst[3] = {b->Ab, b->Bb, b->Ub};
@@ -93,6 +127,74 @@ void batch_step(wenox_batch* b)
         b->bops->bc(b->tab, b->bp, b->split, dst[s - 1], dstT[s - 1], b->ctls, s, st);
     }
     b->bops->advance(b->ctls, b->n, st);
+}
+
+// The persistent step's queues: instances go to the PERSIST_QUEUES queues
+// (one per XCD) largest first, each to the queue with the least work so far
+// (cells); a queue lists, per phase, the blocks of its instances in that
+// order.  Returns the lists (indices into bf, bx, by, bg) and fills Q.
+std::vector<int> persist_queues(const std::vector<BatchInst>& inst, PersistQueue* Q)
+{
+    const int n = (int)inst.size();
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+        return inst[a].nx * inst[a].ny > inst[c].nx * inst[c].ny;
+    });
+    std::vector<std::vector<int>> members(PERSIST_QUEUES);
+    std::vector<int64_t> load(PERSIST_QUEUES, 0);
+    for (int i : order) {
+        const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        members[q].push_back(i);
+        load[q] += inst[i].nx * inst[i].ny;
+    }
+    std::vector<int> items;
+    for (int q = 0; q < PERSIST_QUEUES; ++q) {
+        PersistQueue& P = Q[q];
+        P.off = (int)items.size();
+        const size_t f0 = items.size();
+        for (int i : members[q])
+            for (int k = 0; k < inst[i].cfl_nblk; ++k) items.push_back(inst[i].cfl_blk0 + k);
+        P.n_f = (int)(items.size() - f0);
+        const size_t x0 = items.size();
+        for (int i : members[q])
+            for (int k = 0; k < batch_items_x(inst[i]); ++k) items.push_back(inst[i].blk0_x + k);
+        P.n_x = (int)(items.size() - x0);
+        const size_t y0 = items.size();
+        for (int i : members[q])
+            for (int k = 0; k < batch_items_y(inst[i]); ++k) items.push_back(inst[i].blk0_y + k);
+        P.n_y = (int)(items.size() - y0);
+        const size_t g0 = items.size();
+        for (int i : members[q])
+            for (int k = 0; k < batch_items_g(inst[i]); ++k) items.push_back(inst[i].ghost_blk0 + k);
+        P.n_g = (int)(items.size() - g0);
+    }
+    return items;
+}
+
+// Every queue's tickets of a PERSIST_STEPS-step persistent launch walked in
+// order, as one workgroup would run them: every item's wait must already hold
+// from the items before it -- then the oldest unfinished item of a queue can
+// always run, whatever the number of workgroups -- and every instance ends
+// PERSIST_STEPS steps.
+bool persist_order_holds(const BatchTables& th, const PersistQueue* Q, const int* items, int n)
+{
+    std::vector<int> c((size_t)n * PC_N, 0);
+    for (int q = 0; q < PERSIST_QUEUES; ++q) {
+        const int total = persist_items_per_step(Q[q]) * PERSIST_STEPS;
+        for (int t = 0; t < total; ++t) {
+            const PersistItem it = persist_item(t, Q[q], items, th);
+            if (it.inst < 0 || it.inst >= n) return false;
+            int* ci = c.data() + (size_t)it.inst * PC_N;
+            if (it.wait_ctr >= 0 && ci[it.wait_ctr] < it.wait_target) return false;
+            const int prev = ci[it.sig_ctr]++;
+            if (it.last_count > 0 && prev + 1 == it.last_count) ci[it.done_ctr]++;
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        if (c[(size_t)i * PC_N + PC_STEP] != PERSIST_STEPS || c[(size_t)i * PC_N + PC_CFL] != PERSIST_STEPS)
+            return false;
+    return true;
 }
 
 // Instance table and workgroup maps for sweep chunks (chunk_x, chunk_y),
@@ -153,11 +255,21 @@ int build_tables(wenox_batch* b, int chunk_x, int chunk_y)
     BCHK(hipMemcpy(b->dev_tables, host.data(), tbytes, hipMemcpyHostToDevice));
     b->chunk_x = chunk_x;
     b->chunk_y = chunk_y;
-    // the captured step holds the old grids
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
+    // the persistent step's queues over these tables, their order walked on the host
+    if (b->pcnt) {
+        const std::vector<int> items = persist_queues(b->inst, b->pq);
+        BatchTables th = b->tab;
+        th.inst = b->inst.data();
+        th.blk_x = bx.data(); th.blk_y = by.data(); th.blk_c = bc.data(); th.blk_g = bg.data(); th.blk_f = bf.data();
+        if (!persist_order_holds(th, b->pq, items.data(), b->n))
+            return abi_fail(WENOX_E_DEVICE, "persistent batched step: an item would wait for a later ticket");
+        if (b->pitems) BCHK(hipFree(b->pitems));
+        b->pitems = nullptr;
+        BCHK(hipMalloc(&b->pitems, sizeof(int) * std::max<size_t>(items.size(), 1)));
+        BCHK(hipMemcpy(b->pitems, items.data(), sizeof(int) * items.size(), hipMemcpyHostToDevice));
+    }
+    // the captured steps hold the old grids
+    drop_graphs(b);
     return WENOX_OK;
 }
 
@@ -190,6 +302,25 @@ int read_ctls(wenox_batch* b, std::vector<StepCtl>& h)
     return WENOX_OK;
 }
 
+// The persistent step's error flag: set when a dependency wait passed its spin
+// limit (the launch still drains; its results are not valid).  Cleared on read.
+int persist_error(wenox_batch* b)
+{
+    if (!b->pcnt) return WENOX_OK;
+    int* flag = b->pcnt + (size_t)b->n * PC_N + PW_ERR;
+    int e = 0;
+    BCHK(hipMemcpyAsync(&e, flag, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    BCHK(hipStreamSynchronize(b->stream));
+    if (!e) return WENOX_OK;
+    BCHK(hipMemsetAsync(flag, 0, sizeof(int), b->stream));
+    BCHK(hipStreamSynchronize(b->stream));
+    if (e == 2)
+        return abi_fail(WENOX_E_DEVICE, "persistent batched step: a work queue was left with items (no workgroup "
+                                        "ran on its XCD); the states of this run are not valid");
+    return abi_fail(WENOX_E_DEVICE, "persistent batched step: a dependency wait passed its spin limit; "
+                                    "the states of this run are not valid");
+}
+
 int write_ctls(wenox_batch* b, const std::vector<StepCtl>& h)
 {
     BCHK(hipMemcpyAsync(b->ctls, h.data(), sizeof(StepCtl) * b->n, hipMemcpyHostToDevice, b->stream));
@@ -202,8 +333,9 @@ int write_ctls(wenox_batch* b, const std::vector<StepCtl>& h)
 extern "C" void wenox_batch_destroy(wenox_batch* b)
 {
     if (!b) return;
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
+    drop_graphs(b);
+    if (b->pcnt) (void)hipFree(b->pcnt);
+    if (b->pitems) (void)hipFree(b->pitems);
     for (double* p : {b->U, b->A, b->B, b->UT, b->AT, b->BT, b->Dx, b->DyT, b->work, b->blk_mem})
         if (p) (void)hipFree(p);
     if (b->flags) (void)hipFree(b->flags);
@@ -280,6 +412,13 @@ extern "C" int wenox_batch_create(const wenox_solver_desc* desc, int n, const in
             b->Bb = b->Ab + boff;
             b->Eb = b->Bb + boff;
         }
+    }
+    if (ok && b->blk) {
+        const size_t nc = (size_t)n * PC_N + PW_N;
+        ok = hipMalloc(&b->pcnt, nc * sizeof(int)) == hipSuccess &&
+             hipMemset(b->pcnt, 0, nc * sizeof(int)) == hipSuccess;
+        b->pgrid = ok ? b->bops->persist_grid(BatchCommon{d.gamma, d.jp_lo, d.jp_hi, d.cons_switch, d.force_hlle}) : 0;
+        ok = ok && b->pgrid > 0;
     }
     if (!ok) return cleanup(abi_fail(WENOX_E_DEVICE, "batch allocation failed (" + std::to_string((6 * soff + 2 * ioff + (b->blk ? 4 * boff : 0)) * 8 >> 20) + " MiB)"));
     int rct = build_tables(b, chunk, chunk);
@@ -401,14 +540,26 @@ extern "C" int wenox_batch_run(wenox_batch* b, int64_t max_steps, double t_final
     if ((rc = write_ctls(b, c))) return rc;
     // the steps write the blocked states only
     if (b->blk) b->rows_stale = true;
+    const bool multi = persistent(b);   // PERSIST_STEPS steps per launch where they fit
     if (b->d.use_graph && !b->exec) {
         BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         batch_step(b);
         BCHK(hipStreamEndCapture(b->stream, &b->graph));
         BCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
+    if (b->d.use_graph && multi && !b->exec8) {
+        BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        persist_steps(b, PERSIST_STEPS);
+        BCHK(hipStreamEndCapture(b->stream, &b->graph8));
+        BCHK(hipGraphInstantiate(&b->exec8, b->graph8, nullptr, nullptr, 0));
+    }
     auto launch = [&](int64_t k) -> int {
-        for (int64_t s = 0; s < k; ++s) {
+        int64_t s = 0;
+        for (; multi && s + PERSIST_STEPS <= k; s += PERSIST_STEPS) {
+            if (b->exec8) BCHK(hipGraphLaunch(b->exec8, b->stream));
+            else persist_steps(b, PERSIST_STEPS);
+        }
+        for (; s < k; ++s) {
             if (b->exec) BCHK(hipGraphLaunch(b->exec, b->stream));
             else batch_step(b);
         }
@@ -432,6 +583,7 @@ extern "C" int wenox_batch_run(wenox_batch* b, int64_t max_steps, double t_final
     BCHK(hipEventRecord(b->ev1, b->stream));
     BCHK(hipStreamSynchronize(b->stream));
     BCHK(hipGetLastError());
+    if (multi && (rc = persist_error(b))) return rc;
     float ms = 0.f;
     BCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     if ((rc = read_ctls(b, c))) return rc;
@@ -534,11 +686,100 @@ extern "C" int wenox_batch_set_layout(wenox_batch* b, int blocked)
             b->sops->transpose(b->U + I.soff, b->UT + I.soff, I.nx + 6, I.ny + 6, I.ny + 6, I.nx + 6, b->stream);
         b->blk = false;
     }
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
+    drop_graphs(b);
     BCHK(hipStreamSynchronize(b->stream));
     BCHK(hipGetLastError());
     return WENOX_OK;
+}
+
+extern "C" int wenox_batch_persistent(wenox_batch* b, int* on)
+{
+    if (!b || !on) return abi_fail(WENOX_E_ARG, "wenox_batch_persistent: null argument");
+    *on = b->persist ? 1 : 0;
+    return WENOX_OK;
+}
+
+extern "C" int wenox_batch_set_persistent(wenox_batch* b, int on)
+{
+    if (!b || (on != 0 && on != 1)) return abi_fail(WENOX_E_ARG, "wenox_batch_set_persistent: on is 0 or 1");
+    if (on && !b->pcnt)
+        return abi_fail(WENOX_E_ARG, "the persistent step serves the batched fast FP64 step (blocked layout)");
+    if ((on != 0) == b->persist) return WENOX_OK;
+    BCHK(hipSetDevice(b->device));
+    BCHK(hipStreamSynchronize(b->stream));
+    b->persist = on != 0;
+    drop_graphs(b);
+    return WENOX_OK;
+}
+
+extern "C" int wenox_batch_set_spin_limit(wenox_batch* b, int spins)
+{
+    if (!b || spins < 0) return abi_fail(WENOX_E_ARG, "wenox_batch_set_spin_limit: spins must be >= 0");
+    BCHK(hipSetDevice(b->device));
+    BCHK(hipStreamSynchronize(b->stream));
+    b->pspin = spins;
+    drop_graphs(b);   // the captured launches hold the old limit
+    return WENOX_OK;
+}
+
+extern "C" int wenox_batch_profile_steps(wenox_batch* b, int steps, double* ms_per_step)
+{
+    if (!b || steps < 1 || !ms_per_step) return abi_fail(WENOX_E_ARG, "wenox_batch_profile_steps: bad argument");
+    BCHK(hipSetDevice(b->device));
+    // the state the steps overwrite (blocked: Ub; row-major: U and U^T) is
+    // saved in a scratch allocation and restored with the step controls
+    std::vector<StepCtl> h;
+    int rc = read_ctls(b, h);
+    if (rc) return rc;
+    const size_t sbytes = (size_t)b->inst.back().soff * 8 + (size_t)4 * (b->inst.back().nx + 6) * (b->inst.back().ny + 6) * 8;
+    const size_t bbytes = b->blk ? (size_t)(b->Ab - b->Ub) * 8 : 0;
+    const size_t nbytes = b->blk ? bbytes : 2 * sbytes;
+    char* save = nullptr;
+    BCHK(hipMalloc(&save, nbytes));
+    auto copy_state = [&](bool restore) -> int {
+        auto cp = [&](void* dst, const void* src, size_t n) {
+            return hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, b->stream);
+        };
+        hipError_t e;
+        if (b->blk) e = restore ? cp(b->Ub, save, bbytes) : cp(save, b->Ub, bbytes);
+        else {
+            e = restore ? cp(b->U, save, sbytes) : cp(save, b->U, sbytes);
+            if (e == hipSuccess) e = restore ? cp(b->UT, save + sbytes, sbytes) : cp(save + sbytes, b->UT, sbytes);
+        }
+        if (e != hipSuccess) return abi_fail(WENOX_E_DEVICE, std::string("state copy: ") + hipGetErrorString(e));
+        return WENOX_OK;
+    };
+    std::vector<StepCtl> act = h;
+    for (auto& c : act) {
+        c.active = 1;
+        c.max_steps = c.step + steps + 1;
+        c.t_final = -1.0;
+    }
+    const bool multi = persistent(b);
+    auto go = [&](int k) {
+        int s = 0;
+        for (; multi && s + PERSIST_STEPS <= k; s += PERSIST_STEPS) persist_steps(b, PERSIST_STEPS);
+        for (; s < k; ++s) batch_step(b);
+    };
+    float ms = 0.f;
+    rc = copy_state(false);
+    if (!rc) rc = write_ctls(b, act);
+    if (!rc) {
+        go(1);
+        hipError_t e = hipEventRecord(b->ev0, b->stream);
+        go(steps);
+        if (e == hipSuccess) e = hipEventRecord(b->ev1, b->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, b->ev0, b->ev1);
+        if (e != hipSuccess) rc = abi_fail(WENOX_E_DEVICE, std::string("profile_steps: ") + hipGetErrorString(e));
+    }
+    if (!rc && multi) rc = persist_error(b);
+    const int rc2 = copy_state(true);
+    (void)hipStreamSynchronize(b->stream);
+    (void)hipFree(save);
+    if (rc) return rc;
+    if (rc2) return rc2;
+    *ms_per_step = ms / steps;
+    return write_ctls(b, h);
 }

@@ -286,11 +286,15 @@ __device__ __forceinline__ void apply_pending(StepCtl* ctl)
     }
 }
 
-__device__ __forceinline__ void cfl_finish(StepCtl* ctl, double cmax_state)
+// reset_bits: clear the atomics' words (cmax_bits, blocks_done) with plain
+// stores (callers that update them through atomics only clear them their way)
+__device__ __forceinline__ void cfl_finish(StepCtl* ctl, double cmax_state, bool reset_bits = true)
 {
     apply_pending(ctl);
-    ctl->cmax_bits = 0ull;
-    ctl->blocks_done = 0u;
+    if (reset_bits) {
+        ctl->cmax_bits = 0ull;
+        ctl->blocks_done = 0u;
+    }
     double T = ctl->T;
     bool run = ctl->step < ctl->max_steps;
     if (ctl->t_final >= 0.0) run = run && (T < ctl->t_final - 1e-12);

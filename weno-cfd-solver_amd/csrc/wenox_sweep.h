@@ -153,6 +153,161 @@ struct BatchCommon {
     int cons, force_hlle;
 };
 
+constexpr int GHOST_TB = 256;
+
+// ---- persistent batched step (blocked FP64 fast step) -------------------------
+// One launch advances every instance `steps` steps.  The instances are split
+// over PERSIST_QUEUES work queues, one per XCD (each with its own L2): a
+// workgroup (one wavefront) takes work items, in ticket order, from the queue
+// of the XCD it runs on and runs each to its end, so every item of an
+// instance runs on one XCD and the items hand data to each other through
+// that XCD's L2.  An item first waits, on a counter of its instance, for the
+// items whose results it reads.  The items of step k of a queue, in ticket
+// order: the CFL blocks of its instances (n_f), then for each stage s = 1..3
+// their x-sweep blocks (n_x), y-sweep blocks (n_y) and ghost-fill blocks
+// (n_g).  Waits, all within one instance:
+//   CFL block, step k > 0   on the previous step's end    (PC_STEP >= k)
+//   x sweep, stage 1        on the step's dt              (PC_CFL >= k + 1)
+//   x sweep, stage s > 1    on the ghost fill of stage s - 1
+//   y sweep                 on the x sweep of its stage
+//   ghost fill              on the y sweep of its stage
+// (counts of done items of the instance, summed over the launch's steps).
+// The last CFL block of a step forms dt, the last stage-3 ghost-fill block
+// advances T and the step count.  Every item waits only for items of its
+// queue with smaller tickets, so the oldest unfinished item of a queue can
+// always run; persist_item is shared with a host walk of every queue
+// (build_tables) that checks this for the instance set.
+enum PersistCounter : int {
+    PC_FARR = 0,    // CFL blocks arrived (the last of a step forms dt)
+    PC_CFL = 1,     // dt formed (one per step)
+    PC_SX = 2,      // + 3 (s - 1): x-sweep blocks of stage s done
+    PC_SY = 3,      // + 3 (s - 1): y-sweep blocks of stage s done
+    PC_SG = 4,      // + 3 (s - 1): ghost-fill blocks of stage s done (stage 3: arrived)
+    PC_STEP = 11,   // step ended (T, step advanced)
+    PC_N = 12       // counters per instance
+};
+enum PersistKind : int { PK_CFL = 0, PK_X = 1, PK_Y = 2, PK_G = 3 };
+constexpr int PERSIST_STEPS = 8;   // steps per launch of the captured multi-step graph
+// Spin limit of a dependency wait: about a second, far beyond any wait the
+// item order allows (the items an item waits for have smaller tickets and are
+// already running); passing it sets the error flag, which also ends every
+// other wait, so the launch always drains.
+constexpr int PERSIST_SPIN_LIMIT = 1 << 22;
+
+struct PersistItem {
+    int kind, step, stage, inst, local;
+    int wait_ctr, wait_target;   // wait_ctr < 0: no wait
+    int sig_ctr;                 // counter the finished item increments
+    int last_count;              // > 0: the increment that reaches it finishes the step's
+    int done_ctr;                //   CFL (dt) or the step (advance), then increments done_ctr
+};
+
+__host__ __device__ inline int batch_items_x(const BatchInst& I)
+{
+    return I.gx_x * (int)((I.nx + I.chunk_x - 1) / I.chunk_x);
+}
+__host__ __device__ inline int batch_items_y(const BatchInst& I)
+{
+    return I.gx_y * (int)((I.ny + I.chunk_y - 1) / I.chunk_y);
+}
+__host__ __device__ inline int batch_items_g(const BatchInst& I)
+{
+    return (int)((6 * (I.ny + 6) + 6 * I.nx + GHOST_TB - 1) / GHOST_TB);
+}
+// One work queue: the global indices (into blk_f, blk_x, blk_y, blk_g) of its
+// items of one step, items[off ..]: n_f CFL blocks, n_x x-sweep blocks, n_y
+// y-sweep blocks, n_g ghost-fill blocks (the sweep and ghost lists serve each
+// of the three stages).
+struct PersistQueue {
+    int n_f, n_x, n_y, n_g, off;
+};
+constexpr int PERSIST_QUEUES = 8;   // XCDs of the MI355X
+
+__host__ __device__ inline int persist_items_per_step(const PersistQueue& Q)
+{
+    return Q.n_f + 3 * (Q.n_x + Q.n_y + Q.n_g);
+}
+
+// Work item of ticket t of queue Q (0 <= t < steps * persist_items_per_step(Q)).
+__host__ __device__ inline PersistItem persist_item(int t, const PersistQueue& Q, const int* items,
+                                                    const BatchTables& B)
+{
+    PersistItem it{};
+    const int* const L = items + Q.off;
+    const int per_stage = Q.n_x + Q.n_y + Q.n_g;
+    const int per_step = Q.n_f + 3 * per_stage;
+    it.step = t / per_step;
+    int u = t - it.step * per_step;
+    const int k1 = it.step + 1;
+    it.wait_ctr = -1;
+    it.wait_target = 0;
+    it.last_count = 0;
+    it.done_ctr = -1;
+    if (u < Q.n_f) {
+        const int g = L[u];
+        it.kind = PK_CFL;
+        it.stage = 0;
+        it.inst = B.blk_f[g];
+        const BatchInst& I = B.inst[it.inst];
+        it.local = g - I.cfl_blk0;
+        if (it.step > 0) {
+            it.wait_ctr = PC_STEP;
+            it.wait_target = it.step;
+        }
+        it.sig_ctr = PC_FARR;
+        it.last_count = k1 * I.cfl_nblk;
+        it.done_ctr = PC_CFL;
+        return it;
+    }
+    u -= Q.n_f;
+    const int s = u / per_stage;   // stage - 1
+    u -= s * per_stage;
+    it.stage = s + 1;
+    if (u < Q.n_x) {
+        const int g = L[Q.n_f + u];
+        it.kind = PK_X;
+        it.inst = B.blk_x[g];
+        const BatchInst& I = B.inst[it.inst];
+        it.local = g - I.blk0_x;
+        if (s == 0) {
+            it.wait_ctr = PC_CFL;
+            it.wait_target = k1;
+        } else {
+            it.wait_ctr = PC_SG + 3 * (s - 1);
+            it.wait_target = k1 * batch_items_g(I);
+        }
+        it.sig_ctr = PC_SX + 3 * s;
+    } else if (u < Q.n_x + Q.n_y) {
+        const int g = L[Q.n_f + u];
+        it.kind = PK_Y;
+        it.inst = B.blk_y[g];
+        const BatchInst& I = B.inst[it.inst];
+        it.local = g - I.blk0_y;
+        it.wait_ctr = PC_SX + 3 * s;
+        it.wait_target = k1 * batch_items_x(I);
+        it.sig_ctr = PC_SY + 3 * s;
+    } else {
+        const int g = L[Q.n_f + u];
+        it.kind = PK_G;
+        it.inst = B.blk_g[g];
+        const BatchInst& I = B.inst[it.inst];
+        it.local = g - I.ghost_blk0;
+        it.wait_ctr = PC_SY + 3 * s;
+        it.wait_target = k1 * batch_items_y(I);
+        it.sig_ctr = PC_SG + 3 * s;
+        if (s == 2) {
+            it.last_count = k1 * batch_items_g(I);
+            it.done_ctr = PC_STEP;
+        }
+    }
+    return it;
+}
+
+// Device words after the n_inst * PC_N counters: the queues' tickets, the
+// count of exited workgroups, the error flag (1: a wait passed its spin
+// limit; 2: a queue was left with items, no workgroup running on its XCD).
+constexpr int PW_TICKET = 0, PW_EXIT = PERSIST_QUEUES, PW_ERR = PERSIST_QUEUES + 1, PW_N = PERSIST_QUEUES + 2;
+
 struct BatchOps {
     // one RK stage of every instance (sweeps, FP64 fix-up when split, combine)
     void (*stage)(const BatchTables& t, const BatchCommon& c, int split, const void* Us,
@@ -172,6 +327,18 @@ struct BatchOps {
                       double* out, int stage, StepCtl* ctls, hipStream_t s);
     void (*bc_blk)(const BatchTables& t, BcParams bp, double* U, StepCtl* ctls, int stage, hipStream_t s);
     void (*cfl_blk)(const BatchTables& t, const double* U, double gam, StepCtl* ctls, hipStream_t s);
+    // `steps` whole steps of every instance in one persistent launch (see
+    // persist_item) on the blocked states Ub (U^n), Ab, Bb (stage states) and
+    // Eb, on `grid` workgroups; queues[PERSIST_QUEUES] with their item lists
+    // in items (device); cnt holds n_inst * PC_N counters and the PW_N words
+    // (tickets, exit count, error flag), all but the error flag cleared by a
+    // fill kernel before the launch.
+    void (*persist)(const BatchTables& t, const BatchCommon& c, BcParams bp, double* Ub, double* Ab, double* Bb,
+                    double* Eb, StepCtl* ctls, const PersistQueue* queues, const int* items, int* cnt, int steps,
+                    int grid, int spin_limit, hipStream_t s);
+    // resident workgroups of the persistent kernel on the current device
+    // (occupancy x compute units), for c's scheme
+    int (*persist_grid)(const BatchCommon& c);
 };
 
 extern const BatchOps batch_ops_exact;
@@ -188,7 +355,6 @@ inline int64_t blk_part_slots(const SweepPlan& p)
 }
 
 constexpr int COMBINE_TILE = 32;
-constexpr int GHOST_TB = 256;
 
 // The RK combine writes the ghost cells of the state it forms, in place of a
 // ghost-fill launch, when every ghost cell's source lies in the combine tile

@@ -317,9 +317,12 @@ class Solver:
 
 class Batch:
     """Many independent problems of uneven sizes advanced together on the
-    device, each with its own dx, time and CFL dt; every phase of a step is
-    one launch over all instances (the x and y sweeps of a stage share one)
-    and a step is one hipGraph replay."""
+    device, each with its own dx, time and CFL dt.  The fast FP64 batch can
+    run persistently (set_persistent): one launch per step, or per eight
+    steps, whose workgroups take the instances' work items from a device
+    queue, each waiting only for its own instance's earlier phases.
+    Otherwise every phase of a step is one launch over all instances.  A step
+    (or eight persistent steps) is one hipGraph replay."""
 
     def __init__(self, sizes, dx, gamma=1.4, cfl=0.4, cons_switch=False, force_hlle=False,
                  jp_cri=(0.2, 1.0), bc="mirror", variant="fast", precision="fp64", chunk=16,
@@ -409,6 +412,29 @@ class Batch:
     def set_blocked(self, on: bool = True):
         _lib.check(_lib.load().wenox_batch_set_layout(self._h, int(bool(on))))
 
+    @property
+    def persistent(self) -> bool:
+        """Persistent step of the blocked fast FP64 batch; the same states bit
+        for bit as the launch-per-phase step."""
+        v = ctypes.c_int()
+        _lib.check(_lib.load().wenox_batch_persistent(self._h, ctypes.byref(v)))
+        return bool(v.value)
+
+    def set_persistent(self, on: bool = True):
+        _lib.check(_lib.load().wenox_batch_set_persistent(self._h, int(bool(on))))
+
+    def set_spin_limit(self, spins: int):
+        """Polls of a persistent step's dependency wait before it gives up
+        (run() then raises WenoxError); small values force that path."""
+        _lib.check(_lib.load().wenox_batch_set_spin_limit(self._h, int(spins)))
+
+    def profile_steps(self, steps: int = 8) -> float:
+        """Device time (ms) per step of `steps` whole steps as run() launches
+        them; the states and step controls are restored afterwards."""
+        v = ctypes.c_double()
+        _lib.check(_lib.load().wenox_batch_profile_steps(self._h, int(steps), ctypes.byref(v)))
+        return v.value
+
     def profile(self, reps: int = 5) -> float:
         """Device time (ms) of one RK stage of every instance as a step
         launches it; the state is left unchanged."""
@@ -418,21 +444,53 @@ class Batch:
 
     CHUNKS = (8, 12, 16, 24, 32, 48, 64, 96)
 
-    def autotune(self, candidates=CHUNKS, reps: int = 3) -> dict:
+    def autotune(self, candidates=CHUNKS, reps: int = 3, steps: int = 8) -> dict:
         """Pick the rows marched per sweep workgroup (one value for the x and
         y sweeps of every instance) with the lowest measured stage time and
         keep it; the current chunks stay unless another beats them by 3 %.
-        Returns {chunk: stage ms, "best": chunk}."""
-        c0 = self.chunks
+        Where the persistent step applies (blocked fast FP64), each chunk is
+        also timed as whole steps of both step structures (profile_steps) and
+        the fastest (structure, chunk) pair is kept on the same 3 % rule.
+        Returns {chunk: stage ms, "best": chunk} and, with the persistent
+        step, "step_ms": {"phases" | "persistent": {chunk: ms per step}} and
+        "persistent": the structure kept."""
+        c0, p0 = self.chunks, self.persistent
         times = {}
         for c in candidates:
             self.set_chunks(c, c)
             times[c] = self.profile(reps)
-        self.set_chunks(*c0)
-        t0 = self.profile(reps)
-        best = min(times, key=times.get)
-        if times[best] < 0.97 * t0:
-            self.set_chunks(best, best)
         out = dict(times)
+        modes = [False]
+        if self.blocked:
+            try:
+                self.set_persistent(True)
+                modes.append(True)
+            except WenoxError:
+                pass
+        if len(modes) == 1:
+            self.set_chunks(*c0)
+            t0 = self.profile(reps)
+            best = min(times, key=times.get)
+            if times[best] < 0.97 * t0:
+                self.set_chunks(best, best)
+            out["best"] = self.chunks[0]
+            return out
+        step_ms = {}
+        for m in modes:
+            self.set_persistent(m)
+            step_ms[m] = {}
+            for c in candidates:
+                self.set_chunks(c, c)
+                step_ms[m][c] = self.profile_steps(steps)
+        self.set_persistent(p0)
+        self.set_chunks(*c0)
+        t0 = self.profile_steps(steps)
+        bm, bc = min(((m, c) for m in modes for c in candidates), key=lambda mc: step_ms[mc[0]][mc[1]])
+        if step_ms[bm][bc] < 0.97 * t0:
+            self.set_persistent(bm)
+            self.set_chunks(bc, bc)
         out["best"] = self.chunks[0]
+        out["persistent"] = self.persistent
+        out["step_ms"] = {("persistent" if m else "phases"): {c: round(v, 5) for c, v in d.items()}
+                          for m, d in step_ms.items()}
         return out

@@ -119,6 +119,10 @@ def load():
         "wenox_batch_profile": ([dp, i, dp], i),
         "wenox_batch_layout": ([dp, ctypes.POINTER(i)], i),
         "wenox_batch_set_layout": ([dp, i], i),
+        "wenox_batch_persistent": ([dp, ctypes.POINTER(i)], i),
+        "wenox_batch_set_persistent": ([dp, i], i),
+        "wenox_batch_set_spin_limit": ([dp, i], i),
+        "wenox_batch_profile_steps": ([dp, i, dp], i),
         "wenox_batch_status": ([dp, dp, dp], i),
         "wenox_batch_last_ms_per_step": ([dp], d),
     }
