@@ -143,7 +143,7 @@ def run_gpu(w, q, dx, steps, warmup, engine, variant, precision, profile=True, t
 CONFIG_RANGES = dict(rho=(0.1, 2.0), p=(0.02, 2.0), vel=1.5)
 
 
-def run_batch(w, steps, warmup, precision, chunk=16, ranges=None):
+def run_batch(w, steps, warmup, precision, chunk=16, ranges=None, tune=True):
     """The batched engine (per-instance dt), with the step structure and the
     sweep chunk its autotuner picks: the persistent step (one launch per
     eight steps, work items from a device queue) or one launch per phase over
@@ -160,7 +160,11 @@ def run_batch(w, steps, warmup, precision, chunk=16, ranges=None):
                jp_cri=w["jp"], bc=w["bc"], variant="fast", precision=precision, chunk=chunk) as b:
         for i, (q, _) in enumerate(inst):
             b.upload(i, q)
-        tuned = b.autotune()
+        if tune:
+            tuned = b.autotune()
+        else:
+            # the given chunk and the engine's default step structure
+            tuned = {"best": b.chunks[0], "persistent": b.persistent}
         b.run(nstep=warmup)
         t0 = time.perf_counter()
         b.run(nstep=steps)
@@ -516,12 +520,40 @@ def workload_block(key, steps, warmup):
     return out
 
 
-def batch_block(w, steps, warmup):
-    r = run_batch(w, steps, warmup, "fp64")
-    r32 = run_batch(w, steps, warmup, "fp32split")
+# HBM bytes and VALU wave-instructions per step of the persistent batched
+# step from its counter profile (FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB;
+# SQ_INSTS_VALU per launch / 8 steps), keyed by step structure and chunk
+_BATCH_SRC = "profiles/batch512_fp64_persistent.md"
+BATCH_TRAFFIC = {
+    ("persistent", 48): {"bytes_per_step": 18.7226e9, "wave_instr_per_step": 4.212e9, "source": _BATCH_SRC},
+}
+
+
+def batch_roofline(r):
+    """The batch's HBM roofline at 256 algorithmic B per cell-update over
+    the wall-clock rate; traffic and VALU issue where the counter profile was
+    made on the same step structure and chunk."""
+    out = {"bound": "hbm", "achieved": r["value"] * BYTES_PER_CELL_UPDATE / 1e9, "peak": HBM_PEAK / 1e9,
+           "unit": "GB/s", "frac": r["value"] * BYTES_PER_CELL_UPDATE / HBM_PEAK, "traffic": None,
+           "algorithmic_bytes_per_step": r["cells"] * BYTES_PER_CELL_UPDATE}
+    tr = BATCH_TRAFFIC.get(("persistent" if r["step_structure"] == "persistent" else "phases", r["chunk"]))
+    if tr:
+        rate = tr["wave_instr_per_step"] / (r["ms_per_step"] * 1e-3)
+        out.update(traffic=tr["bytes_per_step"], traffic_source=tr["source"],
+                   valu={"wave_instr_per_step": tr["wave_instr_per_step"], "achieved": rate,
+                         "peak": VALU_WAVE_INSTR_PEAK, "unit": "wave64 VALU instr/s",
+                         "frac": rate / VALU_WAVE_INSTR_PEAK})
+    return out
+
+
+def batch_block(w, steps, warmup, chunk=None):
+    """chunk: fixed rows per sweep workgroup (no autotune), for counter runs."""
+    kw = dict(chunk=chunk, tune=False) if chunk else {}
+    r = run_batch(w, steps, warmup, "fp64", **kw)
+    r32 = run_batch(w, steps, warmup, "fp32split", **kw)
     # the configured (wider) state ranges, for the instability evidence: the
     # reference scheme turns most of those instances non-finite
-    rc = run_batch(w, steps, warmup, "fp64", ranges=CONFIG_RANGES)
+    rc = run_batch(w, steps, warmup, "fp64", ranges=CONFIG_RANGES, **kw)
     return {"workload": describe(w), "cells": r["cells"], "steps": steps, "warmup": warmup,
             "value_fp64": r["value"], "ms_per_step_fp64": r["ms_per_step"], "chunk": r["chunk"],
             "step_structure": r["step_structure"], "autotune_step_ms": r["autotune_step_ms"],
@@ -529,9 +561,7 @@ def batch_block(w, steps, warmup):
             # the reference scheme itself turns some instances non-finite (the oracle
             # reproduces it); they keep stepping, so the count does not change the cost
             "instances_non_finite": r["non_finite"],
-            "roofline_fp64": {"bound": "hbm", "achieved": r["value"] * BYTES_PER_CELL_UPDATE / 1e9,
-                              "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                              "frac": r["value"] * BYTES_PER_CELL_UPDATE / HBM_PEAK, "traffic": None},
+            "roofline_fp64": batch_roofline(r),
             "state_ranges": "rho~U[1,2], p~U[1,2], u,v~U[-0.5,0.5] per quadrant",
             "configured_ranges_fp64": {"state_ranges": "rho~U[0.1,2], p~U[0.02,2], u,v~U[-1.5,1.5]",
                                        "value": rc["value"], "instances_non_finite": rc["non_finite"]},
@@ -552,7 +582,7 @@ def main():
                     help="precision of the main timed run (the headline value is FP64)")
     ap.add_argument("--chunks", type=int, nargs=4, metavar=("CX", "CY", "WX", "WY"),
                     help="sweep tiling: rows per wavefront and wavefronts per workgroup of the x / y "
-                         "sweeps (skips the autotuner)")
+                         "sweeps (skips the autotuner; batch512: CX rows per sweep workgroup)")
     ap.add_argument("--layout", default="auto", choices=["auto", "row", "blocked"],
                     help="state layout of the FP64 fast step (auto: the autotuner's choice; with --chunks: row)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -580,7 +610,8 @@ def main():
         print(json.dumps({"tuning_objective": obj}), flush=True)
         return
     if args.workload == "batch512":
-        b = batch_block(w, args.steps, args.warmup)
+        # --chunks C ...: the batch's sweep chunk fixed at C (no autotune)
+        b = batch_block(w, args.steps, args.warmup, chunk=args.chunks[0] if args.chunks else None)
         res = {"metric": METRIC, "value": b["value_fp64"], "unit": "cell-updates/s", "n_gpus": 1,
                "steps": args.steps, "warmup": args.warmup, "ms_per_step": b["ms_per_step_fp64"],
                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",

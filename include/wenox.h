@@ -237,11 +237,14 @@ int wenox_solver_set_layout(wenox_solver *s, int blocked);
 
 /* ---- 3. batched solver: many independent problems of uneven sizes ---------- */
 /* The n instances share the numerics of desc (nx, ny, dx of desc are ignored;
- * bc must be MIRROR or PERIODIC) and each has its own grid, dx, time and CFL
- * dt.  Every phase of a step (CFL, x and y sweeps, RK combine, ghost fill) is
- * one launch over all instances (both sweeps of a stage share one); a step is
- * one hipGraph replay.  chunk: rows marched per sweep workgroup (<= 0: 16).  No counterpart in the reference,
- * which runs one problem per process; this serves its batched workload. */
+ * bc must be MIRROR, PERIODIC or COPY) and each has its own grid, dx, time and
+ * CFL dt.  Step structures: every phase of a step (CFL, x and y sweeps, RK
+ * combine or stage finish, ghost fill) one launch over all instances, a step
+ * one hipGraph replay; or, for the blocked fast FP64 batch, the persistent
+ * step (wenox_batch_set_persistent), one launch per step or per eight steps.
+ * chunk: rows marched per sweep workgroup (<= 0: 16).  No counterpart in the
+ * reference, which runs one problem per process; this serves its batched
+ * workload. */
 typedef struct wenox_batch wenox_batch;
 int wenox_batch_create(const wenox_solver_desc *desc, int n, const int64_t *nx, const int64_t *ny,
                        const double *dx, int chunk, wenox_batch **out);
@@ -267,7 +270,8 @@ int wenox_batch_profile(wenox_batch *b, int reps, double *stage_ms);
  * No reference counterpart. */
 int wenox_batch_layout(wenox_batch *b, int *blocked);
 int wenox_batch_set_layout(wenox_batch *b, int blocked);
-/* Persistent step of the blocked fast FP64 batch (on = 1): one launch advances
+/* Persistent step of the blocked fast FP64 batch (on = 1, its default; the
+ * getter reports the structure in effect, 0 for the row-major batch): one launch advances
  * every instance one step, or eight in the graph that wenox_batch_run replays
  * while at least eight steps remain.  Its workgroups take work items (CFL,
  * x-sweep, y-sweep and ghost-fill blocks of each instance and stage) in

@@ -76,3 +76,21 @@ def test_profile_commands_name_their_workload():
             assert cmds and all(flag in ln for ln in cmds), (name, prec)
             assert all(("--precision fp32split" in ln) == (prec == "fp32split") for ln in cmds), (name, prec)
             assert all(("--layout blocked" in ln) == prec.endswith("blocked") for ln in cmds), (name, prec)
+
+
+def test_batch_table_matches_profile(bench):
+    """The persistent batch's traffic and VALU per step equal its summary's."""
+    assert bench.BATCH_TRAFFIC
+    for (structure, chunk), entry in bench.BATCH_TRAFFIC.items():
+        text = _read(entry["source"])
+        m = re.search(r"HBM traffic = \*\*([0-9.]+) MB\*\*", text)
+        assert m, entry["source"]
+        assert abs(float(m.group(1)) * 1e6 - entry["bytes_per_step"]) <= 1e5, (structure, chunk)
+        m = re.search(r"VALU: ([0-9.e+]+) wave instructions per step", text)
+        assert m and abs(float(m.group(1)) - entry["wave_instr_per_step"]) <= 1e-3 * entry["wave_instr_per_step"]
+        assert f"--chunks {chunk} {chunk} 1 1" in text and "--workload batch512" in text
+        # the numbers come from the persistent step's launches
+        assert (structure == "persistent") == ("k_batch_persist" in text)
+        r = {"value": 2.4e9, "cells": 24326144, "ms_per_step": 10.0, "chunk": chunk,
+             "step_structure": "persistent" if structure == "persistent" else "launch per phase"}
+        assert bench.batch_roofline(r)["traffic"] == entry["bytes_per_step"]

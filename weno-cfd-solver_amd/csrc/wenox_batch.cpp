@@ -419,6 +419,7 @@ extern "C" int wenox_batch_create(const wenox_solver_desc* desc, int n, const in
              hipMemset(b->pcnt, 0, nc * sizeof(int)) == hipSuccess;
         b->pgrid = ok ? b->bops->persist_grid(BatchCommon{d.gamma, d.jp_lo, d.jp_hi, d.cons_switch, d.force_hlle}) : 0;
         ok = ok && b->pgrid > 0;
+        b->persist = true;   // the default step structure of the blocked fast FP64 batch
     }
     if (!ok) return cleanup(abi_fail(WENOX_E_DEVICE, "batch allocation failed (" + std::to_string((6 * soff + 2 * ioff + (b->blk ? 4 * boff : 0)) * 8 >> 20) + " MiB)"));
     int rct = build_tables(b, chunk, chunk);
@@ -695,7 +696,7 @@ extern "C" int wenox_batch_set_layout(wenox_batch* b, int blocked)
 extern "C" int wenox_batch_persistent(wenox_batch* b, int* on)
 {
     if (!b || !on) return abi_fail(WENOX_E_ARG, "wenox_batch_persistent: null argument");
-    *on = b->persist ? 1 : 0;
+    *on = persistent(b) ? 1 : 0;
     return WENOX_OK;
 }
 

@@ -414,8 +414,8 @@ class Batch:
 
     @property
     def persistent(self) -> bool:
-        """Persistent step of the blocked fast FP64 batch; the same states bit
-        for bit as the launch-per-phase step."""
+        """Persistent step in effect (the default of the blocked fast FP64
+        batch); the same states bit for bit as the launch-per-phase step."""
         v = ctypes.c_int()
         _lib.check(_lib.load().wenox_batch_persistent(self._h, ctypes.byref(v)))
         return bool(v.value)
