@@ -137,3 +137,23 @@ def test_solver_argument_checks_before_device_use():
     assert L.wenox_l_local_noalloc(None, 0.1, 1.4, 0, 0.5, 2.0, 0, None, None) == _lib.E_ARG
     v = _lib.View3(8, (ctypes.c_int64 * 3)(5, 10, 4), (ctypes.c_int64 * 3)(40, 4, 1))
     assert L.wenox_l_local_noalloc(ctypes.byref(v), 0.1, 1.4, 1, 0.5, 2.0, 0, None, None) == _lib.E_SHAPE
+
+
+def test_type_stubs_list_the_module_surface():
+    """weno_ext/__init__.pyi declares every exported function with the
+    module's own parameter names, order and defaults."""
+    import ast
+    import inspect
+
+    import weno_ext
+    path = os.path.join(os.path.dirname(weno_ext.__file__), "__init__.pyi")
+    tree = ast.parse(open(path).read())
+    stubs = {n.name: n for n in tree.body if isinstance(n, ast.FunctionDef)}
+    names = [n for n in weno_ext.__all__ if n != "PanicException"] + ["set_variant"]
+    assert set(names) <= set(stubs)
+    for name in names:
+        params = list(inspect.signature(getattr(weno_ext, name)).parameters.values())
+        args = stubs[name].args
+        assert [a.arg for a in args.args] == [p.name for p in params], name
+        defaults = [p.name for p in params if p.default is not inspect.Parameter.empty]
+        assert len(args.defaults) == len(defaults), name
