@@ -280,3 +280,26 @@ def test_batch_profile_steps_restores_state(persistent):
         T, k = b.status()
         got = [b.download(i) for i in range(len(SIZES))]
     _assert_same({"plain": ref[("phases", True)], "tuned": (T, k, got)})
+
+
+def test_batch512_persistent_full_size():
+    """The configured batched workload at its full size -- 512 instances of
+    sizes drawn from {64, 96, 128, 192, 256, 384}^2, the bench's seed and
+    state ranges -- advanced 16 steps by the persistent step (two eight-step
+    launches) and by the launch-per-phase step: every instance's state, time
+    and step count agree bit for bit, and each instance advanced by its own
+    dt."""
+    sizes_all = np.array([64, 96, 128, 192, 256, 384])
+    from wenox import problems
+    rng = np.random.default_rng(20261203)
+    sizes = [(int(n), int(n)) for n in rng.choice(sizes_all, size=512)]
+    inst = [problems.random_riemann_instance(rng, nx, ny) for nx, ny in sizes]
+    out = _run_modes(sizes, inst, (("phases", True), ("persistent", True)), lambda b: b.run(nstep=16),
+                     cfl=0.4, jp_cri=(1e4, 1.1e4), chunk=48)
+    T, k, q = out[("phases", True)]
+    assert np.all(k == 16) and len(set(np.round(T, 12).tolist())) > 100
+    T1, k1, q1 = out[("persistent", True)]
+    np.testing.assert_array_equal(k1, k)
+    np.testing.assert_array_equal(T1, T)
+    for i in range(len(sizes)):
+        np.testing.assert_array_equal(q1[i], q[i], err_msg=f"instance {i} {sizes[i]}")
