@@ -180,10 +180,11 @@ def _run_modes(sizes, inst, modes, script, **kw):
     return out
 
 
-def _assert_same(out):
+def _assert_same(out, finite=True):
     keys = list(out)
     T0, k0, q0 = out[keys[0]]
-    assert all(np.isfinite(q[3:-3, 3:-3, :4]).all() for q in q0)
+    if finite:
+        assert all(np.isfinite(q[3:-3, 3:-3, :4]).all() for q in q0)
     for key in keys[1:]:
         T1, k1, q1 = out[key]
         np.testing.assert_array_equal(k1, k0, err_msg=str(key))
@@ -303,3 +304,19 @@ def test_batch512_persistent_full_size():
     np.testing.assert_array_equal(T1, T)
     for i in range(len(sizes)):
         np.testing.assert_array_equal(q1[i], q[i], err_msg=f"instance {i} {sizes[i]}")
+
+
+@pytest.mark.parametrize("bc", ["mirror", "periodic"])
+def test_batch_persistent_tiny_and_lone_instances(bc):
+    """Instances down to the smallest grid (3 x 3) beside a large one, and a
+    batch of a single instance (one work queue busy, the other seven empty):
+    the persistent step equals the launch-per-phase step bit for bit (the
+    scheme itself may turn the tiniest grids non-finite: NaN compares equal
+    to NaN here)."""
+    from wenox import problems
+    rng = np.random.default_rng(41)
+    for sizes in ([(3, 3), (5, 7), (130, 6), (4, 64), (70, 70)], [(37, 29)]):
+        inst = [problems.random_riemann_instance(rng, nx, ny) for nx, ny in sizes]
+        out = _run_modes(sizes, inst, (("phases", True), ("persistent", True)), lambda b: b.run(nstep=11),
+                         cfl=0.4, jp_cri=(0.5, 2.0), bc=bc)
+        _assert_same(out, finite=False)
