@@ -12,6 +12,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/wenox.h"
@@ -98,22 +99,44 @@ int dropin_reserve(size_t dev_doubles, size_t pin_doubles)
     return WENOX_OK;
 }
 
+// fn(i0, i1) over row ranges [i0, i1) of [0, M), on several host threads when
+// the rows hold at least `work` doubles in all (the drop-in calls' host-side
+// packing and copies are memory-bound; one thread moves a few GB/s).
+template <class Fn> void par_rows(int64_t M, int64_t work, Fn fn)
+{
+    constexpr int64_t PAR_MIN = (int64_t)1 << 20;   // doubles
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int64_t T = std::min<int64_t>({(int64_t)(hw ? hw : 1), 16, M, std::max<int64_t>(work / PAR_MIN, 1)});
+    if (T <= 1) {
+        fn((int64_t)0, M);
+        return;
+    }
+    std::vector<std::thread> pool;
+    pool.reserve((size_t)T - 1);
+    for (int64_t t = 1; t < T; ++t) pool.emplace_back(fn, t * M / T, (t + 1) * M / T);
+    fn((int64_t)0, M / T);
+    for (auto& th : pool) th.join();
+}
+
 // Packs channels 0..3 of rows [0, M) x cols [0, N) of a strided view into an
 // AoS4 (M, N, 4) contiguous buffer.
 void pack4(const wenox_view3* v, int64_t M, int64_t N, double* dst)
 {
     const int64_t s0 = v->stride[0], s1 = v->stride[1], s2 = v->stride[2];
     const double* p = v->data;
-    if (s2 == 1 && s1 == 4 && s0 == 4 * N) {
-        std::memcpy(dst, p, sizeof(double) * (size_t)(M * N * 4));
-        return;
-    }
-    for (int64_t i = 0; i < M; ++i)
-        for (int64_t j = 0; j < N; ++j) {
-            const double* s = p + i * s0 + j * s1;
-            double* d = dst + (i * N + j) * 4;
-            d[0] = s[0]; d[1] = s[s2]; d[2] = s[2 * s2]; d[3] = s[3 * s2];
+    const bool dense = s2 == 1 && s1 == 4 && s0 == 4 * N;
+    par_rows(M, M * N * 4, [=](int64_t i0, int64_t i1) {
+        if (dense) {
+            std::memcpy(dst + i0 * N * 4, p + i0 * N * 4, sizeof(double) * (size_t)((i1 - i0) * N * 4));
+            return;
         }
+        for (int64_t i = i0; i < i1; ++i)
+            for (int64_t j = 0; j < N; ++j) {
+                const double* s = p + i * s0 + j * s1;
+                double* d = dst + (i * N + j) * 4;
+                d[0] = s[0]; d[1] = s[s2]; d[2] = s[2 * s2]; d[3] = s[3 * s2];
+            }
+    });
 }
 
 bool bad_view(const wenox_view3* v)
@@ -135,10 +158,24 @@ std::string dims(const wenox_view3* v)
     return b;
 }
 
+// The device result into the caller's (pageable) host array: large results
+// through the pinned staging buffer and a copy on several host threads, small
+// ones directly.
 int sync_copy_out(double* host, const double* dev, size_t n)
 {
     if (n == 0) return WENOX_OK;
-    HIPCHK(hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, g_drop.stream));
+    if (n < ((size_t)1 << 20) || n > g_drop.pin_cap) {
+        HIPCHK(hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, g_drop.stream));
+        return WENOX_OK;
+    }
+    double* pin = g_drop.pin;
+    HIPCHK(hipMemcpyAsync(pin, dev, n * sizeof(double), hipMemcpyDeviceToHost, g_drop.stream));
+    HIPCHK(hipStreamSynchronize(g_drop.stream));
+    const int64_t rows = (int64_t)((n + 4095) / 4096);
+    par_rows(rows, (int64_t)n, [=](int64_t r0, int64_t r1) {
+        const size_t b = (size_t)r0 * 4096, e = std::min(n, (size_t)r1 * 4096);
+        if (e > b) std::memcpy(host + b, pin + b, (e - b) * sizeof(double));
+    });
     return WENOX_OK;
 }
 
