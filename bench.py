@@ -522,27 +522,29 @@ def workload_block(key, steps, warmup):
 
 # HBM bytes and VALU wave-instructions per step of the persistent batched
 # step from its counter profile (FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB;
-# SQ_INSTS_VALU per launch / 8 steps), keyed by step structure and chunk
+# SQ_INSTS_VALU per launch / 8 steps), keyed by step structure, with the
+# chunk the profile was made on (another chunk changes only the sweeps'
+# halo re-reads)
 _BATCH_SRC = "profiles/batch512_fp64_persistent.md"
 BATCH_TRAFFIC = {
-    ("persistent", 48): {"bytes_per_step": 18.7226e9, "wave_instr_per_step": 4.212e9, "source": _BATCH_SRC},
+    "persistent": {"bytes_per_step": 18.7226e9, "wave_instr_per_step": 4.212e9, "chunk": 48, "source": _BATCH_SRC},
 }
 
 
 def batch_roofline(r):
     """The batch's HBM roofline at 256 algorithmic B per cell-update over
-    the wall-clock rate; traffic and VALU issue where the counter profile was
-    made on the same step structure and chunk."""
+    the wall-clock rate; traffic and VALU issue from the counter profile of
+    the same step structure (made at the chunk it names)."""
     out = {"bound": "hbm", "achieved": r["value"] * BYTES_PER_CELL_UPDATE / 1e9, "peak": HBM_PEAK / 1e9,
            "unit": "GB/s", "frac": r["value"] * BYTES_PER_CELL_UPDATE / HBM_PEAK, "traffic": None,
            "algorithmic_bytes_per_step": r["cells"] * BYTES_PER_CELL_UPDATE}
-    tr = BATCH_TRAFFIC.get(("persistent" if r["step_structure"] == "persistent" else "phases", r["chunk"]))
+    tr = BATCH_TRAFFIC.get("persistent" if r["step_structure"] == "persistent" else "phases")
     if tr:
         rate = tr["wave_instr_per_step"] / (r["ms_per_step"] * 1e-3)
-        out.update(traffic=tr["bytes_per_step"], traffic_source=tr["source"],
+        out.update(traffic=tr["bytes_per_step"], traffic_source=tr["source"], traffic_chunk=tr["chunk"],
                    valu={"wave_instr_per_step": tr["wave_instr_per_step"], "achieved": rate,
                          "peak": VALU_WAVE_INSTR_PEAK, "unit": "wave64 VALU instr/s",
-                         "frac": rate / VALU_WAVE_INSTR_PEAK})
+                         "frac": rate / VALU_WAVE_INSTR_PEAK, "chunk": tr["chunk"]})
     return out
 
 

@@ -81,7 +81,8 @@ def test_profile_commands_name_their_workload():
 def test_batch_table_matches_profile(bench):
     """The persistent batch's traffic and VALU per step equal its summary's."""
     assert bench.BATCH_TRAFFIC
-    for (structure, chunk), entry in bench.BATCH_TRAFFIC.items():
+    for structure, entry in bench.BATCH_TRAFFIC.items():
+        chunk = entry["chunk"]
         text = _read(entry["source"])
         m = re.search(r"HBM traffic = \*\*([0-9.]+) MB\*\*", text)
         assert m, entry["source"]
@@ -91,6 +92,7 @@ def test_batch_table_matches_profile(bench):
         assert f"--chunks {chunk} {chunk} 1 1" in text and "--workload batch512" in text
         # the numbers come from the persistent step's launches
         assert (structure == "persistent") == ("k_batch_persist" in text)
-        r = {"value": 2.4e9, "cells": 24326144, "ms_per_step": 10.0, "chunk": chunk,
+        r = {"value": 2.4e9, "cells": 24326144, "ms_per_step": 10.0, "chunk": 64,
              "step_structure": "persistent" if structure == "persistent" else "launch per phase"}
-        assert bench.batch_roofline(r)["traffic"] == entry["bytes_per_step"]
+        roof = bench.batch_roofline(r)
+        assert roof["traffic"] == entry["bytes_per_step"] and roof["traffic_chunk"] == chunk
