@@ -67,6 +67,7 @@ struct wenox_batch {
     // persistent blocked step (BatchOps::persist): one launch per step, or per
     // PERSIST_STEPS steps (graph8); pcnt: n * PC_N counters, ticket, error flag
     bool persist = false;
+    bool persist_fits = true;   // the tickets of a PERSIST_STEPS-step launch fit in an int
     int pgrid = 0;
     int pspin = PERSIST_SPIN_LIMIT;
     int* pcnt = nullptr;
@@ -78,7 +79,7 @@ struct wenox_batch {
 
 namespace {
 
-bool persistent(const wenox_batch* b) { return b->blk && b->persist; }
+bool persistent(const wenox_batch* b) { return b->blk && b->persist && b->persist_fits; }
 
 void persist_steps(wenox_batch* b, int steps)
 {
@@ -257,6 +258,15 @@ int build_tables(wenox_batch* b, int chunk_x, int chunk_y)
     b->chunk_y = chunk_y;
     // the persistent step's queues over these tables, their order walked on the host
     if (b->pcnt) {
+        // tickets are ints: a batch whose launch would need more runs phase by phase
+        int64_t most = 0;
+        for (size_t i = 0; i < b->inst.size(); ++i) {
+            const BatchInst& I = b->inst[i];
+            most += I.cfl_nblk + 3 * (int64_t)(batch_items_x(I) + batch_items_y(I) + batch_items_g(I));
+        }
+        b->persist_fits = most * PERSIST_STEPS < ((int64_t)1 << 30);
+    }
+    if (b->pcnt && b->persist_fits) {
         const std::vector<int> items = persist_queues(b->inst, b->pq);
         BatchTables th = b->tab;
         th.inst = b->inst.data();
@@ -705,6 +715,8 @@ extern "C" int wenox_batch_set_persistent(wenox_batch* b, int on)
     if (!b || (on != 0 && on != 1)) return abi_fail(WENOX_E_ARG, "wenox_batch_set_persistent: on is 0 or 1");
     if (on && !b->pcnt)
         return abi_fail(WENOX_E_ARG, "the persistent step serves the batched fast FP64 step (blocked layout)");
+    if (on && !b->persist_fits)
+        return abi_fail(WENOX_E_ARG, "the persistent step: too many work items per launch for this batch and chunk");
     if ((on != 0) == b->persist) return WENOX_OK;
     BCHK(hipSetDevice(b->device));
     BCHK(hipStreamSynchronize(b->stream));
