@@ -274,15 +274,25 @@ int wenox_batch_set_layout(wenox_batch *b, int blocked);
  * getter reports the structure in effect, 0 for the row-major batch): one launch advances
  * every instance one step, or eight in the graph that wenox_batch_run replays
  * while at least eight steps remain.  Its workgroups take work items (CFL,
- * x-sweep, y-sweep and ghost-fill blocks of each instance and stage) in
- * order from a device counter; an item waits only for the earlier items of
- * its own instance whose results it reads, so instances move through the
+ * x-sweep, y-sweep and ghost-fill blocks of each pack of instances and
+ * stage) in order from a device counter; an item waits only for the earlier
+ * items of its own pack whose results it reads, so packs move through the
  * phases of a step independently.  Same states bit for bit as the
  * launch-per-phase step (on = 0).  A wait that passes its spin limit (about a
  * second) makes wenox_batch_run return WENOX_E_DEVICE.  No reference
  * counterpart. */
 int wenox_batch_persistent(wenox_batch *b, int *on);
 int wenox_batch_set_persistent(wenox_batch *b, int on);
+/* Packs of the persistent step: within each work queue, instances of one
+ * shape (nx, ny, dx) whose padded columns its sweeps lay side by side along
+ * a wavefront's 64 lanes, so narrow instances stop leaving lanes idle (a
+ * 64-wide instance alone fills two wavefronts with 64 outputs, five packed
+ * fill six).  Each shape's instances form packs of at most pack_max members
+ * (default 8), the size chosen for the fewest lane-rows; 1 turns packing off.
+ * Same states bit for bit for every pack_max.  n_packs: packs in effect (0
+ * when the persistent step does not run).  No reference counterpart. */
+int wenox_batch_packs(wenox_batch *b, int *pack_max, int *n_packs);
+int wenox_batch_set_pack_max(wenox_batch *b, int pack_max);
 /* Polls of a persistent step's dependency wait before it gives up and sets
  * the error flag (default 4194304, about a second).  A small value forces the
  * error path, for its tests; the launch still drains.  No reference

@@ -320,3 +320,54 @@ def test_batch_persistent_tiny_and_lone_instances(bc):
         out = _run_modes(sizes, inst, (("phases", True), ("persistent", True)), lambda b: b.run(nstep=11),
                          cfl=0.4, jp_cri=(0.5, 2.0), bc=bc)
         _assert_same(out, finite=False)
+
+
+@pytest.mark.parametrize("cons,force_hlle,bc", [(False, False, "mirror"), (True, True, "periodic")])
+def test_batch_packs_same_states(cons, force_hlle, bc):
+    """Instances of one shape packed side by side along the wavefront's lanes
+    (at most 8 or 3 a pack, or none) give every instance the same state, time
+    and step count bit for bit as the launch-per-phase step: single steps,
+    eight-step launches, and a t_final run in which members of one pack stop
+    at different steps (their lanes then write nothing).  24 x 24 instances
+    on another dx form packs of their own; a width of 58 (one wavefront's
+    outputs) stays unpacked where packing would add lanes."""
+    from wenox import Batch, problems
+    rng = np.random.default_rng(43)
+    # the work queues (one per XCD) split every shape's instances about
+    # evenly, so each shape comes in multiples of the eight queues (five
+    # 64 x 64 instances a queue pack by five under pack_max 8, by three
+    # under 3)
+    shapes = ([(24, 24, 1.0)] * 48 + [(40, 16, 1.0)] * 24 + [(64, 64, 1.0)] * 40 + [(58, 58, 1.0)] * 8 +
+              [(24, 24, 1.5)] * 16 + [(7, 5, 1.0)] * 8)
+    sizes = [(nx, ny) for nx, ny, _ in shapes]
+    inst = []
+    for nx, ny, f in shapes:
+        q, dx = problems.random_riemann_instance(rng, nx, ny)
+        inst.append((q, dx * f))
+
+    def script(b):
+        b.run(nstep=3)
+        b.run(nstep=11)
+        T, _ = b.status()
+        # the scheme may turn the 7 x 5 grids non-finite (NaN times)
+        b.run(t_final=float(np.median(T[np.isfinite(T)])) * 1.25)
+
+    out, packs = {}, {}
+    for mode, pack_max in (("phases", 8), ("persistent", 1), ("persistent", 3), ("persistent", 8)):
+        print(f"[batch] {len(sizes)} instances, {mode}, pack_max {pack_max}, cons={cons}, bc={bc}", flush=True)
+        with Batch(sizes, [dx for _, dx in inst], cfl=0.4, jp_cri=(0.5, 2.0), bc=bc, cons_switch=cons,
+                   force_hlle=force_hlle, variant="fast") as b:
+            b.set_persistent(mode == "persistent")
+            b.set_pack_max(pack_max)
+            packs[(mode, pack_max)] = b.packs
+            for i, (q, _) in enumerate(inst):
+                b.upload(i, q)
+            script(b)
+            T, k = b.status()
+            out[(mode, pack_max)] = (T, k, [b.download(i) for i in range(len(sizes))])
+    assert packs[("phases", 8)] == (8, 0)
+    assert packs[("persistent", 1)] == (1, len(sizes))
+    assert packs[("persistent", 8)][1] < packs[("persistent", 3)][1] < len(sizes)
+    T, k, _ = out[("phases", 8)]
+    assert len(set(k.tolist())) > 1        # the t_final run stopped instances at different steps
+    _assert_same(out, finite=False)

@@ -73,6 +73,8 @@ struct wenox_batch {
     int* pcnt = nullptr;
     PersistQueue pq[PERSIST_QUEUES] = {};
     int* pitems = nullptr;   // the queues' item lists (device)
+    int pack_max = PACK_MAX_DEFAULT;   // members per pack of the persistent step
+    int packs_n = 0;                   // packs of the current tables
     hipGraph_t graph8 = nullptr;
     hipGraphExec_t exec8 = nullptr;
 };
@@ -130,12 +132,46 @@ void batch_step(wenox_batch* b)
     b->bops->advance(b->ctls, b->n, st);
 }
 
-// The persistent step's queues: instances go to the PERSIST_QUEUES queues
-// (one per XCD) largest first, each to the queue with the least work so far
-// (cells); a queue lists, per phase, the blocks of its instances in that
-// order.  Returns the lists (indices into bf, bx, by, bg) and fills Q.
-std::vector<int> persist_queues(const std::vector<BatchInst>& inst, PersistQueue* Q)
+// The persistent step's queues and packs.  Instances go to the
+// PERSIST_QUEUES queues (one per XCD) largest first, each to the queue with
+// the least work so far (cells).  In a queue, the instances of one shape (nx,
+// ny, dx) form packs of k members, k (at most pack_max) chosen for the fewest
+// lane-rows of both sweeps (column blocks times rows marched).  A queue lists,
+// per phase, the blocks of its packs (CFL and ghost-fill blocks of their
+// members, sweep blocks of the packs) in pack order.
+struct PackPlan {
+    std::vector<BatchPack> packs;
+    std::vector<int> pack_inst, inst_pack, blk_px, blk_py, items;
+    PersistQueue Q[PERSIST_QUEUES] = {};
+};
+
+// Lane-rows of the two sweeps of one pack of k members of instance I's shape.
+int64_t pack_cost(const BatchInst& I, int k)
 {
+    return (int64_t)pack_blocks(I.ny, k, SWEEP_OUT_COLS) * I.nx + (int64_t)pack_blocks(I.nx, k, SWEEP_OUT_COLS) * I.ny;
+}
+
+// Members per pack for c instances of I's shape: packs of k or k - 1 members.
+std::vector<int> pack_sizes(const BatchInst& I, int c, int pack_max)
+{
+    const int64_t widest = std::max(I.nx, I.ny) + 6;
+    int best_k = 1;
+    int64_t best = -1;
+    for (int k = 1; k <= std::min(c, pack_max) && k * widest < ((int64_t)1 << 30); ++k) {
+        const int np = (c + k - 1) / k;
+        int64_t cost = 0;
+        for (int j = 0; j < np; ++j) cost += pack_cost(I, c / np + (j < c % np ? 1 : 0));
+        if (best < 0 || cost < best) { best = cost; best_k = k; }
+    }
+    const int np = (c + best_k - 1) / best_k;
+    std::vector<int> sz(np);
+    for (int j = 0; j < np; ++j) sz[j] = c / np + (j < c % np ? 1 : 0);
+    return sz;
+}
+
+PackPlan plan_packs(const std::vector<BatchInst>& inst, int pack_max)
+{
+    PackPlan P;
     const int n = (int)inst.size();
     std::vector<int> order(n);
     for (int i = 0; i < n; ++i) order[i] = i;
@@ -149,51 +185,103 @@ std::vector<int> persist_queues(const std::vector<BatchInst>& inst, PersistQueue
         members[q].push_back(i);
         load[q] += inst[i].nx * inst[i].ny;
     }
-    std::vector<int> items;
+    P.inst_pack.assign(n, -1);
+    auto same = [&](int a, int c) {
+        return inst[a].nx == inst[c].nx && inst[a].ny == inst[c].ny && inst[a].dx == inst[c].dx;
+    };
+    std::vector<std::vector<int>> qpacks(PERSIST_QUEUES);   // the packs of each queue
     for (int q = 0; q < PERSIST_QUEUES; ++q) {
-        PersistQueue& P = Q[q];
-        P.off = (int)items.size();
-        const size_t f0 = items.size();
-        for (int i : members[q])
-            for (int k = 0; k < inst[i].cfl_nblk; ++k) items.push_back(inst[i].cfl_blk0 + k);
-        P.n_f = (int)(items.size() - f0);
-        const size_t x0 = items.size();
-        for (int i : members[q])
-            for (int k = 0; k < batch_items_x(inst[i]); ++k) items.push_back(inst[i].blk0_x + k);
-        P.n_x = (int)(items.size() - x0);
-        const size_t y0 = items.size();
-        for (int i : members[q])
-            for (int k = 0; k < batch_items_y(inst[i]); ++k) items.push_back(inst[i].blk0_y + k);
-        P.n_y = (int)(items.size() - y0);
-        const size_t g0 = items.size();
-        for (int i : members[q])
-            for (int k = 0; k < batch_items_g(inst[i]); ++k) items.push_back(inst[i].ghost_blk0 + k);
-        P.n_g = (int)(items.size() - g0);
+        // shape groups in order of first appearance (largest first)
+        std::vector<std::vector<int>> groups;
+        for (int i : members[q]) {
+            auto g = std::find_if(groups.begin(), groups.end(), [&](const std::vector<int>& v) { return same(v[0], i); });
+            if (g == groups.end()) groups.push_back({i});
+            else g->push_back(i);
+        }
+        for (const auto& g : groups) {
+            size_t at = 0;
+            for (int k : pack_sizes(inst[g[0]], (int)g.size(), pack_max)) {
+                BatchPack K{};
+                K.first = (int)P.pack_inst.size();
+                K.k = k;
+                const BatchInst& I = inst[g[at]];
+                K.gx_x = pack_blocks(I.ny, k, SWEEP_OUT_COLS);
+                K.gx_y = pack_blocks(I.nx, k, SWEEP_OUT_COLS);
+                K.npx = K.gx_x * (int)((I.nx + I.chunk_x - 1) / I.chunk_x);
+                K.npy = K.gx_y * (int)((I.ny + I.chunk_y - 1) / I.chunk_y);
+                K.px0 = (int)P.blk_px.size();
+                K.py0 = (int)P.blk_py.size();
+                const int id = (int)P.packs.size();
+                for (int j = 0; j < k; ++j) {
+                    const int i = g[at + j];
+                    P.pack_inst.push_back(i);
+                    P.inst_pack[i] = id;
+                    K.cfl_nblk += inst[i].cfl_nblk;
+                    K.g_nblk += batch_items_g(inst[i]);
+                }
+                at += k;
+                P.blk_px.insert(P.blk_px.end(), K.npx, id);
+                P.blk_py.insert(P.blk_py.end(), K.npy, id);
+                P.packs.push_back(K);
+                qpacks[q].push_back(id);
+            }
+        }
     }
-    return items;
+    for (int q = 0; q < PERSIST_QUEUES; ++q) {
+        PersistQueue& Q = P.Q[q];
+        auto& it = P.items;
+        Q.off = (int)it.size();
+        for (int id : qpacks[q]) {
+            const BatchPack& K = P.packs[id];
+            for (int j = 0; j < K.k; ++j) {
+                const BatchInst& I = inst[P.pack_inst[K.first + j]];
+                for (int b = 0; b < I.cfl_nblk; ++b) it.push_back(I.cfl_blk0 + b);
+            }
+        }
+        Q.n_f = (int)it.size() - Q.off;
+        size_t m = it.size();
+        for (int id : qpacks[q])
+            for (int b = 0; b < P.packs[id].npx; ++b) it.push_back(P.packs[id].px0 + b);
+        Q.n_x = (int)(it.size() - m);
+        m = it.size();
+        for (int id : qpacks[q])
+            for (int b = 0; b < P.packs[id].npy; ++b) it.push_back(P.packs[id].py0 + b);
+        Q.n_y = (int)(it.size() - m);
+        m = it.size();
+        for (int id : qpacks[q]) {
+            const BatchPack& K = P.packs[id];
+            for (int j = 0; j < K.k; ++j) {
+                const BatchInst& I = inst[P.pack_inst[K.first + j]];
+                for (int b = 0; b < batch_items_g(I); ++b) it.push_back(I.ghost_blk0 + b);
+            }
+        }
+        Q.n_g = (int)(it.size() - m);
+    }
+    return P;
 }
 
 // Every queue's tickets of a PERSIST_STEPS-step persistent launch walked in
 // order, as one workgroup would run them: every item's wait must already hold
 // from the items before it -- then the oldest unfinished item of a queue can
-// always run, whatever the number of workgroups -- and every instance ends
+// always run, whatever the number of workgroups -- and every pack ends
 // PERSIST_STEPS steps.
-bool persist_order_holds(const BatchTables& th, const PersistQueue* Q, const int* items, int n)
+bool persist_order_holds(const BatchTables& th, const PersistQueue* Q, const int* items)
 {
-    std::vector<int> c((size_t)n * PC_N, 0);
+    const int np = th.n_packs;
+    std::vector<int> c((size_t)np * PC_N, 0);
     for (int q = 0; q < PERSIST_QUEUES; ++q) {
         const int total = persist_items_per_step(Q[q]) * PERSIST_STEPS;
         for (int t = 0; t < total; ++t) {
             const PersistItem it = persist_item(t, Q[q], items, th);
-            if (it.inst < 0 || it.inst >= n) return false;
-            int* ci = c.data() + (size_t)it.inst * PC_N;
+            if (it.pack < 0 || it.pack >= np) return false;
+            int* ci = c.data() + (size_t)it.pack * PC_N;
             if (it.wait_ctr >= 0 && ci[it.wait_ctr] < it.wait_target) return false;
             const int prev = ci[it.sig_ctr]++;
             if (it.last_count > 0 && prev + 1 == it.last_count) ci[it.done_ctr]++;
         }
     }
-    for (int i = 0; i < n; ++i)
-        if (c[(size_t)i * PC_N + PC_STEP] != PERSIST_STEPS || c[(size_t)i * PC_N + PC_CFL] != PERSIST_STEPS)
+    for (int p = 0; p < np; ++p)
+        if (c[(size_t)p * PC_N + PC_STEP] != PERSIST_STEPS || c[(size_t)p * PC_N + PC_CFL] != PERSIST_STEPS)
             return false;
     return true;
 }
@@ -223,8 +311,19 @@ int build_tables(wenox_batch* b, int chunk_x, int chunk_y)
         I.cfl_nblk = std::min(ceil_div(I.nx * I.ny, GHOST_TB), 64);
         for (int k = 0; k < I.cfl_nblk; ++k) bf.push_back(i);
     }
-    const size_t tbytes = sizeof(BatchInst) * b->n +
-                          sizeof(int) * (bx.size() + by.size() + bc.size() + bg.size() + bf.size());
+    // the persistent step's packs and queues over these blocks
+    PackPlan pp;
+    if (b->pcnt) {
+        pp = plan_packs(b->inst, b->pack_max);
+        // tickets are ints: a batch whose launch would need more runs phase by phase
+        int64_t most = 0;
+        for (const PersistQueue& Q : pp.Q) most = std::max<int64_t>(most, persist_items_per_step(Q));
+        b->persist_fits = most * PERSIST_STEPS < ((int64_t)1 << 30);
+    }
+    const size_t tbytes = sizeof(BatchInst) * b->n + sizeof(BatchPack) * pp.packs.size() +
+                          sizeof(int) * (bx.size() + by.size() + bc.size() + bg.size() + bf.size() +
+                                         pp.pack_inst.size() + pp.inst_pack.size() + pp.blk_px.size() +
+                                         pp.blk_py.size());
     if (b->dev_tables) BCHK(hipFree(b->dev_tables));
     b->dev_tables = nullptr;
     BCHK(hipMalloc(&b->dev_tables, tbytes));
@@ -240,43 +339,49 @@ int build_tables(wenox_batch* b, int chunk_x, int chunk_y)
     char* h = host.data();
     char* dbase = (char*)b->dev_tables;
     auto put = [&](const void* src, size_t bytes) {
-        std::memcpy(h, src, bytes);
+        if (bytes) std::memcpy(h, src, bytes);
         char* dptr = dbase + (h - host.data());
         h += bytes;
         return dptr;
     };
+    // BatchInst and BatchPack first: their 8-byte fields stay aligned
     b->tab.inst = (const BatchInst*)put(b->inst.data(), sizeof(BatchInst) * b->n);
+    b->tab.packs = (const BatchPack*)put(pp.packs.data(), sizeof(BatchPack) * pp.packs.size());
     b->tab.blk_x = (const int*)put(bx.data(), sizeof(int) * bx.size());
     b->tab.blk_y = (const int*)put(by.data(), sizeof(int) * by.size());
     b->tab.blk_c = (const int*)put(bc.data(), sizeof(int) * bc.size());
     b->tab.blk_g = (const int*)put(bg.data(), sizeof(int) * bg.size());
     b->tab.blk_f = (const int*)put(bf.data(), sizeof(int) * bf.size());
+    b->tab.pack_inst = (const int*)put(pp.pack_inst.data(), sizeof(int) * pp.pack_inst.size());
+    b->tab.inst_pack = (const int*)put(pp.inst_pack.data(), sizeof(int) * pp.inst_pack.size());
+    b->tab.blk_px = (const int*)put(pp.blk_px.data(), sizeof(int) * pp.blk_px.size());
+    b->tab.blk_py = (const int*)put(pp.blk_py.data(), sizeof(int) * pp.blk_py.size());
     b->tab.n_x = (int)bx.size(); b->tab.n_y = (int)by.size(); b->tab.n_c = (int)bc.size();
     b->tab.n_g = (int)bg.size(); b->tab.n_f = (int)bf.size(); b->tab.n_inst = b->n;
+    b->tab.n_packs = (int)pp.packs.size();
+    b->tab.n_px = (int)pp.blk_px.size();
+    b->tab.n_py = (int)pp.blk_py.size();
     BCHK(hipMemcpy(b->dev_tables, host.data(), tbytes, hipMemcpyHostToDevice));
     b->chunk_x = chunk_x;
     b->chunk_y = chunk_y;
-    // the persistent step's queues over these tables, their order walked on the host
-    if (b->pcnt) {
-        // tickets are ints: a batch whose launch would need more runs phase by phase
-        int64_t most = 0;
-        for (size_t i = 0; i < b->inst.size(); ++i) {
-            const BatchInst& I = b->inst[i];
-            most += I.cfl_nblk + 3 * (int64_t)(batch_items_x(I) + batch_items_y(I) + batch_items_g(I));
-        }
-        b->persist_fits = most * PERSIST_STEPS < ((int64_t)1 << 30);
-    }
+    // the queues' order walked on the host, then their item lists uploaded
     if (b->pcnt && b->persist_fits) {
-        const std::vector<int> items = persist_queues(b->inst, b->pq);
         BatchTables th = b->tab;
         th.inst = b->inst.data();
         th.blk_x = bx.data(); th.blk_y = by.data(); th.blk_c = bc.data(); th.blk_g = bg.data(); th.blk_f = bf.data();
-        if (!persist_order_holds(th, b->pq, items.data(), b->n))
+        th.packs = pp.packs.data();
+        th.pack_inst = pp.pack_inst.data();
+        th.inst_pack = pp.inst_pack.data();
+        th.blk_px = pp.blk_px.data();
+        th.blk_py = pp.blk_py.data();
+        if (!persist_order_holds(th, pp.Q, pp.items.data()))
             return abi_fail(WENOX_E_DEVICE, "persistent batched step: an item would wait for a later ticket");
+        for (int q = 0; q < PERSIST_QUEUES; ++q) b->pq[q] = pp.Q[q];
         if (b->pitems) BCHK(hipFree(b->pitems));
         b->pitems = nullptr;
-        BCHK(hipMalloc(&b->pitems, sizeof(int) * std::max<size_t>(items.size(), 1)));
-        BCHK(hipMemcpy(b->pitems, items.data(), sizeof(int) * items.size(), hipMemcpyHostToDevice));
+        BCHK(hipMalloc(&b->pitems, sizeof(int) * std::max<size_t>(pp.items.size(), 1)));
+        BCHK(hipMemcpy(b->pitems, pp.items.data(), sizeof(int) * pp.items.size(), hipMemcpyHostToDevice));
+        b->packs_n = (int)pp.packs.size();
     }
     // the captured steps hold the old grids
     drop_graphs(b);
@@ -637,6 +742,24 @@ extern "C" int wenox_batch_set_chunks(wenox_batch* b, int chunk_x, int chunk_y)
     BCHK(hipSetDevice(b->device));
     BCHK(hipStreamSynchronize(b->stream));
     return build_tables(b, chunk_x, chunk_y);
+}
+
+extern "C" int wenox_batch_packs(wenox_batch* b, int* pack_max, int* n_packs)
+{
+    if (!b) return abi_fail(WENOX_E_ARG, "wenox_batch_packs: null batch");
+    if (pack_max) *pack_max = b->pack_max;
+    if (n_packs) *n_packs = persistent(b) ? b->packs_n : 0;
+    return WENOX_OK;
+}
+
+extern "C" int wenox_batch_set_pack_max(wenox_batch* b, int pack_max)
+{
+    if (!b || pack_max < 1) return abi_fail(WENOX_E_ARG, "wenox_batch_set_pack_max: pack_max must be >= 1");
+    if (pack_max == b->pack_max) return WENOX_OK;
+    BCHK(hipSetDevice(b->device));
+    BCHK(hipStreamSynchronize(b->stream));
+    b->pack_max = pack_max;
+    return build_tables(b, b->chunk_x, b->chunk_y);
 }
 
 extern "C" int wenox_batch_profile(wenox_batch* b, int reps, double* stage_ms)

@@ -138,6 +138,29 @@ struct BatchInst {
     int64_t boff, nb;      // blocked step: offset (doubles) of the instance's blocked state, its nb
 };
 
+// A pack of the persistent step: k instances of one shape (nx, ny, dx) whose
+// padded columns its sweeps lay side by side along the wavefront's lanes.
+// The x sweep sees k (ny + 6) padded columns, member m's padded column c at
+// lane column m (ny + 6) + c, and cuts them into column blocks of 58 output
+// lanes as it cuts one grid; a lane computes its own column's faces from its
+// own member's cells, so only ghost columns fall on the lanes where two
+// members meet.  A 64-wide instance alone fills two wavefronts with 64
+// outputs; five packed fill six.  Likewise the y sweep over k (nx + 6).
+struct BatchPack {
+    int first, k;          // members: pack_inst[first .. first + k)
+    int gx_x, gx_y;        // column blocks of the x / y sweeps over the members' padded columns
+    int px0, npx;          // x-sweep items: blk_px[px0 .. px0 + npx)
+    int py0, npy;          // y-sweep items: blk_py[py0 .. py0 + npy)
+    int cfl_nblk, g_nblk;  // CFL and ghost-fill blocks of all members
+};
+
+// Column blocks of a sweep over k members of `nt` interior transverse cells
+// (k (nt + 6) padded columns, the outputs on lanes 3..60 of each block).
+__host__ __device__ inline int pack_blocks(int64_t nt, int k, int out_cols)
+{
+    return (int)((k * (nt + 6) - 6 + out_cols - 1) / out_cols);
+}
+
 struct BatchTables {       // device pointers and grid sizes
     const BatchInst* inst;
     const int* blk_x;
@@ -146,6 +169,15 @@ struct BatchTables {       // device pointers and grid sizes
     const int* blk_g;
     const int* blk_f;
     int n_x, n_y, n_c, n_g, n_f, n_inst;
+    // the persistent step's packs (n_packs = 0 where it does not run):
+    // inst_pack[i] the pack of instance i, blk_px / blk_py the pack of each
+    // x / y sweep item
+    const BatchPack* packs;
+    const int* pack_inst;
+    const int* inst_pack;
+    const int* blk_px;
+    const int* blk_py;
+    int n_packs, n_px, n_py;
 };
 
 struct BatchCommon {
@@ -157,26 +189,27 @@ constexpr int GHOST_TB = 256;
 
 // ---- persistent batched step (blocked FP64 fast step) -------------------------
 // One launch advances every instance `steps` steps.  The instances are split
-// over PERSIST_QUEUES work queues, one per XCD (each with its own L2): a
-// workgroup (one wavefront) takes work items, in ticket order, from the queue
-// of the XCD it runs on and runs each to its end, so every item of an
-// instance runs on one XCD and the items hand data to each other through
-// that XCD's L2.  An item first waits, on a counter of its instance, for the
-// items whose results it reads.  The items of step k of a queue, in ticket
-// order: the CFL blocks of its instances (n_f), then for each stage s = 1..3
-// their x-sweep blocks (n_x), y-sweep blocks (n_y) and ghost-fill blocks
-// (n_g).  Waits, all within one instance:
+// over PERSIST_QUEUES work queues, one per XCD (each with its own L2), and
+// within a queue into packs (BatchPack): a workgroup (one wavefront) takes
+// work items, in ticket order, from the queue of the XCD it runs on and runs
+// each to its end, so every item of an instance runs on one XCD and the items
+// hand data to each other through that XCD's L2.  An item first waits, on a
+// counter of its pack, for the items whose results it reads.  The items of
+// step k of a queue, in ticket order: the CFL blocks of its instances (n_f),
+// then for each stage s = 1..3 the x-sweep blocks of its packs (n_x), their
+// y-sweep blocks (n_y) and the ghost-fill blocks of its instances (n_g).
+// Waits, all within one pack:
 //   CFL block, step k > 0   on the previous step's end    (PC_STEP >= k)
 //   x sweep, stage 1        on the step's dt              (PC_CFL >= k + 1)
 //   x sweep, stage s > 1    on the ghost fill of stage s - 1
 //   y sweep                 on the x sweep of its stage
 //   ghost fill              on the y sweep of its stage
-// (counts of done items of the instance, summed over the launch's steps).
-// The last CFL block of a step forms dt, the last stage-3 ghost-fill block
-// advances T and the step count.  Every item waits only for items of its
-// queue with smaller tickets, so the oldest unfinished item of a queue can
-// always run; persist_item is shared with a host walk of every queue
-// (build_tables) that checks this for the instance set.
+// (counts of done items of the pack, summed over the launch's steps).  The
+// last CFL block of a pack's step forms every member's dt, the last stage-3
+// ghost-fill block advances every member's T and step count.  Every item
+// waits only for items of its queue with smaller tickets, so the oldest
+// unfinished item of a queue can always run; persist_item is shared with a
+// host walk of every queue (build_tables) that checks this for the packs.
 enum PersistCounter : int {
     PC_FARR = 0,    // CFL blocks arrived (the last of a step forms dt)
     PC_CFL = 1,     // dt formed (one per step)
@@ -184,10 +217,13 @@ enum PersistCounter : int {
     PC_SY = 3,      // + 3 (s - 1): y-sweep blocks of stage s done
     PC_SG = 4,      // + 3 (s - 1): ghost-fill blocks of stage s done (stage 3: arrived)
     PC_STEP = 11,   // step ended (T, step advanced)
-    PC_N = 12       // counters per instance
+    PC_N = 12       // counters per pack
 };
 enum PersistKind : int { PK_CFL = 0, PK_X = 1, PK_Y = 2, PK_G = 3 };
 constexpr int PERSIST_STEPS = 8;   // steps per launch of the captured multi-step graph
+// Members per pack, at most (wenox_batch_set_pack_max): measured on the
+// 512-instance batch of sizes {64..384}^2 with the packs' lane-row choice.
+constexpr int PACK_MAX_DEFAULT = 8;
 // Spin limit of a dependency wait: about a second, far beyond any wait the
 // item order allows (the items an item waits for have smaller tickets and are
 // already running); passing it sets the error flag, which also ends every
@@ -195,7 +231,10 @@ constexpr int PERSIST_STEPS = 8;   // steps per launch of the captured multi-ste
 constexpr int PERSIST_SPIN_LIMIT = 1 << 22;
 
 struct PersistItem {
-    int kind, step, stage, inst, local;
+    int kind, step, stage;
+    int pack;                    // the pack whose counters the item waits on and signals
+    int inst;                    // CFL and ghost-fill blocks: their instance (sweeps: -1)
+    int local;                   // block of the instance (CFL, ghost fill) or of the pack (sweeps)
     int wait_ctr, wait_target;   // wait_ctr < 0: no wait
     int sig_ctr;                 // counter the finished item increments
     int last_count;              // > 0: the increment that reaches it finishes the step's
@@ -214,10 +253,10 @@ __host__ __device__ inline int batch_items_g(const BatchInst& I)
 {
     return (int)((6 * (I.ny + 6) + 6 * I.nx + GHOST_TB - 1) / GHOST_TB);
 }
-// One work queue: the global indices (into blk_f, blk_x, blk_y, blk_g) of its
-// items of one step, items[off ..]: n_f CFL blocks, n_x x-sweep blocks, n_y
-// y-sweep blocks, n_g ghost-fill blocks (the sweep and ghost lists serve each
-// of the three stages).
+// One work queue: the global indices (into blk_f, blk_px, blk_py, blk_g) of
+// its items of one step, items[off ..]: n_f CFL blocks, n_x x-sweep blocks,
+// n_y y-sweep blocks, n_g ghost-fill blocks (the sweep and ghost lists serve
+// each of the three stages).
 struct PersistQueue {
     int n_f, n_x, n_y, n_g, off;
 };
@@ -239,6 +278,7 @@ __host__ __device__ inline PersistItem persist_item(int t, const PersistQueue& Q
     it.step = t / per_step;
     int u = t - it.step * per_step;
     const int k1 = it.step + 1;
+    it.inst = -1;
     it.wait_ctr = -1;
     it.wait_target = 0;
     it.last_count = 0;
@@ -248,14 +288,14 @@ __host__ __device__ inline PersistItem persist_item(int t, const PersistQueue& Q
         it.kind = PK_CFL;
         it.stage = 0;
         it.inst = B.blk_f[g];
-        const BatchInst& I = B.inst[it.inst];
-        it.local = g - I.cfl_blk0;
+        it.pack = B.inst_pack[it.inst];
+        it.local = g - B.inst[it.inst].cfl_blk0;
         if (it.step > 0) {
             it.wait_ctr = PC_STEP;
             it.wait_target = it.step;
         }
         it.sig_ctr = PC_FARR;
-        it.last_count = k1 * I.cfl_nblk;
+        it.last_count = k1 * B.packs[it.pack].cfl_nblk;
         it.done_ctr = PC_CFL;
         return it;
     }
@@ -266,44 +306,46 @@ __host__ __device__ inline PersistItem persist_item(int t, const PersistQueue& Q
     if (u < Q.n_x) {
         const int g = L[Q.n_f + u];
         it.kind = PK_X;
-        it.inst = B.blk_x[g];
-        const BatchInst& I = B.inst[it.inst];
-        it.local = g - I.blk0_x;
+        it.pack = B.blk_px[g];
+        const BatchPack& K = B.packs[it.pack];
+        it.local = g - K.px0;
         if (s == 0) {
             it.wait_ctr = PC_CFL;
             it.wait_target = k1;
         } else {
             it.wait_ctr = PC_SG + 3 * (s - 1);
-            it.wait_target = k1 * batch_items_g(I);
+            it.wait_target = k1 * K.g_nblk;
         }
         it.sig_ctr = PC_SX + 3 * s;
     } else if (u < Q.n_x + Q.n_y) {
         const int g = L[Q.n_f + u];
         it.kind = PK_Y;
-        it.inst = B.blk_y[g];
-        const BatchInst& I = B.inst[it.inst];
-        it.local = g - I.blk0_y;
+        it.pack = B.blk_py[g];
+        const BatchPack& K = B.packs[it.pack];
+        it.local = g - K.py0;
         it.wait_ctr = PC_SX + 3 * s;
-        it.wait_target = k1 * batch_items_x(I);
+        it.wait_target = k1 * K.npx;
         it.sig_ctr = PC_SY + 3 * s;
     } else {
         const int g = L[Q.n_f + u];
         it.kind = PK_G;
         it.inst = B.blk_g[g];
-        const BatchInst& I = B.inst[it.inst];
-        it.local = g - I.ghost_blk0;
+        it.pack = B.inst_pack[it.inst];
+        const BatchPack& K = B.packs[it.pack];
+        it.local = g - B.inst[it.inst].ghost_blk0;
         it.wait_ctr = PC_SY + 3 * s;
-        it.wait_target = k1 * batch_items_y(I);
+        it.wait_target = k1 * K.npy;
         it.sig_ctr = PC_SG + 3 * s;
         if (s == 2) {
-            it.last_count = k1 * batch_items_g(I);
+            it.last_count = k1 * K.g_nblk;
             it.done_ctr = PC_STEP;
         }
     }
     return it;
 }
 
-// Device words after the n_inst * PC_N counters: the queues' tickets, the
+// Device words after the n_inst * PC_N counters (packs use the first
+// n_packs * PC_N of them): the queues' tickets, the
 // count of exited workgroups, the error flag (1: a wait passed its spin
 // limit; 2: a queue was left with items, no workgroup running on its XCD).
 constexpr int PW_TICKET = 0, PW_EXIT = PERSIST_QUEUES, PW_ERR = PERSIST_QUEUES + 1, PW_N = PERSIST_QUEUES + 2;

@@ -146,6 +146,15 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A lane of a packed sweep (sweep_body's PACK): its instance's blocked-state
+// offset, its padded column in that instance, the instance's dt and whether
+// the instance is active.
+struct LanePack {
+    int64_t off = 0, col = 0;
+    double dt = 0.0;
+    bool active = false;
+};
+
 // LDS of one sweep workgroup (declared by the kernel, so a kernel running
 // both sweeps holds one copy).
 template <typename T, int W> struct SweepLds {
@@ -230,14 +239,21 @@ __host__ __device__ inline int64_t wave_start(int W, int chunk, bool blend, int 
 // combine performs, so both steps give the same state bit for bit; in stage
 // 3 the workgroup also leaves the maximum of c + |V| over its new cells in
 // ctl->part and marks the step's advance pending, as the stage-3 combine.
-template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND, int W, int FOLD = FOLD_NONE>
+// PACK (blocked folds, one wave, no CFL output): the lanes belong to the
+// instances of a pack (BatchPack); the caller gives each lane its instance's
+// blocked-state offset, its padded column in that instance (clamped into the
+// pack), the instance's dt and whether it is active, in pk.  A.U, A.Un and
+// A.D are then the batch's blocked states, A.N, A.nt the members' widths.
+template <typename T, int IN, typename DT, int MODE, int DIR, bool CONS, bool BLEND, int W, int FOLD = FOLD_NONE,
+          bool PACK = false>
 __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64_t by, int64_t bid,
-                                           SweepLds<T, W>& S)
+                                           SweepLds<T, W>& S, const LanePack& pk = LanePack{})
 {
     static_assert(FOLD == FOLD_NONE || (IN == IN_F64 && MODE == MODE_PLAIN), "fold: FP64 sweeps only");
     static_assert(FOLD == FOLD_NONE || (DIR == DIR_X) == (FOLD != FOLD_BO), "fold: E from the x sweep, state from the y sweep");
     constexpr bool BLK = FOLD == FOLD_BE || FOLD == FOLD_BO;
-    if (A.ctl && !A.ctl->active) return;
+    static_assert(!PACK || (BLK && W == 1), "packs: blocked folds, one wave per workgroup");
+    if (!PACK && A.ctl && !A.ctl->active) return;
     auto& s_p1 = S.p1;
     auto& s_pb = S.pb;
     auto& s_xpb = S.xpb;
@@ -261,11 +277,11 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
     const int wv = W > 1 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE)) : 0;
     const int l = (int)(threadIdx.x % WAVE);
     const int64_t c0 = bx * OUT_COLS;                       // padded column of lane 0
-    const int64_t pc = c0 + l;                              // this lane's padded column
+    const int64_t pc = PACK ? pk.col : c0 + l;              // this lane's padded column
     const int64_t t = pc - 3;                               // interior transverse index
-    const int64_t pcl = pc < A.N ? pc : A.N - 1;            // clamped for loads
+    const int64_t pcl = PACK || pc < A.N ? pc : A.N - 1;    // clamped for loads
     const bool gauss_lane = l >= 2 && l < WAVE - 2;
-    const bool out_lane = l >= 3 && l < WAVE - 3 && t >= 0 && t < A.nt;
+    const bool out_lane = l >= 3 && l < WAVE - 3 && t >= 0 && t < A.nt && (!PACK || pk.active);
     const int lm = l > 0 ? l - 1 : 0, lp = l < WAVE - 1 ? l + 1 : WAVE - 1;   // sensor neighbours
 
     const int64_t g0 = by * group_cells(W, A.chunk, BLEND);
@@ -284,13 +300,13 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
     bool bad = false;
     // element index of padded row r (normal) at this lane's (clamped) column:
     // row layout r ld + pcl; blocked, with rows x (x sweep) or y (y sweep)
-    const int64_t lane_off = !BLK ? pcl : (DIR == DIR_X ? blk_col(pcl) : blk_row(pcl, A.nb));
+    const int64_t lane_off = !BLK ? pcl : (PACK ? pk.off : 0) + (DIR == DIR_X ? blk_col(pcl) : blk_row(pcl, A.nb));
     auto at = [&](int64_t r) -> int64_t {
         if constexpr (!BLK) return r * A.ld + pcl;
         else return (DIR == DIR_X ? blk_row(r, A.nb) : blk_col(r)) + lane_off;
     };
     double wmax = 0.0;   // FOLD_BO, stage 3: max of c + |V| over this lane's new cells
-    const bool cfl_max = FOLD == FOLD_BO && A.stage == 3 && A.cfl_out;
+    const bool cfl_max = !PACK && FOLD == FOLD_BO && A.stage == 3 && A.cfl_out;
 
     T F_prev[4], fc_prev[4], fe_prev[4];
     T pb_m2 = T(0), pb_m1 = T(0), pb_r0n = T(0);   // p̄ of faces a-2, a-1; p̄(r0 + 1)
@@ -300,7 +316,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
     double fa_c = 0.0, fb_c = 0.0, fc_c = 0.0;   // FOLD: a, b and c dt of the stage
     if constexpr (FOLD != FOLD_NONE) {
         stage_coeffs(A.stage, fa_c, fb_c, fc_c);
-        fc_c *= A.ctl->dt;
+        fc_c *= PACK ? pk.dt : A.ctl->dt;
     }
     // Un read: the x fold from stage 2 on (stage 1: a = 0), E always
     const bool fold_un = FOLD == FOLD_BO || (FOLD != FOLD_NONE && A.stage > 1);
@@ -330,7 +346,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                     else v[k] = fma(-T(fc_c), dv, un[k]);
                     d[k * P] = (DT)v[k];
                 }
-                if constexpr (FOLD == FOLD_BO) {
+                if constexpr (FOLD == FOLD_BO && !PACK) {
                     if (cfl_max) wmax = nan_max(wmax, cfl_speed(v, A.gam, A.ctl->rho_floor));
                 }
             } else {
@@ -605,7 +621,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
             put_d(r1 - 1, Fn, FLast, us2, un2);
         }
     }
-    if constexpr (FOLD == FOLD_BO) {
+    if constexpr (FOLD == FOLD_BO && !PACK) {
         if (cfl_max) {
             // the workgroup's maximum into ctl->part[bid] (the launch's grid
             // has gridDim.x * gridDim.y workgroups, bid their row-major index)
@@ -1678,11 +1694,60 @@ k_batch_persist(BatchTables B, BatchCommon C, BcParams bp, PersistArgs P)
         t = __builtin_amdgcn_readfirstlane(t);
         if (t >= total) break;
         const PersistItem it = persist_item(t, Q, P.items, B);
-        int* const cnt = P.cnt + (int64_t)it.inst * PC_N;
+        int* const cnt = P.cnt + (int64_t)it.pack * PC_N;
         const bool ok = it.wait_ctr < 0 || persist_wait(cnt + it.wait_ctr, it.wait_target, err, P.spin_limit);
-        const BatchInst I = B.inst[it.inst];
-        StepCtl* const ctl = P.ctls + it.inst;
-        if (ok) {
+        const BatchPack K = B.packs[it.pack];
+        if (ok && (it.kind == PK_X || it.kind == PK_Y)) {
+            // a column block of the pack's sweep: lane l's column of the
+            // members' padded columns laid side by side (clamped into the
+            // pack), its member, that member's offset, dt and activity (plain
+            // loads, after the acquire of the wait)
+            const bool xd = it.kind == PK_X;
+            const BatchInst I = B.inst[B.pack_inst[K.first]];   // the members' common shape
+            const double* const src = it.stage == 1 ? P.Ub : (it.stage == 2 ? P.Ab : P.Bb);
+            double* const dst = it.stage == 1 ? P.Ab : (it.stage == 2 ? P.Bb : P.Ub);
+            const int nm = (int)(xd ? I.ny + 6 : I.nx + 6);   // padded columns per member
+            const int gx = xd ? K.gx_x : K.gx_y;
+            const int bx = it.local % gx, by = it.local / gx;
+            int v = bx * OUT_COLS + l;
+            if (v > K.k * nm - 1) v = K.k * nm - 1;
+            const int m = v / nm;
+            const int mi = B.pack_inst[K.first + m];
+            LanePack lp;
+            lp.off = B.inst[mi].boff;
+            lp.col = v - m * nm;
+            lp.dt = P.ctls[mi].dt;
+            lp.active = P.ctls[mi].active != 0;
+            SweepArgs A;
+            A.U = src;
+            A.Un = xd ? P.Ub : P.Eb;
+            A.D = xd ? P.Eb : dst;
+            A.M = xd ? I.nx + 6 : I.ny + 6;
+            A.N = nm;
+            A.ld = A.N;
+            A.ldd = xd ? I.ny : I.nx;
+            A.nn = xd ? I.nx : I.ny;
+            A.nt = xd ? I.ny : I.nx;
+            A.chunk = xd ? I.chunk_x : I.chunk_y;
+            A.waves = 1;
+            A.gam = C.gam;
+            A.dx = I.dx;
+            A.two_h = xd ? I.two_h_x : I.two_h_y;
+            A.lo = C.jp_lo;
+            A.hi = C.jp_hi;
+            A.flags = nullptr;
+            A.ctl = nullptr;
+            A.stage = it.stage;
+            A.nb = I.nb;
+            if (xd)
+                sweep_body<double, IN_F64, double, MODE_PLAIN, DIR_X, CONS, BLEND, 1, FOLD_BE, true>(A, bx, by, t, S,
+                                                                                                     lp);
+            else
+                sweep_body<double, IN_F64, double, MODE_PLAIN, DIR_Y, CONS, BLEND, 1, FOLD_BO, true>(A, bx, by, t, S,
+                                                                                                     lp);
+        } else if (ok) {
+            const BatchInst I = B.inst[it.inst];
+            StepCtl* const ctl = P.ctls + it.inst;
             // the step-control fields the item reads (plain loads, after the
             // acquire of its wait)
             StepCtl lc{};
@@ -1715,53 +1780,25 @@ k_batch_persist(BatchTables B, BatchCommon C, BcParams bp, PersistArgs P)
                                    for (int k = 0; k < 4; ++k) Ui.st(k, at(r, c), v[k]);
                                });
                 }
-            } else {
-                const double* const src = it.stage == 1 ? P.Ub : (it.stage == 2 ? P.Ab : P.Bb);
-                double* const dst = it.stage == 1 ? P.Ab : (it.stage == 2 ? P.Bb : P.Ub);
-                const bool xd = it.kind == PK_X;
-                SweepArgs A;
-                A.U = src + I.boff;
-                A.Un = (xd ? P.Ub : P.Eb) + I.boff;
-                A.D = (xd ? P.Eb : dst) + I.boff;
-                A.M = xd ? I.nx + 6 : I.ny + 6;
-                A.N = xd ? I.ny + 6 : I.nx + 6;
-                A.ld = A.N;
-                A.ldd = xd ? I.ny : I.nx;
-                A.nn = xd ? I.nx : I.ny;
-                A.nt = xd ? I.ny : I.nx;
-                A.chunk = xd ? I.chunk_x : I.chunk_y;
-                A.waves = 1;
-                A.gam = C.gam;
-                A.dx = I.dx;
-                A.two_h = xd ? I.two_h_x : I.two_h_y;
-                A.lo = C.jp_lo;
-                A.hi = C.jp_hi;
-                A.flags = nullptr;
-                A.ctl = &lc;
-                A.stage = it.stage;
-                A.nb = I.nb;
-                const int gx = xd ? I.gx_x : I.gx_y;
-                if (xd)
-                    sweep_body<double, IN_F64, double, MODE_PLAIN, DIR_X, CONS, BLEND, 1, FOLD_BE>(
-                        A, it.local % gx, it.local / gx, t, S);
-                else
-                    sweep_body<double, IN_F64, double, MODE_PLAIN, DIR_Y, CONS, BLEND, 1, FOLD_BO>(
-                        A, it.local % gx, it.local / gx, t, S);
             }
         }
         const int prev = persist_signal(cnt + it.sig_ctr);
         if (it.last_count > 0 && prev + 1 == it.last_count) {
-            // every other block of the step has signalled: finish it
+            // every other block of the pack's step has signalled: finish it
+            // for every member
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             if (l == 0) {
-                if (it.kind == PK_CFL) {
-                    const unsigned long long bits =
-                        __hip_atomic_load(&ctl->cmax_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    cfl_finish(ctl, __longlong_as_double((long long)bits), false);
-                    __hip_atomic_store(&ctl->cmax_bits, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                } else if (ctl->active) {
-                    ctl->T += ctl->dt;
-                    ctl->step += 1;
+                for (int m = 0; m < K.k; ++m) {
+                    StepCtl* const ctl = P.ctls + B.pack_inst[K.first + m];
+                    if (it.kind == PK_CFL) {
+                        const unsigned long long bits =
+                            __hip_atomic_load(&ctl->cmax_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        cfl_finish(ctl, __longlong_as_double((long long)bits), false);
+                        __hip_atomic_store(&ctl->cmax_bits, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    } else if (ctl->active) {
+                        ctl->T += ctl->dt;
+                        ctl->step += 1;
+                    }
                 }
             }
             persist_signal(cnt + it.done_ctr);

@@ -423,6 +423,21 @@ class Batch:
     def set_persistent(self, on: bool = True):
         _lib.check(_lib.load().wenox_batch_set_persistent(self._h, int(bool(on))))
 
+    @property
+    def packs(self) -> tuple:
+        """(pack_max, packs in effect): the persistent step lays the padded
+        columns of same-shape instances side by side along a wavefront's
+        lanes, in packs of at most pack_max members (0 packs when the
+        persistent step does not run)."""
+        m, n = ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.load().wenox_batch_packs(self._h, ctypes.byref(m), ctypes.byref(n)))
+        return m.value, n.value
+
+    def set_pack_max(self, pack_max: int):
+        """Members per pack at most (1: no packing); the same states bit for
+        bit for every value."""
+        _lib.check(_lib.load().wenox_batch_set_pack_max(self._h, int(pack_max)))
+
     def set_spin_limit(self, spins: int):
         """Polls of a persistent step's dependency wait before it gives up
         (run() then raises WenoxError); small values force that path."""
