@@ -133,6 +133,36 @@ def test_l_local_five_args(wx, oracle_mod):
     np.testing.assert_array_equal(got, ref)
 
 
+@pytest.mark.parametrize("shape,cons,force_hlle", [((1920, 480), False, False), ((1920, 480), True, True),
+                                                    ((257, 1100), False, False), ((130, 2050), True, False)])
+def test_l_local_large_grid_slabs(wx, oracle_mod, shape, cons, force_hlle):
+    """Grids of a million output values and more run in row slabs (packing,
+    the copies each way and the kernels overlapped; each slab's passes cover
+    one row more on each inner side, the sensor's h that of the whole grid):
+    exact variant bitwise against the oracle on an (m, n, 8) driver array,
+    at the double Mach reflection's size and on uneven slab splits; fast within
+    1e-9 of the largest value, as on small grids."""
+    from wenox import problems
+    nx, ny = shape
+    if shape == (1920, 480):
+        q = problems.double_mach(nx, ny)[0]
+    else:
+        rng = np.random.default_rng(nx + ny)
+        q = problems.random_riemann_instance(rng, nx, ny)[0]
+    assert q.shape == (nx + 6, ny + 6, 8)
+    print(f"[dropin] l_local {shape} cons={cons} force_hlle={force_hlle}", flush=True)
+    ref = oracle_mod.l_local_py(q, 1.0 / ny, 1.4, force_hlle, (0.5, 2.0), cons)
+    wx.set_variant(False)
+    got = wx.l_local_py(q, 1.0 / ny, 1.4, force_hlle, (0.5, 2.0), cons)
+    np.testing.assert_array_equal(got, ref)
+    wx.set_variant(True)
+    try:
+        fast = wx.l_local_py(q, 1.0 / ny, 1.4, force_hlle, (0.5, 2.0), cons)
+    finally:
+        wx.set_variant(False)
+    _close(fast, ref, 1e-9)
+
+
 def test_sqrt_div_correctly_rounded(wx):
     """con2primi's a = sqrt(g p / rho) and u = m/rho on random data exercise the
     device division and square root; they must round like the host's."""

@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
+#include <functional>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -48,18 +50,30 @@ const Ops* ops_for(int variant)
 }
 
 // Checks that a gfx950 device is current; the code objects are built for it only.
+// The device count, and each device's architecture, are looked up once per
+// process (the queries cost hundreds of microseconds, as much as a drop-in
+// call's kernels).
 int require_device()
 {
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
-        return fail(WENOX_E_DEVICE, "no HIP device visible: the wenox kernels need an MI355X (gfx950)");
+    static std::once_flag once;
+    static int count = 0;
+    std::call_once(once, [] {
+        if (hipGetDeviceCount(&count) != hipSuccess) count = 0;
+    });
+    if (count == 0) return fail(WENOX_E_DEVICE, "no HIP device visible: the wenox kernels need an MI355X (gfx950)");
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
-    hipDeviceProp_t prop;
-    HIPCHK(hipGetDeviceProperties(&prop, dev));
-    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-        return fail(WENOX_E_DEVICE, std::string("device is ") + prop.gcnArchName +
-                                        "; the wenox kernels are built for gfx950 only");
+    static std::mutex mu;
+    static std::vector<std::string> arch(count);
+    std::lock_guard<std::mutex> lk(mu);
+    if (dev < 0 || dev >= count) return fail(WENOX_E_DEVICE, "current HIP device out of range");
+    if (arch[dev].empty()) {
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, dev));
+        arch[dev] = prop.gcnArchName[0] ? prop.gcnArchName : "?";
+    }
+    if (arch[dev].compare(0, 6, "gfx950") != 0)
+        return fail(WENOX_E_DEVICE, "device is " + arch[dev] + "; the wenox kernels are built for gfx950 only");
     return WENOX_OK;
 }
 
@@ -67,9 +81,17 @@ int require_device()
 // Drop-in workspace: one stream, a growing device arena and pinned staging.
 // Calls are serialised (the reference holds the GIL for the whole call).
 // ------------------------------------------------------------------------
+// Slabs of a pipelined drop-in l_local (see wenox_l_local).
+constexpr int DROPIN_SLABS = 4;
+
 struct Dropin {
     std::mutex mu;
     hipStream_t stream = nullptr;
+    // the pipelined l_local: input rows copied in and results copied out on
+    // streams of their own; an event per slab for uploaded, computed and
+    // copied out
+    hipStream_t h2d = nullptr, d2h = nullptr;
+    hipEvent_t ev[3][DROPIN_SLABS] = {};
     double* dev = nullptr;
     size_t dev_cap = 0;
     double* pin = nullptr;
@@ -82,6 +104,11 @@ int dropin_reserve(size_t dev_doubles, size_t pin_doubles)
     int rc = require_device();
     if (rc) return rc;
     if (!g_drop.stream) HIPCHK(hipStreamCreateWithFlags(&g_drop.stream, hipStreamNonBlocking));
+    if (!g_drop.h2d) HIPCHK(hipStreamCreateWithFlags(&g_drop.h2d, hipStreamNonBlocking));
+    if (!g_drop.d2h) HIPCHK(hipStreamCreateWithFlags(&g_drop.d2h, hipStreamNonBlocking));
+    for (auto& row : g_drop.ev)
+        for (hipEvent_t& e : row)
+            if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     if (dev_doubles > g_drop.dev_cap) {
         if (g_drop.dev) HIPCHK(hipFree(g_drop.dev));
         g_drop.dev = nullptr;
@@ -99,33 +126,123 @@ int dropin_reserve(size_t dev_doubles, size_t pin_doubles)
     return WENOX_OK;
 }
 
+// Host threads for the drop-in calls' packing and copies (memory-bound; one
+// thread moves a few GB/s): workers started on first use and kept for the
+// life of the process (starting threads per call cost about as much as the
+// copies).  A job runs fn over T row ranges; start() hands it to the workers
+// and returns, so the caller can issue device work meanwhile; wait() runs the
+// ranges no worker has taken on the calling thread (so a job always
+// completes, also in a forked child that has no workers) and returns when all
+// are done.  One job at a time.  The pool is never destroyed: its idle
+// workers end with the process.
+class HostPool {
+public:
+    static constexpr int MAX_THREADS = 16;
+    int threads()
+    {
+        const unsigned hw = std::thread::hardware_concurrency();
+        return (int)std::min<unsigned>(hw ? hw : 1, MAX_THREADS);
+    }
+    void start(int64_t M, int T, std::function<void(int64_t, int64_t)> fn)
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        idle_.wait(lk, [this] { return !active_; });
+        while ((int)th_.size() < std::min(T, MAX_THREADS - 1)) th_.emplace_back([this] { work(); });
+        active_ = true;
+        fn_ = std::move(fn);
+        m_ = M;
+        parts_ = T;
+        next_ = 0;
+        left_ = T;
+        ++gen_;
+        cv_.notify_all();
+    }
+    void wait()
+    {
+        take();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [this] { return left_ == 0; });
+        fn_ = nullptr;
+        active_ = false;
+        idle_.notify_all();
+    }
+    void run(int64_t M, int T, std::function<void(int64_t, int64_t)> fn)
+    {
+        start(M, T, std::move(fn));
+        wait();
+    }
+
+private:
+    // run the job's ranges no thread has taken yet
+    void take()
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        while (active_ && next_ < parts_) {
+            const int t = next_++;
+            const int64_t M = m_, T = parts_;
+            lk.unlock();
+            fn_(t * M / T, (t + 1) * M / T);   // fn_ stays until every range is done
+            lk.lock();
+            if (--left_ == 0) done_.notify_all();
+        }
+    }
+    void work()
+    {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+            }
+            take();
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_, done_, idle_;
+    std::vector<std::thread> th_;
+    std::function<void(int64_t, int64_t)> fn_;
+    bool active_ = false;
+    int64_t m_ = 0;
+    int parts_ = 0, next_ = 0, left_ = 0;
+    uint64_t gen_ = 0;
+};
+HostPool& host_pool()
+{
+    static HostPool* pool = new HostPool;
+    return *pool;
+}
+
+// Host threads for fn over rows [0, M) holding `work` doubles in all (at
+// least PAR_MIN per thread).
+int par_threads(int64_t M, int64_t work)
+{
+    constexpr int64_t PAR_MIN = (int64_t)1 << 15;
+    return (int)std::min<int64_t>({(int64_t)host_pool().threads(), M, std::max<int64_t>(work / PAR_MIN, 1)});
+}
+
 // fn(i0, i1) over row ranges [i0, i1) of [0, M), on several host threads when
-// the rows hold at least `work` doubles in all (the drop-in calls' host-side
-// packing and copies are memory-bound; one thread moves a few GB/s).
+// the rows hold enough doubles in all (`work`).
 template <class Fn> void par_rows(int64_t M, int64_t work, Fn fn)
 {
-    constexpr int64_t PAR_MIN = (int64_t)1 << 20;   // doubles
-    const unsigned hw = std::thread::hardware_concurrency();
-    const int64_t T = std::min<int64_t>({(int64_t)(hw ? hw : 1), 16, M, std::max<int64_t>(work / PAR_MIN, 1)});
+    const int T = par_threads(M, work);
     if (T <= 1) {
         fn((int64_t)0, M);
         return;
     }
-    std::vector<std::thread> pool;
-    pool.reserve((size_t)T - 1);
-    for (int64_t t = 1; t < T; ++t) pool.emplace_back(fn, t * M / T, (t + 1) * M / T);
-    fn((int64_t)0, M / T);
-    for (auto& th : pool) th.join();
+    host_pool().run(M, T, std::function<void(int64_t, int64_t)>(fn));
 }
 
-// Packs channels 0..3 of rows [0, M) x cols [0, N) of a strided view into an
-// AoS4 (M, N, 4) contiguous buffer.
-void pack4(const wenox_view3* v, int64_t M, int64_t N, double* dst)
+// Packing of channels 0..3 of rows r0 + [a, b) x cols [0, N) of a strided
+// view into the same rows of an AoS4 (M, N, 4) contiguous buffer, as a job
+// over row ranges [a, b).
+std::function<void(int64_t, int64_t)> pack4_job(const wenox_view3* v, int64_t r0, int64_t N, double* dst)
 {
     const int64_t s0 = v->stride[0], s1 = v->stride[1], s2 = v->stride[2];
     const double* p = v->data;
     const bool dense = s2 == 1 && s1 == 4 && s0 == 4 * N;
-    par_rows(M, M * N * 4, [=](int64_t i0, int64_t i1) {
+    return [=](int64_t a, int64_t b) {
+        const int64_t i0 = r0 + a, i1 = r0 + b;
         if (dense) {
             std::memcpy(dst + i0 * N * 4, p + i0 * N * 4, sizeof(double) * (size_t)((i1 - i0) * N * 4));
             return;
@@ -136,7 +253,12 @@ void pack4(const wenox_view3* v, int64_t M, int64_t N, double* dst)
                 double* d = dst + (i * N + j) * 4;
                 d[0] = s[0]; d[1] = s[s2]; d[2] = s[2 * s2]; d[3] = s[3 * s2];
             }
-    });
+    };
+}
+
+void pack4(const wenox_view3* v, int64_t M, int64_t N, double* dst)
+{
+    par_rows(M, M * N * 4, pack4_job(v, 0, N, dst));
 }
 
 bool bad_view(const wenox_view3* v)
@@ -278,7 +400,7 @@ int riemann_common(const char* fn, const wenox_view3* ql, const wenox_view3* fl,
         if ((rc = upload4(in[k], F1, F2, d[k], g_drop.pin + k * nf))) return rc;
     if (F1 * F2 > 0)
         ops_for(g_variant)->riemann(d[0], d[1], d[2], d[3], lay_aos4(F1, F2), F1, F2, dir, kind,
-                                    gamma, lo, hi, pbar, dout, lay_aos4(F1, F2), g_drop.stream);
+                                    gamma, lo, hi, pbar, dout, lay_aos4(F1, F2), 0, g_drop.stream);
     if ((rc = sync_copy_out(out, dout, nf))) return rc;
     return finish();
 }
@@ -402,6 +524,66 @@ extern "C" int wenox_hllc_hlle_y(const wenox_view3* ql, const wenox_view3* fl,
                           jp_lo, jp_hi, out);
 }
 
+namespace {
+
+// The reference pass structure of l_local (riemann.rs:489-541) on one padded
+// array of m x n cells (element layout lu) on stream s: face arrays in work
+// (rhs_unfused_scratch(m - 6, n - 6) doubles), the (m - 6, n - 6, 4) AoS4
+// result in dout.  nx_all: the interior rows of the whole grid when the
+// array is a band of its rows (the sensor's h follows the whole grid).
+void l_local_passes(const Ops* ops, const double* du, Lay lu, int64_t m, int64_t n, int64_t nx_all, double dx,
+                    double gamma, int kind, int cons, double jp_lo, double jp_hi, double* work, double* dout,
+                    hipStream_t s)
+{
+    const int64_t nx = m - 6, ny = n - 6;
+    const int64_t p1 = std::max((m - 5) * n, m * (n - 5)) * 4;
+    const int64_t fxs = 4 * (nx + 1) * ny, fys = 4 * nx * (ny + 1), fm = std::max(fxs, fys);
+    double* pl = work;
+    double* pr = pl + p1;
+    double* f0 = pr + p1;
+    double* fx = f0 + 4 * fm;
+    double* fy = fx + fxs;
+    double* pbar = fy + fys;
+    Lay lx = lay_aos4(nx + 1, ny), ly = lay_aos4(nx, ny + 1);
+    ops->weno(du, lu, m, n, 0, cons, gamma, pl, pr, f0, f0 + fm, f0 + 2 * fm, f0 + 3 * fm, lx, s);
+    ops->riemann(f0, f0 + fm, f0 + 2 * fm, f0 + 3 * fm, lx, nx + 1, ny, 0, kind, gamma, jp_lo, jp_hi, pbar, fx, lx,
+                 std::min(nx_all + 1, ny), s);
+    ops->weno(du, lu, m, n, 1, cons, gamma, pl, pr, f0, f0 + fm, f0 + 2 * fm, f0 + 3 * fm, ly, s);
+    ops->riemann(f0, f0 + fm, f0 + 2 * fm, f0 + 3 * fm, ly, nx, ny + 1, 1, kind, gamma, jp_lo, jp_hi, pbar, fy, ly,
+                 std::min(nx_all, ny + 1), s);
+    ops->divergence(fx, lx, fy, ly, nx, ny, dx, dout, lay_aos4(nx, ny), s);
+}
+
+// Interior rows [i0, i1) of slab k of P over nx rows, and the padded rows
+// [a, b) its passes run on: one interior row more on each side that is not
+// a grid edge, so that the rows it keeps see the same faces and the same
+// central sensor differences as one pass over the whole grid.
+struct Slab {
+    int64_t i0, i1, a, b;
+};
+Slab dropin_slab(int k, int P, int64_t nx)
+{
+    Slab q;
+    q.i0 = nx * k / P;
+    q.i1 = nx * (k + 1) / P;
+    q.a = k == 0 ? 0 : q.i0 - 1;
+    q.b = (k == P - 1 ? nx : q.i1 + 1) + 6;
+    return q;
+}
+
+}  // namespace
+
+// Grids of at least DROPIN_PIPE_MIN output doubles run in DROPIN_SLABS slabs of
+// rows: the host packs and uploads slab k + 1 while the device computes slab
+// k and copies slab k - 1 back, which the host then copies into the caller's
+// array.  A slab's passes cover one row more on each inner side and keep the
+// rows in between, so every cell's result is the one a single pass gives, bit
+// for bit.  Measured at the double Mach reflection's size (1926 x 486 x 8
+// input, fast variant, median of 30 calls): one pass 2.86-2.91 ms a call; 2,
+// 3, 4, 6, 8 slabs 2.24, 1.89-1.99, 1.86-2.08, 2.14-2.26, 2.32-2.42 ms (each
+// slab costs about 0.1 ms of launches and copies issued from the host).
+constexpr size_t DROPIN_PIPE_MIN = (size_t)1 << 20;
+
 extern "C" int wenox_l_local(const wenox_view3* u, double dx, double gamma, int force_hlle,
                              double jp_lo, double jp_hi, int cons, double* out)
 {
@@ -415,37 +597,90 @@ extern "C" int wenox_l_local(const wenox_view3* u, double dx, double gamma, int 
     const int64_t nx = m - 6, ny = n - 6;
     const size_t nu = (size_t)(m * n * 4), no = (size_t)(nx * ny * 4);
     if (no > 0 && !out) return fail(WENOX_E_ARG, "l_local_py: null output");
-    const size_t scratch = rhs_unfused_scratch(nx, ny);
+    const Ops* ops = ops_for(g_variant);
+    const int kind = force_hlle ? 1 : 2;
     std::lock_guard<std::mutex> lk(g_drop.mu);
-    int rc = dropin_reserve(nu + scratch + no, nu);
+    const int P = no >= DROPIN_PIPE_MIN ? (int)std::min<int64_t>(DROPIN_SLABS, nx / 16) : 1;
+    if (P <= 1) {
+        const size_t scratch = rhs_unfused_scratch(nx, ny);
+        int rc = dropin_reserve(nu + scratch + no, nu);
+        if (rc) return rc;
+        double* du = g_drop.dev;
+        double* work = du + nu;
+        double* dout = work + scratch;
+        if ((rc = upload4(u, m, n, du, g_drop.pin))) return rc;
+        l_local_passes(ops, du, lay_aos4(m, n), m, n, nx, dx, gamma, kind, cons, jp_lo, jp_hi, work, dout,
+                       g_drop.stream);
+        if ((rc = sync_copy_out(out, dout, no))) return rc;
+        return finish();
+    }
+    // device: the input, each slab's result rows (with its extra rows), the
+    // passes' face arrays for the largest slab; pinned: input and result
+    size_t slab_out = 0, scratch = 0;
+    for (int k = 0; k < P; ++k) {
+        const Slab q = dropin_slab(k, P, nx);
+        slab_out += (size_t)(q.b - 6 - q.a) * ny * 4;
+        scratch = std::max(scratch, rhs_unfused_scratch(q.b - 6 - q.a, ny));
+    }
+    int rc = dropin_reserve(nu + slab_out + scratch, nu + no);
     if (rc) return rc;
     double* du = g_drop.dev;
-    double* work = du + nu;
-    double* dout = work + scratch;
-    if ((rc = upload4(u, m, n, du, g_drop.pin))) return rc;
-    // reference pass structure on AoS4 face arrays, output in AoS4
-    const Ops* ops = ops_for(g_variant);
-    const int64_t p1 = std::max((m - 5) * n, m * (n - 5)) * 4;
-    const int64_t fxs = 4 * (nx + 1) * ny, fys = 4 * nx * (ny + 1), fm = std::max(fxs, fys);
-    double* pl = work;
-    double* pr = pl + p1;
-    double* f0 = pr + p1;
-    double* fx = f0 + 4 * fm;
-    double* fy = fx + fxs;
-    double* pbar = fy + fys;
-    const int kind = force_hlle ? 1 : 2;
-    hipStream_t s = g_drop.stream;
-    Lay lx = lay_aos4(nx + 1, ny), ly = lay_aos4(nx, ny + 1);
-    ops->weno(du, lay_aos4(m, n), m, n, 0, cons, gamma, pl, pr, f0, f0 + fm, f0 + 2 * fm,
-              f0 + 3 * fm, lx, s);
-    ops->riemann(f0, f0 + fm, f0 + 2 * fm, f0 + 3 * fm, lx, nx + 1, ny, 0, kind, gamma, jp_lo,
-                 jp_hi, pbar, fx, lx, s);
-    ops->weno(du, lay_aos4(m, n), m, n, 1, cons, gamma, pl, pr, f0, f0 + fm, f0 + 2 * fm,
-              f0 + 3 * fm, ly, s);
-    ops->riemann(f0, f0 + fm, f0 + 2 * fm, f0 + 3 * fm, ly, nx, ny + 1, 1, kind, gamma, jp_lo,
-                 jp_hi, pbar, fy, ly, s);
-    ops->divergence(fx, lx, fy, ly, nx, ny, dx, dout, lay_aos4(nx, ny), s);
-    if ((rc = sync_copy_out(out, dout, no))) return rc;
+    double* dslab = du + nu;
+    double* work = dslab + slab_out;
+    double* pin_in = g_drop.pin;
+    double* pin_out = pin_in + nu;
+    const int64_t row = n * 4, orow = ny * 4;   // doubles per input / output row
+    // the host pool packs the next slab's input rows while this thread issues
+    // the current slab's copies and kernels (an early return still waits for
+    // the packing before the caller's array may go)
+    struct PackJob {
+        bool on = false;
+        void start(const wenox_view3* v, int64_t r0, int64_t r1, int64_t N, double* dst)
+        {
+            // two host threads fewer than the pool has: this thread and the
+            // runtime's keep running meanwhile
+            const int T = std::max(1, std::min(par_threads(r1 - r0, (r1 - r0) * N * 4), host_pool().threads() - 2));
+            host_pool().start(r1 - r0, T, pack4_job(v, r0, N, dst));
+            on = true;
+        }
+        void wait()
+        {
+            if (on) host_pool().wait();
+            on = false;
+        }
+        ~PackJob() { wait(); }
+    } pack;
+    pack.start(u, 0, dropin_slab(0, P, nx).b, n, pin_in);
+    size_t off = 0;
+    int64_t up = 0;   // input rows uploaded
+    for (int k = 0; k < P; ++k) {
+        const Slab q = dropin_slab(k, P, nx);
+        // rows [up, q.b): the input rows this slab needs beyond those already on their way
+        pack.wait();
+        if (k + 1 < P) pack.start(u, q.b, dropin_slab(k + 1, P, nx).b, n, pin_in);
+        HIPCHK(hipMemcpyAsync(du + up * row, pin_in + up * row, sizeof(double) * (size_t)((q.b - up) * row),
+                              hipMemcpyHostToDevice, g_drop.h2d));
+        HIPCHK(hipEventRecord(g_drop.ev[2][k], g_drop.h2d));
+        HIPCHK(hipStreamWaitEvent(g_drop.stream, g_drop.ev[2][k], 0));
+        up = q.b;
+        double* dk = dslab + off;
+        l_local_passes(ops, du + q.a * row, lay_aos4(q.b - q.a, n), q.b - q.a, n, nx, dx, gamma, kind, cons, jp_lo,
+                       jp_hi, work, dk, g_drop.stream);
+        HIPCHK(hipEventRecord(g_drop.ev[0][k], g_drop.stream));
+        HIPCHK(hipStreamWaitEvent(g_drop.d2h, g_drop.ev[0][k], 0));
+        HIPCHK(hipMemcpyAsync(pin_out + q.i0 * orow, dk + (q.i0 - q.a) * orow,
+                              sizeof(double) * (size_t)((q.i1 - q.i0) * orow), hipMemcpyDeviceToHost, g_drop.d2h));
+        HIPCHK(hipEventRecord(g_drop.ev[1][k], g_drop.d2h));
+        off += (size_t)(q.b - 6 - q.a) * orow;
+    }
+    for (int k = 0; k < P; ++k) {
+        const Slab q = dropin_slab(k, P, nx);
+        HIPCHK(hipEventSynchronize(g_drop.ev[1][k]));
+        par_rows(q.i1 - q.i0, (q.i1 - q.i0) * orow, [=](int64_t r0, int64_t r1) {
+            std::memcpy(out + (q.i0 + r0) * orow, pin_out + (q.i0 + r0) * orow, sizeof(double) * (size_t)((r1 - r0) * orow));
+        });
+    }
+    HIPCHK(hipStreamSynchronize(g_drop.d2h));
     return finish();
 }
 
@@ -511,26 +746,9 @@ extern "C" int wenox_l_local_noalloc(const wenox_view3* u, double dx, double gam
     int rc = scratch_for_current_device(&work);
     if (rc) return rc;
     if (nx * ny == 0) return WENOX_OK;
-    hipStream_t s = (hipStream_t)stream;
-    const Ops* ops = ops_for(g_variant);
     const Lay lu{u->stride[0], u->stride[1], u->stride[2]};
-    const int64_t p1 = std::max((m - 5) * n, m * (n - 5)) * 4;
-    const int64_t fxs = 4 * (nx + 1) * ny, fys = 4 * nx * (ny + 1), fm = std::max(fxs, fys);
-    double* pl = work;
-    double* pr = pl + p1;
-    double* f0 = pr + p1;
-    double* fx = f0 + 4 * fm;
-    double* fy = fx + fxs;
-    double* pbar = fy + fys;
-    const int kind = force_hlle ? 1 : 2;
-    Lay lx = lay_aos4(nx + 1, ny), ly = lay_aos4(nx, ny + 1);
-    ops->weno(u->data, lu, m, n, 0, cons, gamma, pl, pr, f0, f0 + fm, f0 + 2 * fm, f0 + 3 * fm, lx, s);
-    ops->riemann(f0, f0 + fm, f0 + 2 * fm, f0 + 3 * fm, lx, nx + 1, ny, 0, kind, gamma, jp_lo, jp_hi, pbar,
-                 fx, lx, s);
-    ops->weno(u->data, lu, m, n, 1, cons, gamma, pl, pr, f0, f0 + fm, f0 + 2 * fm, f0 + 3 * fm, ly, s);
-    ops->riemann(f0, f0 + fm, f0 + 2 * fm, f0 + 3 * fm, ly, nx, ny + 1, 1, kind, gamma, jp_lo, jp_hi, pbar,
-                 fy, ly, s);
-    ops->divergence(fx, lx, fy, ly, nx, ny, dx, out, lay_aos4(nx, ny), s);
+    l_local_passes(ops_for(g_variant), u->data, lu, m, n, nx, dx, gamma, force_hlle ? 1 : 2, cons, jp_lo, jp_hi,
+                   work, out, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return WENOX_OK;
 }

@@ -279,7 +279,7 @@ static void h_weno(const double* u, Lay lu, int64_t m, int64_t n, int dir, int c
 
 static void h_riemann(const double* ql, const double* fl, const double* qr, const double* fr,
                       Lay li, int64_t F1, int64_t F2, int dir, int kind, double gam, double lo,
-                      double hi, double* pbar, double* out, Lay lout, hipStream_t s)
+                      double hi, double* pbar, double* out, Lay lout, int64_t h_min, hipStream_t s)
 {
     if (F1 * F2 == 0) return;
     Acc QL{ql, li}, FL{fl, li}, QR{qr, li}, FR{fr, li};
@@ -288,7 +288,7 @@ static void h_riemann(const double* ql, const double* fl, const double* qr, cons
                                                       AccW{out, lout});
         return;
     }
-    int64_t mn = F1 < F2 ? F1 : F2;
+    int64_t mn = h_min > 0 ? h_min : (F1 < F2 ? F1 : F2);
     double h = 1.0 / (double)(mn > 1 ? mn : 1);
     k_pbar<<<blocks_for(F1 * F2), TB, 0, s>>>(QL, QR, F1, F2, gam, pbar);
     k_blend<<<blocks_for(F1 * F2), TB, 0, s>>>(QL, FL, QR, FR, pbar, F1, F2, dir, gam, 2.0 * h,
@@ -402,10 +402,10 @@ static void h_rhs_unfused(const double* U, int64_t nx, int64_t ny, int64_t ld, d
     const int kind = force_hlle ? 1 : 2;
     Lay lx = lay_soa4(nx + 1, ny);
     h_weno(U, lu, m, n, DIR_X, cons, gam, pl, pr, f[0], f[1], f[2], f[3], lx, s);
-    h_riemann(f[0], f[1], f[2], f[3], lx, nx + 1, ny, DIR_X, kind, gam, lo, hi, pbar, fx, lx, s);
+    h_riemann(f[0], f[1], f[2], f[3], lx, nx + 1, ny, DIR_X, kind, gam, lo, hi, pbar, fx, lx, 0, s);
     Lay ly = lay_soa4(nx, ny + 1);
     h_weno(U, lu, m, n, DIR_Y, cons, gam, pl, pr, f[0], f[1], f[2], f[3], ly, s);
-    h_riemann(f[0], f[1], f[2], f[3], ly, nx, ny + 1, DIR_Y, kind, gam, lo, hi, pbar, fy, ly, s);
+    h_riemann(f[0], f[1], f[2], f[3], ly, nx, ny + 1, DIR_Y, kind, gam, lo, hi, pbar, fy, ly, 0, s);
     h_divergence(fx, lx, fy, ly, nx, ny, dx, L, lay_soa4(nx, ny), s);
 }
 

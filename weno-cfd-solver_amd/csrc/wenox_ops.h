@@ -92,10 +92,13 @@ struct Ops {
     void (*weno)(const double* u, Lay lu, int64_t m, int64_t n, int dir, int cons, double gam,
                  double* pl, double* pr, double* ql, double* fl, double* qr, double* fr, Lay lo,
                  hipStream_t s);
-    // kind 0 HLLC, 1 HLLE, 2 blend (pbar: scratch of F1*F2 doubles)
+    // kind 0 HLLC, 1 HLLE, 2 blend (pbar: scratch of F1*F2 doubles); the
+    // blend's sensor takes h = 1/h_min (riemann.rs:360-364 with the face
+    // grid's min(F1, F2) when h_min = 0; the whole grid's when (F1, F2) is a
+    // band of its rows)
     void (*riemann)(const double* ql, const double* fl, const double* qr, const double* fr,
                     Lay li, int64_t F1, int64_t F2, int dir, int kind, double gam, double lo,
-                    double hi, double* pbar, double* out, Lay lout, hipStream_t s);
+                    double hi, double* pbar, double* out, Lay lout, int64_t h_min, hipStream_t s);
     // out (Nx, Ny) = -((fx[i+1]-fx[i])/dx + (fy[j+1]-fy[j])/dx)
     void (*divergence)(const double* fx, Lay lfx, const double* fy, Lay lfy, int64_t Nx,
                        int64_t Ny, double dx, double* out, Lay lout, hipStream_t s);
