@@ -293,6 +293,15 @@ int wenox_batch_set_persistent(wenox_batch *b, int on);
  * when the persistent step does not run).  No reference counterpart. */
 int wenox_batch_packs(wenox_batch *b, int *pack_max, int *n_packs);
 int wenox_batch_set_pack_max(wenox_batch *b, int pack_max);
+/* XCDs (bit i: XCC id i) the persistent step's work queues serve.  probed:
+ * those a launch of the persistent grid reached when the batch was created
+ * (all eight on an MI355X in its one-partition mode; fewer on a partitioned
+ * device, where the persistent step then queues work only to those); in_use:
+ * the ones the queues use, by default every probed one below 8.  set: a
+ * non-empty subset of them (0: the default); the states are the same bit for
+ * bit for every subset.  No reference counterpart. */
+int wenox_batch_persist_xcds(wenox_batch *b, unsigned *in_use, unsigned *probed);
+int wenox_batch_set_persist_xcds(wenox_batch *b, unsigned mask);
 /* Polls of a persistent step's dependency wait before it gives up and sets
  * the error flag (default 4194304, about a second).  A small value forces the
  * error path, for its tests; the launch still drains.  No reference

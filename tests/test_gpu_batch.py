@@ -371,3 +371,28 @@ def test_batch_packs_same_states(cons, force_hlle, bc):
     T, k, _ = out[("phases", 8)]
     assert len(set(k.tolist())) > 1        # the t_final run stopped instances at different steps
     _assert_same(out, finite=False)
+
+
+def test_batch_persistent_on_a_subset_of_xcds():
+    """The probe launch at creation reaches all eight XCDs of the MI355X; the
+    persistent step serving only some of them (as on a partitioned device,
+    where a launch reaches fewer) gives the same states bit for bit as the
+    launch-per-phase step; masks outside the probed XCDs are refused."""
+    from wenox import Batch
+    sizes, inst = _mixed_instances(24, 37)
+    ref = _run_modes(sizes, inst, (("phases", True),), lambda b: b.run(nstep=11), cfl=0.4, jp_cri=(0.5, 2.0))
+    for mask in (0b00100001, 0b10000000, 0):
+        with Batch(sizes, [q[1] for q in inst], cfl=0.4, jp_cri=(0.5, 2.0), variant="fast") as b:
+            in_use, probed = b.persist_xcds
+            assert probed == 0xFF and in_use == 0xFF
+            with pytest.raises(ValueError):
+                b.set_persist_xcds(1 << 9)
+            b.set_persist_xcds(mask)
+            assert b.persist_xcds[0] == (mask or 0xFF) and b.persistent
+            print(f"[batch] persistent step on XCD mask {mask:#x}", flush=True)
+            for i, (q, _) in enumerate(inst):
+                b.upload(i, q)
+            b.run(nstep=11)
+            T, k = b.status()
+            got = [b.download(i) for i in range(len(sizes))]
+        _assert_same({"phases": ref[("phases", True)], "masked": (T, k, got)})

@@ -75,13 +75,16 @@ struct wenox_batch {
     int* pitems = nullptr;   // the queues' item lists (device)
     int pack_max = PACK_MAX_DEFAULT;   // members per pack of the persistent step
     int packs_n = 0;                   // packs of the current tables
+    // XCDs (bit i: XCC id i) a launch reached at creation, and those the
+    // queues use (a subset of the probed ones below PERSIST_QUEUES)
+    unsigned xcd_probed = 0, xcd_use = 0;
     hipGraph_t graph8 = nullptr;
     hipGraphExec_t exec8 = nullptr;
 };
 
 namespace {
 
-bool persistent(const wenox_batch* b) { return b->blk && b->persist && b->persist_fits; }
+bool persistent(const wenox_batch* b) { return b->blk && b->persist && b->persist_fits && b->xcd_use != 0; }
 
 void persist_steps(wenox_batch* b, int steps)
 {
@@ -132,9 +135,9 @@ void batch_step(wenox_batch* b)
     b->bops->advance(b->ctls, b->n, st);
 }
 
-// The persistent step's queues and packs.  Instances go to the
-// PERSIST_QUEUES queues (one per XCD) largest first, each to the queue with
-// the least work so far (cells).  In a queue, the instances of one shape (nx,
+// The persistent step's queues and packs.  Instances go to the queues of the
+// XCDs in `xcds` (bit q: the queue of XCC id q, q < PERSIST_QUEUES) largest
+// first, each to the queue with the least work so far (cells).  In a queue, the instances of one shape (nx,
 // ny, dx) form packs of k members, k (at most pack_max) chosen for the fewest
 // lane-rows of both sweeps (column blocks times rows marched).  A queue lists,
 // per phase, the blocks of its packs (CFL and ghost-fill blocks of their
@@ -169,7 +172,7 @@ std::vector<int> pack_sizes(const BatchInst& I, int c, int pack_max)
     return sz;
 }
 
-PackPlan plan_packs(const std::vector<BatchInst>& inst, int pack_max)
+PackPlan plan_packs(const std::vector<BatchInst>& inst, int pack_max, unsigned xcds)
 {
     PackPlan P;
     const int n = (int)inst.size();
@@ -181,7 +184,9 @@ PackPlan plan_packs(const std::vector<BatchInst>& inst, int pack_max)
     std::vector<std::vector<int>> members(PERSIST_QUEUES);
     std::vector<int64_t> load(PERSIST_QUEUES, 0);
     for (int i : order) {
-        const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        int q = -1;
+        for (int c = 0; c < PERSIST_QUEUES; ++c)
+            if (((xcds >> c) & 1u) && (q < 0 || load[c] < load[q])) q = c;
         members[q].push_back(i);
         load[q] += inst[i].nx * inst[i].ny;
     }
@@ -314,7 +319,7 @@ int build_tables(wenox_batch* b, int chunk_x, int chunk_y)
     // the persistent step's packs and queues over these blocks
     PackPlan pp;
     if (b->pcnt) {
-        pp = plan_packs(b->inst, b->pack_max);
+        pp = plan_packs(b->inst, b->pack_max, b->xcd_use ? b->xcd_use : 1u);
         // tickets are ints: a batch whose launch would need more runs phase by phase
         int64_t most = 0;
         for (const PersistQueue& Q : pp.Q) most = std::max<int64_t>(most, persist_items_per_step(Q));
@@ -529,11 +534,16 @@ extern "C" int wenox_batch_create(const wenox_solver_desc* desc, int n, const in
         }
     }
     if (ok && b->blk) {
-        const size_t nc = (size_t)n * PC_N + PW_N;
-        ok = hipMalloc(&b->pcnt, nc * sizeof(int)) == hipSuccess &&
-             hipMemset(b->pcnt, 0, nc * sizeof(int)) == hipSuccess;
+        // counters (at least the 32 words of the XCD probe)
+        const size_t nc = std::max<size_t>((size_t)n * PC_N + PW_N, 32);
+        ok = hipMalloc(&b->pcnt, nc * sizeof(int)) == hipSuccess;
         b->pgrid = ok ? b->bops->persist_grid(BatchCommon{d.gamma, d.jp_lo, d.jp_hi, d.cons_switch, d.force_hlle}) : 0;
         ok = ok && b->pgrid > 0;
+        // the XCDs the persistent launch's workgroups reach: a partitioned
+        // device (fewer XCDs) gets queues only where its workgroups run
+        b->xcd_probed = ok ? b->bops->xcc_mask(b->pgrid, b->pcnt, b->stream) : 0u;
+        b->xcd_use = b->xcd_probed & ((1u << PERSIST_QUEUES) - 1);
+        ok = ok && hipMemset(b->pcnt, 0, nc * sizeof(int)) == hipSuccess;
         b->persist = true;   // the default step structure of the blocked fast FP64 batch
     }
     if (!ok) return cleanup(abi_fail(WENOX_E_DEVICE, "batch allocation failed (" + std::to_string((6 * soff + 2 * ioff + (b->blk ? 4 * boff : 0)) * 8 >> 20) + " MiB)"));
@@ -744,6 +754,29 @@ extern "C" int wenox_batch_set_chunks(wenox_batch* b, int chunk_x, int chunk_y)
     return build_tables(b, chunk_x, chunk_y);
 }
 
+extern "C" int wenox_batch_persist_xcds(wenox_batch* b, unsigned* in_use, unsigned* probed)
+{
+    if (!b) return abi_fail(WENOX_E_ARG, "wenox_batch_persist_xcds: null batch");
+    if (in_use) *in_use = b->xcd_use;
+    if (probed) *probed = b->xcd_probed;
+    return WENOX_OK;
+}
+
+extern "C" int wenox_batch_set_persist_xcds(wenox_batch* b, unsigned mask)
+{
+    if (!b) return abi_fail(WENOX_E_ARG, "wenox_batch_set_persist_xcds: null batch");
+    const unsigned avail = b->xcd_probed & ((1u << PERSIST_QUEUES) - 1);
+    if (mask == 0) mask = avail;
+    if ((mask & ~avail) != 0 || mask == 0)
+        return abi_fail(WENOX_E_ARG, "wenox_batch_set_persist_xcds: the mask must be a non-empty subset of the XCDs "
+                                     "the probe launch reached (" + std::to_string(avail) + ")");
+    if (mask == b->xcd_use) return WENOX_OK;
+    BCHK(hipSetDevice(b->device));
+    BCHK(hipStreamSynchronize(b->stream));
+    b->xcd_use = mask;
+    return build_tables(b, b->chunk_x, b->chunk_y);
+}
+
 extern "C" int wenox_batch_packs(wenox_batch* b, int* pack_max, int* n_packs)
 {
     if (!b) return abi_fail(WENOX_E_ARG, "wenox_batch_packs: null batch");
@@ -840,6 +873,8 @@ extern "C" int wenox_batch_set_persistent(wenox_batch* b, int on)
         return abi_fail(WENOX_E_ARG, "the persistent step serves the batched fast FP64 step (blocked layout)");
     if (on && !b->persist_fits)
         return abi_fail(WENOX_E_ARG, "the persistent step: too many work items per launch for this batch and chunk");
+    if (on && !b->xcd_use)
+        return abi_fail(WENOX_E_DEVICE, "the persistent step: no XCD with a work queue was reached by the probe launch");
     if ((on != 0) == b->persist) return WENOX_OK;
     BCHK(hipSetDevice(b->device));
     BCHK(hipStreamSynchronize(b->stream));

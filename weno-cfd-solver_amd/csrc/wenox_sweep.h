@@ -381,6 +381,9 @@ struct BatchOps {
     // resident workgroups of the persistent kernel on the current device
     // (occupancy x compute units), for c's scheme
     int (*persist_grid)(const BatchCommon& c);
+    // the XCDs a launch of `grid` workgroups runs on (bit i: XCC id i; bit
+    // 31: ids of 31 and more), through `words` (32 device ints); 0 on error
+    unsigned (*xcc_mask)(int grid, int* words, hipStream_t s);
 };
 
 extern const BatchOps batch_ops_exact;

@@ -1824,6 +1824,32 @@ __global__ void k_fill_int(int* p, int n, int v)
     if (i < n) p[i] = v;
 }
 
+// Each workgroup marks the XCD it runs on: seen[id] = 1 for ids below 31,
+// seen[31] for any other.
+__global__ void k_xcc_probe(int* seen)
+{
+    if (threadIdx.x == 0) {
+        const int xcc = (int)(__builtin_amdgcn_s_getreg((15 << 11) | (0 << 6) | 20) & 0xffff);   // HW_REG_XCC_ID
+        seen[xcc < 31 ? xcc : 31] = 1;
+    }
+}
+
+// Bit i set when a workgroup of a `grid`-workgroup launch ran on XCD i (bit
+// 31: an id of 31 or more); words: 32 device ints.  0 on a HIP error.
+static unsigned hb_xcc_mask(int grid, int* words, hipStream_t s)
+{
+    int seen[32] = {};
+    k_fill_int<<<1, 256, 0, s>>>(words, 32, 0);
+    k_xcc_probe<<<(unsigned)(grid > 0 ? grid : 1), WAVE, 0, s>>>(words);
+    if (hipMemcpyAsync(seen, words, sizeof seen, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return 0u;
+    unsigned m = 0;
+    for (int i = 0; i < 32; ++i)
+        if (seen[i]) m |= 1u << i;
+    return m;
+}
+
 template <bool CONS, bool BLEND> static int persist_grid_of()
 {
     static int grid = 0;   // per scheme; the library serves one device type
@@ -1873,11 +1899,11 @@ static void hb_persist(const BatchTables& t, const BatchCommon& c, BcParams bp, 
 #if WX_FAST
 const BatchOps batch_ops_fast = {WX_NS::hb_stage,     WX_NS::hb_bc,     WX_NS::hb_cfl,     WX_NS::hb_advance,
                                  WX_NS::hb_stage_blk, WX_NS::hb_bc_blk, WX_NS::hb_cfl_blk, WX_NS::hb_persist,
-                                 WX_NS::hb_persist_grid};
+                                 WX_NS::hb_persist_grid, WX_NS::hb_xcc_mask};
 #else
 const BatchOps batch_ops_exact = {WX_NS::hb_stage, WX_NS::hb_bc, WX_NS::hb_cfl, WX_NS::hb_advance,
                                   nullptr,         nullptr,      nullptr,       nullptr,
-                                  nullptr};
+                                  nullptr,         nullptr};
 #endif
 #if WX_FAST
 const SweepOps WX_SWEEP_OPS = {WX_NS::h_stage,       WX_NS::h_rhs,        WX_NS::h_transpose,

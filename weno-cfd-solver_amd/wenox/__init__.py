@@ -433,6 +433,20 @@ class Batch:
         _lib.check(_lib.load().wenox_batch_packs(self._h, ctypes.byref(m), ctypes.byref(n)))
         return m.value, n.value
 
+    @property
+    def persist_xcds(self) -> tuple:
+        """(in use, probed): bit masks of the XCDs whose work queues the
+        persistent step serves, and of those a probe launch reached at
+        creation (fewer than eight on a partitioned device)."""
+        u, p = ctypes.c_uint(), ctypes.c_uint()
+        _lib.check(_lib.load().wenox_batch_persist_xcds(self._h, ctypes.byref(u), ctypes.byref(p)))
+        return u.value, p.value
+
+    def set_persist_xcds(self, mask: int):
+        """Serve only the XCDs in `mask` (a non-empty subset of the probed
+        ones; 0 restores the default); the same states bit for bit."""
+        _lib.check(_lib.load().wenox_batch_set_persist_xcds(self._h, int(mask)))
+
     def set_pack_max(self, pack_max: int):
         """Members per pack at most (1: no packing); the same states bit for
         bit for every value."""

@@ -124,6 +124,8 @@ def load():
         "wenox_batch_set_spin_limit": ([dp, i], i),
         "wenox_batch_packs": ([dp, ctypes.POINTER(i), ctypes.POINTER(i)], i),
         "wenox_batch_set_pack_max": ([dp, i], i),
+        "wenox_batch_persist_xcds": ([dp, ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_uint)], i),
+        "wenox_batch_set_persist_xcds": ([dp, ctypes.c_uint], i),
         "wenox_batch_profile_steps": ([dp, i, dp], i),
         "wenox_batch_status": ([dp, dp, dp], i),
         "wenox_batch_last_ms_per_step": ([dp], d),
