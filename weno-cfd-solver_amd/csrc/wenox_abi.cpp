@@ -650,6 +650,16 @@ extern "C" int wenox_l_local(const wenox_view3* u, double dx, double gamma, int 
         }
         ~PackJob() { wait(); }
     } pack;
+    // every exit, an error's included, leaves no copy in flight on the staging
+    // buffers or the arena (declared after `pack`, so it runs first)
+    struct Drain {
+        ~Drain()
+        {
+            (void)hipStreamSynchronize(g_drop.h2d);
+            (void)hipStreamSynchronize(g_drop.stream);
+            (void)hipStreamSynchronize(g_drop.d2h);
+        }
+    } drain;
     pack.start(u, 0, dropin_slab(0, P, nx).b, n, pin_in);
     size_t off = 0;
     int64_t up = 0;   // input rows uploaded

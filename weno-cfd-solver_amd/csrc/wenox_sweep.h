@@ -241,14 +241,6 @@ struct PersistItem {
     int done_ctr;                //   CFL (dt) or the step (advance), then increments done_ctr
 };
 
-__host__ __device__ inline int batch_items_x(const BatchInst& I)
-{
-    return I.gx_x * (int)((I.nx + I.chunk_x - 1) / I.chunk_x);
-}
-__host__ __device__ inline int batch_items_y(const BatchInst& I)
-{
-    return I.gx_y * (int)((I.ny + I.chunk_y - 1) / I.chunk_y);
-}
 __host__ __device__ inline int batch_items_g(const BatchInst& I)
 {
     return (int)((6 * (I.ny + 6) + 6 * I.nx + GHOST_TB - 1) / GHOST_TB);
