@@ -527,7 +527,7 @@ def workload_block(key, steps, warmup):
 # halo re-reads)
 _BATCH_SRC = "profiles/batch512_fp64_persistent.md"
 BATCH_TRAFFIC = {
-    "persistent": {"bytes_per_step": 18.5470e9, "wave_instr_per_step": 3.871e9, "chunk": 48, "source": _BATCH_SRC},
+    "persistent": {"bytes_per_step": 18.0921e9, "wave_instr_per_step": 3.8045e9, "chunk": 64, "source": _BATCH_SRC},
 }
 
 
