@@ -188,7 +188,8 @@ double wenox_solver_last_ms_per_step(wenox_solver *s);
  * n_out >= 7 [6] both sweeps launched as the step launches them ([5] then
  * uses [6] for the sweeps), with n_out >= 8 [7] one whole RK stage as the
  * step launches it (sweeps and combine, pipelined slabs included; [5] then
- * uses [7]).  n_out >= 6. */
+ * uses [7]).  With the blocked layout [1], [2], [6] and [7] are averages
+ * over the three stages.  n_out >= 6. */
 int wenox_solver_profile(wenox_solver *s, int reps, double *out_ms, int n_out);
 /* CFL rule dt = CFL dx / (max(sqrt(g p / max(rho, rho_floor)) + |V|) + cmax_eps):
  * (1e-14, 1e-16) by default as Rie_prob_2d_rs.py:311-319; the 1-D driver

@@ -143,7 +143,10 @@ class Solver:
         the double-Mach-reflection fills on grids whose edge tiles are at
         least 3 cells wide).  "stage" is one whole RK stage as the step
         launches it (sweeps, combine and the slab pipeline), and "step_sum"
-        is then 3 x (stage + ghost_fill) + cfl."""
+        is then 3 x (stage + ghost_fill) + cfl.  With the blocked layout the
+        sweeps and the stage are averaged over the three stages (they differ:
+        the x sweep of stage 1 reads no U^n, the y sweep of stage 3 also
+        reduces the next step's wave speeds)."""
         out = np.zeros(8)
         _lib.check(_lib.load().wenox_solver_profile(self._h, int(reps), out.ctypes.data, 8))
         return dict(zip(self.KERNELS, out.tolist()))
