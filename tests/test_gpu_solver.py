@@ -394,6 +394,67 @@ def test_fused_ghost_cells_for_every_edge_tile_width(bc, grid):
     np.testing.assert_array_equal(out["fused"][0], out["unfused"][0], err_msg=bc)
 
 
+# (nx, ny, tiling): the blocked y sweep's tiles are 58 x rows on the lanes and
+# groups of the march along y (chunk_y, or waves_y * chunk_y - 3 with the
+# blend); an edge tile of 1 or 2 cells leaves the ghost fill to its kernel
+BLK_GHOST_CASES = [
+    (61, 40, (16, 12, 1, 1), True),    # x edge 3 cells, y edge 4
+    (59, 40, (16, 12, 1, 1), False),   # x edge 1 cell
+    (60, 40, (16, 12, 1, 1), False),   # x edge 2 cells
+    (70, 40, (16, 13, 1, 1), False),   # y edge 1 cell
+    (70, 41, (16, 8, 2, 2), False),    # y groups of 13: edge 2 cells
+    (70, 42, (16, 8, 2, 2), True),     # y groups of 13: edge 3 cells
+    (130, 37, (12, 6, 4, 4), True),    # x edges 14, y groups of 21: edge 16
+]
+BLK_GHOST_DMR_CASES = [
+    (136, 34, (16, 10, 1, 1), True),
+    (136, 34, (16, 16, 1, 1), False),  # y edge 2 cells
+    (120, 30, (12, 9, 4, 4), True),    # y groups of 33: one group of 30
+    (60, 15, (8, 8, 1, 1), False),     # x edge 2 cells
+]
+
+
+@pytest.mark.parametrize("bc", ["mirror", "periodic", "copy", "dmr_ref", "dmr"])
+@pytest.mark.parametrize("case", range(4))
+def test_blocked_y_sweep_ghost_cells(bc, case):
+    """The blocked FP64 step's y sweep writes the ghost cells of the state it
+    forms where each ghost's source lies in the sweep tile of its nearest
+    interior cell (local rules, edge tiles of 3 cells or more along x and y),
+    and the ghost-fill kernel runs otherwise (profile()["ghost_fill"] > 0):
+    either way the full arrays, ghost cells included, and the times equal the
+    row-major step's bit for bit after 6 steps, for every boundary rule."""
+    from wenox import Solver, problems
+    dmr = bc in ("dmr", "dmr_ref")
+    cases = BLK_GHOST_DMR_CASES if dmr else BLK_GHOST_CASES
+    for nx, ny, tiling, in_sweep in cases[case::4]:
+        kw = dict(cfl=0.3, jp_cri=(0.5, 2.0))
+        if bc == "dmr":
+            q0, dx = problems.double_mach(nx, ny)
+            kw.update(cfl=0.1, jp_cri=(1e4, 1.1e4))
+        elif bc == "dmr_ref":
+            q0, bound = problems.dmr_reference(nx, ny)
+            dx = 1.0 / ny
+            kw.update(cfl=0.1, jp_cri=(1e4, 1.1e4), dmr_bound=bound)
+        else:
+            q0, dx = problems.four_quadrant(nx, ny=ny)
+        print(f"[solver] blocked ghost cells {bc} {nx}x{ny} tiling {tiling}", flush=True)
+        out = {}
+        for blocked in (False, True):
+            with Solver(nx, ny, dx, bc=bc, engine="fused", variant="fast", **kw) as s:
+                s.set_blocked(blocked)
+                s.set_tiling(*tiling)
+                s.upload(q0)
+                if blocked:
+                    # profile() leaves the state and the step control as they were
+                    fill_ms = s.profile(reps=1)["ghost_fill"]
+                    assert (fill_ms == 0.0) == (in_sweep and bc != "periodic"), (fill_ms, in_sweep)
+                T, k = s.run(nstep=6)
+                out[blocked] = (s.download(), T)
+        assert np.isfinite(out[False][0]).all()
+        assert out[True][1] == out[False][1]
+        np.testing.assert_array_equal(out[True][0], out[False][0], err_msg=f"{bc} {nx}x{ny} {tiling}")
+
+
 @pytest.mark.parametrize("bc", ["mirror", "copy", "dmr", "dmr_ref"])
 def test_fp32_split_ghost_cells_on_store(bc):
     """The FP32-split step's combine writes the ghost cells too where

@@ -405,6 +405,21 @@ inline bool ghost_on_store(const SweepPlan& p)
     return p.nx >= 3 && p.ny >= 3 && rx != 1 && rx != 2 && ry != 1 && ry != 2;
 }
 
+// The blocked step's y sweep writes the ghost cells of the state it forms,
+// in place of a ghost-fill launch, when every ghost cell's source lies in the
+// sweep tile that holds the ghost's nearest interior cell: the local rules
+// (not periodic) on grids whose edge tiles are at least 3 cells wide along x
+// (58-row blocks of the lanes) and along y (the groups of the march).
+inline bool ghost_in_ysweep(const SweepPlan& p)
+{
+    if (!p.ghost || p.bp.kind == BC_PERIODIC || p.nx < 3 || p.ny < 3) return false;
+    const int64_t w = p.waves_y, c = p.chunk_y;
+    const int64_t g = w > 1 ? w * c - (p.force_hlle ? 1 : 3) : c;   // cells per workgroup along y
+    if (g < 3) return false;
+    const int64_t rx = p.nx % SWEEP_OUT_COLS, ry = p.ny % g;
+    return rx != 1 && rx != 2 && ry != 1 && ry != 2;
+}
+
 extern const SweepOps sweep_ops_exact;
 extern const SweepOps sweep_ops_fast;
 

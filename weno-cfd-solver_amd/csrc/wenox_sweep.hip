@@ -104,6 +104,10 @@ struct SweepArgs {
     // for the y sweep rows are y and lanes x, as in the transposed copy)
     int64_t nb = 0;
     int cfl_out = 0;            // FOLD_BO, stage 3: leave the CFL maxima in ctl->part
+    // FOLD_BO: also write the ghost cells anchored in the workgroup's tile by
+    // the rule bp (ghost_in_ysweep holds: every source in the same tile)
+    int ghost = 0;
+    BcParams bp{};
 };
 
 // Shu-Osher coefficients of stage s: U_{s+1} = a Un + b Us + (c dt) L
@@ -622,6 +626,48 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
         }
     }
     if constexpr (FOLD == FOLD_BO && !PACK) {
+        if (A.ghost) {
+            // The ghost cells anchored in this workgroup's tile (x rows
+            // [x0, x1) on the lanes, y cells [y0, y1) of the march): its own
+            // padded range widened by the 3 ghost layers on the grid's edges,
+            // as the RK combine's ghost-on-store; each from its source cell of
+            // the new state, which this workgroup has just written (the
+            // barrier makes every wave's stores visible to the others).
+            __syncthreads();
+            const int64_t nx = A.nt, ny = A.nn;
+            const int64_t x0 = c0, x1 = c0 + OUT_COLS, y0 = g0, y1 = g0 + group_cells(W, A.chunk, BLEND);
+            const bool e0 = x0 == 0, e1 = x1 >= nx, f0 = y0 == 0, f1 = y1 >= ny;
+            if (e0 || e1 || f0 || f1) {
+                const int64_t ia = e0 ? 0 : x0 + 3, ib = e1 ? nx + 5 : x1 + 2;
+                const int64_t ja = f0 ? 0 : y0 + 3, jb = f1 ? ny + 5 : y1 + 2;
+                const int64_t ci = ib - ia + 1, cj = jb - ja + 1;
+                const int64_t ri = e0 ? 3 : 0, rb = e1 ? 3 : 0;       // ghost rows at the low / high x end
+                const int64_t gj = (f0 ? 3 : 0) + (f1 ? 3 : 0);       // ghost columns
+                // ghost rows over the tile's whole column range, then ghost
+                // columns over its interior rows
+                const int64_t nrow = (ri + rb) * cj, n = nrow + (ci - ri - rb) * gj;
+                const double ts = A.bp.kind == BC_DMR_STD ? bc_time(A.ctl, A.bp, A.stage) : 0.0;
+                double* const St = (double*)A.D;
+                for (int64_t q = threadIdx.x; q < n; q += WAVE * W) {
+                    int64_t i, j;
+                    if (q < nrow) {
+                        const int64_t r = q / cj;
+                        i = r < ri ? ia + r : nx + 3 + (r - ri);
+                        j = ja + q % cj;
+                    } else {
+                        const int64_t u = q - nrow, c = u % gj;
+                        i = ia + ri + u / gj;
+                        j = (f0 && c < 3) ? c : ny + 3 + (c - (f0 ? 3 : 0));
+                    }
+                    const BcSrc b = bc_source(i, j, nx, ny, A.bp, ts);
+                    const int64_t so = blk_row(b.si, A.nb) + blk_col(b.sj), go = blk_row(i, A.nb) + blk_col(j);
+                    double v[4];
+                    bc_value(b, A.bp, [&](int k) { return St[so + k]; }, v);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) St[go + k] = v[k];
+                }
+            }
+        }
         if (cfl_max) {
             // the workgroup's maximum into ctl->part[bid] (the launch's grid
             // has gridDim.x * gridDim.y workgroups, bid their row-major index)
@@ -1196,6 +1242,10 @@ static SweepArgs args_blk(const SweepPlan& p, int dir, const double* U, const do
     a.stage = stage;
     a.nb = p.nb;
     a.cfl_out = 1;
+    if (dir == 1 && ghost_in_ysweep(p)) {
+        a.ghost = 1;
+        a.bp = p.bp;
+    }
     return a;
 }
 
