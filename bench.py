@@ -117,7 +117,11 @@ def run_gpu(w, q, dx, steps, warmup, engine, variant, precision, profile=True, t
         state = s.download()[3:-3, 3:-3, :4]
         out = dict(seconds=el, steps=k, value=w["nx"] * w["ny"] * k / el, dev_ms_per_step=s.ms_per_step,
                    state=state, finite=bool(np.isfinite(state).all()), t=T, variant=variant,
-                   segments=-(-steps // window) if window else 1)
+                   segments=-(-steps // window) if window else 1,
+                   # steps from the initial state to `state`
+                   state_steps=seg if window else warmup + steps)
+        if precision == "fp32split":
+            out["split_jump"] = s.split_jump
         if engine == "fused":
             out["chunks"] = list(s.tiling)
             if tuned:
@@ -498,7 +502,8 @@ def split_block(w, q, dx, steps, warmup, r64):
             "speedup_vs_fp64": r32["value"] / r64["value"],
             "ms_per_step": r32["seconds"] / r32["steps"] * 1e3,
             "density_vs_fp64": _dist(r32["state"][..., 0], r64["state"][..., 0]),
-            "after_steps": warmup + steps, "state_finite": r32["finite"], "chunks": r32.get("chunks"),
+            "after_steps": r32["state_steps"], "split_jump": r32["split_jump"],
+            "state_finite": r32["finite"], "chunks": r32.get("chunks"),
             "roofline": roofline(r32, w, split=True)}
 
 

@@ -222,6 +222,17 @@ int wenox_solver_set_combine_cols(wenox_solver *s, int cols);
 int wenox_solver_pipeline(wenox_solver *s, int *parts);
 int wenox_solver_set_pipeline(wenox_solver *s, int parts);
 
+/* FP32-split precision: besides a face whose FP32 Gauss-point states are
+ * inadmissible (rho <= 0, p <= 0, not finite), a face takes FP64 states when
+ * its normal stencil, the normal stencils of the four faces beside it that
+ * feed its Gauss points, or the face values feeding its Gauss points span an
+ * increment of density or energy above tau times the centre value (default
+ * 0.25; inf disables the test).  Smaller tau: more faces in FP64, smaller
+ * FP32 error, slower.  The choice is per face, so the state does not depend
+ * on the tiling.  No reference counterpart. */
+int wenox_solver_split_jump(wenox_solver *s, double *tau);
+int wenox_solver_set_split_jump(wenox_solver *s, double tau);
+
 /* State layout of the fused engine's fast FP64 step.  0: row-major planes
  * with a transposed copy for the y sweep, and per RK stage the x and y
  * sweeps and an RK-combine launch.  1 (the default): a blocked state (a
@@ -308,6 +319,9 @@ int wenox_batch_set_persist_xcds(wenox_batch *b, unsigned mask);
  * error path, for its tests; the launch still drains.  No reference
  * counterpart. */
 int wenox_batch_set_spin_limit(wenox_batch *b, int spins);
+/* The FP32-split threshold of wenox_solver_set_split_jump for every
+ * instance.  No reference counterpart. */
+int wenox_batch_set_split_jump(wenox_batch *b, double tau);
 /* Device time (ms, HIP events) per step of `steps` whole steps of every
  * instance as wenox_batch_run launches them (for the autotuner); the states
  * and step controls are restored afterwards.  No reference counterpart. */

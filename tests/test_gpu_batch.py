@@ -73,6 +73,30 @@ def test_batch_fp32_split():
         assert err <= 1e-6, (i, err)
 
 
+@pytest.mark.parametrize("tau", [0.25, float("inf"), 0.05])
+def test_batch_fp32_split_matches_single_instances(tau):
+    """The batched FP32-split step with a split-jump threshold gives each
+    instance the state of a single FP32-split solver with that threshold,
+    bit for bit."""
+    from wenox import Batch, Solver
+    inst = _instances(7)
+    with Batch(SIZES, [q[1] for q in inst], cfl=0.4, jp_cri=(1e4, 1.1e4), variant="fast",
+               precision="fp32split") as b:
+        b.set_split_jump(tau)
+        for i, (q, dx) in enumerate(inst):
+            b.upload(i, q)
+        b.run(nstep=6)
+        got = [b.download(i) for i in range(len(inst))]
+    for i in (0, 2, 5):
+        (nx, ny), (q, dx) = SIZES[i], inst[i]
+        with Solver(nx, ny, dx, cfl=0.4, jp_cri=(1e4, 1.1e4), engine="fused", variant="fast",
+                    precision="fp32split") as s:
+            s.set_split_jump(tau)
+            s.upload(q)
+            s.run(nstep=6)
+            np.testing.assert_array_equal(got[i][..., :4], s.download()[..., :4], err_msg=f"instance {i}")
+
+
 def test_batch_t_final():
     from wenox import Batch
     inst = _instances(3)

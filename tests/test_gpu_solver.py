@@ -123,10 +123,11 @@ def test_solver_rhs_matches_l_local(oracle_mod):
 def test_fp32_split_accuracy(oracle_mod):
     """FP32-split (FP32 WENO arithmetic on increments, hi+lo state, FP64
     fluxes, Riemann solver, combine and CFL; faces with an inadmissible FP32
-    Gauss-point state recomputed in FP64) after 50 steps on the coarse 128^2
-    four-quadrant grid: the same state bit for bit for every sweep tiling (the
-    FP64 fallback is chosen per face), and relative L1 density error against
-    the FP64 oracle <= 1e-5 (measured 5.0e-6).  This grid is the hard case:
+    Gauss-point state, or whose stencils span a large relative jump,
+    recomputed in FP64) after 50 steps on the coarse 128^2 four-quadrant
+    grid: the same state bit for bit for every sweep tiling (the FP64
+    fallback is chosen per face), and relative L1 density error against
+    the FP64 oracle <= 1e-5 (measured 1.6e-6; 3.1e-6 without the jump test).  This grid is the hard case:
     the reference scheme itself turns it non-finite at step 293, and
     perturbations of FP32 size grow by about 50x within 50 steps (an FP32
     Riemann stage alone, on the oracle, gives 1.6e-5: test_fp32_sensitivity);
@@ -184,6 +185,32 @@ def test_fp32_split_rhs(oracle_mod):
         got = s.rhs()
     scale = np.abs(ref).max()
     assert np.abs(got - ref).max() <= 1e-4 * scale
+
+
+def test_fp32_split_jump_threshold(oracle_mod):
+    """The FP32-split split-jump threshold: with a tiny threshold every face
+    whose stencils vary takes FP64 states, and the RHS is the oracle's up to
+    the FP32 rounding of the stored flux differences; with the test off (inf)
+    the FP32 reconstruction of the four-quadrant jumps shows; non-positive
+    and NaN thresholds are rejected."""
+    from wenox import Solver, problems
+    q0, dx = problems.four_quadrant(64)
+    ref = oracle_mod.l_local_py(q0, dx, 1.4, False, (0.5, 2.0), False)
+    scale = np.abs(ref).max()
+    err = {}
+    with Solver(64, 64, dx, jp_cri=(0.5, 2.0), engine="fused", variant="fast", precision="fp32split") as s:
+        s.upload(q0)
+        for tau in (1e-30, float("inf"), 0.25):
+            s.set_split_jump(tau)
+            assert s.split_jump == np.float32(tau)
+            err[tau] = np.abs(s.rhs() - ref).max() / scale
+        for bad in (0.0, -1.0, float("nan")):
+            with pytest.raises(ValueError, match="tau must be > 0"):
+                s.set_split_jump(bad)
+        assert s.split_jump == 0.25
+    print(f"[solver] fp32-split rhs max rel error by split-jump threshold {err}", flush=True)
+    assert err[1e-30] <= 1e-7, err
+    assert err[float("inf")] <= 1e-4 and err[float("inf")] > err[1e-30], err
 
 
 @pytest.mark.parametrize("cons,force_hlle", [(False, False), (True, False), (False, True)])

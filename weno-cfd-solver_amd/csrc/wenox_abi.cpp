@@ -1535,6 +1535,21 @@ extern "C" int wenox_solver_set_pipeline(wenox_solver* s, int parts)
     return WENOX_OK;
 }
 
+extern "C" int wenox_solver_split_jump(wenox_solver* s, double* tau)
+{
+    if (!s || !tau) return fail(WENOX_E_ARG, "wenox_solver_split_jump: null argument");
+    *tau = s->plan.split_jump;
+    return WENOX_OK;
+}
+
+extern "C" int wenox_solver_set_split_jump(wenox_solver* s, double tau)
+{
+    if (!s || !(tau > 0.0)) return fail(WENOX_E_ARG, "wenox_solver_set_split_jump: tau must be > 0 (inf: no jump test)");
+    s->plan.split_jump = (float)tau;
+    drop_graphs(s);   // the captured sweeps hold the old threshold
+    return WENOX_OK;
+}
+
 extern "C" int wenox_solver_set_chunks(wenox_solver* s, int chunk_x, int chunk_y)
 {
     if (!s) return fail(WENOX_E_ARG, "wenox_solver_set_chunks: null solver");

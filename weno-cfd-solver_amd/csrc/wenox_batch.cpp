@@ -883,6 +883,16 @@ extern "C" int wenox_batch_set_persistent(wenox_batch* b, int on)
     return WENOX_OK;
 }
 
+extern "C" int wenox_batch_set_split_jump(wenox_batch* b, double tau)
+{
+    if (!b || !(tau > 0.0)) return abi_fail(WENOX_E_ARG, "wenox_batch_set_split_jump: tau must be > 0 (inf: no jump test)");
+    BCHK(hipSetDevice(b->device));
+    BCHK(hipStreamSynchronize(b->stream));
+    b->com.split_jump = (float)tau;
+    drop_graphs(b);   // the captured launches hold the old threshold
+    return WENOX_OK;
+}
+
 extern "C" int wenox_batch_set_spin_limit(wenox_batch* b, int spins)
 {
     if (!b || spins < 0) return abi_fail(WENOX_E_ARG, "wenox_batch_set_spin_limit: spins must be >= 0");

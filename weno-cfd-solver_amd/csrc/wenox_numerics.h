@@ -533,6 +533,39 @@ template <typename T> WX_HD void weno5_pair(T u0, T u1, T u2, T u3, T u4, T u5, 
     right = weno5_right(u1, u2, u3, u4, u5);
 }
 
+// Fast FP64 Gauss-point pass (weno5_gq below) from the centre value q2 and
+// the four first differences of its stencil, dmm = q1 - q0, dm = q2 - q1,
+// dp = q3 - q2, dpp = q4 - q3.  The second differences and one-sided slopes
+// of the smoothness indicators follow from them:
+//   q2 - 2 q3 + q4 = dpp - dp          3 q2 - 4 q3 + q4 = (dpp - dp) - 2 dp
+//   q1 - 2 q2 + q3 = dp - dm           q3 - q1 = dm + dp
+//   q0 - 2 q1 + q2 = dm - dmm          3 q2 - 4 q1 + q0 = (dm - dmm) + 2 dm
+// so a lane of the fused sweep forms one difference per value and takes the
+// other three from its neighbours (its dp is lane l+1's dm, lane l+2's dmm
+// and lane l-1's dpp); every engine passes the same differences and rounds
+// alike.  The middle slope is q3 - q1 here, so its terms enter with the
+// opposite sign.
+WX_HD void weno5_gq_d(double q2, double dmm, double dm, double dp, double dpp, double& ga, double& gb)
+{
+    const double S = K<double>::sqrt3_12;
+    const double dA = 0.1928406937, dB = 0.6111111111, dC = 0.1960481952;
+    const double eps = 4.0 * K<double>::eps;   // with the betas from beta4
+    const double c0 = dpp - dp, c1 = dp - dm, c2 = dm - dmm;
+    const double s0 = fma(-2.0, dp, c0), s1 = dm + dp, s2 = fma(2.0, dm, c2);
+    const double b0 = beta4(c0, s0), b1 = beta4(c1, s1), b2 = beta4(c2, s2);
+    const double t0 = sq(eps + b0), t1 = sq(eps + b1), t2 = sq(eps + b2);
+    const double e0 = t1 * t2, e1 = t0 * t2, e2 = t0 * t1;   // weights up to a common factor
+    const double SA = S * dA, SB = S * dB, SC = S * dC;
+    const double u0 = e0 * s0, u1 = e1 * s1, u2 = e2 * s2;
+    const double na = SA * u0 - SB * u1 - SC * u2;
+    const double nb = SC * u0 - SB * u1 - SA * u2;
+    const double da = dA * e0 + dB * e1 + dC * e2;
+    const double db = dC * e0 + dB * e1 + dA * e2;
+    const double r = rcp(da * db);   // one reciprocal for both normalisations
+    ga = q2 + na * (db * r);
+    gb = q2 - nb * (da * r);
+}
+
 // weno.rs:262-303  Gauss-point values (ga at -h/(2 sqrt 3), gb at +h/(2 sqrt 3))
 // from five face values, sharing the smoothness indicators.
 template <typename T> WX_HD void weno5_gq(T q0, T q1, T q2, T q3, T q4, T& ga, T& gb)
@@ -540,6 +573,10 @@ template <typename T> WX_HD void weno5_gq(T q0, T q1, T q2, T q3, T q4, T& ga, T
     const T S = K<T>::sqrt3_12;
     const T dA = T(0.1928406937), dB = T(0.6111111111), dC = T(0.1960481952);
 #if WX_FAST
+    if constexpr (sizeof(T) == 8) {
+        weno5_gq_d(q2, q1 - q0, q2 - q1, q3 - q2, q4 - q3, ga, gb);
+        return;
+    }
     const T eps = T(4) * K<T>::eps;   // with the betas from beta4
     T s0 = T(3) * q2 - T(4) * q3 + q4;
     T s1 = q1 - q3;
@@ -695,28 +732,51 @@ template <typename T> WX_HD void recon_normal(int dir, bool cons, const T c[6][4
 // FP64.  L and R for the increments are built in FP32 from the rounded
 // Roe state.
 // ---------------------------------------------------------------------
-template <typename T> WX_HD void weno5_gq_dev(T q0, T q1, T q2, T q3, T q4, T& ga, T& gb)
+//
+// The FP32 error of a reconstruction scales with the increments it is given,
+// so a face whose stencils span a large relative jump is computed in FP64 by
+// the sweep: a jump is an increment of density or energy above tau times
+// the centre value (tau: the solver's split-jump threshold), in the normal stencil of the face or of one of the
+// four faces beside it that feed its Gauss points (recon_normal_dev's
+// result), or in the face values feeding its Gauss points (weno5_gq_dev's
+// result for those components).  Smooth flow stays below it: the relative
+// increments are a few grid spacings times the relative gradient.
+// ---------------------------------------------------------------------
+template <bool CHECK, typename T> WX_HD bool weno5_gq_dev(T q0, T q1, T q2, T q3, T q4, T& ga, T& gb, float tau)
 {
     float a, b;
-    weno5_gq<float>((float)(q0 - q2), (float)(q1 - q2), 0.0f, (float)(q3 - q2), (float)(q4 - q2), a, b);
+    const float i0 = (float)(q0 - q2), i1 = (float)(q1 - q2), i3 = (float)(q3 - q2), i4 = (float)(q4 - q2);
+    weno5_gq<float>(i0, i1, 0.0f, i3, i4, a, b);
     ga = q2 + (T)a;
     gb = q2 + (T)b;
+    if constexpr (!CHECK) return false;
+    const float m = fmaxf(fmaxf(fabsf(i0), fabsf(i1)), fmaxf(fabsf(i3), fabsf(i4)));
+    return !(m <= tau * fabsf((float)q2));
 }
 
-WX_HD void recon_normal_dev(int dir, bool cons, const double c[6][4], double gam, double ql[4], double qr[4])
+WX_HD bool recon_normal_dev(int dir, bool cons, const double c[6][4], double gam, double ql[4], double qr[4],
+                            float tau)
 {
     float d[6][4];
 #pragma unroll
     for (int t = 0; t < 6; ++t)
 #pragma unroll
         for (int k = 0; k < 4; ++k) d[t][k] = t == 2 ? 0.0f : (float)(c[t][k] - c[2][k]);
+    // relative jump of density or energy across the stencil
+    bool jump = false;
+#pragma unroll
+    for (int k = 0; k < 4; k += 3) {
+        const float m = fmaxf(fmaxf(fmaxf(fabsf(d[0][k]), fabsf(d[1][k])), fmaxf(fabsf(d[3][k]), fabsf(d[4][k]))),
+                              fabsf(d[5][k]));
+        jump = jump || !(m <= tau * fabsf((float)c[2][k]));
+    }
     if (cons) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             ql[k] = c[2][k] + (double)weno5_left<float>(d[0][k], d[1][k], 0.0f, d[3][k], d[4][k]);
             qr[k] = c[2][k] + (double)weno5_right<float>(d[1][k], 0.0f, d[3][k], d[4][k], d[5][k]);
         }
-        return;
+        return jump;
     }
     Roe<double> s = roe_avg(c[2], c[3], gam);
     // FP64 base R L c_2
@@ -755,6 +815,7 @@ WX_HD void recon_normal_dev(int dir, bool cons, const double c[6][4], double gam
     }
     pp_limit(ql, c[2], gam);
     pp_limit(qr, c[3], gam);
+    return jump;
 }
 
 // ---------------------------------------------------------------------

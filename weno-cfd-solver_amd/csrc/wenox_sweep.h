@@ -13,6 +13,9 @@
 
 namespace wenox {
 
+// Default split-jump threshold of the FP32-split sweeps (SweepPlan::split_jump).
+constexpr float SPLIT_JUMP_DEFAULT = 0.25f;
+
 struct SweepPlan {
     int64_t nx, ny;
     double dx, gamma;
@@ -38,6 +41,9 @@ struct SweepPlan {
     // side (32 B), 2 x 2 cells per 128 B line, lines row-major; nb lines per
     // line row, Pb doubles per state (blk_row / blk_col).
     int64_t nb = 0, Pb = 0;
+    // FP32-split: a face whose stencils span a relative jump of density or
+    // energy above this takes FP64 states (recon_normal_dev, weno5_gq_dev)
+    float split_jump = SPLIT_JUMP_DEFAULT;
     int64_t P() const { return (nx + 6) * ld; }       // plane stride of U
     int64_t PT() const { return (ny + 6) * ldT; }     // plane stride of U^T
     int64_t PDx() const { return nx * ldx; }          // plane stride of Dx
@@ -183,6 +189,7 @@ struct BatchTables {       // device pointers and grid sizes
 struct BatchCommon {
     double gam, jp_lo, jp_hi;
     int cons, force_hlle;
+    float split_jump = SPLIT_JUMP_DEFAULT;   // FP32-split (SweepPlan::split_jump)
 };
 
 constexpr int GHOST_TB = 256;

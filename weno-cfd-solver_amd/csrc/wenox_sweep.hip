@@ -108,6 +108,7 @@ struct SweepArgs {
     // the rule bp (ghost_in_ysweep holds: every source in the same tile)
     int ghost = 0;
     BcParams bp{};
+    float split_jump = SPLIT_JUMP_DEFAULT;   // DETECT: the FP64 threshold on relative jumps
 };
 
 // Shu-Osher coefficients of stage s: U_{s+1} = a Un + b Us + (c dt) L
@@ -256,6 +257,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
     static_assert(FOLD == FOLD_NONE || (IN == IN_F64 && MODE == MODE_PLAIN), "fold: FP64 sweeps only");
     static_assert(FOLD == FOLD_NONE || (DIR == DIR_X) == (FOLD != FOLD_BO), "fold: E from the x sweep, state from the y sweep");
     constexpr bool BLK = FOLD == FOLD_BE || FOLD == FOLD_BO;
+    // the fast FP64 Gauss-point pass shares first differences between lanes
+    constexpr bool GQ_DIFF = WX_FAST && MODE == MODE_PLAIN && sizeof(T) == 8;
     static_assert(!PACK || (BLK && W == 1), "packs: blocked folds, one wave per workgroup");
     if (!PACK && A.ctl && !A.ctl->active) return;
     auto& s_p1 = S.p1;
@@ -434,30 +437,60 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
 
             // pass 1 (normal reconstruction) on every lane
             WX_PHASE(pass1);
-            {
-                T ql[4], qr[4];
-                if constexpr (MODE == MODE_DETECT) recon_normal_dev(DIR, CONS, w, gam, ql, qr);
-                else recon_normal(DIR, CONS, w, gam, ql, qr);
+            T pv[8];        // this lane's q_l (0..3), q_r (4..7)
+            bool jn = false;   // FP32-split: a large jump across this lane's normal stencil
+            if constexpr (MODE == MODE_DETECT) jn = recon_normal_dev(DIR, CONS, w, gam, pv, pv + 4, A.split_jump);
+            else recon_normal(DIR, CONS, w, gam, pv, pv + 4);
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    s_p1[wv][k][l] = ql[k];
-                    s_p1[wv][4 + k][l] = qr[k];
-                }
-            }
+            for (int k = 0; k < 8; ++k) s_p1[wv][k][l] = pv[k];
             wave_sync();
 
             T fc[4], fe[4], pb = T(1);
             T l1[4], l2[4], q1[4], q2[4];
+            bool jt = false;   // FP32-split: a large jump across this face's Gauss-point stencil
             WX_PHASE(pass2);
-            if (gauss_lane) {
+            if constexpr (GQ_DIFF) {
+                // pass 2 from first differences (weno5_gq_d): each lane forms
+                // q(l+1) - q(l) of its eight values and leaves it in its own
+                // pass-1 slot (every lane's read of its right neighbour's slot
+                // is issued before any lane's store: DS instructions of a
+                // wavefront run in program order), then reads the differences
+                // of lanes l-2, l-1 and l+1
+                T (*sp)[WAVE] = s_p1[wv];
+                const int lr = l < WAVE - 1 ? l + 1 : l;
+                T dq[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) dq[k] = sp[k][lr] - pv[k];
+                wave_sync();
+#pragma unroll
+                for (int k = 0; k < 8; ++k) sp[k][l] = dq[k];
+                wave_sync();
+                if (gauss_lane) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        weno5_gq_d(pv[k], sp[k][l - 2], sp[k][l - 1], dq[k], sp[k][l + 1], l1[k], l2[k]);
+                        weno5_gq_d(pv[4 + k], sp[4 + k][l - 2], sp[4 + k][l - 1], dq[4 + k], sp[4 + k][l + 1], q1[k],
+                                   q2[k]);
+                    }
+                }
+            } else if (gauss_lane) {
                 // pass 2: Gauss points along the face from lanes l-2..l+2
                 const T(*sp)[WAVE] = s_p1[wv];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     if constexpr (MODE == MODE_DETECT) {
-                        weno5_gq_dev(sp[k][l - 2], sp[k][l - 1], sp[k][l], sp[k][l + 1], sp[k][l + 2], l1[k], l2[k]);
-                        weno5_gq_dev(sp[4 + k][l - 2], sp[4 + k][l - 1], sp[4 + k][l], sp[4 + k][l + 1],
-                                     sp[4 + k][l + 2], q1[k], q2[k]);
+                        // density and energy also report their relative jump
+                        if (k == 0 || k == 3) {
+                            jt |= weno5_gq_dev<true>(sp[k][l - 2], sp[k][l - 1], sp[k][l], sp[k][l + 1], sp[k][l + 2],
+                                                     l1[k], l2[k], A.split_jump);
+                            jt |= weno5_gq_dev<true>(sp[4 + k][l - 2], sp[4 + k][l - 1], sp[4 + k][l],
+                                                     sp[4 + k][l + 1], sp[4 + k][l + 2], q1[k], q2[k], A.split_jump);
+                        } else {
+                            weno5_gq_dev<false>(sp[k][l - 2], sp[k][l - 1], sp[k][l], sp[k][l + 1], sp[k][l + 2],
+                                                l1[k], l2[k], A.split_jump);
+                            weno5_gq_dev<false>(sp[4 + k][l - 2], sp[4 + k][l - 1], sp[4 + k][l], sp[4 + k][l + 1],
+                                                sp[4 + k][l + 2], q1[k], q2[k], A.split_jump);
+                        }
                     } else {
                         weno5_gq(sp[k][l - 2], sp[k][l - 1], sp[k][l], sp[k][l + 1], sp[k][l + 2], l1[k], l2[k]);
                         weno5_gq(sp[4 + k][l - 2], sp[4 + k][l - 1], sp[4 + k][l], sp[4 + k][l + 1],
@@ -472,8 +505,14 @@ __device__ __forceinline__ void sweep_body(const SweepArgs& A, int64_t bx, int64
                 // from the registers' stencil, then pass 2), and the flagged
                 // faces take the FP64 states, so that every face's result is
                 // the same whatever tiling puts it in a row with others.
-                const bool badf = gauss_lane && (bad_state(l1, gam) || bad_state(l2, gam) || bad_state(q1, gam) ||
-                                                 bad_state(q2, gam));
+                // The same holds for a face whose Gauss-point stencil, or the
+                // normal stencil of a face among lanes l-2..l+2 that feed its
+                // Gauss points, spans a large jump (recon_normal_dev,
+                // weno5_gq_dev).
+                const uint64_t jm = __ballot(jn ? 1 : 0);
+                const bool jnear = ((jm >> (gauss_lane ? l - 2 : 0)) & 0x1full) != 0ull;
+                const bool badf = gauss_lane && (jnear || jt || bad_state(l1, gam) || bad_state(l2, gam) ||
+                                                 bad_state(q1, gam) || bad_state(q2, gam));
                 if (__ballot(badf ? 1 : 0) != 0ull) {
                     bad = true;
                     // only the flagged faces take the FP64 states: a face's
@@ -1064,14 +1103,18 @@ static void launch_sweeps(const SweepArgs& ax, const SweepArgs& ay, int cons, in
 
 static SweepArgs args_x(const SweepPlan& p, const void* U, void* D, int* flags, const StepCtl* ctl)
 {
-    return SweepArgs{U,     p.nx + 6,   p.ny + 6, p.ld,    p.ldx,     p.nx,    p.ny,  p.chunk_x,
-                     p.waves_x, p.gamma, p.dx,     p.two_h_x, p.jp_lo, p.jp_hi, D,     flags, ctl};
+    SweepArgs a{U,     p.nx + 6,   p.ny + 6, p.ld,    p.ldx,     p.nx,    p.ny,  p.chunk_x,
+                p.waves_x, p.gamma, p.dx,     p.two_h_x, p.jp_lo, p.jp_hi, D,     flags, ctl};
+    a.split_jump = p.split_jump;
+    return a;
 }
 
 static SweepArgs args_y(const SweepPlan& p, const void* UT, void* D, int* flags, const StepCtl* ctl)
 {
-    return SweepArgs{UT,    p.ny + 6,   p.nx + 6, p.ldT,   p.ldyT,    p.ny,    p.nx,  p.chunk_y,
-                     p.waves_y, p.gamma, p.dx,     p.two_h_y, p.jp_lo, p.jp_hi, D,     flags, ctl};
+    SweepArgs a{UT,    p.ny + 6,   p.nx + 6, p.ldT,   p.ldyT,    p.ny,    p.nx,  p.chunk_y,
+                p.waves_y, p.gamma, p.dx,     p.two_h_y, p.jp_lo, p.jp_hi, D,     flags, ctl};
+    a.split_jump = p.split_jump;
+    return a;
 }
 
 // ---- FP64 ---------------------------------------------------------------
@@ -1428,6 +1471,7 @@ k_sweep_batch_xy(BatchTables B, BatchCommon C, const void* Ubase, const void* UT
     A.dx = I.dx;
     A.two_h = xdir ? I.two_h_x : I.two_h_y;
     A.lo = C.jp_lo;
+    A.split_jump = C.split_jump;
     A.hi = C.jp_hi;
     A.D = (DT*)(xdir ? Dxbase : DyTbase) + I.ioff;
     A.flags = flags;
@@ -1591,6 +1635,7 @@ k_sweep_batch_blk(BatchTables B, BatchCommon C, const double* Ub, const double* 
     A.dx = I.dx;
     A.two_h = DIR == DIR_X ? I.two_h_x : I.two_h_y;
     A.lo = C.jp_lo;
+    A.split_jump = C.split_jump;
     A.hi = C.jp_hi;
     A.flags = nullptr;
     A.ctl = &ctls[i];
@@ -1784,6 +1829,7 @@ k_batch_persist(BatchTables B, BatchCommon C, BcParams bp, PersistArgs P)
             A.dx = I.dx;
             A.two_h = xd ? I.two_h_x : I.two_h_y;
             A.lo = C.jp_lo;
+            A.split_jump = C.split_jump;
             A.hi = C.jp_hi;
             A.flags = nullptr;
             A.ctl = nullptr;

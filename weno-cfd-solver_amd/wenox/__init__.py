@@ -190,6 +190,18 @@ class Solver:
         _lib.check(_lib.load().wenox_solver_set_pipeline(self._h, int(parts)))
 
     @property
+    def split_jump(self) -> float:
+        """FP32-split: relative jump of density or energy across a face's
+        stencils above which the face takes FP64 states."""
+        v = ctypes.c_double()
+        _lib.check(_lib.load().wenox_solver_split_jump(self._h, ctypes.byref(v)))
+        return v.value
+
+    def set_split_jump(self, tau: float):
+        """Set the FP32-split jump threshold (> 0; inf disables the test)."""
+        _lib.check(_lib.load().wenox_solver_set_split_jump(self._h, float(tau)))
+
+    @property
     def blocked(self) -> bool:
         """Blocked state layout of the fused FP64 fast step (its default):
         both sweeps read one blocked copy of the state and the y sweep
@@ -449,6 +461,10 @@ class Batch:
         """Serve only the XCDs in `mask` (a non-empty subset of the probed
         ones; 0 restores the default); the same states bit for bit."""
         _lib.check(_lib.load().wenox_batch_set_persist_xcds(self._h, int(mask)))
+
+    def set_split_jump(self, tau: float):
+        """FP32-split jump threshold for every instance (Solver.set_split_jump)."""
+        _lib.check(_lib.load().wenox_batch_set_split_jump(self._h, float(tau)))
 
     def set_pack_max(self, pack_max: int):
         """Members per pack at most (1: no packing); the same states bit for

@@ -101,6 +101,8 @@ def load():
         "wenox_solver_set_combine_cols": ([dp, i], i),
         "wenox_solver_pipeline": ([dp, ctypes.POINTER(i)], i),
         "wenox_solver_set_pipeline": ([dp, i], i),
+        "wenox_solver_split_jump": ([dp, ctypes.POINTER(d)], i),
+        "wenox_solver_set_split_jump": ([dp, d], i),
         "wenox_solver_layout": ([dp, ctypes.POINTER(i)], i),
         "wenox_solver_set_layout": ([dp, i], i),
         "wenox_solver_set_cfl_rule": ([dp, d, d], i),
@@ -122,6 +124,7 @@ def load():
         "wenox_batch_persistent": ([dp, ctypes.POINTER(i)], i),
         "wenox_batch_set_persistent": ([dp, i], i),
         "wenox_batch_set_spin_limit": ([dp, i], i),
+        "wenox_batch_set_split_jump": ([dp, d], i),
         "wenox_batch_packs": ([dp, ctypes.POINTER(i), ctypes.POINTER(i)], i),
         "wenox_batch_set_pack_max": ([dp, i], i),
         "wenox_batch_persist_xcds": ([dp, ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_uint)], i),
