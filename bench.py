@@ -424,23 +424,23 @@ _KH_SRC64B = "profiles/kh8192_fp64_blocked.md"
 _DMR_SRC64B = "profiles/dmr1920x480_fp64_blocked.md"
 TRAFFIC = {
     ("double_mach_reflection", 1920, 480, False, "row"):
-        {"bytes_per_step": 1.0506e9, "tiling": (10, 9, 4, 4), "source": _DMR_SRC64},
+        {"bytes_per_step": 1.0534e9, "tiling": (10, 9, 4, 4), "source": _DMR_SRC64},
     ("double_mach_reflection", 1920, 480, False, "blocked"):
-        {"bytes_per_step": 0.7440e9, "tiling": (10, 9, 4, 4), "source": _DMR_SRC64B},
+        {"bytes_per_step": 0.7338e9, "tiling": (10, 9, 4, 4), "source": _DMR_SRC64B},
     ("double_mach_reflection", 1920, 480, True, "row"):
-        {"bytes_per_step": 1.1188e9, "tiling": (10, 8, 2, 1), "source": _DMR_SRC32},
+        {"bytes_per_step": 1.1756e9, "tiling": (10, 8, 2, 1), "source": _DMR_SRC32},
     ("four_quadrant_riemann", 4096, 4096, False, "row"):
-        {"bytes_per_step": 16.7482e9, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64},
+        {"bytes_per_step": 16.7864e9, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64},
     ("four_quadrant_riemann", 4096, 4096, False, "blocked"):
-        {"bytes_per_step": 12.3849e9, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64B},
+        {"bytes_per_step": 11.8661e9, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64B},
     ("four_quadrant_riemann", 4096, 4096, True, "row"):
-        {"bytes_per_step": 15.4459e9, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC32},
+        {"bytes_per_step": 16.0526e9, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC32},
     ("kelvin_helmholtz", 8192, 8192, False, "row"):
-        {"bytes_per_step": 66.6840e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC64},
+        {"bytes_per_step": 66.7687e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC64},
     ("kelvin_helmholtz", 8192, 8192, False, "blocked"):
-        {"bytes_per_step": 49.6295e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC64B},
+        {"bytes_per_step": 47.4133e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC64B},
     ("kelvin_helmholtz", 8192, 8192, True, "row"):
-        {"bytes_per_step": 61.1511e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC32},
+        {"bytes_per_step": 62.7354e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC32},
 }
 # VALU wave-instructions of a stage's sweeps as the step launches them (one
 # k_sweep_xy, or the x and y k_sweep launches together when their wave counts
@@ -448,23 +448,23 @@ TRAFFIC = {
 # recompute of flagged face rows)
 VALU = {
     ("double_mach_reflection", 1920, 480, False, "row", "sweeps"):
-        {"wave_instr_per_launch": 5.17e7, "tiling": (10, 9, 4, 4), "source": _DMR_SRC64},
+        {"wave_instr_per_launch": 4.984e7, "tiling": (10, 9, 4, 4), "source": _DMR_SRC64},
     ("double_mach_reflection", 1920, 480, False, "blocked", "sweeps"):
-        {"wave_instr_per_launch": 2.629e7 + 2.501e7, "tiling": (10, 9, 4, 4), "source": _DMR_SRC64B},
+        {"wave_instr_per_launch": 2.532e7 + 2.453e7, "tiling": (10, 9, 4, 4), "source": _DMR_SRC64B},
     ("double_mach_reflection", 1920, 480, True, "row", "sweeps"):
-        {"wave_instr_per_launch": 3.479e7 + 3.795e7, "tiling": (10, 8, 2, 1), "source": _DMR_SRC32},
+        {"wave_instr_per_launch": 3.600e7 + 3.959e7, "tiling": (10, 8, 2, 1), "source": _DMR_SRC32},
     ("four_quadrant_riemann", 4096, 4096, False, "row", "sweeps"):
-        {"wave_instr_per_launch": 8.335e8, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64},
+        {"wave_instr_per_launch": 8.014e8, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64},
     ("four_quadrant_riemann", 4096, 4096, False, "blocked", "sweeps"):
-        {"wave_instr_per_launch": 4.165e8 + 4.143e8, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64B},
+        {"wave_instr_per_launch": 4.000e8 + 4.052e8, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC64B},
     ("four_quadrant_riemann", 4096, 4096, True, "row", "sweeps"):
-        {"wave_instr_per_launch": 1.023e9, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC32},
+        {"wave_instr_per_launch": 1.057e9, "tiling": (48, 48, 1, 1), "source": _QUAD_SRC32},
     ("kelvin_helmholtz", 8192, 8192, False, "row", "sweeps"):
-        {"wave_instr_per_launch": 3.422e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC64},
+        {"wave_instr_per_launch": 3.295e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC64},
     ("kelvin_helmholtz", 8192, 8192, False, "blocked", "sweeps"):
-        {"wave_instr_per_launch": 1.69e9 + 1.722e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC64B},
+        {"wave_instr_per_launch": 1.625e9 + 1.686e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC64B},
     ("kelvin_helmholtz", 8192, 8192, True, "row", "sweeps"):
-        {"wave_instr_per_launch": 4.178e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC32},
+        {"wave_instr_per_launch": 4.294e9, "tiling": (57, 57, 1, 1), "source": _KH_SRC32},
 }
 
 
@@ -532,7 +532,7 @@ def workload_block(key, steps, warmup):
 # halo re-reads)
 _BATCH_SRC = "profiles/batch512_fp64_persistent.md"
 BATCH_TRAFFIC = {
-    "persistent": {"bytes_per_step": 18.0921e9, "wave_instr_per_step": 3.8045e9, "chunk": 64, "source": _BATCH_SRC},
+    "persistent": {"bytes_per_step": 18.2424e9, "wave_instr_per_step": 3.6647e9, "chunk": 64, "source": _BATCH_SRC},
 }
 
 
