@@ -20,7 +20,7 @@ WINDOWS = [(1e4, 1.1e4), (0.5, 2.0), (0.01, 0.05)]
 SEEDS = range(len(SHAPES) + 6)
 
 
-def _case(seed):
+def _case(seed, vacuum_patch=True):
     rng = np.random.default_rng(20261204 + seed)
     if seed < len(SHAPES):
         nx, ny = SHAPES[seed]
@@ -38,7 +38,7 @@ def _case(seed):
         p[i0:i1, j0:j1] = rng.uniform(0.05, 3.0)
         u[i0:i1, j0:j1] = rng.uniform(-1.5, 1.5)
         v[i0:i1, j0:j1] = rng.uniform(-1.5, 1.5)
-    vacuum = seed % 3 == 0
+    vacuum = vacuum_patch and seed % 3 == 0
     if vacuum:
         i0, j0 = int(rng.integers(0, m)), int(rng.integers(0, n))
         rho[i0:i0 + 4, j0:j0 + 4] = 1e-13
@@ -98,3 +98,38 @@ def test_fused_rhs_random(oracle_mod, seed):
             np.testing.assert_array_equal(got, ref, err_msg=f"seed {seed}")
         else:
             _close(got, ref, 1e-9)
+
+
+@pytest.mark.parametrize("seed", SEEDS)
+def test_fused_steps_random(oracle_mod, seed):
+    """Four SSP-RK3 steps with mirror ghost cells and the CFL rule of
+    Rie_prob_2d_rs.main_rs on the random states without the near-vacuum
+    patch: the fused exact step equals the oracle's time loop bit for bit
+    (state and time); the fast blocked step is within 1e-11 relative L1 per
+    conserved variable."""
+    from wenox import Solver
+    nx, ny, q, dx, f = _case(seed, vacuum_patch=False)   # (the reference scheme turns those non-finite)
+    if min(nx, ny) < 3:
+        pytest.skip("the solver's mirror ghost fill takes 3 interior cells per axis")
+    # the drivers carry the primitives in channels 4-7 (the first dt reads them)
+    q = oracle_mod.sync_primitives(oracle_mod.apply_bc_zero_gradient(q.copy()), 1.4)
+    print(f"[random] fused steps seed {seed} {nx}x{ny} {f}", flush=True)
+    ref, ts = oracle_mod.run_riemann(q.copy(), nx, ny, dx, 1.4, 0.3, f["force_hlle"], f["cons"], f["jp"], nstep=4)
+    # (seed 6's component-wise blend turns some cells non-finite in the
+    # reference scheme itself within the four steps: the exact step must then
+    # match the non-finite cells too; the fast one is compared on finite runs)
+    finite = bool(np.isfinite(ref[..., :4]).all())
+    for variant in ("exact", "fast"):
+        with Solver(nx, ny, dx, cfl=0.3, cons_switch=f["cons"], force_hlle=f["force_hlle"], jp_cri=f["jp"],
+                    engine="fused", variant=variant) as s:
+            s.upload(q)
+            T, k = s.run(nstep=4)
+            got = s.download()
+        assert k == 4
+        if variant == "exact":
+            assert T == ts[-1] or (T != T and ts[-1] != ts[-1])
+            np.testing.assert_array_equal(got[..., :4], ref[..., :4], err_msg=f"seed {seed}")
+        elif finite:
+            a, r = got[3:-3, 3:-3, :4], ref[3:-3, 3:-3, :4]
+            err = np.abs(a - r).sum(axis=(0, 1)) / np.abs(r).sum(axis=(0, 1))
+            assert np.all(err <= 1e-11), err
