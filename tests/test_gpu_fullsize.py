@@ -2,8 +2,9 @@
 
 * Double Mach reflection at its production size, 1920 x 480, with the
   reference's own DMR ghost fill (unitest.py:207-227) and driver loop
-  (double_mach_reflection.py:97-134, CFL 0.1): two SSP-RK3 steps of the
-  exact variant on the GPU equal the oracle's time loop bitwise.  The oracle
+  (double_mach_reflection.py:97-134, CFL 0.1): twenty SSP-RK3 steps of the
+  exact variant on the GPU, unfused and fused engines, equal the oracle's
+  time loop bitwise.  The oracle
   runs on the host's cores (its OpenMP split does not change its results).
 * The benchmark workload (standard double Mach reflection, 1920 x 480,
   CFL 0.1): the fast variant stays within 1e-11 relative L1 per conserved
@@ -145,7 +146,7 @@ def test_blocked_layout_at_4096():
 def test_dmr_production_size_matches_oracle(oracle_mod):
     from wenox import Solver, problems
     variant = "exact"
-    nx, ny, steps = 1920, 480, 2
+    nx, ny, steps = 1920, 480, 20
     q0, bound = problems.dmr_reference(nx, ny)
     dx = 1.0 / ny
     print(f"[fullsize] DMR reference fill {nx}x{ny} {steps} steps variant={variant}", flush=True)
@@ -160,13 +161,14 @@ def test_dmr_production_size_matches_oracle(oracle_mod):
             T += dt
     finally:
         oracle_mod.set_threads(1)
-    with Solver(nx, ny, dx, gamma=1.4, cfl=0.1, cons_switch=False, force_hlle=False,
-                jp_cri=(10000, 11000), bc="dmr_ref", dmr_bound=bound, variant=variant) as s:
-        s.upload(q0)
-        T2, k = s.run(nstep=steps, t_final=0.2)
-        got = s.download()
-    assert k == steps and T2 == T
-    np.testing.assert_array_equal(got[..., :4], q[..., :4])
+    for engine in ("unfused", "fused"):
+        with Solver(nx, ny, dx, gamma=1.4, cfl=0.1, cons_switch=False, force_hlle=False,
+                    jp_cri=(10000, 11000), bc="dmr_ref", dmr_bound=bound, variant=variant, engine=engine) as s:
+            s.upload(q0)
+            T2, k = s.run(nstep=steps, t_final=0.2)
+            got = s.download()
+        assert k == steps and T2 == T, engine
+        np.testing.assert_array_equal(got[..., :4], q[..., :4], err_msg=engine)
 
 
 def test_benchmark_workload_fast_matches_exact():
