@@ -507,6 +507,33 @@ def split_block(w, q, dx, steps, warmup, r64):
             "roofline": roofline(r32, w, split=True)}
 
 
+def config1_block():
+    """Configuration 1 as stated: the four-quadrant Riemann problem on 256 x
+    256, component-wise WENO5 + HLLC, SSP-RK3 at CFL 0.4, mirror ghost cells,
+    to t_end = 0.25 (the reference driver's time loop, on the GPU): FP64 fast
+    and FP32-split rates over the whole run, FP32-split's density distance
+    from FP64 at t_end.  (Against the oracle: tests/test_gpu_fullsize.py.)"""
+    from wenox import Solver, problems
+    n = 256
+    q0, dx = problems.four_quadrant(n)
+    out, dens = {"workload": "four_quadrant_riemann 256x256, component-wise WENO5 + HLLC, SSP-RK3, CFL 0.4, "
+                             "bc mirror, t_end 0.25"}, {}
+    for prec in ("fp64", "fp32split"):
+        with Solver(n, n, dx, cfl=0.4, cons_switch=True, jp_cri=(1e4, 1.1e4), bc="mirror", engine="fused",
+                    variant="fast", precision=prec) as s:
+            s.upload(q0)
+            s.run(nstep=9)          # both step graphs captured
+            s.upload(q0)
+            t0 = time.perf_counter()
+            T, k = s.run(t_final=0.25)
+            el = time.perf_counter() - t0
+            dens[prec] = s.download()[3:-3, 3:-3, 0]
+        out[prec] = {"value": n * n * k / el, "steps": k, "t": T, "seconds": el}
+    out["fp32split"]["density_vs_fp64"] = _dist(dens["fp32split"], dens["fp64"])
+    out["fp32split"]["speedup_vs_fp64"] = out["fp32split"]["value"] / out["fp64"]["value"]
+    return out
+
+
 def workload_block(key, steps, warmup):
     """FP64 (fast, autotuned; exact on the same tiling) and FP32-split on one
     more workload."""
@@ -682,6 +709,7 @@ def main():
                         "state_finite": rv["finite"]}
             res["scheme_variants_fp64"] = variants
         r64.pop("state", None)
+        res["config1"] = config1_block()
         for key, steps, warmup in (("dmr", 200, 20), ("kh8192", 100, 10)):
             if key != args.workload:
                 res[key] = workload_block(key, steps, warmup)
